@@ -1,0 +1,54 @@
+"""Build the in-tree shared library aeroflex_amd/libafx_rans.so for gfx950.
+
+hipcc compiles the HIP translation units (device code for gfx950 only) and
+g++ the plain C++ host mesh code; outputs go to aeroflex_amd/_build/ and the
+library next to this file.  Run: python -m aeroflex_amd.build
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libafx_rans.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# packed FP32 instructions are disabled for all device code
+HIP_COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              "-Wno-unused-result"]
+UNITS = [
+    # (source, extra flags)
+    ("afx_k64.hip", ["-ffp-contract=off"]),     # plain FP64 form: reference rounding order
+    ("afx_k32.hip", ["-ffp-contract=fast"]),    # FP32 throughput form
+    ("afx_capi.hip", ["-ffp-contract=off"]),
+]
+CXX_UNITS = [("afx_mesh.cpp", ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"])]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for src, flags in UNITS:
+        obj = os.path.join(BUILD, src + ".o")
+        _run([HIPCC] + HIP_COMMON + flags + ["-c", os.path.join(SRC, src), "-o", obj])
+        objs.append(obj)
+    for src, flags in CXX_UNITS:
+        obj = os.path.join(BUILD, src + ".o")
+        _run(["g++"] + flags + ["-c", os.path.join(SRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,992 @@
+// C ABI of the MI355X residual (include/afx_rans.h).
+//
+// Host side of the drop-in boundary: owns the reference-ordered mesh copy
+// (like rans::solver owns its mesh, solver.h:179), builds the device layout
+// (RCM cell order, per-cell face lists in reference face order), keeps all
+// state resident in HBM and translates between the reference's AoS
+// Eigen layout and the device's SoA layout only when the caller exchanges
+// vectors.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/afx_rans.h"
+#include "afx_kernels.hpp"
+#include "afx_mesh.hpp"
+
+using namespace afx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(AFX_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t count) {
+        n = count;
+        return hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    }
+    hipError_t upload(const T* h, size_t count, hipStream_t s) {
+        return hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s);
+    }
+    hipError_t upload(const std::vector<T>& h, hipStream_t s) { return upload(h.data(), h.size(), s); }
+};
+
+constexpr int NV = 4;
+
+}  // namespace
+
+struct afx_mesh {
+    Mesh m;
+};
+
+struct afx_rans {
+    Mesh m;
+    afx_gas gas{};
+    afx_opts opts{};
+    hipStream_t stream = nullptr;
+    int32_t nC = 0, nT = 0, nF = 0, nB = 0;
+    double cfl = 1.0;
+
+    std::vector<int32_t> c_new2old, c_old2new, f_new2old, f_old2new;
+
+    // geometry (FP64 and FP32 copies)
+    DBuf<double> xc, yc, area, fnx, fny, flen, fxf, fyf, fgw, lsq;
+    DBuf<float> xc32, yc32, area32, fnx32, fny32, flen32, fxf32, fyf32, fgw32, lsq32;
+    DBuf<int32_t> c0, c1;
+    DBuf<int8_t> fkind;
+    DBuf<uint32_t> cface;
+
+    // state and work arrays
+    DBuf<double> q, qk, qW, dt, gx, gy, lim, R, flux;
+    DBuf<float> q32, gx32, gy32, lim32, R32, flux32;
+    DBuf<double> partial, sum1, uni64, wall_d, states;
+    DBuf<float> uni32;
+    DBuf<int32_t> w_owner, w_ghost, b_owner, b_ghost, patch_cells;
+    int32_t n_walls = 0;
+    int nblk_red = 0;
+
+    // boundary conditions
+    bool bcs_set = false;
+    std::vector<int32_t> bface_type;       // per boundary face
+    std::vector<double> bface_vars;        // 4 per boundary face: mach angle T p
+    bool q32_valid = false;
+
+    ~afx_rans() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    CellsT<double> cells64() const {
+        return CellsT<double>{xc.p, yc.p, area.p, cface.p, lsq.p, nC, nT};
+    }
+    CellsT<float> cells32() const {
+        return CellsT<float>{xc32.p, yc32.p, area32.p, cface.p, lsq32.p, nC, nT};
+    }
+    FacesT<double> faces64() const {
+        return FacesT<double>{c0.p, c1.p, fnx.p, fny.p, flen.p, fxf.p, fyf.p, fgw.p, fkind.p, nF};
+    }
+    FacesT<float> faces32() const {
+        return FacesT<float>{c0.p, c1.p, fnx32.p, fny32.p, flen32.p, fxf32.p, fyf32.p, fgw32.p, fkind.p, nF};
+    }
+    GasT<double> gas64() const {
+        return GasT<double>{gas.gamma, gas.R, gas.mu_L, 1.0 * gas.mu_L / gas.Pr_L};
+    }
+    GasT<float> gas32() const {
+        return GasT<float>{(float)gas.gamma, (float)gas.R, (float)gas.mu_L, (float)(1.0 * gas.mu_L / gas.Pr_L)};
+    }
+    bool needs_gradients() const { return opts.second_order || opts.viscosity != 0; }
+};
+
+namespace {
+
+// core.h:73-83
+void conservative(const afx_gas& g, const double v[4], double out[4]) {
+    const double mach = v[0], angle = v[1], T = v[2], p = v[3];
+    double c = std::sqrt(g.gamma * g.R * T);
+    double u = mach * c * std::cos(angle);
+    double vv = mach * c * std::sin(angle);
+    double rho = p / (g.R * T);
+    double rhoE = p / (g.gamma - 1) + 0.5 * rho * (u * u + vv * vv);
+    out[0] = rho; out[1] = rho * u; out[2] = rho * vv; out[3] = rhoE;
+}
+
+// solver.h:597-611: first farfield boundary face's variables (defaults otherwise)
+void boundary_variables(const afx_rans* h, double v[4]) {
+    v[0] = 0.2; v[1] = 0; v[2] = 1; v[3] = 1.;
+    for (int b = 0; b < h->nB; ++b)
+        if (h->bface_type[b] == AFX_BC_FARFIELD) {
+            for (int k = 0; k < 4; ++k) v[k] = h->bface_vars[4 * b + k];
+            return;
+        }
+}
+
+template <typename T>
+std::vector<T> to_device_order_cells(const afx_rans* h, const std::vector<double>& ref) {
+    std::vector<T> out(h->nT);
+    for (int32_t i = 0; i < h->nT; ++i) out[i] = (T)ref[h->c_new2old[i]];
+    return out;
+}
+template <typename T>
+std::vector<T> to_device_order_faces(const afx_rans* h, const std::vector<double>& ref) {
+    std::vector<T> out(h->nF);
+    for (int32_t i = 0; i < h->nF; ++i) out[i] = (T)ref[h->f_new2old[i]];
+    return out;
+}
+
+// device SoA [NV][nT] -> reference AoS q(4*i+k)
+int download_aos(afx_rans* h, const double* d, double* out, int nv = NV) {
+    std::vector<double> tmp((size_t)nv * h->nT);
+    HIPCHK(hipMemcpyAsync(tmp.data(), d, tmp.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int32_t i = 0; i < h->nT; ++i) {
+        int32_t o = h->c_new2old[i];
+        for (int k = 0; k < nv; ++k) out[(size_t)nv * o + k] = tmp[(size_t)k * h->nT + i];
+    }
+    return AFX_OK;
+}
+int download_aos32(afx_rans* h, const float* d, double* out) {
+    std::vector<float> tmp((size_t)NV * h->nT);
+    HIPCHK(hipMemcpyAsync(tmp.data(), d, tmp.size() * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int32_t i = 0; i < h->nT; ++i) {
+        int32_t o = h->c_new2old[i];
+        for (int k = 0; k < NV; ++k) out[(size_t)NV * o + k] = (double)tmp[(size_t)k * h->nT + i];
+    }
+    return AFX_OK;
+}
+int upload_aos(afx_rans* h, const double* in, double* d) {
+    std::vector<double> tmp((size_t)NV * h->nT);
+    for (int32_t i = 0; i < h->nT; ++i) {
+        int32_t o = h->c_new2old[i];
+        for (int k = 0; k < NV; ++k) tmp[(size_t)k * h->nT + i] = in[(size_t)NV * o + k];
+    }
+    HIPCHK(hipMemcpyAsync(d, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return AFX_OK;
+}
+int download_cells(afx_rans* h, const double* d, double* out) {
+    std::vector<double> tmp(h->nT);
+    HIPCHK(hipMemcpyAsync(tmp.data(), d, tmp.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int32_t i = 0; i < h->nT; ++i) out[h->c_new2old[i]] = tmp[i];
+    return AFX_OK;
+}
+
+int norm_of(afx_rans* h, const double* x, int64_t ld, int nrows, double* out) {
+    HIPCHK(launch_sumsq(h->stream, x, ld, ld, nrows, h->partial.p, h->nblk_red));
+    HIPCHK(launch_final_sum(h->stream, h->partial.p, h->nblk_red, h->sum1.p));
+    double s = 0;
+    HIPCHK(hipMemcpyAsync(&s, h->sum1.p, 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *out = std::sqrt(s);
+    return AFX_OK;
+}
+int norm_of32(afx_rans* h, const float* x, int64_t ld, int nrows, double* out) {
+    HIPCHK(launch_sumsq_f32(h->stream, x, ld, ld, nrows, h->partial.p, h->nblk_red));
+    HIPCHK(launch_final_sum(h->stream, h->partial.p, h->nblk_red, h->sum1.p));
+    double s = 0;
+    HIPCHK(hipMemcpyAsync(&s, h->sum1.p, 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *out = std::sqrt(s);
+    return AFX_OK;
+}
+
+int ensure_q32(afx_rans* h) {
+    if (!h->q32_valid) {
+        HIPCHK(launch_to_f32(h->stream, h->q.p, h->q32.p, (int64_t)NV * h->nT));
+        h->q32_valid = true;
+    }
+    return AFX_OK;
+}
+
+// set_walls_from_internal + calc_gradients + calc_limiters (solver.h:1083-1087)
+int run_gradients64(afx_rans* h, const double* qg, const double* ql) {
+    HIPCHK(launch_set_walls<double>(h->stream, h->w_owner.p, h->w_ghost.p, h->n_walls, const_cast<double*>(ql), h->nT, NV));
+    HIPCHK(launch_grad_lim<double>(h->stream, h->cells64(), h->faces64(), qg, ql, h->gas64(), h->opts.gradient,
+                                   h->opts.second_order, h->opts.limiter_k, h->gx.p, h->gy.p, h->lim.p));
+    return AFX_OK;
+}
+int run_gradients32(afx_rans* h) {
+    HIPCHK(launch_set_walls<float>(h->stream, h->w_owner.p, h->w_ghost.p, h->n_walls, h->q32.p, h->nT, NV));
+    HIPCHK(launch_grad_lim<float>(h->stream, h->cells32(), h->faces32(), h->q32.p, h->q32.p, h->gas32(),
+                                  h->opts.gradient, h->opts.second_order, (float)h->opts.limiter_k, h->gx32.p,
+                                  h->gy32.p, h->lim32.p));
+    return AFX_OK;
+}
+
+// enqueue one residual evaluation (no synchronisation)
+int enqueue_residual(afx_rans* h, int precision) {
+    if (precision == AFX_FP64) {
+        if (h->needs_gradients()) {
+            int rc = run_gradients64(h, h->q.p, h->q.p);
+            if (rc) return rc;
+        }
+        HIPCHK(launch_face_flux<double>(h->stream, h->cells64(), h->faces64(), h->q.p, h->q.p, h->gx.p, h->gy.p,
+                                        h->lim.p, h->gas64(), h->opts.viscosity, h->opts.second_order, nullptr,
+                                        h->flux.p));
+        HIPCHK(launch_gather<double>(h->stream, h->cells64(), h->flux.p, h->nF, 0, h->R.p));
+    } else {
+        int rc = ensure_q32(h);
+        if (rc) return rc;
+        if (h->needs_gradients()) {
+            rc = run_gradients32(h);
+            if (rc) return rc;
+        }
+        HIPCHK(launch_face_flux<float>(h->stream, h->cells32(), h->faces32(), h->q32.p, h->q32.p, h->gx32.p,
+                                       h->gy32.p, h->lim32.p, h->gas32(), h->opts.viscosity, h->opts.second_order,
+                                       nullptr, h->flux32.p));
+        HIPCHK(launch_gather<float>(h->stream, h->cells32(), h->flux32.p, h->nF, 0, h->R32.p));
+    }
+    return AFX_OK;
+}
+
+int check_handle(afx_rans* h, bool need_bcs = true) {
+    if (!h) return fail(AFX_EARG, "null solver handle");
+    if (need_bcs && !h->bcs_set) return fail(AFX_ESTATE, "boundary conditions not set (call afx_rans_set_bcs)");
+    return AFX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* afx_last_error(void) { return g_err.c_str(); }
+
+void afx_default_gas(afx_gas* g) {
+    g->R = 0.71428571428; g->mu_L = 1e-5; g->Pr_L = 0.72; g->Pr_T = 0.9; g->gamma = 1.4;
+}
+void afx_default_opts(afx_opts* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->viscosity = 0; o->second_order = 1; o->gradient = 1; o->limiter_k = 5.0; o->renumber = 1;
+}
+
+// ------------------------------------------------------------------ mesh ---
+int afx_mesh_read_msh(const char* path, afx_mesh** out) {
+    if (!path || !out) return fail(AFX_EARG, "null argument");
+    try {
+        Elements el = read_msh41(path);
+        auto* m = new afx_mesh();
+        m->m = build_mesh(el);
+        *out = m;
+        return AFX_OK;
+    } catch (std::exception& e) {
+        return fail(AFX_EIO, e.what());
+    }
+}
+
+int afx_mesh_from_elements(int64_t n_nodes, const double* x, const double* y, int64_t n_cells, const int32_t* cell_nodes,
+                           const int32_t* cell_nverts, int64_t n_bsegs, const int32_t* bseg_nodes,
+                           const int32_t* bseg_keys, int32_t n_names, const int32_t* name_keys,
+                           const char* const* names, afx_mesh** out) {
+    if (!out || n_nodes <= 0 || n_cells <= 0 || !x || !y || !cell_nodes || !cell_nverts) return fail(AFX_EARG, "bad mesh arrays");
+    try {
+        Elements el;
+        el.x.assign(x, x + n_nodes);
+        el.y.assign(y, y + n_nodes);
+        for (int64_t c = 0; c < n_cells; ++c) {
+            int nv = cell_nverts[c];
+            if (nv != 3 && nv != 4) return fail(AFX_EARG, "cells must have 3 or 4 nodes");
+            for (int k = 0; k < 4; ++k) {
+                int32_t v = (k < nv) ? cell_nodes[4 * c + k] : 0;
+                if (v < 0 || v >= n_nodes) return fail(AFX_EARG, "cell node index out of range");
+                el.cell_nodes.push_back((uint32_t)v);
+            }
+            el.cell_nv.push_back((uint8_t)nv);
+        }
+        for (int64_t b = 0; b < n_bsegs; ++b) {
+            for (int k = 0; k < 2; ++k) {
+                int32_t v = bseg_nodes[2 * b + k];
+                if (v < 0 || v >= n_nodes) return fail(AFX_EARG, "boundary node index out of range");
+                el.bseg_nodes.push_back((uint32_t)v);
+            }
+            el.bseg_key.push_back((uint32_t)bseg_keys[b]);
+        }
+        for (int32_t i = 0; i < n_names; ++i) {
+            el.name_keys.push_back((uint32_t)name_keys[i]);
+            el.names.push_back(names[i]);
+        }
+        auto* m = new afx_mesh();
+        m->m = build_mesh(el);
+        *out = m;
+        return AFX_OK;
+    } catch (std::exception& e) {
+        return fail(AFX_EARG, e.what());
+    }
+}
+
+int afx_mesh_gen_naca_ogrid(int32_t ni, int32_t nj, double r_far, double thickness, double first_h, afx_mesh** out) {
+    if (!out || ni < 8 || nj < 3 || r_far <= 1.0 || first_h <= 0) return fail(AFX_EARG, "bad O-grid parameters");
+    try {
+        auto* m = new afx_mesh();
+        m->m = build_mesh(gen_naca_ogrid(ni, nj, r_far, thickness, first_h));
+        *out = m;
+        return AFX_OK;
+    } catch (std::exception& e) {
+        return fail(AFX_EARG, e.what());
+    }
+}
+
+int afx_mesh_gen_jittered(int32_t ni, int32_t nj, double r_far, double thickness, double first_h, double jitter,
+                          double quad_frac, uint64_t seed, afx_mesh** out) {
+    if (!out || ni < 8 || nj < 3 || r_far <= 1.0 || first_h <= 0 || jitter < 0 || jitter > 0.2 || quad_frac < 0 ||
+        quad_frac > 1)
+        return fail(AFX_EARG, "bad jittered-mesh parameters");
+    try {
+        auto* m = new afx_mesh();
+        m->m = build_mesh(gen_jittered_airfoil(ni, nj, r_far, thickness, first_h, jitter, quad_frac, seed));
+        *out = m;
+        return AFX_OK;
+    } catch (std::exception& e) {
+        return fail(AFX_EARG, e.what());
+    }
+}
+
+int afx_mesh_write_msh(const afx_mesh* mh, const char* path) {
+    if (!mh || !path) return fail(AFX_EARG, "null argument");
+    try {
+        const Mesh& m = mh->m;
+        Elements el;
+        el.x = m.nodesX; el.y = m.nodesY;
+        el.name_keys = m.name_keys; el.names = m.names;
+        for (uint32_t c = 0; c < m.nRealCells; ++c) {
+            for (int k = 0; k < 4; ++k) el.cell_nodes.push_back(m.cellsNodes[4 * (size_t)c + k]);
+            el.cell_nv.push_back(m.cellsIsTriangle[c] ? 3 : 4);
+        }
+        // boundary segments in their original node order are not kept; the
+        // face's node pair (creation order) spans the same edge
+        for (uint32_t b = 0; b < m.n_bfaces(); ++b) {
+            uint32_t e = m.boundaryEdges[b];
+            el.bseg_nodes.push_back(m.edgesNodes[2 * (size_t)e]);
+            el.bseg_nodes.push_back(m.edgesNodes[2 * (size_t)e + 1]);
+            el.bseg_key.push_back(m.boundaryEdgesIntTag[b]);
+        }
+        write_msh41(el, path);
+        return AFX_OK;
+    } catch (std::exception& e) {
+        return fail(AFX_EIO, e.what());
+    }
+}
+
+int afx_mesh_counts(const afx_mesh* mh, int64_t c[5]) {
+    if (!mh || !c) return fail(AFX_EARG, "null argument");
+    const Mesh& m = mh->m;
+    c[0] = m.n_nodes(); c[1] = m.nRealCells; c[2] = m.n_cells(); c[3] = m.n_faces(); c[4] = m.n_bfaces();
+    return AFX_OK;
+}
+
+int afx_mesh_get(const afx_mesh* mh, const char* field, void* out, int64_t* n_out) {
+    if (!mh || !field) return fail(AFX_EARG, "null argument");
+    const Mesh& m = mh->m;
+    std::string f(field);
+    const void* src = nullptr;
+    size_t n = 0, esz = 8;
+    auto D = [&](const std::vector<double>& v) { src = v.data(); n = v.size(); esz = 8; };
+    auto U = [&](const std::vector<uint32_t>& v) { src = v.data(); n = v.size(); esz = 4; };
+    if (f == "nodesX") D(m.nodesX);
+    else if (f == "nodesY") D(m.nodesY);
+    else if (f == "edgesNodes") U(m.edgesNodes);
+    else if (f == "edgesCells") U(m.edgesCells);
+    else if (f == "edgesLengths") D(m.edgesLengths);
+    else if (f == "edgesNormalsX") D(m.edgesNormalsX);
+    else if (f == "edgesNormalsY") D(m.edgesNormalsY);
+    else if (f == "edgesCentersX") D(m.edgesCentersX);
+    else if (f == "edgesCentersY") D(m.edgesCentersY);
+    else if (f == "cellsNodes") U(m.cellsNodes);
+    else if (f == "cellsEdges") U(m.cellsEdges);
+    else if (f == "cellsAreas") D(m.cellsAreas);
+    else if (f == "cellsCentersX") D(m.cellsCentersX);
+    else if (f == "cellsCentersY") D(m.cellsCentersY);
+    else if (f == "boundaryEdges") U(m.boundaryEdges);
+    else if (f == "boundaryEdgesIntTag") U(m.boundaryEdgesIntTag);
+    else if (f == "cellsIsTriangle") { src = m.cellsIsTriangle.data(); n = m.cellsIsTriangle.size(); esz = 4; }
+    else return fail(AFX_EARG, "unknown mesh field: " + f);
+    if (n_out) *n_out = (int64_t)n;
+    if (out && n) std::memcpy(out, src, n * esz);
+    return AFX_OK;
+}
+
+int afx_mesh_boundary_name(const afx_mesh* mh, int64_t b, char* buf, int32_t buflen) {
+    if (!mh || !buf || buflen <= 0) return fail(AFX_EARG, "null argument");
+    if (b < 0 || b >= (int64_t)mh->m.n_bfaces()) return fail(AFX_EARG, "boundary index out of range");
+    const std::string& s = mh->m.names[mh->m.boundaryNameIdx[b]];
+    if ((int32_t)s.size() + 1 > buflen) return fail(AFX_EARG, "buffer too small");
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+    return AFX_OK;
+}
+
+int afx_mesh_rcm(const afx_mesh* mh, int32_t* perm) {
+    if (!mh || !perm) return fail(AFX_EARG, "null argument");
+    auto p = rcm_order(mh->m);
+    for (size_t i = 0; i < p.size(); ++i) perm[i] = (int32_t)p[i];
+    return AFX_OK;
+}
+
+void afx_mesh_destroy(afx_mesh* m) { delete m; }
+
+// ---------------------------------------------------------------- solver ---
+int afx_rans_create(const afx_mesh* mh, const afx_gas* g, const afx_opts* o, afx_rans** out) {
+    if (!mh || !out) return fail(AFX_EARG, "null argument");
+    afx_opts opts;
+    afx_default_opts(&opts);
+    if (o) opts = *o;
+    if (opts.viscosity < 0 || opts.viscosity > 1)
+        return fail(AFX_EUNSUP, "viscosity model not available in this build (0 inviscid, 1 laminar)");
+    if (opts.gradient != 0 && opts.gradient != 1) return fail(AFX_EARG, "gradient must be 0 (least-squares) or 1 (green-gauss)");
+    auto h = new afx_rans();
+    try {
+        h->m = mh->m;
+    } catch (std::exception& e) {
+        delete h;
+        return fail(AFX_EARG, e.what());
+    }
+    if (g) h->gas = *g; else afx_default_gas(&h->gas);
+    h->opts = opts;
+    const Mesh& m = h->m;
+    h->nC = (int32_t)m.nRealCells; h->nT = (int32_t)m.n_cells(); h->nF = (int32_t)m.n_faces(); h->nB = (int32_t)m.n_bfaces();
+    const int32_t nC = h->nC, nT = h->nT, nF = h->nF;
+
+    auto bail = [&](int rc) { delete h; return rc; };
+#define HIPCHK_C(x)                                                                                   \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess) return bail(fail(AFX_EHIP, std::string(#x) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+
+    HIPCHK_C(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+
+    // cell order: RCM on real cells, ghosts keep reference order
+    h->c_new2old.resize(nT);
+    if (opts.renumber) {
+        auto p = rcm_order(m);
+        for (int32_t i = 0; i < nC; ++i) h->c_new2old[i] = (int32_t)p[i];
+    } else {
+        for (int32_t i = 0; i < nC; ++i) h->c_new2old[i] = i;
+    }
+    for (int32_t i = nC; i < nT; ++i) h->c_new2old[i] = i;
+    h->c_old2new.resize(nT);
+    for (int32_t i = 0; i < nT; ++i) h->c_old2new[h->c_new2old[i]] = i;
+
+    // face order: by the lowest device index of the cells it touches
+    std::vector<int64_t> key(nF);
+    for (int32_t f = 0; f < nF; ++f) {
+        int32_t a = h->c_old2new[m.edgesCells[2 * (size_t)f]], b = h->c_old2new[m.edgesCells[2 * (size_t)f + 1]];
+        key[f] = (b < nC) ? std::min(a, b) : a;
+    }
+    h->f_new2old.resize(nF);
+    std::iota(h->f_new2old.begin(), h->f_new2old.end(), 0);
+    std::stable_sort(h->f_new2old.begin(), h->f_new2old.end(), [&](int32_t x, int32_t y) { return key[x] < key[y]; });
+    h->f_old2new.resize(nF);
+    for (int32_t i = 0; i < nF; ++i) h->f_old2new[h->f_new2old[i]] = i;
+
+    // per-face arrays in device order
+    std::vector<int32_t> hc0(nF), hc1(nF);
+    std::vector<double> hgw(nF);
+    for (int32_t nf = 0; nf < nF; ++nf) {
+        int32_t f = h->f_new2old[nf];
+        uint32_t i = m.edgesCells[2 * (size_t)f], j = m.edgesCells[2 * (size_t)f + 1];
+        hc0[nf] = h->c_old2new[i];
+        hc1[nf] = h->c_old2new[j];
+        // solver.h:436-444
+        const double dxif = m.edgesCentersX[f] - m.cellsCentersX[i];
+        const double dyif = m.edgesCentersY[f] - m.cellsCentersY[i];
+        const double dif = std::sqrt(dxif * dxif + dyif * dyif);
+        const double dxij = m.cellsCentersX[i] - m.cellsCentersX[j];
+        const double dyij = m.cellsCentersY[i] - m.cellsCentersY[j];
+        const double dij = std::sqrt(dxij * dxij + dyij * dyij);
+        hgw[nf] = dif / dij;
+    }
+    // per-cell face lists sorted by reference face index
+    std::vector<uint32_t> hcface((size_t)4 * nC, 0xFFFFFFFFu);
+    for (int32_t ci = 0; ci < nC; ++ci) {
+        int32_t old = h->c_new2old[ci];
+        uint32_t fs[4];
+        int n = 0;
+        for (int s = 0; s < 4; ++s) {
+            uint32_t e = m.cellsEdges[4 * (size_t)old + s];
+            if (e != kNoEdge) fs[n++] = e;
+        }
+        std::sort(fs, fs + n);
+        for (int s = 0; s < n; ++s) {
+            uint32_t f = fs[s];
+            uint32_t side = (m.edgesCells[2 * (size_t)f] == (uint32_t)old) ? 0u : 1u;
+            hcface[(size_t)s * nC + ci] = (uint32_t)h->f_old2new[f] | (side << 31);
+        }
+    }
+    // least-squares matrices (solver.h:402-422), cellsEdges order
+    std::vector<double> hlsq((size_t)4 * nC, 0.0);
+    for (int32_t ci = 0; ci < nC; ++ci) {
+        int32_t i = h->c_new2old[ci];
+        int n = m.cellsIsTriangle[i] ? 3 : 4;
+        double a = 0, b = 0, d = 0;
+        for (int j = 0; j < n; ++j) {
+            uint32_t e = m.cellsEdges[4 * (size_t)i + j];
+            uint32_t cn = m.edgesCells[2 * (size_t)e] == (uint32_t)i ? m.edgesCells[2 * (size_t)e + 1] : m.edgesCells[2 * (size_t)e];
+            double dx = m.cellsCentersX[cn] - m.cellsCentersX[i];
+            double dy = m.cellsCentersY[cn] - m.cellsCentersY[i];
+            a += dx * dx; b += dx * dy; d += dy * dy;
+        }
+        double det = a * d - b * b;
+        hlsq[ci] = d / det; hlsq[(size_t)nC + ci] = -b / det;
+        hlsq[2 * (size_t)nC + ci] = -b / det; hlsq[3 * (size_t)nC + ci] = a / det;
+    }
+
+    hipStream_t s = h->stream;
+    // geometry
+    HIPCHK_C(h->xc.alloc(nT)); HIPCHK_C(h->xc.upload(to_device_order_cells<double>(h, m.cellsCentersX), s));
+    HIPCHK_C(h->yc.alloc(nT)); HIPCHK_C(h->yc.upload(to_device_order_cells<double>(h, m.cellsCentersY), s));
+    HIPCHK_C(h->area.alloc(nT)); HIPCHK_C(h->area.upload(to_device_order_cells<double>(h, m.cellsAreas), s));
+    HIPCHK_C(h->xc32.alloc(nT)); HIPCHK_C(h->xc32.upload(to_device_order_cells<float>(h, m.cellsCentersX), s));
+    HIPCHK_C(h->yc32.alloc(nT)); HIPCHK_C(h->yc32.upload(to_device_order_cells<float>(h, m.cellsCentersY), s));
+    HIPCHK_C(h->area32.alloc(nT)); HIPCHK_C(h->area32.upload(to_device_order_cells<float>(h, m.cellsAreas), s));
+    HIPCHK_C(h->fnx.alloc(nF)); HIPCHK_C(h->fnx.upload(to_device_order_faces<double>(h, m.edgesNormalsX), s));
+    HIPCHK_C(h->fny.alloc(nF)); HIPCHK_C(h->fny.upload(to_device_order_faces<double>(h, m.edgesNormalsY), s));
+    HIPCHK_C(h->flen.alloc(nF)); HIPCHK_C(h->flen.upload(to_device_order_faces<double>(h, m.edgesLengths), s));
+    HIPCHK_C(h->fxf.alloc(nF)); HIPCHK_C(h->fxf.upload(to_device_order_faces<double>(h, m.edgesCentersX), s));
+    HIPCHK_C(h->fyf.alloc(nF)); HIPCHK_C(h->fyf.upload(to_device_order_faces<double>(h, m.edgesCentersY), s));
+    HIPCHK_C(h->fnx32.alloc(nF)); HIPCHK_C(h->fnx32.upload(to_device_order_faces<float>(h, m.edgesNormalsX), s));
+    HIPCHK_C(h->fny32.alloc(nF)); HIPCHK_C(h->fny32.upload(to_device_order_faces<float>(h, m.edgesNormalsY), s));
+    HIPCHK_C(h->flen32.alloc(nF)); HIPCHK_C(h->flen32.upload(to_device_order_faces<float>(h, m.edgesLengths), s));
+    HIPCHK_C(h->fxf32.alloc(nF)); HIPCHK_C(h->fxf32.upload(to_device_order_faces<float>(h, m.edgesCentersX), s));
+    HIPCHK_C(h->fyf32.alloc(nF)); HIPCHK_C(h->fyf32.upload(to_device_order_faces<float>(h, m.edgesCentersY), s));
+    HIPCHK_C(h->fgw.alloc(nF)); HIPCHK_C(h->fgw.upload(hgw, s));
+    {
+        std::vector<float> g32(hgw.begin(), hgw.end());
+        HIPCHK_C(h->fgw32.alloc(nF)); HIPCHK_C(h->fgw32.upload(g32, s));
+        std::vector<float> l32(hlsq.begin(), hlsq.end());
+        HIPCHK_C(h->lsq.alloc(hlsq.size())); HIPCHK_C(h->lsq.upload(hlsq, s));
+        HIPCHK_C(h->lsq32.alloc(l32.size())); HIPCHK_C(h->lsq32.upload(l32, s));
+        HIPCHK_C(hipStreamSynchronize(s));
+    }
+    HIPCHK_C(h->c0.alloc(nF)); HIPCHK_C(h->c0.upload(hc0, s));
+    HIPCHK_C(h->c1.alloc(nF)); HIPCHK_C(h->c1.upload(hc1, s));
+    HIPCHK_C(h->cface.alloc(hcface.size())); HIPCHK_C(h->cface.upload(hcface, s));
+    {
+        std::vector<int8_t> kind(nF, K_INTERNAL);
+        HIPCHK_C(h->fkind.alloc(nF)); HIPCHK_C(h->fkind.upload(kind, s));
+        HIPCHK_C(hipStreamSynchronize(s));
+    }
+    // state arrays
+    const size_t nq = (size_t)NV * nT;
+    for (DBuf<double>* b : {&h->q, &h->qk, &h->qW, &h->gx, &h->gy, &h->lim, &h->R}) {
+        HIPCHK_C(b->alloc(nq));
+        HIPCHK_C(hipMemsetAsync(b->p, 0, nq * 8, s));
+    }
+    for (DBuf<float>* b : {&h->q32, &h->gx32, &h->gy32, &h->lim32, &h->R32}) {
+        HIPCHK_C(b->alloc(nq));
+        HIPCHK_C(hipMemsetAsync(b->p, 0, nq * 4, s));
+    }
+    HIPCHK_C(h->dt.alloc(nT)); HIPCHK_C(hipMemsetAsync(h->dt.p, 0, (size_t)nT * 8, s));
+    HIPCHK_C(h->wall_d.alloc(nT)); HIPCHK_C(hipMemsetAsync(h->wall_d.p, 0, (size_t)nT * 8, s));
+    HIPCHK_C(h->flux.alloc((size_t)NV * nF));
+    HIPCHK_C(h->flux32.alloc((size_t)NV * nF));
+    // ghost limiters are 1 and ghost gradients 0 (solver.h:466-469,519,540)
+    {
+        std::vector<double> ones(nq, 1.0);
+        std::vector<float> ones32(nq, 1.0f);
+        HIPCHK_C(h->lim.upload(ones, s));
+        HIPCHK_C(h->lim32.upload(ones32, s));
+        HIPCHK_C(hipStreamSynchronize(s));
+    }
+    h->nblk_red = 1024;
+    HIPCHK_C(h->partial.alloc(h->nblk_red));
+    HIPCHK_C(h->sum1.alloc(1));
+    HIPCHK_C(h->uni64.alloc(NV));
+    HIPCHK_C(h->uni32.alloc(NV));
+    HIPCHK_C(hipStreamSynchronize(s));
+#undef HIPCHK_C
+    *out = h;
+    return AFX_OK;
+}
+
+int afx_rans_set_bcs(afx_rans* h, int32_t n, const char* const* names, const int32_t* types, const double* vars) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!names || !types || !vars))) return fail(AFX_EARG, "bad BC arrays");
+    const Mesh& m = h->m;
+    std::vector<int32_t> type_of_name(m.names.size(), -1);
+    std::vector<int32_t> entry_of_name(m.names.size(), -1);
+    for (int32_t i = 0; i < n; ++i) {
+        if (types[i] < AFX_BC_FARFIELD || types[i] > AFX_BC_WALL)
+            return fail(AFX_EUNSUP, std::string("unsupported boundary type for '") + names[i] + "'");
+        for (size_t k = 0; k < m.names.size(); ++k)
+            if (m.names[k] == names[i]) { type_of_name[k] = types[i]; entry_of_name[k] = i; }
+    }
+    std::vector<int32_t> bt(h->nB);
+    std::vector<double> bv((size_t)4 * h->nB);
+    std::vector<int8_t> kind(h->nF, K_INTERNAL);
+    std::vector<int32_t> wo, wg, bo, bg;
+    for (int32_t b = 0; b < h->nB; ++b) {
+        int ni = m.boundaryNameIdx[b];
+        if (type_of_name[ni] < 0) return fail(AFX_EARG, "no boundary condition for patch '" + m.names[ni] + "'");
+        bt[b] = type_of_name[ni];
+        double v[4] = {0.2, 0, 1, 1.};   // boundary_variables() defaults (core.h:62-65)
+        if (bt[b] == AFX_BC_FARFIELD)
+            for (int k = 0; k < 4; ++k) v[k] = vars[4 * entry_of_name[ni] + k];
+        for (int k = 0; k < 4; ++k) bv[4 * (size_t)b + k] = v[k];
+        uint32_t e = m.boundaryEdges[b];
+        kind[h->f_old2new[e]] = (int8_t)bt[b];   // AFX_BC_* == K_* for the three BC functors
+        int32_t own = h->c_old2new[m.edgesCells[2 * (size_t)e]], gh = h->c_old2new[m.edgesCells[2 * (size_t)e + 1]];
+        bo.push_back(own); bg.push_back(gh);
+        if (bt[b] == AFX_BC_WALL || bt[b] == AFX_BC_SLIP) { wo.push_back(own); wg.push_back(gh); }
+    }
+    hipStream_t s = h->stream;
+    HIPCHK(h->fkind.upload(kind, s));
+    h->n_walls = (int32_t)wo.size();
+    HIPCHK(h->w_owner.alloc(wo.size())); HIPCHK(h->w_owner.upload(wo, s));
+    HIPCHK(h->w_ghost.alloc(wg.size())); HIPCHK(h->w_ghost.upload(wg, s));
+    HIPCHK(h->b_owner.alloc(bo.size())); HIPCHK(h->b_owner.upload(bo, s));
+    HIPCHK(h->b_ghost.alloc(bg.size())); HIPCHK(h->b_ghost.upload(bg, s));
+    HIPCHK(hipStreamSynchronize(s));
+    h->bface_type = bt;
+    h->bface_vars = bv;
+    h->bcs_set = true;
+    // freestream state used by the uniform residual
+    double v[4], qf[4];
+    boundary_variables(h, v);
+    conservative(h->gas, v, qf);
+    float qf32[4] = {(float)qf[0], (float)qf[1], (float)qf[2], (float)qf[3]};
+    HIPCHK(h->uni64.upload(qf, 4, s));
+    HIPCHK(h->uni32.upload(qf32, 4, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return AFX_OK;
+}
+
+int afx_rans_set_options(afx_rans* h, const afx_opts* o) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (!o) return fail(AFX_EARG, "null options");
+    if (o->gradient != 0 && o->gradient != 1) return fail(AFX_EARG, "gradient must be 0 (least-squares) or 1 (green-gauss)");
+    h->opts.second_order = o->second_order;
+    h->opts.gradient = o->gradient;
+    h->opts.limiter_k = o->limiter_k;
+    return AFX_OK;
+}
+
+int afx_rans_init(afx_rans* h) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    double v[4], qi[4];
+    boundary_variables(h, v);
+    conservative(h->gas, v, qi);
+    std::vector<double> tmp((size_t)NV * h->nT);
+    HIPCHK(hipMemcpyAsync(tmp.data(), h->q.p, tmp.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < NV; ++k)
+        for (int32_t i = 0; i < h->nC; ++i) tmp[(size_t)k * h->nT + i] = qi[k];
+    HIPCHK(hipMemcpyAsync(h->q.p, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->q32_valid = false;
+    return AFX_OK;
+}
+
+int afx_rans_refill_bcs(afx_rans* h) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    // ghost b is device cell nC + b (ghosts keep reference order)
+    std::vector<double> g((size_t)NV * h->nB);
+    for (int32_t b = 0; b < h->nB; ++b) {
+        double qc[4];
+        conservative(h->gas, &h->bface_vars[4 * (size_t)b], qc);
+        for (int k = 0; k < NV; ++k) g[(size_t)k * h->nB + b] = qc[k];
+    }
+    for (int k = 0; k < NV; ++k)
+        HIPCHK(hipMemcpyAsync(h->q.p + (size_t)k * h->nT + h->nC, g.data() + (size_t)k * h->nB, (size_t)h->nB * 8,
+                              hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->q32_valid = false;
+    return AFX_OK;
+}
+
+int afx_rans_bcs_from_internal(afx_rans* h) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    HIPCHK(launch_set_walls<double>(h->stream, h->b_owner.p, h->b_ghost.p, h->nB, h->q.p, h->nT, NV));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->q32_valid = false;
+    return AFX_OK;
+}
+
+int afx_rans_set_walls_from_internal(afx_rans* h) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    HIPCHK(launch_set_walls<double>(h->stream, h->w_owner.p, h->w_ghost.p, h->n_walls, h->q.p, h->nT, NV));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->q32_valid = false;
+    return AFX_OK;
+}
+
+int afx_rans_set_state(afx_rans* h, const double* q) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (!q) return fail(AFX_EARG, "null state");
+    h->q32_valid = false;
+    return upload_aos(h, q, h->q.p);
+}
+
+int afx_rans_get_state(afx_rans* h, double* q) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (!q) return fail(AFX_EARG, "null state");
+    return download_aos(h, h->q.p, q);
+}
+
+int afx_rans_set_cfl(afx_rans* h, double cfl) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    h->cfl = cfl;
+    return AFX_OK;
+}
+
+int afx_rans_calc_dt(afx_rans* h, double* dt_out) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    HIPCHK(launch_dt<double>(h->stream, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->cfl, h->dt.p));
+    if (dt_out) return download_cells(h, h->dt.p, dt_out);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return AFX_OK;
+}
+
+int afx_rans_gradients(afx_rans* h, int32_t precision, double* gx, double* gy, double* limiters) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (precision == AFX_FP64) {
+        rc = run_gradients64(h, h->q.p, h->q.p);
+        if (rc) return rc;
+        if (gx && (rc = download_aos(h, h->gx.p, gx))) return rc;
+        if (gy && (rc = download_aos(h, h->gy.p, gy))) return rc;
+        if (limiters && (rc = download_aos(h, h->lim.p, limiters))) return rc;
+    } else if (precision == AFX_FP32) {
+        if ((rc = ensure_q32(h))) return rc;
+        if ((rc = run_gradients32(h))) return rc;
+        if (gx && (rc = download_aos32(h, h->gx32.p, gx))) return rc;
+        if (gy && (rc = download_aos32(h, h->gy32.p, gy))) return rc;
+        if (limiters && (rc = download_aos32(h, h->lim32.p, limiters))) return rc;
+    } else {
+        return fail(AFX_EARG, "precision must be AFX_FP64 or AFX_FP32");
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return AFX_OK;
+}
+
+int afx_rans_residual(afx_rans* h, int32_t precision, double* R, double* norm2) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (precision != AFX_FP64 && precision != AFX_FP32) return fail(AFX_EARG, "precision must be AFX_FP64 or AFX_FP32");
+    if ((rc = enqueue_residual(h, precision))) return rc;
+    if (precision == AFX_FP64) {
+        if (R && (rc = download_aos(h, h->R.p, R))) return rc;
+        if (norm2 && (rc = norm_of(h, h->R.p, h->nT, NV, norm2))) return rc;
+    } else {
+        if (R && (rc = download_aos32(h, h->R32.p, R))) return rc;
+        if (norm2 && (rc = norm_of32(h, h->R32.p, h->nT, NV, norm2))) return rc;
+    }
+    return AFX_OK;
+}
+
+int afx_rans_explicit_residual(afx_rans* h, const double* qk, double* qW) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!qk) return fail(AFX_EARG, "null stage state");
+    if ((rc = upload_aos(h, qk, h->qk.p))) return rc;
+    if (h->needs_gradients() && (rc = run_gradients64(h, h->q.p, h->qk.p))) return rc;
+    HIPCHK(launch_face_flux<double>(h->stream, h->cells64(), h->faces64(), h->qk.p, h->q.p, h->gx.p, h->gy.p, h->lim.p,
+                                    h->gas64(), h->opts.viscosity, h->opts.second_order, nullptr, h->flux.p));
+    HIPCHK(launch_gather<double>(h->stream, h->cells64(), h->flux.p, h->nF, 1, h->qW.p));
+    if (qW) return download_aos(h, h->qW.p, qW);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return AFX_OK;
+}
+
+int afx_rans_explicit_step(afx_rans* h, double relaxation, double* err) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    static const double alpha[3] = {0.25, 0.5, 1.};
+    hipStream_t s = h->stream;
+    const int64_t nq = (int64_t)NV * h->nT;
+    HIPCHK(launch_dt<double>(s, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->cfl, h->dt.p));
+    HIPCHK(launch_copy(s, h->q.p, h->qk.p, nq));
+    for (double ai : alpha) {
+        if (h->needs_gradients() && (rc = run_gradients64(h, h->q.p, h->qk.p))) return rc;
+        HIPCHK(launch_face_flux<double>(s, h->cells64(), h->faces64(), h->qk.p, h->q.p, h->gx.p, h->gy.p, h->lim.p,
+                                        h->gas64(), h->opts.viscosity, h->opts.second_order, nullptr, h->flux.p));
+        HIPCHK(launch_gather<double>(s, h->cells64(), h->flux.p, h->nF, 1, h->qW.p));
+        HIPCHK(launch_rk_update(s, h->q.p, h->qW.p, h->dt.p, ai, relaxation, h->nC, h->nT, NV, h->qk.p));
+    }
+    HIPCHK(launch_copy(s, h->qk.p, h->q.p, nq));
+    h->q32_valid = false;
+    if (err) return norm_of(h, h->qW.p, h->nT, NV, err);
+    HIPCHK(hipStreamSynchronize(s));
+    return AFX_OK;
+}
+
+int afx_rans_uniform_residual(afx_rans* h, double* err) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    HIPCHK(launch_face_flux<double>(h->stream, h->cells64(), h->faces64(), h->q.p, h->q.p, h->gx.p, h->gy.p, h->lim.p,
+                                    h->gas64(), h->opts.viscosity, h->opts.second_order, h->uni64.p, h->flux.p));
+    HIPCHK(launch_gather<double>(h->stream, h->cells64(), h->flux.p, h->nF, 2, h->qW.p));
+    double e = 0;
+    if ((rc = norm_of(h, h->qW.p, h->nT, NV, &e))) return rc;
+    if (err) *err = e;
+    return AFX_OK;
+}
+
+int afx_rans_wall_forces(afx_rans* h, const char* patch, double* cl, double* cd, double* cm) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!patch) return fail(AFX_EARG, "null patch name");
+    const Mesh& m = h->m;
+    std::vector<int32_t> bsel, cells;
+    for (int32_t b = 0; b < h->nB; ++b)
+        if (m.names[m.boundaryNameIdx[b]] == patch) {
+            bsel.push_back(b);
+            cells.push_back(h->c_old2new[m.edgesCells[2 * (size_t)m.boundaryEdges[b]]]);
+        }
+    if (bsel.empty()) return fail(AFX_EARG, std::string("no boundary faces on patch '") + patch + "'");
+    if (h->patch_cells.n < cells.size()) {
+        if (h->patch_cells.p) { (void)hipFree(h->patch_cells.p); h->patch_cells.p = nullptr; }
+        if (h->states.p) { (void)hipFree(h->states.p); h->states.p = nullptr; }
+        HIPCHK(h->patch_cells.alloc(cells.size()));
+        HIPCHK(h->states.alloc(cells.size() * NV));
+    }
+    HIPCHK(h->patch_cells.upload(cells, h->stream));
+    HIPCHK(launch_gather_states(h->stream, h->q.p, h->nT, NV, h->patch_cells.p, (int32_t)cells.size(), h->states.p));
+    std::vector<double> st(cells.size() * NV);
+    HIPCHK(hipMemcpyAsync(st.data(), h->states.p, st.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    // post.h:301-387, in the reference's boundary order
+    double v[4];
+    boundary_variables(h, v);
+    const double p_inf = v[3], mach_inf = v[0], aoa = v[1];
+    double xmin = 0., xmax = 0., y_moment = 0.;
+    uint32_t n_added = 0;
+    for (size_t k = 0; k < bsel.size(); ++k) {
+        uint32_t e = m.boundaryEdges[bsel[k]];
+        if (k == 0) { xmin = xmax = m.edgesCentersX[e]; y_moment = m.edgesCentersY[e]; }
+        else { xmin = std::min(xmin, m.edgesCentersX[e]); xmax = std::max(xmax, m.edgesCentersX[e]); y_moment += m.edgesCentersY[e]; }
+        n_added += 1;
+    }
+    y_moment /= (double)n_added;
+    const double x_moment = (xmax - xmin) * 0.25 + xmin;
+    const double gm = h->gas.gamma;
+    double fx = 0., fy = 0., mm = 0.;
+    for (size_t k = 0; k < bsel.size(); ++k) {
+        uint32_t e = m.boundaryEdges[bsel[k]];
+        const double* qc = &st[k * NV];
+        const double p = (gm - 1) * (qc[3] - 0.5 / qc[0] * (qc[1] * qc[1] + qc[2] * qc[2]));
+        const double cp = 2. / (gm * mach_inf * mach_inf) * (p / p_inf - 1.);
+        const double ecx = m.edgesCentersX[e], ecy = m.edgesCentersY[e];
+        const double fxi = cp * m.edgesNormalsX[e] * m.edgesLengths[e] / (xmax - xmin);
+        const double fyi = cp * m.edgesNormalsY[e] * m.edgesLengths[e] / (xmax - xmin);
+        const double mi = (ecx - x_moment) / (xmax - xmin) * fyi - (ecy - y_moment) / (xmax - xmin) * fxi;
+        fx += fxi; fy += fyi; mm -= mi;
+    }
+    if (cd) *cd = fx * std::cos(aoa) + fy * std::sin(aoa);
+    if (cl) *cl = -fx * std::sin(aoa) + fy * std::cos(aoa);
+    if (cm) *cm = mm;
+    return AFX_OK;
+}
+
+int afx_rans_wall_dist(afx_rans* h, double* wall_dist) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    const Mesh& m = h->m;
+    std::vector<double> wx, wy;
+    for (int32_t b = 0; b < h->nB; ++b)
+        if (h->bface_type[b] == AFX_BC_WALL || h->bface_type[b] == AFX_BC_SLIP) {
+            wx.push_back(m.edgesCentersX[m.boundaryEdges[b]]);
+            wy.push_back(m.edgesCentersY[m.boundaryEdges[b]]);
+        }
+    DBuf<double> dwx, dwy;
+    HIPCHK(dwx.alloc(wx.size())); HIPCHK(dwx.upload(wx, h->stream));
+    HIPCHK(dwy.alloc(wy.size())); HIPCHK(dwy.upload(wy, h->stream));
+    HIPCHK(launch_wall_dist(h->stream, h->xc.p, h->yc.p, h->nT, dwx.p, dwy.p, (int32_t)wx.size(), h->wall_d.p));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (wall_dist) return download_cells(h, h->wall_d.p, wall_dist);
+    return AFX_OK;
+}
+
+int afx_rans_counts(afx_rans* h, int64_t c[5]) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    c[0] = h->m.n_nodes(); c[1] = h->nC; c[2] = h->nT; c[3] = h->nF; c[4] = h->nB;
+    return AFX_OK;
+}
+
+int afx_rans_stream(afx_rans* h, void** stream) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    *stream = (void*)h->stream;
+    return AFX_OK;
+}
+
+int afx_rans_launch_stage(afx_rans* h, int32_t precision, int32_t stage, int32_t n) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (precision != AFX_FP64 && precision != AFX_FP32) return fail(AFX_EARG, "bad precision");
+    if (stage < 0 || stage > 3 || n < 0) return fail(AFX_EARG, "bad stage");
+    if (precision == AFX_FP32 && (rc = ensure_q32(h))) return rc;
+    for (int32_t it = 0; it < n; ++it) {
+        if (stage == 3) { if ((rc = enqueue_residual(h, precision))) return rc; continue; }
+        if (precision == AFX_FP64) {
+            if (stage == 0) { if ((rc = run_gradients64(h, h->q.p, h->q.p))) return rc; }
+            else if (stage == 1) {
+                HIPCHK(launch_face_flux<double>(h->stream, h->cells64(), h->faces64(), h->q.p, h->q.p, h->gx.p, h->gy.p,
+                                                h->lim.p, h->gas64(), h->opts.viscosity, h->opts.second_order, nullptr,
+                                                h->flux.p));
+            } else {
+                HIPCHK(launch_gather<double>(h->stream, h->cells64(), h->flux.p, h->nF, 0, h->R.p));
+            }
+        } else {
+            if (stage == 0) { if ((rc = run_gradients32(h))) return rc; }
+            else if (stage == 1) {
+                HIPCHK(launch_face_flux<float>(h->stream, h->cells32(), h->faces32(), h->q32.p, h->q32.p, h->gx32.p,
+                                               h->gy32.p, h->lim32.p, h->gas32(), h->opts.viscosity,
+                                               h->opts.second_order, nullptr, h->flux32.p));
+            } else {
+                HIPCHK(launch_gather<float>(h->stream, h->cells32(), h->flux32.p, h->nF, 0, h->R32.p));
+            }
+        }
+    }
+    return AFX_OK;
+}
+
+int afx_rans_synchronize(afx_rans* h) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return AFX_OK;
+}
+
+int afx_rans_destroy(afx_rans* h) {
+    if (!h) return AFX_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    delete h;
+    return AFX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,7 @@
+// FP32 instantiation of the residual kernels (throughput form; built with
+// FMA contraction allowed).
+#include "afx_device.hpp"
+
+namespace afx {
+AFX_INSTANTIATE_LAUNCHERS(float)
+}  // namespace afx

@@ -1,0 +1,369 @@
+"""Python mirror of the reference's rans::mesh / rans::solver interface over the
+C ABI in include/afx_rans.h (libafx_rans.so).
+
+Names and argument meanings follow src/rans/include/rans:
+  Mesh.read_file(path)                       mesh.h:834-884
+  Solver(mesh, gas, viscosity_model)         solver.h:108-110
+  Solver.set_bcs({name: BoundaryCondition})  solver.h:200-247
+  set_second_order / set_gradient_scheme / set_limiter_k / set_cfl
+  init / refill_bcs / bcs_from_internal / get_q / set_q
+  calc_dt                                    solver.h:308-356
+  fillRhoRHS                                 solver.h:1079-1152 (returns RhoVector)
+  explicit_solve(relaxation)                 solver.h:802-828
+  get_uniform_residual                       solver.h:636-690
+  get_wall_profile(patch)                    post.h:301-387
+Vectors use the reference AoS layout q(4*i+k) over real + ghost cells.
+
+Errors raise RansError (the reference raises std::runtime_error /
+std::invalid_argument / std::out_of_range at the same places).  There is no
+CPU fallback: when the HIP library is missing, importing this module fails.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libafx_rans.so")
+
+AFX_FP64 = 0
+AFX_FP32 = 1
+BC_TYPES = {"farfield": 1, "slip-wall": 2, "wall": 3}
+
+
+class RansError(RuntimeError):
+    pass
+
+
+class AfxGas(C.Structure):
+    _fields_ = [("R", C.c_double), ("mu_L", C.c_double), ("Pr_L", C.c_double),
+                ("Pr_T", C.c_double), ("gamma", C.c_double)]
+
+
+class AfxOpts(C.Structure):
+    _fields_ = [("viscosity", C.c_int32), ("second_order", C.c_int32), ("gradient", C.c_int32),
+                ("limiter_k", C.c_double), ("renumber", C.c_int32), ("reserved", C.c_int32 * 7)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m aeroflex_amd.build` (or __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    i32, i64, f64 = C.c_int32, C.c_int64, C.c_double
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "afx_last_error": (C.c_char_p, []),
+        "afx_default_gas": (None, [C.POINTER(AfxGas)]),
+        "afx_default_opts": (None, [C.POINTER(AfxOpts)]),
+        "afx_mesh_read_msh": (C.c_int, [C.c_char_p, C.POINTER(P)]),
+        "afx_mesh_from_elements": (C.c_int, [i64, dp, dp, i64, P, P, i64, P, P, i32, P, P, C.POINTER(P)]),
+        "afx_mesh_gen_naca_ogrid": (C.c_int, [i32, i32, f64, f64, f64, C.POINTER(P)]),
+        "afx_mesh_gen_jittered": (C.c_int, [i32, i32, f64, f64, f64, f64, f64, C.c_uint64, C.POINTER(P)]),
+        "afx_mesh_write_msh": (C.c_int, [P, C.c_char_p]),
+        "afx_mesh_counts": (C.c_int, [P, C.POINTER(i64)]),
+        "afx_mesh_get": (C.c_int, [P, C.c_char_p, P, C.POINTER(i64)]),
+        "afx_mesh_boundary_name": (C.c_int, [P, i64, C.c_char_p, i32]),
+        "afx_mesh_rcm": (C.c_int, [P, P]),
+        "afx_mesh_destroy": (None, [P]),
+        "afx_rans_create": (C.c_int, [P, C.POINTER(AfxGas), C.POINTER(AfxOpts), C.POINTER(P)]),
+        "afx_rans_set_bcs": (C.c_int, [P, i32, P, P, dp]),
+        "afx_rans_set_options": (C.c_int, [P, C.POINTER(AfxOpts)]),
+        "afx_rans_init": (C.c_int, [P]),
+        "afx_rans_refill_bcs": (C.c_int, [P]),
+        "afx_rans_bcs_from_internal": (C.c_int, [P]),
+        "afx_rans_set_walls_from_internal": (C.c_int, [P]),
+        "afx_rans_set_state": (C.c_int, [P, dp]),
+        "afx_rans_get_state": (C.c_int, [P, dp]),
+        "afx_rans_set_cfl": (C.c_int, [P, f64]),
+        "afx_rans_calc_dt": (C.c_int, [P, dp]),
+        "afx_rans_gradients": (C.c_int, [P, i32, dp, dp, dp]),
+        "afx_rans_residual": (C.c_int, [P, i32, dp, dp]),
+        "afx_rans_explicit_residual": (C.c_int, [P, dp, dp]),
+        "afx_rans_explicit_step": (C.c_int, [P, f64, dp]),
+        "afx_rans_uniform_residual": (C.c_int, [P, dp]),
+        "afx_rans_wall_forces": (C.c_int, [P, C.c_char_p, dp, dp, dp]),
+        "afx_rans_wall_dist": (C.c_int, [P, dp]),
+        "afx_rans_counts": (C.c_int, [P, C.POINTER(i64)]),
+        "afx_rans_stream": (C.c_int, [P, C.POINTER(P)]),
+        "afx_rans_launch_stage": (C.c_int, [P, i32, i32, i32]),
+        "afx_rans_synchronize": (C.c_int, [P]),
+        "afx_rans_destroy": (C.c_int, [P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib.afx_last_error().decode(errors="replace")
+        raise RansError(f"afx error {rc}: {msg}")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _vptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Gas:
+    """rans::gas (core.h:27-46)."""
+
+    def __init__(self, R=0.71428571428, mu_L=1e-5, Pr_L=0.72, Pr_T=0.9, gamma=1.4):
+        self.R, self.mu_L, self.Pr_L, self.Pr_T, self.gamma = R, mu_L, Pr_L, Pr_T, gamma
+
+    def c_struct(self):
+        return AfxGas(self.R, self.mu_L, self.Pr_L, self.Pr_T, self.gamma)
+
+
+class BoundaryVariables:
+    """rans::boundary_variables (core.h:61-84); angle in radians."""
+
+    def __init__(self, mach=0.2, angle=0.0, T=1.0, p=1.0):
+        self.mach, self.angle, self.T, self.p = mach, angle, T, p
+
+    def get_conservative(self, g):
+        c = np.sqrt(g.gamma * g.R * self.T)
+        u = self.mach * c * np.cos(self.angle)
+        v = self.mach * c * np.sin(self.angle)
+        rho = self.p / (g.R * self.T)
+        rhoE = self.p / (g.gamma - 1) + 0.5 * rho * (u * u + v * v)
+        return np.array([rho, rho * u, rho * v, rhoE])
+
+
+class BoundaryCondition:
+    """rans::boundary_condition (core.h:112-115)."""
+
+    def __init__(self, bc_type, vars_far=None):
+        if bc_type not in BC_TYPES:
+            raise RansError(f"unsupported boundary type '{bc_type}'")
+        self.bc_type = bc_type
+        self.vars_far = vars_far if vars_far is not None else BoundaryVariables()
+
+
+class Mesh:
+    """rans::mesh (mesh.h:198-273), host side, reference order."""
+
+    _U32 = {"edgesNodes", "edgesCells", "cellsNodes", "cellsEdges", "boundaryEdges", "boundaryEdgesIntTag"}
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    @classmethod
+    def read_file(cls, path):
+        h = C.c_void_p()
+        _check(lib.afx_mesh_read_msh(str(path).encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def naca_ogrid(cls, ni=128, nj=65, r_far=20.0, thickness=0.12, first_h=2e-3):
+        h = C.c_void_p()
+        _check(lib.afx_mesh_gen_naca_ogrid(ni, nj, r_far, thickness, first_h, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def jittered(cls, ni, nj, r_far=20.0, thickness=0.12, first_h=1e-3, jitter=0.1, quad_frac=0.25, seed=1):
+        h = C.c_void_p()
+        _check(lib.afx_mesh_gen_jittered(ni, nj, r_far, thickness, first_h, jitter, quad_frac, seed, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_elements(cls, x, y, cells, nverts, bsegs, bkeys, names):
+        """names: dict physical key -> name."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        cells = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 4)
+        nverts = np.ascontiguousarray(nverts, dtype=np.int32)
+        bsegs = np.ascontiguousarray(bsegs, dtype=np.int32).reshape(-1, 2)
+        bkeys = np.ascontiguousarray(bkeys, dtype=np.int32)
+        keys = np.array(list(names.keys()), dtype=np.int32)
+        cn = (C.c_char_p * len(names))(*[v.encode() for v in names.values()])
+        h = C.c_void_p()
+        _check(lib.afx_mesh_from_elements(len(x), _dptr(x), _dptr(y), len(cells), _vptr(cells), _vptr(nverts),
+                                          len(bsegs), _vptr(bsegs), _vptr(bkeys), len(keys), _vptr(keys),
+                                          C.cast(cn, C.c_void_p), C.byref(h)))
+        return cls(h)
+
+    def write_msh(self, path):
+        _check(lib.afx_mesh_write_msh(self.h, str(path).encode()))
+
+    def counts(self):
+        c = (C.c_int64 * 5)()
+        _check(lib.afx_mesh_counts(self.h, c))
+        return dict(zip(("nodes", "real_cells", "cells", "faces", "bfaces"), list(c)))
+
+    def get(self, field):
+        n = C.c_int64()
+        _check(lib.afx_mesh_get(self.h, field.encode(), None, C.byref(n)))
+        dt = np.uint32 if field in self._U32 else (np.int32 if field == "cellsIsTriangle" else np.float64)
+        out = np.empty(n.value, dtype=dt)
+        _check(lib.afx_mesh_get(self.h, field.encode(), _vptr(out), C.byref(n)))
+        return out
+
+    def boundary_names(self):
+        nb = self.counts()["bfaces"]
+        buf = C.create_string_buffer(256)
+        out = []
+        for b in range(nb):
+            _check(lib.afx_mesh_boundary_name(self.h, b, buf, 256))
+            out.append(buf.value.decode())
+        return out
+
+    def rcm(self):
+        p = np.empty(self.counts()["real_cells"], dtype=np.int32)
+        _check(lib.afx_mesh_rcm(self.h, _vptr(p)))
+        return p
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and lib is not None:
+            lib.afx_mesh_destroy(self.h)
+            self.h = None
+
+
+VISCOSITY = {"inviscid": 0, "laminar": 1}
+GRADIENT = {"least-squares": 0, "green-gauss": 1}
+
+
+class Solver:
+    """Device residual behind the rans::solver interface (solver.h:40-175)."""
+
+    def __init__(self, mesh, gas=None, viscosity_model="inviscid", second_order=True,
+                 gradient_scheme="green-gauss", limiter_k=5.0, renumber=True):
+        if viscosity_model not in VISCOSITY:
+            raise RansError(f"unsupported viscosity model '{viscosity_model}'")
+        self.gas = gas if gas is not None else Gas()
+        self.mesh = mesh
+        o = AfxOpts()
+        lib.afx_default_opts(C.byref(o))
+        o.viscosity = VISCOSITY[viscosity_model]
+        o.second_order = int(second_order)
+        o.gradient = GRADIENT[gradient_scheme]
+        o.limiter_k = limiter_k
+        o.renumber = int(renumber)
+        self._opts = o
+        g = self.gas.c_struct()
+        h = C.c_void_p()
+        _check(lib.afx_rans_create(mesh.h, C.byref(g), C.byref(o), C.byref(h)))
+        self.h = h
+        c = (C.c_int64 * 5)()
+        _check(lib.afx_rans_counts(self.h, c))
+        self.n_real, self.n_cells, self.n_faces, self.n_bfaces = c[1], c[2], c[3], c[4]
+        self.bcs = {}
+
+    # -- options (solver.h:135,149-152,162,250-252)
+    def _push_opts(self):
+        _check(lib.afx_rans_set_options(self.h, C.byref(self._opts)))
+
+    def set_second_order(self, x=True):
+        self._opts.second_order = int(x)
+        self._push_opts()
+
+    def set_gradient_scheme(self, s):
+        self._opts.gradient = GRADIENT[s]
+        self._push_opts()
+
+    def set_limiter_k(self, k):
+        self._opts.limiter_k = k
+        self._push_opts()
+
+    def set_cfl(self, cfl):
+        _check(lib.afx_rans_set_cfl(self.h, cfl))
+
+    def set_bcs(self, bcs):
+        self.bcs = dict(bcs)
+        names = list(bcs.keys())
+        cn = (C.c_char_p * len(names))(*[n.encode() for n in names])
+        types = np.array([BC_TYPES[bcs[n].bc_type] for n in names], dtype=np.int32)
+        vars_ = np.array([[bcs[n].vars_far.mach, bcs[n].vars_far.angle, bcs[n].vars_far.T, bcs[n].vars_far.p]
+                          for n in names], dtype=np.float64).reshape(-1)
+        _check(lib.afx_rans_set_bcs(self.h, len(names), C.cast(cn, C.c_void_p), _vptr(types), _dptr(vars_)))
+
+    def init(self):
+        _check(lib.afx_rans_init(self.h))
+
+    def refill_bcs(self):
+        _check(lib.afx_rans_refill_bcs(self.h))
+
+    def bcs_from_internal(self):
+        _check(lib.afx_rans_bcs_from_internal(self.h))
+
+    def set_walls_from_internal(self):
+        _check(lib.afx_rans_set_walls_from_internal(self.h))
+
+    def get_q(self):
+        q = np.empty(4 * self.n_cells)
+        _check(lib.afx_rans_get_state(self.h, _dptr(q)))
+        return q
+
+    def set_q(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if q.size != 4 * self.n_cells:
+            raise RansError("state has the wrong length")
+        _check(lib.afx_rans_set_state(self.h, _dptr(q)))
+
+    def calc_dt(self):
+        dt = np.empty(self.n_cells)
+        _check(lib.afx_rans_calc_dt(self.h, _dptr(dt)))
+        return dt
+
+    def gradients(self, precision=AFX_FP64):
+        gx, gy, lim = (np.empty(4 * self.n_cells) for _ in range(3))
+        _check(lib.afx_rans_gradients(self.h, precision, _dptr(gx), _dptr(gy), _dptr(lim)))
+        return gx, gy, lim
+
+    def fillRhoRHS(self, precision=AFX_FP64):
+        R = np.empty(4 * self.n_cells)
+        nrm = C.c_double()
+        _check(lib.afx_rans_residual(self.h, precision, _dptr(R), C.byref(nrm)))
+        return R, nrm.value
+
+    def explicit_residual(self, qk):
+        qk = np.ascontiguousarray(qk, dtype=np.float64)
+        qW = np.empty(4 * self.n_cells)
+        _check(lib.afx_rans_explicit_residual(self.h, _dptr(qk), _dptr(qW)))
+        return qW
+
+    def explicit_solve(self, relaxation=1.0):
+        e = C.c_double()
+        _check(lib.afx_rans_explicit_step(self.h, relaxation, C.byref(e)))
+        return e.value
+
+    def get_uniform_residual(self):
+        e = C.c_double()
+        _check(lib.afx_rans_uniform_residual(self.h, C.byref(e)))
+        return e.value
+
+    def get_wall_profile(self, patch="wall"):
+        cl, cd, cm = C.c_double(), C.c_double(), C.c_double()
+        _check(lib.afx_rans_wall_forces(self.h, patch.encode(), C.byref(cl), C.byref(cd), C.byref(cm)))
+        return cl.value, cd.value, cm.value
+
+    def wall_dist(self):
+        d = np.empty(self.n_cells)
+        _check(lib.afx_rans_wall_dist(self.h, _dptr(d)))
+        return d
+
+    def stream(self):
+        s = C.c_void_p()
+        _check(lib.afx_rans_stream(self.h, C.byref(s)))
+        return s.value
+
+    def launch_stage(self, precision, stage, n):
+        _check(lib.afx_rans_launch_stage(self.h, precision, stage, n))
+
+    def synchronize(self):
+        _check(lib.afx_rans_synchronize(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and lib is not None:
+            lib.afx_rans_destroy(self.h)
+            self.h = None

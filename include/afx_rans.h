@@ -1,0 +1,174 @@
+/*
+ * afx_rans.h -- C ABI of the MI355X-native residual of AeroFLEX's
+ * unstructured finite-volume RANS solver (reference: src/rans/include/rans).
+ *
+ * Conventions
+ *   - Every function returns int: AFX_OK (0) or a negative AFX_E* code;
+ *     afx_last_error() returns a thread-local description.  No C++ exception
+ *     crosses this boundary (the reference throws std::runtime_error /
+ *     std::invalid_argument / std::out_of_range at the corresponding places).
+ *   - Host arrays are caller-owned and copied; all vectors exchanged with the
+ *     caller are in the reference's order and layout: state is the
+ *     Eigen::VectorXd layout q(4*i+k) over all cells (real cells, then ghost
+ *     cells in boundary-segment order, solver.h:54,184), per-face arrays are in
+ *     face-creation order (mesh.h:317-341).
+ *   - One HIP stream per solver handle; calls are synchronous unless stated.
+ *     Handles are thread-compatible, not thread-safe.
+ */
+#ifndef AFX_RANS_H
+#define AFX_RANS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AFX_OK 0
+#define AFX_EARG (-1)        /* invalid argument / unknown name            */
+#define AFX_EHIP (-2)        /* HIP runtime error                          */
+#define AFX_EIO (-3)         /* file / parse error                         */
+#define AFX_ESTATE (-4)      /* call out of order (e.g. no BCs set)        */
+#define AFX_EUNSUP (-5)      /* unsupported option combination            */
+
+/* boundary-condition types (solver.h:216-230 strings) */
+#define AFX_BC_OTHER 0       /* any other name: interior Roe flux with the ghost */
+#define AFX_BC_FARFIELD 1    /* "farfield"  -> farfield_flux  (physics.h:409-530) */
+#define AFX_BC_SLIP 2        /* "slip-wall" -> slip_wall_flux (physics.h:279-339) */
+#define AFX_BC_WALL 3        /* "wall"      -> wall_flux      (physics.h:344-405) */
+
+/* residual precision forms */
+#define AFX_FP64 0           /* plain form: FP64 everywhere, reference operation order */
+#define AFX_FP32 1           /* fluxes/gradients/limiters FP32, compensated FP32 cell sums */
+
+typedef struct afx_mesh afx_mesh;
+typedef struct afx_rans afx_rans;
+
+/* core.h:27-46 (rans::gas; cp is fixed to 1 as in the reference) */
+typedef struct afx_gas {
+    double R;        /* 0.71428571428 */
+    double mu_L;     /* 1e-5          */
+    double Pr_L;     /* 0.72          */
+    double Pr_T;     /* 0.9           */
+    double gamma;    /* 1.4           */
+} afx_gas;
+
+/* solver options: core.h:166-232 (rans::Settings) as used by
+ * multigrid.h:72-76 (solver construction + set_second_order /
+ * set_gradient_scheme / set_limiter_k) */
+typedef struct afx_opts {
+    int32_t viscosity;     /* 0 "inviscid", 1 "laminar", 2 Spalart-Allmaras (this
+                              library's SA-noft2; the reference's SA branch is empty) */
+    int32_t second_order;  /* Settings::second_order (default 1) */
+    int32_t gradient;      /* 0 "least-squares", 1 "green-gauss" (default) */
+    double limiter_k;      /* Settings::limiter_k (default 5) */
+    int32_t renumber;      /* 1: RCM cell order on the device (default), 0: file order */
+    int32_t reserved[7];
+} afx_opts;
+
+const char* afx_last_error(void);
+void afx_default_gas(afx_gas* g);
+void afx_default_opts(afx_opts* o);
+
+/* ------------------------------------------------------------------ mesh --
+ * rans::mesh (mesh.h:198-273).  Host object in reference order. */
+
+/* mesh::read_file (mesh.h:834-884): gmsh MSH 4.1 ASCII */
+int afx_mesh_read_msh(const char* path, afx_mesh** out);
+
+/* Build from elements, i.e. what mesh::read_nodes/read_boundaries/read_elements
+ * (mesh.h:524-738) leave before add_cell_edges: node coordinates; cells with
+ * 3 or 4 node indices (cell_nodes: 4 per cell, 4th ignored for triangles);
+ * boundary segments (2 node indices each) with their physical key
+ * (tag + 1000*dim); physical names as parallel arrays of key and name. */
+int afx_mesh_from_elements(int64_t n_nodes, const double* x, const double* y,
+                           int64_t n_cells, const int32_t* cell_nodes, const int32_t* cell_nverts,
+                           int64_t n_bsegs, const int32_t* bseg_nodes, const int32_t* bseg_keys,
+                           int32_t n_names, const int32_t* name_keys, const char* const* names,
+                           afx_mesh** out);
+
+/* Analytic NACA 4-digit O-grid: ni points around, nj from wall to a circle of
+ * r_far chords, quads split into triangles (physical names "farfield",
+ * "wall"). */
+int afx_mesh_gen_naca_ogrid(int32_t ni, int32_t nj, double r_far, double thickness,
+                            double first_h, afx_mesh** out);
+/* Seeded mixed tri/quad mesh of jittered points around a NACA 4-digit section. */
+int afx_mesh_gen_jittered(int32_t ni, int32_t nj, double r_far, double thickness, double first_h,
+                          double jitter, double quad_frac, uint64_t seed, afx_mesh** out);
+int afx_mesh_write_msh(const afx_mesh* m, const char* path);
+/* counts[5] = {n_nodes, n_real_cells, n_cells (real + ghost), n_faces, n_boundary_faces} */
+int afx_mesh_counts(const afx_mesh* m, int64_t counts[5]);
+/* Copy one reference array: nodesX nodesY edgesNodes edgesCells edgesLengths
+ * edgesNormalsX edgesNormalsY edgesCentersX edgesCentersY cellsNodes
+ * cellsEdges cellsIsTriangle cellsAreas cellsCentersX cellsCentersY
+ * boundaryEdges boundaryEdgesIntTag.  Doubles are f64, indices u32 (i32 for
+ * cellsIsTriangle).  out may be NULL to query the length. */
+int afx_mesh_get(const afx_mesh* m, const char* field, void* out, int64_t* n_out);
+/* boundaryEdgesPhysicals[b] (mesh.h:225) */
+int afx_mesh_boundary_name(const afx_mesh* m, int64_t b, char* buf, int32_t buflen);
+/* Cell order the device uses (perm[new] = old real-cell index). */
+int afx_mesh_rcm(const afx_mesh* m, int32_t* perm);
+void afx_mesh_destroy(afx_mesh* m);
+
+/* ---------------------------------------------------------------- solver --
+ * rans::solver / explicitSolver / implicitSolver (solver.h:40-1213). */
+
+/* solver(const mesh&, const gas&, std::string viscosity_model) (solver.h:108-110)
+ * + set_second_order / set_gradient_scheme / set_limiter_k (multigrid.h:73-76) */
+int afx_rans_create(const afx_mesh* m, const afx_gas* g, const afx_opts* o, afx_rans** out);
+/* solver::set_bcs (solver.h:200-247): per physical name its type and, for
+ * farfield, boundary_variables {mach, angle[rad], T, p} (core.h:61-84).
+ * Names absent from the list make the call fail (reference: std::out_of_range
+ * from bcs.at(), solver.h:220). */
+int afx_rans_set_bcs(afx_rans* h, int32_t n, const char* const* names, const int32_t* types,
+                     const double* farfield_vars /* 4*n: mach, angle, T, p */);
+/* set_second_order / set_gradient_scheme / set_limiter_k (solver.h:135,149-152,162);
+ * viscosity and renumber are fixed at creation and ignored here */
+int afx_rans_set_options(afx_rans* h, const afx_opts* o);
+int afx_rans_init(afx_rans* h);                 /* solver::init (solver.h:615-631) */
+int afx_rans_refill_bcs(afx_rans* h);           /* solver::refill_bcs (solver.h:259-274) */
+int afx_rans_bcs_from_internal(afx_rans* h);    /* solver::bcs_from_internal (solver.h:276-287) */
+int afx_rans_set_walls_from_internal(afx_rans* h); /* solver.h:289-305 on q */
+/* get_q() (solver.h:255-257): AoS q(4*i+k), 4*n_cells doubles */
+int afx_rans_set_state(afx_rans* h, const double* q);
+int afx_rans_get_state(afx_rans* h, double* q);
+int afx_rans_set_cfl(afx_rans* h, double cfl);  /* solver::set_cfl (solver.h:250-252) */
+/* solver::calc_dt (solver.h:308-356); dt_out (n_cells, nullable) */
+int afx_rans_calc_dt(afx_rans* h, double* dt_out);
+/* set_walls_from_internal + calc_gradients + calc_limiters on q
+ * (solver.h:1083-1087); outputs AoS 4*n_cells each, nullable */
+int afx_rans_gradients(afx_rans* h, int32_t precision, double* gx, double* gy, double* limiters);
+/* implicitSolver::fillRhoRHS (solver.h:1079-1152) without the calc_dt it
+ * performs first (call afx_rans_calc_dt for that): R = RhoVector (AoS,
+ * nullable), norm2 = ||R||_2 (nullable).  With both outputs NULL the call
+ * only enqueues the kernels on the handle's stream and returns without
+ * synchronising (inputs and outputs stay resident in HBM). */
+int afx_rans_residual(afx_rans* h, int32_t precision, double* R, double* norm2);
+/* explicitSolver::calc_residual (solver.h:745-799) for a stage state qk
+ * (AoS), with gradients from the current q (A3); qW out (AoS, nullable) */
+int afx_rans_explicit_residual(afx_rans* h, const double* qk, double* qW);
+/* explicitSolver::solve (solver.h:802-828): 3-stage pseudo-time step on the
+ * device, err = ||qW||_2 of the last stage */
+int afx_rans_explicit_step(afx_rans* h, double relaxation, double* err);
+/* solver::get_uniform_residual (solver.h:636-690), with the stale-qW rule */
+int afx_rans_uniform_residual(afx_rans* h, double* err);
+/* get_wall_profile (post.h:301-387): CL, CD, CM on the named patch */
+int afx_rans_wall_forces(afx_rans* h, const char* patch, double* cl, double* cd, double* cm);
+/* mesh::compute_wall_dist (mesh.h:794-830) on the device; wall_dist out
+ * (n_cells, nullable).  Requires BCs. */
+int afx_rans_wall_dist(afx_rans* h, double* wall_dist);
+/* counts[5] as afx_mesh_counts */
+int afx_rans_counts(afx_rans* h, int64_t counts[5]);
+/* The handle's hipStream_t (as void*), for event timing by the caller. */
+int afx_rans_stream(afx_rans* h, void** stream);
+/* Enqueue n back-to-back launches of one residual stage without
+ * synchronising (stage: 0 gradients+limiters, 1 face flux, 2 cell gather,
+ * 3 whole residual).  For timing the individual kernels. */
+int afx_rans_launch_stage(afx_rans* h, int32_t precision, int32_t stage, int32_t n);
+int afx_rans_synchronize(afx_rans* h);
+int afx_rans_destroy(afx_rans* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AFX_RANS_H */

@@ -1,0 +1,57 @@
+"""Matched (device solver, oracle solver) pairs on the same mesh and state."""
+import os
+
+import numpy as np
+
+import orc
+
+GOLDEN = orc.GOLDEN
+QUAD = os.path.join(GOLDEN, "naca0012q_coarse.msh")
+TRI = os.path.join(GOLDEN, "naca0012_coarse.msh")
+
+FAR = dict(mach=0.5, angle=2 * 0.01745, T=1.0, p=1.0)
+
+
+def perturbed_state(n_real, n_cells, ghost_q, qinf, seed=2, amp=0.01):
+    """Freestream with seeded relative perturbations on the primitive
+    variables rho, u, v, p of the real cells; ghost rows as given."""
+    rng = np.random.default_rng(seed)
+    g = 1.4
+    rho = qinf[0] * (1 + amp * rng.standard_normal(n_real))
+    u = qinf[1] / qinf[0] * (1 + amp * rng.standard_normal(n_real)) + amp * rng.standard_normal(n_real) * 0.1
+    v = qinf[2] / qinf[0] * (1 + amp * rng.standard_normal(n_real)) + amp * rng.standard_normal(n_real) * 0.1
+    p0 = (g - 1) * (qinf[3] - 0.5 * (qinf[1] ** 2 + qinf[2] ** 2) / qinf[0])
+    p = p0 * (1 + amp * rng.standard_normal(n_real))
+    q = ghost_q.copy()
+    q[0:4 * n_real:4] = rho
+    q[1:4 * n_real:4] = rho * u
+    q[2:4 * n_real:4] = rho * v
+    q[3:4 * n_real:4] = p / (g - 1) + 0.5 * rho * (u * u + v * v)
+    return q
+
+
+def make_pair(path=QUAD, viscosity="inviscid", wall="slip-wall", second_order=True, gradient="green-gauss",
+              limiter_k=5.0, mach=0.5, angle=2 * 0.01745, mu_L=1e-5, seed=2, renumber=True, mesh=None, omesh=None):
+    from aeroflex_amd import rans
+    m = mesh if mesh is not None else rans.Mesh.read_file(path)
+    om = omesh if omesh is not None else orc.OMesh.read(path)
+    gas = rans.Gas(mu_L=mu_L)
+    s = rans.Solver(m, gas=gas, viscosity_model=viscosity, second_order=second_order, gradient_scheme=gradient,
+                    limiter_k=limiter_k, renumber=renumber)
+    far = rans.BoundaryVariables(mach=mach, angle=angle)
+    s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition(wall)})
+    s.init()
+    s.refill_bcs()
+    q0 = s.get_q()
+    qinf = orc.freestream(mach, angle)
+    q = perturbed_state(s.n_real, s.n_cells, q0, qinf, seed=seed) if seed is not None else q0
+    s.set_q(q)
+    o = orc.OSolver(om, mu_L=mu_L, viscous=1 if viscosity == "laminar" else 0)
+    o.set_bcs({"farfield": ("farfield", (mach, angle, 1.0, 1.0)), "wall": (wall, (0.2, 0.0, 1.0, 1.0))})
+    o.options(second_order=second_order, green_gauss=gradient == "green-gauss", limiter_k=limiter_k, cfl=1.0)
+    o.vec("q", q)
+    return s, o, q
+
+
+def rel_max(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))

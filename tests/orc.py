@@ -1,0 +1,194 @@
+"""ctypes access to the oracle (oracle/liboracle_rans.so): the CPU FP64
+restatement of the reference residual.  Test infrastructure only."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle_rans.so")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+BC = {"farfield": 1, "slip-wall": 2, "wall": 3}
+
+
+def _load():
+    if not os.path.exists(ORACLE_LIB):
+        import subprocess
+        subprocess.run(["python", os.path.join(ROOT, "oracle", "build_oracle.py")], check=True)
+    lib = C.CDLL(ORACLE_LIB)
+    P, dp = C.c_void_p, C.POINTER(C.c_double)
+    sig = {
+        "orc_last_error": (C.c_char_p, []),
+        "orc_mesh_read": (P, [C.c_char_p]),
+        "orc_mesh_build": (P, [C.c_int, dp, dp, C.c_int, P, P, C.c_int, P, P, C.c_char_p]),
+        "orc_mesh_free": (None, [P]),
+        "orc_mesh_counts": (None, [P, P]),
+        "orc_mesh_get": (C.c_long, [P, C.c_char_p, P]),
+        "orc_mesh_bname": (C.c_char_p, [P, C.c_int]),
+        "orc_solver_create": (P, [P, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
+        "orc_solver_free": (None, [P]),
+        "orc_solver_set_bcs": (C.c_int, [P, C.c_int, C.c_char_p, P, dp]),
+        "orc_solver_options": (None, [P, C.c_int, C.c_int, C.c_double, C.c_double]),
+        "orc_solver_init": (None, [P]),
+        "orc_solver_refill_bcs": (None, [P]),
+        "orc_solver_bcs_from_internal": (None, [P]),
+        "orc_solver_set_walls": (None, [P]),
+        "orc_solver_calc_dt": (None, [P]),
+        "orc_solver_calc_gradients": (None, [P]),
+        "orc_solver_calc_limiters": (None, [P]),
+        "orc_solver_fill_rhs": (None, [P]),
+        "orc_solver_calc_residual": (None, [P, dp]),
+        "orc_solver_explicit_step": (C.c_double, [P, C.c_double]),
+        "orc_solver_uniform_residual": (C.c_double, [P]),
+        "orc_solver_wall_profile": (None, [P, C.c_char_p, dp]),
+        "orc_solver_face_jacobian": (None, [P, C.c_int, dp]),
+        "orc_solver_wall_dist": (None, [P]),
+        "orc_solver_vec": (C.c_long, [P, C.c_char_p, dp, dp]),
+        "orc_face_flux": (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, dp, dp, dp, dp, dp]),
+        "orc_face_vars": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp]),
+        "orc_freestream": (None, [C.c_double] * 6 + [dp]),
+    }
+    for n, (r, a) in sig.items():
+        f = getattr(lib, n)
+        f.restype = r
+        f.argtypes = a
+    return lib
+
+
+lib = _load()
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _v(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+U32 = {"edgesNodes", "edgesCells", "cellsNodes", "cellsEdges", "boundaryEdges", "boundaryEdgesIntTag"}
+
+
+class OMesh:
+    def __init__(self, h):
+        if not h:
+            raise RuntimeError(lib.orc_last_error().decode())
+        self.h = h
+
+    @classmethod
+    def read(cls, path):
+        return cls(lib.orc_mesh_read(str(path).encode()))
+
+    @classmethod
+    def build(cls, x, y, cells, nverts, bsegs, bkeys, names):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        cells = np.ascontiguousarray(cells, np.int32).reshape(-1, 4)
+        is_tri = np.ascontiguousarray(np.asarray(nverts) == 3, np.int32)
+        bsegs = np.ascontiguousarray(bsegs, np.int32).reshape(-1, 2)
+        bkeys = np.ascontiguousarray(bkeys, np.int32)
+        nm = "\n".join(f"{k}={v}" for k, v in names.items()).encode()
+        return cls(lib.orc_mesh_build(len(x), _d(x), _d(y), len(cells), _v(cells), _v(is_tri), len(bsegs),
+                                      _v(bsegs), _v(bkeys), nm))
+
+    def counts(self):
+        c = np.zeros(5, np.int32)
+        lib.orc_mesh_counts(self.h, _v(c))
+        return dict(zip(("nodes", "real_cells", "cells", "faces", "bfaces"), c.tolist()))
+
+    def get(self, f):
+        n = lib.orc_mesh_get(self.h, f.encode(), None)
+        dt = np.uint32 if f in U32 else (np.int32 if f == "cellsIsTriangle" else np.float64)
+        out = np.empty(n, dt)
+        lib.orc_mesh_get(self.h, f.encode(), _v(out))
+        return out
+
+    def boundary_names(self):
+        return [lib.orc_mesh_bname(self.h, b).decode() for b in range(self.counts()["bfaces"])]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_mesh_free(self.h)
+            self.h = None
+
+
+class OSolver:
+    """Oracle solver.  viscous: 0 inviscid, 1 laminar."""
+
+    def __init__(self, mesh, R=0.71428571428, mu_L=1e-5, Pr_L=0.72, gamma=1.4, viscous=0):
+        self.mesh = mesh
+        self.h = lib.orc_solver_create(mesh.h, R, mu_L, Pr_L, gamma, viscous)
+        c = mesh.counts()
+        self.n_cells, self.n_real, self.n_faces = c["cells"], c["real_cells"], c["faces"]
+
+    def set_bcs(self, bcs):
+        """bcs: {name: (type_str, (mach, angle, T, p))}"""
+        names = list(bcs.keys())
+        types = np.array([BC[bcs[n][0]] for n in names], np.int32)
+        vars_ = np.array([bcs[n][1] for n in names], np.float64).reshape(-1)
+        rc = lib.orc_solver_set_bcs(self.h, len(names), "\n".join(names).encode(), _v(types), _d(vars_))
+        if rc:
+            raise RuntimeError(lib.orc_last_error().decode())
+
+    def options(self, second_order=True, green_gauss=True, limiter_k=5.0, cfl=1.0):
+        lib.orc_solver_options(self.h, int(second_order), int(green_gauss), limiter_k, cfl)
+
+    def vec(self, name, value=None):
+        n = lib.orc_solver_vec(self.h, name.encode(), None, None)
+        out = np.empty(n)
+        if value is not None:
+            value = np.ascontiguousarray(value, np.float64)
+            lib.orc_solver_vec(self.h, name.encode(), _d(out), _d(value))
+        else:
+            lib.orc_solver_vec(self.h, name.encode(), _d(out), None)
+        return out
+
+    def __getattr__(self, name):
+        fn = getattr(lib, "orc_solver_" + name)
+        return lambda *a: fn(self.h, *a)
+
+    def wall_profile(self, patch="wall"):
+        out = np.zeros(3)
+        lib.orc_solver_wall_profile(self.h, patch.encode(), _d(out))
+        return out
+
+    def face_jacobian(self, e):
+        J = np.zeros(64)
+        lib.orc_solver_face_jacobian(self.h, e, _d(J))
+        return J.reshape(8, 8)
+
+    def calc_residual(self, qk):
+        qk = np.ascontiguousarray(qk, np.float64)
+        lib.orc_solver_calc_residual(self.h, _d(qk))
+        return self.vec("qW")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_solver_free(self.h)
+            self.h = None
+
+
+def face_flux(kind, qL, qR, nx, ny, viscous=0, gx=None, gy=None, gamma=1.4, R=0.71428571428, mu=1e-5, Pr=0.72):
+    qL = np.ascontiguousarray(qL, np.float64)
+    qR = np.ascontiguousarray(qR, np.float64)
+    gx = np.zeros(4) if gx is None else np.ascontiguousarray(gx, np.float64)
+    gy = np.zeros(4) if gy is None else np.ascontiguousarray(gy, np.float64)
+    f = np.zeros(4)
+    lib.orc_face_flux(kind, viscous, nx, ny, gamma, R, mu, Pr, _d(qL), _d(qR), _d(gx), _d(gy), _d(f))
+    return f
+
+
+def face_vars(kind, qL, qbc, nx, ny, gamma=1.4, R=0.71428571428):
+    qL = np.ascontiguousarray(qL, np.float64)
+    qbc = np.ascontiguousarray(qbc, np.float64)
+    out = np.zeros(4)
+    lib.orc_face_vars(kind, nx, ny, gamma, R, _d(qL), _d(qbc), _d(out))
+    return out
+
+
+def freestream(mach, angle, T=1.0, p=1.0, gamma=1.4, R=0.71428571428):
+    out = np.zeros(4)
+    lib.orc_freestream(gamma, R, mach, angle, T, p, _d(out))
+    return out
