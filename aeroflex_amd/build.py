@@ -24,7 +24,8 @@ UNITS = [
     ("afx_k32.hip", ["-ffp-contract=fast"]),    # FP32 throughput form
     ("afx_capi.hip", ["-ffp-contract=off"]),
 ]
-CXX_UNITS = [("afx_mesh.cpp", ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"])]
+CXX_UNITS = [("afx_mesh.cpp", ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]),
+             ("afx_tiles.cpp", ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"])]
 
 
 def _run(cmd):

@@ -20,6 +20,7 @@
 #include "../../include/afx_rans.h"
 #include "afx_kernels.hpp"
 #include "afx_mesh.hpp"
+#include "afx_tiles.hpp"
 
 using namespace afx;
 
@@ -90,6 +91,20 @@ struct afx_rans {
     int32_t n_walls = 0;
     int nblk_red = 0;
 
+    // fused tile residual
+    TileLayout tl;
+    DBuf<int32_t> t_cell0, t_loff, t_ng, t_foff, t_nft, t_goff, t_Lglob;
+    DBuf<uint64_t> t_slots;
+    DBuf<uint32_t> t_fcell;
+    DBuf<int8_t> t_fkind;
+    DBuf<double> t_geo64;
+    DBuf<float> t_geo32, iarea32, a15_32;
+    DBuf<float4> q32v, R32v, t_geo4;   // FP32 fused form: AoS float4 per cell / two per tile face
+    DBuf<uint32_t> t_fck;              // c0 | c1 << 12 | kind << 24 per tile face
+    bool r32_aos = false;              // last FP32 residual is in R32v (fused) rather than R32
+    std::vector<int32_t> t_lg;   // device cell of each local cell (unflagged)
+    bool tile64_ok = false, tile32_ok = false;
+
     // boundary conditions
     bool bcs_set = false;
     std::vector<int32_t> bface_type;       // per boundary face
@@ -119,6 +134,15 @@ struct afx_rans {
         return GasT<float>{(float)gas.gamma, (float)gas.R, (float)gas.mu_L, (float)(1.0 * gas.mu_L / gas.Pr_L)};
     }
     bool needs_gradients() const { return opts.second_order || opts.viscosity != 0; }
+    TilesF32 tiles32() const {
+        return TilesF32{t_cell0.p, t_loff.p, t_ng.p, t_foff.p, t_nft.p, t_goff.p, t_Lglob.p, t_slots.p,
+                        t_fck.p, t_geo4.p, tl.ntiles, tl.lmax, tl.gmax, tl.ftmax};
+    }
+    template <typename T>
+    TilesT<T> tiles(const T* geo) const {
+        return TilesT<T>{t_cell0.p, t_loff.p, t_ng.p, t_foff.p, t_nft.p, t_goff.p, t_Lglob.p, t_slots.p,
+                         t_fcell.p, t_fkind.p, geo, (int64_t)tl.fref.size(), tl.ntiles, tl.lmax, tl.gmax, tl.ftmax};
+    }
 };
 
 namespace {
@@ -218,6 +242,7 @@ int norm_of32(afx_rans* h, const float* x, int64_t ld, int nrows, double* out) {
 int ensure_q32(afx_rans* h) {
     if (!h->q32_valid) {
         HIPCHK(launch_to_f32(h->stream, h->q.p, h->q32.p, (int64_t)NV * h->nT));
+        if (h->q32v.p) HIPCHK(launch_to_f32x4(h->stream, h->q.p, h->nT, h->q32v.p));
         h->q32_valid = true;
     }
     return AFX_OK;
@@ -240,6 +265,24 @@ int run_gradients32(afx_rans* h) {
 
 // enqueue one residual evaluation (no synchronisation)
 int enqueue_residual(afx_rans* h, int precision) {
+    const bool fused = h->opts.fused != 0;
+    if (precision == AFX_FP64 && fused && h->tile64_ok) {
+        HIPCHK(launch_residual_tile<double>(h->stream, h->tiles<double>(h->t_geo64.p), h->q.p, h->nT, h->nC, h->area.p,
+                                           h->lsq.p, h->xc.p, h->yc.p, h->gas64(), h->opts.gradient,
+                                           h->opts.second_order, h->opts.limiter_k, h->opts.viscosity, h->R.p, h->q.p));
+        return AFX_OK;
+    }
+    if (precision == AFX_FP32 && fused && h->tile32_ok) {
+        int rc = ensure_q32(h);
+        if (rc) return rc;
+        const double k = h->opts.limiter_k;
+        HIPCHK(launch_residual_tile_f32(h->stream, h->tiles32(), h->q32v.p, h->nC, h->iarea32.p, h->a15_32.p,
+                                        h->lsq32.p, h->xc32.p, h->yc32.p, h->gas32(), h->opts.gradient,
+                                        h->opts.second_order, (float)(k * k * k), h->opts.viscosity, h->R32v.p,
+                                        h->q32v.p));
+        h->r32_aos = true;
+        return AFX_OK;
+    }
     if (precision == AFX_FP64) {
         if (h->needs_gradients()) {
             int rc = run_gradients64(h, h->q.p, h->q.p);
@@ -260,6 +303,7 @@ int enqueue_residual(afx_rans* h, int precision) {
                                        h->gy32.p, h->lim32.p, h->gas32(), h->opts.viscosity, h->opts.second_order,
                                        nullptr, h->flux32.p));
         HIPCHK(launch_gather<float>(h->stream, h->cells32(), h->flux32.p, h->nF, 0, h->R32.p));
+        h->r32_aos = false;
     }
     return AFX_OK;
 }
@@ -282,6 +326,7 @@ void afx_default_gas(afx_gas* g) {
 void afx_default_opts(afx_opts* o) {
     std::memset(o, 0, sizeof(*o));
     o->viscosity = 0; o->second_order = 1; o->gradient = 1; o->limiter_k = 5.0; o->renumber = 1;
+    o->tile_cells = 256; o->fused = 1;
 }
 
 // ------------------------------------------------------------------ mesh ---
@@ -445,6 +490,18 @@ int afx_mesh_rcm(const afx_mesh* mh, int32_t* perm) {
     return AFX_OK;
 }
 
+int afx_mesh_tile_stats(const afx_mesh* mh, int32_t tile_cells, int64_t st[8]) {
+    if (!mh || !st) return fail(AFX_EARG, "null argument");
+    try {
+        TileLayout tl = build_tiles(mh->m, rcm_order(mh->m), tile_cells);
+        st[0] = tl.ntiles; st[1] = tl.lmax; st[2] = tl.gmax; st[3] = tl.fmax; st[4] = tl.ftmax;
+        st[5] = (int64_t)tl.L_ref.size(); st[6] = (int64_t)tl.slots.size(); st[7] = (int64_t)tl.fref.size();
+        return AFX_OK;
+    } catch (std::exception& e) {
+        return fail(AFX_EARG, e.what());
+    }
+}
+
 void afx_mesh_destroy(afx_mesh* m) { delete m; }
 
 // ---------------------------------------------------------------- solver ---
@@ -478,13 +535,23 @@ int afx_rans_create(const afx_mesh* mh, const afx_gas* g, const afx_opts* o, afx
 
     HIPCHK_C(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
 
-    // cell order: RCM on real cells, ghosts keep reference order
+    // cell order: tiles grown from RCM seeds (or file order), ghosts keep
+    // reference order
     h->c_new2old.resize(nT);
-    if (opts.renumber) {
-        auto p = rcm_order(m);
-        for (int32_t i = 0; i < nC; ++i) h->c_new2old[i] = (int32_t)p[i];
-    } else {
-        for (int32_t i = 0; i < nC; ++i) h->c_new2old[i] = i;
+    {
+        std::vector<uint32_t> seeds;
+        if (opts.renumber) seeds = rcm_order(m);
+        else { seeds.resize(nC); std::iota(seeds.begin(), seeds.end(), 0u); }
+        const int32_t target = opts.tile_cells > 0 ? opts.tile_cells : 256;
+        try {
+            h->tl = build_tiles(m, seeds, target);
+        } catch (std::exception& e) {
+            return bail(fail(AFX_EARG, e.what()));
+        }
+        if (opts.renumber)
+            for (int32_t i = 0; i < nC; ++i) h->c_new2old[i] = h->tl.cell_new2old[i];
+        else
+            for (int32_t i = 0; i < nC; ++i) h->c_new2old[i] = i;
     }
     for (int32_t i = nC; i < nT; ++i) h->c_new2old[i] = i;
     h->c_old2new.resize(nT);
@@ -611,6 +678,71 @@ int afx_rans_create(const afx_mesh* mh, const afx_gas* g, const afx_opts* o, afx
         HIPCHK_C(h->lim32.upload(ones32, s));
         HIPCHK_C(hipStreamSynchronize(s));
     }
+    // tile tables (only usable when the cell order is the tile order)
+    if (opts.renumber) {
+        const TileLayout& tl = h->tl;
+        std::vector<int32_t> lg(tl.L_ref.size());
+        for (size_t i = 0; i < lg.size(); ++i) lg[i] = h->c_old2new[tl.L_ref[i]];
+        h->t_lg = lg;
+        {
+            // FP32 form: 1/A and A^1.5 per cell (K3a = K^3 A^1.5, solver.h:551-552)
+            std::vector<float> ia(nT), a15(nT);
+            for (int32_t i = 0; i < nT; ++i) {
+                double A = m.cellsAreas[h->c_new2old[i]];
+                ia[i] = (float)(1.0 / A);
+                a15[i] = (float)(A * std::sqrt(A));
+            }
+            HIPCHK_C(h->iarea32.alloc(nT)); HIPCHK_C(h->iarea32.upload(ia, s));
+            HIPCHK_C(h->a15_32.alloc(nT)); HIPCHK_C(h->a15_32.upload(a15, s));
+            HIPCHK_C(hipStreamSynchronize(s));
+        }
+        const size_t nft = tl.fref.size();
+        std::vector<double> geo(8 * nft);
+        std::vector<float> geo32(8 * nft);
+        for (size_t i = 0; i < nft; ++i) {
+            int32_t f = tl.fref[i];
+            uint32_t a = m.edgesCells[2 * (size_t)f], b = m.edgesCells[2 * (size_t)f + 1];
+            double v[8];
+            v[0] = m.edgesNormalsX[f]; v[1] = m.edgesNormalsY[f]; v[2] = m.edgesLengths[f];
+            v[3] = hgw[h->f_old2new[f]];
+            v[4] = m.edgesCentersX[f] - m.cellsCentersX[a];   // solver.h:1122-1123
+            v[5] = m.edgesCentersY[f] - m.cellsCentersY[a];
+            v[6] = m.edgesCentersX[f] - m.cellsCentersX[b];   // solver.h:1125-1126
+            v[7] = m.edgesCentersY[f] - m.cellsCentersY[b];
+            for (int k = 0; k < 8; ++k) { geo[k * nft + i] = v[k]; geo32[k * nft + i] = (float)v[k]; }
+        }
+        HIPCHK_C(h->t_cell0.alloc(tl.cell0.size())); HIPCHK_C(h->t_cell0.upload(tl.cell0, s));
+        HIPCHK_C(h->t_loff.alloc(tl.loff.size())); HIPCHK_C(h->t_loff.upload(tl.loff, s));
+        HIPCHK_C(h->t_ng.alloc(tl.ng.size())); HIPCHK_C(h->t_ng.upload(tl.ng, s));
+        HIPCHK_C(h->t_foff.alloc(tl.foff.size())); HIPCHK_C(h->t_foff.upload(tl.foff, s));
+        HIPCHK_C(h->t_nft.alloc(tl.nft.size())); HIPCHK_C(h->t_nft.upload(tl.nft, s));
+        HIPCHK_C(h->t_goff.alloc(tl.goff.size())); HIPCHK_C(h->t_goff.upload(tl.goff, s));
+        HIPCHK_C(h->t_Lglob.alloc(lg.size())); HIPCHK_C(h->t_Lglob.upload(lg, s));
+        HIPCHK_C(h->t_slots.alloc(tl.slots.size())); HIPCHK_C(h->t_slots.upload(tl.slots, s));
+        HIPCHK_C(h->t_fcell.alloc(tl.fcell.size())); HIPCHK_C(h->t_fcell.upload(tl.fcell, s));
+        HIPCHK_C(h->t_fkind.alloc(nft));
+        HIPCHK_C(hipMemsetAsync(h->t_fkind.p, 0, std::max<size_t>(nft, 1), s));
+        HIPCHK_C(h->t_geo64.alloc(geo.size())); HIPCHK_C(h->t_geo64.upload(geo, s));
+        HIPCHK_C(h->t_geo32.alloc(geo32.size())); HIPCHK_C(h->t_geo32.upload(geo32, s));
+        {
+            std::vector<float4> g4(2 * nft);
+            std::vector<uint32_t> fck(nft);
+            for (size_t i = 0; i < nft; ++i) {
+                g4[2 * i] = make_float4(geo32[i], geo32[nft + i], geo32[2 * nft + i], geo32[3 * nft + i]);
+                g4[2 * i + 1] = make_float4(geo32[4 * nft + i], geo32[5 * nft + i], geo32[6 * nft + i], geo32[7 * nft + i]);
+                uint32_t c = tl.fcell[i];
+                fck[i] = (c & 0xFFFFu) | ((c >> 16) << 12);   // kinds are added by set_bcs
+            }
+            HIPCHK_C(h->t_geo4.alloc(g4.size())); HIPCHK_C(h->t_geo4.upload(g4, s));
+            HIPCHK_C(h->t_fck.alloc(nft)); HIPCHK_C(h->t_fck.upload(fck, s));
+            HIPCHK_C(h->q32v.alloc(nT)); HIPCHK_C(hipMemsetAsync(h->q32v.p, 0, (size_t)nT * 16, s));
+            HIPCHK_C(h->R32v.alloc(nT)); HIPCHK_C(hipMemsetAsync(h->R32v.p, 0, (size_t)nT * 16, s));
+            HIPCHK_C(hipStreamSynchronize(s));
+        }
+        HIPCHK_C(hipStreamSynchronize(s));
+        h->tile64_ok = tile_lds_bytes<double>(tl.lmax, tl.gmax, tl.ftmax) <= 65536;
+        h->tile32_ok = tile_lds_bytes<float>(tl.lmax, tl.gmax, tl.ftmax) <= 65536;
+    }
     h->nblk_red = 1024;
     HIPCHK_C(h->partial.alloc(h->nblk_red));
     HIPCHK_C(h->sum1.alloc(1));
@@ -655,6 +787,24 @@ int afx_rans_set_bcs(afx_rans* h, int32_t n, const char* const* names, const int
     }
     hipStream_t s = h->stream;
     HIPCHK(h->fkind.upload(kind, s));
+    if (h->t_fkind.p && !h->tl.fref.empty()) {
+        std::vector<int8_t> tk(h->tl.fref.size());
+        for (size_t i = 0; i < tk.size(); ++i) tk[i] = kind[h->f_old2new[h->tl.fref[i]]];
+        HIPCHK(h->t_fkind.upload(tk, s));
+        // wall/slip ghosts are never read by the fused kernels: flag them
+        std::vector<char> wall_ghost(h->nT, 0);
+        for (size_t i = 0; i < wg.size(); ++i) wall_ghost[wg[i]] = 1;
+        std::vector<int32_t> lg = h->t_lg;
+        for (auto& v : lg) if (wall_ghost[v]) v = -1 - v;
+        HIPCHK(h->t_Lglob.upload(lg, s));
+        std::vector<uint32_t> fck(h->tl.fref.size());
+        for (size_t i = 0; i < fck.size(); ++i) {
+            uint32_t c = h->tl.fcell[i];
+            fck[i] = (c & 0xFFFFu) | ((c >> 16) << 12) | ((uint32_t)tk[i] << 24);
+        }
+        HIPCHK(h->t_fck.upload(fck, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
     h->n_walls = (int32_t)wo.size();
     HIPCHK(h->w_owner.alloc(wo.size())); HIPCHK(h->w_owner.upload(wo, s));
     HIPCHK(h->w_ghost.alloc(wg.size())); HIPCHK(h->w_ghost.upload(wg, s));
@@ -800,6 +950,16 @@ int afx_rans_residual(afx_rans* h, int32_t precision, double* R, double* norm2) 
     if (precision == AFX_FP64) {
         if (R && (rc = download_aos(h, h->R.p, R))) return rc;
         if (norm2 && (rc = norm_of(h, h->R.p, h->nT, NV, norm2))) return rc;
+    } else if (h->r32_aos) {
+        if (R) {
+            std::vector<float> tmp((size_t)NV * h->nT);
+            HIPCHK(hipMemcpyAsync(tmp.data(), h->R32v.p, tmp.size() * 4, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            for (int32_t i = 0; i < h->nT; ++i)
+                for (int k = 0; k < NV; ++k) R[(size_t)NV * h->c_new2old[i] + k] = (double)tmp[(size_t)NV * i + k];
+        }
+        if (norm2 && (rc = norm_of32(h, reinterpret_cast<const float*>(h->R32v.p), (int64_t)NV * h->nT, 1, norm2)))
+            return rc;
     } else {
         if (R && (rc = download_aos32(h, h->R32.p, R))) return rc;
         if (norm2 && (rc = norm_of32(h, h->R32.p, h->nT, NV, norm2))) return rc;

@@ -93,6 +93,16 @@ hipError_t launch_to_f32(hipStream_t s, const double* src, float* dst, int64_t n
     return hipGetLastError();
 }
 
+// SoA FP64 state -> AoS float4 copy used by the FP32 fused residual
+__global__ void __launch_bounds__(kBlock) k_to_f32x4(const double* __restrict__ a, int32_t nT, float4* __restrict__ b) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nT) b[i] = make_float4((float)a[i], (float)a[(size_t)nT + i], (float)a[2 * (size_t)nT + i], (float)a[3 * (size_t)nT + i]);
+}
+hipError_t launch_to_f32x4(hipStream_t s, const double* src, int32_t nT, float4* dst) {
+    hipLaunchKernelGGL(k_to_f32x4, dim3(nblocks(nT)), dim3(kBlock), 0, s, src, nT, dst);
+    return hipGetLastError();
+}
+
 // mesh.h:794-830: minimum distance to the wall/slip-wall face centres (LDS tiles)
 __global__ void __launch_bounds__(kBlock) k_wall_dist(const double* __restrict__ xc, const double* __restrict__ yc,
                                                       int32_t nT, const double* __restrict__ wx,

@@ -48,7 +48,62 @@ struct CellsT {
     int32_t nT;
 };
 
-struct ResidualArgs64;  // fwd
+// Per-tile tables of the fused residual (afx_tiles.hpp), device side.
+template <typename T>
+struct TilesT {
+    const int32_t* cell0;    // ntiles+1: first device cell of each tile
+    const int32_t* loff;     // ntiles+1: local-cell table offsets
+    const int32_t* ng;       // ntiles: gradient cells (T u H1)
+    const int32_t* foff;     // ntiles+1: local-face table offsets
+    const int32_t* nft;      // ntiles: faces touching T
+    const int32_t* goff;     // ntiles+1: slot table offsets
+    const int32_t* Lglob;    // device cell index of each local cell (-1 - index: wall/slip ghost, not staged)
+    const uint64_t* slots;   // per G cell: 4 x u16 local face (bit 15: cell is the neighbour), 0xFFFF pad
+    const uint32_t* fcell;   // per local face: c0 | c1 << 16 (local cells)
+    const int8_t* fkind;     // per local face
+    const T* geo;            // 8 x nFtot: nx, ny, len, gw, d0x, d0y, d1x, d1y (d = x_f - x_cell)
+    int64_t nFtot;
+    int32_t ntiles;
+    int32_t lmax, gmax, ftmax;
+};
+
+template <typename T>
+inline size_t tile_lds_bytes(int32_t lmax, int32_t gmax, int32_t ftmax) {
+    return sizeof(T) * (size_t)(4 * lmax + 12 * gmax + 4 * ftmax);
+}
+
+// fused residual of every tile: gradients, limiters, fluxes and cell sums in
+// LDS; R written once per cell; wall/slip ghost states of the tile's cells
+// written to qghost (set_walls_from_internal) when non-null
+template <typename T>
+hipError_t launch_residual_tile(hipStream_t s, TilesT<T> S, const T* q, int32_t nT, int32_t nC, const T* area,
+                                const T* lsq, const T* xc, const T* yc, GasT<T> g, int gg, int second_order,
+                                T limiter_k, int viscous, T* R, T* qghost);
+
+// FP32 throughput form of the same (afx_k32.hip): cell states, gradients,
+// limiters, fluxes and residuals are float4 per cell / face (rho, rho u,
+// rho v, rho E); per tile face two float4 (nx, ny, len, g) and (d0x, d0y,
+// d1x, d1y) and one word c0 | c1 << 12 | kind << 24.
+struct TilesF32 {
+    const int32_t* cell0;
+    const int32_t* loff;
+    const int32_t* ng;
+    const int32_t* foff;
+    const int32_t* nft;
+    const int32_t* goff;
+    const int32_t* Lglob;
+    const uint64_t* slots;
+    const uint32_t* fck;
+    const float4* geo;
+    int32_t ntiles;
+    int32_t lmax, gmax, ftmax;
+};
+// iarea = 1/A, a15 = A^1.5, K3 = limiter_k^3 (K3a of solver.h:551-552)
+hipError_t launch_residual_tile_f32(hipStream_t s, TilesF32 S, const float4* q, int32_t nC, const float* iarea,
+                                    const float* a15, const float* lsq, const float* xc, const float* yc,
+                                    GasT<float> g, int gg, int second_order, float K3, int viscous, float4* R,
+                                    float4* qghost);
+hipError_t launch_to_f32x4(hipStream_t s, const double* src, int32_t nT, float4* dst);
 
 // ---- launchers (instantiated for double in afx_k64.hip, float in afx_k32.hip)
 template <typename T>

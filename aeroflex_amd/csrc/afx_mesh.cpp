@@ -204,31 +204,39 @@ void write_msh41(const Elements& el, const std::string& path) {
     for (size_t i = 0; i < nn; ++i) std::fprintf(f, "%.17g %.17g 0\n", el.x[i], el.y[i]);
     std::fprintf(f, "$EndNodes\n");
     size_t nb = el.bseg_key.size(), nc = el.cell_nv.size();
-    // element blocks: one per curve name, then triangles, then quads (file
-    // order of cells is preserved inside each type, so write cells type-grouped
-    // only if the input is homogeneous; mixed meshes get one block per run)
-    std::vector<std::pair<int, std::pair<size_t, size_t>>> cell_runs;   // (nv, [begin,end))
+    // element blocks follow the input order: one block per run of boundary
+    // segments with the same physical key, then one per run of cells of the
+    // same type, so that reading the file back reproduces the face and ghost
+    // order exactly
+    std::vector<std::pair<size_t, size_t>> bruns, cruns;
+    for (size_t b = 0; b < nb;) {
+        size_t e = b;
+        while (e < nb && el.bseg_key[e] == el.bseg_key[b]) ++e;
+        bruns.push_back({b, e});
+        b = e;
+    }
     for (size_t c = 0; c < nc;) {
         size_t e = c;
         while (e < nc && el.cell_nv[e] == el.cell_nv[c]) ++e;
-        cell_runs.push_back({el.cell_nv[c], {c, e}});
+        cruns.push_back({c, e});
         c = e;
     }
-    std::fprintf(f, "$Elements\n%zu %zu 1 %zu\n", curve_names.size() + cell_runs.size(), nb + nc, nb + nc);
+    std::fprintf(f, "$Elements\n%zu %zu 1 %zu\n", bruns.size() + cruns.size(), nb + nc, nb + nc);
     size_t tag = 1;
-    for (size_t k = 0; k < curve_names.size(); ++k) {
-        uint32_t key = el.name_keys[curve_names[k]];
-        size_t cnt = 0;
-        for (size_t b = 0; b < nb; ++b) cnt += el.bseg_key[b] == key;
-        std::fprintf(f, "1 %zu 1 %zu\n", k + 1, cnt);
-        for (size_t b = 0; b < nb; ++b)
-            if (el.bseg_key[b] == key)
-                std::fprintf(f, "%zu %u %u\n", tag++, el.bseg_nodes[2 * b] + 1, el.bseg_nodes[2 * b + 1] + 1);
+    for (auto& run : bruns) {
+        uint32_t key = el.bseg_key[run.first];
+        size_t ent = 0;
+        for (size_t k = 0; k < curve_names.size(); ++k)
+            if (el.name_keys[curve_names[k]] == key) ent = k + 1;
+        if (ent == 0) { std::fclose(f); throw std::runtime_error("boundary segment key without a physical name"); }
+        std::fprintf(f, "1 %zu 1 %zu\n", ent, run.second - run.first);
+        for (size_t b = run.first; b < run.second; ++b)
+            std::fprintf(f, "%zu %u %u\n", tag++, el.bseg_nodes[2 * b] + 1, el.bseg_nodes[2 * b + 1] + 1);
     }
-    for (auto& run : cell_runs) {
-        int nv = run.first;
-        std::fprintf(f, "2 1 %d %zu\n", nv == 3 ? 2 : 3, run.second.second - run.second.first);
-        for (size_t c = run.second.first; c < run.second.second; ++c) {
+    for (auto& run : cruns) {
+        int nv = el.cell_nv[run.first];
+        std::fprintf(f, "2 1 %d %zu\n", nv == 3 ? 2 : 3, run.second - run.first);
+        for (size_t c = run.first; c < run.second; ++c) {
             std::fprintf(f, "%zu", tag++);
             for (int v = 0; v < nv; ++v) std::fprintf(f, " %u", el.cell_nodes[4 * c + v] + 1);
             std::fprintf(f, "\n");

@@ -42,7 +42,8 @@ class AfxGas(C.Structure):
 
 class AfxOpts(C.Structure):
     _fields_ = [("viscosity", C.c_int32), ("second_order", C.c_int32), ("gradient", C.c_int32),
-                ("limiter_k", C.c_double), ("renumber", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("limiter_k", C.c_double), ("renumber", C.c_int32), ("tile_cells", C.c_int32),
+                ("fused", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 def _load():
@@ -66,6 +67,7 @@ def _load():
         "afx_mesh_get": (C.c_int, [P, C.c_char_p, P, C.POINTER(i64)]),
         "afx_mesh_boundary_name": (C.c_int, [P, i64, C.c_char_p, i32]),
         "afx_mesh_rcm": (C.c_int, [P, P]),
+        "afx_mesh_tile_stats": (C.c_int, [P, i32, C.POINTER(i64)]),
         "afx_mesh_destroy": (None, [P]),
         "afx_rans_create": (C.c_int, [P, C.POINTER(AfxGas), C.POINTER(AfxOpts), C.POINTER(P)]),
         "afx_rans_set_bcs": (C.c_int, [P, i32, P, P, dp]),
@@ -218,6 +220,12 @@ class Mesh:
             out.append(buf.value.decode())
         return out
 
+    def tile_stats(self, tile_cells):
+        st = (C.c_int64 * 8)()
+        _check(lib.afx_mesh_tile_stats(self.h, tile_cells, st))
+        keys = ("tiles", "lmax", "gmax", "fmax", "ftmax", "sum_local", "sum_grad", "sum_faces")
+        return dict(zip(keys, list(st)))
+
     def rcm(self):
         p = np.empty(self.counts()["real_cells"], dtype=np.int32)
         _check(lib.afx_mesh_rcm(self.h, _vptr(p)))
@@ -237,7 +245,7 @@ class Solver:
     """Device residual behind the rans::solver interface (solver.h:40-175)."""
 
     def __init__(self, mesh, gas=None, viscosity_model="inviscid", second_order=True,
-                 gradient_scheme="green-gauss", limiter_k=5.0, renumber=True):
+                 gradient_scheme="green-gauss", limiter_k=5.0, renumber=True, tile_cells=0, fused=True):
         if viscosity_model not in VISCOSITY:
             raise RansError(f"unsupported viscosity model '{viscosity_model}'")
         self.gas = gas if gas is not None else Gas()
@@ -249,6 +257,8 @@ class Solver:
         o.gradient = GRADIENT[gradient_scheme]
         o.limiter_k = limiter_k
         o.renumber = int(renumber)
+        o.tile_cells = int(tile_cells)
+        o.fused = int(fused)
         self._opts = o
         g = self.gas.c_struct()
         h = C.c_void_p()

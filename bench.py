@@ -1,0 +1,268 @@
+"""Residual-throughput benchmark (one GPU).
+
+One step = one evaluation of the unstructured finite-volume residual
+(gradients + limiters, Roe face fluxes, signed cell sums) on a 2.0M-cell
+jittered mixed triangle/quad mesh around a NACA 0012 section, state =
+freestream (M 0.7, alpha 1.5 deg) with 1% seeded Gaussian perturbations of
+rho, u, v, p.  The timed form is FP32 (compensated FP32 cell sums); the plain
+FP64 form is timed beside it.  Inputs are resident in HBM before timing.
+
+Prints one JSON line (see the keys below).  Kernel durations come from HIP
+events recorded on the library's own stream around back-to-back launches.
+The CPU baseline is the oracle (serial FP64 restatement of the reference's
+fillRhoRHS, calc_dt included) on a bounded sample mesh from the same
+generator.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N] [--no-cpu]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E peak, GB/s
+
+
+class HipEvents:
+    """hipEvent timing on an arbitrary stream (the library's)."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+        self.hip.hipDeviceSynchronize.argtypes = []
+
+    def event(self):
+        e = C.c_void_p()
+        rc = self.hip.hipEventCreate(C.byref(e))
+        if rc != 0:
+            raise RuntimeError(f"hipEventCreate failed ({rc})")
+        return e
+
+    def record(self, e, stream):
+        rc = self.hip.hipEventRecord(e, C.c_void_p(stream))
+        if rc != 0:
+            raise RuntimeError(f"hipEventRecord failed ({rc})")
+
+    def elapsed_ms(self, a, b):
+        self.hip.hipEventSynchronize(b)
+        ms = C.c_float()
+        rc = self.hip.hipEventElapsedTime(C.byref(ms), a, b)
+        if rc != 0:
+            raise RuntimeError(f"hipEventElapsedTime failed ({rc})")
+        return ms.value
+
+    def device_sync(self):
+        rc = self.hip.hipDeviceSynchronize()
+        if rc != 0:
+            raise RuntimeError(f"hipDeviceSynchronize failed ({rc})")
+
+
+def mesh_dims(target_cells, quad_frac=0.25):
+    """O-grid size (ni around, nj radial) for about target_cells cells."""
+    per_quad = 2.0 - quad_frac
+    nq = target_cells / per_quad
+    ni = max(64, int(round(np.sqrt(nq / 1.09) / 64.0)) * 64)
+    nj = int(round(nq / ni)) + 1
+    return ni, nj
+
+
+def build_case(rans, target_cells, seed_mesh=1, seed_state=2):
+    ni, nj = mesh_dims(target_cells)
+    t0 = time.perf_counter()
+    m = rans.Mesh.jittered(ni, nj, r_far=20.0, thickness=0.12, first_h=1e-4, jitter=0.1, quad_frac=0.25,
+                           seed=seed_mesh)
+    t_mesh = time.perf_counter() - t0
+    s = rans.Solver(m, viscosity_model="inviscid", second_order=True, gradient_scheme="green-gauss",
+                    limiter_k=5.0)
+    far = rans.BoundaryVariables(mach=0.7, angle=np.deg2rad(1.5))
+    s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition("slip-wall")})
+    s.init()
+    s.refill_bcs()
+    q = s.get_q()
+    g = s.gas
+    qinf = far.get_conservative(g)
+    rng = np.random.default_rng(seed_state)
+    n = s.n_real
+    rho = qinf[0] * (1 + 0.01 * rng.standard_normal(n))
+    u = qinf[1] / qinf[0] * (1 + 0.01 * rng.standard_normal(n))
+    v = qinf[2] / qinf[0] * (1 + 0.01 * rng.standard_normal(n))
+    p0 = (g.gamma - 1) * (qinf[3] - 0.5 * (qinf[1] ** 2 + qinf[2] ** 2) / qinf[0])
+    p = p0 * (1 + 0.01 * rng.standard_normal(n))
+    q[0:4 * n:4] = rho
+    q[1:4 * n:4] = rho * u
+    q[2:4 * n:4] = rho * v
+    q[3:4 * n:4] = p / (g.gamma - 1) + 0.5 * rho * (u * u + v * v)
+    s.set_q(q)
+    return m, s, dict(ni=ni, nj=nj, mesh_seconds=t_mesh)
+
+
+def alg_bytes(nc, nf, nb, word, V=4, index_bytes=4):
+    """Algorithmic HBM bytes of one residual evaluation: per real cell state V,
+    geometry 4 (area, x_c, y_c, d), gradients 2V written + 2V read, limiter V
+    written + V read, residual V written; per face two cell indices and
+    n_x, n_y, length, x_f, y_f; per boundary face a BC tag and a ghost state."""
+    return word * nc * (8 * V + 4) + nf * (2 * index_bytes + 5 * word) + word * nb * (V + 1)
+
+
+def time_stage(ev, s, stream, prec, stage, n):
+    a, b = ev.event(), ev.event()
+    s.synchronize()
+    ev.record(a, stream)
+    s.launch_stage(prec, stage, n)
+    ev.record(b, stream)
+    return ev.elapsed_ms(a, b) / n
+
+
+def read_traffic(path):
+    """HBM bytes per residual from a committed rocprofv3 PMC summary, if any."""
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+    try:
+        return float(vals["traffic_bytes_per_residual"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(rans, budget_s=12.0):
+    """Oracle (serial FP64 fillRhoRHS, calc_dt included) on a 120k-cell sample
+    mesh of the same generator, 1 thread."""
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    ni, nj = mesh_dims(120_000)
+    m = rans.Mesh.jittered(ni, nj, r_far=20.0, thickness=0.12, first_h=1e-4, jitter=0.1, quad_frac=0.25, seed=1)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "sample.msh")
+        m.write_msh(path)
+        om = orc.OMesh.read(path)
+    o = orc.OSolver(om)
+    o.set_bcs({"farfield": ("farfield", (0.7, np.deg2rad(1.5), 1.0, 1.0)), "wall": ("slip-wall", (0.2, 0, 1, 1))})
+    o.options(second_order=True, green_gauss=True, limiter_k=5.0, cfl=1.0)
+    o.init()
+    o.refill_bcs()
+    nc = om.counts()["real_cells"]
+    o.fill_rhs()
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.fill_rhs()
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 2000:
+            break
+    return {"value": round(nc * n / el / 1e6, 4), "unit": "Mcell-residuals/s", "cores": 1, "kind": "port",
+            "sample": f"oracle fillRhoRHS (FP64, serial, calc_dt included) x{n} on a {nc}-cell mesh of the same "
+                      f"generator, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cells", type=int, default=2_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-steps", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the residual's multi-GPU sharding is a design, not a build)", file=sys.stderr)
+        sys.exit(2)
+
+    from aeroflex_amd import rans
+    ev = HipEvents()
+    m, s, info = build_case(rans, args.cells)
+    nc, nf, nb = s.n_real, s.n_faces, s.n_bfaces
+    stream = s.stream()
+    FP32, FP64 = rans.AFX_FP32, rans.AFX_FP64
+
+    # warmup (also converts the FP64 state to the FP32 copy)
+    s.launch_stage(FP32, 3, max(args.warmup, 1))
+    s.synchronize()
+    ev.device_sync()
+
+    # timed region: exactly K residual evaluations
+    a, b = ev.event(), ev.event()
+    s.synchronize()
+    t0 = time.perf_counter()
+    ev.record(a, stream)
+    s.launch_stage(FP32, 3, args.steps)
+    ev.record(b, stream)
+    s.synchronize()
+    ev.device_sync()
+    wall = time.perf_counter() - t0
+    ev_ms = ev.elapsed_ms(a, b)
+    ms_per_step = wall * 1e3 / args.steps
+    value = nc * args.steps / wall / 1e6
+
+    # per-kernel averages (events around back-to-back launches of one stage)
+    nk = min(args.steps, 200)
+    k32 = {"grad_lim": time_stage(ev, s, stream, FP32, 0, nk),
+           "face_flux": time_stage(ev, s, stream, FP32, 1, nk),
+           "gather": time_stage(ev, s, stream, FP32, 2, nk)}
+    res32_ms = ev_ms / args.steps
+    b32 = alg_bytes(nc, nf, nb, 4)
+    achieved = b32 / (res32_ms * 1e-3) / 1e9
+
+    # plain FP64 form on the same mesh and state
+    s.launch_stage(FP64, 3, 5)
+    s.synchronize()
+    t64 = time_stage(ev, s, stream, FP64, 3, args.fp64_steps)
+    k64 = {"grad_lim": time_stage(ev, s, stream, FP64, 0, nk // 2),
+           "face_flux": time_stage(ev, s, stream, FP64, 1, nk // 2),
+           "gather": time_stage(ev, s, stream, FP64, 2, nk // 2)}
+    b64 = alg_bytes(nc, nf, nb, 8)
+
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "residual_fp32_2M_traffic.txt"))
+    out = {
+        "metric": "Mcell-residuals/s + % HBM roofline; FP32/batched vs plain FP64; adjoint/primal",
+        "value": round(value, 2),
+        "unit": "Mcell-residuals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "FP32 residual evaluation (GG gradients + limiter, Roe fluxes, compensated cell "
+                               "sums) on a jittered mixed tri/quad mesh around NACA 0012, M0.7 a1.5 deg + 1% "
+                               "seeded perturbations",
+                   "cells": nc, "faces": nf, "boundary_faces": nb, "ogrid": [info["ni"], info["nj"]],
+                   "variables": 4, "seeds": {"mesh": 1, "state": 2}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "residual = afx::k_grad_lim + afx::k_face_flux + afx::k_gather (FP32)",
+                     "alg_bytes_per_launch": b32, "residual_ms": round(res32_ms, 5),
+                     "kernel_ms": {k: round(v, 5) for k, v in k32.items()}},
+        "fp64_plain": {"value": round(nc / (t64 * 1e-3) / 1e6, 2), "residual_ms": round(t64, 5),
+                       "roofline_frac": round(b64 / (t64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "kernel_ms": {k: round(v, 5) for k, v in k64.items()}},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(rans)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -62,8 +62,10 @@ typedef struct afx_opts {
     int32_t second_order;  /* Settings::second_order (default 1) */
     int32_t gradient;      /* 0 "least-squares", 1 "green-gauss" (default) */
     double limiter_k;      /* Settings::limiter_k (default 5) */
-    int32_t renumber;      /* 1: RCM cell order on the device (default), 0: file order */
-    int32_t reserved[7];
+    int32_t renumber;      /* 1: tiled RCM cell order on the device (default), 0: file order */
+    int32_t tile_cells;    /* cells per tile of the fused residual kernel (0: default 256) */
+    int32_t fused;         /* 1: fused tile kernel for the residual (default), 0: three-kernel path */
+    int32_t reserved[5];
 } afx_opts;
 
 const char* afx_last_error(void);
@@ -106,8 +108,12 @@ int afx_mesh_counts(const afx_mesh* m, int64_t counts[5]);
 int afx_mesh_get(const afx_mesh* m, const char* field, void* out, int64_t* n_out);
 /* boundaryEdgesPhysicals[b] (mesh.h:225) */
 int afx_mesh_boundary_name(const afx_mesh* m, int64_t b, char* buf, int32_t buflen);
-/* Cell order the device uses (perm[new] = old real-cell index). */
+/* Reverse Cuthill-McKee order of the real cells (perm[new] = old). */
 int afx_mesh_rcm(const afx_mesh* m, int32_t* perm);
+/* Tile statistics for a tile size: stats[8] = {n_tiles, max local cells,
+ * max gradient cells, max local faces, max faces touching a tile,
+ * total local cells, total gradient cells, total local faces}. */
+int afx_mesh_tile_stats(const afx_mesh* m, int32_t tile_cells, int64_t stats[8]);
 void afx_mesh_destroy(afx_mesh* m);
 
 /* ---------------------------------------------------------------- solver --

@@ -14,11 +14,12 @@ pytestmark = pytest.mark.gpu
 FP64_TOL = 1e-12
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("path", [QUAD, TRI])
 @pytest.mark.parametrize("second_order", [True, False])
-def test_residual_fp64_inviscid(path, second_order):
+def test_residual_fp64_inviscid(path, second_order, fused):
     from aeroflex_amd import rans
-    s, o, q = make_pair(path, second_order=second_order)
+    s, o, q = make_pair(path, second_order=second_order, fused=fused)
     R, nrm = s.fillRhoRHS(rans.AFX_FP64)
     o.fill_rhs()
     Ro = o.vec("RhoVector")
@@ -26,18 +27,20 @@ def test_residual_fp64_inviscid(path, second_order):
     assert abs(nrm - np.linalg.norm(Ro)) <= 1e-12 * np.linalg.norm(Ro)
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("path", [QUAD, TRI])
-def test_residual_fp64_laminar_wall(path):
+def test_residual_fp64_laminar_wall(path, fused):
     from aeroflex_amd import rans
-    s, o, q = make_pair(path, viscosity="laminar", wall="wall", mu_L=7e-4)
+    s, o, q = make_pair(path, viscosity="laminar", wall="wall", mu_L=7e-4, fused=fused)
     R, _ = s.fillRhoRHS(rans.AFX_FP64)
     o.fill_rhs()
     assert rel_max(R, o.vec("RhoVector")) <= FP64_TOL
 
 
-def test_residual_fp64_least_squares():
+@pytest.mark.parametrize("fused", [True, False])
+def test_residual_fp64_least_squares(fused):
     from aeroflex_amd import rans
-    s, o, q = make_pair(QUAD, gradient="least-squares")
+    s, o, q = make_pair(QUAD, gradient="least-squares", fused=fused)
     R, _ = s.fillRhoRHS(rans.AFX_FP64)
     o.fill_rhs()
     assert rel_max(R, o.vec("RhoVector")) <= 1e-10
@@ -105,10 +108,11 @@ def test_wall_distance():
     assert np.array_equal(d, o.vec("wall_dist"))
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("path", [QUAD, TRI])
-def test_residual_fp32(path):
+def test_residual_fp32(path, fused):
     from aeroflex_amd import rans
-    s, o, q = make_pair(path)
+    s, o, q = make_pair(path, fused=fused)
     R, _ = s.fillRhoRHS(rans.AFX_FP32)
     o.fill_rhs()
     Ro = o.vec("RhoVector")
@@ -122,3 +126,26 @@ def test_renumbering_invariance():
     R1, _ = s1.fillRhoRHS(rans.AFX_FP64)
     R2, _ = s2.fillRhoRHS(rans.AFX_FP64)
     assert np.array_equal(R1, R2)
+
+
+@pytest.mark.parametrize("tile_cells", [16, 48, 128, 512])
+def test_fused_tile_sizes_bitexact(tile_cells):
+    """Small tiles put most faces on tile boundaries (halo gradients and cut
+    faces computed by two workgroups); the result must not change."""
+    from aeroflex_amd import rans
+    s, o, q = make_pair(TRI, tile_cells=tile_cells)
+    R, _ = s.fillRhoRHS(rans.AFX_FP64)
+    o.fill_rhs()
+    Ro = o.vec("RhoVector")
+    assert rel_max(R, Ro) <= FP64_TOL
+    # wall ghosts are refreshed from their owners as set_walls_from_internal does
+    assert np.array_equal(s.get_q(), o.vec("q"))
+
+
+def test_fused_matches_three_kernel_path_fp32():
+    from aeroflex_amd import rans
+    s1, o, q = make_pair(QUAD, fused=True)
+    s2, _, _ = make_pair(QUAD, fused=False)
+    R1, _ = s1.fillRhoRHS(rans.AFX_FP32)
+    R2, _ = s2.fillRhoRHS(rans.AFX_FP32)
+    assert np.linalg.norm(R1 - R2) / np.linalg.norm(R2) <= 1e-5
