@@ -694,9 +694,13 @@ __device__ __forceinline__ void roe_flux_f32(float gamma, float nx, float ny, co
 #ifdef AFX_FP32_UNIT   // defined in afx_k32.hip only
 __device__ __forceinline__ float4 f4(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
 __device__ __forceinline__ float get(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+// slot s of a cell's packed slot word (4 x u16, see TilesF32)
+__device__ __forceinline__ uint32_t slot_of(const uint2& w, int s) {
+    return s == 0 ? (w.x & 0xFFFFu) : s == 1 ? (w.x >> 16) : s == 2 ? (w.y & 0xFFFFu) : (w.y >> 16);
+}
 #define F4_LOOP(expr) { expr(x); expr(y); expr(z); expr(w); }
 
-__global__ void __launch_bounds__(kBlock) k_residual_tile_f32(TilesF32 S, const float4* __restrict__ q, int32_t nC,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) k_residual_tile_f32(TilesF32 S, const float4* __restrict__ q, int32_t nC,
                                                               const float* __restrict__ iarea,
                                                               const float* __restrict__ a15,
                                                               const float* __restrict__ lsq,
@@ -729,8 +733,9 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile_f32(TilesF32 S, const 
     __syncthreads();
 
     // 2. gradients and limiters of T u H1
+    const uint2* slots2 = reinterpret_cast<const uint2*>(S.slots) + go;
     for (int c = threadIdx.x; c < nG; c += blockDim.x) {
-        const uint64_t sw = S.slots[go + c];
+        const uint2 sw = slots2[c];
         const int gc = Lg[c];
         const float4 qc = sq[c];
         float4 qmin = qc, qmax = qc;
@@ -740,7 +745,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile_f32(TilesF32 S, const 
         if (gg) {
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const uint32_t e = (uint32_t)(sw >> (16 * s)) & 0xFFFFu;
+                const uint32_t e = slot_of(sw, s);
                 if (e == 0xFFFFu) break;
                 const int f = (int)(e & 0x7FFFu);
                 const bool nb = (e >> 15) != 0;
@@ -787,7 +792,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile_f32(TilesF32 S, const 
             float mn[4] = {qc.x, qc.y, qc.z, qc.w}, mx[4] = {qc.x, qc.y, qc.z, qc.w};
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const uint32_t e = (uint32_t)(sw >> (16 * s)) & 0xFFFFu;
+                const uint32_t e = slot_of(sw, s);
                 if (e == 0xFFFFu) break;
                 const int f = (int)(e & 0x7FFFu);
                 const bool nb = (e >> 15) != 0;
@@ -830,6 +835,8 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile_f32(TilesF32 S, const 
             for (int s = 0; s < 4; ++s) {
                 if (s >= ns) break;
                 const float dx = dxs[s], dy = dys[s];
+                // D has the sign of dqg, so the denominator D^2 + K3a + dqg D
+                // + 2 dqg^2 is positive whenever dqg != 0 (no NaN reaches fminf)
 #define LIM(m) { const float dqg = sx.m * dx + sy.m * dy; const bool pos = dqg > 1e-16f, neg = dqg < -1e-16f; \
                  const float D = pos ? dM.m : dm.m, DD = pos ? ddM.m : ddm.m; const float tq = dqg * D; \
                  float phi = (DD + 2.f * tq) * frcp(DD + tq + 2.f * dqg * dqg); phi = (pos || neg) ? phi : 1.f; \
@@ -896,16 +903,19 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile_f32(TilesF32 S, const 
 
     // 4. signed sums in reference face order (exact FP64 accumulation of the FP32 fluxes)
     for (int c = threadIdx.x; c < nTt; c += blockDim.x) {
-        const uint64_t sw = S.slots[go + c];
+        const uint2 sw = slots2[c];
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const uint32_t e = (uint32_t)(sw >> (16 * s)) & 0xFFFFu;
+            const uint32_t e = slot_of(sw, s);
             if (e == 0xFFFFu) break;
             const int fl = (int)(e & 0x7FFFu);
             const float4 v = sfl[fl];
-            if (e >> 15) { a0 += (double)v.x; a1 += (double)v.y; a2 += (double)v.z; a3 += (double)v.w; }
-            else { a0 -= (double)v.x; a1 -= (double)v.y; a2 -= (double)v.z; a3 -= (double)v.w; }
+            const bool nb = (e >> 15) != 0;
+            a0 += (double)(nb ? v.x : -v.x);
+            a1 += (double)(nb ? v.y : -v.y);
+            a2 += (double)(nb ? v.z : -v.z);
+            a3 += (double)(nb ? v.w : -v.w);
             if (qghost) {
                 const uint32_t fc = fck[fl];
                 const int kind = (int)(fc >> 24);
