@@ -22,6 +22,7 @@ UNITS = [
     # (source, extra flags)
     ("afx_k64.hip", ["-ffp-contract=off"]),     # plain FP64 form: reference rounding order
     ("afx_k32.hip", ["-ffp-contract=fast"]),    # FP32 throughput form
+    ("afx_implicit.hip", ["-ffp-contract=off"]),
     ("afx_capi.hip", ["-ffp-contract=off"]),
 ]
 CXX_UNITS = [("afx_mesh.cpp", ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]),
@@ -47,6 +48,12 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # C++ driver over the C ABI (rans_gpu.hpp), linked against the library
+    drv = os.path.join(HERE, "rans_gpu_driver")
+    _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(os.path.dirname(HERE), "include"),
+          os.path.join(HERE, "cpp", "rans_gpu_driver.cpp"), "-L", HERE, "-lafx_rans",
+          "-Wl,-rpath,$ORIGIN", "-o", drv + ".tmp"])
+    os.replace(drv + ".tmp", drv)
     return LIB
 
 

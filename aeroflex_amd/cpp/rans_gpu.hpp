@@ -1,0 +1,260 @@
+// C++ interface of the MI355X residual, shaped like the reference's
+// src/rans/include/rans headers so that a caller of rans::implicitSolver /
+// rans::explicitSolver can switch to the device solver:
+//
+//   reference                                   here
+//   rans::gas (core.h:27-46)                    rans::gpu::gas
+//   rans::boundary_variables (core.h:61-84)     rans::gpu::boundary_variables
+//   rans::boundary_condition (core.h:112-115)   rans::gpu::boundary_condition
+//   rans::mesh(filename) (mesh.h:198-273)       rans::gpu::mesh
+//   rans::implicitSolver (solver.h:852-970)     rans::gpu::implicitSolver
+//   rans::explicitSolver (solver.h:721-742)     rans::gpu::explicitSolver
+//   multigrid<T>::run_solver (multigrid.h:183-293) rans::gpu::run_solver
+//   get_wall_profile (post.h:301-387)           rans::gpu::get_wall_profile
+//
+// Every call goes through the C ABI in include/afx_rans.h; errors come back
+// as the exceptions the reference throws at the same places
+// (std::runtime_error for I/O, std::invalid_argument for bad meshes,
+// std::out_of_range for unknown boundary names).  Vectors use the reference's
+// Eigen layout q(4*i+k) in a std::vector<double>.
+#pragma once
+
+#include <afx_rans.h>
+
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace rans {
+namespace gpu {
+
+inline void check(int rc) {
+    if (rc == AFX_OK) return;
+    std::string msg = afx_last_error();
+    if (rc == AFX_EIO) throw std::runtime_error(msg);
+    if (rc == AFX_EARG && msg.find("no boundary condition") != std::string::npos) throw std::out_of_range(msg);
+    if (rc == AFX_EARG) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+struct gas {
+    double R = 0.71428571428, mu_L = 1e-5, Pr_L = 0.72, Pr_T = 0.9, cp = 1., gamma = 1.4;
+    gas(double R_ = 0.71428571428, double mu_L_ = 1e-5, double Pr_L_ = 0.72, double Pr_T_ = 0.9,
+        double gamma_ = 1.4)
+        : R(R_), mu_L(mu_L_), Pr_L(Pr_L_), Pr_T(Pr_T_), gamma(gamma_) {}
+    double k() const { return cp * mu_L / Pr_L; }
+};
+
+struct boundary_variables {
+    double mach = 0.2, angle = 0, T = 1, p = 1.;
+    boundary_variables() {}
+    boundary_variables(double m, double a = 0, double t = 1, double pp = 1.) : mach(m), angle(a), T(t), p(pp) {}
+};
+
+struct boundary_condition {
+    std::string bc_type;
+    boundary_variables vars_far;
+};
+
+class mesh {
+    afx_mesh* h_ = nullptr;
+
+public:
+    mesh() {}
+    explicit mesh(const std::string& filename) { read_file(filename); }
+    mesh(const mesh&) = delete;
+    mesh& operator=(const mesh&) = delete;
+    mesh(mesh&& o) noexcept : h_(o.h_) { o.h_ = nullptr; }
+    mesh& operator=(mesh&& o) noexcept {
+        if (this != &o) {
+            if (h_) afx_mesh_destroy(h_);
+            h_ = o.h_;
+            o.h_ = nullptr;
+        }
+        return *this;
+    }
+    ~mesh() { if (h_) afx_mesh_destroy(h_); }
+    void read_file(const std::string& filename) {
+        afx_mesh* m = nullptr;
+        check(afx_mesh_read_msh(filename.c_str(), &m));
+        if (h_) afx_mesh_destroy(h_);
+        h_ = m;
+    }
+    static mesh naca_ogrid(int ni, int nj, double r_far = 20.0, double thickness = 0.12, double first_h = 2e-3) {
+        mesh out;
+        check(afx_mesh_gen_naca_ogrid(ni, nj, r_far, thickness, first_h, &out.h_));
+        return out;
+    }
+    const afx_mesh* handle() const { return h_; }
+    int64_t count(int k) const {
+        int64_t c[5];
+        check(afx_mesh_counts(h_, c));
+        return c[k];
+    }
+    int64_t nRealCells() const { return count(1); }
+    int64_t nCells() const { return count(2); }
+};
+
+// Common part of rans::solver (solver.h:40-175)
+class solver {
+protected:
+    afx_rans* h_ = nullptr;
+    afx_opts opts_{};
+    std::vector<double> q_;
+    int64_t nT_ = 0, nR_ = 0;
+    double cfl_ = 1;
+    std::string viscosity_model_;
+    std::map<std::string, boundary_condition> bcs_;
+
+    static int viscosity_code(const std::string& v) {
+        if (v == "inviscid") return 0;
+        if (v == "laminar") return 1;
+        throw std::invalid_argument("unsupported viscosity model: " + v);
+    }
+
+public:
+    solver(const mesh& m, const gas& g, std::string viscosity_model) : viscosity_model_(viscosity_model) {
+        afx_default_opts(&opts_);
+        opts_.viscosity = viscosity_code(viscosity_model);
+        afx_gas ag{g.R, g.mu_L, g.Pr_L, g.Pr_T, g.gamma};
+        check(afx_rans_create(m.handle(), &ag, &opts_, &h_));
+        nR_ = m.nRealCells();
+        nT_ = m.nCells();
+        q_.assign(4 * nT_, 0.0);
+    }
+    solver(const solver&) = delete;
+    solver& operator=(const solver&) = delete;
+    virtual ~solver() { if (h_) afx_rans_destroy(h_); }
+
+    void set_bcs(std::map<std::string, boundary_condition> bcs_in) {
+        bcs_ = bcs_in;
+        std::vector<const char*> names;
+        std::vector<int32_t> types;
+        std::vector<double> vars;
+        for (auto& kv : bcs_) {
+            const std::string& t = kv.second.bc_type;
+            int32_t code = t == "farfield" ? AFX_BC_FARFIELD : t == "slip-wall" ? AFX_BC_SLIP : t == "wall" ? AFX_BC_WALL : -1;
+            if (code < 0) throw std::invalid_argument("unsupported boundary type: " + t);
+            names.push_back(kv.first.c_str());
+            types.push_back(code);
+            const boundary_variables& v = kv.second.vars_far;
+            vars.insert(vars.end(), {v.mach, v.angle, v.T, v.p});
+        }
+        check(afx_rans_set_bcs(h_, (int32_t)names.size(), names.data(), types.data(), vars.data()));
+    }
+    void set_cfl(const double& cfl_in) { cfl_ = cfl_in; check(afx_rans_set_cfl(h_, cfl_in)); }
+    void set_second_order(const bool x = true) { opts_.second_order = x; check(afx_rans_set_options(h_, &opts_)); }
+    void set_gradient_scheme(std::string grads) {
+        if (grads == "least-squares") opts_.gradient = 0;
+        else if (grads == "green-gauss") opts_.gradient = 1;
+        else throw std::invalid_argument("unknown gradient scheme: " + grads);
+        check(afx_rans_set_options(h_, &opts_));
+    }
+    void set_limiter_k(const double& x) { opts_.limiter_k = x; check(afx_rans_set_options(h_, &opts_)); }
+    void init() { check(afx_rans_init(h_)); }
+    void refill_bcs() { check(afx_rans_refill_bcs(h_)); }
+    void bcs_from_internal() { check(afx_rans_bcs_from_internal(h_)); }
+    double get_uniform_residual() {
+        double e = 0;
+        check(afx_rans_uniform_residual(h_, &e));
+        return e;
+    }
+    // get_q() (solver.h:255-257): a host copy; write back with set_q()
+    std::vector<double>& get_q() {
+        check(afx_rans_get_state(h_, q_.data()));
+        return q_;
+    }
+    void set_q(const std::vector<double>& q) {
+        if ((int64_t)q.size() != 4 * nT_) throw std::invalid_argument("state has the wrong length");
+        check(afx_rans_set_state(h_, q.data()));
+    }
+    std::map<std::string, boundary_condition> get_bcs() const { return bcs_; }
+    std::string get_viscosity_model() const { return viscosity_model_; }
+    afx_rans* handle() const { return h_; }
+    unsigned get_print_interval() const { return 1; }
+
+    virtual void fill() {}
+    virtual int compute() { return -1; }
+    virtual double solve(const double relaxation = 1, const double tol = 0, const int rhs_iterations = 5) = 0;
+};
+
+// rans::implicitSolver (solver.h:852-1213)
+class implicitSolver : public solver {
+public:
+    implicitSolver(const mesh& m, const gas& g, std::string viscosity_model) : solver(m, g, viscosity_model) {}
+    void fill() override { check(afx_rans_implicit_fill(h_)); }
+    int compute() override {
+        int32_t st = -1;
+        check(afx_rans_implicit_compute(h_, &st));
+        return st;
+    }
+    double solve(const double relaxation = 1, const double tol = 0, const int rhs_iterations = 5) override {
+        double e = 0;
+        check(afx_rans_implicit_solve(h_, relaxation, tol, rhs_iterations, AFX_FP64, &e));
+        return e;
+    }
+};
+
+// rans::explicitSolver (solver.h:721-828)
+class explicitSolver : public solver {
+public:
+    explicitSolver(const mesh& m, const gas& g, std::string viscosity_model) : solver(m, g, viscosity_model) {}
+    void fill() override {}
+    int compute() override { return 0; }
+    double solve(const double relaxation = 1, const double = 0, const int = 5) override {
+        double e = 0;
+        check(afx_rans_explicit_step(h_, relaxation, &e));
+        return e;
+    }
+};
+
+struct Settings {
+    double start_cfl = 40., slope_cfl = 50., max_cfl = 100., relaxation = 0.9, tolerance = 1e-4;
+    int rhs_iterations = 5, max_iterations = 300;
+};
+
+struct wallProfile {
+    double cd = 0., cl = 0., cm = 0.;
+};
+
+// get_wall_profile (post.h:301-387)
+inline wallProfile get_wall_profile(solver& s, const std::string& patch) {
+    wallProfile w;
+    check(afx_rans_wall_forces(s.handle(), patch.c_str(), &w.cl, &w.cd, &w.cm));
+    return w;
+}
+
+// multigrid<implicitSolver>::run_solver (multigrid.h:240-293); returns 0 on
+// success, 1 when the solver reports failure; residuals receives the
+// normalised residual of every iteration
+inline int run_solver(solver& s, const Settings& st, std::vector<double>& residuals,
+                      const std::function<bool()>& stop = nullptr) {
+    double cfl = st.start_cfl;
+    int i = 0;
+    double err = 0;
+    double err_0 = s.get_uniform_residual();
+    do {
+        s.set_cfl(cfl);
+        s.fill();
+        int ok = s.compute();
+        if (ok == 0) {
+            err = s.solve(st.relaxation, err_0 * st.tolerance, st.rhs_iterations);
+            if ((i == 0) && (err > 2 * err_0)) err_0 = err;
+        } else {
+            err = -1;
+        }
+        err /= err_0;
+        cfl = std::min(st.start_cfl + (i + 1) * st.slope_cfl, st.max_cfl);
+        i++;
+        if (err < 0) return 1;
+        residuals.push_back(err);
+    } while ((err > st.tolerance) && (i < st.max_iterations) && !(stop && stop()));
+    return 0;
+}
+
+}  // namespace gpu
+}  // namespace rans

@@ -21,6 +21,7 @@
 #include "afx_kernels.hpp"
 #include "afx_mesh.hpp"
 #include "afx_tiles.hpp"
+#include "afx_implicit.hpp"
 
 using namespace afx;
 
@@ -102,6 +103,17 @@ struct afx_rans {
     DBuf<float4> q32v, R32v, t_geo4;   // FP32 fused form: AoS float4 per cell / two per tile face
     DBuf<uint32_t> t_fck;              // c0 | c1 << 12 | kind << 24 per tile face
     bool r32_aos = false;              // last FP32 residual is in R32v (fused) rather than R32
+
+    // implicit step: Jacobian blocks, diagonal blocks and their inverses,
+    // GMRES work space
+    DBuf<double> Jf, Dblk, Dinv, kV, kw, kz, kr, ktmp, kh, kpart;
+    DBuf<int> kbad;
+    int32_t gm_restart = 30, gm_maxit = 500;
+    double gm_tol = 1e-2;
+    bool lhs_ready = false;
+    int lhs_bad = 0;
+    int64_t lin_iters = 0;
+    int kblk = 0;
     std::vector<int32_t> t_lg;   // device cell of each local cell (unflagged)
     bool tile64_ok = false, tile32_ok = false;
 
@@ -312,6 +324,121 @@ int check_handle(afx_rans* h, bool need_bcs = true) {
     if (!h) return fail(AFX_EARG, "null solver handle");
     if (need_bcs && !h->bcs_set) return fail(AFX_ESTATE, "boundary conditions not set (call afx_rans_set_bcs)");
     return AFX_OK;
+}
+
+
+// ------------------------------------------------------------ implicit step --
+int ensure_implicit(afx_rans* h) {
+    if (h->Jf.p) return AFX_OK;
+    const size_t n = (size_t)NV * h->nT;
+    const int m = h->gm_restart;
+    h->kblk = 512;
+    HIPCHK(h->Jf.alloc((size_t)32 * h->nF));
+    HIPCHK(h->Dblk.alloc((size_t)16 * h->nC));
+    HIPCHK(h->Dinv.alloc((size_t)16 * h->nC));
+    HIPCHK(h->kV.alloc((size_t)(m + 1) * n));
+    HIPCHK(h->kw.alloc(n));
+    HIPCHK(h->kz.alloc(n));
+    HIPCHK(h->kr.alloc(n));
+    HIPCHK(h->ktmp.alloc(n));
+    HIPCHK(h->kh.alloc(2 * (size_t)(m + 2)));
+    HIPCHK(h->kpart.alloc((size_t)(m + 2) * h->kblk));
+    HIPCHK(h->kbad.alloc(1));
+    HIPCHK(hipMemsetAsync(h->Jf.p, 0, (size_t)32 * h->nF * 8, h->stream));
+    return AFX_OK;
+}
+
+// dot products of w with the first m vectors of V (stride ld) into dev[0..m)
+int dots(afx_rans* h, const double* V, int64_t ld, int m, const double* w, int64_t n, double* dev) {
+    HIPCHK(launch_multidot(h->stream, V, ld, m, w, n, h->kpart.p, h->kblk, dev));
+    return AFX_OK;
+}
+
+// Right-preconditioned restarted GMRES (classical Gram-Schmidt applied twice)
+// for LHS x = b; b and x are device vectors of length 4*nT.  Returns AFX_OK
+// and *converged, or a negative error code.
+int gmres(afx_rans* h, const double* b, double* x, bool* converged) {
+    const int64_t n = (int64_t)NV * h->nT;
+    const int m = h->gm_restart;
+    hipStream_t s = h->stream;
+    double* V = h->kV.p;
+    double* hd = h->kh.p;            // [0, m+2): first pass, [m+2, 2m+4): second pass + norm
+    *converged = false;
+    HIPCHK(hipMemsetAsync(x, 0, n * 8, s));
+    int rc;
+    if ((rc = dots(h, b, n, 1, b, n, hd))) return rc;
+    double beta0 = 0;
+    HIPCHK(hipMemcpyAsync(&beta0, hd, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    beta0 = std::sqrt(beta0);
+    if (!(beta0 > 0)) { *converged = true; return AFX_OK; }
+    HIPCHK(launch_copy(s, b, h->kr.p, n));
+    double beta = beta0;
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hv(2 * (m + 2));
+    int total = 0;
+    while (true) {
+        HIPCHK(launch_scale(s, 1.0 / beta, h->kr.p, V, n));
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int jend = 0;
+        bool conv = false;
+        for (int j = 0; j < m; ++j) {
+            double* vj = V + (size_t)j * n;
+            HIPCHK(launch_block_jacobi(s, h->nC, h->nT, h->Dinv.p, vj, h->kz.p));
+            HIPCHK(launch_lhs_apply(s, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, h->kz.p, h->kw.p));
+            if ((rc = dots(h, V, n, j + 1, h->kw.p, n, hd))) return rc;
+            HIPCHK(launch_multiaxpy(s, V, n, j + 1, hd, h->kw.p, n));
+            if ((rc = dots(h, V, n, j + 1, h->kw.p, n, hd + m + 2))) return rc;
+            HIPCHK(launch_multiaxpy(s, V, n, j + 1, hd + m + 2, h->kw.p, n));
+            if ((rc = dots(h, h->kw.p, n, 1, h->kw.p, n, hd + m + 2 + j + 1))) return rc;
+            HIPCHK(hipMemcpyAsync(hv.data(), hd, hv.size() * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hv[i] + hv[m + 2 + i];
+            const double nrm = std::sqrt(std::max(hv[m + 2 + j + 1], 0.0));
+            H[(size_t)(j + 1) * m + j] = nrm;
+            for (int i = 0; i < j; ++i) {
+                const double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
+                H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
+                H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * bb;
+            }
+            const double a = H[(size_t)j * m + j], bb = H[(size_t)(j + 1) * m + j];
+            const double r = std::hypot(a, bb);
+            cs[j] = r > 0 ? a / r : 1.0;
+            sn[j] = r > 0 ? bb / r : 0.0;
+            H[(size_t)j * m + j] = r;
+            H[(size_t)(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            ++total;
+            jend = j + 1;
+            if (std::fabs(g[j + 1]) <= h->gm_tol * beta0) { conv = true; break; }
+            if (!(nrm > 0) || total >= h->gm_maxit) break;
+            HIPCHK(launch_scale(s, 1.0 / nrm, h->kw.p, V + (size_t)(j + 1) * n, n));
+        }
+        // y = H^-1 g (upper triangular), x += M^-1 (V y)
+        for (int i = jend - 1; i >= 0; --i) {
+            double t = g[i];
+            for (int k = i + 1; k < jend; ++k) t -= H[(size_t)i * m + k] * y[k];
+            y[i] = t / H[(size_t)i * m + i];
+        }
+        HIPCHK(hipMemcpyAsync(hd, y.data(), jend * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(h->ktmp.p, 0, n * 8, s));
+        HIPCHK(launch_lincomb(s, V, n, jend, hd, h->ktmp.p, n));
+        HIPCHK(launch_block_jacobi(s, h->nC, h->nT, h->Dinv.p, h->ktmp.p, h->kz.p));
+        HIPCHK(launch_axpy(s, 1.0, h->kz.p, x, n));
+        h->lin_iters += jend;
+        if (conv) { *converged = true; HIPCHK(hipStreamSynchronize(s)); return AFX_OK; }
+        if (total >= h->gm_maxit) { HIPCHK(hipStreamSynchronize(s)); return AFX_OK; }
+        // restart from the true residual
+        HIPCHK(launch_lhs_apply(s, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, x, h->kw.p));
+        HIPCHK(launch_copy(s, b, h->kr.p, n));
+        HIPCHK(launch_axpy(s, -1.0, h->kw.p, h->kr.p, n));
+        if ((rc = dots(h, h->kr.p, n, 1, h->kr.p, n, hd))) return rc;
+        HIPCHK(hipMemcpyAsync(&beta, hd, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        beta = std::sqrt(beta);
+        if (beta <= h->gm_tol * beta0) { *converged = true; return AFX_OK; }
+    }
 }
 
 }  // namespace
@@ -1000,6 +1127,100 @@ int afx_rans_explicit_step(afx_rans* h, double relaxation, double* err) {
     h->q32_valid = false;
     if (err) return norm_of(h, h->qW.p, h->nT, NV, err);
     HIPCHK(hipStreamSynchronize(s));
+    return AFX_OK;
+}
+
+int afx_rans_set_linear_solver(afx_rans* h, int32_t restart, double tol, int32_t maxit) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (restart < 1 || restart > 200 || !(tol > 0) || maxit < 1) return fail(AFX_EARG, "bad linear solver settings");
+    if (h->Jf.p && restart != h->gm_restart) return fail(AFX_ESTATE, "restart length is fixed after the first fill");
+    h->gm_restart = restart;
+    h->gm_tol = tol;
+    h->gm_maxit = maxit;
+    return AFX_OK;
+}
+
+int afx_rans_implicit_fill(afx_rans* h) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if ((rc = ensure_implicit(h))) return rc;
+    hipStream_t s = h->stream;
+    HIPCHK(launch_dt<double>(s, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->cfl, h->dt.p));
+    if (h->opts.viscosity != 0 && (rc = run_gradients64(h, h->q.p, h->q.p))) return rc;
+    HIPCHK(launch_face_jacobian(s, h->cells64(), h->faces64(), h->q.p, h->gx.p, h->gy.p, h->gas64(),
+                                h->opts.viscosity, h->Jf.p));
+    HIPCHK(hipMemsetAsync(h->kbad.p, 0, sizeof(int), s));
+    HIPCHK(launch_assemble_diag(s, h->cells64(), h->faces64(), h->dt.p, h->Jf.p, h->Dblk.p, h->Dinv.p, h->kbad.p));
+    HIPCHK(hipMemcpyAsync(&h->lhs_bad, h->kbad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    h->lhs_ready = true;
+    h->q32_valid = false;
+    return AFX_OK;
+}
+
+int afx_rans_implicit_compute(afx_rans* h, int32_t* status) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!h->lhs_ready) return fail(AFX_ESTATE, "call afx_rans_implicit_fill first");
+    if (status) *status = h->lhs_bad ? -1 : 0;
+    return AFX_OK;
+}
+
+int afx_rans_implicit_solve(afx_rans* h, double relaxation, double tol, int32_t rhs_iterations, int32_t precision,
+                            double* err_out) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!h->lhs_ready) return fail(AFX_ESTATE, "call afx_rans_implicit_fill first");
+    if (precision != AFX_FP64) return fail(AFX_EUNSUP, "the implicit step uses the FP64 residual");
+    hipStream_t s = h->stream;
+    const int64_t n = (int64_t)NV * h->nT;
+    h->lin_iters = 0;
+    auto residual_norm = [&](double* e) -> int {
+        int r2 = enqueue_residual(h, AFX_FP64);
+        if (r2) return r2;
+        return norm_of(h, h->R.p, h->nT, NV, e);
+    };
+    auto step = [&](bool* ok) -> int {
+        int r2 = gmres(h, h->R.p, h->qW.p, ok);
+        if (r2) return r2;
+        if (*ok) {
+            HIPCHK(launch_axpy(s, relaxation, h->qW.p, h->q.p, n));
+            h->q32_valid = false;
+        }
+        return AFX_OK;
+    };
+    double err = 0;
+    bool ok = false;
+    if ((rc = residual_norm(&err))) return rc;
+    if (err < tol) { if (err_out) *err_out = err; return AFX_OK; }
+    const double err_ini = err;
+    if ((rc = step(&ok))) return rc;
+    if (!ok) { if (err_out) *err_out = -1; return AFX_OK; }
+    for (int i = 0; i < rhs_iterations; ++i) {
+        if ((rc = residual_norm(&err))) return rc;
+        if (err < tol || err > 10 * err_ini) { if (err_out) *err_out = err; return AFX_OK; }
+        if ((rc = step(&ok))) return rc;
+        if (!ok) { if (err_out) *err_out = -1; return AFX_OK; }
+    }
+    if ((rc = residual_norm(&err))) return rc;
+    if (err_out) *err_out = err;
+    return AFX_OK;
+}
+
+int afx_rans_lhs_apply(afx_rans* h, const double* x, double* y) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!h->lhs_ready) return fail(AFX_ESTATE, "call afx_rans_implicit_fill first");
+    if ((rc = upload_aos(h, x, h->ktmp.p))) return rc;
+    HIPCHK(launch_lhs_apply(h->stream, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, h->ktmp.p, h->kw.p));
+    return download_aos(h, h->kw.p, y);
+}
+
+int afx_rans_linear_iterations(afx_rans* h, int64_t* iters) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    *iters = h->lin_iters;
     return AFX_OK;
 }
 

@@ -85,6 +85,12 @@ def _load():
         "afx_rans_explicit_residual": (C.c_int, [P, dp, dp]),
         "afx_rans_explicit_step": (C.c_int, [P, f64, dp]),
         "afx_rans_uniform_residual": (C.c_int, [P, dp]),
+        "afx_rans_set_linear_solver": (C.c_int, [P, i32, f64, i32]),
+        "afx_rans_implicit_fill": (C.c_int, [P]),
+        "afx_rans_implicit_compute": (C.c_int, [P, C.POINTER(i32)]),
+        "afx_rans_implicit_solve": (C.c_int, [P, f64, f64, i32, i32, dp]),
+        "afx_rans_lhs_apply": (C.c_int, [P, dp, dp]),
+        "afx_rans_linear_iterations": (C.c_int, [P, C.POINTER(i64)]),
         "afx_rans_wall_forces": (C.c_int, [P, C.c_char_p, dp, dp, dp]),
         "afx_rans_wall_dist": (C.c_int, [P, dp]),
         "afx_rans_counts": (C.c_int, [P, C.POINTER(i64)]),
@@ -347,6 +353,37 @@ class Solver:
         _check(lib.afx_rans_explicit_step(self.h, relaxation, C.byref(e)))
         return e.value
 
+    # -- implicit pseudo-time step (implicitSolver, solver.h:852-1213)
+    def set_linear_solver(self, restart=30, tol=1e-2, maxit=500):
+        _check(lib.afx_rans_set_linear_solver(self.h, restart, tol, maxit))
+
+    def fill(self):
+        """fillRhoLHS (solver.h:973-1075)."""
+        _check(lib.afx_rans_implicit_fill(self.h))
+
+    def compute(self):
+        """Preconditioner set-up; 0 = ok, -1 = failure (solver.h:1160-1167)."""
+        st = C.c_int32()
+        _check(lib.afx_rans_implicit_compute(self.h, C.byref(st)))
+        return st.value
+
+    def solve(self, relaxation=1.0, tol=0.0, rhs_iterations=5, precision=AFX_FP64):
+        """implicitSolver::solve (solver.h:1170-1213); -1 when the linear solve fails."""
+        e = C.c_double()
+        _check(lib.afx_rans_implicit_solve(self.h, relaxation, tol, rhs_iterations, precision, C.byref(e)))
+        return e.value
+
+    def lhs_apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        _check(lib.afx_rans_lhs_apply(self.h, _dptr(x), _dptr(y)))
+        return y
+
+    def linear_iterations(self):
+        n = C.c_int64()
+        _check(lib.afx_rans_linear_iterations(self.h, C.byref(n)))
+        return n.value
+
     def get_uniform_residual(self):
         e = C.c_double()
         _check(lib.afx_rans_uniform_residual(self.h, C.byref(e)))
@@ -377,3 +414,45 @@ class Solver:
         if getattr(self, "h", None) is not None and lib is not None:
             lib.afx_rans_destroy(self.h)
             self.h = None
+
+
+class Settings:
+    """Solver settings of rans::Settings (core.h:166-232) that the iteration
+    loop reads."""
+
+    def __init__(self, start_cfl=40.0, slope_cfl=50.0, max_cfl=100.0, relaxation=0.9, tolerance=1e-4,
+                 rhs_iterations=5, max_iterations=300, airfoil_name="wall"):
+        self.start_cfl, self.slope_cfl, self.max_cfl = start_cfl, slope_cfl, max_cfl
+        self.relaxation, self.tolerance = relaxation, tolerance
+        self.rhs_iterations, self.max_iterations = rhs_iterations, max_iterations
+        self.airfoil_name = airfoil_name
+
+
+def run_solver(s, settings, residuals=None, stop=None):
+    """multigrid<implicitSolver>::run_solver (multigrid.h:240-293) over the
+    device solver: CFL ramp, first-iteration normalisation rebase, stop on
+    tolerance / iteration count / stop flag.  Returns (state, residuals):
+    state 0 ok, 1 linear-solver failure."""
+    residuals = [] if residuals is None else residuals
+    cfl = settings.start_cfl
+    i = 0
+    err = 0.0
+    err_0 = s.get_uniform_residual()
+    while True:
+        s.set_cfl(cfl)
+        s.fill()
+        ok = s.compute()
+        if ok == 0:
+            err = s.solve(settings.relaxation, err_0 * settings.tolerance, settings.rhs_iterations)
+            if i == 0 and err > 2 * err_0:
+                err_0 = err
+        else:
+            err = -1
+        err /= err_0
+        cfl = min(settings.start_cfl + (i + 1) * settings.slope_cfl, settings.max_cfl)
+        i += 1
+        if err < 0:
+            return 1, residuals
+        residuals.append(err)
+        if not (err > settings.tolerance and i < settings.max_iterations and not (stop and stop())):
+            return 0, residuals

@@ -156,6 +156,31 @@ int afx_rans_explicit_residual(afx_rans* h, const double* qk, double* qW);
 /* explicitSolver::solve (solver.h:802-828): 3-stage pseudo-time step on the
  * device, err = ||qW||_2 of the last stage */
 int afx_rans_explicit_step(afx_rans* h, double relaxation, double* err);
+/* ---- implicit pseudo-time step (implicitSolver, solver.h:852-1213) ----
+ * The linear solver replacing Eigen's GMRES + IncompleteLUT (solver.h:886,
+ * 907-910): restarted GMRES(restart) with a block-Jacobi (4x4) right
+ * preconditioner, FP64, stopping at ||b - A x|| <= tol ||b|| or after maxit
+ * iterations (defaults 30, 1e-2, 500 as the reference sets them). */
+int afx_rans_set_linear_solver(afx_rans* h, int32_t restart, double tol, int32_t maxit);
+/* implicitSolver::fill -> fillRhoLHS (solver.h:973-1075): calc_dt, gradients
+ * when viscous, FD flux Jacobian blocks (physics.h:533-578), A/dt diagonal */
+int afx_rans_implicit_fill(afx_rans* h);
+/* implicitSolver::compute (solver.h:1160-1167): factorises the
+ * preconditioner; *status = 0 on success, -1 when a diagonal block is
+ * singular (the reference's info() != Success) */
+int afx_rans_implicit_compute(afx_rans* h, int32_t* status);
+/* implicitSolver::solve (solver.h:1170-1213): RHS, linear solve,
+ * q += relaxation*dq, repeated rhs_iterations times with the frozen matrix
+ * and the same early exits; *err = final ||R||_2, or -1 when the linear
+ * solver did not converge (the reference's sentinel).  precision selects
+ * the residual form (AFX_FP64 or AFX_FP32); the update and solve are FP64. */
+int afx_rans_implicit_solve(afx_rans* h, double relaxation, double tol, int32_t rhs_iterations, int32_t precision,
+                            double* err);
+/* y = LHS x with the matrix of the last fill (AoS, 4*n_cells); for tests */
+int afx_rans_lhs_apply(afx_rans* h, const double* x, double* y);
+/* iterations of the last linear solves (sum over the last implicit_solve) */
+int afx_rans_linear_iterations(afx_rans* h, int64_t* iters);
+
 /* solver::get_uniform_residual (solver.h:636-690), with the stale-qW rule */
 int afx_rans_uniform_residual(afx_rans* h, double* err);
 /* get_wall_profile (post.h:301-387): CL, CD, CM on the named patch */

@@ -891,6 +891,39 @@ struct Solver {
         *cl = -fx * std::sin(aoa) + fy * std::cos(aoa);
         *cm = mm;
     }
+    // solver.h:979-1075 (fillRhoLHS): the 4N x 4N matrix as COO triplets in
+    // the order the reference touches them (duplicates are summed by the
+    // caller); ghost rows become identity rows
+    void fill_lhs(std::vector<int64_t>& rows, std::vector<int64_t>& cols, std::vector<double>& vals) {
+        calc_dt();
+        if (viscous_model_named_non_inviscid) {
+            set_walls_from_internal(q);
+            calc_gradients();
+        }
+        rows.clear(); cols.clear(); vals.clear();
+        const int64_t nT = m.nCells(), nR = m.nRealCells;
+        for (int64_t i = 0; i < nR; ++i)
+            for (int j = 0; j < 4; ++j) { rows.push_back(4 * i + j); cols.push_back(4 * i + j); vals.push_back(m.cellsAreas[i] / dt[i]); }
+        double J[64];
+        for (u32 e = 0; e < m.nFaces(); ++e) {
+            const int64_t k0 = 4 * (int64_t)m.edgesCells[2 * e], k1 = 4 * (int64_t)m.edgesCells[2 * e + 1];
+            face_jacobian(e, J);
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) {
+                    rows.push_back(k0 + i); cols.push_back(k0 + j); vals.push_back(J[8 * i + j]);
+                    rows.push_back(k0 + i); cols.push_back(k1 + j); vals.push_back(J[8 * i + j + 4]);
+                    if (ff[e].two_sided()) {
+                        rows.push_back(k1 + i); cols.push_back(k0 + j); vals.push_back(J[8 * (i + 4) + j]);
+                        rows.push_back(k1 + i); cols.push_back(k1 + j); vals.push_back(J[8 * (i + 4) + j + 4]);
+                    }
+                }
+        }
+        // ghost rows: identity (solver.h:1061-1071); ghost rows receive no
+        // other entries because boundary faces are one-sided
+        for (int64_t i = nR; i < nT; ++i)
+            for (int j = 0; j < 4; ++j) { rows.push_back(4 * i + j); cols.push_back(4 * i + j); vals.push_back(1.0); }
+    }
+
     // physics.h:533-578 (calc_convective_jacobian) for face e at first-order
     // states as in fillRhoLHS (solver.h:1030-1043), times the face length
     void face_jacobian(u32 e, double J[64]) {
@@ -1084,6 +1117,17 @@ void orc_solver_wall_profile(void* h, const char* patch, double* out3) {
     ((Solver*)h)->wall_profile(patch, &out3[0], &out3[1], &out3[2]);
 }
 void orc_solver_face_jacobian(void* h, int e, double* J) { ((Solver*)h)->face_jacobian((u32)e, J); }
+
+// fillRhoLHS triplets: returns the count; call with null buffers to size
+static thread_local std::vector<int64_t> g_rows, g_cols;
+static thread_local std::vector<double> g_vals;
+long orc_solver_fill_lhs(void* h, int64_t* rows, int64_t* cols, double* vals) {
+    if (!rows) { ((Solver*)h)->fill_lhs(g_rows, g_cols, g_vals); return (long)g_vals.size(); }
+    std::memcpy(rows, g_rows.data(), g_rows.size() * 8);
+    std::memcpy(cols, g_cols.data(), g_cols.size() * 8);
+    std::memcpy(vals, g_vals.data(), g_vals.size() * 8);
+    return (long)g_vals.size();
+}
 void orc_solver_wall_dist(void* h) { Solver* s = (Solver*)h; compute_wall_dist(s->m, s->bface_type); }
 
 // vector access: name in {q, qW, dt, gx, gy, limiters, q_min, q_max, RhoVector}

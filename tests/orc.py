@@ -43,6 +43,7 @@ def _load():
         "orc_solver_uniform_residual": (C.c_double, [P]),
         "orc_solver_wall_profile": (None, [P, C.c_char_p, dp]),
         "orc_solver_face_jacobian": (None, [P, C.c_int, dp]),
+        "orc_solver_fill_lhs": (C.c_long, [P, P, P, P]),
         "orc_solver_wall_dist": (None, [P]),
         "orc_solver_vec": (C.c_long, [P, C.c_char_p, dp, dp]),
         "orc_face_flux": (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -158,6 +159,17 @@ class OSolver:
         J = np.zeros(64)
         lib.orc_solver_face_jacobian(self.h, e, _d(J))
         return J.reshape(8, 8)
+
+    def fill_lhs(self):
+        """fillRhoLHS (solver.h:979-1075) as a scipy CSR matrix (4N x 4N)."""
+        import scipy.sparse as sp
+        n = lib.orc_solver_fill_lhs(self.h, None, None, None)
+        r = np.empty(n, np.int64)
+        c = np.empty(n, np.int64)
+        v = np.empty(n)
+        lib.orc_solver_fill_lhs(self.h, _v(r), _v(c), _v(v))
+        N = 4 * self.n_cells
+        return sp.coo_matrix((v, (r, c)), shape=(N, N)).tocsr()
 
     def calc_residual(self, qk):
         qk = np.ascontiguousarray(qk, np.float64)
