@@ -58,7 +58,8 @@ struct DBuf {
     hipError_t upload(const std::vector<T>& h, hipStream_t s) { return upload(h.data(), h.size(), s); }
 };
 
-constexpr int NV = 4;
+constexpr int NV = 4;       // mean-flow variables exchanged with the caller (q(4*i+k))
+constexpr int NV_MAX = 5;   // device planes: + rho nu~ of the Spalart-Allmaras model
 
 }  // namespace
 
@@ -103,6 +104,16 @@ struct afx_rans {
     DBuf<float4> q32v, R32v, t_geo4;   // FP32 fused form: AoS float4 per cell / two per tile face
     DBuf<uint32_t> t_fck;              // c0 | c1 << 12 | kind << 24 per tile face
     bool r32_aos = false;              // last FP32 residual is in R32v (fused) rather than R32
+    int last_precision = -1;           // precision of the last residual (for its rho nu~ row)
+
+    // Spalart-Allmaras: variables per cell (4, or 5 with rho nu~ in plane 4 of
+    // q / R), freestream rho nu~ = nu_ratio * mu_L, FP32 rho nu~ planes and
+    // (area, wall distance) per cell
+    int nv = 4;
+    double nu_ratio = 3.0;
+    bool wd_ready = false;
+    DBuf<float> q32n, R32n;
+    DBuf<float2> sa_ad32;
 
     // implicit step: Jacobian blocks, diagonal blocks and their inverses,
     // GMRES work space
@@ -140,11 +151,13 @@ struct afx_rans {
         return FacesT<float>{c0.p, c1.p, fnx32.p, fny32.p, flen32.p, fxf32.p, fyf32.p, fgw32.p, fkind.p, nF};
     }
     GasT<double> gas64() const {
-        return GasT<double>{gas.gamma, gas.R, gas.mu_L, 1.0 * gas.mu_L / gas.Pr_L};
+        return GasT<double>{gas.gamma, gas.R, gas.mu_L, 1.0 * gas.mu_L / gas.Pr_L, 1.0, gas.Pr_L, gas.Pr_T};
     }
     GasT<float> gas32() const {
-        return GasT<float>{(float)gas.gamma, (float)gas.R, (float)gas.mu_L, (float)(1.0 * gas.mu_L / gas.Pr_L)};
+        return GasT<float>{(float)gas.gamma, (float)gas.R, (float)gas.mu_L, (float)(1.0 * gas.mu_L / gas.Pr_L), 1.0f,
+                           (float)gas.Pr_L, (float)gas.Pr_T};
     }
+    bool sa() const { return nv == 5; }
     bool needs_gradients() const { return opts.second_order || opts.viscosity != 0; }
     TilesF32 tiles32() const {
         return TilesF32{t_cell0.p, t_loff.p, t_ng.p, t_foff.p, t_nft.p, t_goff.p, t_Lglob.p, t_slots.p,
@@ -255,6 +268,7 @@ int ensure_q32(afx_rans* h) {
     if (!h->q32_valid) {
         HIPCHK(launch_to_f32(h->stream, h->q.p, h->q32.p, (int64_t)NV * h->nT));
         if (h->q32v.p) HIPCHK(launch_to_f32x4(h->stream, h->q.p, h->nT, h->q32v.p));
+        if (h->q32n.p) HIPCHK(launch_to_f32(h->stream, h->q.p + (size_t)4 * h->nT, h->q32n.p, h->nT));
         h->q32_valid = true;
     }
     return AFX_OK;
@@ -275,12 +289,42 @@ int run_gradients32(afx_rans* h) {
     return AFX_OK;
 }
 
+// mesh::compute_wall_dist (mesh.h:794-830) on the device: distance from each
+// cell centroid to the nearest wall / slip-wall face centre, capped at 1.
+// With the SA model also fills the FP32 (area, distance) pairs.
+int compute_wall_distance(afx_rans* h) {
+    const Mesh& m = h->m;
+    std::vector<double> wx, wy;
+    for (int32_t b = 0; b < h->nB; ++b)
+        if (h->bface_type[b] == AFX_BC_WALL || h->bface_type[b] == AFX_BC_SLIP) {
+            wx.push_back(m.edgesCentersX[m.boundaryEdges[b]]);
+            wy.push_back(m.edgesCentersY[m.boundaryEdges[b]]);
+        }
+    DBuf<double> dwx, dwy;
+    HIPCHK(dwx.alloc(wx.size())); HIPCHK(dwx.upload(wx, h->stream));
+    HIPCHK(dwy.alloc(wy.size())); HIPCHK(dwy.upload(wy, h->stream));
+    HIPCHK(launch_wall_dist(h->stream, h->xc.p, h->yc.p, h->nT, dwx.p, dwy.p, (int32_t)wx.size(), h->wall_d.p));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->sa_ad32.p) {
+        std::vector<double> d(h->nT);
+        HIPCHK(hipMemcpyAsync(d.data(), h->wall_d.p, d.size() * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        std::vector<float2> ad(h->nT);
+        for (int32_t i = 0; i < h->nT; ++i) ad[i] = make_float2((float)m.cellsAreas[h->c_new2old[i]], (float)d[i]);
+        HIPCHK(h->sa_ad32.upload(ad, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    h->wd_ready = true;
+    return AFX_OK;
+}
+
 // enqueue one residual evaluation (no synchronisation)
 int enqueue_residual(afx_rans* h, int precision) {
     const bool fused = h->opts.fused != 0;
+    h->last_precision = precision;
     if (precision == AFX_FP64 && fused && h->tile64_ok) {
         HIPCHK(launch_residual_tile<double>(h->stream, h->tiles<double>(h->t_geo64.p), h->q.p, h->nT, h->nC, h->area.p,
-                                           h->lsq.p, h->xc.p, h->yc.p, h->gas64(), h->opts.gradient,
+                                           h->lsq.p, h->xc.p, h->yc.p, h->wall_d.p, h->gas64(), h->opts.gradient,
                                            h->opts.second_order, h->opts.limiter_k, h->opts.viscosity, h->R.p, h->q.p));
         return AFX_OK;
     }
@@ -288,13 +332,17 @@ int enqueue_residual(afx_rans* h, int precision) {
         int rc = ensure_q32(h);
         if (rc) return rc;
         const double k = h->opts.limiter_k;
+        SAFields32 sa{h->q32n.p, h->sa_ad32.p, h->R32n.p, h->q32n.p};
         HIPCHK(launch_residual_tile_f32(h->stream, h->tiles32(), h->q32v.p, h->nC, h->iarea32.p, h->a15_32.p,
                                         h->lsq32.p, h->xc32.p, h->yc32.p, h->gas32(), h->opts.gradient,
                                         h->opts.second_order, (float)(k * k * k), h->opts.viscosity, h->R32v.p,
-                                        h->q32v.p));
+                                        h->q32v.p, sa));
         h->r32_aos = true;
         return AFX_OK;
     }
+    if (h->sa())
+        return fail(AFX_EUNSUP, "the Spalart-Allmaras residual runs in the fused tile kernels only "
+                                "(renumber = 1, fused = 1, tiles within the LDS limit)");
     if (precision == AFX_FP64) {
         if (h->needs_gradients()) {
             int rc = run_gradients64(h, h->q.p, h->q.p);
@@ -637,8 +685,8 @@ int afx_rans_create(const afx_mesh* mh, const afx_gas* g, const afx_opts* o, afx
     afx_opts opts;
     afx_default_opts(&opts);
     if (o) opts = *o;
-    if (opts.viscosity < 0 || opts.viscosity > 1)
-        return fail(AFX_EUNSUP, "viscosity model not available in this build (0 inviscid, 1 laminar)");
+    if (opts.viscosity < 0 || opts.viscosity > 2)
+        return fail(AFX_EUNSUP, "viscosity model must be 0 (inviscid), 1 (laminar) or 2 (Spalart-Allmaras)");
     if (opts.gradient != 0 && opts.gradient != 1) return fail(AFX_EARG, "gradient must be 0 (least-squares) or 1 (green-gauss)");
     auto h = new afx_rans();
     try {
@@ -649,6 +697,7 @@ int afx_rans_create(const afx_mesh* mh, const afx_gas* g, const afx_opts* o, afx
     }
     if (g) h->gas = *g; else afx_default_gas(&h->gas);
     h->opts = opts;
+    h->nv = opts.viscosity == 2 ? 5 : 4;
     const Mesh& m = h->m;
     h->nC = (int32_t)m.nRealCells; h->nT = (int32_t)m.n_cells(); h->nF = (int32_t)m.n_faces(); h->nB = (int32_t)m.n_bfaces();
     const int32_t nC = h->nC, nT = h->nT, nF = h->nF;
@@ -669,9 +718,16 @@ int afx_rans_create(const afx_mesh* mh, const afx_gas* g, const afx_opts* o, afx
         std::vector<uint32_t> seeds;
         if (opts.renumber) seeds = rcm_order(m);
         else { seeds.resize(nC); std::iota(seeds.begin(), seeds.end(), 0u); }
-        const int32_t target = opts.tile_cells > 0 ? opts.tile_cells : 256;
+        // automatic size: the largest tile (from 256 cells down) whose FP64
+        // working set fits the 64 KiB of LDS one workgroup may use
+        int32_t target = opts.tile_cells > 0 ? opts.tile_cells : 256;
         try {
             h->tl = build_tiles(m, seeds, target);
+            while (opts.tile_cells <= 0 && target > 64 &&
+                   tile_lds_bytes<double>(h->tl.lmax, h->tl.gmax, h->tl.ftmax, h->nv) > 65536) {
+                target -= 32;
+                h->tl = build_tiles(m, seeds, target);
+            }
         } catch (std::exception& e) {
             return bail(fail(AFX_EARG, e.what()));
         }
@@ -783,8 +839,8 @@ int afx_rans_create(const afx_mesh* mh, const afx_gas* g, const afx_opts* o, afx
         HIPCHK_C(h->fkind.alloc(nF)); HIPCHK_C(h->fkind.upload(kind, s));
         HIPCHK_C(hipStreamSynchronize(s));
     }
-    // state arrays
-    const size_t nq = (size_t)NV * nT;
+    // state arrays (NV_MAX planes; the fifth holds rho nu~ with the SA model)
+    const size_t nq = (size_t)NV_MAX * nT;
     for (DBuf<double>* b : {&h->q, &h->qk, &h->qW, &h->gx, &h->gy, &h->lim, &h->R}) {
         HIPCHK_C(b->alloc(nq));
         HIPCHK_C(hipMemsetAsync(b->p, 0, nq * 8, s));
@@ -864,17 +920,22 @@ int afx_rans_create(const afx_mesh* mh, const afx_gas* g, const afx_opts* o, afx
             HIPCHK_C(h->t_fck.alloc(nft)); HIPCHK_C(h->t_fck.upload(fck, s));
             HIPCHK_C(h->q32v.alloc(nT)); HIPCHK_C(hipMemsetAsync(h->q32v.p, 0, (size_t)nT * 16, s));
             HIPCHK_C(h->R32v.alloc(nT)); HIPCHK_C(hipMemsetAsync(h->R32v.p, 0, (size_t)nT * 16, s));
+            if (h->sa()) {
+                HIPCHK_C(h->q32n.alloc(nT)); HIPCHK_C(hipMemsetAsync(h->q32n.p, 0, (size_t)nT * 4, s));
+                HIPCHK_C(h->R32n.alloc(nT)); HIPCHK_C(hipMemsetAsync(h->R32n.p, 0, (size_t)nT * 4, s));
+                HIPCHK_C(h->sa_ad32.alloc(nT)); HIPCHK_C(hipMemsetAsync(h->sa_ad32.p, 0, (size_t)nT * 8, s));
+            }
             HIPCHK_C(hipStreamSynchronize(s));
         }
         HIPCHK_C(hipStreamSynchronize(s));
-        h->tile64_ok = tile_lds_bytes<double>(tl.lmax, tl.gmax, tl.ftmax) <= 65536;
-        h->tile32_ok = tile_lds_bytes<float>(tl.lmax, tl.gmax, tl.ftmax) <= 65536;
+        h->tile64_ok = tile_lds_bytes<double>(tl.lmax, tl.gmax, tl.ftmax, h->nv) <= 65536;
+        h->tile32_ok = tile_lds_bytes<float>(tl.lmax, tl.gmax, tl.ftmax, h->nv) <= 65536;
     }
     h->nblk_red = 1024;
     HIPCHK_C(h->partial.alloc(h->nblk_red));
     HIPCHK_C(h->sum1.alloc(1));
-    HIPCHK_C(h->uni64.alloc(NV));
-    HIPCHK_C(h->uni32.alloc(NV));
+    HIPCHK_C(h->uni64.alloc(NV_MAX));
+    HIPCHK_C(h->uni32.alloc(NV_MAX));
     HIPCHK_C(hipStreamSynchronize(s));
 #undef HIPCHK_C
     *out = h;
@@ -941,6 +1002,11 @@ int afx_rans_set_bcs(afx_rans* h, int32_t n, const char* const* names, const int
     h->bface_type = bt;
     h->bface_vars = bv;
     h->bcs_set = true;
+    h->wd_ready = false;
+    if (h->sa()) {
+        int r2 = compute_wall_distance(h);
+        if (r2) return r2;
+    }
     // freestream state used by the uniform residual
     double v[4], qf[4];
     boundary_variables(h, v);
@@ -975,6 +1041,10 @@ int afx_rans_init(afx_rans* h) {
     for (int k = 0; k < NV; ++k)
         for (int32_t i = 0; i < h->nC; ++i) tmp[(size_t)k * h->nT + i] = qi[k];
     HIPCHK(hipMemcpyAsync(h->q.p, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, h->stream));
+    if (h->sa()) {   // freestream rho nu~ in the real cells
+        std::vector<double> nu(h->nC, h->nu_ratio * h->gas.mu_L);
+        HIPCHK(hipMemcpyAsync(h->q.p + (size_t)4 * h->nT, nu.data(), nu.size() * 8, hipMemcpyHostToDevice, h->stream));
+    }
     HIPCHK(hipStreamSynchronize(h->stream));
     h->q32_valid = false;
     return AFX_OK;
@@ -993,6 +1063,10 @@ int afx_rans_refill_bcs(afx_rans* h) {
     for (int k = 0; k < NV; ++k)
         HIPCHK(hipMemcpyAsync(h->q.p + (size_t)k * h->nT + h->nC, g.data() + (size_t)k * h->nB, (size_t)h->nB * 8,
                               hipMemcpyHostToDevice, h->stream));
+    std::vector<double> gn(h->sa() ? h->nB : 0, h->nu_ratio * h->gas.mu_L);
+    if (h->sa())
+        HIPCHK(hipMemcpyAsync(h->q.p + (size_t)4 * h->nT + h->nC, gn.data(), gn.size() * 8, hipMemcpyHostToDevice,
+                              h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->q32_valid = false;
     return AFX_OK;
@@ -1001,7 +1075,7 @@ int afx_rans_refill_bcs(afx_rans* h) {
 int afx_rans_bcs_from_internal(afx_rans* h) {
     int rc = check_handle(h);
     if (rc) return rc;
-    HIPCHK(launch_set_walls<double>(h->stream, h->b_owner.p, h->b_ghost.p, h->nB, h->q.p, h->nT, NV));
+    HIPCHK(launch_set_walls<double>(h->stream, h->b_owner.p, h->b_ghost.p, h->nB, h->q.p, h->nT, h->nv));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->q32_valid = false;
     return AFX_OK;
@@ -1010,7 +1084,7 @@ int afx_rans_bcs_from_internal(afx_rans* h) {
 int afx_rans_set_walls_from_internal(afx_rans* h) {
     int rc = check_handle(h);
     if (rc) return rc;
-    HIPCHK(launch_set_walls<double>(h->stream, h->w_owner.p, h->w_ghost.p, h->n_walls, h->q.p, h->nT, NV));
+    HIPCHK(launch_set_walls<double>(h->stream, h->w_owner.p, h->w_ghost.p, h->n_walls, h->q.p, h->nT, h->nv));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->q32_valid = false;
     return AFX_OK;
@@ -1076,7 +1150,7 @@ int afx_rans_residual(afx_rans* h, int32_t precision, double* R, double* norm2) 
     if ((rc = enqueue_residual(h, precision))) return rc;
     if (precision == AFX_FP64) {
         if (R && (rc = download_aos(h, h->R.p, R))) return rc;
-        if (norm2 && (rc = norm_of(h, h->R.p, h->nT, NV, norm2))) return rc;
+        if (norm2 && (rc = norm_of(h, h->R.p, h->nT, h->nv, norm2))) return rc;
     } else if (h->r32_aos) {
         if (R) {
             std::vector<float> tmp((size_t)NV * h->nT);
@@ -1085,8 +1159,14 @@ int afx_rans_residual(afx_rans* h, int32_t precision, double* R, double* norm2) 
             for (int32_t i = 0; i < h->nT; ++i)
                 for (int k = 0; k < NV; ++k) R[(size_t)NV * h->c_new2old[i] + k] = (double)tmp[(size_t)NV * i + k];
         }
-        if (norm2 && (rc = norm_of32(h, reinterpret_cast<const float*>(h->R32v.p), (int64_t)NV * h->nT, 1, norm2)))
-            return rc;
+        if (norm2) {
+            if ((rc = norm_of32(h, reinterpret_cast<const float*>(h->R32v.p), (int64_t)NV * h->nT, 1, norm2))) return rc;
+            if (h->sa()) {
+                double n5 = 0;
+                if ((rc = norm_of32(h, h->R32n.p, h->nT, 1, &n5))) return rc;
+                *norm2 = std::sqrt(*norm2 * *norm2 + n5 * n5);
+            }
+        }
     } else {
         if (R && (rc = download_aos32(h, h->R32.p, R))) return rc;
         if (norm2 && (rc = norm_of32(h, h->R32.p, h->nT, NV, norm2))) return rc;
@@ -1097,6 +1177,7 @@ int afx_rans_residual(afx_rans* h, int32_t precision, double* R, double* norm2) 
 int afx_rans_explicit_residual(afx_rans* h, const double* qk, double* qW) {
     int rc = check_handle(h);
     if (rc) return rc;
+    if (h->sa()) return fail(AFX_EUNSUP, "explicit stages are not available with the Spalart-Allmaras model");
     if (!qk) return fail(AFX_EARG, "null stage state");
     if ((rc = upload_aos(h, qk, h->qk.p))) return rc;
     if (h->needs_gradients() && (rc = run_gradients64(h, h->q.p, h->qk.p))) return rc;
@@ -1111,6 +1192,7 @@ int afx_rans_explicit_residual(afx_rans* h, const double* qk, double* qW) {
 int afx_rans_explicit_step(afx_rans* h, double relaxation, double* err) {
     int rc = check_handle(h);
     if (rc) return rc;
+    if (h->sa()) return fail(AFX_EUNSUP, "explicit stages are not available with the Spalart-Allmaras model");
     static const double alpha[3] = {0.25, 0.5, 1.};
     hipStream_t s = h->stream;
     const int64_t nq = (int64_t)NV * h->nT;
@@ -1144,6 +1226,7 @@ int afx_rans_set_linear_solver(afx_rans* h, int32_t restart, double tol, int32_t
 int afx_rans_implicit_fill(afx_rans* h) {
     int rc = check_handle(h);
     if (rc) return rc;
+    if (h->sa()) return fail(AFX_EUNSUP, "the implicit step is not available with the Spalart-Allmaras model");
     if ((rc = ensure_implicit(h))) return rc;
     hipStream_t s = h->stream;
     HIPCHK(launch_dt<double>(s, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->cfl, h->dt.p));
@@ -1228,7 +1311,7 @@ int afx_rans_uniform_residual(afx_rans* h, double* err) {
     int rc = check_handle(h);
     if (rc) return rc;
     HIPCHK(launch_face_flux<double>(h->stream, h->cells64(), h->faces64(), h->q.p, h->q.p, h->gx.p, h->gy.p, h->lim.p,
-                                    h->gas64(), h->opts.viscosity, h->opts.second_order, h->uni64.p, h->flux.p));
+                                    h->gas64(), std::min(h->opts.viscosity, 1), h->opts.second_order, h->uni64.p, h->flux.p));
     HIPCHK(launch_gather<double>(h->stream, h->cells64(), h->flux.p, h->nF, 2, h->qW.p));
     double e = 0;
     if ((rc = norm_of(h, h->qW.p, h->nT, NV, &e))) return rc;
@@ -1295,19 +1378,52 @@ int afx_rans_wall_forces(afx_rans* h, const char* patch, double* cl, double* cd,
 int afx_rans_wall_dist(afx_rans* h, double* wall_dist) {
     int rc = check_handle(h);
     if (rc) return rc;
-    const Mesh& m = h->m;
-    std::vector<double> wx, wy;
-    for (int32_t b = 0; b < h->nB; ++b)
-        if (h->bface_type[b] == AFX_BC_WALL || h->bface_type[b] == AFX_BC_SLIP) {
-            wx.push_back(m.edgesCentersX[m.boundaryEdges[b]]);
-            wy.push_back(m.edgesCentersY[m.boundaryEdges[b]]);
-        }
-    DBuf<double> dwx, dwy;
-    HIPCHK(dwx.alloc(wx.size())); HIPCHK(dwx.upload(wx, h->stream));
-    HIPCHK(dwy.alloc(wy.size())); HIPCHK(dwy.upload(wy, h->stream));
-    HIPCHK(launch_wall_dist(h->stream, h->xc.p, h->yc.p, h->nT, dwx.p, dwy.p, (int32_t)wx.size(), h->wall_d.p));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    if (!h->wd_ready && (rc = compute_wall_distance(h))) return rc;
     if (wall_dist) return download_cells(h, h->wall_d.p, wall_dist);
+    return AFX_OK;
+}
+
+int afx_rans_set_turbulence(afx_rans* h, double nu_ratio) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (!h->sa()) return fail(AFX_ESTATE, "the solver was not created with the Spalart-Allmaras model");
+    if (!(nu_ratio >= 0)) return fail(AFX_EARG, "nu_ratio must be >= 0");
+    h->nu_ratio = nu_ratio;
+    return AFX_OK;
+}
+
+int afx_rans_set_nu(afx_rans* h, const double* nu) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (!h->sa()) return fail(AFX_ESTATE, "the solver was not created with the Spalart-Allmaras model");
+    if (!nu) return fail(AFX_EARG, "null rho nu~ vector");
+    std::vector<double> tmp(h->nT);
+    for (int32_t i = 0; i < h->nT; ++i) tmp[i] = nu[h->c_new2old[i]];
+    HIPCHK(hipMemcpyAsync(h->q.p + (size_t)4 * h->nT, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->q32_valid = false;
+    return AFX_OK;
+}
+
+int afx_rans_get_nu(afx_rans* h, double* nu) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (!h->sa()) return fail(AFX_ESTATE, "the solver was not created with the Spalart-Allmaras model");
+    if (!nu) return fail(AFX_EARG, "null rho nu~ vector");
+    return download_cells(h, h->q.p + (size_t)4 * h->nT, nu);
+}
+
+int afx_rans_nu_residual(afx_rans* h, double* Rnu) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (!h->sa()) return fail(AFX_ESTATE, "the solver was not created with the Spalart-Allmaras model");
+    if (!Rnu) return fail(AFX_EARG, "null residual vector");
+    if (h->last_precision == AFX_FP64) return download_cells(h, h->R.p + (size_t)4 * h->nT, Rnu);
+    if (h->last_precision != AFX_FP32) return fail(AFX_ESTATE, "no residual has been evaluated");
+    std::vector<float> tmp(h->nT);
+    HIPCHK(hipMemcpyAsync(tmp.data(), h->R32n.p, tmp.size() * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int32_t i = 0; i < h->nT; ++i) Rnu[h->c_new2old[i]] = (double)tmp[i];
     return AFX_OK;
 }
 

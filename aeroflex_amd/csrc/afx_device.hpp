@@ -39,6 +39,43 @@ __device__ __forceinline__ T limiter_phi(T dqg, T dmax, T dmin, T K3a) {
     return (pos || neg) ? l : T(1.0);
 }
 
+// physics.h:230-257: stresses and heat flux at the face from the face-averaged
+// state and the face gradients, with viscosity mu and conductivity kc as
+// arguments (laminar: mu_L, cp mu_L/Pr_L; SA: mu_L + mu_t, cp (mu_L/Pr_L + mu_t/Pr_T))
+template <typename T>
+__device__ __forceinline__ void viscous_terms(const GasT<T>& g, T nx, T ny, const T* q_L, const T* q_R, const T* gx,
+                                              const T* gy, T mu, T kc, T& f1, T& f2, T& f3) {
+    const T gm1 = g.gamma - T(1);
+    T qc[4];
+    for (int k = 0; k < 4; ++k) qc[k] = T(0.5) * (q_L[k] + q_R[k]);
+    const T r2 = qc[0] * qc[0];
+    // physics.h:33-68 temperature gradient through the pressure gradient
+    const T p = gm1 * (qc[3] - ddiv<T>(T(0.5), qc[0]) * (qc[1] * qc[1] + qc[2] * qc[2]));
+    T gp0 = T(2.) * gx[3];
+    gp0 -= ddiv<T>(gx[1] * qc[1], qc[0]) + ddiv<T>(qc[1] * (gx[1] * qc[0] - gx[0] * qc[1]), r2);
+    gp0 -= ddiv<T>(gx[2] * qc[2], qc[0]) + ddiv<T>(qc[2] * (gx[2] * qc[0] - gx[0] * qc[2]), r2);
+    gp0 *= T(0.5) * gm1;
+    T gp1 = T(2.) * gy[3];
+    gp1 -= ddiv<T>(gy[1] * qc[1], qc[0]) + ddiv<T>(qc[1] * (gy[1] * qc[0] - gy[0] * qc[1]), r2);
+    gp1 -= ddiv<T>(gy[2] * qc[2], qc[0]) + ddiv<T>(qc[2] * (gy[2] * qc[0] - gy[0] * qc[2]), r2);
+    gp1 *= T(0.5) * gm1;
+    const T gT0 = ddiv<T>(T(1.), g.R) * ddiv<T>(gp0 * qc[0] - gx[0] * p, r2);
+    const T gT1 = ddiv<T>(T(1.), g.R) * ddiv<T>(gp1 * qc[0] - gy[0] * p, r2);
+    const T gu0 = ddiv<T>(qc[0] * gx[1] - qc[1] * gx[0], r2);
+    const T gu1 = ddiv<T>(qc[0] * gy[1] - qc[1] * gy[0], r2);
+    const T gv0 = ddiv<T>(qc[0] * gx[2] - qc[2] * gx[0], r2);
+    const T gv1 = ddiv<T>(qc[0] * gy[2] - qc[2] * gy[0], r2);
+    const T div_v = gu0 + gv1;
+    const T tau_xx = T(2.) * mu * (gu0 - ddiv<T>(div_v, T(3.)));
+    const T tau_yy = T(2.) * mu * (gv1 - ddiv<T>(div_v, T(3.)));
+    const T tau_xy = mu * (gu1 + gv0);
+    const T phi0 = ddiv<T>(qc[1], qc[0]) * tau_xx + ddiv<T>(qc[2], qc[0]) * tau_xy + kc * gT0;
+    const T phi1 = ddiv<T>(qc[1], qc[0]) * tau_xy + ddiv<T>(qc[2], qc[0]) * tau_yy + kc * gT1;
+    f1 -= nx * tau_xx + ny * tau_xy;
+    f2 -= nx * tau_xy + ny * tau_yy;
+    f3 -= nx * phi0 + ny * phi1;
+}
+
 // physics.h:160-228 (Roe + Harten) and 230-258 (laminar), internal_flux::operator()
 template <typename T>
 __device__ __forceinline__ void roe_flux(const GasT<T>& g, T nx, T ny, const T* q_L, const T* q_R, bool laminar,
@@ -83,36 +120,7 @@ __device__ __forceinline__ void roe_flux(const GasT<T>& g, T nx, T ny, const T* 
     f1 -= T(0.5) * (kF1 * (u - c * nx) + kF234_0 * u + kF234_1 * (uR - uL - dV * nx) + kF5 * (u + c * nx));
     f2 -= T(0.5) * (kF1 * (v - c * ny) + kF234_0 * v + kF234_1 * (vR - vL - dV * ny) + kF5 * (v + c * ny));
     f3 -= T(0.5) * (kF1 * (h - c * V) + kF234_0 * q2 * T(0.5) + kF234_1 * (u * (uR - uL) + v * (vR - vL) - V * dV) + kF5 * (h + c * V));
-    if (laminar) {
-        T qc[4];
-        for (int k = 0; k < 4; ++k) qc[k] = T(0.5) * (q_L[k] + q_R[k]);
-        const T r2 = qc[0] * qc[0];
-        // physics.h:33-68 temperature gradient through the pressure gradient
-        const T p = gm1 * (qc[3] - ddiv<T>(T(0.5), qc[0]) * (qc[1] * qc[1] + qc[2] * qc[2]));
-        T gp0 = T(2.) * gx[3];
-        gp0 -= ddiv<T>(gx[1] * qc[1], qc[0]) + ddiv<T>(qc[1] * (gx[1] * qc[0] - gx[0] * qc[1]), r2);
-        gp0 -= ddiv<T>(gx[2] * qc[2], qc[0]) + ddiv<T>(qc[2] * (gx[2] * qc[0] - gx[0] * qc[2]), r2);
-        gp0 *= T(0.5) * gm1;
-        T gp1 = T(2.) * gy[3];
-        gp1 -= ddiv<T>(gy[1] * qc[1], qc[0]) + ddiv<T>(qc[1] * (gy[1] * qc[0] - gy[0] * qc[1]), r2);
-        gp1 -= ddiv<T>(gy[2] * qc[2], qc[0]) + ddiv<T>(qc[2] * (gy[2] * qc[0] - gy[0] * qc[2]), r2);
-        gp1 *= T(0.5) * gm1;
-        const T gT0 = ddiv<T>(T(1.), g.R) * ddiv<T>(gp0 * qc[0] - gx[0] * p, r2);
-        const T gT1 = ddiv<T>(T(1.), g.R) * ddiv<T>(gp1 * qc[0] - gy[0] * p, r2);
-        const T gu0 = ddiv<T>(qc[0] * gx[1] - qc[1] * gx[0], r2);
-        const T gu1 = ddiv<T>(qc[0] * gy[1] - qc[1] * gy[0], r2);
-        const T gv0 = ddiv<T>(qc[0] * gx[2] - qc[2] * gx[0], r2);
-        const T gv1 = ddiv<T>(qc[0] * gy[2] - qc[2] * gy[0], r2);
-        const T div_v = gu0 + gv1;
-        const T tau_xx = T(2.) * g.mu * (gu0 - ddiv<T>(div_v, T(3.)));
-        const T tau_yy = T(2.) * g.mu * (gv1 - ddiv<T>(div_v, T(3.)));
-        const T tau_xy = g.mu * (gu1 + gv0);
-        const T phi0 = ddiv<T>(qc[1], qc[0]) * tau_xx + ddiv<T>(qc[2], qc[0]) * tau_xy + g.k * gT0;
-        const T phi1 = ddiv<T>(qc[1], qc[0]) * tau_xy + ddiv<T>(qc[2], qc[0]) * tau_yy + g.k * gT1;
-        f1 -= nx * tau_xx + ny * tau_xy;
-        f2 -= nx * tau_xy + ny * tau_yy;
-        f3 -= nx * phi0 + ny * phi1;
-    }
+    if (laminar) viscous_terms<T>(g, nx, ny, q_L, q_R, gx, gy, g.mu, g.k, f1, f2, f3);
     f[0] = f0; f[1] = f1; f[2] = f2; f[3] = f3;
 }
 
@@ -184,6 +192,119 @@ __device__ __forceinline__ void face_flux(const GasT<T>& g, int kind, T nx, T ny
         face_vars<T>(g, kind, nx, ny, q_L, q_R, qb);
         roe_flux<T>(g, nx, ny, q_L, qb, false, gx, gy, f);
     }
+}
+
+// ------------------------------------------------------ Spalart-Allmaras --
+// The reference's SA branch is empty (physics.h:259-261).  Model: SA-noft2 of
+// Spalart & Allmaras (1994) with the S-tilde limiter and negative-nu~ branch
+// of SA-neg (Allmaras, Johnson & Spalart 2012), rho nu~ as fifth conservative
+// variable.  Every expression keeps the operation order of the FP64 oracle so
+// the plain form agrees with it to rounding (cbrt aside, bit for bit).
+namespace sa {
+constexpr double cb1 = 0.1355, cb2 = 0.622, sigma = 2.0 / 3.0, kappa = 0.41, cv1 = 7.1, cw2 = 0.3, cw3 = 2.0;
+constexpr double cv2 = 0.7, cv3 = 0.9, ct3 = 1.2, cn1 = 16.0;
+constexpr double cw1 = cb1 / (kappa * kappa) + (1.0 + cb2) / sigma;
+constexpr double cv1_3 = cv1 * cv1 * cv1;
+constexpr double cw3_6 = (cw3 * cw3) * (cw3 * cw3) * (cw3 * cw3);
+}  // namespace sa
+
+template <typename T> __device__ __forceinline__ T dcbrt(T x);
+template <> __device__ __forceinline__ double dcbrt<double>(double x) { return cbrt(x); }
+template <> __device__ __forceinline__ float dcbrt<float>(float x) { return cbrtf(x); }
+
+// f_v1 (zero for chi <= 0: no eddy viscosity from negative nu~)
+template <typename T>
+__device__ __forceinline__ T sa_fv1(T chi) {
+    const T c3 = chi * chi * chi;
+    return chi <= T(0) ? T(0) : ddiv<T>(c3, c3 + T(sa::cv1_3));
+}
+// f_n of the diffusion coefficient (1 for chi >= 0)
+template <typename T>
+__device__ __forceinline__ T sa_fn(T chi) {
+    const T c3 = chi * chi * chi;
+    return chi >= T(0) ? T(1) : ddiv<T>(T(sa::cn1) + c3, T(sa::cn1) - c3);
+}
+template <typename T>
+__device__ __forceinline__ T sa_fw(T r) {
+    const T r2 = r * r;
+    const T gg = r + T(sa::cw2) * (r2 * r2 * r2 - r);
+    const T g2 = gg * gg;
+    return gg * dcbrt<T>(dsqrt<T>(ddiv<T>(T(1) + T(sa::cw3_6), g2 * g2 * g2 + T(sa::cw3_6))));
+}
+// per-volume source of rho nu~: rho (P - D) + c_b2/sigma rho |grad nu~|^2
+template <typename T>
+__device__ __forceinline__ T sa_source(T rho, T nut, T mu, T omega, T d, T gvx, T gvy) {
+    const T cross = T(sa::cb2 / sa::sigma) * (gvx * gvx + gvy * gvy);
+    const T nd = ddiv<T>(nut, d);
+    if (nut < T(0)) return rho * (T(sa::cb1 * (1.0 - sa::ct3)) * omega * nut + T(sa::cw1) * nd * nd + cross);
+    const T chi = ddiv<T>(rho * nut, mu);
+    const T f1 = sa_fv1<T>(chi);
+    const T f2 = T(1) - ddiv<T>(chi, T(1) + chi * f1);
+    const T kd2 = T(sa::kappa * sa::kappa) * d * d;
+    const T sbar = ddiv<T>(nut * f2, kd2);
+    const T st = (sbar >= -T(sa::cv2) * omega)
+                     ? omega + sbar
+                     : omega + ddiv<T>(omega * (T(sa::cv2 * sa::cv2) * omega + T(sa::cv3) * sbar),
+                                       T(sa::cv3 - 2.0 * sa::cv2) * omega - sbar);
+    T r = T(10);
+    if (st > T(0)) {
+        r = ddiv<T>(nut, st * kd2);
+        r = r < T(10) ? r : T(10);
+    }
+    const T P = T(sa::cb1) * st * nut;
+    const T D = T(sa::cw1) * sa_fw<T>(r) * nd * nd;
+    return rho * (P - D + cross);
+}
+// rho nu~ on the far side of a boundary face: no-slip wall -rho nu~ (nu~ = 0
+// at the wall), slip wall the interior value, farfield the ghost nu~ on inflow
+// and the interior nu~ on outflow, times the boundary density
+template <typename T>
+__device__ __forceinline__ T sa_bnd_nu(int kind, T nx, T ny, const T* qL, T nL, const T* qR, T nR, const T* qb) {
+    if (kind == K_WALL) return -nL;
+    if (kind == K_SLIP) return nL;
+    const T io = qL[1] * nx + qL[2] * ny;
+    return qb[0] * (io < T(0) ? ddiv<T>(nR, qR[0]) : ddiv<T>(nL, qL[0]));
+}
+// five-variable face flux: Roe on the mean flow, upwind convection of rho nu~
+// by the sign of the mass flux; internal faces add the viscous terms with
+// mu_L + mu_t and the nu~ diffusion.  gx/gy: face gradients of the 5 variables.
+template <typename T>
+__device__ __forceinline__ void sa_face_flux(const GasT<T>& g, int kind, T nx, T ny, const T* qL, const T* qR,
+                                             const T* gx, const T* gy, T* f) {
+    if (kind != K_INTERNAL) {
+        T qb[4];
+        face_vars<T>(g, kind, nx, ny, qL, qR, qb);
+        const T nb = sa_bnd_nu<T>(kind, nx, ny, qL, qL[4], qR, qR[4], qb);
+        roe_flux<T>(g, nx, ny, qL, qb, false, gx, gy, f);
+        f[4] = f[0] * (f[0] >= T(0) ? ddiv<T>(qL[4], qL[0]) : ddiv<T>(nb, qb[0]));
+        return;
+    }
+    roe_flux<T>(g, nx, ny, qL, qR, false, gx, gy, f);
+    f[4] = f[0] * (f[0] >= T(0) ? ddiv<T>(qL[4], qL[0]) : ddiv<T>(qR[4], qR[0]));
+    const T nc = T(0.5) * (qL[4] + qR[4]);
+    const T chi = ddiv<T>(nc, g.mu);
+    const T mut = nc * sa_fv1<T>(chi);
+    const T mu = g.mu + mut;
+    const T kc = g.cp * (ddiv<T>(g.mu, g.PrL) + ddiv<T>(mut, g.PrT));
+    T f1 = f[1], f2 = f[2], f3 = f[3];
+    viscous_terms<T>(g, nx, ny, qL, qR, gx, gy, mu, kc, f1, f2, f3);
+    f[1] = f1; f[2] = f2; f[3] = f3;
+    const T qc0 = T(0.5) * (qL[0] + qR[0]);
+    const T nut = ddiv<T>(nc, qc0);
+    const T gvx = ddiv<T>(gx[4] - nut * gx[0], qc0), gvy = ddiv<T>(gy[4] - nut * gy[0], qc0);
+    const T dcoef = ddiv<T>(g.mu + nc * sa_fn<T>(chi), T(sa::sigma));
+    f[4] -= dcoef * (nx * gvx + ny * gvy);
+}
+// source of cell with state q (5), unlimited gradients gx/gy (5) and wall
+// distance d: vorticity from the momentum gradients, grad nu~ from grad(rho nu~)
+template <typename T>
+__device__ __forceinline__ T sa_cell_source(const GasT<T>& g, const T* q, const T* gx, const T* gy, T d) {
+    const T rho = q[0], nut = ddiv<T>(q[4], rho);
+    const T dudy = ddiv<T>(gy[1] - ddiv<T>(q[1], rho) * gy[0], rho);
+    const T dvdx = ddiv<T>(gx[2] - ddiv<T>(q[2], rho) * gx[0], rho);
+    const T omega = dabs<T>(dvdx - dudy);
+    const T gvx = ddiv<T>(gx[4] - nut * gx[0], rho), gvy = ddiv<T>(gy[4] - nut * gy[0], rho);
+    return sa_source<T>(rho, nut, g.mu, omega, d, gvx, gvy);
 }
 
 // ---------------------------------------------------------------- kernels --
@@ -435,12 +556,14 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
     return (b & 7) * per + (b >> 3);
 }
 
-template <typename T>
+template <typename T, int NV>
 __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* __restrict__ q, int32_t nT, int32_t nC,
                                                           const T* __restrict__ area, const T* __restrict__ lsq,
                                                           const T* __restrict__ xc, const T* __restrict__ yc,
-                                                          GasT<T> g, int gg, int second_order, T limiter_k,
-                                                          int viscous, T* __restrict__ R, T* __restrict__ qghost) {
+                                                          const T* __restrict__ wd, GasT<T> g, int gg,
+                                                          int second_order, T limiter_k, int viscous,
+                                                          T* __restrict__ R, T* __restrict__ qghost) {
+    constexpr bool SA = (NV == 5);
     extern __shared__ unsigned char smem_raw[];
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= S.ntiles) return;
@@ -451,27 +574,28 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
     const int go = S.goff[t];
     const int LM = S.lmax, GM = S.gmax, FM = S.ftmax;
     const int64_t NF = S.nFtot;
-    T* sq = reinterpret_cast<T*>(smem_raw);   // [4][LM] states of L
-    T* sgx = sq + 4 * LM;                      // [4][GM]
-    T* sgy = sgx + 4 * GM;
-    T* slm = sgy + 4 * GM;
-    T* sfl = slm + 4 * GM;                     // [4][FM] face fluxes x length
+    T* sq = reinterpret_cast<T*>(smem_raw);   // [NV][LM] states of L
+    T* sgx = sq + NV * LM;                     // [NV][GM]
+    T* sgy = sgx + NV * GM;
+    T* slm = sgy + NV * GM;
+    T* sfl = slm + NV * GM;                    // [NV][FM] face fluxes x length
     const int* Lg = S.Lglob + lo;
 
     // 1. stage the states of the local cells
     for (int i = threadIdx.x; i < nL; i += blockDim.x) {
         const int gi = Lg[i];
         if (gi < 0) continue;   // wall/slip ghost: never read
-        for (int k = 0; k < 4; ++k) sq[k * LM + i] = q[(size_t)k * nT + gi];
+        for (int k = 0; k < NV; ++k) sq[k * LM + i] = q[(size_t)k * nT + gi];
     }
     __syncthreads();
 
     // 2. gradients (solver.h:425-514) and limiters (solver.h:517-593) of T u H1
     for (int c = threadIdx.x; c < nG; c += blockDim.x) {
         const uint64_t sw = S.slots[go + c];
-        T qc[4];
-        for (int k = 0; k < 4; ++k) qc[k] = sq[k * LM + c];
-        T sx[4] = {T(0), T(0), T(0), T(0)}, sy[4] = {T(0), T(0), T(0), T(0)};
+        T qc[NV];
+        for (int k = 0; k < NV; ++k) qc[k] = sq[k * LM + c];
+        T sx[NV], sy[NV];
+        for (int k = 0; k < NV; ++k) { sx[k] = T(0); sy[k] = T(0); }
         const int gc = Lg[c];
         if (gg) {
             for (int s = 0; s < 4; ++s) {
@@ -481,20 +605,29 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
                 const bool nb_side = (e >> 15) != 0;
                 const uint32_t fc = S.fcell[f];
                 const int o = (int)(fc & 0xFFFFu), n = (int)(fc >> 16);
-                T qo[4], qn[4], qv[4];
+                T qo[4], qn[4], qv[NV];
                 for (int k = 0; k < 4; ++k) { qo[k] = sq[k * LM + o]; qn[k] = sq[k * LM + n]; }
                 const T nx = S.geo[f], ny = S.geo[NF + f], L = S.geo[2 * NF + f], gf = S.geo[3 * NF + f];
-                face_vars<T>(g, S.fkind[f], nx, ny, qo, qn, qv);
-                for (int k = 0; k < 4; ++k) {
-                    const T fk = (qo[k] * (T(1.0) - gf) + qv[k] * gf) * L;
+                const int kind = S.fkind[f];
+                face_vars<T>(g, kind, nx, ny, qo, qn, qv);
+                T vo[NV];
+                for (int k = 0; k < 4; ++k) vo[k] = qo[k];
+                if constexpr (SA) {
+                    const T no = sq[4 * LM + o], nn = sq[4 * LM + n];
+                    vo[4] = no;
+                    qv[4] = (kind == K_INTERNAL) ? (no + nn) * T(0.5) : sa_bnd_nu<T>(kind, nx, ny, qo, no, qn, nn, qv);
+                }
+                for (int k = 0; k < NV; ++k) {
+                    const T fk = (vo[k] * (T(1.0) - gf) + qv[k] * gf) * L;
                     if (!nb_side) { sx[k] += fk * nx; sy[k] += fk * ny; }
                     else { sx[k] -= fk * nx; sy[k] -= fk * ny; }
                 }
             }
             const T A = area[gc];
-            for (int k = 0; k < 4; ++k) { sx[k] = ddiv<T>(sx[k], A); sy[k] = ddiv<T>(sy[k], A); }
+            for (int k = 0; k < NV; ++k) { sx[k] = ddiv<T>(sx[k], A); sy[k] = ddiv<T>(sy[k], A); }
         } else {
-            T rx[4] = {T(0), T(0), T(0), T(0)}, ry[4] = {T(0), T(0), T(0), T(0)};
+            T rx[NV], ry[NV];
+            for (int k = 0; k < NV; ++k) { rx[k] = T(0); ry[k] = T(0); }
             for (int s = 0; s < 4; ++s) {
                 const uint32_t e = (uint32_t)(sw >> (16 * s)) & 0xFFFFu;
                 if (e == 0xFFFFu) break;
@@ -507,25 +640,31 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
                 const T dox = nb_side ? S.geo[4 * NF + f] : S.geo[6 * NF + f];
                 const T doy = nb_side ? S.geo[5 * NF + f] : S.geo[7 * NF + f];
                 const T dx = dcx - dox, dy = dcy - doy;   // x_other - x_cell
-                T qn[4], qv[4];
+                const int kind = S.fkind[f];
+                T qn[4], qv[NV];
                 for (int k = 0; k < 4; ++k) qn[k] = sq[k * LM + other];
-                face_vars<T>(g, S.fkind[f], S.geo[f], S.geo[NF + f], qc, qn, qv);
-                for (int k = 0; k < 4; ++k) {
+                face_vars<T>(g, kind, S.geo[f], S.geo[NF + f], qc, qn, qv);
+                if constexpr (SA) {
+                    const T nn = sq[4 * LM + other];
+                    qv[4] = (kind == K_INTERNAL) ? (qc[4] + nn) * T(0.5)
+                                                 : sa_bnd_nu<T>(kind, S.geo[f], S.geo[NF + f], qc, qc[4], qn, nn, qv);
+                }
+                for (int k = 0; k < NV; ++k) {
                     const T del = qc[k] - qv[k];
                     rx[k] += dx * del;
                     ry[k] += dy * del;
                 }
             }
             const T m0 = lsq[gc], m1 = lsq[(size_t)nC + gc], m2 = lsq[2 * (size_t)nC + gc], m3 = lsq[3 * (size_t)nC + gc];
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < NV; ++k) {
                 sx[k] = m0 * rx[k] + m1 * ry[k];
                 sy[k] = m2 * rx[k] + m3 * ry[k];
             }
         }
-        for (int k = 0; k < 4; ++k) { sgx[k * GM + c] = sx[k]; sgy[k * GM + c] = sy[k]; }
+        for (int k = 0; k < NV; ++k) { sgx[k * GM + c] = sx[k]; sgy[k * GM + c] = sy[k]; }
         if (second_order) {
-            T qmin[4], qmax[4];
-            for (int k = 0; k < 4; ++k) { qmin[k] = qc[k]; qmax[k] = qc[k]; }
+            T qmin[NV], qmax[NV];
+            for (int k = 0; k < NV; ++k) { qmin[k] = qc[k]; qmax[k] = qc[k]; }
             for (int s = 0; s < 4; ++s) {
                 const uint32_t e = (uint32_t)(sw >> (16 * s)) & 0xFFFFu;
                 if (e == 0xFFFFu) break;
@@ -535,7 +674,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
                 if (kind == K_WALL || kind == K_SLIP) continue;
                 const uint32_t fc = S.fcell[f];
                 const int other = (e >> 15) ? (int)(fc & 0xFFFFu) : (int)(fc >> 16);
-                for (int k = 0; k < 4; ++k) {
+                for (int k = 0; k < NV; ++k) {
                     const T v = sq[k * LM + other];
                     qmin[k] = (v < qmin[k]) ? v : qmin[k];
                     qmax[k] = (qmax[k] < v) ? v : qmax[k];
@@ -543,7 +682,8 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
             }
             const T Ka = limiter_k * dsqrt<T>(area[gc]);
             const T K3a = Ka * Ka * Ka;
-            T lm[4] = {T(1), T(1), T(1), T(1)};
+            T lm[NV];
+            for (int k = 0; k < NV; ++k) lm[k] = T(1);
             for (int s = 0; s < 4; ++s) {
                 const uint32_t e = (uint32_t)(sw >> (16 * s)) & 0xFFFFu;
                 if (e == 0xFFFFu) break;
@@ -551,7 +691,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
                 const bool nb_side = (e >> 15) != 0;
                 const T dx = nb_side ? S.geo[6 * NF + f] : S.geo[4 * NF + f];
                 const T dy = nb_side ? S.geo[7 * NF + f] : S.geo[5 * NF + f];
-                for (int k = 0; k < 4; ++k) {
+                for (int k = 0; k < NV; ++k) {
                     const T dqg = sx[k] * dx + sy[k] * dy;
                     const T dmax = qmax[k] - qc[k];
                     const T dmin = qmin[k] - qc[k];
@@ -559,7 +699,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
                     lm[k] = (l < lm[k]) ? l : lm[k];
                 }
             }
-            for (int k = 0; k < 4; ++k) slm[k * GM + c] = lm[k];
+            for (int k = 0; k < NV; ++k) slm[k * GM + c] = lm[k];
         }
     }
     __syncthreads();
@@ -571,28 +711,29 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
         const int o = (int)(fc & 0xFFFFu), n = (int)(fc >> 16);
         const int kind = S.fkind[f];
         const T nx = S.geo[f], ny = S.geo[NF + f], L = S.geo[2 * NF + f];
-        T qL[4], qR[4];
-        for (int k = 0; k < 4; ++k) { qL[k] = sq[k * LM + o]; qR[k] = sq[k * LM + n]; }
+        T qL[NV], qR[NV];
+        for (int k = 0; k < NV; ++k) { qL[k] = sq[k * LM + o]; qR[k] = sq[k * LM + n]; }
         if (second_order) {
             const T d0x = S.geo[4 * NF + f], d0y = S.geo[5 * NF + f];
-            for (int k = 0; k < 4; ++k)
+            for (int k = 0; k < NV; ++k)
                 qL[k] = qL[k] + (sgx[k * GM + o] * d0x + sgy[k * GM + o] * d0y) * slm[k * GM + o];
             if (n < nG) {
                 const T d1x = S.geo[6 * NF + f], d1y = S.geo[7 * NF + f];
-                for (int k = 0; k < 4; ++k)
+                for (int k = 0; k < NV; ++k)
                     qR[k] = qR[k] + (sgx[k * GM + n] * d1x + sgy[k * GM + n] * d1y) * slm[k * GM + n];
             }
         }
-        const bool laminar = (viscous == 1) && (kind == K_INTERNAL);
-        T gxa[4] = {T(0), T(0), T(0), T(0)}, gya[4] = {T(0), T(0), T(0), T(0)};
-        if (laminar) {
+        const bool visc = (viscous >= 1) && (kind == K_INTERNAL);
+        T gxa[NV], gya[NV];
+        for (int k = 0; k < NV; ++k) { gxa[k] = T(0); gya[k] = T(0); }
+        if (visc) {
             // solver.h:359-398 (direction from the SUM of the two centroids)
             const int go_ = Lg[o], gn_ = Lg[n];
             T t0 = xc[go_] + xc[gn_];
             T t1 = yc[go_] + yc[gn_];
             const T l = dsqrt<T>(t0 * t0 + t1 * t1);
             t0 = ddiv<T>(t0, l); t1 = ddiv<T>(t1, l);
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < NV; ++k) {
                 const T dir = ddiv<T>(sq[k * LM + o] - sq[k * LM + n], l);
                 const T gxb = (sgx[k * GM + o] + sgx[k * GM + n]) * T(0.5);
                 const T gyb = (sgy[k * GM + o] + sgy[k * GM + n]) * T(0.5);
@@ -601,31 +742,38 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
                 gya[k] = gyb - (dot - dir) * t1;
             }
         }
-        T F[4];
-        face_flux<T>(g, kind, nx, ny, qL, qR, laminar, gxa, gya, F);
-        for (int k = 0; k < 4; ++k) sfl[k * FM + fl] = F[k] * L;
+        T F[NV];
+        if constexpr (SA) sa_face_flux<T>(g, kind, nx, ny, qL, qR, gxa, gya, F);
+        else face_flux<T>(g, kind, nx, ny, qL, qR, visc, gxa, gya, F);
+        for (int k = 0; k < NV; ++k) sfl[k * FM + fl] = F[k] * L;
     }
     __syncthreads();
 
     // 4. signed sums in reference face order; one coalesced write per cell
     for (int c = threadIdx.x; c < nTt; c += blockDim.x) {
         const uint64_t sw = S.slots[go + c];
-        Acc<T> acc[4];
+        Acc<T> acc[NV];
         for (int s = 0; s < 4; ++s) {
             const uint32_t e = (uint32_t)(sw >> (16 * s)) & 0xFFFFu;
             if (e == 0xFFFFu) break;
             const int fl = (int)(e & 0x7FFFu);
-            if (e >> 15) for (int k = 0; k < 4; ++k) acc[k].add(sfl[k * FM + fl]);
-            else for (int k = 0; k < 4; ++k) acc[k].sub(sfl[k * FM + fl]);
+            if (e >> 15) for (int k = 0; k < NV; ++k) acc[k].add(sfl[k * FM + fl]);
+            else for (int k = 0; k < NV; ++k) acc[k].sub(sfl[k * FM + fl]);
             if (qghost) {
                 const int kind = S.fkind[fo + fl];
                 if (kind == K_WALL || kind == K_SLIP) {
                     const int gh = -1 - Lg[(int)(S.fcell[fo + fl] >> 16)];
-                    for (int k = 0; k < 4; ++k) qghost[(size_t)k * nT + gh] = sq[k * LM + c];
+                    for (int k = 0; k < NV; ++k) qghost[(size_t)k * nT + gh] = sq[k * LM + c];
                 }
             }
         }
         for (int k = 0; k < 4; ++k) R[(size_t)k * nT + c0g + c] = acc[k].value();
+        if constexpr (SA) {
+            T qs[NV], gxs[NV], gys[NV];
+            for (int k = 0; k < NV; ++k) { qs[k] = sq[k * LM + c]; gxs[k] = sgx[k * GM + c]; gys[k] = sgy[k * GM + c]; }
+            const int gc = c0g + c;
+            R[(size_t)4 * nT + gc] = acc[4].value() + area[gc] * sa_cell_source<T>(g, qs, gxs, gys, wd[gc]);
+        }
     }
 }
 
@@ -691,6 +839,32 @@ __device__ __forceinline__ void roe_flux_f32(float gamma, float nx, float ny, co
     f[0] = f0; f[1] = f1; f[2] = f2; f[3] = f3;
 }
 
+// five-variable face flux of sa_face_flux with the FP32 Roe kernel above
+__device__ __forceinline__ void sa_face_flux_f32(const GasT<float>& g, int kind, float nx, float ny, const float* qL,
+                                                 const float* qR, const float* gx, const float* gy, float* f) {
+    if (kind != K_INTERNAL) {
+        float qb[4];
+        face_vars<float>(g, kind, nx, ny, qL, qR, qb);
+        const float nb = sa_bnd_nu<float>(kind, nx, ny, qL, qL[4], qR, qR[4], qb);
+        roe_flux_f32(g.gamma, nx, ny, qL, qb, f);
+        f[4] = f[0] * (f[0] >= 0.f ? qL[4] * frcp(qL[0]) : nb * frcp(qb[0]));
+        return;
+    }
+    roe_flux_f32(g.gamma, nx, ny, qL, qR, f);
+    f[4] = f[0] * (f[0] >= 0.f ? qL[4] * frcp(qL[0]) : qR[4] * frcp(qR[0]));
+    const float nc = 0.5f * (qL[4] + qR[4]);
+    const float chi = nc * frcp(g.mu);
+    const float mut = nc * sa_fv1<float>(chi);
+    const float kc = g.cp * (g.mu * frcp(g.PrL) + mut * frcp(g.PrT));
+    float f1 = f[1], f2 = f[2], f3 = f[3];
+    viscous_terms<float>(g, nx, ny, qL, qR, gx, gy, g.mu + mut, kc, f1, f2, f3);
+    f[1] = f1; f[2] = f2; f[3] = f3;
+    const float iqc0 = frcp(0.5f * (qL[0] + qR[0]));
+    const float nut = nc * iqc0;
+    const float gvx = (gx[4] - nut * gx[0]) * iqc0, gvy = (gy[4] - nut * gy[0]) * iqc0;
+    f[4] -= (g.mu + nc * sa_fn<float>(chi)) * float(1.0 / sa::sigma) * (nx * gvx + ny * gvy);
+}
+
 #ifdef AFX_FP32_UNIT   // defined in afx_k32.hip only
 __device__ __forceinline__ float4 f4(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
 __device__ __forceinline__ float get(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
@@ -700,14 +874,16 @@ __device__ __forceinline__ uint32_t slot_of(const uint2& w, int s) {
 }
 #define F4_LOOP(expr) { expr(x); expr(y); expr(z); expr(w); }
 
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) k_residual_tile_f32(TilesF32 S, const float4* __restrict__ q, int32_t nC,
+template <bool SA>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA ? 5 : 6))) k_residual_tile_f32(TilesF32 S, const float4* __restrict__ q, int32_t nC,
                                                               const float* __restrict__ iarea,
                                                               const float* __restrict__ a15,
                                                               const float* __restrict__ lsq,
                                                               const float* __restrict__ xc,
                                                               const float* __restrict__ yc, GasT<float> g, int gg,
                                                               int second_order, float K3, int viscous,
-                                                              float4* __restrict__ R, float4* __restrict__ qghost) {
+                                                              float4* __restrict__ R, float4* __restrict__ qghost,
+                                                              SAFields32 sa) {
     extern __shared__ float4 smem4[];
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= S.ntiles) return;
@@ -721,6 +897,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))
     float4* sgy = sgx + S.gmax;
     float4* slm = sgy + S.gmax;
     float4* sfl = slm + S.gmax;      // [FM] face fluxes x length
+    // SA: rho nu~ planes after the float4 arrays
+    float2* sgn = reinterpret_cast<float2*>(sfl + S.ftmax);   // [GM] gradient of rho nu~
+    float* sqn = reinterpret_cast<float*>(sgn + S.gmax);      // [LM]
+    float* slmn = sqn + S.lmax;                                 // [GM]
+    float* sfln = slmn + S.gmax;                                // [FM]
     const int* Lg = S.Lglob + lo;
     const uint32_t* fck = S.fck + fo;
     const float4* geo = S.geo + 2 * (int64_t)fo;
@@ -728,7 +909,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))
     // 1. stage states
     for (int i = threadIdx.x; i < nL; i += blockDim.x) {
         const int gi = Lg[i];
-        if (gi >= 0) sq[i] = q[gi];   // wall/slip ghosts (flagged < 0) are never read
+        if (gi >= 0) {   // wall/slip ghosts (flagged < 0) are never read
+            sq[i] = q[gi];
+            if constexpr (SA) sqn[i] = sa.qn[gi];
+        }
     }
     __syncthreads();
 
@@ -740,6 +924,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))
         const float4 qc = sq[c];
         float4 qmin = qc, qmax = qc;
         float4 sx = f4(0.f, 0.f, 0.f, 0.f), sy = f4(0.f, 0.f, 0.f, 0.f);
+        const float nuc = SA ? sqn[c] : 0.f;
+        float sxn = 0.f, syn = 0.f, mnn = nuc, mxn = nuc;
         float dxs[4], dys[4];
         int ns = 0;
         if (gg) {
@@ -766,14 +952,22 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))
                     qmin.m = fminf(qmin.m, qn.m); qmax.m = fmaxf(qmax.m, qn.m); }
                     F4_LOOP(GG_INT)
 #undef GG_INT
+                    if constexpr (SA) {
+                        const float nun = sqn[other];
+                        const float fv = a * nuc + b * nun;
+                        sxn += fv * nlx; syn += fv * nly;
+                        mnn = fminf(mnn, nun); mxn = fmaxf(mxn, nun);
+                    }
                 } else {
                     float qb[4] = {0.f, 0.f, 0.f, 0.f}, qv[4];
                     const float qa[4] = {qc.x, qc.y, qc.z, qc.w};
+                    float nug = 0.f;
                     if (kind == K_FARFIELD) {
                         const float4 t4 = sq[other];
                         qb[0] = t4.x; qb[1] = t4.y; qb[2] = t4.z; qb[3] = t4.w;
                         qmin = f4(fminf(qmin.x, t4.x), fminf(qmin.y, t4.y), fminf(qmin.z, t4.z), fminf(qmin.w, t4.w));
                         qmax = f4(fmaxf(qmax.x, t4.x), fmaxf(qmax.y, t4.y), fmaxf(qmax.z, t4.z), fmaxf(qmax.w, t4.w));
+                        if constexpr (SA) { nug = sqn[other]; mnn = fminf(mnn, nug); mxn = fmaxf(mxn, nug); }
                     }
                     face_vars<float>(g, kind, ga.x, ga.y, qa, qb, qv);
                     const float w1 = 1.f - ga.w;
@@ -781,11 +975,17 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))
                     const float fv2 = qa[2] * w1 + qv[2] * ga.w, fv3 = qa[3] * w1 + qv[3] * ga.w;
                     sx.x += fv0 * nlx; sx.y += fv1 * nlx; sx.z += fv2 * nlx; sx.w += fv3 * nlx;
                     sy.x += fv0 * nly; sy.y += fv1 * nly; sy.z += fv2 * nly; sy.w += fv3 * nly;
+                    if constexpr (SA) {
+                        const float vr = sa_bnd_nu<float>(kind, ga.x, ga.y, qa, nuc, qb, nug, qv);
+                        const float fv = nuc * w1 + vr * ga.w;
+                        sxn += fv * nlx; syn += fv * nly;
+                    }
                 }
             }
             const float ia = iarea[gc];
             sx = f4(sx.x * ia, sx.y * ia, sx.z * ia, sx.w * ia);
             sy = f4(sy.x * ia, sy.y * ia, sy.z * ia, sy.w * ia);
+            sxn *= ia; syn *= ia;
         } else {
             float rx[4] = {0.f, 0.f, 0.f, 0.f}, ry[4] = {0.f, 0.f, 0.f, 0.f};
             const float qa[4] = {qc.x, qc.y, qc.z, qc.w};
@@ -815,15 +1015,30 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))
                     ry[k] += dy * del;
                     if (!walls) { mn[k] = fminf(mn[k], qb[k]); mx[k] = fmaxf(mx[k], qb[k]); }
                 }
+                if constexpr (SA) {
+                    const float nuo = walls ? 0.f : sqn[other];
+                    const float vr = (kind == K_INTERNAL) ? (nuc + nuo) * 0.5f
+                                                          : sa_bnd_nu<float>(kind, ga.x, ga.y, qa, nuc, qb, nuo, qv);
+                    const float del = nuc - vr;
+                    sxn += dx * del;
+                    syn += dy * del;
+                    if (!walls) { mnn = fminf(mnn, nuo); mxn = fmaxf(mxn, nuo); }
+                }
             }
             const float m0 = lsq[gc], m1 = lsq[(size_t)nC + gc], m2 = lsq[2 * (size_t)nC + gc], m3 = lsq[3 * (size_t)nC + gc];
             sx = f4(m0 * rx[0] + m1 * ry[0], m0 * rx[1] + m1 * ry[1], m0 * rx[2] + m1 * ry[2], m0 * rx[3] + m1 * ry[3]);
             sy = f4(m2 * rx[0] + m3 * ry[0], m2 * rx[1] + m3 * ry[1], m2 * rx[2] + m3 * ry[2], m2 * rx[3] + m3 * ry[3]);
             qmin = f4(mn[0], mn[1], mn[2], mn[3]);
             qmax = f4(mx[0], mx[1], mx[2], mx[3]);
+            if constexpr (SA) {
+                const float rxn = sxn, ryn = syn;
+                sxn = m0 * rxn + m1 * ryn;
+                syn = m2 * rxn + m3 * ryn;
+            }
         }
         sgx[c] = sx;
         sgy[c] = sy;
+        if constexpr (SA) sgn[c] = make_float2(sxn, syn);
         if (second_order) {
             const float K3a = K3 * a15[gc];
             float4 lm = f4(1.f, 1.f, 1.f, 1.f);
@@ -845,6 +1060,21 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))
 #undef LIM
             }
             slm[c] = lm;
+            if constexpr (SA) {
+                const float dM = mxn - nuc, dm = mnn - nuc;
+                const float ddM = dM * dM + K3a, ddm = dm * dm + K3a;
+                float l = 1.f;
+                for (int s = 0; s < ns; ++s) {
+                    const float dqg = sxn * dxs[s] + syn * dys[s];
+                    const bool pos = dqg > 1e-16f, neg = dqg < -1e-16f;
+                    const float D = pos ? dM : dm, DD = pos ? ddM : ddm;
+                    const float tq = dqg * D;
+                    float phi = (DD + 2.f * tq) * frcp(DD + tq + 2.f * dqg * dqg);
+                    phi = (pos || neg) ? phi : 1.f;
+                    l = fminf(l, phi);
+                }
+                slmn[c] = l;
+            }
         }
     }
     __syncthreads();
@@ -869,7 +1099,43 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))
 #undef RECR
             }
         }
-        float F[4];
+        float F[5];
+        if constexpr (SA) {
+            const bool staged = (kind == K_INTERNAL || kind == K_FARFIELD);
+            float nL = sqn[o], nR = staged ? sqn[n] : 0.f;
+            if (second_order) {
+                const float2 ga_ = sgn[o];
+                nL += (ga_.x * gb.x + ga_.y * gb.y) * slmn[o];
+                if (n < nG) { const float2 gn2 = sgn[n]; nR += (gn2.x * gb.z + gn2.y * gb.w) * slmn[n]; }
+            }
+            const float qL5[5] = {qL.x, qL.y, qL.z, qL.w, nL};
+            const float qR5[5] = {qR.x, qR.y, qR.z, qR.w, nR};
+            float gxa[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, gya[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+            if (kind == K_INTERNAL) {
+                const int go_ = Lg[o], gn_ = Lg[n];
+                float t0 = xc[go_] + xc[gn_], t1 = yc[go_] + yc[gn_];
+                const float il = frcp(__builtin_amdgcn_sqrtf(t0 * t0 + t1 * t1));
+                t0 *= il; t1 *= il;
+                const float4 so = sq[o], sn = sq[n], xo = sgx[o], xn = sgx[n], yo = sgy[o], yn = sgy[n];
+                const float2 go2 = sgn[o], gn2 = sgn[n];
+                const float so5[5] = {so.x, so.y, so.z, so.w, sqn[o]}, sn5[5] = {sn.x, sn.y, sn.z, sn.w, sqn[n]};
+                const float xo5[5] = {xo.x, xo.y, xo.z, xo.w, go2.x}, xn5[5] = {xn.x, xn.y, xn.z, xn.w, gn2.x};
+                const float yo5[5] = {yo.x, yo.y, yo.z, yo.w, go2.y}, yn5[5] = {yn.x, yn.y, yn.z, yn.w, gn2.y};
+#pragma unroll
+                for (int k = 0; k < 5; ++k) {
+                    const float dir = (so5[k] - sn5[k]) * il;
+                    const float gxb = 0.5f * (xo5[k] + xn5[k]);
+                    const float gyb = 0.5f * (yo5[k] + yn5[k]);
+                    const float dot = gxb * t0 + gyb * t1;
+                    gxa[k] = gxb - (dot - dir) * t0;
+                    gya[k] = gyb - (dot - dir) * t1;
+                }
+            }
+            sa_face_flux_f32(g, kind, ga.x, ga.y, qL5, qR5, gxa, gya, F);
+            sfl[fl] = f4(F[0] * ga.z, F[1] * ga.z, F[2] * ga.z, F[3] * ga.z);
+            sfln[fl] = F[4] * ga.z;
+            continue;
+        }
         const float qLa[4] = {qL.x, qL.y, qL.z, qL.w};
         float qRa[4] = {qR.x, qR.y, qR.z, qR.w};
         if (kind == K_INTERNAL) {
@@ -904,7 +1170,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))
     // 4. signed sums in reference face order (exact FP64 accumulation of the FP32 fluxes)
     for (int c = threadIdx.x; c < nTt; c += blockDim.x) {
         const uint2 sw = slots2[c];
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const uint32_t e = slot_of(sw, s);
@@ -916,24 +1182,43 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))
             a1 += (double)(nb ? v.y : -v.y);
             a2 += (double)(nb ? v.z : -v.z);
             a3 += (double)(nb ? v.w : -v.w);
+            if constexpr (SA) { const float vn = sfln[fl]; a4 += (double)(nb ? vn : -vn); }
             if (qghost) {
                 const uint32_t fc = fck[fl];
                 const int kind = (int)(fc >> 24);
-                if (kind == K_WALL || kind == K_SLIP) qghost[-1 - Lg[(int)((fc >> 12) & 0xFFFu)]] = sq[c];
+                if (kind == K_WALL || kind == K_SLIP) {
+                    const int gh = -1 - Lg[(int)((fc >> 12) & 0xFFFu)];
+                    qghost[gh] = sq[c];
+                    if constexpr (SA) sa.qghost_n[gh] = sqn[c];
+                }
             }
         }
         R[c0g + c] = f4((float)a0, (float)a1, (float)a2, (float)a3);
+        if constexpr (SA) {
+            const float4 qs = sq[c], xs = sgx[c], ys = sgy[c];
+            const float2 gs = sgn[c];
+            const float q5[5] = {qs.x, qs.y, qs.z, qs.w, sqn[c]};
+            const float gx5[5] = {xs.x, xs.y, xs.z, xs.w, gs.x}, gy5[5] = {ys.x, ys.y, ys.z, ys.w, gs.y};
+            const float2 ad = sa.sa_ad[c0g + c];
+            sa.Rn[c0g + c] = (float)(a4 + (double)(ad.x * sa_cell_source<float>(g, q5, gx5, gy5, ad.y)));
+        }
     }
 }
 
 hipError_t launch_residual_tile_f32(hipStream_t s, TilesF32 S, const float4* q, int32_t nC, const float* iarea,
                                     const float* a15, const float* lsq, const float* xc, const float* yc,
                                     GasT<float> g, int gg, int second_order, float K3, int viscous, float4* R,
-                                    float4* qghost) {
+                                    float4* qghost, SAFields32 sa) {
     const int nb = ((S.ntiles + 7) / 8) * 8;
-    const size_t lds = tile_lds_bytes<float>(S.lmax, S.gmax, S.ftmax);
-    hipLaunchKernelGGL(k_residual_tile_f32, dim3(nb), dim3(kBlock), lds, s, S, q, nC, iarea, a15, lsq, xc, yc, g, gg,
-                       second_order, K3, viscous, R, qghost);
+    if (viscous == 2) {
+        const size_t lds = tile_lds_bytes<float>(S.lmax, S.gmax, S.ftmax, 5);
+        hipLaunchKernelGGL(k_residual_tile_f32<true>, dim3(nb), dim3(kBlock), lds, s, S, q, nC, iarea, a15, lsq, xc,
+                           yc, g, gg, second_order, K3, viscous, R, qghost, sa);
+    } else {
+        const size_t lds = tile_lds_bytes<float>(S.lmax, S.gmax, S.ftmax, 4);
+        hipLaunchKernelGGL(k_residual_tile_f32<false>, dim3(nb), dim3(kBlock), lds, s, S, q, nC, iarea, a15, lsq, xc,
+                           yc, g, gg, second_order, K3, viscous, R, qghost, sa);
+    }
     return hipGetLastError();
 }
 #endif  // AFX_FP32_UNIT
@@ -971,12 +1256,18 @@ inline int nblocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
     }                                                                                                             \
     template <>                                                                                                   \
     hipError_t launch_residual_tile<T>(hipStream_t s, TilesT<T> S, const T* q, int32_t nT, int32_t nC,             \
-                                       const T* area, const T* lsq, const T* xc, const T* yc, GasT<T> g, int gg,   \
-                                       int so, T lk, int viscous, T* R, T* qghost) {                               \
+                                       const T* area, const T* lsq, const T* xc, const T* yc, const T* wd,         \
+                                       GasT<T> g, int gg, int so, T lk, int viscous, T* R, T* qghost) {            \
         const int nb = ((S.ntiles + 7) / 8) * 8;                                                                  \
-        const size_t lds = tile_lds_bytes<T>(S.lmax, S.gmax, S.ftmax);                                            \
-        hipLaunchKernelGGL(k_residual_tile<T>, dim3(nb), dim3(kBlock), lds, s, S, q, nT, nC, area, lsq, xc, yc, g, \
-                           gg, so, lk, viscous, R, qghost);                                                       \
+        if (viscous == 2) {                                                                                       \
+            const size_t lds = tile_lds_bytes<T>(S.lmax, S.gmax, S.ftmax, 5);                                     \
+            hipLaunchKernelGGL((k_residual_tile<T, 5>), dim3(nb), dim3(kBlock), lds, s, S, q, nT, nC, area, lsq,  \
+                               xc, yc, wd, g, gg, so, lk, viscous, R, qghost);                                    \
+        } else {                                                                                                  \
+            const size_t lds = tile_lds_bytes<T>(S.lmax, S.gmax, S.ftmax, 4);                                     \
+            hipLaunchKernelGGL((k_residual_tile<T, 4>), dim3(nb), dim3(kBlock), lds, s, S, q, nT, nC, area, lsq,  \
+                               xc, yc, wd, g, gg, so, lk, viscous, R, qghost);                                    \
+        }                                                                                                         \
         return hipGetLastError();                                                                                 \
     }                                                                                                             \
     template <>                                                                                                   \

@@ -21,6 +21,7 @@ enum FaceKind : int8_t { K_INTERNAL = 0, K_FARFIELD = 1, K_SLIP = 2, K_WALL = 3 
 template <typename T>
 struct GasT {
     T gamma, R, mu, k;   // k = cp*mu_L/Pr_L with cp = 1 (core.h:32,43-45)
+    T cp, PrL, PrT;      // turbulent conductivity cp (mu_L/Pr_L + mu_t/Pr_T) of the SA model
 };
 
 template <typename T>
@@ -67,18 +68,21 @@ struct TilesT {
     int32_t lmax, gmax, ftmax;
 };
 
+// nv = 4 (Euler / laminar) or 5 (Spalart-Allmaras: rho nu~ as fifth variable)
 template <typename T>
-inline size_t tile_lds_bytes(int32_t lmax, int32_t gmax, int32_t ftmax) {
-    return sizeof(T) * (size_t)(4 * lmax + 12 * gmax + 4 * ftmax);
+inline size_t tile_lds_bytes(int32_t lmax, int32_t gmax, int32_t ftmax, int nv = 4) {
+    return sizeof(T) * (size_t)nv * (size_t)(lmax + 3 * gmax + ftmax);
 }
 
 // fused residual of every tile: gradients, limiters, fluxes and cell sums in
 // LDS; R written once per cell; wall/slip ghost states of the tile's cells
-// written to qghost (set_walls_from_internal) when non-null
+// written to qghost (set_walls_from_internal) when non-null.  viscous: 0
+// inviscid, 1 laminar, 2 Spalart-Allmaras (nv = 5 planes of q and R; wd = wall
+// distance per device cell)
 template <typename T>
 hipError_t launch_residual_tile(hipStream_t s, TilesT<T> S, const T* q, int32_t nT, int32_t nC, const T* area,
-                                const T* lsq, const T* xc, const T* yc, GasT<T> g, int gg, int second_order,
-                                T limiter_k, int viscous, T* R, T* qghost);
+                                const T* lsq, const T* xc, const T* yc, const T* wd, GasT<T> g, int gg,
+                                int second_order, T limiter_k, int viscous, T* R, T* qghost);
 
 // FP32 throughput form of the same (afx_k32.hip): cell states, gradients,
 // limiters, fluxes and residuals are float4 per cell / face (rho, rho u,
@@ -98,11 +102,19 @@ struct TilesF32 {
     int32_t ntiles;
     int32_t lmax, gmax, ftmax;
 };
-// iarea = 1/A, a15 = A^1.5, K3 = limiter_k^3 (K3a of solver.h:551-552)
+// iarea = 1/A, a15 = A^1.5, K3 = limiter_k^3 (K3a of solver.h:551-552).
+// Spalart-Allmaras (viscous == 2): rho nu~ in qn / Rn / qghost_n (one float
+// per cell) and (area, wall distance) per cell in sa_ad.
+struct SAFields32 {
+    const float* qn;
+    const float2* sa_ad;
+    float* Rn;
+    float* qghost_n;
+};
 hipError_t launch_residual_tile_f32(hipStream_t s, TilesF32 S, const float4* q, int32_t nC, const float* iarea,
                                     const float* a15, const float* lsq, const float* xc, const float* yc,
                                     GasT<float> g, int gg, int second_order, float K3, int viscous, float4* R,
-                                    float4* qghost);
+                                    float4* qghost, SAFields32 sa);
 hipError_t launch_to_f32x4(hipStream_t s, const double* src, int32_t nT, float4* dst);
 
 // ---- launchers (instantiated for double in afx_k64.hip, float in afx_k32.hip)

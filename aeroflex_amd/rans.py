@@ -98,6 +98,10 @@ def _load():
         "afx_rans_launch_stage": (C.c_int, [P, i32, i32, i32]),
         "afx_rans_synchronize": (C.c_int, [P]),
         "afx_rans_destroy": (C.c_int, [P]),
+        "afx_rans_set_turbulence": (C.c_int, [P, f64]),
+        "afx_rans_set_nu": (C.c_int, [P, dp]),
+        "afx_rans_get_nu": (C.c_int, [P, dp]),
+        "afx_rans_nu_residual": (C.c_int, [P, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -243,7 +247,9 @@ class Mesh:
             self.h = None
 
 
-VISCOSITY = {"inviscid": 0, "laminar": 1}
+# "spalart-allmaras" is the name solver::set_bcs tests (solver.h:206);
+# "spallart-allmaras" the one rans::Settings offers (core.h:176,224)
+VISCOSITY = {"inviscid": 0, "laminar": 1, "spalart-allmaras": 2, "spallart-allmaras": 2}
 GRADIENT = {"least-squares": 0, "green-gauss": 1}
 
 
@@ -398,6 +404,27 @@ class Solver:
         d = np.empty(self.n_cells)
         _check(lib.afx_rans_wall_dist(self.h, _dptr(d)))
         return d
+
+    # -- Spalart-Allmaras: rho nu~ per cell (the reference's nu, solver.h:55)
+    def set_turbulence(self, nu_ratio=3.0):
+        _check(lib.afx_rans_set_turbulence(self.h, nu_ratio))
+
+    def set_nu(self, nu):
+        nu = np.ascontiguousarray(nu, np.float64)
+        if nu.shape != (self.n_cells,):
+            raise RansError("rho nu~ vector has the wrong length")
+        _check(lib.afx_rans_set_nu(self.h, _dptr(nu)))
+
+    def get_nu(self):
+        nu = np.empty(self.n_cells)
+        _check(lib.afx_rans_get_nu(self.h, _dptr(nu)))
+        return nu
+
+    def nu_residual(self):
+        """rho nu~ row of the last fillRhoRHS."""
+        r = np.empty(self.n_cells)
+        _check(lib.afx_rans_nu_residual(self.h, _dptr(r)))
+        return r
 
     def stream(self):
         s = C.c_void_p()

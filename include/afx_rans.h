@@ -32,7 +32,8 @@ extern "C" {
 #define AFX_EUNSUP (-5)      /* unsupported option combination            */
 
 /* boundary-condition types (solver.h:216-230 strings) */
-#define AFX_BC_OTHER 0       /* any other name: interior Roe flux with the ghost */
+#define AFX_BC_OTHER 0       /* any other name in the reference (interior flux with the
+                                ghost); rejected by afx_rans_set_bcs */
 #define AFX_BC_FARFIELD 1    /* "farfield"  -> farfield_flux  (physics.h:409-530) */
 #define AFX_BC_SLIP 2        /* "slip-wall" -> slip_wall_flux (physics.h:279-339) */
 #define AFX_BC_WALL 3        /* "wall"      -> wall_flux      (physics.h:344-405) */
@@ -57,13 +58,16 @@ typedef struct afx_gas {
  * multigrid.h:72-76 (solver construction + set_second_order /
  * set_gradient_scheme / set_limiter_k) */
 typedef struct afx_opts {
-    int32_t viscosity;     /* 0 "inviscid", 1 "laminar", 2 Spalart-Allmaras (this
-                              library's SA-noft2; the reference's SA branch is empty) */
+    int32_t viscosity;     /* 0 "inviscid", 1 "laminar", 2 "spalart-allmaras": SA-noft2
+                              with rho nu~ as fifth variable (the reference's SA
+                              branch is empty, physics.h:259-261); residual only,
+                              fused tile kernels */
     int32_t second_order;  /* Settings::second_order (default 1) */
     int32_t gradient;      /* 0 "least-squares", 1 "green-gauss" (default) */
     double limiter_k;      /* Settings::limiter_k (default 5) */
     int32_t renumber;      /* 1: tiled RCM cell order on the device (default), 0: file order */
-    int32_t tile_cells;    /* cells per tile of the fused residual kernel (0: default 256) */
+    int32_t tile_cells;    /* cells per tile of the fused residual kernel (0: automatic, the
+                              largest of 256, 224, ... whose FP64 tile fits 64 KiB of LDS) */
     int32_t fused;         /* 1: fused tile kernel for the residual (default), 0: three-kernel path */
     int32_t reserved[5];
 } afx_opts;
@@ -190,6 +194,18 @@ int afx_rans_wall_forces(afx_rans* h, const char* patch, double* cl, double* cd,
 int afx_rans_wall_dist(afx_rans* h, double* wall_dist);
 /* counts[5] as afx_mesh_counts */
 int afx_rans_counts(afx_rans* h, int64_t counts[5]);
+
+/* ---------------------------------------------------- Spalart-Allmaras --
+ * Replace the reference's unused SA members nu / nuW (solver.h:55,59,187-188):
+ * rho nu~ per cell (n_cells, reference order, ghosts included).  Valid only
+ * for a solver created with viscosity 2 (AFX_ESTATE otherwise).
+ * set_turbulence: freestream rho nu~ = nu_ratio * mu_L used by init and
+ * refill_bcs (default 3).  nu_residual: the rho nu~ row of the last
+ * afx_rans_residual (its norm2 covers all five rows). */
+int afx_rans_set_turbulence(afx_rans* h, double nu_ratio);
+int afx_rans_set_nu(afx_rans* h, const double* nu);
+int afx_rans_get_nu(afx_rans* h, double* nu);
+int afx_rans_nu_residual(afx_rans* h, double* Rnu);
 /* The handle's hipStream_t (as void*), for event timing by the caller. */
 int afx_rans_stream(afx_rans* h, void** stream);
 /* Enqueue n back-to-back launches of one residual stage without

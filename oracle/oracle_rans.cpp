@@ -956,6 +956,294 @@ struct Solver {
     }
 };
 
+
+// ===================================================================== SA ==
+// Spalart-Allmaras one-equation model.  The reference's SA branch is empty
+// (physics.h:259-261; nu/nuW allocated but unused, solver.h:55,59,187-188), so
+// there is nothing to restate: this is the published model,
+//   SA-noft2  (Spalart & Allmaras 1994; NASA Turbulence Modeling Resource),
+//   S-tilde limiter and negative-nu~ branch of SA-neg (Allmaras, Johnson &
+//   Spalart 2012, ICCFD7-1902),
+// in compressible conservative form with rho*nu~ as fifth variable:
+//   d(rho nu~)/dt + div(rho u nu~) = rho (P - D)
+//        + (1/sigma) [div((mu + rho nu~ f_n) grad nu~) + c_b2 rho |grad nu~|^2]
+// discretised on the reference's machinery:
+//   * gradients of rho nu~ by the same Green-Gauss / least-squares operator
+//     as q (solver.h:425-514), limiter of solver.h:517-593, MUSCL reconstruction
+//     of solver.h:1097-1120;
+//   * face gradient of rho nu~ by average_gradients (solver.h:359-398, A1);
+//   * convection: rho nu~ upwinded by the sign of the Roe mass flux;
+//   * mean-flow viscous terms of physics.h:230-257 with mu_L + mu_t,
+//     mu_t = rho nu~ f_v1, k = cp (mu_L/Pr_L + mu_t/Pr_T);
+//   * boundary faces carry no gradient terms, like the reference's boundary
+//     functors (A6): wall ghost rho nu~ = -rho nu~ (nu~ = 0 on the wall face),
+//     slip wall = interior, farfield = ghost nu~ on inflow, interior on outflow;
+//   * sources with the cell's unlimited gradients and wall distance
+//     (mesh.h:794-830).
+// At rho nu~ = 0 everywhere the mean-flow rows equal the laminar residual.
+namespace sa {
+const double cb1 = 0.1355, cb2 = 0.622, sigma = 2.0 / 3.0, kappa = 0.41, cv1 = 7.1, cw2 = 0.3, cw3 = 2.0;
+const double cv2 = 0.7, cv3 = 0.9, ct3 = 1.2, cn1 = 16.0;
+inline double cw1() { return cb1 / (kappa * kappa) + (1.0 + cb2) / sigma; }
+// f_v1 (chi >= 0); SA-neg: mu_t = 0 for chi < 0
+inline double fv1(double chi) {
+    if (chi <= 0.0) return 0.0;
+    const double c3 = chi * chi * chi;
+    return c3 / (c3 + cv1 * cv1 * cv1);
+}
+// diffusion coefficient multiplier f_n (1 for chi >= 0)
+inline double fn(double chi) {
+    if (chi >= 0.0) return 1.0;
+    const double c3 = chi * chi * chi;
+    return (cn1 + c3) / (cn1 - c3);
+}
+// f_w(r) with g = r + c_w2 (r^6 - r)
+inline double fw(double r) {
+    const double r2 = r * r, g = r + cw2 * (r2 * r2 * r2 - r);
+    const double g2 = g * g, c2 = cw3 * cw3;
+    const double c6 = c2 * c2 * c2;
+    return g * std::cbrt(std::sqrt((1.0 + c6) / (g2 * g2 * g2 + c6)));
+}
+// per-volume source of rho nu~: rho (P - D) + c_b2/sigma rho |grad nu~|^2
+inline double source(double rho, double nut, double mu, double omega, double d, double gnx, double gny) {
+    const double cross = cb2 / sigma * (gnx * gnx + gny * gny);
+    if (nut < 0.0)   // SA-neg
+        return rho * (cb1 * (1.0 - ct3) * omega * nut + cw1() * (nut / d) * (nut / d) + cross);
+    const double chi = rho * nut / mu;
+    const double f1 = fv1(chi);
+    const double f2 = 1.0 - chi / (1.0 + chi * f1);
+    const double kd2 = kappa * kappa * d * d;
+    const double sbar = nut * f2 / kd2;
+    const double st = (sbar >= -cv2 * omega)
+                          ? omega + sbar
+                          : omega + omega * (cv2 * cv2 * omega + cv3 * sbar) / ((cv3 - 2.0 * cv2) * omega - sbar);
+    double r = 10.0;
+    if (st > 0.0) { r = nut / (st * kd2); r = r < 10.0 ? r : 10.0; }
+    const double P = cb1 * st * nut;
+    const double D = cw1() * fw(r) * (nut / d) * (nut / d);
+    return rho * (P - D + cross);
+}
+}  // namespace sa
+
+enum { SA_CONV = 1, SA_DIFF = 2, SA_SOURCE = 4, SA_MUT = 8, SA_ALL = 15 };
+
+struct SASolver {
+    Solver* s;
+    std::vector<double> nu, gnx, gny, lnu, TurbVector;   // rho nu~ per cell (the reference's nu)
+    int terms = SA_ALL;                                   // for tests: which terms enter
+    explicit SASolver(Solver* s_) : s(s_) {
+        const size_t N = s->m.nCells();
+        nu.assign(N, 0.0); gnx.assign(N, 0.0); gny.assign(N, 0.0); lnu.assign(N, 1.0); TurbVector.assign(N, 0.0);
+    }
+    // rho nu~ on the far side of face e from the owner state (qL, nL) and the
+    // ghost/neighbour state (qR, nR): gradient form (internal: average, A4)
+    double vars_nu(u32 e, const double* qL, double nL, const double* qR, double nR) const {
+        const FaceFlux& F = s->ff[e];
+        if (F.kind == BC_INTERNAL) return (nL + nR) * 0.5;
+        if (F.kind == BC_WALL) return -nL;
+        if (F.kind == BC_SLIP) return nL;
+        double qb[4];
+        face_vars(F, qL, qR, qb);
+        const double io = qL[1] * F.nx + qL[2] * F.ny;   // as physics.h:480
+        return qb[0] * (io < 0 ? nR / qR[0] : nL / qL[0]);
+    }
+    void gradients() {
+        const Solver& S = *s;
+        const Mesh& m = S.m;
+        const u32 nR = m.nRealCells, nT = m.nCells();
+        std::fill(gnx.begin(), gnx.end(), 0.0);
+        std::fill(gny.begin(), gny.end(), 0.0);
+        if (S.gg) {   // solver.h:429-465 applied to rho nu~
+            for (u32 e = 0; e < m.nFaces(); ++e) {
+                const u32 i = m.edgesCells[2 * e], j = m.edgesCells[2 * e + 1];
+                const double dxif = m.edgesCentersX[e] - m.cellsCentersX[i];
+                const double dyif = m.edgesCentersY[e] - m.cellsCentersY[i];
+                const double dif = std::sqrt(dxif * dxif + dyif * dyif);
+                const double dxij = m.cellsCentersX[i] - m.cellsCentersX[j];
+                const double dyij = m.cellsCentersY[i] - m.cellsCentersY[j];
+                const double gf = dif / std::sqrt(dxij * dxij + dyij * dyij);
+                const double vr = vars_nu(e, &S.q[4 * i], nu[i], &S.q[4 * j], nu[j]);
+                const double fk = (nu[i] * (1.0 - gf) + vr * gf) * m.edgesLengths[e];
+                gnx[i] += fk * m.edgesNormalsX[e]; gny[i] += fk * m.edgesNormalsY[e];
+                gnx[j] -= fk * m.edgesNormalsX[e]; gny[j] -= fk * m.edgesNormalsY[e];
+            }
+            for (u32 i = 0; i < nR; ++i) { gnx[i] /= m.cellsAreas[i]; gny[i] /= m.cellsAreas[i]; }
+        } else {      // solver.h:467-505 applied to rho nu~
+            for (u32 i = 0; i < nR; ++i) {
+                const u32 n = m.cellsIsTriangle[i] ? 3 : 4;
+                double rx = 0, ry = 0;
+                for (u32 jj = 0; jj < n; ++jj) {
+                    const u32 e = m.cellsEdges[4 * i + jj];
+                    const u32 cn = m.edgesCells[2 * e] == i ? m.edgesCells[2 * e + 1] : m.edgesCells[2 * e];
+                    const double dx = m.cellsCentersX[cn] - m.cellsCentersX[i];
+                    const double dy = m.cellsCentersY[cn] - m.cellsCentersY[i];
+                    const double del = nu[i] - vars_nu(e, &S.q[4 * i], nu[i], &S.q[4 * cn], nu[cn]);
+                    rx += dx * del; ry += dy * del;
+                }
+                gnx[i] = S.lsq[4 * i + 0] * rx + S.lsq[4 * i + 1] * ry;
+                gny[i] = S.lsq[4 * i + 2] * rx + S.lsq[4 * i + 3] * ry;
+            }
+        }
+        for (u32 i = nR; i < nT; ++i) { gnx[i] = 0; gny[i] = 0; }
+    }
+    void limiters() {   // solver.h:517-593 applied to rho nu~
+        const Solver& S = *s;
+        const Mesh& m = S.m;
+        const u32 nR = m.nRealCells;
+        std::fill(lnu.begin(), lnu.end(), 1.0);
+        std::vector<double> mn(nu), mx(nu);
+        for (u32 e = 0; e < m.nFaces(); ++e) {
+            const u32 i = m.edgesCells[2 * e], j = m.edgesCells[2 * e + 1];
+            mn[i] = std::min(mn[i], nu[j]); mn[j] = std::min(mn[j], nu[i]);
+            mx[i] = std::max(mx[i], nu[j]); mx[j] = std::max(mx[j], nu[i]);
+        }
+        for (u32 e = 0; e < m.nFaces(); ++e) {
+            const u32 ids[2] = {m.edgesCells[2 * e], m.edgesCells[2 * e + 1]};
+            for (u32 id : ids) {
+                if (id >= nR) continue;
+                const double dx = m.edgesCentersX[e] - m.cellsCentersX[id];
+                const double dy = m.edgesCentersY[e] - m.cellsCentersY[id];
+                const double Ka = S.limiter_k * std::sqrt(m.cellsAreas[id]);
+                const double K3a = Ka * Ka * Ka;
+                const double dqg = gnx[id] * dx + gny[id] * dy;
+                const double dmax = mx[id] - nu[id], dmin = mn[id] - nu[id];
+                double lim = 1.0;
+                if (dqg > 1e-16)
+                    lim = 1 / dqg * ((dmax * dmax + K3a) * dqg + 2 * dqg * dqg * dmax) / (dmax * dmax + 2 * dqg * dqg + dmax * dqg + K3a);
+                else if (dqg < -1e-16)
+                    lim = 1 / dqg * ((dmin * dmin + K3a) * dqg + 2 * dqg * dqg * dmin) / (dmin * dmin + 2 * dqg * dqg + dmin * dqg + K3a);
+                lnu[id] = std::min(lnu[id], lim);
+            }
+        }
+    }
+    // flux of face e (times nothing: caller multiplies by the length)
+    void face_flux5(u32 e, const double* qL, double nL, const double* qR, double nR, double f[5]) const {
+        const Solver& S = *s;
+        const FaceFlux& F = S.ff[e];
+        const Gas& g = S.g;
+        const u32 c0 = S.m.edgesCells[2 * e], c1 = S.m.edgesCells[2 * e + 1];
+        static const double zero[4] = {0, 0, 0, 0};
+        FaceFlux inv = F;
+        inv.kind = BC_INTERNAL;
+        inv.viscous = 0;
+        if (F.kind != BC_INTERNAL) {   // boundary functors: Roe on (qL, vars(qL, q_ghost)), no gradients
+            double qb[4];
+            face_vars(F, qL, qR, qb);
+            const double nb = vars_nu(e, qL, nL, qR, nR);
+            roe_flux(inv, qL, qb, zero, zero, f);
+            f[4] = (terms & SA_CONV) ? f[0] * (f[0] >= 0 ? nL / qL[0] : nb / qb[0]) : 0.0;
+            return;
+        }
+        roe_flux(inv, qL, qR, zero, zero, f);
+        f[4] = (terms & SA_CONV) ? f[0] * (f[0] >= 0 ? nL / qL[0] : nR / qR[0]) : 0.0;
+        double gxa[4], gya[4];
+        S.average_gradients(gxa, gya, c0, c1);
+        double qc[4];
+        for (int k = 0; k < 4; ++k) qc[k] = 0.5 * (qL[k] + qR[k]);
+        const double nc = 0.5 * (nL + nR);
+        const double chi = nc / g.mu_L;
+        const double mut = (terms & SA_MUT) ? nc * sa::fv1(chi) : 0.0;
+        // physics.h:230-257 with mu_L + mu_t
+        {
+            const double mu = g.mu_L + mut;
+            const double kk = g.cp * (g.mu_L / g.Pr_L + mut / g.Pr_T);
+            double gradT[2];
+            calc_grad_temp(gradT, qc, gxa, gya, g);
+            const double r2 = qc[0] * qc[0];
+            const double gu0 = (qc[0] * gxa[1] - qc[1] * gxa[0]) / r2, gu1 = (qc[0] * gya[1] - qc[1] * gya[0]) / r2;
+            const double gv0 = (qc[0] * gxa[2] - qc[2] * gxa[0]) / r2, gv1 = (qc[0] * gya[2] - qc[2] * gya[0]) / r2;
+            const double div_v = gu0 + gv1;
+            const double txx = 2. * mu * (gu0 - div_v / 3.), tyy = 2. * mu * (gv1 - div_v / 3.), txy = mu * (gu1 + gv0);
+            const double phi0 = qc[1] / qc[0] * txx + qc[2] / qc[0] * txy + kk * gradT[0];
+            const double phi1 = qc[1] / qc[0] * txy + qc[2] / qc[0] * tyy + kk * gradT[1];
+            f[1] -= F.nx * txx + F.ny * txy;
+            f[2] -= F.nx * txy + F.ny * tyy;
+            f[3] -= F.nx * phi0 + F.ny * phi1;
+        }
+        if (terms & SA_DIFF) {
+            // face gradient of rho nu~ (solver.h:359-398), then
+            // grad nu~ = (grad(rho nu~) - nu~ grad rho) / rho
+            const Mesh& m = S.m;
+            double t0 = m.cellsCentersX[c0] + m.cellsCentersX[c1], t1 = m.cellsCentersY[c0] + m.cellsCentersY[c1];
+            const double l = std::sqrt(t0 * t0 + t1 * t1);
+            t0 /= l; t1 /= l;
+            const double dir = (nu[c0] - nu[c1]) / l;
+            const double gxb = (gnx[c0] + gnx[c1]) * 0.5, gyb = (gny[c0] + gny[c1]) * 0.5;
+            const double dot = gxb * t0 + gyb * t1;
+            const double gfx = gxb - (dot - dir) * t0, gfy = gyb - (dot - dir) * t1;
+            const double nut = nc / qc[0];
+            const double gvx = (gfx - nut * gxa[0]) / qc[0], gvy = (gfy - nut * gya[0]) / qc[0];
+            const double dcoef = (g.mu_L + nc * sa::fn(chi)) / sa::sigma;
+            f[4] -= dcoef * (F.nx * gvx + F.ny * gvy);
+        }
+    }
+    // fillRhoRHS (solver.h:1079-1152) with the fifth equation; RhoVector and
+    // TurbVector are the un-divided residuals, ghost rows zero
+    void fill_rhs() {
+        Solver& S = *s;
+        const Mesh& m = S.m;
+        const Gas& g = S.g;
+        const u32 nR = m.nRealCells, nT = m.nCells();
+        S.calc_dt();
+        S.set_walls_from_internal(S.q);
+        for (size_t b = 0; b < m.boundaryEdges.size(); ++b)
+            if (S.bface_type[b] == BC_WALL || S.bface_type[b] == BC_SLIP) {
+                const u32 e = m.boundaryEdges[b];
+                nu[m.edgesCells[2 * e + 1]] = nu[m.edgesCells[2 * e]];
+            }
+        if (!S.gg && S.lsq.empty()) S.calc_least_squares_matrices();
+        S.calc_gradients();
+        gradients();
+        if (S.second_order) { S.calc_limiters(S.q); limiters(); }
+        std::fill(S.RhoVector.begin(), S.RhoVector.end(), 0.0);
+        std::fill(TurbVector.begin(), TurbVector.end(), 0.0);
+        for (u32 e = 0; e < m.nFaces(); ++e) {
+            const u32 c0 = m.edgesCells[2 * e], c1 = m.edgesCells[2 * e + 1];
+            double qL[4], qR[4], nL = nu[c0], nRv = nu[c1];
+            for (int k = 0; k < 4; ++k) { qL[k] = S.q[4 * c0 + k]; qR[k] = S.q[4 * c1 + k]; }
+            if (S.second_order) {
+                const double d0x = m.edgesCentersX[e] - m.cellsCentersX[c0], d0y = m.edgesCentersY[e] - m.cellsCentersY[c0];
+                const double d1x = m.edgesCentersX[e] - m.cellsCentersX[c1], d1y = m.edgesCentersY[e] - m.cellsCentersY[c1];
+                for (int k = 0; k < 4; ++k) {
+                    qL[k] = qL[k] + (S.gx[4 * c0 + k] * d0x + S.gy[4 * c0 + k] * d0y) * S.limiters[4 * c0 + k];
+                    qR[k] = qR[k] + (S.gx[4 * c1 + k] * d1x + S.gy[4 * c1 + k] * d1y) * S.limiters[4 * c1 + k];
+                }
+                nL = nL + (gnx[c0] * d0x + gny[c0] * d0y) * lnu[c0];
+                nRv = nRv + (gnx[c1] * d1x + gny[c1] * d1y) * lnu[c1];
+            }
+            double f[5];
+            face_flux5(e, qL, nL, qR, nRv, f);
+            for (int k = 0; k < 5; ++k) f[k] = f[k] * m.edgesLengths[e];
+            for (int k = 0; k < 4; ++k) {
+                S.RhoVector[4 * c0 + k] -= f[k];
+                if (S.ff[e].two_sided()) S.RhoVector[4 * c1 + k] += f[k];
+            }
+            TurbVector[c0] -= f[4];
+            if (S.ff[e].two_sided()) TurbVector[c1] += f[4];
+        }
+        if (terms & SA_SOURCE)
+            for (u32 i = 0; i < nR; ++i) TurbVector[i] += m.cellsAreas[i] * cell_source(i);
+        for (u32 i = nR; i < nT; ++i) {
+            for (int k = 0; k < 4; ++k) S.RhoVector[4 * i + k] = 0;
+            TurbVector[i] = 0;
+        }
+        (void)g;
+    }
+    // per-volume source of cell i from its unlimited gradients
+    double cell_source(u32 i) const {
+        const Solver& S = *s;
+        const double* qi = &S.q[4 * i];
+        const double* gxi = &S.gx[4 * i];
+        const double* gyi = &S.gy[4 * i];
+        const double rho = qi[0], nut = nu[i] / rho;
+        const double dudy = (gyi[1] - qi[1] / rho * gyi[0]) / rho;
+        const double dvdx = (gxi[2] - qi[2] / rho * gxi[0]) / rho;
+        const double omega = std::fabs(dvdx - dudy);
+        const double gvx = (gnx[i] - nut * gxi[0]) / rho, gvy = (gny[i] - nut * gyi[0]) / rho;
+        return sa::source(rho, nut, S.g.mu_L, omega, S.m.wall_dist[i], gvx, gvy);
+    }
+};
+
 // mesh.h:794-830
 static void compute_wall_dist(Mesh& m, const std::vector<int>& bface_type) {
     for (u32 i = 0; i < m.nCells(); ++i) {
@@ -1070,10 +1358,11 @@ void* orc_solver_create(void* mh, double R, double mu_L, double Pr_L, double gam
     auto* s = new Solver();
     Gas g; g.R = R; g.mu_L = mu_L; g.Pr_L = Pr_L; g.gamma = gamma;
     s->set_mesh_and_gas(*(Mesh*)mh, g);
-    s->viscous = viscous == 1 ? 1 : 0;
+    s->viscous = viscous == 1 ? 1 : 0;   // 2 (SA): mean-flow viscous terms come from SASolver
     s->viscous_model_named_non_inviscid = viscous != 0;
     return s;
 }
+void orc_solver_set_prt(void* h, double Pr_T) { ((Solver*)h)->g.Pr_T = Pr_T; }
 void orc_solver_free(void* h) { delete (Solver*)h; }
 
 // bcs: nb entries; names '\n'-separated; types BC_*; per-entry farfield vars (mach, angle, T, p)
@@ -1112,6 +1401,34 @@ void orc_solver_calc_residual(void* h, const double* qk) {
     s->calc_residual(v);
 }
 double orc_solver_explicit_step(void* h, double relax) { return ((Solver*)h)->explicit_solve(relax); }
+
+// ---- Spalart-Allmaras (the model of namespace sa) ----
+void* orc_sa_create(void* solver_h) { return new SASolver((Solver*)solver_h); }
+void orc_sa_free(void* h) { delete (SASolver*)h; }
+void orc_sa_set_terms(void* h, int terms) { ((SASolver*)h)->terms = terms; }
+void orc_sa_fill_rhs(void* h) {
+    SASolver* a = (SASolver*)h;
+    bool have_d = false;
+    for (double d : a->s->m.wall_dist) have_d = have_d || d != 0.0;
+    if (!have_d) compute_wall_dist(a->s->m, a->s->bface_type);
+    a->fill_rhs();
+}
+long orc_sa_vec(void* h, const char* name, double* out, const double* in) {
+    SASolver* a = (SASolver*)h;
+    std::string n(name);
+    std::vector<double>* v = nullptr;
+    if (n == "nu") v = &a->nu; else if (n == "TurbVector") v = &a->TurbVector; else if (n == "gnx") v = &a->gnx;
+    else if (n == "gny") v = &a->gny; else if (n == "lnu") v = &a->lnu;
+    if (!v) return -1;
+    if (in) std::memcpy(v->data(), in, v->size() * 8);
+    if (out) std::memcpy(out, v->data(), v->size() * 8);
+    return (long)v->size();
+}
+double orc_sa_source(double rho, double nut, double mu, double omega, double d, double gnx, double gny) {
+    return sa::source(rho, nut, mu, omega, d, gnx, gny);
+}
+double orc_sa_fw(double r) { return sa::fw(r); }
+double orc_sa_fv1(double chi) { return sa::fv1(chi); }
 double orc_solver_uniform_residual(void* h) { return ((Solver*)h)->get_uniform_residual(); }
 void orc_solver_wall_profile(void* h, const char* patch, double* out3) {
     ((Solver*)h)->wall_profile(patch, &out3[0], &out3[1], &out3[2]);

@@ -50,6 +50,15 @@ def _load():
                                  C.c_double, dp, dp, dp, dp, dp]),
         "orc_face_vars": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp]),
         "orc_freestream": (None, [C.c_double] * 6 + [dp]),
+        "orc_solver_set_prt": (None, [P, C.c_double]),
+        "orc_sa_create": (P, [P]),
+        "orc_sa_free": (None, [P]),
+        "orc_sa_set_terms": (None, [P, C.c_int]),
+        "orc_sa_fill_rhs": (None, [P]),
+        "orc_sa_vec": (C.c_long, [P, C.c_char_p, dp, dp]),
+        "orc_sa_source": (C.c_double, [C.c_double] * 7),
+        "orc_sa_fw": (C.c_double, [C.c_double]),
+        "orc_sa_fv1": (C.c_double, [C.c_double]),
     }
     for n, (r, a) in sig.items():
         f = getattr(lib, n)
@@ -115,14 +124,34 @@ class OMesh:
             self.h = None
 
 
-class OSolver:
-    """Oracle solver.  viscous: 0 inviscid, 1 laminar."""
+SA_CONV, SA_DIFF, SA_SOURCE, SA_MUT, SA_ALL = 1, 2, 4, 8, 15
 
-    def __init__(self, mesh, R=0.71428571428, mu_L=1e-5, Pr_L=0.72, gamma=1.4, viscous=0):
+
+class OSolver:
+    """Oracle solver.  viscous: 0 inviscid, 1 laminar, 2 Spalart-Allmaras."""
+
+    def __init__(self, mesh, R=0.71428571428, mu_L=1e-5, Pr_L=0.72, gamma=1.4, viscous=0, Pr_T=0.9):
         self.mesh = mesh
         self.h = lib.orc_solver_create(mesh.h, R, mu_L, Pr_L, gamma, viscous)
+        lib.orc_solver_set_prt(self.h, Pr_T)
+        self.sa = lib.orc_sa_create(self.h) if viscous == 2 else None
         c = mesh.counts()
         self.n_cells, self.n_real, self.n_faces = c["cells"], c["real_cells"], c["faces"]
+
+    def sa_vec(self, name, value=None):
+        n = lib.orc_sa_vec(self.sa, name.encode(), None, None)
+        out = np.empty(n)
+        lib.orc_sa_vec(self.sa, name.encode(), _d(out),
+                       None if value is None else _d(np.ascontiguousarray(value, np.float64)))
+        return out
+
+    def sa_terms(self, terms):
+        lib.orc_sa_set_terms(self.sa, terms)
+
+    def sa_fill_rhs(self):
+        """fillRhoRHS with the SA equation: (RhoVector (4N), TurbVector (N))."""
+        lib.orc_sa_fill_rhs(self.sa)
+        return self.vec("RhoVector"), self.sa_vec("TurbVector")
 
     def set_bcs(self, bcs):
         """bcs: {name: (type_str, (mach, angle, T, p))}"""
@@ -177,6 +206,9 @@ class OSolver:
         return self.vec("qW")
 
     def __del__(self):
+        if getattr(self, "sa", None):
+            lib.orc_sa_free(self.sa)
+            self.sa = None
         if getattr(self, "h", None):
             lib.orc_solver_free(self.h)
             self.h = None
