@@ -1,17 +1,20 @@
 """Residual-throughput benchmark (one GPU).
 
-One step = one evaluation of the unstructured finite-volume residual
-(gradients + limiters, Roe face fluxes, signed cell sums) on a 2.0M-cell
-jittered mixed triangle/quad mesh around a NACA 0012 section, state =
-freestream (M 0.7, alpha 1.5 deg) with 1% seeded Gaussian perturbations of
-rho, u, v, p.  The timed form is FP32 (compensated FP32 cell sums); the plain
-FP64 form is timed beside it.  Inputs are resident in HBM before timing.
+One step = one evaluation of the unstructured finite-volume RANS residual
+with the Spalart-Allmaras model (gradients + limiters of the five variables,
+Roe + viscous + nu~ face fluxes, SA sources, signed cell sums) on a 2.0M-cell
+jittered mixed triangle/quad mesh around a NACA 0012 section, Re 1e6 on the
+chord, state = freestream (M 0.7, alpha 1.5 deg, rho nu~ = 3 mu) with 1%
+seeded Gaussian perturbations of rho, u, v, p and nu~.  The timed form is FP32
+(FP64-accumulated cell sums); the plain FP64 form is timed beside it.  Inputs
+are resident in HBM before timing.  --model inviscid runs the four-variable
+Euler residual instead.
 
 Prints one JSON line (see the keys below).  Kernel durations come from HIP
 events recorded on the library's own stream around back-to-back launches.
 The CPU baseline is the oracle (serial FP64 restatement of the reference's
-fillRhoRHS, calc_dt included) on a bounded sample mesh from the same
-generator.
+fillRhoRHS with the same SA terms, calc_dt included) on a bounded sample
+mesh from the same generator.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N] [--no-cpu]
 """
@@ -98,34 +101,55 @@ def mesh_dims(target_cells, quad_frac=0.25):
     return ni, nj
 
 
-def build_case(rans, target_cells, seed_mesh=1, seed_state=2, tile_cells=0):
+MACH, ALPHA_DEG, REYNOLDS = 0.7, 1.5, 1e6
+
+
+def freestream_mu(rans):
+    """mu_L for Re 1e6 on the unit chord at the freestream state."""
+    g = rans.Gas()
+    q = rans.BoundaryVariables(mach=MACH, angle=np.deg2rad(ALPHA_DEG)).get_conservative(g)
+    return float(np.hypot(q[1], q[2]) / REYNOLDS)   # rho |u| c / Re with c = 1
+
+
+def perturbed(q, nu, n, qinf, gamma, rng):
+    """Freestream with 1% seeded Gaussian perturbations of rho, u, v, p, nu~."""
+    rho = qinf[0] * (1 + 0.01 * rng.standard_normal(n))
+    u = qinf[1] / qinf[0] * (1 + 0.01 * rng.standard_normal(n))
+    v = qinf[2] / qinf[0] * (1 + 0.01 * rng.standard_normal(n))
+    p0 = (gamma - 1) * (qinf[3] - 0.5 * (qinf[1] ** 2 + qinf[2] ** 2) / qinf[0])
+    p = p0 * (1 + 0.01 * rng.standard_normal(n))
+    q[0:4 * n:4] = rho
+    q[1:4 * n:4] = rho * u
+    q[2:4 * n:4] = rho * v
+    q[3:4 * n:4] = p / (gamma - 1) + 0.5 * rho * (u * u + v * v)
+    if nu is not None:
+        nut_inf = nu[0] / qinf[0]
+        nu[:n] = rho * nut_inf * (1 + 0.01 * rng.standard_normal(n))
+    return q, nu
+
+
+def build_case(rans, target_cells, model="spalart-allmaras", seed_mesh=1, seed_state=2, tile_cells=0):
     ni, nj = mesh_dims(target_cells)
     t0 = time.perf_counter()
     m = rans.Mesh.jittered(ni, nj, r_far=20.0, thickness=0.12, first_h=1e-4, jitter=0.1, quad_frac=0.25,
                            seed=seed_mesh)
     t_mesh = time.perf_counter() - t0
-    s = rans.Solver(m, viscosity_model="inviscid", second_order=True, gradient_scheme="green-gauss",
+    sa = model == "spalart-allmaras"
+    gas = rans.Gas(mu_L=freestream_mu(rans)) if sa else rans.Gas()
+    s = rans.Solver(m, gas=gas, viscosity_model=model, second_order=True, gradient_scheme="green-gauss",
                     limiter_k=5.0, tile_cells=tile_cells)
-    far = rans.BoundaryVariables(mach=0.7, angle=np.deg2rad(1.5))
-    s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition("slip-wall")})
+    far = rans.BoundaryVariables(mach=MACH, angle=np.deg2rad(ALPHA_DEG))
+    s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far),
+               "wall": rans.BoundaryCondition("wall" if sa else "slip-wall")})
     s.init()
     s.refill_bcs()
-    q = s.get_q()
-    g = s.gas
-    qinf = far.get_conservative(g)
+    qinf = far.get_conservative(s.gas)
     rng = np.random.default_rng(seed_state)
-    n = s.n_real
-    rho = qinf[0] * (1 + 0.01 * rng.standard_normal(n))
-    u = qinf[1] / qinf[0] * (1 + 0.01 * rng.standard_normal(n))
-    v = qinf[2] / qinf[0] * (1 + 0.01 * rng.standard_normal(n))
-    p0 = (g.gamma - 1) * (qinf[3] - 0.5 * (qinf[1] ** 2 + qinf[2] ** 2) / qinf[0])
-    p = p0 * (1 + 0.01 * rng.standard_normal(n))
-    q[0:4 * n:4] = rho
-    q[1:4 * n:4] = rho * u
-    q[2:4 * n:4] = rho * v
-    q[3:4 * n:4] = p / (g.gamma - 1) + 0.5 * rho * (u * u + v * v)
+    q, nu = perturbed(s.get_q(), s.get_nu() if sa else None, s.n_real, qinf, s.gas.gamma, rng)
     s.set_q(q)
-    return m, s, dict(ni=ni, nj=nj, mesh_seconds=t_mesh)
+    if sa:
+        s.set_nu(nu)
+    return m, s, dict(ni=ni, nj=nj, mesh_seconds=t_mesh, mu_L=gas.mu_L)
 
 
 def alg_bytes(nc, nf, nb, word, V=4, index_bytes=4):
@@ -161,9 +185,9 @@ def read_traffic(path):
         return None
 
 
-def cpu_baseline(rans, budget_s=12.0):
-    """Oracle (serial FP64 fillRhoRHS, calc_dt included) on a 120k-cell sample
-    mesh of the same generator, 1 thread."""
+def cpu_baseline(rans, model, budget_s=12.0):
+    """Oracle (serial FP64 fillRhoRHS with the same model, calc_dt included) on
+    a 120k-cell sample mesh of the same generator and state, 1 thread."""
     import tempfile
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
@@ -173,23 +197,34 @@ def cpu_baseline(rans, budget_s=12.0):
         path = os.path.join(d, "sample.msh")
         m.write_msh(path)
         om = orc.OMesh.read(path)
-    o = orc.OSolver(om)
-    o.set_bcs({"farfield": ("farfield", (0.7, np.deg2rad(1.5), 1.0, 1.0)), "wall": ("slip-wall", (0.2, 0, 1, 1))})
+    sa = model == "spalart-allmaras"
+    mu = freestream_mu(rans) if sa else 1e-5
+    o = orc.OSolver(om, mu_L=mu, viscous=2 if sa else 0)
+    o.set_bcs({"farfield": ("farfield", (MACH, np.deg2rad(ALPHA_DEG), 1.0, 1.0)),
+               "wall": ("wall" if sa else "slip-wall", (0.2, 0, 1, 1))})
     o.options(second_order=True, green_gauss=True, limiter_k=5.0, cfl=1.0)
     o.init()
     o.refill_bcs()
     nc = om.counts()["real_cells"]
-    o.fill_rhs()
+    rng = np.random.default_rng(2)
+    qinf = orc.freestream(MACH, np.deg2rad(ALPHA_DEG))
+    nu = np.full(o.n_cells, 3.0 * mu) if sa else None
+    q, nu = perturbed(o.vec("q"), nu, nc, qinf, 1.4, rng)
+    o.vec("q", q)
+    fill = o.sa_fill_rhs if sa else o.fill_rhs
+    if sa:
+        o.sa_vec("nu", nu)
+    fill()
     n, t0 = 0, time.perf_counter()
     while True:
-        o.fill_rhs()
+        fill()
         n += 1
         el = time.perf_counter() - t0
         if el >= budget_s or n >= 2000:
             break
     return {"value": round(nc * n / el / 1e6, 4), "unit": "Mcell-residuals/s", "cores": 1, "kind": "port",
-            "sample": f"oracle fillRhoRHS (FP64, serial, calc_dt included) x{n} on a {nc}-cell mesh of the same "
-                      f"generator, {el:.1f} s"}
+            "sample": f"oracle fillRhoRHS ({model}, FP64, serial, calc_dt included) x{n} on a {nc}-cell mesh "
+                      f"of the same generator and state, {el:.1f} s"}
 
 
 def main():
@@ -204,6 +239,7 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run D2D copies of this size (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--only-fp32", action="store_true", help="skip per-kernel and FP64 timings (profiling runs)")
+    ap.add_argument("--model", default="spalart-allmaras", choices=["spalart-allmaras", "inviscid"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -214,7 +250,8 @@ def main():
     ev = HipEvents()
     if args.copy_probe_mib:
         ev.copy_probe(args.copy_probe_mib)
-    m, s, info = build_case(rans, args.cells, tile_cells=args.tile)
+    m, s, info = build_case(rans, args.cells, model=args.model, tile_cells=args.tile)
+    V = 5 if args.model == "spalart-allmaras" else 4
     nc, nf, nb = s.n_real, s.n_faces, s.n_bfaces
     stream = s.stream()
     FP32, FP64 = rans.AFX_FP32, rans.AFX_FP64
@@ -247,20 +284,23 @@ def main():
     # duration from events around back-to-back launches on the library's stream
     nk = max(50, min(args.steps, 500))
     res32_ms = time_stage(ev, s, stream, FP32, 3, nk)
-    b32 = alg_bytes(nc, nf, nb, 4)
+    b32 = alg_bytes(nc, nf, nb, 4, V=V)
     achieved = b32 / (res32_ms * 1e-3) / 1e9
-    # three-kernel FP32 path (gradients+limiters, face fluxes, cell sums)
-    unf = {"grad_lim": time_stage(ev, s, stream, FP32, 0, nk // 5),
-           "face_flux": time_stage(ev, s, stream, FP32, 1, nk // 5),
-           "gather": time_stage(ev, s, stream, FP32, 2, nk // 5)}
+    # three-kernel FP32 path (gradients+limiters, face fluxes, cell sums; four variables only)
+    unf = None
+    if V == 4:
+        unf = {"grad_lim": time_stage(ev, s, stream, FP32, 0, nk // 5),
+               "face_flux": time_stage(ev, s, stream, FP32, 1, nk // 5),
+               "gather": time_stage(ev, s, stream, FP32, 2, nk // 5)}
 
     # plain FP64 form on the same mesh and state (fused kernel, reference rounding)
     s.launch_stage(FP64, 3, 5)
     s.synchronize()
     t64 = time_stage(ev, s, stream, FP64, 3, args.fp64_steps)
-    b64 = alg_bytes(nc, nf, nb, 8)
+    b64 = alg_bytes(nc, nf, nb, 8, V=V)
 
-    traffic = read_traffic(os.path.join(ROOT, "profiles", "residual_fp32_2M_traffic.txt"))
+    prof = "residual_fp32_sa_2M_traffic.txt" if V == 5 else "residual_fp32_2M_traffic.txt"
+    traffic = read_traffic(os.path.join(ROOT, "profiles", prof))
     out = {
         "metric": "Mcell-residuals/s + % HBM roofline; FP32/batched vs plain FP64; adjoint/primal",
         "value": round(value, 2),
@@ -274,22 +314,26 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "FP32 residual evaluation (GG gradients + limiter, Roe fluxes, cell sums) on a "
-                               "jittered mixed tri/quad mesh around NACA 0012, M0.7 a1.5 deg + 1% seeded "
-                               "perturbations",
+        "config": {"workload": ("FP32 SA-RANS residual evaluation (GG gradients + limiter of 5 variables, Roe + "
+                                "viscous + nu~ fluxes, SA sources, cell sums)" if V == 5 else
+                                "FP32 Euler residual evaluation (GG gradients + limiter, Roe fluxes, cell sums)")
+                               + " on a jittered mixed tri/quad mesh around NACA 0012, M0.7 a1.5 deg Re 1e6 + 1% "
+                                 "seeded perturbations",
                    "cells": nc, "faces": nf, "boundary_faces": nb, "ogrid": [info["ni"], info["nj"]],
-                   "variables": 4, "tile_cells": args.tile, "seeds": {"mesh": 1, "state": 2}},
+                   "variables": V, "model": args.model, "mu_L": info["mu_L"], "tile_cells": args.tile,
+                   "seeds": {"mesh": 1, "state": 2}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "afx::k_residual_tile_f32", "alg_bytes_per_launch": b32,
+                     "kernel": f"afx::k_residual_tile_f32<{'true' if V == 5 else 'false'}>",
+                     "alg_bytes_per_launch": b32,
                      "kernel_ms": round(res32_ms, 5), "timed_region_ms_per_step": round(ev_ms / args.steps, 5)},
         "fp64_plain": {"value": round(nc / (t64 * 1e-3) / 1e6, 2), "kernel_ms": round(t64, 5),
-                       "kernel": "afx::k_residual_tile<double>", "alg_bytes_per_launch": b64,
+                       "kernel": f"afx::k_residual_tile<double, {V}>", "alg_bytes_per_launch": b64,
                        "roofline_frac": round(b64 / (t64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "fp32_three_kernel_ms": {k: round(v, 5) for k, v in unf.items()},
+        "fp32_three_kernel_ms": None if unf is None else {k: round(v, 5) for k, v in unf.items()},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(rans)
+        out["cpu_baseline"] = cpu_baseline(rans, args.model)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
