@@ -839,6 +839,65 @@ __device__ __forceinline__ void roe_flux_f32(float gamma, float nx, float ny, co
     f[0] = f0; f[1] = f1; f[2] = f2; f[3] = f3;
 }
 
+// physics.h:230-257 in FP32 with one reciprocal of the face density: the
+// reference's pressure / temperature / velocity gradients (physics.h:33-82)
+// in the algebraically equal forms
+//   grad p = (gamma-1) (grad E - u grad(rho u) - v grad(rho v) + |u|^2/2 grad rho)
+//   grad T = (grad p - (p/rho) grad rho) / (rho R),  grad u = (grad(rho u) - u grad rho) / rho
+__device__ __forceinline__ void viscous_terms_f32(float gm1, float iR, float nx, float ny, const float* qL,
+                                                  const float* qR, const float* gx, const float* gy, float mu, float kc,
+                                                  float& f1, float& f2, float& f3) {
+    const float r = 0.5f * (qL[0] + qR[0]), ru = 0.5f * (qL[1] + qR[1]);
+    const float rv = 0.5f * (qL[2] + qR[2]), re = 0.5f * (qL[3] + qR[3]);
+    const float ir = frcp(r);
+    const float u = ru * ir, v = rv * ir;
+    const float ke = 0.5f * (u * u + v * v);
+    const float pr = gm1 * (re * ir - ke);   // p / rho
+    const float gpx = gm1 * (gx[3] - u * gx[1] - v * gx[2] + ke * gx[0]);
+    const float gpy = gm1 * (gy[3] - u * gy[1] - v * gy[2] + ke * gy[0]);
+    const float cT = ir * iR;
+    const float gTx = (gpx - pr * gx[0]) * cT, gTy = (gpy - pr * gy[0]) * cT;
+    const float gu0 = (gx[1] - u * gx[0]) * ir, gu1 = (gy[1] - u * gy[0]) * ir;
+    const float gv0 = (gx[2] - v * gx[0]) * ir, gv1 = (gy[2] - v * gy[0]) * ir;
+    const float div3 = (gu0 + gv1) * (1.f / 3.f);
+    const float m2 = 2.f * mu;
+    const float txx = m2 * (gu0 - div3), tyy = m2 * (gv1 - div3), txy = mu * (gu1 + gv0);
+    f1 -= nx * txx + ny * txy;
+    f2 -= nx * txy + ny * tyy;
+    f3 -= nx * (u * txx + v * txy + kc * gTx) + ny * (u * txy + v * tyy + kc * gTy);
+}
+
+// x^(1/6) for x > 0 (f_w of the SA model) through the hardware log2 / exp2
+__device__ __forceinline__ float pow_sixth_f32(float x) {
+    return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.f / 6.f));
+}
+
+// sa_source in FP32 with shared reciprocals (imu = 1/mu_L)
+__device__ __forceinline__ float sa_source_f32(float rho, float nut, float imu, float omega, float d, float gvx,
+                                               float gvy) {
+    const float cross = float(sa::cb2 / sa::sigma) * (gvx * gvx + gvy * gvy);
+    const float id = frcp(d);
+    const float nd = nut * id;
+    const float neg = float(sa::cb1 * (1.0 - sa::ct3)) * omega * nut + float(sa::cw1) * nd * nd + cross;
+    const float chi = rho * nut * imu;
+    const float c3 = chi * chi * chi;
+    const float f1 = c3 * frcp(c3 + float(sa::cv1_3));
+    const float f2 = 1.f - chi * frcp(1.f + chi * f1);
+    const float ikd2 = float(1.0 / (sa::kappa * sa::kappa)) * id * id;
+    const float sbar = nut * f2 * ikd2;
+    const float st = (sbar >= -float(sa::cv2) * omega)
+                         ? omega + sbar
+                         : omega + omega * (float(sa::cv2 * sa::cv2) * omega + float(sa::cv3) * sbar) *
+                                       frcp(float(sa::cv3 - 2.0 * sa::cv2) * omega - sbar);
+    const float r = st > 0.f ? fminf(nut * frcp(st) * ikd2, 10.f) : 10.f;
+    const float r2 = r * r;
+    const float gg = r + float(sa::cw2) * (r2 * r2 * r2 - r);
+    const float g2 = gg * gg;
+    const float fw = gg * pow_sixth_f32(float(1.0 + sa::cw3_6) * frcp(g2 * g2 * g2 + float(sa::cw3_6)));
+    const float pos = float(sa::cb1) * st * nut - float(sa::cw1) * fw * nd * nd + cross;
+    return rho * (nut < 0.f ? neg : pos);
+}
+
 // five-variable face flux of sa_face_flux with the FP32 Roe kernel above
 __device__ __forceinline__ void sa_face_flux_f32(const GasT<float>& g, int kind, float nx, float ny, const float* qL,
                                                  const float* qR, const float* gx, const float* gy, float* f) {
@@ -854,15 +913,27 @@ __device__ __forceinline__ void sa_face_flux_f32(const GasT<float>& g, int kind,
     f[4] = f[0] * (f[0] >= 0.f ? qL[4] * frcp(qL[0]) : qR[4] * frcp(qR[0]));
     const float nc = 0.5f * (qL[4] + qR[4]);
     const float chi = nc * frcp(g.mu);
-    const float mut = nc * sa_fv1<float>(chi);
-    const float kc = g.cp * (g.mu * frcp(g.PrL) + mut * frcp(g.PrT));
+    const float c3 = chi * chi * chi;
+    const float mut = chi > 0.f ? nc * c3 * frcp(c3 + float(sa::cv1_3)) : 0.f;
+    const float fn = chi >= 0.f ? 1.f : (float(sa::cn1) + c3) * frcp(float(sa::cn1) - c3);
+    const float kc = g.k + mut * (g.cp * frcp(g.PrT));
     float f1 = f[1], f2 = f[2], f3 = f[3];
-    viscous_terms<float>(g, nx, ny, qL, qR, gx, gy, g.mu + mut, kc, f1, f2, f3);
+    viscous_terms_f32(g.gamma - 1.f, frcp(g.R), nx, ny, qL, qR, gx, gy, g.mu + mut, kc, f1, f2, f3);
     f[1] = f1; f[2] = f2; f[3] = f3;
     const float iqc0 = frcp(0.5f * (qL[0] + qR[0]));
     const float nut = nc * iqc0;
     const float gvx = (gx[4] - nut * gx[0]) * iqc0, gvy = (gy[4] - nut * gy[0]) * iqc0;
-    f[4] -= (g.mu + nc * sa_fn<float>(chi)) * float(1.0 / sa::sigma) * (nx * gvx + ny * gvy);
+    f[4] -= (g.mu + nc * fn) * float(1.0 / sa::sigma) * (nx * gvx + ny * gvy);
+}
+
+// sa_cell_source in FP32
+__device__ __forceinline__ float sa_cell_source_f32(const GasT<float>& g, const float* q, const float* gx,
+                                                    const float* gy, float d) {
+    const float ir = frcp(q[0]);
+    const float nut = q[4] * ir, u = q[1] * ir, v = q[2] * ir;
+    const float omega = fabsf((gx[2] - v * gx[0]) * ir - (gy[1] - u * gy[0]) * ir);
+    const float gvx = (gx[4] - nut * gx[0]) * ir, gvy = (gy[4] - nut * gy[0]) * ir;
+    return sa_source_f32(q[0], nut, frcp(g.mu), omega, d, gvx, gvy);
 }
 
 #ifdef AFX_FP32_UNIT   // defined in afx_k32.hip only
@@ -1154,7 +1225,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA 
                     gxa[k] = gxb - (dot - dir) * t0;
                     gya[k] = gyb - (dot - dir) * t1;
                 }
-                roe_flux<float>(g, ga.x, ga.y, qLa, qRa, true, gxa, gya, F);
+                roe_flux_f32(g.gamma, ga.x, ga.y, qLa, qRa, F);
+                viscous_terms_f32(g.gamma - 1.f, frcp(g.R), ga.x, ga.y, qLa, qRa, gxa, gya, g.mu, g.k, F[1], F[2], F[3]);
             } else {
                 roe_flux_f32(g.gamma, ga.x, ga.y, qLa, qRa, F);
             }
@@ -1200,7 +1272,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA 
             const float q5[5] = {qs.x, qs.y, qs.z, qs.w, sqn[c]};
             const float gx5[5] = {xs.x, xs.y, xs.z, xs.w, gs.x}, gy5[5] = {ys.x, ys.y, ys.z, ys.w, gs.y};
             const float2 ad = sa.sa_ad[c0g + c];
-            sa.Rn[c0g + c] = (float)(a4 + (double)(ad.x * sa_cell_source<float>(g, q5, gx5, gy5, ad.y)));
+            sa.Rn[c0g + c] = (float)(a4 + (double)(ad.x * sa_cell_source_f32(g, q5, gx5, gy5, ad.y)));
         }
     }
 }

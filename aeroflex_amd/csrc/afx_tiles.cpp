@@ -33,27 +33,53 @@ TileLayout build_tiles(const Mesh& m, const std::vector<uint32_t>& rcm, int32_t 
 
     // breadth-first growth from seeds in RCM order
     std::vector<int32_t> tile_of(nC, -1);
+    std::vector<std::vector<int32_t>> cells_of;
+    for (uint32_t s : rcm) {
+        if (tile_of[s] >= 0) continue;
+        const int32_t t = (int32_t)cells_of.size();
+        cells_of.emplace_back();
+        std::vector<int32_t>& tc = cells_of.back();
+        size_t head = 0;
+        tc.push_back((int32_t)s);
+        tile_of[s] = t;
+        while (head < tc.size() && (int32_t)tc.size() < target) {
+            int32_t v = tc[head++];
+            for (int64_t k = ptr[v]; k < ptr[v + 1] && (int32_t)tc.size() < target; ++k) {
+                int32_t w = adj[k];
+                if (tile_of[w] < 0) { tile_of[w] = t; tc.push_back(w); }
+            }
+        }
+    }
+    // a tile whose frontier ran out early is a pocket enclosed by earlier
+    // tiles: merge pockets of at most target/8 cells into their smallest
+    // neighbouring tile (a block per 1-2 cells would idle most of its lanes)
+    const int32_t pocket = std::max(1, target / 8);
+    for (size_t t = 0; t < cells_of.size(); ++t) {
+        if (cells_of[t].empty() || (int32_t)cells_of[t].size() > pocket) continue;
+        int32_t best = -1;
+        for (int32_t c : cells_of[t])
+            for (int64_t k = ptr[c]; k < ptr[c + 1]; ++k) {
+                const int32_t u = tile_of[adj[k]];
+                if (u == (int32_t)t) continue;
+                if (best < 0 || cells_of[u].size() < cells_of[best].size() ||
+                    (cells_of[u].size() == cells_of[best].size() && u < best))
+                    best = u;
+            }
+        if (best < 0) continue;
+        for (int32_t c : cells_of[t]) { tile_of[c] = best; cells_of[best].push_back(c); }
+        cells_of[t].clear();
+    }
     std::vector<int32_t> order;
     order.reserve(nC);
     L.cell0.push_back(0);
-    for (uint32_t s : rcm) {
-        if (tile_of[s] >= 0) continue;
-        const int32_t t = L.ntiles++;
-        size_t head = order.size();
-        order.push_back((int32_t)s);
-        tile_of[s] = t;
-        while (head < order.size() && (int32_t)(order.size() - L.cell0.back()) < target) {
-            int32_t v = order[head++];
-            for (int64_t k = ptr[v]; k < ptr[v + 1] && (int32_t)(order.size() - L.cell0.back()) < target; ++k) {
-                int32_t w = adj[k];
-                if (tile_of[w] < 0) { tile_of[w] = t; order.push_back(w); }
-            }
-        }
+    for (auto& tc : cells_of) {
+        if (tc.empty()) continue;
         // quadrilaterals first inside the tile, so that most wavefronts see
         // cells with the same number of faces
-        std::stable_partition(order.begin() + L.cell0.back(), order.end(),
-                              [&](int32_t c) { return m.cellsIsTriangle[c] == 0; });
+        std::stable_partition(tc.begin(), tc.end(), [&](int32_t c) { return m.cellsIsTriangle[c] == 0; });
+        order.insert(order.end(), tc.begin(), tc.end());
         L.cell0.push_back((int32_t)order.size());
+        L.ntiles++;
     }
     L.cell_new2old = order;
 
