@@ -114,6 +114,9 @@ struct afx_rans {
     bool wd_ready = false;
     DBuf<float> q32n, R32n;
     DBuf<float2> sa_ad32;
+    DBuf<float> dt32;   // FP32 local time step of the batched explicit form
+    DBuf<float4> q32v_b, q32v_c;   // further FP32 stage states of the batched form
+    DBuf<float> q32n_b, q32n_c;
 
     // implicit step: Jacobian blocks, diagonal blocks and their inverses,
     // GMRES work space
@@ -680,8 +683,22 @@ int afx_mesh_tile_stats(const afx_mesh* mh, int32_t tile_cells, int64_t st[8]) {
 void afx_mesh_destroy(afx_mesh* m) { delete m; }
 
 // ---------------------------------------------------------------- solver ---
+namespace {
+int create_impl(const Mesh& mesh_in, const afx_gas* g, const afx_opts* o, const afx_rans* like, afx_rans** out);
+}
+
 int afx_rans_create(const afx_mesh* mh, const afx_gas* g, const afx_opts* o, afx_rans** out) {
     if (!mh || !out) return fail(AFX_EARG, "null argument");
+    return create_impl(mh->m, g, o, nullptr, out);
+}
+
+int afx_rans_create_like(const afx_rans* base, const afx_gas* g, afx_rans** out) {
+    if (!base || !out) return fail(AFX_EARG, "null argument");
+    return create_impl(base->m, g, &base->opts, base, out);
+}
+
+namespace {
+int create_impl(const Mesh& mesh_in, const afx_gas* g, const afx_opts* o, const afx_rans* like, afx_rans** out) {
     afx_opts opts;
     afx_default_opts(&opts);
     if (o) opts = *o;
@@ -690,7 +707,7 @@ int afx_rans_create(const afx_mesh* mh, const afx_gas* g, const afx_opts* o, afx
     if (opts.gradient != 0 && opts.gradient != 1) return fail(AFX_EARG, "gradient must be 0 (least-squares) or 1 (green-gauss)");
     auto h = new afx_rans();
     try {
-        h->m = mh->m;
+        h->m = mesh_in;
     } catch (std::exception& e) {
         delete h;
         return fail(AFX_EARG, e.what());
@@ -714,7 +731,10 @@ int afx_rans_create(const afx_mesh* mh, const afx_gas* g, const afx_opts* o, afx
     // cell order: tiles grown from RCM seeds (or file order), ghosts keep
     // reference order
     h->c_new2old.resize(nT);
-    {
+    if (like) {
+        h->tl = like->tl;   // same mesh and options: same tiles and cell order
+        for (int32_t i = 0; i < nC; ++i) h->c_new2old[i] = like->c_new2old[i];
+    } else {
         std::vector<uint32_t> seeds;
         if (opts.renumber) seeds = rcm_order(m);
         else { seeds.resize(nC); std::iota(seeds.begin(), seeds.end(), 0u); }
@@ -920,6 +940,7 @@ int afx_rans_create(const afx_mesh* mh, const afx_gas* g, const afx_opts* o, afx
             HIPCHK_C(h->t_fck.alloc(nft)); HIPCHK_C(h->t_fck.upload(fck, s));
             HIPCHK_C(h->q32v.alloc(nT)); HIPCHK_C(hipMemsetAsync(h->q32v.p, 0, (size_t)nT * 16, s));
             HIPCHK_C(h->R32v.alloc(nT)); HIPCHK_C(hipMemsetAsync(h->R32v.p, 0, (size_t)nT * 16, s));
+            HIPCHK_C(h->dt32.alloc(nT)); HIPCHK_C(hipMemsetAsync(h->dt32.p, 0, (size_t)nT * 4, s));
             if (h->sa()) {
                 HIPCHK_C(h->q32n.alloc(nT)); HIPCHK_C(hipMemsetAsync(h->q32n.p, 0, (size_t)nT * 4, s));
                 HIPCHK_C(h->R32n.alloc(nT)); HIPCHK_C(hipMemsetAsync(h->R32n.p, 0, (size_t)nT * 4, s));
@@ -941,6 +962,7 @@ int afx_rans_create(const afx_mesh* mh, const afx_gas* g, const afx_opts* o, afx
     *out = h;
     return AFX_OK;
 }
+}  // namespace
 
 int afx_rans_set_bcs(afx_rans* h, int32_t n, const char* const* names, const int32_t* types, const double* vars) {
     int rc = check_handle(h, false);
@@ -1115,7 +1137,8 @@ int afx_rans_set_cfl(afx_rans* h, double cfl) {
 int afx_rans_calc_dt(afx_rans* h, double* dt_out) {
     int rc = check_handle(h);
     if (rc) return rc;
-    HIPCHK(launch_dt<double>(h->stream, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->cfl, h->dt.p));
+    HIPCHK(launch_dt<double>(h->stream, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->cfl, h->dt.p,
+                             h->sa() ? 1 : 0));
     if (dt_out) return download_cells(h, h->dt.p, dt_out);
     HIPCHK(hipStreamSynchronize(h->stream));
     return AFX_OK;
@@ -1192,7 +1215,30 @@ int afx_rans_explicit_residual(afx_rans* h, const double* qk, double* qW) {
 int afx_rans_explicit_step(afx_rans* h, double relaxation, double* err) {
     int rc = check_handle(h);
     if (rc) return rc;
-    if (h->sa()) return fail(AFX_EUNSUP, "explicit stages are not available with the Spalart-Allmaras model");
+    if (h->sa()) {
+        // explicitSolver::solve (solver.h:802-828) with the five-variable
+        // fused residual: gradients and limiters from the stage state
+        if (!h->tile64_ok) return fail(AFX_EUNSUP, "the Spalart-Allmaras residual needs the fused tile kernels");
+        static const double alpha5[3] = {0.25, 0.5, 1.};
+        hipStream_t s = h->stream;
+        const int64_t nq = (int64_t)h->nv * h->nT;
+        HIPCHK(launch_dt<double>(s, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->cfl, h->dt.p, 1));
+        HIPCHK(launch_copy(s, h->q.p, h->qk.p, nq));
+        for (double ai : alpha5) {
+            HIPCHK(launch_residual_tile<double>(s, h->tiles<double>(h->t_geo64.p), h->qk.p, h->nT, h->nC, h->area.p,
+                                                h->lsq.p, h->xc.p, h->yc.p, h->wall_d.p, h->gas64(), h->opts.gradient,
+                                                h->opts.second_order, h->opts.limiter_k, h->opts.viscosity, h->R.p,
+                                                h->qk.p));
+            HIPCHK(launch_rk_update_area(s, h->q.p, h->R.p, h->area.p, h->dt.p, ai * relaxation, h->nC, h->nT, h->nv,
+                                         h->qk.p, h->qW.p));
+        }
+        HIPCHK(launch_copy(s, h->qk.p, h->q.p, nq));
+        h->q32_valid = false;
+        h->last_precision = AFX_FP64;
+        if (err) return norm_of(h, h->qW.p, h->nT, h->nv, err);
+        HIPCHK(hipStreamSynchronize(s));
+        return AFX_OK;
+    }
     static const double alpha[3] = {0.25, 0.5, 1.};
     hipStream_t s = h->stream;
     const int64_t nq = (int64_t)NV * h->nT;
@@ -1468,6 +1514,207 @@ int afx_rans_launch_stage(afx_rans* h, int32_t precision, int32_t stage, int32_t
                 HIPCHK(launch_gather<float>(h->stream, h->cells32(), h->flux32.p, h->nF, 0, h->R32.p));
             }
         }
+    }
+    return AFX_OK;
+}
+
+int afx_rans_time_residual(afx_rans* h, int32_t precision, int32_t reps, double* ms) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (reps < 1 || !ms) return fail(AFX_EARG, "reps must be >= 1 and ms non-null");
+    if ((rc = enqueue_residual(h, precision))) return rc;   // warm-up (and FP32 state copy)
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, h->stream));
+    for (int32_t i = 0; i < reps; ++i)
+        if ((rc = enqueue_residual(h, precision))) return rc;
+    HIPCHK(hipEventRecord(b, h->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, a, b));
+    HIPCHK(hipEventDestroy(a));
+    HIPCHK(hipEventDestroy(b));
+    *ms = (double)t / reps;
+    return AFX_OK;
+}
+
+// ------------------------------------------------------------------ batch --
+// Many flow conditions / meshes advanced together: each pseudo-time step is
+// the explicit three-stage scheme of explicitSolver::solve (solver.h:802-828);
+// each stage is ONE launch of the FP32 tile kernel over the (member, tile)
+// work items of all members, which also applies the FP64 stage update of its
+// cells (and, at stage 0, computes the local time step).  The FP32 stage
+// states rotate through three buffers per member (A -> B -> C -> A), so a
+// launch never rewrites the states it reads.  Steps are replayed from a
+// captured hipGraph.
+struct afx_batch {
+    std::vector<afx_rans*> mem;
+    hipStream_t stream = nullptr;
+    DBuf<TileArgsF32> stage[3];
+    DBuf<int2> items;
+    int32_t nitems = 0;
+    size_t lds = 0, lds_dt = 0;
+    bool sa = false;
+    double relax = 1.0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int32_t graph_steps = 0;
+    ~afx_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+int batch_step(afx_batch* b) {
+    for (int st = 0; st < 3; ++st)
+        HIPCHK(launch_residual_tile_f32_batch(b->stream, b->stage[st].p, b->items.p, b->nitems,
+                                              st == 0 ? b->lds_dt : b->lds, b->sa));
+    return AFX_OK;
+}
+int batch_tables(afx_batch* b) {
+    static const double alpha[3] = {0.25, 0.5, 1.};
+    const size_t n = b->mem.size();
+    std::vector<TileArgsF32> ta[3];
+    for (auto& v : ta) v.resize(n);
+    std::vector<int2> items;
+    b->lds = b->lds_dt = 0;
+    for (size_t i = 0; i < n; ++i) {
+        afx_rans* h = b->mem[i];
+        const double k = h->opts.limiter_k;
+        float4* X[3] = {h->q32v.p, h->q32v_b.p, h->q32v_c.p};
+        float* Xn[3] = {h->q32n.p, h->q32n_b.p, h->q32n_c.p};
+        for (int st = 0; st < 3; ++st) {
+            SAFields32 saf{Xn[st], h->sa_ad32.p, nullptr, nullptr};
+            TileArgsF32 A{h->tiles32(), X[st], h->iarea32.p, h->a15_32.p, h->lsq32.p, h->xc32.p, h->yc32.p,
+                          nullptr, nullptr, saf, st == 0 ? h->dt32.p : nullptr, h->gas32(), (float)(k * k * k),
+                          (float)h->cfl, h->nC, h->opts.gradient, h->opts.second_order, h->opts.viscosity,
+                          h->q.p, st == 2 ? h->q.p : h->qk.p, X[(st + 1) % 3], Xn[(st + 1) % 3],
+                          st == 0 ? nullptr : h->dt32.p, h->area.p, alpha[st] * b->relax, h->nT};
+            ta[st][i] = A;
+        }
+        const int nv = h->nv;
+        b->lds = std::max(b->lds, tile_lds_bytes_f32(ta[0][i].S, nv, false));
+        b->lds_dt = std::max(b->lds_dt, tile_lds_bytes_f32(ta[0][i].S, nv, true));
+        for (int32_t t = 0; t < h->tl.ntiles; ++t) items.push_back(make_int2((int)i, t));
+    }
+    while (items.size() % 8) items.push_back(make_int2(-1, -1));
+    if (b->lds_dt > 65536) return fail(AFX_EUNSUP, "a member's tiles exceed 64 KiB of LDS");
+    b->nitems = (int32_t)items.size();
+    hipStream_t s = b->stream;
+    if (!b->items.p) {
+        for (auto& d : b->stage) HIPCHK(d.alloc(n));
+        HIPCHK(b->items.alloc(items.size()));
+    }
+    for (int st = 0; st < 3; ++st) HIPCHK(b->stage[st].upload(ta[st], s));
+    HIPCHK(b->items.upload(items, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return AFX_OK;
+}
+}  // namespace
+
+int afx_batch_create(int32_t n, afx_rans* const* members, double relaxation, afx_batch** out) {
+    if (n < 1 || !members || !out) return fail(AFX_EARG, "need at least one member and an output pointer");
+    auto b = new afx_batch();
+    b->relax = relaxation;
+    for (int32_t i = 0; i < n; ++i) {
+        afx_rans* h = members[i];
+        int rc = check_handle(h);
+        if (!rc && !(h->tile32_ok && h->opts.fused && h->iarea32.p))
+            rc = fail(AFX_EUNSUP, "batch members need the fused tile kernels (renumber = 1, fused = 1)");
+        if (!rc && i > 0 && h->sa() != members[0]->sa())
+            rc = fail(AFX_EARG, "batch members must all use the same number of variables");
+        if (!rc && h->sa() && !h->wd_ready) rc = compute_wall_distance(h);
+        if (!rc && !h->q32v_b.p) {   // the two further FP32 stage buffers
+            if (h->q32v_b.alloc(h->nT) != hipSuccess || h->q32v_c.alloc(h->nT) != hipSuccess ||
+                (h->sa() && (h->q32n_b.alloc(h->nT) != hipSuccess || h->q32n_c.alloc(h->nT) != hipSuccess)))
+                rc = fail(AFX_EHIP, "hipMalloc of the batch stage buffers failed");
+        }
+        if (rc) { delete b; return rc; }
+        b->mem.push_back(h);
+    }
+    b->sa = members[0]->sa();
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return fail(AFX_EHIP, "hipStreamCreateWithFlags failed");
+    }
+    int rc = batch_tables(b);
+    if (rc) { delete b; return rc; }
+    *out = b;
+    return AFX_OK;
+}
+
+int afx_batch_run(afx_batch* b, int32_t steps, int32_t graph_steps) {
+    if (!b) return fail(AFX_EARG, "null batch");
+    if (steps < 0 || graph_steps < 0) return fail(AFX_EARG, "steps and graph_steps must be >= 0");
+    // current CFL numbers, then the FP32 copies of every member's state
+    // (ghosts included) before the first stage; the tables keep their device
+    // addresses, so a captured graph reads the refreshed contents
+    int rt = batch_tables(b);
+    if (rt) return rt;
+    for (afx_rans* h : b->mem) {
+        int rc = ensure_q32(h);
+        if (rc) return rc;
+        // ghost rows of every stage buffer (farfield states stay constant)
+        HIPCHK(hipMemcpyAsync(h->q32v_b.p, h->q32v.p, (size_t)h->nT * 16, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->q32v_c.p, h->q32v.p, (size_t)h->nT * 16, hipMemcpyDeviceToDevice, h->stream));
+        if (h->sa()) {
+            HIPCHK(hipMemcpyAsync(h->q32n_b.p, h->q32n.p, (size_t)h->nT * 4, hipMemcpyDeviceToDevice, h->stream));
+            HIPCHK(hipMemcpyAsync(h->q32n_c.p, h->q32n.p, (size_t)h->nT * 4, hipMemcpyDeviceToDevice, h->stream));
+        }
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    int32_t done = 0;
+    if (graph_steps > 0 && steps >= graph_steps) {
+        if (b->graph_steps != graph_steps) {
+            if (b->exec) { HIPCHK(hipGraphExecDestroy(b->exec)); b->exec = nullptr; }
+            if (b->graph) { HIPCHK(hipGraphDestroy(b->graph)); b->graph = nullptr; }
+            HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            for (int32_t i = 0; i < graph_steps; ++i) {
+                int rc = batch_step(b);
+                if (rc) {
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(b->stream, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+            }
+            HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
+            HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            b->graph_steps = graph_steps;
+        }
+        for (; done + graph_steps <= steps; done += graph_steps) HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    }
+    for (; done < steps; ++done) {
+        int rc = batch_step(b);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (afx_rans* h : b->mem) h->q32_valid = true;   // q32v / q32n hold the final state
+    return AFX_OK;
+}
+
+int afx_batch_residual_norms(afx_batch* b, int32_t precision, double* norms) {
+    if (!b || !norms) return fail(AFX_EARG, "null argument");
+    for (size_t i = 0; i < b->mem.size(); ++i) {
+        int rc = afx_rans_residual(b->mem[i], precision, nullptr, &norms[i]);
+        if (rc) return rc;
+    }
+    return AFX_OK;
+}
+
+int afx_batch_stream(afx_batch* b, void** stream) {
+    if (!b || !stream) return fail(AFX_EARG, "null argument");
+    *stream = (void*)b->stream;
+    return AFX_OK;
+}
+
+int afx_batch_destroy(afx_batch* b) {
+    if (b) {
+        (void)hipStreamSynchronize(b->stream);
+        delete b;
     }
     return AFX_OK;
 }

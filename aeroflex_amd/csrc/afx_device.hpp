@@ -267,16 +267,20 @@ __device__ __forceinline__ T sa_bnd_nu(int kind, T nx, T ny, const T* qL, T nL, 
 }
 // five-variable face flux: Roe on the mean flow, upwind convection of rho nu~
 // by the sign of the mass flux; internal faces add the viscous terms with
-// mu_L + mu_t and the nu~ diffusion.  gx/gy: face gradients of the 5 variables.
+// mu_L + mu_t and the nu~ diffusion; a no-slip wall face adds the diffusion
+// (mu_L/sigma) (0 - nu~_c) / d_n of the nu~ = 0 wall condition (nut_c: the
+// owner's cell-centre nu~, dn: its normal distance to the face).  gx/gy: face
+// gradients of the 5 variables.
 template <typename T>
 __device__ __forceinline__ void sa_face_flux(const GasT<T>& g, int kind, T nx, T ny, const T* qL, const T* qR,
-                                             const T* gx, const T* gy, T* f) {
+                                             const T* gx, const T* gy, T nut_c, T dn, T* f) {
     if (kind != K_INTERNAL) {
         T qb[4];
         face_vars<T>(g, kind, nx, ny, qL, qR, qb);
         const T nb = sa_bnd_nu<T>(kind, nx, ny, qL, qL[4], qR, qR[4], qb);
         roe_flux<T>(g, nx, ny, qL, qb, false, gx, gy, f);
         f[4] = f[0] * (f[0] >= T(0) ? ddiv<T>(qL[4], qL[0]) : ddiv<T>(nb, qb[0]));
+        if (kind == K_WALL) f[4] -= ddiv<T>(g.mu, T(sa::sigma)) * ddiv<T>(T(0) - nut_c, dn);
         return;
     }
     roe_flux<T>(g, nx, ny, qL, qR, false, gx, gy, f);
@@ -307,6 +311,23 @@ __device__ __forceinline__ T sa_cell_source(const GasT<T>& g, const T* q, const 
     return sa_source<T>(rho, nut, g.mu, omega, d, gvx, gvy);
 }
 
+// viscous spectral-radius coefficient of a face for the explicit local time
+// step of the SA model, max(4/3, gamma) (mu_L/Pr_L + mu_t/Pr_T) / rho (Blazek,
+// Computational Fluid Dynamics, eq. 6.22) from the face-averaged cell states
+// (owner state on boundary faces); the cell's viscous radius is the sum of
+// coef len^2 / A and enters with weight 4.  q: planes of nT cells.
+template <typename T>
+__device__ __forceinline__ T sa_viscous_radius(const GasT<T>& g, const T* q, size_t nT, int o, int n, bool internal) {
+    T rho = q[o], nuf = q[4 * nT + o];
+    if (internal) {
+        rho = T(0.5) * (rho + q[n]);
+        nuf = T(0.5) * (nuf + q[4 * nT + n]);
+    }
+    const T mut = nuf * sa_fv1<T>(ddiv<T>(nuf, g.mu));
+    const T kmax = g.gamma > T(4.0 / 3.0) ? g.gamma : T(4.0 / 3.0);
+    return ddiv<T>(kmax * (ddiv<T>(g.mu, g.PrL) + ddiv<T>(mut, g.PrT)), rho);
+}
+
 // ---------------------------------------------------------------- kernels --
 constexpr int kBlock = 256;
 
@@ -322,11 +343,11 @@ __global__ void __launch_bounds__(kBlock) k_set_walls(const int32_t* __restrict_
 // solver.h:308-356: dt_i = cfl*A_i / sum_e eig_e*len_e, ghost cells get cfl*A/0
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_dt(CellsT<T> C, FacesT<T> F, const T* __restrict__ q, GasT<T> g, T cfl,
-                                               T* __restrict__ dt) {
+                                               int sa, T* __restrict__ dt) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C.nT) return;
     const size_t nT = (size_t)C.nT;
-    T s = T(0);
+    T s = T(0), sv = T(0);
     if (c < C.nC) {
         for (int slot = 0; slot < 4; ++slot) {
             uint32_t e = C.cface[(size_t)slot * C.nC + c];
@@ -347,9 +368,10 @@ __global__ void __launch_bounds__(kBlock) k_dt(CellsT<T> C, FacesT<T> F, const T
                 eig = (eig < eig_R) ? eig_R : eig;   // std::max(eig_L, eig_R)
             }
             s += eig * F.len[f];
+            if (sa) sv += sa_viscous_radius<T>(g, q, nT, o, n, F.kind[f] == K_INTERNAL) * F.len[f] * F.len[f];
         }
     }
-    dt[c] = ddiv<T>(cfl * C.area[c], s);
+    dt[c] = sa ? ddiv<T>(cfl * C.area[c], s + T(4) * ddiv<T>(sv, C.area[c])) : ddiv<T>(cfl * C.area[c], s);
 }
 
 // solver.h:425-514 (gradients from qg) fused with solver.h:517-593 (limiters from ql)
@@ -727,14 +749,16 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
         T gxa[NV], gya[NV];
         for (int k = 0; k < NV; ++k) { gxa[k] = T(0); gya[k] = T(0); }
         if (visc) {
-            // solver.h:359-398 (direction from the SUM of the two centroids)
+            // solver.h:359-398: direction from the SUM of the two centroids
+            // (reference, A1); the SA model takes the textbook direction
+            // x_n - x_o and derivative (q_n - q_o) / |x_n - x_o|
             const int go_ = Lg[o], gn_ = Lg[n];
-            T t0 = xc[go_] + xc[gn_];
-            T t1 = yc[go_] + yc[gn_];
+            T t0 = SA ? xc[gn_] - xc[go_] : xc[go_] + xc[gn_];
+            T t1 = SA ? yc[gn_] - yc[go_] : yc[go_] + yc[gn_];
             const T l = dsqrt<T>(t0 * t0 + t1 * t1);
             t0 = ddiv<T>(t0, l); t1 = ddiv<T>(t1, l);
             for (int k = 0; k < NV; ++k) {
-                const T dir = ddiv<T>(sq[k * LM + o] - sq[k * LM + n], l);
+                const T dir = SA ? ddiv<T>(sq[k * LM + n] - sq[k * LM + o], l) : ddiv<T>(sq[k * LM + o] - sq[k * LM + n], l);
                 const T gxb = (sgx[k * GM + o] + sgx[k * GM + n]) * T(0.5);
                 const T gyb = (sgy[k * GM + o] + sgy[k * GM + n]) * T(0.5);
                 const T dot = gxb * t0 + gyb * t1;
@@ -743,8 +767,13 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
             }
         }
         T F[NV];
-        if constexpr (SA) sa_face_flux<T>(g, kind, nx, ny, qL, qR, gxa, gya, F);
-        else face_flux<T>(g, kind, nx, ny, qL, qR, visc, gxa, gya, F);
+        if constexpr (SA) {
+            const T dn = S.geo[4 * NF + f] * nx + S.geo[5 * NF + f] * ny;
+            const T nut_c = ddiv<T>(sq[4 * LM + o], sq[o]);
+            sa_face_flux<T>(g, kind, nx, ny, qL, qR, gxa, gya, nut_c, dn, F);
+        } else {
+            face_flux<T>(g, kind, nx, ny, qL, qR, visc, gxa, gya, F);
+        }
         for (int k = 0; k < NV; ++k) sfl[k * FM + fl] = F[k] * L;
     }
     __syncthreads();
@@ -900,13 +929,15 @@ __device__ __forceinline__ float sa_source_f32(float rho, float nut, float imu, 
 
 // five-variable face flux of sa_face_flux with the FP32 Roe kernel above
 __device__ __forceinline__ void sa_face_flux_f32(const GasT<float>& g, int kind, float nx, float ny, const float* qL,
-                                                 const float* qR, const float* gx, const float* gy, float* f) {
+                                                 const float* qR, const float* gx, const float* gy, float nut_c,
+                                                 float dn, float* f) {
     if (kind != K_INTERNAL) {
         float qb[4];
         face_vars<float>(g, kind, nx, ny, qL, qR, qb);
         const float nb = sa_bnd_nu<float>(kind, nx, ny, qL, qL[4], qR, qR[4], qb);
         roe_flux_f32(g.gamma, nx, ny, qL, qb, f);
         f[4] = f[0] * (f[0] >= 0.f ? qL[4] * frcp(qL[0]) : nb * frcp(qb[0]));
+        if (kind == K_WALL) f[4] += g.mu * float(1.0 / sa::sigma) * nut_c * frcp(dn);
         return;
     }
     roe_flux_f32(g.gamma, nx, ny, qL, qR, f);
@@ -945,19 +976,25 @@ __device__ __forceinline__ uint32_t slot_of(const uint2& w, int s) {
 }
 #define F4_LOOP(expr) { expr(x); expr(y); expr(z); expr(w); }
 
-template <bool SA>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA ? 5 : 6))) k_residual_tile_f32(TilesF32 S, const float4* __restrict__ q, int32_t nC,
-                                                              const float* __restrict__ iarea,
-                                                              const float* __restrict__ a15,
-                                                              const float* __restrict__ lsq,
-                                                              const float* __restrict__ xc,
-                                                              const float* __restrict__ yc, GasT<float> g, int gg,
-                                                              int second_order, float K3, int viscous,
-                                                              float4* __restrict__ R, float4* __restrict__ qghost,
-                                                              SAFields32 sa) {
-    extern __shared__ float4 smem4[];
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    if (t >= S.ntiles) return;
+// one tile of the FP32 residual (the body of both the single-condition and
+// the batched kernels)
+template <bool SA, bool STAGE>
+__device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const int t, float4* smem4) {
+    const TilesF32& S = A.S;
+    const float4* __restrict__ q = A.q;
+    const int32_t nC = A.nC;
+    const float* __restrict__ iarea = A.iarea;
+    const float* __restrict__ a15 = A.a15;
+    const float* __restrict__ lsq = A.lsq;
+    const float* __restrict__ xc = A.xc;
+    const float* __restrict__ yc = A.yc;
+    const GasT<float> g = A.g;
+    const int gg = A.gg, second_order = A.second_order, viscous = A.viscous;
+    const float K3 = A.K3;
+    float4* __restrict__ R = A.R;
+    float4* __restrict__ qghost = A.qghost;
+    const SAFields32 sa = A.sa;
+    float* __restrict__ dt_out = STAGE ? A.dt : nullptr;   // the local time step belongs to the stage form
     const int c0g = S.cell0[t], nTt = S.cell0[t + 1] - c0g;
     const int lo = S.loff[t], nL = S.loff[t + 1] - lo;
     const int nG = S.ng[t];
@@ -973,6 +1010,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA 
     float* sqn = reinterpret_cast<float*>(sgn + S.gmax);      // [LM]
     float* slmn = sqn + S.lmax;                                 // [GM]
     float* sfln = slmn + S.gmax;                                // [FM]
+    // local time step: lambda len per face touching T (SA: and the viscous
+    // coefficient times len^2)
+    float* sev = SA ? sfln + S.ftmax : reinterpret_cast<float*>(sfl + S.ftmax);   // [FM]
+    float* sev2 = sev + S.ftmax;                                                   // [FM], SA only
     const int* Lg = S.Lglob + lo;
     const uint32_t* fck = S.fck + fo;
     const float4* geo = S.geo + 2 * (int64_t)fo;
@@ -1158,6 +1199,31 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA 
         const float4 ga = geo[2 * fl], gb = geo[2 * fl + 1];
         float4 qL = sq[o];
         float4 qR = (kind == K_INTERNAL || kind == K_FARFIELD) ? sq[n] : f4(0.f, 0.f, 0.f, 0.f);
+        if (dt_out) {
+            // solver.h:313-350: max of the two sides' c + |V_n| (owner side
+            // only on boundary faces), from the cell states
+            const float gm1 = g.gamma - 1.f;
+            const float irL = frcp(qL.x);
+            const float VL = (qL.y * ga.x + qL.z * ga.y) * irL;
+            const float pL = gm1 * (qL.w - 0.5f * (qL.y * qL.y + qL.z * qL.z) * irL);
+            float eig = __builtin_amdgcn_sqrtf(g.gamma * pL * irL) + fabsf(VL);
+            if (kind == K_INTERNAL) {
+                const float irR = frcp(qR.x);
+                const float VR = (qR.y * ga.x + qR.z * ga.y) * irR;
+                const float pR = gm1 * (qR.w - 0.5f * (qR.y * qR.y + qR.z * qR.z) * irR);
+                eig = fmaxf(eig, __builtin_amdgcn_sqrtf(g.gamma * pR * irR) + fabsf(VR));
+            }
+            sev[fl] = eig * ga.z;
+            if constexpr (SA) {
+                const bool in = kind == K_INTERNAL;
+                const float rho = in ? 0.5f * (qL.x + qR.x) : qL.x;
+                const float nuf = in ? 0.5f * (sqn[o] + sqn[n]) : sqn[o];
+                const float chi = nuf * frcp(g.mu), c3 = chi * chi * chi;
+                const float mut = chi > 0.f ? nuf * c3 * frcp(c3 + float(sa::cv1_3)) : 0.f;
+                const float kmax = fmaxf(g.gamma, 4.f / 3.f);
+                sev2[fl] = kmax * (g.k + mut * frcp(g.PrT)) * frcp(rho) * ga.z * ga.z;
+            }
+        }
         if (second_order) {
             const float4 ax = sgx[o], ay = sgy[o], al = slm[o];
 #define RECL(m) qL.m += (ax.m * gb.x + ay.m * gb.y) * al.m;
@@ -1183,8 +1249,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA 
             const float qR5[5] = {qR.x, qR.y, qR.z, qR.w, nR};
             float gxa[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, gya[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
             if (kind == K_INTERNAL) {
+                // face gradients along the cell connection (textbook direction)
                 const int go_ = Lg[o], gn_ = Lg[n];
-                float t0 = xc[go_] + xc[gn_], t1 = yc[go_] + yc[gn_];
+                float t0 = xc[gn_] - xc[go_], t1 = yc[gn_] - yc[go_];
                 const float il = frcp(__builtin_amdgcn_sqrtf(t0 * t0 + t1 * t1));
                 t0 *= il; t1 *= il;
                 const float4 so = sq[o], sn = sq[n], xo = sgx[o], xn = sgx[n], yo = sgy[o], yn = sgy[n];
@@ -1194,7 +1261,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA 
                 const float yo5[5] = {yo.x, yo.y, yo.z, yo.w, go2.y}, yn5[5] = {yn.x, yn.y, yn.z, yn.w, gn2.y};
 #pragma unroll
                 for (int k = 0; k < 5; ++k) {
-                    const float dir = (so5[k] - sn5[k]) * il;
+                    const float dir = (sn5[k] - so5[k]) * il;
                     const float gxb = 0.5f * (xo5[k] + xn5[k]);
                     const float gyb = 0.5f * (yo5[k] + yn5[k]);
                     const float dot = gxb * t0 + gyb * t1;
@@ -1202,7 +1269,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA 
                     gya[k] = gyb - (dot - dir) * t1;
                 }
             }
-            sa_face_flux_f32(g, kind, ga.x, ga.y, qL5, qR5, gxa, gya, F);
+            const float4 so_ = sq[o];
+            sa_face_flux_f32(g, kind, ga.x, ga.y, qL5, qR5, gxa, gya, sqn[o] * frcp(so_.x),
+                             gb.x * ga.x + gb.y * ga.y, F);
             sfl[fl] = f4(F[0] * ga.z, F[1] * ga.z, F[2] * ga.z, F[3] * ga.z);
             sfln[fl] = F[4] * ga.z;
             continue;
@@ -1243,11 +1312,16 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA 
     for (int c = threadIdx.x; c < nTt; c += blockDim.x) {
         const uint2 sw = slots2[c];
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+        float lsum = 0.f, vsum = 0.f;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const uint32_t e = slot_of(sw, s);
             if (e == 0xFFFFu) break;
             const int fl = (int)(e & 0x7FFFu);
+            if (dt_out) {
+                lsum += sev[fl];
+                if constexpr (SA) vsum += sev2[fl];
+            }
             const float4 v = sfl[fl];
             const bool nb = (e >> 15) != 0;
             a0 += (double)(nb ? v.x : -v.x);
@@ -1265,16 +1339,61 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA 
                 }
             }
         }
-        R[c0g + c] = f4((float)a0, (float)a1, (float)a2, (float)a3);
+        const int gc = c0g + c;
+        float dtc = 0.f;
+        if (dt_out) {
+            const float ia = iarea[gc];
+            dtc = A.cfl * frcp(ia * lsum + (SA ? 4.f * vsum * ia * ia : 0.f));
+            dt_out[gc] = dtc;
+        }
         if constexpr (SA) {
             const float4 qs = sq[c], xs = sgx[c], ys = sgy[c];
             const float2 gs = sgn[c];
             const float q5[5] = {qs.x, qs.y, qs.z, qs.w, sqn[c]};
             const float gx5[5] = {xs.x, xs.y, xs.z, xs.w, gs.x}, gy5[5] = {ys.x, ys.y, ys.z, ys.w, gs.y};
-            const float2 ad = sa.sa_ad[c0g + c];
-            sa.Rn[c0g + c] = (float)(a4 + (double)(ad.x * sa_cell_source_f32(g, q5, gx5, gy5, ad.y)));
+            const float2 ad = sa.sa_ad[gc];
+            a4 += (double)(ad.x * sa_cell_source_f32(g, q5, gx5, gy5, ad.y));
+        }
+        if (STAGE) {
+            if (!dt_out) dtc = A.dt_in[gc];
+            const double w = A.coef * (double)dtc / A.area64[gc];
+            const size_t nT = (size_t)A.nT;
+            const double v0 = A.q0[gc] + w * a0, v1 = A.q0[nT + gc] + w * a1;
+            const double v2 = A.q0[2 * nT + gc] + w * a2, v3 = A.q0[3 * nT + gc] + w * a3;
+            A.qout[gc] = v0;
+            A.qout[nT + gc] = v1;
+            A.qout[2 * nT + gc] = v2;
+            A.qout[3 * nT + gc] = v3;
+            A.q32out[gc] = f4((float)v0, (float)v1, (float)v2, (float)v3);
+            if constexpr (SA) {
+                const double v4 = A.q0[4 * nT + gc] + w * a4;
+                A.qout[4 * nT + gc] = v4;
+                A.q32nout[gc] = (float)v4;
+            }
+        } else {
+            R[gc] = f4((float)a0, (float)a1, (float)a2, (float)a3);
+            if constexpr (SA) sa.Rn[gc] = (float)a4;
         }
     }
+}
+
+template <bool SA>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA ? 5 : 6)))
+k_residual_tile_f32(TileArgsF32 A) {
+    extern __shared__ float4 smem4[];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= A.S.ntiles) return;
+    tile_residual_f32<SA, false>(A, t, smem4);
+}
+
+// batched explicit stage: residual + stage update of every (member, tile)
+template <bool SA>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA ? 4 : 5)))
+k_residual_tile_f32_batch(const TileArgsF32* __restrict__ members, const int2* __restrict__ items) {
+    extern __shared__ float4 smem4[];
+    const int2 it = items[xcd_tile(blockIdx.x, gridDim.x)];
+    if (it.x < 0) return;
+    tile_residual_f32<SA, true>(members[it.x], it.y, smem4);
 }
 
 hipError_t launch_residual_tile_f32(hipStream_t s, TilesF32 S, const float4* q, int32_t nC, const float* iarea,
@@ -1282,15 +1401,22 @@ hipError_t launch_residual_tile_f32(hipStream_t s, TilesF32 S, const float4* q, 
                                     GasT<float> g, int gg, int second_order, float K3, int viscous, float4* R,
                                     float4* qghost, SAFields32 sa) {
     const int nb = ((S.ntiles + 7) / 8) * 8;
-    if (viscous == 2) {
-        const size_t lds = tile_lds_bytes<float>(S.lmax, S.gmax, S.ftmax, 5);
-        hipLaunchKernelGGL(k_residual_tile_f32<true>, dim3(nb), dim3(kBlock), lds, s, S, q, nC, iarea, a15, lsq, xc,
-                           yc, g, gg, second_order, K3, viscous, R, qghost, sa);
-    } else {
-        const size_t lds = tile_lds_bytes<float>(S.lmax, S.gmax, S.ftmax, 4);
-        hipLaunchKernelGGL(k_residual_tile_f32<false>, dim3(nb), dim3(kBlock), lds, s, S, q, nC, iarea, a15, lsq, xc,
-                           yc, g, gg, second_order, K3, viscous, R, qghost, sa);
-    }
+    TileArgsF32 A{S, q, iarea, a15, lsq, xc, yc, R, qghost, sa, nullptr, g, K3, 0.f, nC, gg, second_order, viscous,
+                  nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, 0};
+    if (viscous == 2)
+        hipLaunchKernelGGL(k_residual_tile_f32<true>, dim3(nb), dim3(kBlock), tile_lds_bytes_f32(S, 5, false), s, A);
+    else
+        hipLaunchKernelGGL(k_residual_tile_f32<false>, dim3(nb), dim3(kBlock), tile_lds_bytes_f32(S, 4, false), s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_tile_f32_batch(hipStream_t s, const TileArgsF32* members, const int2* items, int32_t nitems,
+                                          size_t lds, bool sa) {
+    if (nitems <= 0 || (nitems & 7)) return hipErrorInvalidValue;
+    if (sa)
+        hipLaunchKernelGGL(k_residual_tile_f32_batch<true>, dim3(nitems), dim3(kBlock), lds, s, members, items);
+    else
+        hipLaunchKernelGGL(k_residual_tile_f32_batch<false>, dim3(nitems), dim3(kBlock), lds, s, members, items);
     return hipGetLastError();
 }
 #endif  // AFX_FP32_UNIT
@@ -1307,8 +1433,8 @@ inline int nblocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
         return hipGetLastError();                                                                                 \
     }                                                                                                             \
     template <>                                                                                                   \
-    hipError_t launch_dt<T>(hipStream_t s, CellsT<T> C, FacesT<T> F, const T* q, GasT<T> g, T cfl, T* dt) {       \
-        hipLaunchKernelGGL(k_dt<T>, dim3(nblocks(C.nT)), dim3(kBlock), 0, s, C, F, q, g, cfl, dt);                \
+    hipError_t launch_dt<T>(hipStream_t s, CellsT<T> C, FacesT<T> F, const T* q, GasT<T> g, T cfl, T* dt, int sa) { \
+        hipLaunchKernelGGL(k_dt<T>, dim3(nblocks(C.nT)), dim3(kBlock), 0, s, C, F, q, g, cfl, sa, dt);            \
         return hipGetLastError();                                                                                 \
     }                                                                                                             \
     template <>                                                                                                   \

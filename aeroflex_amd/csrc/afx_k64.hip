@@ -83,6 +83,30 @@ hipError_t launch_rk_update(hipStream_t s, const double* q, const double* qW, co
     return hipGetLastError();
 }
 
+// explicit stage update from an undivided residual (R / A as qW of
+// solver.h:790-792), for the Spalart-Allmaras explicit step
+__global__ void __launch_bounds__(kBlock) k_rk_update_area(const double* __restrict__ q, const double* __restrict__ R,
+                                                           const double* __restrict__ area,
+                                                           const double* __restrict__ dt, double coef, int32_t nC,
+                                                           int32_t nT, int nv, double* __restrict__ qk,
+                                                           double* __restrict__ qW) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    const double ia = 1.0 / area[c];
+    for (int k = 0; k < nv; ++k) {
+        size_t i = (size_t)k * nT + c;
+        const double w = R[i] * ia;
+        qW[i] = w;
+        qk[i] = q[i] + w * dt[c] * coef;
+    }
+}
+hipError_t launch_rk_update_area(hipStream_t s, const double* q, const double* R, const double* area, const double* dt,
+                                 double coef, int32_t nC, int32_t nT, int nv, double* qk, double* qW) {
+    hipLaunchKernelGGL(k_rk_update_area, dim3(nblocks(nC)), dim3(kBlock), 0, s, q, R, area, dt, coef, nC, nT, nv, qk,
+                       qW);
+    return hipGetLastError();
+}
+
 __global__ void __launch_bounds__(kBlock) k_to_f32(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (float)a[i];

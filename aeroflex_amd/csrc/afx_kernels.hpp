@@ -115,14 +115,64 @@ hipError_t launch_residual_tile_f32(hipStream_t s, TilesF32 S, const float4* q, 
                                     const float* a15, const float* lsq, const float* xc, const float* yc,
                                     GasT<float> g, int gg, int second_order, float K3, int viscous, float4* R,
                                     float4* qghost, SAFields32 sa);
+
+// Everything one FP32 tile residual reads and writes (one mesh, one flow
+// condition).  dt != nullptr: also the local time step of calc_dt
+// (solver.h:308-356), cfl A / sum_e lambda_e len_e, from the unreconstructed
+// cell states.
+// qout != nullptr: an explicit stage (solver.h:818-823) instead of writing R:
+// qout = q0 + coef dt R / A in FP64 from the FP64 cell sums, with the FP32
+// copy of qout in q32out / q32nout (a different buffer from the one read, so
+// no block reads a state another block of the same launch rewrites); dt from
+// this launch (dt != nullptr) or from dt_in.
+struct TileArgsF32 {
+    TilesF32 S;
+    const float4* q;
+    const float* iarea;
+    const float* a15;
+    const float* lsq;
+    const float* xc;
+    const float* yc;
+    float4* R;
+    float4* qghost;
+    SAFields32 sa;
+    float* dt;
+    GasT<float> g;
+    float K3, cfl;
+    int32_t nC, gg, second_order, viscous;
+    // explicit stage (batched form)
+    const double* q0;
+    double* qout;
+    float4* q32out;
+    float* q32nout;
+    const float* dt_in;
+    const double* area64;
+    double coef;
+    int32_t nT;
+};
+inline size_t tile_lds_bytes_f32(const TilesF32& S, int nv, bool dt) {
+    return tile_lds_bytes<float>(S.lmax, S.gmax, S.ftmax, nv) +
+           (dt ? sizeof(float) * (size_t)S.ftmax * (nv == 5 ? 2 : 1) : 0);
+}
+// one launch over the (member, tile) work items of many meshes / conditions;
+// items padded with member -1 to a multiple of 8 blocks; lds = the largest
+// member's tile_lds_bytes_f32; sa: every member runs the SA model
+hipError_t launch_residual_tile_f32_batch(hipStream_t s, const TileArgsF32* members, const int2* items, int32_t nitems,
+                                          size_t lds, bool sa);
+
+// SA explicit step of one solver: qk = q + R / A dt coef (real cells, nv planes), qW = R / A
+hipError_t launch_rk_update_area(hipStream_t s, const double* q, const double* R, const double* area, const double* dt,
+                                 double coef, int32_t nC, int32_t nT, int nv, double* qk, double* qW);
 hipError_t launch_to_f32x4(hipStream_t s, const double* src, int32_t nT, float4* dst);
 
 // ---- launchers (instantiated for double in afx_k64.hip, float in afx_k32.hip)
 template <typename T>
 hipError_t launch_set_walls(hipStream_t s, const int32_t* owner, const int32_t* ghost, int32_t n, T* q, int32_t nT, int nv);
 
+// calc_dt (solver.h:308-356); sa = 1: plus the viscous spectral radius of
+// the SA model (q with five planes)
 template <typename T>
-hipError_t launch_dt(hipStream_t s, CellsT<T> C, FacesT<T> F, const T* q, GasT<T> g, T cfl, T* dt);
+hipError_t launch_dt(hipStream_t s, CellsT<T> C, FacesT<T> F, const T* q, GasT<T> g, T cfl, T* dt, int sa = 0);
 
 // gradients from qg (solver.h:425-514) and limiters from ql (solver.h:517-593)
 template <typename T>

@@ -473,17 +473,35 @@ Elements gen_jittered_airfoil(int ni, int nj, double r_far, double thickness, do
     add_names(el);
     SplitMix64 rng(seed);
     const double PI = 3.141592653589793;
-    // jitter interior nodes: sigma = jitter * local spacing in each grid direction,
-    // Gaussian clipped at 2 sigma so that cells cannot fold
+    // jitter interior nodes: sigma = jitter * h, h = the smallest altitude
+    // (area / longest edge) of the node's four grid quads before jittering,
+    // Gaussian clipped at 2 sigma (the grid spacings alone overestimate the
+    // room in the skewed cells at a sharp trailing edge); the cells built
+    // below are checked for a single orientation and convexity
+    auto quad_alt = [&](int i, int j) {   // grid quad with lower-left node (i, j)
+        const size_t a = (size_t)j * ni + i, b = (size_t)j * ni + (i + 1) % ni;
+        const size_t c = (size_t)(j + 1) * ni + (i + 1) % ni, d = (size_t)(j + 1) * ni + i;
+        const double area = 0.5 * std::fabs((el.x[c] - el.x[a]) * (el.y[d] - el.y[b]) -
+                                            (el.y[c] - el.y[a]) * (el.x[d] - el.x[b]));
+        const size_t v[4] = {a, b, c, d};
+        double lmax = 0.0;
+        for (int e = 0; e < 4; ++e) {
+            const double dx = el.x[v[(e + 1) % 4]] - el.x[v[e]], dy = el.y[v[(e + 1) % 4]] - el.y[v[e]];
+            lmax = std::max(lmax, std::sqrt(dx * dx + dy * dy));
+        }
+        return area / lmax;
+    };
+    std::vector<double> halt((size_t)ni * nj, 0.0);
+    for (int j = 1; j < nj - 1; ++j)
+        for (int i = 0; i < ni; ++i) {
+            const int im = (i + ni - 1) % ni;
+            halt[(size_t)j * ni + i] = std::min(std::min(quad_alt(im, j - 1), quad_alt(i, j - 1)),
+                                                std::min(quad_alt(im, j), quad_alt(i, j)));
+        }
     for (int j = 1; j < nj - 1; ++j) {
         for (int i = 0; i < ni; ++i) {
             size_t k = (size_t)j * ni + i;
-            size_t kp = (size_t)j * ni + (i + 1) % ni, km = (size_t)j * ni + (i + ni - 1) % ni;
-            size_t ku = (size_t)(j + 1) * ni + i, kd = (size_t)(j - 1) * ni + i;
-            double hi_x = 0.5 * (el.x[kp] - el.x[km]), hi_y = 0.5 * (el.y[kp] - el.y[km]);
-            double hj_x = 0.5 * (el.x[ku] - el.x[kd]), hj_y = 0.5 * (el.y[ku] - el.y[kd]);
-            double hi = std::sqrt(hi_x * hi_x + hi_y * hi_y), hj = std::sqrt(hj_x * hj_x + hj_y * hj_y);
-            double h = std::min(hi, hj);
+            double h = halt[k];
             double gx = rng.normal(), gy = rng.normal();
             gx = std::max(-2.0, std::min(2.0, gx));
             gy = std::max(-2.0, std::min(2.0, gy));
@@ -514,6 +532,21 @@ Elements gen_jittered_airfoil(int ni, int nj, double r_far, double thickness, do
                 el.cell_nodes.insert(el.cell_nodes.end(), t1, t1 + 4); el.cell_nv.push_back(3);
                 el.cell_nodes.insert(el.cell_nodes.end(), t2, t2 + 4); el.cell_nv.push_back(3);
             }
+        }
+    }
+    // every cell must keep the orientation of the unjittered grid and quads
+    // must stay convex: all corner cross products of one sign
+    int sign0 = 0;
+    for (size_t c = 0; c < el.cell_nv.size(); ++c) {
+        const int nv = el.cell_nv[c];
+        const uint32_t* v = &el.cell_nodes[4 * c];
+        for (int k = 0; k < nv; ++k) {
+            const uint32_t a = v[k], b = v[(k + 1) % nv], d = v[(k + 2) % nv];
+            const double o = orient(el.x[a], el.y[a], el.x[b], el.y[b], el.x[d], el.y[d]);
+            const int sg = o > 0 ? 1 : (o < 0 ? -1 : 0);
+            if (sign0 == 0) sign0 = sg;
+            if (sg == 0 || sg != sign0)
+                throw std::runtime_error("jittered mesh has a folded or non-convex cell; lower the jitter");
         }
     }
     for (int i = 0; i < ni; ++i) {

@@ -102,6 +102,13 @@ def _load():
         "afx_rans_set_nu": (C.c_int, [P, dp]),
         "afx_rans_get_nu": (C.c_int, [P, dp]),
         "afx_rans_nu_residual": (C.c_int, [P, dp]),
+        "afx_rans_create_like": (C.c_int, [P, C.POINTER(AfxGas), C.POINTER(P)]),
+        "afx_rans_time_residual": (C.c_int, [P, i32, i32, dp]),
+        "afx_batch_create": (C.c_int, [i32, P, f64, C.POINTER(P)]),
+        "afx_batch_run": (C.c_int, [P, i32, i32]),
+        "afx_batch_residual_norms": (C.c_int, [P, i32, dp]),
+        "afx_batch_stream": (C.c_int, [P, C.POINTER(P)]),
+        "afx_batch_destroy": (C.c_int, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -298,6 +305,7 @@ class Solver:
         self._push_opts()
 
     def set_cfl(self, cfl):
+        self.cfl = cfl
         _check(lib.afx_rans_set_cfl(self.h, cfl))
 
     def set_bcs(self, bcs):
@@ -437,10 +445,107 @@ class Solver:
     def synchronize(self):
         _check(lib.afx_rans_synchronize(self.h))
 
+    def like(self, gas=None):
+        """A second flow condition on this solver's mesh and tiles (no
+        re-tiling); BCs and state are set on the new solver as usual."""
+        other = Solver.__new__(Solver)
+        other.gas = gas if gas is not None else self.gas
+        other.mesh = self.mesh
+        other._opts = AfxOpts()
+        C.memmove(C.byref(other._opts), C.byref(self._opts), C.sizeof(AfxOpts))
+        g = other.gas.c_struct()
+        h = C.c_void_p()
+        _check(lib.afx_rans_create_like(self.h, C.byref(g), C.byref(h)))
+        other.h = h
+        other.n_real, other.n_cells, other.n_faces, other.n_bfaces = (self.n_real, self.n_cells, self.n_faces,
+                                                                      self.n_bfaces)
+        other.bcs = {}
+        return other
+
+    def clone(self):
+        """Same mesh, tiles, gas, BCs, CFL and state (like() + copies)."""
+        c = self.like(self.gas)
+        c.set_bcs(self.bcs)
+        c.set_q(self.get_q())
+        if self._opts.viscosity == 2:
+            c.set_nu(self.get_nu())
+        c.set_cfl(getattr(self, "cfl", 1.0))
+        return c
+
+    def time_residual(self, precision=AFX_FP32, reps=20):
+        """Average ms of one residual evaluation (HIP events)."""
+        ms = C.c_double()
+        _check(lib.afx_rans_time_residual(self.h, precision, reps, C.byref(ms)))
+        return ms.value
+
     def __del__(self):
         if getattr(self, "h", None) is not None and lib is not None:
             lib.afx_rans_destroy(self.h)
             self.h = None
+
+
+class Batch:
+    """Explicit pseudo-time steps of many solvers at once (FP32 fused
+    residuals of all members in one launch per stage, FP64 updates,
+    hipGraph replay).  Keeps references to its members."""
+
+    def __init__(self, solvers, relaxation=1.0):
+        self.members = list(solvers)
+        arr = (C.c_void_p * len(self.members))(*[m.h.value if isinstance(m.h, C.c_void_p) else m.h
+                                                 for m in self.members])
+        h = C.c_void_p()
+        _check(lib.afx_batch_create(len(self.members), arr, relaxation, C.byref(h)))
+        self.h = h
+
+    def run(self, steps, graph_steps=25):
+        _check(lib.afx_batch_run(self.h, steps, graph_steps))
+
+    def residual_norms(self, precision=AFX_FP32):
+        out = np.empty(len(self.members))
+        _check(lib.afx_batch_residual_norms(self.h, precision, _dptr(out)))
+        return out
+
+    def stream(self):
+        s = C.c_void_p()
+        _check(lib.afx_batch_stream(self.h, C.byref(s)))
+        return s.value
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and lib is not None:
+            lib.afx_batch_destroy(self.h)
+            self.h = None
+
+
+def autotune_tile(make_solver, candidates=(128, 160, 192, 224, 256), precision=AFX_FP32, reps=20, replicas=1):
+    """Tile-size autotuner: build the solver at each candidate tile size with
+    make_solver(tile_cells) (a fully set-up solver: BCs and state) and time
+    it, returning (best tile, {tile: ms}).  replicas == 1: the fused residual
+    (HIP events); replicas > 1: one batched explicit step of that many copies
+    of the solver (the work of a multi-condition sweep, where the GPU is
+    filled by all members together).  Candidates whose tiles do not fit the
+    kernel are skipped."""
+    import time as _time
+    times = {}
+    for t in candidates:
+        s = None
+        try:
+            s = make_solver(t)
+            if replicas <= 1:
+                times[t] = s.time_residual(precision, reps)
+            else:
+                b = Batch([s] + [s.clone() for _ in range(replicas - 1)])
+                b.run(2, 0)
+                t0 = _time.perf_counter()
+                b.run(reps, 0)
+                times[t] = (_time.perf_counter() - t0) * 1e3 / reps
+                b = None
+        except RansError:
+            continue
+        finally:
+            s = None
+    if not times:
+        raise RansError("no candidate tile size fits the fused kernel")
+    return min(times, key=times.get), times
 
 
 class Settings:

@@ -227,6 +227,115 @@ def cpu_baseline(rans, model, budget_s=12.0):
                       f"of the same generator and state, {el:.1f} s"}
 
 
+SWEEP_CFL = 0.3
+
+
+def sweep_solver(rans, mesh, mach, alpha_deg, reynolds, tile_cells=0, base=None):
+    """SA solver for one condition (mu_L from Re on the unit chord), state =
+    freestream with the 1% perturbations of the headline workload."""
+    g = rans.Gas()
+    far = rans.BoundaryVariables(mach=mach, angle=np.deg2rad(alpha_deg))
+    qinf = far.get_conservative(g)
+    gas = rans.Gas(mu_L=float(np.hypot(qinf[1], qinf[2]) / reynolds))
+    if base is None:
+        s = rans.Solver(mesh, gas=gas, viscosity_model="spalart-allmaras", tile_cells=tile_cells)
+    else:
+        s = base.like(gas)
+    s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition("wall")})
+    s.init()
+    s.refill_bcs()
+    q, nu = perturbed(s.get_q(), s.get_nu(), s.n_real, qinf, g.gamma, np.random.default_rng(2))
+    s.set_q(q)
+    s.set_nu(nu)
+    s.set_cfl(SWEEP_CFL)
+    return s
+
+
+def batched_sweep(rans, steps, graph_steps=25, n_cond=64, n_mesh=24, cond_cells=250_000):
+    """64 conditions on one 250k-cell mesh (alpha ~ U[-4, 8] deg, M ~ U[0.3,
+    0.75], Re log-uniform in [1e6, 1e7], seed 4) plus 24 meshes of log-uniform
+    size in [2e4, 5e5] cells (seed 5; M 0.5, alpha 2 deg, Re 1e6; first-layer
+    height 2e-3 sqrt(16384 / cells)), every mesh's tile size autotuned; all
+    advanced together (FP32 fused residuals of all members in one launch per
+    stage, FP64 updates, hipGraph replay) against the plain form: one FP64
+    explicit solve per condition, in sequence, for the same number of steps."""
+    t_setup = time.perf_counter()
+    rng = np.random.default_rng(4)
+    alpha = rng.uniform(-4.0, 8.0, n_cond)
+    mach = rng.uniform(0.3, 0.75, n_cond)
+    reyn = 10 ** rng.uniform(6.0, 7.0, n_cond)
+    cands = (128, 160, 192, 224)
+    ni, nj = mesh_dims(cond_cells)
+    m0 = rans.Mesh.jittered(ni, nj, r_far=20.0, thickness=0.12, first_h=1e-4, jitter=0.1, quad_frac=0.25, seed=1)
+    tile0, _ = rans.autotune_tile(lambda t: sweep_solver(rans, m0, mach[0], alpha[0], reyn[0], t), cands,
+                                  reps=4, replicas=8)
+    conds = [sweep_solver(rans, m0, mach[0], alpha[0], reyn[0], tile0)]
+    conds += [sweep_solver(rans, m0, mach[i], alpha[i], reyn[i], base=conds[0]) for i in range(1, n_cond)]
+    rng5 = np.random.default_rng(5)
+    sizes = 10 ** rng5.uniform(np.log10(2e4), np.log10(5e5), n_mesh)
+    uneven, tiles = [], []
+    for k, nc in enumerate(sizes):
+        a, b = mesh_dims(int(nc))
+        # first-layer height scaled with the resolution (2e-3 at 16k cells, the
+        # O-grid's), so that coarse meshes keep explicit-stable aspect ratios
+        fh = 2e-3 * np.sqrt(16384.0 / nc)
+        mk = rans.Mesh.jittered(a, b, r_far=20.0, thickness=0.12, first_h=fh, jitter=0.1, quad_frac=0.25,
+                                seed=100 + k)
+        tk, _ = rans.autotune_tile(lambda t: sweep_solver(rans, mk, 0.5, 2.0, 1e6, t), (128, 192), reps=4,
+                                   replicas=4)
+        tiles.append(tk)
+        uneven.append(sweep_solver(rans, mk, 0.5, 2.0, 1e6, tk))
+    t_setup = time.perf_counter() - t_setup
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
+    # every timed phase starts from the same freestream states
+    init = [(m.get_q(), m.get_nu()) for m in conds + uneven]
+
+    def reset():
+        for m, (q, nu) in zip(conds + uneven, init):
+            m.set_q(q)
+            m.set_nu(nu)
+
+    everyone = rans.Batch(conds + uneven, relaxation=0.8)
+    everyone.run(graph_steps, graph_steps)   # warm-up + graph capture
+    reset()
+    t_all = timed(lambda: everyone.run(steps, graph_steps))
+    norms = everyone.residual_norms(rans.AFX_FP64)
+    only = rans.Batch(conds, relaxation=0.8)
+    only.run(graph_steps, graph_steps)
+    reset()
+    t_cond = timed(lambda: only.run(steps, graph_steps))
+    conds[0].explicit_solve(0.8)             # warm-up of the plain form
+    reset()
+    t_plain = timed(lambda: [c.explicit_solve(0.8) for c in conds for _ in range(steps)])
+    cells_all = sum(m.n_real for m in conds + uneven)
+    cells_cond = sum(m.n_real for m in conds)
+    return {
+        "members": len(conds) + len(uneven), "conditions": n_cond, "condition_mesh_cells": conds[0].n_real,
+        "uneven_mesh_cells": [m.n_real for m in uneven], "tiles": {"condition_mesh": tile0, "uneven": tiles},
+        "steps": steps, "graph_steps": graph_steps, "stages_per_step": 3,
+        "batched_all_s": round(t_all, 4),
+        "batched_all_Mcell_res_per_s": round(3 * steps * cells_all / t_all / 1e6, 1),
+        "batched_64_conditions_s": round(t_cond, 4),
+        "plain_64_sequential_s": round(t_plain, 4),
+        "speedup_vs_plain_sequential": round(t_plain / t_cond, 2),
+        "plain_Mcell_res_per_s": round(3 * steps * cells_cond / t_plain / 1e6, 1),
+        "batched_64_Mcell_res_per_s": round(3 * steps * cells_cond / t_cond / 1e6, 1),
+        "finite_final_residuals": bool(np.all(np.isfinite(norms))),
+        "nonfinite_members": [int(i) for i in np.where(~np.isfinite(norms))[0]],
+        "setup_s": round(t_setup, 1),
+        "note": f"explicit three-stage pseudo-time steps (SA, CFL {SWEEP_CFL}, relaxation 0.8) from freestream; "
+                "plain = FP64 fused residual "
+                "+ FP64 update per condition with a norm per step, batched = one launch per stage over all "
+                "members (FP32 residual, FP64 stage update), replayed from a hipGraph; tiles autotuned on the "
+                "batched step",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +349,8 @@ def main():
                     help="also run D2D copies of this size (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--only-fp32", action="store_true", help="skip per-kernel and FP64 timings (profiling runs)")
     ap.add_argument("--model", default="spalart-allmaras", choices=["spalart-allmaras", "inviscid"])
+    ap.add_argument("--sweep-steps", type=int, default=50,
+                    help="pseudo-time steps of the batched-sweep comparison (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -332,6 +443,9 @@ def main():
                        "roofline_frac": round(b64 / (t64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "fp32_three_kernel_ms": None if unf is None else {k: round(v, 5) for k, v in unf.items()},
     }
+    if args.sweep_steps > 0:
+        del s, m
+        out["batched_sweep"] = batched_sweep(rans, args.sweep_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(rans, args.model)
     else:

@@ -44,6 +44,7 @@ extern "C" {
 
 typedef struct afx_mesh afx_mesh;
 typedef struct afx_rans afx_rans;
+typedef struct afx_batch afx_batch;
 
 /* core.h:27-46 (rans::gas; cp is fixed to 1 as in the reference) */
 typedef struct afx_gas {
@@ -206,6 +207,33 @@ int afx_rans_set_turbulence(afx_rans* h, double nu_ratio);
 int afx_rans_set_nu(afx_rans* h, const double* nu);
 int afx_rans_get_nu(afx_rans* h, double* nu);
 int afx_rans_nu_residual(afx_rans* h, double* Rnu);
+
+/* ------------------------------------------- conditions, batching, tuning --
+ * No counterpart in the reference, whose Rans facade runs one condition at a
+ * time (rans.h:78-106 sweeps alpha by re-solving).
+ * create_like: a second flow condition on the mesh of `base` (same options,
+ * tiles and cell order; no re-tiling), with its own gas (e.g. mu_L from Re);
+ * set its BCs / state as for afx_rans_create.
+ * time_residual: average duration (ms, HIP events on the handle's stream) of
+ * `reps` back-to-back residual evaluations - the measurement the tile
+ * autotuner compares across tile sizes. */
+int afx_rans_create_like(const afx_rans* base, const afx_gas* g, afx_rans** out);
+int afx_rans_time_residual(afx_rans* h, int32_t precision, int32_t reps, double* ms);
+/* Batched explicit pseudo-time stepping of many solvers (conditions and/or
+ * meshes of any size; all with the same number of variables): every step is
+ * the three-stage scheme of explicitSolver::solve (solver.h:802-828) with
+ * each member's CFL (afx_rans_set_cfl) and the given relaxation; the FP32
+ * fused residuals of ALL members run in one kernel launch per stage and the
+ * FP64 stage updates in one more.  run: `steps` pseudo-time steps, replaying
+ * a hipGraph of graph_steps captured steps (0: plain launches).  Members'
+ * FP64 states (afx_rans_get_state / get_nu) hold the result; their ghost
+ * rows are refreshed by the next residual.  residual_norms: one residual per
+ * member in the given precision, its 2-norm in norms[n]. */
+int afx_batch_create(int32_t n, afx_rans* const* members, double relaxation, afx_batch** out);
+int afx_batch_run(afx_batch* b, int32_t steps, int32_t graph_steps);
+int afx_batch_residual_norms(afx_batch* b, int32_t precision, double* norms);
+int afx_batch_stream(afx_batch* b, void** stream);
+int afx_batch_destroy(afx_batch* b);
 /* The handle's hipStream_t (as void*), for event timing by the caller. */
 int afx_rans_stream(afx_rans* h, void** stream);
 /* Enqueue n back-to-back launches of one residual stage without

@@ -622,7 +622,11 @@ struct Solver {
         }
         for (size_t i = 0; i < dt.size(); ++i) dt[i] = cfl * m.cellsAreas[i] / dt[i];
     }
-    // solver.h:359-398 (A1: direction = normalised SUM of centroids; reads member q)
+    // solver.h:359-398 (A1: direction = normalised SUM of centroids; reads
+    // member q).  a1_sum_direction = false: the textbook correction along the
+    // cell connection, t = (x1 - x0)/|x1 - x0|, derivative (q1 - q0)/|x1 - x0|
+    // (used by the Spalart-Allmaras model, not by the reference's laminar flux)
+    bool a1_sum_direction = true;
     void average_gradients(double gxo[4], double gyo[4], u32 c0, u32 c1) const {
         if (c0 == c1) {
             for (int i = 0; i < 4; ++i) { gxo[i] = gx[4 * c0 + i]; gyo[i] = gy[4 * c0 + i]; }
@@ -630,10 +634,14 @@ struct Solver {
         }
         double t0 = m.cellsCentersX[c0] + m.cellsCentersX[c1];
         double t1 = m.cellsCentersY[c0] + m.cellsCentersY[c1];
+        if (!a1_sum_direction) {
+            t0 = m.cellsCentersX[c1] - m.cellsCentersX[c0];
+            t1 = m.cellsCentersY[c1] - m.cellsCentersY[c0];
+        }
         const double l = std::sqrt(t0 * t0 + t1 * t1);
         t0 /= l; t1 /= l;
         for (int i = 0; i < 4; ++i) {
-            double dir = (q[4 * c0 + i] - q[4 * c1 + i]) / l;
+            double dir = a1_sum_direction ? (q[4 * c0 + i] - q[4 * c1 + i]) / l : (q[4 * c1 + i] - q[4 * c0 + i]) / l;
             double gxb = (gx[4 * c0 + i] + gx[4 * c1 + i]) * 0.5;
             double gyb = (gy[4 * c0 + i] + gy[4 * c1 + i]) * 0.5;
             const double dot = gxb * t0 + gyb * t1;
@@ -971,13 +979,20 @@ struct Solver {
 //   * gradients of rho nu~ by the same Green-Gauss / least-squares operator
 //     as q (solver.h:425-514), limiter of solver.h:517-593, MUSCL reconstruction
 //     of solver.h:1097-1120;
-//   * face gradient of rho nu~ by average_gradients (solver.h:359-398, A1);
+//   * face gradients of all five variables by average_gradients
+//     (solver.h:359-398) with the textbook direction t = (x1 - x0)/|x1 - x0|
+//     and derivative (q1 - q0)/|x1 - x0| in place of the sum of centroids
+//     (A1), which makes the viscous operator non-dissipative once mu_t is
+//     hundreds of mu_L;
 //   * convection: rho nu~ upwinded by the sign of the Roe mass flux;
 //   * mean-flow viscous terms of physics.h:230-257 with mu_L + mu_t,
 //     mu_t = rho nu~ f_v1, k = cp (mu_L/Pr_L + mu_t/Pr_T);
-//   * boundary faces carry no gradient terms, like the reference's boundary
-//     functors (A6): wall ghost rho nu~ = -rho nu~ (nu~ = 0 on the wall face),
-//     slip wall = interior, farfield = ghost nu~ on inflow, interior on outflow;
+//   * boundary faces carry no mean-flow gradient terms, like the reference's
+//     boundary functors (A6): wall ghost rho nu~ = -rho nu~ (nu~ = 0 on the
+//     wall face), slip wall = interior, farfield = ghost nu~ on inflow,
+//     interior on outflow; the nu~ = 0 wall condition enters the fifth
+//     equation as the diffusion flux (mu_L/sigma) (0 - nu~_c)/d_n through
+//     no-slip wall faces (d_n: centroid-to-face normal distance);
 //   * sources with the cell's unlimited gradients and wall distance
 //     (mesh.h:794-830).
 // At rho nu~ = 0 everywhere the mean-flow rows equal the laminar residual.
@@ -1025,13 +1040,14 @@ inline double source(double rho, double nut, double mu, double omega, double d, 
 }
 }  // namespace sa
 
-enum { SA_CONV = 1, SA_DIFF = 2, SA_SOURCE = 4, SA_MUT = 8, SA_ALL = 15 };
+enum { SA_CONV = 1, SA_DIFF = 2, SA_SOURCE = 4, SA_MUT = 8, SA_WALL = 16, SA_ALL = 31 };
 
 struct SASolver {
     Solver* s;
     std::vector<double> nu, gnx, gny, lnu, TurbVector;   // rho nu~ per cell (the reference's nu)
     int terms = SA_ALL;                                   // for tests: which terms enter
     explicit SASolver(Solver* s_) : s(s_) {
+        s->a1_sum_direction = false;   // textbook face gradients for every viscous term of the model
         const size_t N = s->m.nCells();
         nu.assign(N, 0.0); gnx.assign(N, 0.0); gny.assign(N, 0.0); lnu.assign(N, 1.0); TurbVector.assign(N, 0.0);
     }
@@ -1132,6 +1148,15 @@ struct SASolver {
             const double nb = vars_nu(e, qL, nL, qR, nR);
             roe_flux(inv, qL, qb, zero, zero, f);
             f[4] = (terms & SA_CONV) ? f[0] * (f[0] >= 0 ? nL / qL[0] : nb / qb[0]) : 0.0;
+            if (F.kind == BC_WALL && (terms & SA_WALL)) {
+                // nu~ = 0 on a no-slip wall: one-sided normal derivative from
+                // the cell centre, (0 - nu~_c) / d_n, d_n = (x_f - x_c) . n
+                const Mesh& m = S.m;
+                const double dn = (m.edgesCentersX[e] - m.cellsCentersX[c0]) * F.nx +
+                                  (m.edgesCentersY[e] - m.cellsCentersY[c0]) * F.ny;
+                const double nut_c = nu[c0] / S.q[4 * c0];
+                f[4] -= (g.mu_L / sa::sigma) * ((0.0 - nut_c) / dn);
+            }
             return;
         }
         roe_flux(inv, qL, qR, zero, zero, f);
@@ -1161,13 +1186,13 @@ struct SASolver {
             f[3] -= F.nx * phi0 + F.ny * phi1;
         }
         if (terms & SA_DIFF) {
-            // face gradient of rho nu~ (solver.h:359-398), then
-            // grad nu~ = (grad(rho nu~) - nu~ grad rho) / rho
+            // face gradient of rho nu~ (average_gradients with the textbook
+            // direction), then grad nu~ = (grad(rho nu~) - nu~ grad rho) / rho
             const Mesh& m = S.m;
-            double t0 = m.cellsCentersX[c0] + m.cellsCentersX[c1], t1 = m.cellsCentersY[c0] + m.cellsCentersY[c1];
+            double t0 = m.cellsCentersX[c1] - m.cellsCentersX[c0], t1 = m.cellsCentersY[c1] - m.cellsCentersY[c0];
             const double l = std::sqrt(t0 * t0 + t1 * t1);
             t0 /= l; t1 /= l;
-            const double dir = (nu[c0] - nu[c1]) / l;
+            const double dir = (nu[c1] - nu[c0]) / l;
             const double gxb = (gnx[c0] + gnx[c1]) * 0.5, gyb = (gny[c0] + gny[c1]) * 0.5;
             const double dot = gxb * t0 + gyb * t1;
             const double gfx = gxb - (dot - dir) * t0, gfy = gyb - (dot - dir) * t1;
@@ -1228,6 +1253,44 @@ struct SASolver {
             TurbVector[i] = 0;
         }
         (void)g;
+    }
+    // local time step of the SA explicit step: calc_dt (solver.h:308-356) plus
+    // the viscous spectral radius 4 sum_e coef_e len_e^2 / A_i with
+    // coef = max(4/3, gamma) (mu_L/Pr_L + mu_t/Pr_T) / rho at the face average
+    // (owner state on boundary faces) - Blazek, Computational Fluid Dynamics,
+    // eq. 6.22 with C = 4
+    void calc_dt() {
+        Solver& S = *s;
+        const Mesh& m = S.m;
+        const Gas& g = S.g;
+        std::vector<double> sc(m.nCells(), 0.0), sv(m.nCells(), 0.0);
+        for (u32 e = 0; e < m.nFaces(); ++e) {
+            const u32 c0 = m.edgesCells[2 * e], c1 = m.edgesCells[2 * e + 1];
+            const double* q0 = &S.q[4 * c0];
+            const double* q1 = &S.q[4 * c1];
+            const double n0 = m.edgesNormalsX[e], n1 = m.edgesNormalsY[e];
+            const bool two = S.ff[e].two_sided();
+            const double V_L = (q0[1] * n0 + q0[2] * n1) / q0[0];
+            const double p_L = (g.gamma - 1) * (q0[3] - 0.5 / q0[0] * (q0[1] * q0[1] + q0[2] * q0[2]));
+            double eig = std::sqrt(p_L * g.gamma / q0[0]) + std::fabs(V_L);
+            double rho = q0[0], nuf = nu[c0];
+            if (two) {
+                const double V_R = (q1[1] * n0 + q1[2] * n1) / q1[0];
+                const double p_R = (g.gamma - 1) * (q1[3] - 0.5 / q1[0] * (q1[1] * q1[1] + q1[2] * q1[2]));
+                eig = std::max(eig, std::sqrt(p_R * g.gamma / q1[0]) + std::fabs(V_R));
+                rho = 0.5 * (rho + q1[0]);
+                nuf = 0.5 * (nuf + nu[c1]);
+            }
+            const double mut = nuf * sa::fv1(nuf / g.mu_L);
+            const double kmax = g.gamma > 4.0 / 3.0 ? g.gamma : 4.0 / 3.0;
+            const double coef = kmax * (g.mu_L / g.Pr_L + mut / g.Pr_T) / rho;
+            const double len = m.edgesLengths[e];
+            sc[c0] += eig * len;
+            sv[c0] += coef * len * len;
+            if (two) { sc[c1] += eig * len; sv[c1] += coef * len * len; }
+        }
+        for (size_t i = 0; i < S.dt.size(); ++i)
+            S.dt[i] = S.cfl * m.cellsAreas[i] / (sc[i] + 4.0 * (sv[i] / m.cellsAreas[i]));
     }
     // per-volume source of cell i from its unlimited gradients
     double cell_source(u32 i) const {
@@ -1363,6 +1426,12 @@ void* orc_solver_create(void* mh, double R, double mu_L, double Pr_L, double gam
     return s;
 }
 void orc_solver_set_prt(void* h, double Pr_T) { ((Solver*)h)->g.Pr_T = Pr_T; }
+void orc_solver_set_a1(void* h, int on) { ((Solver*)h)->a1_sum_direction = on != 0; }
+// average_gradients of face e from the current gx, gy, q: out = gx[4], gy[4]
+void orc_solver_face_gradient(void* h, int e, double* out) {
+    Solver* s = (Solver*)h;
+    s->average_gradients(out, out + 4, s->m.edgesCells[2 * e], s->m.edgesCells[2 * e + 1]);
+}
 void orc_solver_free(void* h) { delete (Solver*)h; }
 
 // bcs: nb entries; names '\n'-separated; types BC_*; per-entry farfield vars (mach, angle, T, p)
@@ -1406,6 +1475,7 @@ double orc_solver_explicit_step(void* h, double relax) { return ((Solver*)h)->ex
 void* orc_sa_create(void* solver_h) { return new SASolver((Solver*)solver_h); }
 void orc_sa_free(void* h) { delete (SASolver*)h; }
 void orc_sa_set_terms(void* h, int terms) { ((SASolver*)h)->terms = terms; }
+void orc_sa_calc_dt(void* h) { ((SASolver*)h)->calc_dt(); }
 void orc_sa_fill_rhs(void* h) {
     SASolver* a = (SASolver*)h;
     bool have_d = false;

@@ -51,9 +51,12 @@ def _load():
         "orc_face_vars": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp]),
         "orc_freestream": (None, [C.c_double] * 6 + [dp]),
         "orc_solver_set_prt": (None, [P, C.c_double]),
+        "orc_solver_set_a1": (None, [P, C.c_int]),
+        "orc_solver_face_gradient": (None, [P, C.c_int, dp]),
         "orc_sa_create": (P, [P]),
         "orc_sa_free": (None, [P]),
         "orc_sa_set_terms": (None, [P, C.c_int]),
+        "orc_sa_calc_dt": (None, [P]),
         "orc_sa_fill_rhs": (None, [P]),
         "orc_sa_vec": (C.c_long, [P, C.c_char_p, dp, dp]),
         "orc_sa_source": (C.c_double, [C.c_double] * 7),
@@ -124,7 +127,7 @@ class OMesh:
             self.h = None
 
 
-SA_CONV, SA_DIFF, SA_SOURCE, SA_MUT, SA_ALL = 1, 2, 4, 8, 15
+SA_CONV, SA_DIFF, SA_SOURCE, SA_MUT, SA_WALL, SA_ALL = 1, 2, 4, 8, 16, 31
 
 
 class OSolver:
@@ -144,6 +147,11 @@ class OSolver:
         lib.orc_sa_vec(self.sa, name.encode(), _d(out),
                        None if value is None else _d(np.ascontiguousarray(value, np.float64)))
         return out
+
+    def sa_calc_dt(self):
+        """Local time step of the SA explicit step (calc_dt + viscous radius)."""
+        lib.orc_sa_calc_dt(self.sa)
+        return self.vec("dt")
 
     def sa_terms(self, terms):
         lib.orc_sa_set_terms(self.sa, terms)

@@ -205,3 +205,24 @@ def test_python_mirror_rejects_unknown_names():
         r.BoundaryCondition("inlet-outlet")
     with pytest.raises(r.RansError):
         r.Solver.__init__(object.__new__(r.Solver), None, viscosity_model="spalart")
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_jittered_mesh_has_no_folded_cells(seed):
+    """Every corner cross product of every cell has the same sign (no
+    inverted triangle, no non-convex quad), also in the skewed trailing-edge
+    cells of a 1e-4 first layer."""
+    m = rans().Mesh.jittered(256, 120, first_h=1e-4, jitter=0.1, quad_frac=0.3, seed=seed)
+    nr = m.counts()["real_cells"]
+    cn = m.get("cellsNodes").reshape(-1, 4)[:nr].astype(np.int64)
+    tri = m.get("cellsIsTriangle")[:nr] == 1
+    x, y = m.get("nodesX"), m.get("nodesY")
+    nv = np.where(tri, 3, 4)
+    signs = []
+    for k in range(4):
+        a = cn[np.arange(nr), k % nv]
+        b = cn[np.arange(nr), (k + 1) % nv]
+        c = cn[np.arange(nr), (k + 2) % nv]
+        signs.append(np.sign((x[b] - x[a]) * (y[c] - y[a]) - (y[b] - y[a]) * (x[c] - x[a])))
+    signs = np.array(signs)
+    assert np.all(signs == signs[0, 0])
