@@ -23,6 +23,7 @@ UNITS = [
     ("afx_k64.hip", ["-ffp-contract=off"]),     # plain FP64 form: reference rounding order
     ("afx_k32.hip", ["-ffp-contract=fast"]),    # FP32 throughput form
     ("afx_implicit.hip", ["-ffp-contract=off"]),
+    ("afx_adjoint.hip", ["-ffp-contract=off"]),
     ("afx_capi.hip", ["-ffp-contract=off"]),
 ]
 CXX_UNITS = [("afx_mesh.cpp", ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]),

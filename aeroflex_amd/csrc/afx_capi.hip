@@ -22,6 +22,7 @@
 #include "afx_mesh.hpp"
 #include "afx_tiles.hpp"
 #include "afx_implicit.hpp"
+#include "afx_adjoint_kernels.hpp"
 
 using namespace afx;
 
@@ -115,6 +116,10 @@ struct afx_rans {
     DBuf<float> q32n, R32n;
     DBuf<float2> sa_ad32;
     DBuf<float> dt32;   // FP32 local time step of the batched explicit form
+    // state adjoint work arrays
+    DBuf<double> a_w, a_lam, a_bL, a_bR, a_lgx, a_lgy, a_nbv, a_co, a_cn;
+    DBuf<int8_t> a_act;
+    DBuf<int32_t> a_jmax, a_jmin, a_nbj;
     DBuf<float4> q32v_b, q32v_c;   // further FP32 stage states of the batched form
     DBuf<float> q32n_b, q32n_c;
 
@@ -1515,6 +1520,66 @@ int afx_rans_launch_stage(afx_rans* h, int32_t precision, int32_t stage, int32_t
             }
         }
     }
+    return AFX_OK;
+}
+
+namespace {
+int adjoint_check(afx_rans* h) {
+    if (h->opts.viscosity != 0) return fail(AFX_EUNSUP, "the state adjoint covers the inviscid residual");
+    if (h->opts.gradient != 1) return fail(AFX_EUNSUP, "the state adjoint covers Green-Gauss gradients");
+    if (!h->a_w.p) {
+        const size_t c4 = (size_t)4 * h->nC, t4 = (size_t)4 * h->nT, f4 = (size_t)4 * h->nF;
+        for (DBuf<double>* b : {&h->a_w, &h->a_lam, &h->a_lgx, &h->a_lgy}) HIPCHK(b->alloc(t4));
+        for (DBuf<double>* b : {&h->a_bL, &h->a_bR, &h->a_co, &h->a_cn}) HIPCHK(b->alloc(f4));
+        HIPCHK(h->a_nbv.alloc(c4));
+        HIPCHK(h->a_act.alloc(c4));
+        HIPCHK(h->a_jmax.alloc(c4)); HIPCHK(h->a_jmin.alloc(c4)); HIPCHK(h->a_nbj.alloc(c4));
+        HIPCHK(hipMemsetAsync(h->a_lam.p, 0, t4 * 8, h->stream));
+    }
+    return AFX_OK;
+}
+int enqueue_adjoint(afx_rans* h) {
+    AdjBufs A{h->gx.p, h->gy.p, h->lim.p, h->a_act.p, h->a_jmax.p, h->a_jmin.p, h->a_bL.p, h->a_bR.p,
+              h->a_lgx.p, h->a_lgy.p, h->a_nbv.p, h->a_nbj.p, h->a_co.p, h->a_cn.p};
+    HIPCHK(launch_adjoint_state(h->stream, h->cells64(), h->faces64(), h->q.p, h->a_w.p, h->gas64(),
+                                h->opts.second_order, h->opts.limiter_k, A, h->a_lam.p));
+    return AFX_OK;
+}
+}  // namespace
+
+int afx_rans_adjoint(afx_rans* h, const double* w, double* lambda) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!w || !lambda) return fail(AFX_EARG, "null vector");
+    if ((rc = adjoint_check(h))) return rc;
+    if ((rc = upload_aos(h, w, h->a_w.p))) return rc;
+    // wall/slip ghosts as fillRhoRHS sees them (they enter no flux, but the
+    // state is then the one the residual reads)
+    HIPCHK(launch_set_walls<double>(h->stream, h->w_owner.p, h->w_ghost.p, h->n_walls, h->q.p, h->nT, NV));
+    if ((rc = enqueue_adjoint(h))) return rc;
+    h->q32_valid = false;
+    return download_aos(h, h->a_lam.p, lambda);
+}
+
+int afx_rans_time_adjoint(afx_rans* h, int32_t reps, double* ms) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (reps < 1 || !ms) return fail(AFX_EARG, "reps must be >= 1 and ms non-null");
+    if ((rc = adjoint_check(h))) return rc;
+    if ((rc = enqueue_adjoint(h))) return rc;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, h->stream));
+    for (int32_t i = 0; i < reps; ++i)
+        if ((rc = enqueue_adjoint(h))) return rc;
+    HIPCHK(hipEventRecord(b, h->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, a, b));
+    HIPCHK(hipEventDestroy(a));
+    HIPCHK(hipEventDestroy(b));
+    *ms = (double)t / reps;
     return AFX_OK;
 }
 

@@ -104,6 +104,8 @@ def _load():
         "afx_rans_nu_residual": (C.c_int, [P, dp]),
         "afx_rans_create_like": (C.c_int, [P, C.POINTER(AfxGas), C.POINTER(P)]),
         "afx_rans_time_residual": (C.c_int, [P, i32, i32, dp]),
+        "afx_rans_adjoint": (C.c_int, [P, dp, dp]),
+        "afx_rans_time_adjoint": (C.c_int, [P, i32, dp]),
         "afx_batch_create": (C.c_int, [i32, P, f64, C.POINTER(P)]),
         "afx_batch_run": (C.c_int, [P, i32, i32]),
         "afx_batch_residual_norms": (C.c_int, [P, i32, dp]),
@@ -471,6 +473,20 @@ class Solver:
             c.set_nu(self.get_nu())
         c.set_cfl(getattr(self, "cfl", 1.0))
         return c
+
+    def adjoint(self, w):
+        """(dR/dq)^T w of fillRhoRHS at the current state (inviscid, Green-Gauss)."""
+        w = np.ascontiguousarray(w, np.float64)
+        if w.shape != (4 * self.n_cells,):
+            raise RansError("adjoint seed has the wrong length")
+        lam = np.empty(4 * self.n_cells)
+        _check(lib.afx_rans_adjoint(self.h, _dptr(w), _dptr(lam)))
+        return lam
+
+    def time_adjoint(self, reps=20):
+        ms = C.c_double()
+        _check(lib.afx_rans_time_adjoint(self.h, reps, C.byref(ms)))
+        return ms.value
 
     def time_residual(self, precision=AFX_FP32, reps=20):
         """Average ms of one residual evaluation (HIP events)."""

@@ -336,6 +336,51 @@ def batched_sweep(rans, steps, graph_steps=25, n_cond=64, n_mesh=24, cond_cells=
     }
 
 
+def adjoint_check(rans, cells=500_000, n_dir=64, h=1e-7):
+    """State adjoint of the FP64 residual on a 500k-cell mesh of the same
+    generator (inviscid, Green-Gauss, second order, M 0.7 alpha 1.5 deg + 1%
+    perturbations): time against the FP64 residual, and central differences
+    of the residual along 64 random unit directions (seed 3) against
+    lambda . v for a random unit seed w."""
+    ni, nj = mesh_dims(cells)
+    m = rans.Mesh.jittered(ni, nj, r_far=20.0, thickness=0.12, first_h=1e-4, jitter=0.1, quad_frac=0.25, seed=1)
+    s = rans.Solver(m, viscosity_model="inviscid", second_order=True, gradient_scheme="green-gauss", limiter_k=5.0)
+    far = rans.BoundaryVariables(mach=MACH, angle=np.deg2rad(ALPHA_DEG))
+    s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition("slip-wall")})
+    s.init()
+    s.refill_bcs()
+    q, _ = perturbed(s.get_q(), None, s.n_real, far.get_conservative(s.gas), s.gas.gamma, np.random.default_rng(2))
+    s.set_q(q)
+    t_primal = s.time_residual(rans.AFX_FP64, 50)
+    t_adj = s.time_adjoint(20)
+    rng = np.random.default_rng(3)
+
+    def unit():
+        x = np.zeros(4 * s.n_cells)
+        x[:4 * s.n_real] = rng.standard_normal(4 * s.n_real)
+        return x / np.linalg.norm(x)
+
+    w = unit()
+    s.set_q(q)
+    lam = s.adjoint(w)
+    scale = max(1.0, float(np.max(np.abs(q))))
+    worst = 0.0
+    for _ in range(n_dir):
+        v = unit()
+        s.set_q(q + h * scale * v)
+        rp = s.fillRhoRHS(rans.AFX_FP64)[0]
+        s.set_q(q - h * scale * v)
+        rm = s.fillRhoRHS(rans.AFX_FP64)[0]
+        fd = float(w @ (rp - rm)) / (2 * h * scale)
+        ad = float(lam @ v)
+        worst = max(worst, abs(ad - fd) / max(abs(fd), 1e-3 * float(np.linalg.norm(lam))))
+    return {"cells": s.n_real, "model": "inviscid, Green-Gauss, second order (active-branch limiter)",
+            "adjoint_ms": round(t_adj, 4), "primal_fp64_ms": round(t_primal, 4),
+            "adjoint_over_primal": round(t_adj / t_primal, 2), "fd_directions": n_dir, "fd_step": h,
+            "fd_max_rel_err": float(f"{worst:.3e}"),
+            "kernels": "5 passes: afx::k_adj_forward, k_adj_face, k_adj_cell1, k_adj_face2, k_adj_cell2"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -351,6 +396,8 @@ def main():
     ap.add_argument("--model", default="spalart-allmaras", choices=["spalart-allmaras", "inviscid"])
     ap.add_argument("--sweep-steps", type=int, default=50,
                     help="pseudo-time steps of the batched-sweep comparison (0: skip it)")
+    ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
+                    help="skip the adjoint timing and finite-difference check")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -443,8 +490,11 @@ def main():
                        "roofline_frac": round(b64 / (t64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "fp32_three_kernel_ms": None if unf is None else {k: round(v, 5) for k, v in unf.items()},
     }
-    if args.sweep_steps > 0:
+    if args.sweep_steps > 0 or args.adjoint:
         del s, m
+    if args.adjoint:
+        out["adjoint"] = adjoint_check(rans)
+    if args.sweep_steps > 0:
         out["batched_sweep"] = batched_sweep(rans, args.sweep_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(rans, args.model)

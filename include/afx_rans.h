@@ -218,6 +218,14 @@ int afx_rans_nu_residual(afx_rans* h, double* Rnu);
  * `reps` back-to-back residual evaluations - the measurement the tile
  * autotuner compares across tile sizes. */
 int afx_rans_create_like(const afx_rans* base, const afx_gas* g, afx_rans** out);
+/* Adjoint of the residual (no counterpart in the reference): lambda =
+ * (dR/dq)^T w for R = fillRhoRHS (solver.h:1079-1152) at the current state,
+ * for the inviscid residual with Green-Gauss gradients, first or second order
+ * (the limiter differentiated on its active branch).  w, lambda: reference
+ * AoS layout 4 * n_cells; ghost rows of w are ignored and of lambda zero.
+ * time_adjoint: average ms of one adjoint evaluation (HIP events). */
+int afx_rans_adjoint(afx_rans* h, const double* w, double* lambda);
+int afx_rans_time_adjoint(afx_rans* h, int32_t reps, double* ms);
 int afx_rans_time_residual(afx_rans* h, int32_t precision, int32_t reps, double* ms);
 /* Batched explicit pseudo-time stepping of many solvers (conditions and/or
  * meshes of any size; all with the same number of variables): every step is
