@@ -110,9 +110,12 @@ protected:
     std::string viscosity_model_;
     std::map<std::string, boundary_condition> bcs_;
 
+    // "spalart-allmaras" is the name solver::set_bcs tests (solver.h:206),
+    // "spallart-allmaras" the one rans::Settings offers (core.h:176,224)
     static int viscosity_code(const std::string& v) {
         if (v == "inviscid") return 0;
         if (v == "laminar") return 1;
+        if (v == "spalart-allmaras" || v == "spallart-allmaras") return 2;
         throw std::invalid_argument("unsupported viscosity model: " + v);
     }
 
@@ -171,6 +174,16 @@ public:
     void set_q(const std::vector<double>& q) {
         if ((int64_t)q.size() != 4 * nT_) throw std::invalid_argument("state has the wrong length");
         check(afx_rans_set_state(h_, q.data()));
+    }
+    // rho nu~ of the Spalart-Allmaras model (the reference's nu, solver.h:55)
+    std::vector<double> get_nu() {
+        std::vector<double> nu(nT_);
+        check(afx_rans_get_nu(h_, nu.data()));
+        return nu;
+    }
+    void set_nu(const std::vector<double>& nu) {
+        if ((int64_t)nu.size() != nT_) throw std::invalid_argument("rho nu~ vector has the wrong length");
+        check(afx_rans_set_nu(h_, nu.data()));
     }
     std::map<std::string, boundary_condition> get_bcs() const { return bcs_; }
     std::string get_viscosity_model() const { return viscosity_model_; }
