@@ -25,9 +25,11 @@ UNITS = [
     ("afx_implicit.hip", ["-ffp-contract=off"]),
     ("afx_adjoint.hip", ["-ffp-contract=off"]),
     ("afx_capi.hip", ["-ffp-contract=off"]),
+    ("afx_vlm.hip", ["-ffp-contract=fast"]),    # vortex lattice, FP32 LU, FP64 refinement
 ]
 CXX_UNITS = [("afx_mesh.cpp", ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]),
-             ("afx_tiles.cpp", ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"])]
+             ("afx_tiles.cpp", ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]),
+             ("afx_structure.cpp", ["-O2", "-std=c++17", "-fPIC"])]
 
 
 def _run(cmd):
@@ -47,7 +49,8 @@ def build(verbose=True):
         _run(["g++"] + flags + ["-c", os.path.join(SRC, src), "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs)
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs +
+         ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     # C++ driver over the C ABI (rans_gpu.hpp), linked against the library
     drv = os.path.join(HERE, "rans_gpu_driver")

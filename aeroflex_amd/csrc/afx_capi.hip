@@ -64,6 +64,11 @@ constexpr int NV_MAX = 5;   // device planes: + rho nu~ of the Spalart-Allmaras 
 
 }  // namespace
 
+namespace afx {
+// error channel shared by the other C-ABI units (afx_vlm.hip, afx_beam.cpp)
+int api_fail(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace afx
+
 struct afx_mesh {
     Mesh m;
 };

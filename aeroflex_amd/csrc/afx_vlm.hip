@@ -1,0 +1,1058 @@
+// Vortex-lattice aerodynamic influence matrix and its dense solve on MI355X
+// (include/afx_vlm.h).
+//
+// Device side of the coupling path of AeroFLEX's src/vlm:
+//   - panel geometry, collocation points, right-hand side and wake rings
+//     (panel.cpp:46-128, surface.cpp:35-58, solver.cpp:256-290);
+//   - the FP64 influence matrix, one thread per matrix entry, four Biot-Savart
+//     segments per ring (potential.cpp:36-58, solver.cpp:139-254);
+//   - FP32 LU with partial pivoting: a cooperative panel kernel (rows of the
+//     panel in LDS, one grid barrier per column), row interchanges, a
+//     wavefront triangular solve for the block row and the trailing update
+//     as an FP32 GEMM on the matrix cores (rocBLAS sgemm);
+//   - FP64 iterative refinement on the FP64 matrix with the FP32 factor as
+//     the preconditioner (replaces Eigen's BiCGSTAB, solver.cpp:41-50);
+//   - Kutta-Joukowski ring forces (surface.cpp:164-201).
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "../../include/afx_vlm.h"
+#include "afx_vlm.hpp"
+
+using afx::api_fail;
+
+namespace {
+
+#define VCHK(x)                                                                                     \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) return api_fail(AFX_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define RBCHK(x)                                                                                    \
+    do {                                                                                            \
+        rocblas_status s_ = (x);                                                                    \
+        if (s_ != rocblas_status_success)                                                           \
+            return api_fail(AFX_EHIP, std::string(#x) + ": " + rocblas_status_to_string(s_));      \
+    } while (0)
+
+constexpr int LU_NB = 128;          // panel width = trailing-update depth = solve block
+constexpr int PANEL_R = 120;        // panel rows per workgroup held in LDS (120 x 128 FP32 = 60 KiB)
+constexpr int MAX_PANEL_WG = 512;   // -> n <= 61440
+constexpr unsigned SPIN_LIMIT = 1u << 21;
+constexpr double FOUR_PI = 12.566370614359172;
+
+// ------------------------------------------------------- grid barrier ----
+struct GridBarrier {
+    unsigned count, gen, abort, pad;
+};
+
+// All workgroups of a cooperative launch meet here.  The wait is bounded:
+// a workgroup that waits longer than SPIN_LIMIT polls raises `abort`, every
+// waiting workgroup leaves on seeing it, and the caller returns, so the grid
+// always drains.  Returns false after an abort.
+__device__ bool grid_sync(GridBarrier* b, unsigned nwg, int* s_flag) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ab = 0;
+        const unsigned gen = __hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned prev = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev + 1u == nwg) {
+            __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&b->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned spins = 0;
+            while (__hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (__hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) { ab = 1; break; }
+                if (++spins > SPIN_LIMIT) {
+                    __hip_atomic_store(&b->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ab = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        *s_flag = ab;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return *s_flag == 0;
+}
+
+__device__ inline void argmax_combine(float& v, int& i, float ov, int oi) {
+    if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+}
+
+__device__ inline void wave_argmax(float& v, int& i) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ov = __shfl_xor(v, off);
+        const int oi = __shfl_xor(i, off);
+        argmax_combine(v, i, ov, oi);
+    }
+}
+
+// --------------------------------------------------- panel factorisation --
+// Unblocked LU with partial pivoting of the m x nbc panel whose top-left
+// entry is A(j0, j0) (column-major, leading dimension lda).  Workgroup w holds
+// panel rows [w R, w R + R) of all nbc columns in LDS.  Per column: every
+// workgroup posts its largest |a| (ties: lowest row) and that row's values,
+// the owner of the diagonal row posts it too, one grid barrier, then every
+// workgroup picks the same pivot, performs the interchange on the rows it
+// owns, scales its part of the column and updates its part of the panel.
+// The candidate slots alternate between two buffers so that posting for
+// column j + 1 cannot overwrite what a slower workgroup still reads for j.
+__global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda, int m, int nbc, int j0,
+                                                  int* __restrict__ ipiv, float* __restrict__ s_abs,
+                                                  int* __restrict__ s_row, float* __restrict__ s_vals,
+                                                  float* __restrict__ s_diag, GridBarrier* bar,
+                                                  int* __restrict__ info) {
+    __shared__ float slab[LU_NB * PANEL_R];   // panel column c at slab[c * PANEL_R]
+    __shared__ float urow[LU_NB];
+    __shared__ float rv[4];
+    __shared__ int ri[4];
+    __shared__ int s_lrow, s_win, s_flag;
+    const int t = threadIdx.x, w = blockIdx.x, nwg = gridDim.x;
+    const int rbase = w * PANEL_R;
+    const int rows = min(PANEL_R, m - rbase);
+    float* P = A + (size_t)j0 * lda + j0;
+
+    for (int idx = t; idx < nbc * PANEL_R; idx += 256) {
+        const int c = idx / PANEL_R, r = idx - c * PANEL_R;
+        slab[idx] = r < rows ? P[(size_t)c * lda + rbase + r] : 0.f;
+    }
+    __syncthreads();
+
+    for (int j = 0; j < nbc; ++j) {
+        const int par = j & 1;
+        float best = -1.f;
+        int bi = INT_MAX;
+        if (t < rows && rbase + t >= j) {
+            best = fabsf(slab[j * PANEL_R + t]);
+            if (!(best == best)) best = INFINITY;   // a NaN entry is taken as the largest
+            bi = rbase + t;
+        }
+        wave_argmax(best, bi);
+        if ((t & 63) == 0) { rv[t >> 6] = best; ri[t >> 6] = bi; }
+        __syncthreads();
+        if (t == 0) {
+            for (int q = 1; q < 4; ++q) argmax_combine(rv[0], ri[0], rv[q], ri[q]);
+            const bool any = ri[0] != INT_MAX;
+            s_abs[par * MAX_PANEL_WG + w] = rv[0];
+            s_row[par * MAX_PANEL_WG + w] = any ? ri[0] : -1;
+            s_lrow = any ? ri[0] - rbase : -1;
+        }
+        __syncthreads();
+        if (s_lrow >= 0)
+            for (int c = t; c < nbc; c += 256)
+                s_vals[((size_t)par * MAX_PANEL_WG + w) * LU_NB + c] = slab[c * PANEL_R + s_lrow];
+        const int jw = j / PANEL_R;
+        if (w == jw)
+            for (int c = t; c < nbc; c += 256) s_diag[par * LU_NB + c] = slab[c * PANEL_R + (j - rbase)];
+
+        if (!grid_sync(bar, (unsigned)nwg, &s_flag)) return;
+
+        best = -1.f;
+        bi = INT_MAX;
+        for (int q = t; q < nwg; q += 256) {
+            const int r = s_row[par * MAX_PANEL_WG + q];
+            if (r >= 0) argmax_combine(best, bi, s_abs[par * MAX_PANEL_WG + q], r);
+        }
+        wave_argmax(best, bi);
+        if ((t & 63) == 0) { rv[t >> 6] = best; ri[t >> 6] = bi; }
+        __syncthreads();
+        if (t == 0) {
+            for (int q = 1; q < 4; ++q) argmax_combine(rv[0], ri[0], rv[q], ri[q]);
+            s_win = ri[0] == INT_MAX ? j : ri[0];   // every candidate row is valid; keep j otherwise
+        }
+        __syncthreads();
+        const int p = s_win;                 // panel-relative pivot row, >= j
+        const int pw = p / PANEL_R;
+        for (int c = t; c < nbc; c += 256) urow[c] = s_vals[((size_t)par * MAX_PANEL_WG + pw) * LU_NB + c];
+        if (w == 0 && t == 0) ipiv[j0 + j] = j0 + p;
+        __syncthreads();
+        if (p != j) {
+            if (w == pw)
+                for (int c = t; c < nbc; c += 256) slab[c * PANEL_R + (p - rbase)] = s_diag[par * LU_NB + c];
+            if (w == jw)
+                for (int c = t; c < nbc; c += 256) slab[c * PANEL_R + (j - rbase)] = urow[c];
+        }
+        __syncthreads();
+        const float piv = urow[j];
+        if (piv != 0.f) {
+            if (t < rows && rbase + t > j) slab[j * PANEL_R + t] /= piv;
+            __syncthreads();
+            const int ncols = nbc - j - 1;
+            for (int idx = t; idx < ncols * PANEL_R; idx += 256) {
+                const int cc = idx / PANEL_R, r = idx - cc * PANEL_R;
+                const int c = j + 1 + cc;
+                if (r < rows && rbase + r > j) slab[c * PANEL_R + r] -= slab[j * PANEL_R + r] * urow[c];
+            }
+        } else if (w == 0 && t == 0 && *info == 0) {
+            *info = j0 + j + 1;
+        }
+        __syncthreads();
+    }
+
+    for (int idx = t; idx < nbc * PANEL_R; idx += 256) {
+        const int c = idx / PANEL_R, r = idx - c * PANEL_R;
+        if (r < rows) P[(size_t)c * lda + rbase + r] = slab[idx];
+    }
+}
+
+// row interchanges of the panel [j0, j0 + nbc) applied to every column
+// outside it (LAPACK slaswp), one thread per column
+__global__ __launch_bounds__(256) void k_laswp(float* __restrict__ A, int lda, int n, int j0, int nbc,
+                                               const int* __restrict__ ipiv) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    const int col = c < j0 ? c : c + nbc;
+    if (col >= n) return;
+    float* cp = A + (size_t)col * lda;
+    for (int k = 0; k < nbc; ++k) {
+        const int p = ipiv[j0 + k];
+        if (p != j0 + k) {
+            const float a = cp[j0 + k];
+            cp[j0 + k] = cp[p];
+            cp[p] = a;
+        }
+    }
+}
+
+__device__ inline float readlane_f(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// A(j0:j0+nbc, j0+nbc:n) <- L11^-1 A(j0:j0+nbc, j0+nbc:n), L11 unit lower
+// (strsm L,L,N,U).  One wavefront per column, two rows per lane, L11 in LDS.
+constexpr int TRSM_CPW = 4;   // columns per wavefront
+__global__ __launch_bounds__(256) void k_trsm_llnu(float* __restrict__ A, int lda, int n, int j0, int nbc) {
+    __shared__ float Ls[LU_NB * LU_NB];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int idx = t; idx < nbc * nbc; idx += 256) {
+        const int k = idx / nbc, r = idx - k * nbc;
+        Ls[k * LU_NB + r] = A[(size_t)(j0 + k) * lda + j0 + r];
+    }
+    __syncthreads();
+    const int c0 = j0 + nbc + (blockIdx.x * 4 + wv) * TRSM_CPW;
+    for (int q = 0; q < TRSM_CPW; ++q) {
+        const int col = c0 + q;
+        if (col >= n) break;
+        float* cp = A + (size_t)col * lda + j0;
+        float b0 = lane < nbc ? cp[lane] : 0.f;
+        float b1 = lane + 64 < nbc ? cp[lane + 64] : 0.f;
+        for (int k = 0; k < nbc; ++k) {
+            const float xk = k < 64 ? readlane_f(b0, k) : readlane_f(b1, k - 64);
+            if (lane > k && lane < nbc) b0 -= Ls[k * LU_NB + lane] * xk;
+            if (lane + 64 > k && lane + 64 < nbc) b1 -= Ls[k * LU_NB + lane + 64] * xk;
+        }
+        if (lane < nbc) cp[lane] = b0;
+        if (lane + 64 < nbc) cp[lane + 64] = b1;
+    }
+}
+
+// FP64 row-major -> FP32 column-major (the factor's working copy)
+__global__ __launch_bounds__(256) void k_to_cm32(const double* __restrict__ A64, int n, float* __restrict__ A32,
+                                                 int lda) {
+    __shared__ float tile[64][65];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+    for (int k = ty; k < 64; k += 4) {
+        const int i = i0 + k, j = j0 + tx;
+        tile[k][tx] = (i < n && j < n) ? (float)A64[(size_t)i * n + j] : 0.f;
+    }
+    __syncthreads();
+    for (int k = ty; k < 64; k += 4) {
+        const int j = j0 + k, i = i0 + tx;
+        if (i < n && j < n) A32[(size_t)j * lda + i] = tile[tx][k];
+    }
+}
+
+// ------------------------------------------------- triangular solves ----
+// y = (float) r[perm]  (P applied to the FP64 residual)
+__global__ void k_gather_perm(const double* __restrict__ r, const int* __restrict__ perm, float* __restrict__ y,
+                              int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (float)r[perm[i]];
+}
+
+// diagonal block of L (unit lower) or U (upper): y_b <- T_bb^-1 y_b, one
+// wavefront doing the substitution from LDS
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_diag_solve(const float* __restrict__ A, int lda, float* __restrict__ y,
+                                                    int b0, int nb) {
+    __shared__ float Ts[LU_NB * LU_NB];
+    const int t = threadIdx.x;
+    for (int idx = t; idx < nb * nb; idx += 256) {
+        const int k = idx / nb, r = idx - k * nb;
+        Ts[k * LU_NB + r] = A[(size_t)(b0 + k) * lda + b0 + r];
+    }
+    __syncthreads();
+    if (t >= 64) return;
+    const int lane = t;
+    float v0 = lane < nb ? y[b0 + lane] : 0.f;
+    float v1 = lane + 64 < nb ? y[b0 + lane + 64] : 0.f;
+    if (LOWER) {
+        for (int k = 0; k < nb; ++k) {
+            const float xk = k < 64 ? readlane_f(v0, k) : readlane_f(v1, k - 64);
+            if (lane > k && lane < nb) v0 -= Ts[k * LU_NB + lane] * xk;
+            if (lane + 64 > k && lane + 64 < nb) v1 -= Ts[k * LU_NB + lane + 64] * xk;
+        }
+    } else {
+        for (int k = nb - 1; k >= 0; --k) {
+            const float d = Ts[k * LU_NB + k];
+            if (k < 64) { if (lane == k) v0 /= d; } else { if (lane + 64 == k) v1 /= d; }
+            const float xk = k < 64 ? readlane_f(v0, k) : readlane_f(v1, k - 64);
+            if (lane < k) v0 -= Ts[k * LU_NB + lane] * xk;
+            if (lane + 64 < k) v1 -= Ts[k * LU_NB + lane + 64] * xk;
+        }
+    }
+    if (lane < nb) y[b0 + lane] = v0;
+    if (lane + 64 < nb) y[b0 + lane + 64] = v1;
+}
+
+// y_i -= sum_k A(i, b0 + k) y_(b0 + k) for rows i in [r0, r1), one thread per row
+__global__ __launch_bounds__(64) void k_block_update(const float* __restrict__ A, int lda, float* __restrict__ y,
+                                                     int b0, int nb, int r0, int r1) {
+    __shared__ float yb[LU_NB];
+    const int t = threadIdx.x;
+    for (int k = t; k < nb; k += 64) yb[k] = y[b0 + k];
+    __syncthreads();
+    const int i = r0 + blockIdx.x * 64 + t;
+    if (i >= r1) return;
+    const float* col = A + (size_t)b0 * lda + i;
+    float acc0 = 0.f, acc1 = 0.f;
+    int k = 0;
+    for (; k + 1 < nb; k += 2) {
+        acc0 += col[(size_t)k * lda] * yb[k];
+        acc1 += col[(size_t)(k + 1) * lda] * yb[k + 1];
+    }
+    if (k < nb) acc0 += col[(size_t)k * lda] * yb[k];
+    y[i] -= acc0 + acc1;
+}
+
+// x += (double) y
+__global__ void k_add_f32(double* __restrict__ x, const float* __restrict__ y, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += (double)y[i];
+}
+
+// ------------------------------------------------------ FP64 residual ----
+constexpr int RES_ROWS = 8;   // matrix rows per wavefront (x is read once per 8 rows)
+__global__ __launch_bounds__(256) void k_resid64(const double* __restrict__ A, int n, const double* __restrict__ x,
+                                                 const double* __restrict__ b, double* __restrict__ r,
+                                                 double* __restrict__ rowabs) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int row0 = (blockIdx.x * 4 + wv) * RES_ROWS;
+    if (row0 >= n) return;
+    const int nrow = min(RES_ROWS, n - row0);
+    double acc[RES_ROWS], aab[RES_ROWS];
+#pragma unroll
+    for (int q = 0; q < RES_ROWS; ++q) { acc[q] = 0.0; aab[q] = 0.0; }
+    for (int j = lane; j < n; j += 64) {
+        const double xj = x[j];
+#pragma unroll
+        for (int q = 0; q < RES_ROWS; ++q) {
+            if (q < nrow) {
+                const double a = A[(size_t)(row0 + q) * n + j];
+                acc[q] += a * xj;
+                aab[q] += fabs(a);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RES_ROWS; ++q) {
+        for (int off = 32; off > 0; off >>= 1) {
+            acc[q] += __shfl_xor(acc[q], off);
+            aab[q] += __shfl_xor(aab[q], off);
+        }
+    }
+    if (lane < nrow) {
+        double a = 0.0, s = 0.0;
+#pragma unroll
+        for (int q = 0; q < RES_ROWS; ++q) if (q == lane) { a = acc[q]; s = aab[q]; }
+        r[row0 + lane] = b[row0 + lane] - a;
+        if (rowabs) rowabs[row0 + lane] = s;
+    }
+}
+
+// out = {max|r|, max|x|, max|b|, max rowabs}: one workgroup
+__global__ __launch_bounds__(1024) void k_norms(const double* __restrict__ r, const double* __restrict__ x,
+                                                const double* __restrict__ b, const double* __restrict__ rowabs,
+                                                int n, double* __restrict__ out) {
+    __shared__ double red[4][16];
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        m[0] = fmax(m[0], fabs(r[i]));
+        m[1] = fmax(m[1], fabs(x[i]));
+        m[2] = fmax(m[2], fabs(b[i]));
+        if (rowabs) m[3] = fmax(m[3], rowabs[i]);
+    }
+    for (int q = 0; q < 4; ++q)
+        for (int off = 32; off > 0; off >>= 1) m[q] = fmax(m[q], __shfl_xor(m[q], off));
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) for (int q = 0; q < 4; ++q) red[q][wv] = m[q];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double v = 0.0;
+        for (int k = 0; k < 16; ++k) v = fmax(v, red[threadIdx.x][k]);
+        out[threadIdx.x] = v;
+    }
+}
+
+// ---------------------------------------------------------- VLM kernels --
+// Ring geometry (panel.cpp:46-72, 103-120): corners copied per ring, centre,
+// unit normal -sum dl_i x dl_i+1, collocation point k centre + (1 - k) leading-
+// edge midpoint, and rhs_i = -Vinf . n_i (solver.cpp:256-268).
+__global__ void k_ring_geom(const double* __restrict__ nodes, const int* __restrict__ rn, int nr, double kfac,
+                            double vx, double vy, double vz, double* __restrict__ corner,
+                            double* __restrict__ colloc, double* __restrict__ normal, double* __restrict__ rhs) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nr) return;
+    double p[4][3];
+    for (int a = 0; a < 4; ++a) {
+        const int id = rn[4 * k + a];
+        for (int d = 0; d < 3; ++d) {
+            p[a][d] = nodes[3 * id + d];
+            corner[12 * k + 3 * a + d] = p[a][d];
+        }
+    }
+    double c[3], nrm[3] = {0.0, 0.0, 0.0}, dl[4][3];
+    for (int d = 0; d < 3; ++d) c[d] = (((p[0][d] + p[1][d]) + p[2][d]) + p[3][d]) / 4.0;
+    for (int a = 0; a < 4; ++a)
+        for (int d = 0; d < 3; ++d) dl[a][d] = p[(a + 1) & 3][d] - p[a][d];
+    for (int a = 0; a < 4; ++a) {
+        const double* u = dl[a];
+        const double* v = dl[(a + 1) & 3];
+        nrm[0] -= u[1] * v[2] - u[2] * v[1];
+        nrm[1] -= u[2] * v[0] - u[0] * v[2];
+        nrm[2] -= u[0] * v[1] - u[1] * v[0];
+    }
+    const double len = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    for (int d = 0; d < 3; ++d) {
+        nrm[d] /= len;
+        normal[3 * k + d] = nrm[d];
+        colloc[3 * k + d] = kfac * c[d] + (1.0 - kfac) * (0.5 * (p[0][d] + p[1][d]));
+    }
+    rhs[k] = -(vx * nrm[0] + vy * nrm[1] + vz * nrm[2]);
+}
+
+// wake ring of every station (surface.cpp:35-58): trailing-edge nodes 3 and
+// 2 of the station's last ring, then the same points moved wake_length along
+// the unit freestream
+__global__ void k_wake(const double* __restrict__ corner, const int* __restrict__ soff, int ns, double L,
+                       double wx, double wy, double wz, double* __restrict__ wcorner) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int k = soff[s + 1] - 1;
+    const double* p = corner + 12 * k;
+    const double w[3] = {wx, wy, wz};
+    for (int d = 0; d < 3; ++d) {
+        const double p1 = p[9 + d], p2 = p[6 + d];
+        wcorner[12 * s + 0 + d] = p1;
+        wcorner[12 * s + 3 + d] = p2;
+        wcorner[12 * s + 6 + d] = p2 + L * w[d];
+        wcorner[12 * s + 9 + d] = p1 + L * w[d];
+    }
+}
+
+// velocity induced at P by a unit-strength ring with corners q[0..3]
+// (4 pi times vortexRing::influence, panel.cpp:122-128).  Per segment a->b
+// with r1 = P - a, r2 = P - b the reference's h and direction
+// (potential.cpp:36-58) give v = (dl.r1/|r1| - dl.r2/|r2|) (r1 x r2) / |r1 x r2|^2
+// since dl x r1 = r1 x r2 and |dl x r1| = |r1 x r2|; the Lamb-Oseen factor
+// 1 - exp(-(h/rc)^2) multiplies it when rc > 0.
+__device__ inline void ring_velocity(const double P[3], const double q[12], double rc, double v[3]) {
+    double r[4][3], ir[4];
+    for (int a = 0; a < 4; ++a) {
+        for (int d = 0; d < 3; ++d) r[a][d] = P[d] - q[3 * a + d];
+        ir[a] = rsqrt(r[a][0] * r[a][0] + r[a][1] * r[a][1] + r[a][2] * r[a][2]);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int b = (a + 1) & 3;
+        const double* r1 = r[a];
+        const double* r2 = r[b];
+        const double c0 = r1[1] * r2[2] - r1[2] * r2[1];
+        const double c1 = r1[2] * r2[0] - r1[0] * r2[2];
+        const double c2 = r1[0] * r2[1] - r1[1] * r2[0];
+        const double cc = c0 * c0 + c1 * c1 + c2 * c2;
+        const double d0 = q[3 * b] - q[3 * a], d1 = q[3 * b + 1] - q[3 * a + 1], d2 = q[3 * b + 2] - q[3 * a + 2];
+        const double tt = (d0 * r1[0] + d1 * r1[1] + d2 * r1[2]) * ir[a] - (d0 * r2[0] + d1 * r2[1] + d2 * r2[2]) * ir[b];
+        double s = tt / cc;
+        if (rc > 0.0) {
+            const double h = sqrt(cc) / tt;
+            s *= 1.0 - exp(-(h / rc) * (h / rc));
+        }
+        v[0] += s * c0;
+        v[1] += s * c1;
+        v[2] += s * c2;
+    }
+}
+
+// AIC ring part: A(i, j) = v_j(P_i) . n_i.  Block = 64 columns x 64 rows;
+// a wavefront holds 64 consecutive columns of one row, so its stores are
+// contiguous in the row-major matrix and its row data is an LDS broadcast.
+constexpr int AIC_ROWS_PER_WAVE = 16;
+__global__ __launch_bounds__(256) void k_aic(const double* __restrict__ colloc, const double* __restrict__ normal,
+                                             const double* __restrict__ corner, int n, double rc,
+                                             double* __restrict__ A) {
+    __shared__ double sP[64][6];
+    const int t = threadIdx.x;
+    const int rbase = blockIdx.y * 64;
+    if (t < 64) {
+        const int i = rbase + t;
+        for (int d = 0; d < 3; ++d) {
+            sP[t][d] = i < n ? colloc[3 * i + d] : 0.0;
+            sP[t][3 + d] = i < n ? normal[3 * i + d] : 0.0;
+        }
+    }
+    __syncthreads();
+    const int j = blockIdx.x * 64 + (t & 63);
+    if (j >= n) return;
+    double q[12];
+    for (int a = 0; a < 12; ++a) q[a] = corner[12 * j + a];
+    const int r0 = (t >> 6) * AIC_ROWS_PER_WAVE;
+    for (int rr = 0; rr < AIC_ROWS_PER_WAVE; ++rr) {
+        const int il = r0 + rr, i = rbase + il;
+        if (i >= n) break;
+        const double P[3] = {sP[il][0], sP[il][1], sP[il][2]};
+        double v[3] = {0.0, 0.0, 0.0};
+        ring_velocity(P, q, rc, v);
+        A[(size_t)i * n + j] = (v[0] * sP[il][3] + v[1] * sP[il][4] + v[2] * sP[il][5]) / FOUR_PI;
+    }
+}
+
+// wake rings added to their trailing-edge ring's column (solver.cpp:221-236)
+__global__ __launch_bounds__(256) void k_aic_wake(const double* __restrict__ colloc, const double* __restrict__ normal,
+                                                  const double* __restrict__ wcorner, const int* __restrict__ soff,
+                                                  int ns, int n, double rc, double* __restrict__ A) {
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (long long)n * ns) return;
+    const int s = (int)(idx % ns), i = (int)(idx / ns);
+    double q[12];
+    for (int a = 0; a < 12; ++a) q[a] = wcorner[12 * s + a];
+    const double P[3] = {colloc[3 * i], colloc[3 * i + 1], colloc[3 * i + 2]};
+    double v[3] = {0.0, 0.0, 0.0};
+    ring_velocity(P, q, rc, v);
+    const int te = soff[s + 1] - 1;
+    A[(size_t)i * n + te] += (v[0] * normal[3 * i] + v[1] * normal[3 * i + 1] + v[2] * normal[3 * i + 2]) / FOUR_PI;
+}
+
+// Kutta-Joukowski per ring, leading to trailing edge of each station
+// (wingStation::computeForces, surface.cpp:164-201, with the local stream
+// equal to the freestream of the linear solver)
+__global__ void k_forces(const double* __restrict__ corner, const double* __restrict__ gamma,
+                         const int* __restrict__ soff, int ns, double rho, double vx, double vy, double vz,
+                         double* __restrict__ rforce, double* __restrict__ sforce) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    double prev = 0.0, F[3] = {0.0, 0.0, 0.0};
+    for (int k = soff[s]; k < soff[s + 1]; ++k) {
+        const double* p = corner + 12 * k;
+        const double dl0 = p[3] - p[0], dl1 = p[4] - p[1], dl2 = p[5] - p[2];
+        const double g = rho * (gamma[k] - prev);
+        const double f0 = (vy * dl2 - vz * dl1) * g;
+        const double f1 = (vz * dl0 - vx * dl2) * g;
+        const double f2 = (vx * dl1 - vy * dl0) * g;
+        rforce[3 * k] = f0;
+        rforce[3 * k + 1] = f1;
+        rforce[3 * k + 2] = f2;
+        F[0] += f0;
+        F[1] += f1;
+        F[2] += f2;
+        prev = gamma[k];
+    }
+    for (int d = 0; d < 3; ++d) sforce[3 * s + d] = F[d];
+}
+
+__global__ void k_zero_barrier(GridBarrier* b, int* info) {
+    if (threadIdx.x == 0) {
+        b->count = 0u;
+        b->gen = 0u;
+        b->abort = 0u;
+        b->pad = 0u;
+        *info = 0;
+    }
+}
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+    return hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T));
+}
+
+}  // namespace
+
+// ======================================================= dense solver ====
+struct afx_dense {
+    int n = 0, lda = 0;
+    hipStream_t s = nullptr;
+    rocblas_handle rb = nullptr;
+    double* A64 = nullptr;     // row-major n x n
+    float* A32 = nullptr;      // column-major lda x n, LU in place
+    int* ipiv = nullptr;
+    int* perm = nullptr;
+    double *b = nullptr, *x = nullptr, *r = nullptr, *rowabs = nullptr, *norms = nullptr;
+    float* y = nullptr;
+    float* s_abs = nullptr;
+    int* s_row = nullptr;
+    float* s_vals = nullptr;
+    float* s_diag = nullptr;
+    GridBarrier* bar = nullptr;
+    int* info = nullptr;
+    double anorm = -1.0;       // |A|inf of the current FP64 matrix (-1: not yet known)
+    bool factored = false;
+    ~afx_dense() {
+        for (void* p : {(void*)A64, (void*)A32, (void*)ipiv, (void*)perm, (void*)b, (void*)x, (void*)r,
+                        (void*)rowabs, (void*)norms, (void*)y, (void*)s_abs, (void*)s_row, (void*)s_vals,
+                        (void*)s_diag, (void*)bar, (void*)info})
+            if (p) (void)hipFree(p);
+        if (rb) rocblas_destroy_handle(rb);
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+namespace {
+
+int dense_init(afx_dense* h, int n) {
+    if (n <= 0) return api_fail(AFX_EARG, "matrix order must be positive");
+    if ((n + PANEL_R - 1) / PANEL_R > MAX_PANEL_WG)
+        return api_fail(AFX_EUNSUP, "matrix order " + std::to_string(n) + " exceeds the panel kernel's " +
+                                        std::to_string(MAX_PANEL_WG * PANEL_R));
+    int dev = 0, coop = 0;
+    VCHK(hipGetDevice(&dev));
+    VCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    if (!coop) return api_fail(AFX_EUNSUP, "device does not support cooperative launches");
+    h->n = n;
+    h->lda = ((n + 31) / 32) * 32 + 32;   // padded so that columns do not alias HBM channels
+    VCHK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    RBCHK(rocblas_create_handle(&h->rb));
+    RBCHK(rocblas_set_stream(h->rb, h->s));
+    const size_t nn = (size_t)n;
+    VCHK(dalloc(&h->A64, nn * nn));
+    VCHK(dalloc(&h->A32, (size_t)h->lda * nn));
+    VCHK(dalloc(&h->ipiv, nn));
+    VCHK(dalloc(&h->perm, nn));
+    VCHK(dalloc(&h->b, nn));
+    VCHK(dalloc(&h->x, nn));
+    VCHK(dalloc(&h->r, nn));
+    VCHK(dalloc(&h->rowabs, nn));
+    VCHK(dalloc(&h->norms, 4));
+    VCHK(dalloc(&h->y, nn));
+    VCHK(dalloc(&h->s_abs, 2 * MAX_PANEL_WG));
+    VCHK(dalloc(&h->s_row, 2 * MAX_PANEL_WG));
+    VCHK(dalloc(&h->s_vals, (size_t)2 * MAX_PANEL_WG * LU_NB));
+    VCHK(dalloc(&h->s_diag, 2 * LU_NB));
+    VCHK(dalloc(&h->bar, 1));
+    VCHK(dalloc(&h->info, 1));
+    return AFX_OK;
+}
+
+int dense_factor(afx_dense* h, int32_t* info_out) {
+    const int n = h->n, lda = h->lda;
+    hipStream_t s = h->s;
+    h->factored = false;
+    dim3 tg((n + 63) / 64, (n + 63) / 64);
+    k_to_cm32<<<tg, 256, 0, s>>>(h->A64, n, h->A32, lda);
+    VCHK(hipGetLastError());
+    k_zero_barrier<<<1, 64, 0, s>>>(h->bar, h->info);
+    VCHK(hipGetLastError());
+    for (int j0 = 0; j0 < n; j0 += LU_NB) {
+        const int nbc = std::min(LU_NB, n - j0);
+        const int m = n - j0;
+        const int nwg = (m + PANEL_R - 1) / PANEL_R;
+        float* A = h->A32;
+        int ldai = lda, mi = m, nbci = nbc, j0i = j0;
+        void* args[] = {&A, &ldai, &mi, &nbci, &j0i, &h->ipiv, &h->s_abs, &h->s_row, &h->s_vals, &h->s_diag,
+                        &h->bar, &h->info};
+        VCHK(hipLaunchCooperativeKernel((const void*)k_panel_lu, dim3(nwg), dim3(256), args, 0, s));
+        const int others = n - nbc;
+        if (others > 0) {
+            k_laswp<<<(others + 255) / 256, 256, 0, s>>>(h->A32, lda, n, j0, nbc, h->ipiv);
+            VCHK(hipGetLastError());
+        }
+        const int rest = n - j0 - nbc;
+        if (rest > 0) {
+            const int wgs = (rest + 4 * TRSM_CPW - 1) / (4 * TRSM_CPW);
+            k_trsm_llnu<<<wgs, 256, 0, s>>>(h->A32, lda, n, j0, nbc);
+            VCHK(hipGetLastError());
+            const float alpha = -1.f, beta = 1.f;
+            float* A21 = h->A32 + (size_t)j0 * lda + j0 + nbc;
+            float* A12 = h->A32 + (size_t)(j0 + nbc) * lda + j0;
+            float* A22 = h->A32 + (size_t)(j0 + nbc) * lda + j0 + nbc;
+            RBCHK(rocblas_sgemm(h->rb, rocblas_operation_none, rocblas_operation_none, rest, rest, nbc, &alpha, A21,
+                                lda, A12, lda, &beta, A22, lda));
+        }
+    }
+    GridBarrier hb{};
+    int info = 0;
+    VCHK(hipMemcpyAsync(&hb, h->bar, sizeof(hb), hipMemcpyDeviceToHost, s));
+    VCHK(hipMemcpyAsync(&info, h->info, sizeof(int), hipMemcpyDeviceToHost, s));
+    std::vector<int> ipiv(n);
+    VCHK(hipMemcpyAsync(ipiv.data(), h->ipiv, n * sizeof(int), hipMemcpyDeviceToHost, s));
+    VCHK(hipStreamSynchronize(s));
+    if (hb.abort) return api_fail(AFX_EHIP, "LU panel: grid barrier timed out (workgroups not co-resident)");
+    // P b = b[perm]: the interchanges applied in order to the identity
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int k = 0; k < n; ++k) {
+        const int p = ipiv[k];
+        if (p < k || p >= n) return api_fail(AFX_EHIP, "LU panel produced an invalid pivot");
+        std::swap(perm[k], perm[p]);
+    }
+    VCHK(hipMemcpyAsync(h->perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+    VCHK(hipStreamSynchronize(s));
+    if (info_out) *info_out = info;
+    h->factored = info == 0;
+    if (info) return api_fail(AFX_EARG, "matrix is singular in FP32: zero pivot at column " + std::to_string(info - 1));
+    return AFX_OK;
+}
+
+// y <- U^-1 L^-1 y (y already permuted)
+int dense_lu_solve(afx_dense* h) {
+    const int n = h->n, lda = h->lda;
+    hipStream_t s = h->s;
+    for (int b0 = 0; b0 < n; b0 += LU_NB) {
+        const int nb = std::min(LU_NB, n - b0);
+        k_diag_solve<true><<<1, 256, 0, s>>>(h->A32, lda, h->y, b0, nb);
+        const int r0 = b0 + nb;
+        if (r0 < n) k_block_update<<<(n - r0 + 63) / 64, 64, 0, s>>>(h->A32, lda, h->y, b0, nb, r0, n);
+    }
+    const int last = ((n - 1) / LU_NB) * LU_NB;
+    for (int b0 = last; b0 >= 0; b0 -= LU_NB) {
+        const int nb = std::min(LU_NB, n - b0);
+        k_diag_solve<false><<<1, 256, 0, s>>>(h->A32, lda, h->y, b0, nb);
+        if (b0 > 0) k_block_update<<<(b0 + 63) / 64, 64, 0, s>>>(h->A32, lda, h->y, b0, nb, 0, b0);
+    }
+    VCHK(hipGetLastError());
+    return AFX_OK;
+}
+
+// FP64 iterative refinement: x = 0; repeat r = b - A x (FP64), stop when the
+// normwise backward error |r| / (|A| |x| + |b|) <= tol, else x += LU^-1 P r.
+int dense_solve_dev(afx_dense* h, const double* b_dev, double* x_dev, double tol, int max_iter,
+                    afx_solve_info* info) {
+    if (!h->factored) return api_fail(AFX_ESTATE, "no factorisation: call factor first");
+    const int n = h->n;
+    hipStream_t s = h->s;
+    if (tol <= 0.0) tol = std::sqrt((double)n) * 1.1102230246251565e-16;
+    if (max_iter <= 0) max_iter = 30;
+    VCHK(hipMemsetAsync(x_dev, 0, n * sizeof(double), s));
+    afx_solve_info inf{};
+    double prev = -1.0;
+    const int tb = 256;
+    for (int it = 0;; ++it) {
+        const bool need_norm = h->anorm < 0.0;
+        k_resid64<<<(n + 4 * RES_ROWS - 1) / (4 * RES_ROWS), 256, 0, s>>>(h->A64, n, x_dev, b_dev, h->r,
+                                                                         need_norm ? h->rowabs : nullptr);
+        k_norms<<<1, 1024, 0, s>>>(h->r, x_dev, b_dev, need_norm ? h->rowabs : nullptr, n, h->norms);
+        VCHK(hipGetLastError());
+        double nrm[4];
+        VCHK(hipMemcpyAsync(nrm, h->norms, sizeof(nrm), hipMemcpyDeviceToHost, s));
+        VCHK(hipStreamSynchronize(s));
+        if (need_norm) h->anorm = nrm[3];
+        const double denom = h->anorm * nrm[1] + nrm[2];
+        inf.backward_error = denom > 0.0 ? nrm[0] / denom : 0.0;
+        if (prev > 0.0) inf.contraction = nrm[0] / prev;
+        prev = nrm[0];
+        inf.iterations = it;
+        if (!std::isfinite(inf.backward_error)) break;
+        if (inf.backward_error <= tol) { inf.converged = 1; break; }
+        if (it >= max_iter) break;
+        k_gather_perm<<<(n + tb - 1) / tb, tb, 0, s>>>(h->r, h->perm, h->y, n);
+        VCHK(hipGetLastError());
+        int rc = dense_lu_solve(h);
+        if (rc) return rc;
+        k_add_f32<<<(n + tb - 1) / tb, tb, 0, s>>>(x_dev, h->y, n);
+        VCHK(hipGetLastError());
+    }
+    if (info) *info = inf;
+    return AFX_OK;
+}
+
+}  // namespace
+
+// ================================================================ VLM =====
+struct afx_vlm {
+    int nn = 0, nr = 0, ns = 0;
+    std::vector<double> xyz;
+    std::vector<int> rn, soff;
+    afx_vlm_flow flow{};
+    bool flow_set = false, built = false, solved = false;
+    afx_dense* dense = nullptr;
+    double *d_nodes = nullptr, *d_corner = nullptr, *d_wcorner = nullptr, *d_colloc = nullptr,
+           *d_normal = nullptr, *d_rhs = nullptr, *d_gamma = nullptr, *d_rforce = nullptr, *d_sforce = nullptr;
+    int *d_rn = nullptr, *d_soff = nullptr;
+    ~afx_vlm() {
+        for (void* p : {(void*)d_nodes, (void*)d_corner, (void*)d_wcorner, (void*)d_colloc, (void*)d_normal,
+                        (void*)d_rhs, (void*)d_gamma, (void*)d_rforce, (void*)d_sforce, (void*)d_rn, (void*)d_soff})
+            if (p) (void)hipFree(p);
+        delete dense;
+    }
+};
+
+namespace {
+double flow_kfac(double alpha_deg) {
+    // vortexRing::computeCollocationPoint (panel.cpp:111-120): the branch
+    // condition keeps k = 1 for every angle below 1e-6 degrees
+    if (alpha_deg < 1e-6 && alpha_deg > -1e6) return 1.0;
+    const double a = alpha_deg * M_PI / 180.0;
+    return alpha_deg * M_PI / (180.0 * std::sin(a));
+}
+}  // namespace
+
+extern "C" {
+
+int afx_dense_create(int32_t n, afx_dense** out) {
+    if (!out) return api_fail(AFX_EARG, "null argument");
+    auto* h = new afx_dense();
+    int rc = dense_init(h, n);
+    if (rc) { delete h; return rc; }
+    *out = h;
+    return AFX_OK;
+}
+
+int afx_dense_destroy(afx_dense* h) {
+    delete h;
+    return AFX_OK;
+}
+
+int afx_dense_set(afx_dense* h, const double* a) {
+    if (!h || !a) return api_fail(AFX_EARG, "null argument");
+    VCHK(hipMemcpyAsync(h->A64, a, (size_t)h->n * h->n * sizeof(double), hipMemcpyHostToDevice, h->s));
+    VCHK(hipStreamSynchronize(h->s));
+    h->anorm = -1.0;
+    h->factored = false;
+    return AFX_OK;
+}
+
+int afx_dense_factor(afx_dense* h, int32_t* info) {
+    if (!h) return api_fail(AFX_EARG, "null argument");
+    return dense_factor(h, info);
+}
+
+int afx_dense_solve(afx_dense* h, const double* b, double tol, int32_t max_iter, double* x, afx_solve_info* info) {
+    if (!h || !b || !x) return api_fail(AFX_EARG, "null argument");
+    const int n = h->n;
+    VCHK(hipMemcpyAsync(h->b, b, n * sizeof(double), hipMemcpyHostToDevice, h->s));
+    int rc = dense_solve_dev(h, h->b, h->x, tol, max_iter, info);
+    if (rc) return rc;
+    VCHK(hipMemcpyAsync(x, h->x, n * sizeof(double), hipMemcpyDeviceToHost, h->s));
+    VCHK(hipStreamSynchronize(h->s));
+    return AFX_OK;
+}
+
+int afx_dense_get_lu(afx_dense* h, float* lu, int32_t* ipiv) {
+    if (!h) return api_fail(AFX_EARG, "null argument");
+    const int n = h->n;
+    if (lu) VCHK(hipMemcpy2DAsync(lu, n * sizeof(float), h->A32, h->lda * sizeof(float), n * sizeof(float), n,
+                                  hipMemcpyDeviceToHost, h->s));
+    if (ipiv) VCHK(hipMemcpyAsync(ipiv, h->ipiv, n * sizeof(int), hipMemcpyDeviceToHost, h->s));
+    VCHK(hipStreamSynchronize(h->s));
+    return AFX_OK;
+}
+
+int afx_vlm_create(int32_t n_nodes, const double* xyz, int32_t n_stations, const int32_t* soff,
+                   const int32_t* ring_nodes, afx_vlm** out) {
+    if (!xyz || !soff || !ring_nodes || !out) return api_fail(AFX_EARG, "null argument");
+    if (n_nodes <= 0 || n_stations <= 0) return api_fail(AFX_EARG, "empty lattice");
+    if (soff[0] != 0) return api_fail(AFX_EARG, "station_offsets[0] must be 0");
+    for (int s = 0; s < n_stations; ++s)
+        if (soff[s + 1] <= soff[s]) return api_fail(AFX_EARG, "every station needs at least one ring");
+    const int nr = soff[n_stations];
+    for (int k = 0; k < 4 * nr; ++k)
+        if (ring_nodes[k] < 0 || ring_nodes[k] >= n_nodes) return api_fail(AFX_EARG, "ring node id out of range");
+    auto* h = new afx_vlm();
+    auto bail = [&](int rc) { delete h; return rc; };
+    h->nn = n_nodes;
+    h->nr = nr;
+    h->ns = n_stations;
+    h->xyz.assign(xyz, xyz + 3 * (size_t)n_nodes);
+    h->rn.assign(ring_nodes, ring_nodes + 4 * (size_t)nr);
+    h->soff.assign(soff, soff + n_stations + 1);
+    h->dense = new afx_dense();
+    int rc = dense_init(h->dense, nr);
+    if (rc) return bail(rc);
+#define VCHK_C(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return bail(api_fail(AFX_EHIP, std::string(#x) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+    VCHK_C(dalloc(&h->d_nodes, 3 * (size_t)n_nodes));
+    VCHK_C(dalloc(&h->d_corner, 12 * (size_t)nr));
+    VCHK_C(dalloc(&h->d_wcorner, 12 * (size_t)n_stations));
+    VCHK_C(dalloc(&h->d_colloc, 3 * (size_t)nr));
+    VCHK_C(dalloc(&h->d_normal, 3 * (size_t)nr));
+    VCHK_C(dalloc(&h->d_rhs, nr));
+    VCHK_C(dalloc(&h->d_gamma, nr));
+    VCHK_C(dalloc(&h->d_rforce, 3 * (size_t)nr));
+    VCHK_C(dalloc(&h->d_sforce, 3 * (size_t)n_stations));
+    VCHK_C(dalloc(&h->d_rn, 4 * (size_t)nr));
+    VCHK_C(dalloc(&h->d_soff, n_stations + 1));
+    hipStream_t s = h->dense->s;
+    VCHK_C(hipMemcpyAsync(h->d_nodes, xyz, 3 * (size_t)n_nodes * sizeof(double), hipMemcpyHostToDevice, s));
+    VCHK_C(hipMemcpyAsync(h->d_rn, ring_nodes, 4 * (size_t)nr * sizeof(int), hipMemcpyHostToDevice, s));
+    VCHK_C(hipMemcpyAsync(h->d_soff, soff, (n_stations + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    VCHK_C(hipStreamSynchronize(s));
+#undef VCHK_C
+    *out = h;
+    return AFX_OK;
+}
+
+int afx_vlm_destroy(afx_vlm* h) {
+    delete h;
+    return AFX_OK;
+}
+
+int afx_vlm_counts(const afx_vlm* h, int32_t* nn, int32_t* nr, int32_t* ns) {
+    if (!h) return api_fail(AFX_EARG, "null argument");
+    if (nn) *nn = h->nn;
+    if (nr) *nr = h->nr;
+    if (ns) *ns = h->ns;
+    return AFX_OK;
+}
+
+int afx_vlm_set_flow(afx_vlm* h, const afx_vlm_flow* f) {
+    if (!h || !f) return api_fail(AFX_EARG, "null argument");
+    if (!(f->vinf > 0.0) || !(f->rho > 0.0) || !(f->sref > 0.0) || !(f->cref > 0.0))
+        return api_fail(AFX_EARG, "vinf, rho, sref and cref must be positive");
+    if (f->core_radius < 0.0) return api_fail(AFX_EARG, "core radius must be >= 0");
+    h->flow = *f;
+    if (!(h->flow.wake_length > 0.0)) h->flow.wake_length = 100.0 * f->cref;   // solver.cpp:36
+    h->flow_set = true;
+    h->built = h->solved = false;
+    return AFX_OK;
+}
+
+int afx_vlm_set_nodes(afx_vlm* h, const double* xyz) {
+    if (!h || !xyz) return api_fail(AFX_EARG, "null argument");
+    std::copy(xyz, xyz + 3 * (size_t)h->nn, h->xyz.begin());
+    VCHK(hipMemcpyAsync(h->d_nodes, xyz, 3 * (size_t)h->nn * sizeof(double), hipMemcpyHostToDevice, h->dense->s));
+    VCHK(hipStreamSynchronize(h->dense->s));
+    h->built = h->solved = false;
+    return AFX_OK;
+}
+
+int afx_vlm_get_nodes(const afx_vlm* h, double* xyz) {
+    if (!h || !xyz) return api_fail(AFX_EARG, "null argument");
+    std::copy(h->xyz.begin(), h->xyz.end(), xyz);
+    return AFX_OK;
+}
+
+int afx_vlm_get_rings(const afx_vlm* h, int32_t* ring_nodes, int32_t* soff) {
+    if (!h) return api_fail(AFX_EARG, "null argument");
+    if (ring_nodes) std::copy(h->rn.begin(), h->rn.end(), ring_nodes);
+    if (soff) std::copy(h->soff.begin(), h->soff.end(), soff);
+    return AFX_OK;
+}
+
+int afx_vlm_build(afx_vlm* h) {
+    if (!h) return api_fail(AFX_EARG, "null argument");
+    if (!h->flow_set) return api_fail(AFX_ESTATE, "flow not set");
+    const afx_vlm_flow& f = h->flow;
+    const double a = f.alpha_deg * M_PI / 180.0;
+    const double vx = f.vinf * std::cos(a), vy = 0.0, vz = f.vinf * std::sin(a);
+    const double kf = flow_kfac(f.alpha_deg);
+    const int nr = h->nr, ns = h->ns;
+    hipStream_t s = h->dense->s;
+    k_ring_geom<<<(nr + 127) / 128, 128, 0, s>>>(h->d_nodes, h->d_rn, nr, kf, vx, vy, vz, h->d_corner, h->d_colloc,
+                                                 h->d_normal, h->d_rhs);
+    k_wake<<<(ns + 127) / 128, 128, 0, s>>>(h->d_corner, h->d_soff, ns, f.wake_length, std::cos(a), 0.0, std::sin(a),
+                                            h->d_wcorner);
+    dim3 g((nr + 63) / 64, (nr + 63) / 64);
+    k_aic<<<g, 256, 0, s>>>(h->d_colloc, h->d_normal, h->d_corner, nr, f.core_radius, h->dense->A64);
+    const long long nw = (long long)nr * ns;
+    k_aic_wake<<<(unsigned)((nw + 255) / 256), 256, 0, s>>>(h->d_colloc, h->d_normal, h->d_wcorner, h->d_soff, ns,
+                                                            nr, f.core_radius, h->dense->A64);
+    VCHK(hipGetLastError());
+    VCHK(hipStreamSynchronize(s));
+    h->dense->anorm = -1.0;
+    h->built = true;
+    h->solved = false;
+    return AFX_OK;
+}
+
+int afx_vlm_factor(afx_vlm* h) {
+    if (!h) return api_fail(AFX_EARG, "null argument");
+    if (!h->built) return api_fail(AFX_ESTATE, "matrix not built");
+    int32_t info = 0;
+    return dense_factor(h->dense, &info);
+}
+
+int afx_vlm_solve(afx_vlm* h, double tol, int32_t max_iter, double* gamma, afx_solve_info* info) {
+    if (!h) return api_fail(AFX_EARG, "null argument");
+    if (!h->built) return api_fail(AFX_ESTATE, "matrix not built");
+    afx_solve_info inf{};
+    int rc = dense_solve_dev(h->dense, h->d_rhs, h->d_gamma, tol, max_iter, &inf);
+    if (rc) return rc;
+    if (info) *info = inf;
+    if (gamma) {
+        VCHK(hipMemcpyAsync(gamma, h->d_gamma, h->nr * sizeof(double), hipMemcpyDeviceToHost, h->dense->s));
+        VCHK(hipStreamSynchronize(h->dense->s));
+    }
+    h->solved = true;
+    return AFX_OK;
+}
+
+int afx_vlm_forces(afx_vlm* h, double* ring_force, double* station_force, double* coeffs) {
+    if (!h) return api_fail(AFX_EARG, "null argument");
+    if (!h->solved) return api_fail(AFX_ESTATE, "no solution: call afx_vlm_solve first");
+    const afx_vlm_flow& f = h->flow;
+    const double a = f.alpha_deg * M_PI / 180.0;
+    const double vx = f.vinf * std::cos(a), vz = f.vinf * std::sin(a);
+    hipStream_t s = h->dense->s;
+    k_forces<<<(h->ns + 127) / 128, 128, 0, s>>>(h->d_corner, h->d_gamma, h->d_soff, h->ns, f.rho, vx, 0.0, vz,
+                                                 h->d_rforce, h->d_sforce);
+    VCHK(hipGetLastError());
+    std::vector<double> sf(3 * (size_t)h->ns);
+    VCHK(hipMemcpyAsync(sf.data(), h->d_sforce, sf.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (ring_force)
+        VCHK(hipMemcpyAsync(ring_force, h->d_rforce, 3 * (size_t)h->nr * sizeof(double), hipMemcpyDeviceToHost, s));
+    VCHK(hipStreamSynchronize(s));
+    if (station_force) std::copy(sf.begin(), sf.end(), station_force);
+    if (coeffs) {
+        // lift axis = freestream x unit-y (input.cpp:46-52), stream axis = freestream
+        const double q = 0.5 * f.rho * f.vinf * f.vinf;
+        const double lx = -std::sin(a), lz = std::cos(a);
+        double cl = 0.0, cy = 0.0, cd = 0.0;
+        for (int st = 0; st < h->ns; ++st) {
+            const double* F = &sf[3 * st];
+            cl += (F[0] * lx + F[2] * lz) / (q * f.sref);
+            cy += F[1] / (q * f.sref);
+            cd += (F[0] * std::cos(a) + F[2] * std::sin(a)) / (q * f.sref);
+        }
+        coeffs[0] = cl;
+        coeffs[1] = cy;
+        coeffs[2] = cd;
+    }
+    return AFX_OK;
+}
+
+int afx_vlm_get_aic(afx_vlm* h, double* aic) {
+    if (!h || !aic) return api_fail(AFX_EARG, "null argument");
+    if (!h->built) return api_fail(AFX_ESTATE, "matrix not built");
+    VCHK(hipMemcpyAsync(aic, h->dense->A64, (size_t)h->nr * h->nr * sizeof(double), hipMemcpyDeviceToHost,
+                        h->dense->s));
+    VCHK(hipStreamSynchronize(h->dense->s));
+    return AFX_OK;
+}
+
+int afx_vlm_get_system(afx_vlm* h, double* rhs, double* gamma, double* colloc, double* normal) {
+    if (!h) return api_fail(AFX_EARG, "null argument");
+    hipStream_t s = h->dense->s;
+    const size_t nr = h->nr;
+    if (rhs) VCHK(hipMemcpyAsync(rhs, h->d_rhs, nr * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (gamma) VCHK(hipMemcpyAsync(gamma, h->d_gamma, nr * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (colloc) VCHK(hipMemcpyAsync(colloc, h->d_colloc, 3 * nr * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (normal) VCHK(hipMemcpyAsync(normal, h->d_normal, 3 * nr * sizeof(double), hipMemcpyDeviceToHost, s));
+    VCHK(hipStreamSynchronize(s));
+    return AFX_OK;
+}
+
+}  // extern "C"

@@ -1,4 +1,5 @@
-"""Build the oracle (CPU FP64 restatement of the reference residual).
+"""Build the oracle (CPU FP64 restatement of the reference residual and of
+the vortex-lattice / beam coupling path).
 
 Test infrastructure only: produces oracle/liboracle_rans.so with g++ and
 -ffp-contract=off so that every operation rounds as written.
@@ -12,10 +13,10 @@ LIB = os.path.join(HERE, "liboracle_rans.so")
 
 
 def build():
-    src = os.path.join(HERE, "oracle_rans.cpp")
+    srcs = [os.path.join(HERE, "oracle_rans.cpp"), os.path.join(HERE, "oracle_vlm.cpp")]
     tmp = LIB + ".tmp"
-    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-           src, "-o", tmp]
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math"] + srcs + [
+        "-o", tmp]
     print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)

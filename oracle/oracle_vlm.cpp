@@ -1,0 +1,393 @@
+// Oracle (test infrastructure only): CPU FP64 restatement of the reference's
+// vortex-lattice path (src/vlm/src) and of its linear Euler-Bernoulli frame
+// model (src/structure/include/structure), plus the load / displacement
+// transfer that src/aeroelasticite intends (aerotools.cpp:25-217 does not
+// compile, so the transfer follows its stated design: nearest two beam nodes
+// along the span, lever-arm moments).  Only tests, the smoke check and the
+// CPU baseline of the benchmark call it.
+//
+// Each function follows the reference line by line where one exists:
+//   ovl_segment   potential.cpp:36-58 (vortexLine::influence, distToPoint,
+//                 inducedVelOrientation)
+//   ovl_ring      panel.cpp:122-128 (vortexRing::influence)
+//   ovl_geometry  panel.cpp:46-72 (centre, normal, area), 111-120 (collocation)
+//   ovl_build     solver.cpp:139-254 (buildLHS), 256-290 (buildRHS),
+//                 surface.cpp:35-58 (generateWake), solver.cpp:33-37 (wake
+//                 length 100 cref)
+//   ovl_solve     the exact solution that solver.cpp:41-50 approximates with
+//                 BiCGSTAB: dense LU with partial pivoting in FP64
+//   ovl_forces    surface.cpp:164-201 (wingStation::computeForces), solver.cpp:
+//                 112-128 and surface.cpp:262-277 (CL summed over stations)
+//   ovl_beam_*    element.hpp:77-160 (set_K_local, set_T_Rotation,
+//                 set_K_elem_global), model.hpp:103-131 (assembly),
+//                 225-238 (linear solve) with SPC1 as row/column zeroing
+//
+// Pinning (tests/test_oracle_vlm.py): segment and ring closed forms of the
+// Biot-Savart law, Helmbold's lifting-surface lift slope, symmetry, Kutta-
+// Joukowski total lift against the circulation sum, cantilever closed forms
+// (examples/structure/Moment.txt, tip force, torsion, axial), rotation
+// invariance of the frame, and conservation of force and moment by the load
+// transfer.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+struct V3 {
+    double x, y, z;
+};
+V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3 operator*(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+double nrm(V3 a) { return std::sqrt(dot(a, a)); }
+V3 load3(const double* p) { return {p[0], p[1], p[2]}; }
+void store3(double* p, V3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+
+// potential.cpp:36-58 with gamma = 1 (vortexRing::initialize, panel.cpp:87-93)
+V3 segment(V3 P, V3 a, V3 b, double rc) {
+    V3 r1 = P - a, r2 = P - b;
+    V3 r1r2 = cross(r1, r2);
+    double norm = nrm(r1r2);
+    V3 dl = b - a;
+    double dlr1 = dot(dl, r1), dlr2 = dot(dl, r2);
+    double h = norm / (dlr1 / nrm(r1) - dlr2 / nrm(r2));
+    V3 dir = cross(dl, r1);
+    dir = (1.0 / nrm(dir)) * dir;
+    double lo = rc > 0.0 ? 1.0 - std::exp(-(h / rc) * (h / rc)) : 1.0;   // rc = 0: exp(-inf) = 0
+    return (1.0 / (4.0 * M_PI * h) * lo) * dir;
+}
+
+V3 ring(V3 P, const V3 q[4], double rc) {
+    V3 v{0, 0, 0};
+    for (int i = 0; i < 4; ++i) v = v + segment(P, q[i], q[(i + 1) % 4], rc);
+    return v;
+}
+
+double kfac(double aoa) {
+    if (aoa < 1e-6 && aoa > -1e6) return 1.0;
+    return aoa * M_PI / (180.0 * std::sin(aoa * M_PI / 180.0));
+}
+
+struct Lattice {
+    int nn, nr, ns;
+    const double* xyz;
+    const int32_t* soff;
+    const int32_t* rn;
+    V3 node(int ring, int c) const { return load3(xyz + 3 * rn[4 * ring + c]); }
+};
+
+void geometry(const Lattice& L, int k, double aoa, V3& colloc, V3& normal, double& area) {
+    V3 p[4];
+    for (int c = 0; c < 4; ++c) p[c] = L.node(k, c);
+    V3 center{0, 0, 0};
+    for (int c = 0; c < 4; ++c) center = center + p[c];
+    center = (1.0 / 4.0) * center;
+    V3 dl[4];
+    for (int c = 0; c < 4; ++c) dl[c] = p[(c + 1) % 4] - p[c];
+    V3 n{0, 0, 0};
+    for (int c = 0; c < 3; ++c) n = n - cross(dl[c], dl[c + 1]);
+    n = n - cross(dl[3], dl[0]);
+    normal = (1.0 / nrm(n)) * n;
+    area = 0.0;
+    for (int c = 0; c < 4; ++c) area += 0.5 * nrm(cross(dl[c], center - p[c]));
+    double kk = kfac(aoa);
+    colloc = kk * center + (1 - kk) * (0.5 * (p[0] + p[1]));
+}
+
+V3 freestream(double aoa, double vinf) {
+    double a = aoa * M_PI / 180.0;
+    return {vinf * std::cos(a), 0.0, vinf * std::sin(a)};
+}
+
+// dense LU with partial pivoting, FP64, in place on a copy
+bool lu_solve(int n, std::vector<double> A, std::vector<double> b, double* x) {
+    std::vector<int> piv(n);
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(A[(size_t)i * n + k]) > std::fabs(A[(size_t)p * n + k])) p = i;
+        if (A[(size_t)p * n + k] == 0.0) return false;
+        if (p != k) {
+            for (int j = 0; j < n; ++j) std::swap(A[(size_t)k * n + j], A[(size_t)p * n + j]);
+            std::swap(b[k], b[p]);
+        }
+        for (int i = k + 1; i < n; ++i) {
+            double l = A[(size_t)i * n + k] / A[(size_t)k * n + k];
+            A[(size_t)i * n + k] = l;
+            for (int j = k + 1; j < n; ++j) A[(size_t)i * n + j] -= l * A[(size_t)k * n + j];
+            b[i] -= l * b[k];
+        }
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double s = b[i];
+        for (int j = i + 1; j < n; ++j) s -= A[(size_t)i * n + j] * x[j];
+        x[i] = s / A[(size_t)i * n + i];
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ beam --
+struct Beam {
+    int n, ne;
+    const double* xyz;
+    const int32_t* en;
+    const double* props;
+    std::vector<char> fixed;
+};
+
+void element_global(const Beam& B, int e, double Kg[12][12]) {
+    V3 N1 = load3(B.xyz + 3 * B.en[2 * e]), N2 = load3(B.xyz + 3 * B.en[2 * e + 1]);
+    const double* p = B.props + 7 * e;
+    double L = nrm(N2 - N1);
+    double Li = 1 / L;
+    // set_K_local: X, Y1..Y4 (Iz), Z1..Z4 (Iy), S
+    double X = p[0] * Li;
+    double Y2 = 6. * p[2] * Li * Li, Y1 = Y2 * 2. * Li, Y3 = 4. * p[2] * Li, Y4 = Y3 / 2.;
+    double Z2 = 6. * p[1] * Li * Li, Z1 = Z2 * 2. * Li, Z3 = 4. * p[1] * Li, Z4 = Z3 / 2.;
+    double S = p[3] * Li;
+    const double Kl[12][12] = {
+        {X, 0, 0, 0, 0, 0, -X, 0, 0, 0, 0, 0},       {0, Y1, 0, 0, 0, Y2, 0, -Y1, 0, 0, 0, Y2},
+        {0, 0, Z1, 0, -Z2, 0, 0, 0, -Z1, 0, -Z2, 0}, {0, 0, 0, S, 0, 0, 0, 0, 0, -S, 0, 0},
+        {0, 0, -Z2, 0, Z3, 0, 0, 0, Z2, 0, Z4, 0},   {0, Y2, 0, 0, 0, Y3, 0, -Y2, 0, 0, 0, Y4},
+        {-X, 0, 0, 0, 0, 0, X, 0, 0, 0, 0, 0},       {0, -Y1, 0, 0, 0, -Y2, 0, Y1, 0, 0, 0, -Y2},
+        {0, 0, -Z1, 0, Z2, 0, 0, 0, Z1, 0, Z2, 0},   {0, 0, 0, -S, 0, 0, 0, 0, 0, S, 0, 0},
+        {0, 0, -Z2, 0, Z4, 0, 0, 0, Z2, 0, Z3, 0},   {0, Y2, 0, 0, 0, Y4, 0, -Y2, 0, 0, 0, Y3}};
+    // set_T_Rotation
+    V3 ex = Li * (N2 - N1);
+    V3 v = load3(p + 4);
+    v = (1.0 / nrm(v)) * v;
+    V3 ez = cross(ex, v);
+    ez = (1.0 / nrm(ez)) * ez;
+    V3 ey = -1.0 * cross(ex, ez);
+    double lam[3][3] = {{ex.x, ex.y, ex.z}, {ey.x, ey.y, ey.z}, {ez.x, ez.y, ez.z}};
+    double T[12][12] = {};
+    for (int b = 0; b < 4; ++b)
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) T[3 * b + r][3 * b + c] = lam[r][c];
+    // set_K_elem_global: T^T K T
+    for (int i = 0; i < 12; ++i)
+        for (int j = 0; j < 12; ++j) {
+            double s = 0;
+            for (int a = 0; a < 12; ++a)
+                for (int b = 0; b < 12; ++b) s += T[a][i] * Kl[a][b] * T[b][j];
+            Kg[i][j] = s;
+        }
+}
+
+void beam_solve(const Beam& B, const double* f, double* u) {
+    const int nd = 6 * B.n;
+    std::vector<double> K((size_t)nd * nd, 0.0), rhs(f, f + nd);
+    for (int e = 0; e < B.ne; ++e) {
+        double Kg[12][12];
+        element_global(B, e, Kg);
+        int d[2] = {6 * B.en[2 * e], 6 * B.en[2 * e + 1]};
+        for (int a = 0; a < 2; ++a)
+            for (int c = 0; c < 2; ++c)
+                for (int i = 0; i < 6; ++i)
+                    for (int j = 0; j < 6; ++j) K[(size_t)(d[a] + i) * nd + d[c] + j] += Kg[6 * a + i][6 * c + j];
+    }
+    for (int i = 0; i < nd; ++i)
+        if (B.fixed[i / 6]) {
+            for (int j = 0; j < nd; ++j) K[(size_t)i * nd + j] = K[(size_t)j * nd + i] = 0.0;
+            K[(size_t)i * nd + i] = 1.0;
+            rhs[i] = 0.0;
+        }
+    lu_solve(nd, K, rhs, u);
+}
+
+// nearest two beam nodes bracketing y, else the nearest node
+void bracket(const Beam& B, double y, int& a, int& c, double& wa) {
+    for (int e = 0; e < B.ne; ++e) {
+        int n1 = B.en[2 * e], n2 = B.en[2 * e + 1];
+        double y1 = B.xyz[3 * n1 + 1], y2 = B.xyz[3 * n2 + 1];
+        if (y1 == y2) continue;
+        if (y >= std::min(y1, y2) && y <= std::max(y1, y2)) {
+            a = n1;
+            c = n2;
+            wa = (y2 - y) / (y2 - y1);
+            return;
+        }
+    }
+    int best = 0;
+    for (int k = 1; k < B.n; ++k)
+        if (std::fabs(B.xyz[3 * k + 1] - y) < std::fabs(B.xyz[3 * best + 1] - y)) best = k;
+    a = c = best;
+    wa = 1.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ovl_segment(const double* P, const double* a, const double* b, double rc, double* v) {
+    store3(v, segment(load3(P), load3(a), load3(b), rc));
+}
+
+void ovl_ring(const double* P, const double* q12, double rc, double* v) {
+    V3 q[4];
+    for (int c = 0; c < 4; ++c) q[c] = load3(q12 + 3 * c);
+    store3(v, ring(load3(P), q, rc));
+}
+
+void ovl_geometry(int nn, const double* xyz, int ns, const int32_t* soff, const int32_t* rn, double aoa,
+                  double* colloc, double* normal, double* area) {
+    Lattice L{nn, soff[ns], ns, xyz, soff, rn};
+    for (int k = 0; k < L.nr; ++k) {
+        V3 c, n;
+        double a;
+        geometry(L, k, aoa, c, n, a);
+        if (colloc) store3(colloc + 3 * k, c);
+        if (normal) store3(normal + 3 * k, n);
+        if (area) area[k] = a;
+    }
+}
+
+// lhs (row-major N x N) and rhs of the linear steady solver
+void ovl_build(int nn, const double* xyz, int ns, const int32_t* soff, const int32_t* rn, double aoa, double vinf,
+               double wake_length, double rc, double* A, double* rhs) {
+    Lattice L{nn, soff[ns], ns, xyz, soff, rn};
+    const int N = L.nr;
+    std::vector<V3> P(N), n(N);
+    for (int k = 0; k < N; ++k) {
+        double a;
+        geometry(L, k, aoa, P[k], n[k], a);
+    }
+    V3 Vf = freestream(aoa, vinf);
+    V3 w = (1.0 / nrm(Vf)) * Vf;
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            V3 q[4];
+            for (int c = 0; c < 4; ++c) q[c] = L.node(j, c);
+            A[(size_t)i * N + j] = dot(ring(P[i], q, rc), n[i]);
+        }
+    for (int s = 0; s < ns; ++s) {
+        int te = soff[s + 1] - 1;
+        V3 p1 = L.node(te, 3), p2 = L.node(te, 2);
+        V3 q[4] = {p1, p2, p2 + wake_length * w, p1 + wake_length * w};
+        for (int i = 0; i < N; ++i) A[(size_t)i * N + te] += dot(ring(P[i], q, rc), n[i]);
+    }
+    for (int i = 0; i < N; ++i) rhs[i] = -dot(Vf, n[i]);
+}
+
+int ovl_solve(int n, const double* A, const double* b, double* x) {
+    return lu_solve(n, std::vector<double>(A, A + (size_t)n * n), std::vector<double>(b, b + n), x) ? 0 : -1;
+}
+
+// ring forces (3 per ring, nullable) and {CL, CY, CD_kj} over (q sref)
+void ovl_forces(int nn, const double* xyz, int ns, const int32_t* soff, const int32_t* rn, const double* gamma,
+                double aoa, double vinf, double rho, double sref, double* rforce, double* coeffs) {
+    Lattice L{nn, soff[ns], ns, xyz, soff, rn};
+    V3 Vf = freestream(aoa, vinf);
+    double a = aoa * M_PI / 180.0;
+    V3 lift{-std::sin(a), 0.0, std::cos(a)};
+    V3 stream = (1.0 / nrm(Vf)) * Vf;
+    double q = 0.5 * rho * vinf * vinf;
+    double cl = 0, cy = 0, cd = 0;
+    for (int s = 0; s < ns; ++s) {
+        double prev = 0.0;
+        V3 F{0, 0, 0};
+        for (int k = soff[s]; k < soff[s + 1]; ++k) {
+            V3 dl = L.node(k, 1) - L.node(k, 0);
+            V3 f = (rho * (gamma[k] - prev)) * cross(Vf, dl);
+            if (rforce) store3(rforce + 3 * k, f);
+            F = F + f;
+            prev = gamma[k];
+        }
+        cl += dot(F, lift) / (q * sref);
+        cy += F.y / (q * sref);
+        cd += dot(F, stream) / (q * sref);
+    }
+    if (coeffs) { coeffs[0] = cl; coeffs[1] = cy; coeffs[2] = cd; }
+}
+
+// linear frame model: K u = f with SPC1 (all six dofs) on the clamped nodes
+void ovl_beam_solve(int n, const double* xyz, int ne, const int32_t* en, const double* props, int nclamped,
+                    const int32_t* clamped, const double* f, double* u) {
+    Beam B{n, ne, xyz, en, props, std::vector<char>(n, 0)};
+    for (int k = 0; k < nclamped; ++k) B.fixed[clamped[k]] = 1;
+    beam_solve(B, f, u);
+}
+
+// beam loads from ring forces at the undeformed force points, and lattice
+// node displacements from a beam displacement (the transfer's two halves)
+void ovl_transfer_loads(int nr, const double* fpoints, const double* rforce, int n, const double* bxyz, int ne,
+                        const int32_t* en, double* load) {
+    Beam B{n, ne, bxyz, en, nullptr, {}};
+    std::fill(load, load + 6 * n, 0.0);
+    for (int k = 0; k < nr; ++k) {
+        V3 p = load3(fpoints + 3 * k), F = load3(rforce + 3 * k);
+        int a, c;
+        double wa;
+        bracket(B, p.y, a, c, wa);
+        int nodes[2] = {a, c};
+        double ws[2] = {wa, 1.0 - wa};
+        for (int q = 0; q < 2; ++q) {
+            V3 M = cross(p - load3(bxyz + 3 * nodes[q]), F);
+            double* l = load + 6 * nodes[q];
+            l[0] += ws[q] * F.x; l[1] += ws[q] * F.y; l[2] += ws[q] * F.z;
+            l[3] += ws[q] * M.x; l[4] += ws[q] * M.y; l[5] += ws[q] * M.z;
+        }
+    }
+}
+
+void ovl_transfer_disp(int nn, const double* x0, int n, const double* bxyz, int ne, const int32_t* en,
+                       const double* disp, double* xyz) {
+    Beam B{n, ne, bxyz, en, nullptr, {}};
+    for (int k = 0; k < nn; ++k) {
+        V3 p = load3(x0 + 3 * k);
+        int a, c;
+        double wa;
+        bracket(B, p.y, a, c, wa);
+        int nodes[2] = {a, c};
+        double ws[2] = {wa, 1.0 - wa};
+        V3 u{0, 0, 0};
+        for (int q = 0; q < 2; ++q) {
+            const double* d = disp + 6 * nodes[q];
+            V3 rot{d[3], d[4], d[5]};
+            u = u + ws[q] * (load3(d) + cross(rot, p - load3(bxyz + 3 * nodes[q])));
+        }
+        store3(xyz + 3 * k, p + u);
+    }
+}
+
+// fixed-point coupling with exact FP64 solves; out = {iterations, tip, CL}
+void ovl_aeroelastic(int nn, const double* x0, int ns, const int32_t* soff, const int32_t* rn, double aoa,
+                     double vinf, double rho, double sref, double cref, int n, const double* bxyz, int ne,
+                     const int32_t* en, const double* props, int nclamped, const int32_t* clamped, double tol,
+                     int max_iter, double relax, double* disp, double* out) {
+    const int N = soff[ns];
+    int tip = 0;
+    for (int k = 1; k < n; ++k)
+        if (bxyz[3 * k + 1] > bxyz[3 * tip + 1]) tip = k;
+    std::vector<double> xyz(3 * (size_t)nn), A((size_t)N * N), rhs(N), g(N), rf(3 * (size_t)N), fp(3 * (size_t)N);
+    std::vector<double> load(6 * (size_t)n), unew(6 * (size_t)n);
+    for (int k = 0; k < N; ++k)
+        for (int d = 0; d < 3; ++d) fp[3 * k + d] = 0.5 * (x0[3 * rn[4 * k] + d] + x0[3 * rn[4 * k + 1] + d]);
+    std::fill(disp, disp + 6 * n, 0.0);
+    double prev = 0.0, cl = 0.0;
+    int it = 1;
+    for (; it <= max_iter; ++it) {
+        ovl_transfer_disp(nn, x0, n, bxyz, ne, en, disp, xyz.data());
+        ovl_build(nn, xyz.data(), ns, soff, rn, aoa, vinf, 100.0 * cref, 0.0, A.data(), rhs.data());
+        ovl_solve(N, A.data(), rhs.data(), g.data());
+        double c[3];
+        ovl_forces(nn, xyz.data(), ns, soff, rn, g.data(), aoa, vinf, rho, sref, rf.data(), c);
+        cl = c[0];
+        ovl_transfer_loads(N, fp.data(), rf.data(), n, bxyz, ne, en, load.data());
+        ovl_beam_solve(n, bxyz, ne, en, props, nclamped, clamped, load.data(), unew.data());
+        for (int i = 0; i < 6 * n; ++i) disp[i] += relax * (unew[i] - disp[i]);
+        double t = disp[6 * tip + 2];
+        if (std::fabs(t - prev) <= tol * std::max(std::fabs(t), 1e-300)) break;
+        prev = t;
+    }
+    out[0] = std::min(it, max_iter);
+    out[1] = disp[6 * tip + 2];
+    out[2] = cl;
+}
+
+}  // extern "C"

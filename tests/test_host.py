@@ -166,7 +166,10 @@ def test_tile_layout_statistics(tile):
 
 
 def header_symbols():
-    txt = open(os.path.join(ROOT, "include", "afx_rans.h")).read()
+    txt = ""
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            txt += open(os.path.join(ROOT, "include", h)).read()
     return sorted(set(re.findall(r"\b(afx_[a-z0-9_]+)\s*\(", txt)))
 
 
