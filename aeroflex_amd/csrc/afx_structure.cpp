@@ -411,12 +411,13 @@ int afx_aeroelastic_solve(afx_aeroelastic* h, const afx_coupling_opts* o, afx_co
             ++inf.refactors;
         }
         afx_solve_info si{};
-        if ((rc = afx_vlm_solve(h->v, opts.solve_tol, opts.solve_max_iter, nullptr, &si))) return rc;
+        const int warm = it > 1 ? 1 : 0;   // the previous iteration's Gamma starts the refinement
+        if ((rc = afx_vlm_solve(h->v, opts.solve_tol, opts.solve_max_iter, warm, nullptr, &si))) return rc;
         inf.refine_steps += si.iterations;
         if (!si.converged || si.contraction > opts.refactor_above) {
             if ((rc = afx_vlm_factor(h->v))) return rc;
             ++inf.refactors;
-            if ((rc = afx_vlm_solve(h->v, opts.solve_tol, opts.solve_max_iter, nullptr, &si))) return rc;
+            if ((rc = afx_vlm_solve(h->v, opts.solve_tol, opts.solve_max_iter, warm, nullptr, &si))) return rc;
             inf.refine_steps += si.iterations;
             if (!si.converged) return api_fail(AFX_EARG, "vortex-lattice refinement did not converge after refactoring");
         }

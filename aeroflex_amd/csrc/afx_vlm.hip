@@ -282,59 +282,118 @@ __global__ void k_gather_perm(const double* __restrict__ r, const int* __restric
     if (i < n) y[i] = (float)r[perm[i]];
 }
 
-// diagonal block of L (unit lower) or U (upper): y_b <- T_bb^-1 y_b, one
-// wavefront doing the substitution from LDS
+// Inverse of every 128 x 128 diagonal block of the factor, computed in
+// place in LDS column by column (LAPACK strti2): unit lower L -> L^-1 and
+// upper U -> U^-1, blocks past n padded with the identity.  One workgroup per
+// block; the triangular solves apply these with a matrix-vector product
+// instead of a 128-step substitution.
 template <bool LOWER>
-__global__ __launch_bounds__(256) void k_diag_solve(const float* __restrict__ A, int lda, float* __restrict__ y,
-                                                    int b0, int nb) {
-    __shared__ float Ts[LU_NB * LU_NB];
-    const int t = threadIdx.x;
-    for (int idx = t; idx < nb * nb; idx += 256) {
-        const int k = idx / nb, r = idx - k * nb;
-        Ts[k * LU_NB + r] = A[(size_t)(b0 + k) * lda + b0 + r];
+__global__ __launch_bounds__(128) void k_tri_inv(const float* __restrict__ A, int lda, int n,
+                                                 float* __restrict__ Tinv) {
+    __shared__ float X[LU_NB * LU_NB];   // column k at X[k * LU_NB]
+    const int i = threadIdx.x, b0 = blockIdx.x * LU_NB, nb = min(LU_NB, n - b0);
+    for (int k = 0; k < LU_NB; ++k) {
+        float v = (k == i) ? 1.f : 0.f;
+        if (k < nb && i < nb) {
+            const bool keep = LOWER ? (i > k) : (i <= k);
+            v = keep ? A[(size_t)(b0 + k) * lda + b0 + i] : ((k == i) ? 1.f : 0.f);
+        }
+        X[k * LU_NB + i] = v;
     }
     __syncthreads();
-    if (t >= 64) return;
-    const int lane = t;
-    float v0 = lane < nb ? y[b0 + lane] : 0.f;
-    float v1 = lane + 64 < nb ? y[b0 + lane + 64] : 0.f;
     if (LOWER) {
-        for (int k = 0; k < nb; ++k) {
-            const float xk = k < 64 ? readlane_f(v0, k) : readlane_f(v1, k - 64);
-            if (lane > k && lane < nb) v0 -= Ts[k * LU_NB + lane] * xk;
-            if (lane + 64 > k && lane + 64 < nb) v1 -= Ts[k * LU_NB + lane + 64] * xk;
+        for (int j = LU_NB - 2; j >= 0; --j) {
+            float s = 0.f;
+            if (i > j) {
+                s = X[j * LU_NB + i];
+                for (int k = j + 1; k < i; ++k) s += X[k * LU_NB + i] * X[j * LU_NB + k];
+            }
+            __syncthreads();
+            if (i > j) X[j * LU_NB + i] = -s;
+            __syncthreads();
         }
     } else {
-        for (int k = nb - 1; k >= 0; --k) {
-            const float d = Ts[k * LU_NB + k];
-            if (k < 64) { if (lane == k) v0 /= d; } else { if (lane + 64 == k) v1 /= d; }
-            const float xk = k < 64 ? readlane_f(v0, k) : readlane_f(v1, k - 64);
-            if (lane < k) v0 -= Ts[k * LU_NB + lane] * xk;
-            if (lane + 64 < k) v1 -= Ts[k * LU_NB + lane + 64] * xk;
+        for (int j = 0; j < LU_NB; ++j) {
+            const float d = 1.f / X[j * LU_NB + j];
+            float s = 0.f;
+            if (i < j)
+                for (int k = i; k < j; ++k) s += X[k * LU_NB + i] * X[j * LU_NB + k];
+            __syncthreads();
+            if (i < j) X[j * LU_NB + i] = -d * s;
+            if (i == j) X[j * LU_NB + j] = d;
+            __syncthreads();
         }
     }
-    if (lane < nb) y[b0 + lane] = v0;
-    if (lane + 64 < nb) y[b0 + lane + 64] = v1;
+    float* out = Tinv + (size_t)blockIdx.x * LU_NB * LU_NB;
+    for (int k = 0; k < LU_NB; ++k) out[k * LU_NB + i] = X[k * LU_NB + i];
 }
 
-// y_i -= sum_k A(i, b0 + k) y_(b0 + k) for rows i in [r0, r1), one thread per row
-__global__ __launch_bounds__(64) void k_block_update(const float* __restrict__ A, int lda, float* __restrict__ y,
-                                                     int b0, int nb, int r0, int r1) {
-    __shared__ float yb[LU_NB];
-    const int t = threadIdx.x;
-    for (int k = t; k < nb; k += 64) yb[k] = y[b0 + k];
+// Blocked triangular solve in one launch (y <- L^-1 y or U^-1 y in place).
+// Each workgroup takes a ticket; ticket t owns block row t (forward) or
+// nblk - 1 - t (backward), so a workgroup only waits on blocks owned by
+// workgroups that started before it.  It accumulates T(b, c) x_c over the
+// blocks c it depends on as their flags are published (the block of T is
+// loaded before the wait), applies the inverse of its diagonal block and
+// publishes x_b.  Every wait is bounded (abort flag, the grid still drains).
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv(const float* __restrict__ A, int lda, const float* __restrict__ Tinv,
+                                              float* __restrict__ y, int n, int nblk, unsigned* ticket,
+                                              unsigned base, unsigned* flags, unsigned epoch, unsigned* abort_flag) {
+    __shared__ float xs[LU_NB];
+    __shared__ float part[2][LU_NB];
+    __shared__ int s_b;
+    const int t = threadIdx.x, r = t & (LU_NB - 1), half = t >> 7;
+    if (t == 0) s_b = (int)(atomicAdd(ticket, 1u) - base);
     __syncthreads();
-    const int i = r0 + blockIdx.x * 64 + t;
-    if (i >= r1) return;
-    const float* col = A + (size_t)b0 * lda + i;
-    float acc0 = 0.f, acc1 = 0.f;
-    int k = 0;
-    for (; k + 1 < nb; k += 2) {
-        acc0 += col[(size_t)k * lda] * yb[k];
-        acc1 += col[(size_t)(k + 1) * lda] * yb[k + 1];
+    const int b = LOWER ? s_b : nblk - 1 - s_b;
+    const int b0 = b * LU_NB, nb = min(LU_NB, n - b0);
+    float tinv[LU_NB / 2];
+    const float* Tb = Tinv + (size_t)b * LU_NB * LU_NB;
+#pragma unroll
+    for (int q = 0; q < LU_NB / 2; ++q) tinv[q] = Tb[(2 * q + half) * LU_NB + r];
+    float acc = 0.f;
+    const int ndep = LOWER ? b : nblk - 1 - b;
+    for (int d = 0; d < ndep; ++d) {
+        const int c = LOWER ? d : nblk - 1 - d;
+        const int c0 = c * LU_NB, nc = min(LU_NB, n - c0);
+        float a[LU_NB / 2];
+#pragma unroll
+        for (int q = 0; q < LU_NB / 2; ++q) {
+            const int k = 2 * q + half;
+            a[q] = (k < nc && r < nb) ? A[(size_t)(c0 + k) * lda + b0 + r] : 0.f;
+        }
+        if (t == 0) {
+            unsigned spins = 0;
+            while (__hip_atomic_load(&flags[c], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+                if (++spins > SPIN_LIMIT) {
+                    __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (t < LU_NB) xs[t] = t < nc ? y[c0 + t] : 0.f;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < LU_NB / 2; ++q) acc += a[q] * xs[2 * q + half];
+        __syncthreads();
     }
-    if (k < nb) acc0 += col[(size_t)k * lda] * yb[k];
-    y[i] -= acc0 + acc1;
+    part[half][r] = acc;
+    __syncthreads();
+    if (t < LU_NB) xs[t] = t < nb ? y[b0 + t] - (part[0][t] + part[1][t]) : 0.f;
+    __syncthreads();
+    float s = 0.f;
+#pragma unroll
+    for (int q = 0; q < LU_NB / 2; ++q) s += tinv[q] * xs[2 * q + half];
+    part[half][r] = s;
+    __syncthreads();
+    if (t < nb) y[b0 + t] = part[0][t] + part[1][t];
+    __threadfence();
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // x += (double) y
@@ -344,7 +403,7 @@ __global__ void k_add_f32(double* __restrict__ x, const float* __restrict__ y, i
 }
 
 // ------------------------------------------------------ FP64 residual ----
-constexpr int RES_ROWS = 8;   // matrix rows per wavefront (x is read once per 8 rows)
+constexpr int RES_ROWS = 4;   // matrix rows per wavefront (x is read once per 4 rows)
 __global__ __launch_bounds__(256) void k_resid64(const double* __restrict__ A, int n, const double* __restrict__ x,
                                                  const double* __restrict__ b, double* __restrict__ r,
                                                  double* __restrict__ rowabs) {
@@ -352,18 +411,36 @@ __global__ __launch_bounds__(256) void k_resid64(const double* __restrict__ A, i
     const int row0 = (blockIdx.x * 4 + wv) * RES_ROWS;
     if (row0 >= n) return;
     const int nrow = min(RES_ROWS, n - row0);
+    const double* Ar[RES_ROWS];
+#pragma unroll
+    for (int q = 0; q < RES_ROWS; ++q) Ar[q] = A + (size_t)(row0 + min(q, nrow - 1)) * n;
     double acc[RES_ROWS], aab[RES_ROWS];
 #pragma unroll
     for (int q = 0; q < RES_ROWS; ++q) { acc[q] = 0.0; aab[q] = 0.0; }
-    for (int j = lane; j < n; j += 64) {
+    int j = lane;
+    for (; j + 192 < n; j += 256) {
+        double xv[4], av[RES_ROWS][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xv[u] = x[j + 64 * u];
+#pragma unroll
+        for (int q = 0; q < RES_ROWS; ++q)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) av[q][u] = __builtin_nontemporal_load(Ar[q] + j + 64 * u);
+#pragma unroll
+        for (int q = 0; q < RES_ROWS; ++q)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                acc[q] += av[q][u] * xv[u];
+                aab[q] += fabs(av[q][u]);
+            }
+    }
+    for (; j < n; j += 64) {
         const double xj = x[j];
 #pragma unroll
         for (int q = 0; q < RES_ROWS; ++q) {
-            if (q < nrow) {
-                const double a = A[(size_t)(row0 + q) * n + j];
-                acc[q] += a * xj;
-                aab[q] += fabs(a);
-            }
+            const double a = Ar[q][j];
+            acc[q] += a * xj;
+            aab[q] += fabs(a);
         }
     }
 #pragma unroll
@@ -606,12 +683,17 @@ struct afx_dense {
     float* s_diag = nullptr;
     GridBarrier* bar = nullptr;
     int* info = nullptr;
+    float *Linv = nullptr, *Uinv = nullptr;   // inverses of the 128 x 128 diagonal blocks
+    unsigned *tk = nullptr, *flags = nullptr, *abort = nullptr;   // triangular-solve tickets and block flags
+    unsigned tk_base = 0, epoch = 0;
+    int nblk = 0;
     double anorm = -1.0;       // |A|inf of the current FP64 matrix (-1: not yet known)
     bool factored = false;
     ~afx_dense() {
         for (void* p : {(void*)A64, (void*)A32, (void*)ipiv, (void*)perm, (void*)b, (void*)x, (void*)r,
                         (void*)rowabs, (void*)norms, (void*)y, (void*)s_abs, (void*)s_row, (void*)s_vals,
-                        (void*)s_diag, (void*)bar, (void*)info})
+                        (void*)s_diag, (void*)bar, (void*)info, (void*)Linv, (void*)Uinv, (void*)tk,
+                        (void*)flags, (void*)abort})
             if (p) (void)hipFree(p);
         if (rb) rocblas_destroy_handle(rb);
         if (s) (void)hipStreamDestroy(s);
@@ -651,6 +733,16 @@ int dense_init(afx_dense* h, int n) {
     VCHK(dalloc(&h->s_diag, 2 * LU_NB));
     VCHK(dalloc(&h->bar, 1));
     VCHK(dalloc(&h->info, 1));
+    h->nblk = (n + LU_NB - 1) / LU_NB;
+    VCHK(dalloc(&h->Linv, (size_t)h->nblk * LU_NB * LU_NB));
+    VCHK(dalloc(&h->Uinv, (size_t)h->nblk * LU_NB * LU_NB));
+    VCHK(dalloc(&h->tk, 1));
+    VCHK(dalloc(&h->flags, h->nblk));
+    VCHK(dalloc(&h->abort, 1));
+    VCHK(hipMemsetAsync(h->tk, 0, sizeof(unsigned), h->s));
+    VCHK(hipMemsetAsync(h->flags, 0, h->nblk * sizeof(unsigned), h->s));
+    VCHK(hipMemsetAsync(h->abort, 0, sizeof(unsigned), h->s));
+    VCHK(hipStreamSynchronize(h->s));
     return AFX_OK;
 }
 
@@ -707,6 +799,11 @@ int dense_factor(afx_dense* h, int32_t* info_out) {
         std::swap(perm[k], perm[p]);
     }
     VCHK(hipMemcpyAsync(h->perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+    if (!info) {
+        k_tri_inv<true><<<h->nblk, LU_NB, 0, s>>>(h->A32, lda, n, h->Linv);
+        k_tri_inv<false><<<h->nblk, LU_NB, 0, s>>>(h->A32, lda, n, h->Uinv);
+        VCHK(hipGetLastError());
+    }
     VCHK(hipStreamSynchronize(s));
     if (info_out) *info_out = info;
     h->factored = info == 0;
@@ -714,36 +811,30 @@ int dense_factor(afx_dense* h, int32_t* info_out) {
     return AFX_OK;
 }
 
-// y <- U^-1 L^-1 y (y already permuted)
+// y <- U^-1 L^-1 y (y already permuted): one launch per triangle
 int dense_lu_solve(afx_dense* h) {
-    const int n = h->n, lda = h->lda;
     hipStream_t s = h->s;
-    for (int b0 = 0; b0 < n; b0 += LU_NB) {
-        const int nb = std::min(LU_NB, n - b0);
-        k_diag_solve<true><<<1, 256, 0, s>>>(h->A32, lda, h->y, b0, nb);
-        const int r0 = b0 + nb;
-        if (r0 < n) k_block_update<<<(n - r0 + 63) / 64, 64, 0, s>>>(h->A32, lda, h->y, b0, nb, r0, n);
-    }
-    const int last = ((n - 1) / LU_NB) * LU_NB;
-    for (int b0 = last; b0 >= 0; b0 -= LU_NB) {
-        const int nb = std::min(LU_NB, n - b0);
-        k_diag_solve<false><<<1, 256, 0, s>>>(h->A32, lda, h->y, b0, nb);
-        if (b0 > 0) k_block_update<<<(b0 + 63) / 64, 64, 0, s>>>(h->A32, lda, h->y, b0, nb, 0, b0);
-    }
+    const int nb = h->nblk;
+    k_trsv<true><<<nb, 256, 0, s>>>(h->A32, h->lda, h->Linv, h->y, h->n, nb, h->tk, h->tk_base, h->flags,
+                                    ++h->epoch, h->abort);
+    h->tk_base += (unsigned)nb;
+    k_trsv<false><<<nb, 256, 0, s>>>(h->A32, h->lda, h->Uinv, h->y, h->n, nb, h->tk, h->tk_base, h->flags,
+                                     ++h->epoch, h->abort);
+    h->tk_base += (unsigned)nb;
     VCHK(hipGetLastError());
     return AFX_OK;
 }
 
-// FP64 iterative refinement: x = 0; repeat r = b - A x (FP64), stop when the
+// FP64 iterative refinement: x = 0 (or the caller's x when warm); repeat r = b - A x (FP64), stop when the
 // normwise backward error |r| / (|A| |x| + |b|) <= tol, else x += LU^-1 P r.
-int dense_solve_dev(afx_dense* h, const double* b_dev, double* x_dev, double tol, int max_iter,
+int dense_solve_dev(afx_dense* h, const double* b_dev, double* x_dev, double tol, int max_iter, bool warm,
                     afx_solve_info* info) {
     if (!h->factored) return api_fail(AFX_ESTATE, "no factorisation: call factor first");
     const int n = h->n;
     hipStream_t s = h->s;
     if (tol <= 0.0) tol = std::sqrt((double)n) * 1.1102230246251565e-16;
     if (max_iter <= 0) max_iter = 30;
-    VCHK(hipMemsetAsync(x_dev, 0, n * sizeof(double), s));
+    if (!warm) VCHK(hipMemsetAsync(x_dev, 0, n * sizeof(double), s));
     afx_solve_info inf{};
     double prev = -1.0;
     const int tb = 256;
@@ -754,8 +845,11 @@ int dense_solve_dev(afx_dense* h, const double* b_dev, double* x_dev, double tol
         k_norms<<<1, 1024, 0, s>>>(h->r, x_dev, b_dev, need_norm ? h->rowabs : nullptr, n, h->norms);
         VCHK(hipGetLastError());
         double nrm[4];
+        unsigned aborted = 0;
         VCHK(hipMemcpyAsync(nrm, h->norms, sizeof(nrm), hipMemcpyDeviceToHost, s));
+        VCHK(hipMemcpyAsync(&aborted, h->abort, sizeof(unsigned), hipMemcpyDeviceToHost, s));
         VCHK(hipStreamSynchronize(s));
+        if (aborted) return api_fail(AFX_EHIP, "triangular solve: a block wait timed out");
         if (need_norm) h->anorm = nrm[3];
         const double denom = h->anorm * nrm[1] + nrm[2];
         inf.backward_error = denom > 0.0 ? nrm[0] / denom : 0.0;
@@ -784,7 +878,7 @@ struct afx_vlm {
     std::vector<double> xyz;
     std::vector<int> rn, soff;
     afx_vlm_flow flow{};
-    bool flow_set = false, built = false, solved = false;
+    bool flow_set = false, built = false, solved = false, have_gamma = false;
     afx_dense* dense = nullptr;
     double *d_nodes = nullptr, *d_corner = nullptr, *d_wcorner = nullptr, *d_colloc = nullptr,
            *d_normal = nullptr, *d_rhs = nullptr, *d_gamma = nullptr, *d_rforce = nullptr, *d_sforce = nullptr;
@@ -841,7 +935,7 @@ int afx_dense_solve(afx_dense* h, const double* b, double tol, int32_t max_iter,
     if (!h || !b || !x) return api_fail(AFX_EARG, "null argument");
     const int n = h->n;
     VCHK(hipMemcpyAsync(h->b, b, n * sizeof(double), hipMemcpyHostToDevice, h->s));
-    int rc = dense_solve_dev(h, h->b, h->x, tol, max_iter, info);
+    int rc = dense_solve_dev(h, h->b, h->x, tol, max_iter, false, info);
     if (rc) return rc;
     VCHK(hipMemcpyAsync(x, h->x, n * sizeof(double), hipMemcpyDeviceToHost, h->s));
     VCHK(hipStreamSynchronize(h->s));
@@ -985,18 +1079,18 @@ int afx_vlm_factor(afx_vlm* h) {
     return dense_factor(h->dense, &info);
 }
 
-int afx_vlm_solve(afx_vlm* h, double tol, int32_t max_iter, double* gamma, afx_solve_info* info) {
+int afx_vlm_solve(afx_vlm* h, double tol, int32_t max_iter, int32_t warm, double* gamma, afx_solve_info* info) {
     if (!h) return api_fail(AFX_EARG, "null argument");
     if (!h->built) return api_fail(AFX_ESTATE, "matrix not built");
     afx_solve_info inf{};
-    int rc = dense_solve_dev(h->dense, h->d_rhs, h->d_gamma, tol, max_iter, &inf);
+    int rc = dense_solve_dev(h->dense, h->d_rhs, h->d_gamma, tol, max_iter, warm != 0 && h->have_gamma, &inf);
     if (rc) return rc;
     if (info) *info = inf;
     if (gamma) {
         VCHK(hipMemcpyAsync(gamma, h->d_gamma, h->nr * sizeof(double), hipMemcpyDeviceToHost, h->dense->s));
         VCHK(hipStreamSynchronize(h->dense->s));
     }
-    h->solved = true;
+    h->solved = h->have_gamma = true;
     return AFX_OK;
 }
 

@@ -71,7 +71,7 @@ def _load():
         "afx_vlm_get_rings": (C.c_int, [_P, ip, ip]),
         "afx_vlm_build": (C.c_int, [_P]),
         "afx_vlm_factor": (C.c_int, [_P]),
-        "afx_vlm_solve": (C.c_int, [_P, _f64, _i32, _dp, C.POINTER(SolveInfo)]),
+        "afx_vlm_solve": (C.c_int, [_P, _f64, _i32, _i32, _dp, C.POINTER(SolveInfo)]),
         "afx_vlm_forces": (C.c_int, [_P, _dp, _dp, _dp]),
         "afx_vlm_get_aic": (C.c_int, [_P, _dp]),
         "afx_vlm_get_system": (C.c_int, [_P, _dp, _dp, _dp, _dp]),
@@ -164,10 +164,10 @@ class VLM:
     def factor(self):
         _chk(_lib.afx_vlm_factor(self.h))
 
-    def solve(self, tol=0.0, max_iter=30):
+    def solve(self, tol=0.0, max_iter=30, warm=False):
         g = np.zeros(self.n)
         info = SolveInfo()
-        _chk(_lib.afx_vlm_solve(self.h, tol, max_iter, _d(g), C.byref(info)))
+        _chk(_lib.afx_vlm_solve(self.h, tol, max_iter, int(bool(warm)), _d(g), C.byref(info)))
         return g, info
 
     def forces(self):

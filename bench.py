@@ -381,6 +381,101 @@ def adjoint_check(rans, cells=500_000, n_dir=64, h=1e-7):
             "kernels": "5 passes: afx::k_adj_forward, k_adj_face, k_adj_cell1, k_adj_face2, k_adj_cell2"}
 
 
+def aeroelastic_bench(n_chord=64, n_span=256, n_cases=32, alpha_max=8.0, cpu=True):
+    """Vortex-lattice / beam coupling: 64 x 256 rings (N = 16384; AR 10, taper
+    0.4, quarter-chord sweep 25 deg, cosine spacing) on a 256-element beam
+    (EI, GJ linear from root to tip), n_cases load cases alpha 0..alpha_max to
+    a converged tip deflection.  Timings are wall clock around synchronous
+    calls; the factor runs once on the alpha = 0 matrix and preconditions the
+    FP64 refinement of every later matrix (refactored when refinement
+    contracts worse than 0.5).  Parity against the FP64 oracle (exact dense
+    solves) at 16 x 64 rings, one load case at alpha_max."""
+    from aeroflex_amd import vlm as V
+    xyz, rn, so, ref = V.wing_lattice(n_chord, n_span)
+    N = n_chord * n_span
+    h = V.VLM(xyz, rn, so)
+    h.set_flow(0.0, sref=ref["sref"], cref=ref["cref"])
+    h.build()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        h.build()
+    t_build = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    h.factor()
+    t_factor = time.perf_counter() - t0
+    h.set_flow(alpha_max, sref=ref["sref"], cref=ref["cref"])
+    h.build()
+    t0 = time.perf_counter()
+    _, sinfo = h.solve()
+    t_solve = time.perf_counter() - t0
+    beam = V.Beam(*V.wing_beam(xyz, so, n_chord))
+    ae = V.Aeroelastic(h, beam)
+    alphas = np.linspace(0.0, alpha_max, n_cases)
+    cases = []
+    t0 = time.perf_counter()
+    for a in alphas:
+        h.set_flow(float(a), sref=ref["sref"], cref=ref["cref"])
+        info = ae.solve(tol=1e-8, max_iter=60)
+        cases.append(info)
+    t_cases = time.perf_counter() - t0
+    last = cases[-1]
+    out = {
+        "workload": f"VLM {n_chord}x{n_span} rings (N={N}), AR 10, taper 0.4, sweep 25 deg, cosine spacing; "
+                    f"{n_span}-element Euler-Bernoulli beam; {n_cases} load cases alpha 0..{alpha_max} deg",
+        "aic_build_ms": round(t_build * 1e3, 3),
+        "aic_entries_per_s": round(N * N / t_build / 1e9, 2),
+        "lu_factor_ms": round(t_factor * 1e3, 2),
+        "lu_tflops": round(2.0 / 3.0 * N ** 3 / t_factor / 1e12, 2),
+        "lu_dtype": "f32 (rocBLAS sgemm on MFMA for the trailing update) + f64 refinement",
+        "solve_ms": round(t_solve * 1e3, 3),
+        "solve_refinement_steps": sinfo.iterations,
+        "solve_backward_error": sinfo.backward_error,
+        "coupling_total_s": round(t_cases, 3),
+        "coupling_ms_per_case": round(t_cases * 1e3 / n_cases, 2),
+        "coupling_iterations": [c.iterations for c in cases],
+        "refactors": int(sum(c.refactors for c in cases)),
+        "refine_steps_total": int(sum(c.refine_steps for c in cases)),
+        "all_converged": all(bool(c.converged) for c in cases),
+        "tip_deflection_at_alpha_max": last.tip_deflection,
+        "cl_at_alpha_max": last.cl,
+        "semispan": ref["span"] / 2,
+    }
+    if cpu:
+        # CPU leg: the oracle (one core) as baseline and as the checker of the
+        # FP32-LU + FP64-refinement coupling against exact FP64 solves, on a
+        # 16 x 64 lattice (N = 1024), one load case at alpha_max; build and
+        # dense solve times scaled to N = 16384 by N^2 and N^3
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import orc_vlm as ov
+        nc2, ns2 = 16, 64
+        x2, r2, s2, ref2 = V.wing_lattice(nc2, ns2)
+        b2 = V.wing_beam(x2, s2, nc2)
+        h2 = V.VLM(x2, r2, s2)
+        h2.set_flow(alpha_max, sref=ref2["sref"], cref=ref2["cref"])
+        i2 = V.Aeroelastic(h2, V.Beam(*b2)).solve(tol=1e-11, max_iter=80)
+        t0 = time.perf_counter()
+        o2 = ov.aeroelastic(x2, r2, s2, alpha_max, b2, sref=ref2["sref"], cref=ref2["cref"], tol=1e-11,
+                            max_iter=80)
+        t_oracle = time.perf_counter() - t0
+        out["vs_fp64_oracle"] = {"lattice": f"{nc2}x{ns2}", "alpha": alpha_max,
+                                 "tip_rel_err": abs(i2.tip_deflection - o2["tip"]) / abs(o2["tip"]),
+                                 "cl_rel_err": abs(i2.cl - o2["cl"]) / abs(o2["cl"])}
+        A, rhs = ov.build(x2, r2, s2, 2.0, wake_length=100 * ref2["cref"])
+        t0 = time.perf_counter()
+        A, rhs = ov.build(x2, r2, s2, 2.0, wake_length=100 * ref2["cref"])
+        tb = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ov.solve(A, rhs)
+        ts = time.perf_counter() - t0
+        n2 = nc2 * ns2
+        out["cpu_baseline"] = {"kind": "port", "cores": 1, "sample": f"oracle build + dense solve at N={n2}",
+                               "aic_build_ms_at_N": round(tb * (N / n2) ** 2 * 1e3, 1),
+                               "dense_solve_ms_at_N": round(ts * (N / n2) ** 3 * 1e3, 1),
+                               "coupled_case_s_at_sample": round(t_oracle, 3)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -398,12 +493,19 @@ def main():
                     help="pseudo-time steps of the batched-sweep comparison (0: skip it)")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false",
                     help="skip the adjoint timing and finite-difference check")
+    ap.add_argument("--no-aeroelastic", dest="aeroelastic", action="store_false",
+                    help="skip the vortex-lattice / beam coupling section")
+    ap.add_argument("--only-aeroelastic", action="store_true",
+                    help="run only the vortex-lattice / beam coupling section (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the residual's multi-GPU sharding is a design, not a build)", file=sys.stderr)
         sys.exit(2)
 
+    if args.only_aeroelastic:
+        print(json.dumps({"aeroelastic": aeroelastic_bench(cpu=not args.no_cpu)}), flush=True)
+        return
     from aeroflex_amd import rans
     ev = HipEvents()
     if args.copy_probe_mib:
@@ -496,6 +598,8 @@ def main():
         out["adjoint"] = adjoint_check(rans)
     if args.sweep_steps > 0:
         out["batched_sweep"] = batched_sweep(rans, args.sweep_steps)
+    if args.aeroelastic:
+        out["aeroelastic"] = aeroelastic_bench(cpu=not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(rans, args.model)
     else:

@@ -100,8 +100,10 @@ typedef struct afx_solve_info {
 
 /* Gamma = AIC^-1 rhs (solver.cpp:343-361): FP32 LU of the last factored
  * matrix as the preconditioner of FP64 iterative refinement on the current
- * FP64 matrix.  tol <= 0 selects sqrt(N) * eps64.  gamma: N doubles or null. */
-int afx_vlm_solve(afx_vlm* h, double tol, int32_t max_iter, double* gamma, afx_solve_info* info);
+ * FP64 matrix.  tol <= 0 selects sqrt(N) * eps64; warm != 0 starts from the
+ * handle's previous Gamma (zero before the first solve).  gamma: N doubles
+ * or null. */
+int afx_vlm_solve(afx_vlm* h, double tol, int32_t max_iter, int32_t warm, double* gamma, afx_solve_info* info);
 
 /* wingStation::computeForces (surface.cpp:164-201), linear form: Kutta-
  * Joukowski force rho (G_k - G_k-1)(Vinf x dl) of every ring at its leading-
