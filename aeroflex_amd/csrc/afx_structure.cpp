@@ -338,6 +338,7 @@ void afx_default_coupling_opts(afx_coupling_opts* o) {
     o->solve_max_iter = 30;
     o->refactor_above = 0.5;
     o->warm_start = 0;
+    o->aitken = 1;
 }
 
 int afx_aeroelastic_create(afx_vlm* v, afx_beam* b, afx_aeroelastic** out) {
@@ -384,8 +385,8 @@ int afx_aeroelastic_destroy(afx_aeroelastic* h) {
 // on the device, Gamma by FP64 refinement preconditioned with the FP32 LU of
 // an earlier matrix (refactored when refinement contracts worse than
 // refactor_above or does not converge), ring forces -> beam loads, K u = f,
-// under-relaxed update, until the tip deflection changes by less than tol
-// (relative).
+// relaxed update (fixed or Aitken), until the tip deflection changes by less
+// than tol (relative).
 int afx_aeroelastic_solve(afx_aeroelastic* h, const afx_coupling_opts* o, afx_coupling_info* info) {
     if (!h) return api_fail(AFX_EARG, "null argument");
     afx_coupling_opts opts;
@@ -396,8 +397,21 @@ int afx_aeroelastic_solve(afx_aeroelastic* h, const afx_coupling_opts* o, afx_co
     const afx_beam* b = h->b;
     std::vector<double> xyz(3 * (size_t)h->nn), rf(3 * (size_t)h->nr), load(6 * (size_t)b->n), unew(6 * (size_t)b->n);
     double prev_tip = h->disp[6 * h->tip + 2];
+    // inexact fixed point: with solve_tol <= 0 each Gamma solve is only as
+    // accurate as the coupling needs, 1e-5 of the last relative tip change
+    // (the matrix's condition amplifies it into Gamma), clamped to
+    // [sqrt(N) eps64, 1e-9], and convergence is confirmed by one iteration
+    // with the strict tolerance; solve_tol > 0 fixes it
+    const double tol_floor = std::sqrt((double)h->nr) * 1.1102230246251565e-16;
+    double last_change = 1.0;
+    // Aitken dynamic relaxation (Irons-Tuck form, as in partitioned
+    // fluid-structure coupling): omega_k = -omega_k-1 r_k-1.(r_k - r_k-1) / |r_k - r_k-1|^2
+    std::vector<double> r_prev(unew.size(), 0.0);
+    double omega = opts.relax;
     int rc = AFX_OK;
     for (int it = 1; it <= opts.max_iter; ++it) {
+        const double stol =
+            opts.solve_tol > 0.0 ? opts.solve_tol : std::min(1e-9, std::max(tol_floor, 1e-5 * last_change));
         for (int k = 0; k < h->nn; ++k) {
             const V3 p{h->x0[3 * k], h->x0[3 * k + 1], h->x0[3 * k + 2]};
             const V3 u = follow(b, h->disp, h->na[k], h->nb[k], h->nw[k], p);
@@ -412,12 +426,12 @@ int afx_aeroelastic_solve(afx_aeroelastic* h, const afx_coupling_opts* o, afx_co
         }
         afx_solve_info si{};
         const int warm = it > 1 ? 1 : 0;   // the previous iteration's Gamma starts the refinement
-        if ((rc = afx_vlm_solve(h->v, opts.solve_tol, opts.solve_max_iter, warm, nullptr, &si))) return rc;
+        if ((rc = afx_vlm_solve(h->v, stol, opts.solve_max_iter, warm, nullptr, &si))) return rc;
         inf.refine_steps += si.iterations;
         if (!si.converged || si.contraction > opts.refactor_above) {
             if ((rc = afx_vlm_factor(h->v))) return rc;
             ++inf.refactors;
-            if ((rc = afx_vlm_solve(h->v, opts.solve_tol, opts.solve_max_iter, warm, nullptr, &si))) return rc;
+            if ((rc = afx_vlm_solve(h->v, stol, opts.solve_max_iter, warm, nullptr, &si))) return rc;
             inf.refine_steps += si.iterations;
             if (!si.converged) return api_fail(AFX_EARG, "vortex-lattice refinement did not converge after refactoring");
         }
@@ -442,14 +456,32 @@ int afx_aeroelastic_solve(afx_aeroelastic* h, const afx_coupling_opts* o, afx_co
             }
         }
         beam_solve_impl(b, load.data(), unew.data());
-        for (size_t i = 0; i < unew.size(); ++i) h->disp[i] += opts.relax * (unew[i] - h->disp[i]);
+        if (opts.aitken && it > 1) {
+            double num = 0.0, den = 0.0;
+            for (size_t i = 0; i < unew.size(); ++i) {
+                const double r = unew[i] - h->disp[i], dr = r - r_prev[i];
+                num += r_prev[i] * dr;
+                den += dr * dr;
+            }
+            if (den > 0.0) omega = -omega * num / den;
+        }
+        for (size_t i = 0; i < unew.size(); ++i) {
+            const double r = unew[i] - h->disp[i];
+            r_prev[i] = r;
+            h->disp[i] += (opts.aitken ? omega : opts.relax) * r;
+        }
         const double tip = h->disp[6 * h->tip + 2];
         inf.iterations = it;
         inf.tip_deflection = tip;
         if (!std::isfinite(tip)) return api_fail(AFX_EARG, "coupling diverged (non-finite tip deflection)");
+        last_change = std::fabs(tip - prev_tip) / std::max(std::fabs(tip), 1e-300);
         if (std::fabs(tip - prev_tip) <= opts.tol * std::max(std::fabs(tip), 1e-300)) {
-            inf.converged = 1;
-            break;
+            // accepted only after an iteration whose solve met the strict tolerance
+            if (opts.solve_tol > 0.0 || stol <= tol_floor) {
+                inf.converged = 1;
+                break;
+            }
+            last_change = 0.0;
         }
         prev_tip = tip;
     }

@@ -49,42 +49,15 @@ constexpr int MAX_PANEL_WG = 512;   // -> n <= 61440
 constexpr unsigned SPIN_LIMIT = 1u << 21;
 constexpr double FOUR_PI = 12.566370614359172;
 
-// ------------------------------------------------------- grid barrier ----
-struct GridBarrier {
-    unsigned count, gen, abort, pad;
-};
-
-// All workgroups of a cooperative launch meet here.  The wait is bounded:
-// a workgroup that waits longer than SPIN_LIMIT polls raises `abort`, every
-// waiting workgroup leaves on seeing it, and the caller returns, so the grid
-// always drains.  Returns false after an abort.
-__device__ bool grid_sync(GridBarrier* b, unsigned nwg, int* s_flag) {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int ab = 0;
-        const unsigned gen = __hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned prev = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev + 1u == nwg) {
-            __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&b->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            unsigned spins = 0;
-            while (__hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                if (__hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) { ab = 1; break; }
-                if (++spins > SPIN_LIMIT) {
-                    __hip_atomic_store(&b->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    ab = 1;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-        *s_flag = ab;
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return *s_flag == 0;
+// ------------------------------------------------- panel pivot exchange --
+// Candidate key of one workgroup for one column: |a| bits (31) | row (17) |
+// sequence (16).  The sequence is the 1-based column of the whole matrix, so
+// a key left from an earlier column (or an earlier factorisation: the keys
+// are cleared before each one) never matches.  Row 0x1ffff: no candidate.
+constexpr unsigned NO_ROW = 0x1ffffu;
+__device__ inline unsigned long long make_key(float absval, unsigned row, unsigned seq) {
+    const unsigned long long ab = (unsigned long long)(__float_as_uint(absval) & 0x7fffffffu);
+    return (ab << 33) | ((unsigned long long)(row & NO_ROW) << 16) | (unsigned long long)(seq & 0xffffu);
 }
 
 __device__ inline void argmax_combine(float& v, int& i, float ov, int oi) {
@@ -102,23 +75,26 @@ __device__ inline void wave_argmax(float& v, int& i) {
 // --------------------------------------------------- panel factorisation --
 // Unblocked LU with partial pivoting of the m x nbc panel whose top-left
 // entry is A(j0, j0) (column-major, leading dimension lda).  Workgroup w holds
-// panel rows [w R, w R + R) of all nbc columns in LDS.  Per column: every
-// workgroup posts its largest |a| (ties: lowest row) and that row's values,
-// the owner of the diagonal row posts it too, one grid barrier, then every
-// workgroup picks the same pivot, performs the interchange on the rows it
-// owns, scales its part of the column and updates its part of the panel.
-// The candidate slots alternate between two buffers so that posting for
-// column j + 1 cannot overwrite what a slower workgroup still reads for j.
+// panel rows [w R, w R + R) of all nbc columns in LDS.  Per column every
+// workgroup posts its largest |a| (ties: lowest row) as one 64-bit key after
+// the row's values (and the owner of the diagonal row posts that row), then
+// polls the keys of all workgroups: once every key carries this column's
+// sequence, all workgroups know the same pivot without any counter, read the
+// pivot row, perform the interchange on the rows they own, scale their part
+// of the column and update their part of the panel.  Keys and rows alternate
+// between two buffers, so posting for column j + 1 cannot overwrite what a
+// slower workgroup still reads for column j (it cannot post j + 2 before
+// every workgroup has posted j + 1).  Every wait is bounded: a workgroup that
+// polls longer than SPIN_LIMIT raises `abort` and every workgroup returns.
 __global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda, int m, int nbc, int j0,
-                                                  int* __restrict__ ipiv, float* __restrict__ s_abs,
-                                                  int* __restrict__ s_row, float* __restrict__ s_vals,
-                                                  float* __restrict__ s_diag, GridBarrier* bar,
-                                                  int* __restrict__ info) {
+                                                  int* __restrict__ ipiv, unsigned long long* keys,
+                                                  float* __restrict__ s_vals, float* __restrict__ s_diag,
+                                                  unsigned* abort_flag, int* __restrict__ info) {
     __shared__ float slab[LU_NB * PANEL_R];   // panel column c at slab[c * PANEL_R]
     __shared__ float urow[LU_NB];
     __shared__ float rv[4];
     __shared__ int ri[4];
-    __shared__ int s_lrow, s_win, s_flag;
+    __shared__ int s_lrow, s_win;
     const int t = threadIdx.x, w = blockIdx.x, nwg = gridDim.x;
     const int rbase = w * PANEL_R;
     const int rows = min(PANEL_R, m - rbase);
@@ -132,6 +108,7 @@ __global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda
 
     for (int j = 0; j < nbc; ++j) {
         const int par = j & 1;
+        const unsigned seq = (unsigned)(j0 + j + 1);
         float best = -1.f;
         int bi = INT_MAX;
         if (t < rows && rbase + t >= j) {
@@ -144,33 +121,55 @@ __global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda
         __syncthreads();
         if (t == 0) {
             for (int q = 1; q < 4; ++q) argmax_combine(rv[0], ri[0], rv[q], ri[q]);
-            const bool any = ri[0] != INT_MAX;
-            s_abs[par * MAX_PANEL_WG + w] = rv[0];
-            s_row[par * MAX_PANEL_WG + w] = any ? ri[0] : -1;
-            s_lrow = any ? ri[0] - rbase : -1;
+            s_lrow = ri[0] != INT_MAX ? ri[0] - rbase : -1;
         }
         __syncthreads();
-        if (s_lrow >= 0)
+        const int lrow = s_lrow;
+        if (lrow >= 0)
             for (int c = t; c < nbc; c += 256)
-                s_vals[((size_t)par * MAX_PANEL_WG + w) * LU_NB + c] = slab[c * PANEL_R + s_lrow];
+                s_vals[((size_t)par * MAX_PANEL_WG + w) * LU_NB + c] = slab[c * PANEL_R + lrow];
         const int jw = j / PANEL_R;
         if (w == jw)
             for (int c = t; c < nbc; c += 256) s_diag[par * LU_NB + c] = slab[c * PANEL_R + (j - rbase)];
-
-        if (!grid_sync(bar, (unsigned)nwg, &s_flag)) return;
-
-        best = -1.f;
-        bi = INT_MAX;
-        for (int q = t; q < nwg; q += 256) {
-            const int r = s_row[par * MAX_PANEL_WG + q];
-            if (r >= 0) argmax_combine(best, bi, s_abs[par * MAX_PANEL_WG + q], r);
+        __threadfence();
+        __syncthreads();
+        if (t == 0) {
+            const unsigned long long key =
+                lrow >= 0 ? make_key(rv[0], (unsigned)ri[0], seq) : make_key(0.f, NO_ROW, seq);
+            __hip_atomic_store(&keys[par * MAX_PANEL_WG + w], key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+
+        // gather: every key of this column's sequence
+        unsigned spins = 0;
+        for (;;) {
+            int ok = 1;
+            best = -1.f;
+            bi = INT_MAX;
+            for (int q = t; q < nwg; q += 256) {
+                const unsigned long long k =
+                    __hip_atomic_load(&keys[par * MAX_PANEL_WG + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((unsigned)(k & 0xffffu) != (seq & 0xffffu)) {
+                    ok = 0;
+                } else {
+                    const unsigned row = (unsigned)(k >> 16) & NO_ROW;
+                    if (row != NO_ROW) argmax_combine(best, bi, __uint_as_float((unsigned)(k >> 33)), (int)row);
+                }
+            }
+            if (__syncthreads_and(ok)) break;
+            const int ab = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+            if (__syncthreads_or(ab || ++spins > SPIN_LIMIT)) {
+                if (t == 0) __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         wave_argmax(best, bi);
         if ((t & 63) == 0) { rv[t >> 6] = best; ri[t >> 6] = bi; }
         __syncthreads();
         if (t == 0) {
             for (int q = 1; q < 4; ++q) argmax_combine(rv[0], ri[0], rv[q], ri[q]);
-            s_win = ri[0] == INT_MAX ? j : ri[0];   // every candidate row is valid; keep j otherwise
+            s_win = ri[0] == INT_MAX ? j : ri[0];   // every posted row is valid; keep j otherwise
         }
         __syncthreads();
         const int p = s_win;                 // panel-relative pivot row, >= j
@@ -649,12 +648,10 @@ __global__ void k_forces(const double* __restrict__ corner, const double* __rest
     for (int d = 0; d < 3; ++d) sforce[3 * s + d] = F[d];
 }
 
-__global__ void k_zero_barrier(GridBarrier* b, int* info) {
+__global__ void k_panel_reset(unsigned long long* keys, unsigned* abort_flag, int* info) {
+    for (int q = threadIdx.x; q < 2 * MAX_PANEL_WG; q += blockDim.x) keys[q] = 0ull;
     if (threadIdx.x == 0) {
-        b->count = 0u;
-        b->gen = 0u;
-        b->abort = 0u;
-        b->pad = 0u;
+        *abort_flag = 0u;
         *info = 0;
     }
 }
@@ -677,11 +674,10 @@ struct afx_dense {
     int* perm = nullptr;
     double *b = nullptr, *x = nullptr, *r = nullptr, *rowabs = nullptr, *norms = nullptr;
     float* y = nullptr;
-    float* s_abs = nullptr;
-    int* s_row = nullptr;
+    unsigned long long* keys = nullptr;
     float* s_vals = nullptr;
     float* s_diag = nullptr;
-    GridBarrier* bar = nullptr;
+    unsigned* pabort = nullptr;
     int* info = nullptr;
     float *Linv = nullptr, *Uinv = nullptr;   // inverses of the 128 x 128 diagonal blocks
     unsigned *tk = nullptr, *flags = nullptr, *abort = nullptr;   // triangular-solve tickets and block flags
@@ -691,8 +687,8 @@ struct afx_dense {
     bool factored = false;
     ~afx_dense() {
         for (void* p : {(void*)A64, (void*)A32, (void*)ipiv, (void*)perm, (void*)b, (void*)x, (void*)r,
-                        (void*)rowabs, (void*)norms, (void*)y, (void*)s_abs, (void*)s_row, (void*)s_vals,
-                        (void*)s_diag, (void*)bar, (void*)info, (void*)Linv, (void*)Uinv, (void*)tk,
+                        (void*)rowabs, (void*)norms, (void*)y, (void*)keys, (void*)s_vals,
+                        (void*)s_diag, (void*)pabort, (void*)info, (void*)Linv, (void*)Uinv, (void*)tk,
                         (void*)flags, (void*)abort})
             if (p) (void)hipFree(p);
         if (rb) rocblas_destroy_handle(rb);
@@ -727,11 +723,10 @@ int dense_init(afx_dense* h, int n) {
     VCHK(dalloc(&h->rowabs, nn));
     VCHK(dalloc(&h->norms, 4));
     VCHK(dalloc(&h->y, nn));
-    VCHK(dalloc(&h->s_abs, 2 * MAX_PANEL_WG));
-    VCHK(dalloc(&h->s_row, 2 * MAX_PANEL_WG));
+    VCHK(dalloc(&h->keys, 2 * MAX_PANEL_WG));
     VCHK(dalloc(&h->s_vals, (size_t)2 * MAX_PANEL_WG * LU_NB));
     VCHK(dalloc(&h->s_diag, 2 * LU_NB));
-    VCHK(dalloc(&h->bar, 1));
+    VCHK(dalloc(&h->pabort, 1));
     VCHK(dalloc(&h->info, 1));
     h->nblk = (n + LU_NB - 1) / LU_NB;
     VCHK(dalloc(&h->Linv, (size_t)h->nblk * LU_NB * LU_NB));
@@ -753,7 +748,7 @@ int dense_factor(afx_dense* h, int32_t* info_out) {
     dim3 tg((n + 63) / 64, (n + 63) / 64);
     k_to_cm32<<<tg, 256, 0, s>>>(h->A64, n, h->A32, lda);
     VCHK(hipGetLastError());
-    k_zero_barrier<<<1, 64, 0, s>>>(h->bar, h->info);
+    k_panel_reset<<<1, 256, 0, s>>>(h->keys, h->pabort, h->info);
     VCHK(hipGetLastError());
     for (int j0 = 0; j0 < n; j0 += LU_NB) {
         const int nbc = std::min(LU_NB, n - j0);
@@ -761,8 +756,8 @@ int dense_factor(afx_dense* h, int32_t* info_out) {
         const int nwg = (m + PANEL_R - 1) / PANEL_R;
         float* A = h->A32;
         int ldai = lda, mi = m, nbci = nbc, j0i = j0;
-        void* args[] = {&A, &ldai, &mi, &nbci, &j0i, &h->ipiv, &h->s_abs, &h->s_row, &h->s_vals, &h->s_diag,
-                        &h->bar, &h->info};
+        void* args[] = {&A, &ldai, &mi, &nbci, &j0i, &h->ipiv, &h->keys, &h->s_vals, &h->s_diag, &h->pabort,
+                        &h->info};
         VCHK(hipLaunchCooperativeKernel((const void*)k_panel_lu, dim3(nwg), dim3(256), args, 0, s));
         const int others = n - nbc;
         if (others > 0) {
@@ -782,14 +777,14 @@ int dense_factor(afx_dense* h, int32_t* info_out) {
                                 lda, A12, lda, &beta, A22, lda));
         }
     }
-    GridBarrier hb{};
+    unsigned aborted = 0;
     int info = 0;
-    VCHK(hipMemcpyAsync(&hb, h->bar, sizeof(hb), hipMemcpyDeviceToHost, s));
+    VCHK(hipMemcpyAsync(&aborted, h->pabort, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     VCHK(hipMemcpyAsync(&info, h->info, sizeof(int), hipMemcpyDeviceToHost, s));
     std::vector<int> ipiv(n);
     VCHK(hipMemcpyAsync(ipiv.data(), h->ipiv, n * sizeof(int), hipMemcpyDeviceToHost, s));
     VCHK(hipStreamSynchronize(s));
-    if (hb.abort) return api_fail(AFX_EHIP, "LU panel: grid barrier timed out (workgroups not co-resident)");
+    if (aborted) return api_fail(AFX_EHIP, "LU panel: pivot exchange timed out (workgroups not co-resident)");
     // P b = b[perm]: the interchanges applied in order to the identity
     std::vector<int> perm(n);
     for (int i = 0; i < n; ++i) perm[i] = i;

@@ -45,7 +45,7 @@ class SolveInfo(C.Structure):
 
 class CouplingOpts(C.Structure):
     _fields_ = [("tol", _f64), ("max_iter", _i32), ("relax", _f64), ("solve_tol", _f64),
-                ("solve_max_iter", _i32), ("refactor_above", _f64), ("warm_start", _i32)]
+                ("solve_max_iter", _i32), ("refactor_above", _f64), ("warm_start", _i32), ("aitken", _i32)]
 
 
 class CouplingInfo(C.Structure):

@@ -388,8 +388,10 @@ def aeroelastic_bench(n_chord=64, n_span=256, n_cases=32, alpha_max=8.0, cpu=Tru
     a converged tip deflection.  Timings are wall clock around synchronous
     calls; the factor runs once on the alpha = 0 matrix and preconditions the
     FP64 refinement of every later matrix (refactored when refinement
-    contracts worse than 0.5).  Parity against the FP64 oracle (exact dense
-    solves) at 16 x 64 rings, one load case at alpha_max."""
+    contracts worse than 0.5).  Each load case starts from the previous one's
+    displacement (an alpha sweep) and relaxes with Aitken's factor.  Parity
+    against the FP64 oracle (exact dense solves) at 16 x 64 rings, one load
+    case at alpha_max, in the CPU leg."""
     from aeroflex_amd import vlm as V
     xyz, rn, so, ref = V.wing_lattice(n_chord, n_span)
     N = n_chord * n_span
@@ -416,7 +418,7 @@ def aeroelastic_bench(n_chord=64, n_span=256, n_cases=32, alpha_max=8.0, cpu=Tru
     t0 = time.perf_counter()
     for a in alphas:
         h.set_flow(float(a), sref=ref["sref"], cref=ref["cref"])
-        info = ae.solve(tol=1e-8, max_iter=60)
+        info = ae.solve(tol=1e-8, max_iter=60, warm_start=1)
         cases.append(info)
     t_cases = time.perf_counter() - t0
     last = cases[-1]

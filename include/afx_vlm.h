@@ -157,11 +157,13 @@ int afx_aeroelastic_destroy(afx_aeroelastic* h);
 typedef struct afx_coupling_opts {
     double tol;              /* on the change of tip deflection, relative */
     int32_t max_iter;
-    double relax;            /* under-relaxation of the beam displacement */
-    double solve_tol;        /* refinement tolerance (<= 0: sqrt(N) eps64) */
+    double relax;            /* relaxation of the beam displacement (first Aitken factor) */
+    double solve_tol;        /* refinement tolerance; <= 0: 1 % of the last relative tip change,
+                                clamped to [sqrt(N) eps64, 1e-6] */
     int32_t solve_max_iter;
     double refactor_above;   /* refactor when refinement contracts worse than this */
     int32_t warm_start;      /* start from the previous solve's displacement */
+    int32_t aitken;          /* Aitken dynamic relaxation, starting from relax (default 1) */
 } afx_coupling_opts;
 
 typedef struct afx_coupling_info {

@@ -154,9 +154,11 @@ def test_stale_factor_still_converges(vlm, ov):
 
 # (20, 2): tip deflection ~4 % of the half span, plain fixed point; (4, 0.4):
 # ~14 %, where the undamped iteration oscillates and relaxation 0.7 is needed
-@pytest.mark.parametrize("ei,relax", [((20.0, 2.0), 1.0), ((4.0, 0.4), 0.7)])
-def test_aeroelastic_matches_oracle(vlm, ov, ei, relax):
-    print(f"[vlm] aeroelastic small ei={ei} relax={relax}", flush=True)
+# (the oracle runs the fixed relaxation; Aitken's must reach the same point)
+@pytest.mark.parametrize("ei,relax,aitken", [((20.0, 2.0), 1.0, 0), ((20.0, 2.0), 1.0, 1), ((4.0, 0.4), 0.7, 0),
+                                             ((4.0, 0.4), 0.7, 1)])
+def test_aeroelastic_matches_oracle(vlm, ov, ei, relax, aitken):
+    print(f"[vlm] aeroelastic small ei={ei} relax={relax} aitken={aitken}", flush=True)
     nc, ns = 4, 16
     xyz, rn, so, ref = vlm.wing_lattice(nc, ns)
     beam = vlm.wing_beam(xyz, so, nc, ei_flap=ei, ei_chord=(10 * ei[0], 10 * ei[1]),
@@ -165,7 +167,7 @@ def test_aeroelastic_matches_oracle(vlm, ov, ei, relax):
     h.set_flow(5.0, sref=ref["sref"], cref=ref["cref"])
     b = vlm.Beam(*beam)
     ae = vlm.Aeroelastic(h, b)
-    info = ae.solve(tol=1e-11, max_iter=80, relax=relax)
+    info = ae.solve(tol=1e-11, max_iter=80, relax=relax, aitken=aitken)
     r = ov.aeroelastic(xyz, rn, so, 5.0, beam, sref=ref["sref"], cref=ref["cref"], tol=1e-11, max_iter=80,
                        relax=relax)
     assert info.converged and info.tip_deflection > 0
