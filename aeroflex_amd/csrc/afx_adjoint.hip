@@ -16,6 +16,16 @@
 //   5 cells  gather of pass 4 and of the neighbours' limiter parts
 // The wall/slip ghost states are copies of their owners that no flux reads;
 // the farfield ghost states are boundary data and get no adjoint.
+//
+// With geometry on, passes 2-4 also accumulate the adjoints of the face and
+// cell metrics the residual reads (normal, length, Green-Gauss weight,
+// reconstruction offsets x_f - x_c, cell area), and three more passes carry
+// them back to the node coordinates through the metric definitions of
+// mesh.h:378-453, 744-787 and solver.h:436-444:
+//   6 faces  face ends and the two centroids (normal, length, midpoint,
+//            weight |x_f - x_0| / |x_0 - x_1|, ghost centroid x_f + |x_f - x_0| n)
+//   7 cells  centroid (mean of the nodes) and area (two triangles) to vertices
+//   8 nodes  gather over each node's face ends and cell vertices (CSR)
 #include "afx_device.hpp"
 #include "afx_adjoint.hpp"
 #include "afx_adjoint_kernels.hpp"
@@ -45,6 +55,36 @@ __device__ __forceinline__ void face_vars_vjp(const GasT<double>& g, int kind, d
         for (int k = 0; k < 4; ++k) s += out[k].d * a[k];
         b[j] += s;
     }
+}
+
+// d face_vars(kind, qL, qbc; n) / d n_dir, dir 0 = x, 1 = y
+__device__ __forceinline__ void face_vars_dn(const GasT<double>& g, int kind, double nx, double ny, const double* qL,
+                                             const double* qbc, int dir, double* out_d) {
+    const GasT<Dual1> gd = dual_gas(g);
+    Dual1 a[4], bc[4], o[4];
+    for (int k = 0; k < 4; ++k) { a[k] = Dual1(qL[k]); bc[k] = Dual1(qbc[k]); }
+    face_vars<Dual1>(gd, kind, Dual1(nx, dir == 0 ? 1.0 : 0.0), Dual1(ny, dir == 1 ? 1.0 : 0.0), a, bc, o);
+    for (int k = 0; k < 4; ++k) out_d[k] = o[k].d;
+}
+
+// bF . d roe(qL, qR(n), n) / d n_dir with qR = face_vars(qL, qbc; n) on boundary faces
+__device__ __forceinline__ double roe_dn(const GasT<double>& g, int kind, double nx, double ny, const double* qL,
+                                         const double* qR, const double* bF, int dir) {
+    const GasT<Dual1> gd = dual_gas(g);
+    const Dual1 nxd(nx, dir == 0 ? 1.0 : 0.0), nyd(ny, dir == 1 ? 1.0 : 0.0);
+    Dual1 a[4], b[4], f[4];
+    for (int k = 0; k < 4; ++k) a[k] = Dual1(qL[k]);
+    if (kind == K_INTERNAL) {
+        for (int k = 0; k < 4; ++k) b[k] = Dual1(qR[k]);
+    } else {
+        Dual1 bc[4];
+        for (int k = 0; k < 4; ++k) bc[k] = Dual1(qR[k]);
+        face_vars<Dual1>(gd, kind, nxd, nyd, a, bc, b);
+    }
+    roe_flux<Dual1>(gd, nxd, nyd, a, b, false, nullptr, nullptr, f);
+    double s = 0.0;
+    for (int k = 0; k < 4; ++k) s += bF[k] * f[k].d;
+    return s;
 }
 
 // pass 1: gradients, limiters and the active branch of every (cell, variable)
@@ -118,7 +158,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_forward(CellsT<double> C, FacesT
 // pass 2: per face, adjoints of the reconstructed states
 __global__ void __launch_bounds__(kBlock) k_adj_face(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                      const double* __restrict__ w, GasT<double> g, int second_order,
-                                                     AdjBufs A) {
+                                                     AdjBufs A, GeoBufs G, int geo) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F.nF) return;
     const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
@@ -152,12 +192,46 @@ __global__ void __launch_bounds__(kBlock) k_adj_face(CellsT<double> C, FacesT<do
         A.bL[k * nF + f] = bL[k];
         A.bR[k * nF + f] = bR[k];
     }
+    if (!geo) return;
+    // metric adjoints of F(qL, qR; n) len: length, normal, reconstruction offsets
+    double fl[4], qRuse[4];
+    if (kind == K_INTERNAL) {
+        for (int k = 0; k < 4; ++k) qRuse[k] = qR[k];
+    } else {
+        face_vars<double>(g, kind, nx, ny, qL, qR, qRuse);
+    }
+    roe_flux<double>(g, nx, ny, qL, qRuse, false, nullptr, nullptr, fl);
+    double lb = 0.0;
+    for (int k = 0; k < 4; ++k) lb += (kind == K_INTERNAL ? w[k * nT + n] - w[k * nT + o] : -w[k * nT + o]) * fl[k];
+    const double nbx = roe_dn(g, kind, nx, ny, qL, qR, bF, 0), nby = roe_dn(g, kind, nx, ny, qL, qR, bF, 1);
+    double d0b[2] = {0.0, 0.0}, d1b[2] = {0.0, 0.0};
+    if (second_order) {
+        for (int k = 0; k < 4; ++k) {
+            const double ph = A.lim[k * nT + o] * bL[k];
+            d0b[0] += ph * A.gx[k * nT + o];
+            d0b[1] += ph * A.gy[k * nT + o];
+        }
+        if (kind == K_INTERNAL)
+            for (int k = 0; k < 4; ++k) {
+                const double ph = A.lim[k * nT + n] * bR[k];
+                d1b[0] += ph * A.gx[k * nT + n];
+                d1b[1] += ph * A.gy[k * nT + n];
+            }
+    }
+    G.fbar[0 * nF + f] = nbx;
+    G.fbar[1 * nF + f] = nby;
+    G.fbar[2 * nF + f] = lb;
+    G.fbar[3 * nF + f] = 0.0;
+    G.fbar[4 * nF + f] = d0b[0];
+    G.fbar[5 * nF + f] = d0b[1];
+    G.fbar[6 * nF + f] = d1b[0];
+    G.fbar[7 * nF + f] = d1b[1];
 }
 
 // pass 3: per cell, state / gradient / limiter adjoints
 __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                       int second_order, double limiter_k, AdjBufs A,
-                                                      double* __restrict__ lam) {
+                                                      double* __restrict__ lam, GeoBufs G, int geo) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C.nC) return;
     const size_t nT = (size_t)C.nT, nC = (size_t)C.nC, nF = (size_t)F.nF;
@@ -180,22 +254,31 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<d
             }
         }
     }
+    double abar = 0.0;
     for (int k = 0; k < 4; ++k) {
         double nbv = 0.0;
         int nbj = -1;
         const int s = second_order ? (int)A.act[k * nC + c] : -1;
         if (s >= 0) {
-            const int f = (int)(C.cface[(size_t)s * nC + c] & 0x7FFFFFFFu);
+            const uint32_t e = C.cface[(size_t)s * nC + c];
+            const int f = (int)(e & 0x7FFFFFFFu);
             const double dx = F.xf[f] - C.xc[c], dy = F.yf[f] - C.yc[c];
             const double qc = q[k * nT + c];
             const double dqg = gx[k] * dx + gy[k] * dy;
             const int j = dqg > 1e-16 ? A.jmax[k * nC + c] : A.jmin[k * nC + c];
             const double D = (j >= 0 ? q[k * nT + j] : qc) - qc;
             const double Ka = limiter_k * sqrt(C.area[c]), K3a = Ka * Ka * Ka;
-            double d_dqg, d_D;
-            (void)limiter_phi_d<double>(dqg, D, K3a, &d_dqg, &d_D);
+            double d_dqg, d_D, d_K;
+            (void)limiter_phi_d<double>(dqg, D, K3a, &d_dqg, &d_D, &d_K);
             lgx[k] += lphi[k] * d_dqg * dx;
             lgy[k] += lphi[k] * d_dqg * dy;
+            if (geo) {
+                // K3a = (K sqrt(A))^3, and the offset of the limiting face
+                abar += lphi[k] * d_K * 1.5 * K3a / C.area[c];
+                const size_t side = (e >> 31) ? 6 : 4;
+                G.fbar[side * nF + f] += lphi[k] * d_dqg * gx[k];
+                G.fbar[(side + 1) * nF + f] += lphi[k] * d_dqg * gy[k];
+            }
             if (j >= 0) {   // D = q_j - q_c (a limiter on q_c itself has D = 0 and no net dependence)
                 lq[k] -= lphi[k] * d_D;
                 if (j < C.nC) { nbv = lphi[k] * d_D; nbj = j; }
@@ -207,11 +290,17 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<d
         A.nbv[k * nC + c] = nbv;
         A.nbj[k * nC + c] = nbj;
     }
+    if (geo) {
+        // gradient = S / A
+        if (second_order)
+            for (int k = 0; k < 4; ++k) abar -= (lgx[k] * gx[k] + lgy[k] * gy[k]) / C.area[c];
+        G.abar[c] = abar;
+    }
 }
 
 // pass 4: transpose of the Green-Gauss face values
 __global__ void __launch_bounds__(kBlock) k_adj_face2(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
-                                                      GasT<double> g, AdjBufs A) {
+                                                      GasT<double> g, AdjBufs A, GeoBufs G, int geo) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F.nF) return;
     const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
@@ -242,6 +331,152 @@ __global__ void __launch_bounds__(kBlock) k_adj_face2(CellsT<double> C, FacesT<d
         A.co[k * nF + f] = co[k];
         A.cn[k * nF + f] = cn[k];
     }
+    if (!geo) return;
+    // face value fv (as in pass 1) and the metric adjoints of fv n len / A
+    double qo[4], qn[4], qv[4], fv[4];
+    for (int k = 0; k < 4; ++k) { qo[k] = q[k * nT + o]; qn[k] = q[k * nT + n]; }
+    face_vars<double>(g, kind, nx, ny, qo, qn, qv);
+    for (int k = 0; k < 4; ++k) fv[k] = qo[k] * (1.0 - gw) + qv[k] * gw;
+    const double Ao = C.area[o], An = two ? C.area[n] : 1.0;
+    double nbx = 0.0, nby = 0.0, lb = 0.0, gb = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        double ax = A.lgx[k * nT + o] / Ao, ay = A.lgy[k * nT + o] / Ao;
+        if (two) { ax -= A.lgx[k * nT + n] / An; ay -= A.lgy[k * nT + n] / An; }
+        nbx += fv[k] * L * ax;
+        nby += fv[k] * L * ay;
+        lb += fv[k] * (ax * nx + ay * ny);
+        gb += lfv[k] * (qv[k] - qo[k]);
+    }
+    if (!two) {
+        double dvx[4], dvy[4];
+        face_vars_dn(g, kind, nx, ny, qo, qn, 0, dvx);
+        face_vars_dn(g, kind, nx, ny, qo, qn, 1, dvy);
+        for (int k = 0; k < 4; ++k) {
+            nbx += lfv[k] * gw * dvx[k];
+            nby += lfv[k] * gw * dvy[k];
+        }
+    }
+    G.fbar[0 * nF + f] += nbx;
+    G.fbar[1 * nF + f] += nby;
+    G.fbar[2 * nF + f] += lb;
+    G.fbar[3 * nF + f] += gb;
+}
+
+// pass 6: face metrics -> face ends and centroids (mesh.h:378-453, 744-787;
+// solver.h:436-444).  n = s (-dy, dx) / L with the sign s fixed by the mesh,
+// L = |b - a|, x_f = (a + b) / 2, gw = |x_f - x_o| / |x_o - x_n| with x_n the
+// ghost centroid x_f + |x_f - x_o| n on boundary faces.
+__global__ void __launch_bounds__(kBlock) k_geo_face(CellsT<double> C, FacesT<double> F, GeoBufs G) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F.nF) return;
+    const size_t nF = (size_t)F.nF;
+    const int a = G.fnode[f], b = G.fnode[nF + f];
+    const int o = F.c0[f], n = F.c1[f];
+    const bool two = F.kind[f] == K_INTERNAL;
+    const double xa = G.nodex[a], ya = G.nodey[a], xb = G.nodex[b], yb = G.nodey[b];
+    const double dx = xb - xa, dy = yb - ya, L = sqrt(dx * dx + dy * dy);
+    const double nx = F.nx[f], ny = F.ny[f];
+    const double xf = 0.5 * (xa + xb), yf = 0.5 * (ya + yb);
+    double nbx = G.fbar[0 * nF + f], nby = G.fbar[1 * nF + f], lb = G.fbar[2 * nF + f], gb = G.fbar[3 * nF + f];
+    const double d0x = G.fbar[4 * nF + f], d0y = G.fbar[5 * nF + f], d1x = G.fbar[6 * nF + f], d1y = G.fbar[7 * nF + f];
+    double xfb = d0x + d1x, yfb = d0y + d1y;
+    double xob = -d0x, yob = -d0y, xnb = -d1x, ynb = -d1y;
+    // Green-Gauss weight
+    const double e1x = xf - C.xc[o], e1y = yf - C.yc[o], r1 = sqrt(e1x * e1x + e1y * e1y);
+    const double e2x = C.xc[o] - C.xc[n], e2y = C.yc[o] - C.yc[n], r2 = sqrt(e2x * e2x + e2y * e2y);
+    if (r1 > 0.0 && r2 > 0.0) {
+        const double u1x = e1x / r1, u1y = e1y / r1, u2x = e2x / r2, u2y = e2y / r2;
+        xfb += gb * u1x / r2;
+        yfb += gb * u1y / r2;
+        xob += gb * (-u1x / r2 - r1 / (r2 * r2) * u2x);
+        yob += gb * (-u1y / r2 - r1 / (r2 * r2) * u2y);
+        xnb += gb * (r1 / (r2 * r2) * u2x);
+        ynb += gb * (r1 / (r2 * r2) * u2y);
+        if (!two) {   // ghost centroid x_f + r1 n
+            const double sg = nx * xnb + ny * ynb;
+            xfb += xnb + sg * u1x;
+            yfb += ynb + sg * u1y;
+            xob -= sg * u1x;
+            yob -= sg * u1y;
+            nbx += r1 * xnb;
+            nby += r1 * ynb;
+            xnb = 0.0;
+            ynb = 0.0;
+        }
+    }
+    // normal and length -> edge vector (dx, dy)
+    const double sgn = (nx * (-dy) + ny * dx) >= 0.0 ? 1.0 : -1.0;
+    const double nn = nx * nbx + ny * nby;
+    const double ubx = sgn * (nbx - nx * nn) / L, uby = sgn * (nby - ny * nn) / L;
+    const double dxb = uby + lb * dx / L, dyb = -ubx + lb * dy / L;
+    G.fend[0 * nF + f] = 0.5 * xfb - dxb;
+    G.fend[1 * nF + f] = 0.5 * yfb - dyb;
+    G.fend[2 * nF + f] = 0.5 * xfb + dxb;
+    G.fend[3 * nF + f] = 0.5 * yfb + dyb;
+    G.fcen[0 * nF + f] = xob;
+    G.fcen[1 * nF + f] = yob;
+    G.fcen[2 * nF + f] = xnb;
+    G.fcen[3 * nF + f] = ynb;
+}
+
+// pass 7: centroid (node mean) and area (|triangle| sums) to cell vertices
+__global__ void __launch_bounds__(kBlock) k_geo_cell(CellsT<double> C, FacesT<double> F, GeoBufs G) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C.nC) return;
+    const size_t nC = (size_t)C.nC, nF = (size_t)F.nF;
+    double xcb = 0.0, ycb = 0.0;
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t e = C.cface[(size_t)s * nC + c];
+        if (e == 0xFFFFFFFFu) break;
+        const size_t f = e & 0x7FFFFFFFu;
+        const size_t side = (e >> 31) ? 2 : 0;
+        xcb += G.fcen[side * nF + f];
+        ycb += G.fcen[(side + 1) * nF + f];
+    }
+    int v[4];
+    for (int s = 0; s < 4; ++s) v[s] = G.cnode[(size_t)s * nC + c];
+    const int nv = v[3] < 0 ? 3 : 4;
+    double bx[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0};
+    for (int s = 0; s < nv; ++s) { bx[s] = xcb / nv; by[s] = ycb / nv; }
+    const double Ab = G.abar[c];
+    auto tri = [&](int i1, int i2, int i3) {
+        const double x1 = G.nodex[v[i1]], y1 = G.nodey[v[i1]], x2 = G.nodex[v[i2]], y2 = G.nodey[v[i2]];
+        const double x3 = G.nodex[v[i3]], y3 = G.nodey[v[i3]];
+        const double S = x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2);
+        const double h = 0.5 * Ab * (S >= 0.0 ? 1.0 : -1.0);
+        bx[i1] += h * (y2 - y3); by[i1] += h * (x3 - x2);
+        bx[i2] += h * (y3 - y1); by[i2] += h * (x1 - x3);
+        bx[i3] += h * (y1 - y2); by[i3] += h * (x2 - x1);
+    };
+    tri(0, 1, 2);
+    if (nv == 4) tri(0, 2, 3);
+    for (int s = 0; s < 4; ++s) {
+        G.cvert[(size_t)(2 * s) * nC + c] = bx[s];
+        G.cvert[(size_t)(2 * s + 1) * nC + c] = by[s];
+    }
+}
+
+// pass 8: per node, the sum over its face ends and cell vertices
+__global__ void __launch_bounds__(kBlock) k_geo_node(CellsT<double> C, FacesT<double> F, GeoBufs G) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= G.nN) return;
+    const size_t nF = (size_t)F.nF, nC = (size_t)C.nC;
+    double x = 0.0, y = 0.0;
+    for (int i = G.n_off[v]; i < G.n_off[v + 1]; ++i) {
+        const int e = G.n_ent[i];
+        if (e >= 0) {
+            const size_t f = (size_t)(e >> 1), end = (size_t)(e & 1);
+            x += G.fend[(2 * end) * nF + f];
+            y += G.fend[(2 * end + 1) * nF + f];
+        } else {
+            const int idx = -(e + 1);
+            const size_t c = (size_t)(idx >> 2), slot = (size_t)(idx & 3);
+            x += G.cvert[(2 * slot) * nC + c];
+            y += G.cvert[(2 * slot + 1) * nC + c];
+        }
+    }
+    G.xbar[2 * (size_t)v] = x;
+    G.xbar[2 * (size_t)v + 1] = y;
 }
 
 // pass 5: gather of the Green-Gauss transpose and the neighbours' limiter parts
@@ -269,14 +504,22 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell2(CellsT<double> C, FacesT<d
 }  // namespace
 
 hipError_t launch_adjoint_state(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* w,
-                                GasT<double> g, int second_order, double limiter_k, AdjBufs A, double* lam) {
+                                GasT<double> g, int second_order, double limiter_k, AdjBufs A, double* lam,
+                                const GeoBufs* Gp) {
     const int nbc = (C.nC + kBlock - 1) / kBlock, nbf = (F.nF + kBlock - 1) / kBlock;
+    const int geo = Gp != nullptr;
+    const GeoBufs G = geo ? *Gp : GeoBufs{};
     hipLaunchKernelGGL(k_adj_forward, dim3(nbc), dim3(kBlock), 0, s, C, F, q, g, second_order, limiter_k, A);
-    hipLaunchKernelGGL(k_adj_face, dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A);
-    hipLaunchKernelGGL(k_adj_cell1, dim3(nbc), dim3(kBlock), 0, s, C, F, q, second_order, limiter_k, A, lam);
+    hipLaunchKernelGGL(k_adj_face, dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A, G, geo);
+    hipLaunchKernelGGL(k_adj_cell1, dim3(nbc), dim3(kBlock), 0, s, C, F, q, second_order, limiter_k, A, lam, G, geo);
     if (second_order) {
-        hipLaunchKernelGGL(k_adj_face2, dim3(nbf), dim3(kBlock), 0, s, C, F, q, g, A);
+        hipLaunchKernelGGL(k_adj_face2, dim3(nbf), dim3(kBlock), 0, s, C, F, q, g, A, G, geo);
         hipLaunchKernelGGL(k_adj_cell2, dim3(nbc), dim3(kBlock), 0, s, C, F, A, lam);
+    }
+    if (geo) {
+        hipLaunchKernelGGL(k_geo_face, dim3(nbf), dim3(kBlock), 0, s, C, F, G);
+        hipLaunchKernelGGL(k_geo_cell, dim3(nbc), dim3(kBlock), 0, s, C, F, G);
+        hipLaunchKernelGGL(k_geo_node, dim3((G.nN + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F, G);
     }
     return hipGetLastError();
 }

@@ -175,11 +175,12 @@ __host__ __device__ inline Dual1 dual_sqrt(Dual1 a) {
 // phi = (D^2 + K3a + 2 dqg D) / (D^2 + 2 dqg^2 + D dqg + K3a) and its
 // partial derivatives with respect to dqg and D
 template <typename T>
-__host__ __device__ inline T limiter_phi_d(T dqg, T D, T K3a, T* d_dqg, T* d_D) {
+__host__ __device__ inline T limiter_phi_d(T dqg, T D, T K3a, T* d_dqg, T* d_D, T* d_K3a = nullptr) {
     const T N = D * D + K3a + T(2) * dqg * D;
     const T M = D * D + T(2) * dqg * dqg + D * dqg + K3a;
     *d_dqg = (T(2) * D * M - N * (T(4) * dqg + D)) / (M * M);
     *d_D = ((T(2) * D + T(2) * dqg) * M - N * (T(2) * D + dqg)) / (M * M);
+    if (d_K3a) *d_K3a = (M - N) / (M * M);
     return N / M;
 }
 

@@ -22,9 +22,28 @@ struct AdjBufs {
     double* cn;
 };
 
+// work arrays of the adjoint with respect to the mesh nodes
+struct GeoBufs {
+    double* fbar;            // [8][nF] adjoints of nx, ny, len, gw, d0x, d0y, d1x, d1y (d = x_f - x_cell)
+    double* abar;            // [nC] adjoint of the cell area
+    const int32_t* fnode;    // [2][nF] end nodes of each face (device face order)
+    const int32_t* cnode;    // [4][nC] nodes of each real cell (device cell order); slot 3 = -1 for triangles
+    const double* nodex;     // [nN] node coordinates
+    const double* nodey;
+    double* fend;            // [4][nF] (x, y) adjoints of face end a, then end b
+    double* fcen;            // [4][nF] (x, y) adjoints of the owner, then the neighbour centroid
+    double* cvert;           // [8][nC] (x, y) adjoints of each vertex slot
+    const int32_t* n_off;    // [nN + 1] incidences per node (CSR)
+    const int32_t* n_ent;    // 2 f + end (face ends) or -(1 + 4 c + slot) (cell vertices)
+    int32_t nN;
+    double* xbar;            // [nN][2] output: (x, y) adjoint per node
+};
+
 // lam[k*nT + i] (real cells) = ((dR/dq)^T w)[i, k] for the inviscid residual
-// with Green-Gauss gradients; w in the same layout (ghost rows ignored)
+// with Green-Gauss gradients; w in the same layout (ghost rows ignored).
+// With G non-null also G->xbar = (dR/dX)^T w over the node coordinates X.
 hipError_t launch_adjoint_state(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* w,
-                                GasT<double> g, int second_order, double limiter_k, AdjBufs A, double* lam);
+                                GasT<double> g, int second_order, double limiter_k, AdjBufs A, double* lam,
+                                const GeoBufs* G = nullptr);
 
 }  // namespace afx

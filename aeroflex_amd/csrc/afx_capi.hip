@@ -125,6 +125,11 @@ struct afx_rans {
     DBuf<double> a_w, a_lam, a_bL, a_bR, a_lgx, a_lgy, a_nbv, a_co, a_cn;
     DBuf<int8_t> a_act;
     DBuf<int32_t> a_jmax, a_jmin, a_nbj;
+    // adjoint with respect to the mesh nodes: metric adjoints, node data in
+    // device face / cell order, node incidence lists (CSR)
+    DBuf<double> g_fbar, g_abar, g_nodex, g_nodey, g_fend, g_fcen, g_cvert, g_xbar;
+    DBuf<int32_t> g_fnode, g_cnode, g_noff, g_nent;
+    int32_t nN = 0;
     DBuf<float4> q32v_b, q32v_c;   // further FP32 stage states of the batched form
     DBuf<float> q32n_b, q32n_c;
 
@@ -1543,6 +1548,55 @@ int adjoint_check(afx_rans* h) {
     }
     return AFX_OK;
 }
+int geo_check(afx_rans* h) {
+    if (h->g_fbar.p) return AFX_OK;
+    const Mesh& m = h->m;
+    const int32_t nC = h->nC, nF = h->nF;
+    const int32_t nN = (int32_t)m.nodesX.size();
+    h->nN = nN;
+    std::vector<int32_t> fnode(2 * (size_t)nF), cnode(4 * (size_t)nC);
+    for (int32_t nf = 0; nf < nF; ++nf) {
+        const int32_t f = h->f_new2old[nf];
+        fnode[nf] = (int32_t)m.edgesNodes[2 * (size_t)f];
+        fnode[nF + nf] = (int32_t)m.edgesNodes[2 * (size_t)f + 1];
+    }
+    for (int32_t ci = 0; ci < nC; ++ci) {
+        const int32_t c = h->c_new2old[ci];
+        for (int s = 0; s < 4; ++s)
+            cnode[(size_t)s * nC + ci] = (s == 3 && m.cellsIsTriangle[c]) ? -1 : (int32_t)m.cellsNodes[4 * (size_t)c + s];
+    }
+    // incidences: face ends first (face order, end a then b), then cell vertices
+    std::vector<int32_t> off(nN + 1, 0);
+    for (int32_t f = 0; f < nF; ++f) { ++off[fnode[f] + 1]; ++off[fnode[nF + f] + 1]; }
+    for (int32_t c = 0; c < nC; ++c)
+        for (int s = 0; s < 4; ++s)
+            if (cnode[(size_t)s * nC + c] >= 0) ++off[cnode[(size_t)s * nC + c] + 1];
+    for (int32_t v = 0; v < nN; ++v) off[v + 1] += off[v];
+    std::vector<int32_t> ent(off[nN]), pos(off.begin(), off.end() - 1);
+    for (int32_t f = 0; f < nF; ++f) {
+        ent[pos[fnode[f]]++] = 2 * f;
+        ent[pos[fnode[nF + f]]++] = 2 * f + 1;
+    }
+    for (int32_t c = 0; c < nC; ++c)
+        for (int s = 0; s < 4; ++s) {
+            const int32_t v = cnode[(size_t)s * nC + c];
+            if (v >= 0) ent[pos[v]++] = -(1 + 4 * c + s);
+        }
+    HIPCHK(h->g_fbar.alloc(8 * (size_t)nF));
+    HIPCHK(h->g_abar.alloc(nC));
+    HIPCHK(h->g_fend.alloc(4 * (size_t)nF));
+    HIPCHK(h->g_fcen.alloc(4 * (size_t)nF));
+    HIPCHK(h->g_cvert.alloc(8 * (size_t)nC));
+    HIPCHK(h->g_xbar.alloc(2 * (size_t)nN));
+    HIPCHK(h->g_nodex.alloc(nN)); HIPCHK(h->g_nodex.upload(m.nodesX, h->stream));
+    HIPCHK(h->g_nodey.alloc(nN)); HIPCHK(h->g_nodey.upload(m.nodesY, h->stream));
+    HIPCHK(h->g_fnode.alloc(fnode.size())); HIPCHK(h->g_fnode.upload(fnode, h->stream));
+    HIPCHK(h->g_cnode.alloc(cnode.size())); HIPCHK(h->g_cnode.upload(cnode, h->stream));
+    HIPCHK(h->g_noff.alloc(off.size())); HIPCHK(h->g_noff.upload(off, h->stream));
+    HIPCHK(h->g_nent.alloc(ent.size())); HIPCHK(h->g_nent.upload(ent, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return AFX_OK;
+}
 int enqueue_adjoint(afx_rans* h) {
     AdjBufs A{h->gx.p, h->gy.p, h->lim.p, h->a_act.p, h->a_jmax.p, h->a_jmin.p, h->a_bL.p, h->a_bR.p,
               h->a_lgx.p, h->a_lgy.p, h->a_nbv.p, h->a_nbj.p, h->a_co.p, h->a_cn.p};
@@ -1564,6 +1618,27 @@ int afx_rans_adjoint(afx_rans* h, const double* w, double* lambda) {
     if ((rc = enqueue_adjoint(h))) return rc;
     h->q32_valid = false;
     return download_aos(h, h->a_lam.p, lambda);
+}
+
+int afx_rans_adjoint_nodes(afx_rans* h, const double* w, double* lambda, double* xbar) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!w || !xbar) return fail(AFX_EARG, "null vector");
+    if ((rc = adjoint_check(h))) return rc;
+    if ((rc = geo_check(h))) return rc;
+    if ((rc = upload_aos(h, w, h->a_w.p))) return rc;
+    HIPCHK(launch_set_walls<double>(h->stream, h->w_owner.p, h->w_ghost.p, h->n_walls, h->q.p, h->nT, NV));
+    AdjBufs A{h->gx.p, h->gy.p, h->lim.p, h->a_act.p, h->a_jmax.p, h->a_jmin.p, h->a_bL.p, h->a_bR.p,
+              h->a_lgx.p, h->a_lgy.p, h->a_nbv.p, h->a_nbj.p, h->a_co.p, h->a_cn.p};
+    GeoBufs G{h->g_fbar.p, h->g_abar.p, h->g_fnode.p, h->g_cnode.p, h->g_nodex.p, h->g_nodey.p, h->g_fend.p,
+              h->g_fcen.p, h->g_cvert.p, h->g_noff.p, h->g_nent.p, h->nN, h->g_xbar.p};
+    HIPCHK(launch_adjoint_state(h->stream, h->cells64(), h->faces64(), h->q.p, h->a_w.p, h->gas64(),
+                                h->opts.second_order, h->opts.limiter_k, A, h->a_lam.p, &G));
+    h->q32_valid = false;
+    if (lambda && (rc = download_aos(h, h->a_lam.p, lambda))) return rc;
+    HIPCHK(hipMemcpyAsync(xbar, h->g_xbar.p, 2 * (size_t)h->nN * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return AFX_OK;
 }
 
 int afx_rans_time_adjoint(afx_rans* h, int32_t reps, double* ms) {

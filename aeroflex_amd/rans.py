@@ -105,6 +105,7 @@ def _load():
         "afx_rans_create_like": (C.c_int, [P, C.POINTER(AfxGas), C.POINTER(P)]),
         "afx_rans_time_residual": (C.c_int, [P, i32, i32, dp]),
         "afx_rans_adjoint": (C.c_int, [P, dp, dp]),
+        "afx_rans_adjoint_nodes": (C.c_int, [P, dp, dp, dp]),
         "afx_rans_time_adjoint": (C.c_int, [P, i32, dp]),
         "afx_batch_create": (C.c_int, [i32, P, f64, C.POINTER(P)]),
         "afx_batch_run": (C.c_int, [P, i32, i32]),
@@ -482,6 +483,18 @@ class Solver:
         lam = np.empty(4 * self.n_cells)
         _check(lib.afx_rans_adjoint(self.h, _dptr(w), _dptr(lam)))
         return lam
+
+    def adjoint_nodes(self, w):
+        """((dR/dq)^T w, (dR/dX)^T w): the state adjoint and the adjoint with
+        respect to the node coordinates, shape (n_nodes, 2)."""
+        w = np.ascontiguousarray(w, np.float64)
+        if w.shape != (4 * self.n_cells,):
+            raise RansError("adjoint seed has the wrong length")
+        lam = np.empty(4 * self.n_cells)
+        nn = self.mesh.counts()["nodes"]
+        xb = np.empty(2 * nn)
+        _check(lib.afx_rans_adjoint_nodes(self.h, _dptr(w), _dptr(lam), _dptr(xb)))
+        return lam, xb.reshape(-1, 2)
 
     def time_adjoint(self, reps=20):
         ms = C.c_double()

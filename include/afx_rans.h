@@ -225,6 +225,12 @@ int afx_rans_create_like(const afx_rans* base, const afx_gas* g, afx_rans** out)
  * AoS layout 4 * n_cells; ghost rows of w are ignored and of lambda zero.
  * time_adjoint: average ms of one adjoint evaluation (HIP events). */
 int afx_rans_adjoint(afx_rans* h, const double* w, double* lambda);
+/* The same evaluation plus the adjoint with respect to the mesh nodes:
+ * xbar[2 i + d] = ((dR/dX)^T w) for coordinate d of node i (reference node
+ * order, mesh.h nodesX / nodesY), through the metrics of mesh.h:378-453 and
+ * 744-787 (normals, lengths, midpoints, centroids, areas, ghost centroids)
+ * and the Green-Gauss weight of solver.h:436-444.  lambda is nullable. */
+int afx_rans_adjoint_nodes(afx_rans* h, const double* w, double* lambda, double* xbar);
 int afx_rans_time_adjoint(afx_rans* h, int32_t reps, double* ms);
 int afx_rans_time_residual(afx_rans* h, int32_t precision, int32_t reps, double* ms);
 /* Batched explicit pseudo-time stepping of many solvers (conditions and/or
