@@ -57,6 +57,22 @@ __device__ __forceinline__ void face_vars_vjp(const GasT<double>& g, int kind, d
     }
 }
 
+// b += (d face_vars(kind, qL, qbc) / d qbc)^T a  (farfield boundary data)
+__device__ __forceinline__ void face_vars_vjp_bc(const GasT<double>& g, int kind, double nx, double ny,
+                                                 const double* qL, const double* qbc, const double* a, double* b) {
+    const GasT<Dual1> gd = dual_gas(g);
+    Dual1 in[4], out[4];
+    for (int k = 0; k < 4; ++k) in[k] = Dual1(qL[k]);
+    for (int j = 0; j < 4; ++j) {
+        Dual1 bc[4];
+        for (int k = 0; k < 4; ++k) bc[k] = Dual1(qbc[k], k == j ? 1.0 : 0.0);
+        face_vars<Dual1>(gd, kind, Dual1(nx), Dual1(ny), in, bc, out);
+        double s = 0.0;
+        for (int k = 0; k < 4; ++k) s += out[k].d * a[k];
+        b[j] += s;
+    }
+}
+
 // d face_vars(kind, qL, qbc; n) / d n_dir, dir 0 = x, 1 = y
 __device__ __forceinline__ void face_vars_dn(const GasT<double>& g, int kind, double nx, double ny, const double* qL,
                                              const double* qbc, int dir, double* out_d) {
@@ -88,9 +104,9 @@ __device__ __forceinline__ double roe_dn(const GasT<double>& g, int kind, double
 }
 
 // pass 1: gradients, limiters and the active branch of every (cell, variable)
-__global__ void __launch_bounds__(kBlock) k_adj_forward(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
-                                                        GasT<double> g, int second_order, double limiter_k,
-                                                        AdjBufs A) {
+__global__ void __launch_bounds__(kBlock) k_adj_forward(CellsT<double> C, FacesT<double> F, const double* __restrict__ qg,
+                                                        const double* __restrict__ q, GasT<double> g, int second_order,
+                                                        double limiter_k, AdjBufs A) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C.nC) return;
     const size_t nT = (size_t)C.nT, nC = (size_t)C.nC;
@@ -102,7 +118,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_forward(CellsT<double> C, FacesT
         const int f = (int)(e & 0x7FFFFFFFu);
         const int o = F.c0[f], n = F.c1[f];
         double qo[4], qn[4], qv[4];
-        for (int k = 0; k < 4; ++k) { qo[k] = q[k * nT + o]; qn[k] = q[k * nT + n]; }
+        for (int k = 0; k < 4; ++k) { qo[k] = qg[k * nT + o]; qn[k] = qg[k * nT + n]; }
         face_vars<double>(g, F.kind[f], F.nx[f], F.ny[f], qo, qn, qv);
         const double gw = F.gw[f], L = F.len[f];
         for (int k = 0; k < 4; ++k) {
@@ -158,7 +174,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_forward(CellsT<double> C, FacesT
 // pass 2: per face, adjoints of the reconstructed states
 __global__ void __launch_bounds__(kBlock) k_adj_face(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                      const double* __restrict__ w, GasT<double> g, int second_order,
-                                                     AdjBufs A, GeoBufs G, int geo) {
+                                                     AdjBufs A, GeoBufs G, int geo, double* __restrict__ ghost) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F.nF) return;
     const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
@@ -187,6 +203,11 @@ __global__ void __launch_bounds__(kBlock) k_adj_face(CellsT<double> C, FacesT<do
         roe_flux_vjp<double>(g.gamma, nx, ny, qL, qb, bF, bL, bB);
         face_vars_vjp(g, kind, nx, ny, qL, qR, bB, bL);
         for (int k = 0; k < 4; ++k) bR[k] = 0.0;
+        if (ghost && kind == K_FARFIELD) {   // the ghost's first adjoint write of this evaluation
+            double gb[4] = {0.0, 0.0, 0.0, 0.0};
+            face_vars_vjp_bc(g, kind, nx, ny, qL, qR, bB, gb);
+            for (int k = 0; k < 4; ++k) ghost[k * nT + n] = gb[k];
+        }
     }
     for (int k = 0; k < 4; ++k) {
         A.bL[k * nF + f] = bL[k];
@@ -231,7 +252,8 @@ __global__ void __launch_bounds__(kBlock) k_adj_face(CellsT<double> C, FacesT<do
 // pass 3: per cell, state / gradient / limiter adjoints
 __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                       int second_order, double limiter_k, AdjBufs A,
-                                                      double* __restrict__ lam, GeoBufs G, int geo) {
+                                                      double* __restrict__ lam, GeoBufs G, int geo,
+                                                      double* __restrict__ ghost) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C.nC) return;
     const size_t nT = (size_t)C.nT, nC = (size_t)C.nC, nF = (size_t)F.nF;
@@ -282,6 +304,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<d
             if (j >= 0) {   // D = q_j - q_c (a limiter on q_c itself has D = 0 and no net dependence)
                 lq[k] -= lphi[k] * d_D;
                 if (j < C.nC) { nbv = lphi[k] * d_D; nbj = j; }
+                else if (ghost) ghost[k * nT + j] += lphi[k] * d_D;   // farfield ghost (its owner is this cell)
             }
         }
         lam[k * nT + c] = lq[k];
@@ -300,7 +323,8 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<d
 
 // pass 4: transpose of the Green-Gauss face values
 __global__ void __launch_bounds__(kBlock) k_adj_face2(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
-                                                      GasT<double> g, AdjBufs A, GeoBufs G, int geo) {
+                                                      GasT<double> g, AdjBufs A, GeoBufs G, int geo,
+                                                      double* __restrict__ ghost) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F.nF) return;
     const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
@@ -326,6 +350,11 @@ __global__ void __launch_bounds__(kBlock) k_adj_face2(CellsT<double> C, FacesT<d
             qn[k] = q[k * nT + n];
         }
         face_vars_vjp(g, kind, nx, ny, qo, qn, a, co);
+        if (ghost && kind == K_FARFIELD) {
+            double gb[4] = {0.0, 0.0, 0.0, 0.0};
+            face_vars_vjp_bc(g, kind, nx, ny, qo, qn, a, gb);
+            for (int k = 0; k < 4; ++k) ghost[k * nT + n] += gb[k];
+        }
     }
     for (int k = 0; k < 4; ++k) {
         A.co[k * nF + f] = co[k];
@@ -360,6 +389,11 @@ __global__ void __launch_bounds__(kBlock) k_adj_face2(CellsT<double> C, FacesT<d
     G.fbar[1 * nF + f] += nby;
     G.fbar[2 * nF + f] += lb;
     G.fbar[3 * nF + f] += gb;
+}
+
+__global__ void k_fill0(double* __restrict__ x, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
 }
 
 // pass 6: face metrics -> face ends and centroids (mesh.h:378-453, 744-787;
@@ -480,12 +514,14 @@ __global__ void __launch_bounds__(kBlock) k_geo_node(CellsT<double> C, FacesT<do
 }
 
 // pass 5: gather of the Green-Gauss transpose and the neighbours' limiter parts
+// lam gets the state part (pass 3 + neighbours' limiter parts), lam_g the
+// Green-Gauss part (the gradient's source state); with lam_g == lam both add
 __global__ void __launch_bounds__(kBlock) k_adj_cell2(CellsT<double> C, FacesT<double> F, AdjBufs A,
-                                                      double* __restrict__ lam) {
+                                                      double* __restrict__ lam, double* lam_g) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C.nC) return;
     const size_t nT = (size_t)C.nT, nC = (size_t)C.nC, nF = (size_t)F.nF;
-    double acc[4];
+    double acc[4], accg[4] = {0.0, 0.0, 0.0, 0.0};
     for (int k = 0; k < 4; ++k) acc[k] = lam[k * nT + c];
     for (int s = 0; s < 4; ++s) {
         const uint32_t e = C.cface[(size_t)s * nC + c];
@@ -494,27 +530,165 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell2(CellsT<double> C, FacesT<d
         const bool nb = (e >> 31) != 0;
         const int other = nb ? F.c0[f] : F.c1[f];
         for (int k = 0; k < 4; ++k) {
-            acc[k] += nb ? A.cn[k * nF + f] : A.co[k * nF + f];
+            accg[k] += nb ? A.cn[k * nF + f] : A.co[k * nF + f];
             if (other < C.nC && A.nbj[k * nC + other] == c) acc[k] += A.nbv[k * nC + other];
         }
     }
-    for (int k = 0; k < 4; ++k) lam[k * nT + c] = acc[k];
+    if (lam_g == lam) {
+        for (int k = 0; k < 4; ++k) lam[k * nT + c] = acc[k] + accg[k];
+    } else {
+        for (int k = 0; k < 4; ++k) {
+            lam[k * nT + c] = acc[k];
+            lam_g[k * nT + c] = accg[k];
+        }
+    }
 }
 
 }  // namespace
 
-hipError_t launch_adjoint_state(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* w,
-                                GasT<double> g, int second_order, double limiter_k, AdjBufs A, double* lam,
-                                const GeoBufs* Gp) {
+namespace {
+
+// spectral radius sqrt(gamma p / rho) + |V| of one state on a face and its
+// gradient with respect to the state (k_dt, solver.h:308-356)
+__device__ inline double eig_and_grad(const GasT<double>& g, double nx, double ny, const double* qc, double* d) {
+    const double gm1 = g.gamma - 1.0;
+    const double r = qc[0], ru = qc[1], rv = qc[2], re = qc[3];
+    const double V = (ru * nx + rv * ny) / r;
+    const double p = gm1 * (re - 0.5 / r * (ru * ru + rv * rv));
+    const double a = sqrt(p * g.gamma / r);
+    const double dp[4] = {gm1 * 0.5 * (ru * ru + rv * rv) / (r * r), -gm1 * ru / r, -gm1 * rv / r, gm1};
+    const double sv = V < 0.0 ? -1.0 : 1.0;
+    const double dV[4] = {-V / r, nx / r, ny / r, 0.0};
+    for (int k = 0; k < 4; ++k) {
+        const double da = g.gamma / (2.0 * a) * (dp[k] / r - (k == 0 ? p / (r * r) : 0.0));
+        d[k] = da + sv * dV[k];
+    }
+    return a + (V < 0.0 ? -V : V);
+}
+
+// ebar_f = len_f (sbar_o + sbar_n), sbar_c = d dt_c / d s_c dtbar_c = -dtbar_c dt_c^2 / (cfl A_c)
+__global__ void __launch_bounds__(kBlock) k_dt_bar_face(CellsT<double> C, FacesT<double> F,
+                                                        const double* __restrict__ dt, const double* __restrict__ dtbar,
+                                                        double cfl, double* __restrict__ ebar) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F.nF) return;
+    const int o = F.c0[f], n = F.c1[f];
+    double e = -dtbar[o] * dt[o] * dt[o] / (cfl * C.area[o]);
+    if (F.kind[f] == K_INTERNAL) e += -dtbar[n] * dt[n] * dt[n] / (cfl * C.area[n]);
+    ebar[f] = e * F.len[f];
+}
+
+// per cell: the faces on which this cell's state gives max(eig_o, eig_n)
+__global__ void __launch_bounds__(kBlock) k_dt_bar_cell(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
+                                                        GasT<double> g, const double* __restrict__ ebar,
+                                                        double* __restrict__ qbar) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C.nC) return;
+    const size_t nT = (size_t)C.nT, nC = (size_t)C.nC;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t e = C.cface[(size_t)s * nC + c];
+        if (e == 0xFFFFFFFFu) break;
+        const int f = (int)(e & 0x7FFFFFFFu);
+        const int o = F.c0[f], n = F.c1[f];
+        const double nx = F.nx[f], ny = F.ny[f];
+        double qo[4], d[4];
+        for (int k = 0; k < 4; ++k) qo[k] = q[k * nT + o];
+        const double eo = eig_and_grad(g, nx, ny, qo, d);
+        int active = o;
+        if (F.kind[f] == K_INTERNAL) {
+            double qn[4], dn[4];
+            for (int k = 0; k < 4; ++k) qn[k] = q[k * nT + n];
+            const double en = eig_and_grad(g, nx, ny, qn, dn);
+            if (eo < en) { active = n; for (int k = 0; k < 4; ++k) d[k] = dn[k]; }
+        }
+        if (active != c) continue;
+        for (int k = 0; k < 4; ++k) acc[k] += ebar[f] * d[k];
+    }
+    for (int k = 0; k < 4; ++k) qbar[k * nT + c] += acc[k];
+}
+
+// reverse of qk_s+1 = q + W_s dt coef for the real cells: qbar += g,
+// wseed = g dt coef / A (the seed of the undivided residual), dtbar += g . W coef
+__global__ void __launch_bounds__(kBlock) k_rev_stage(const double* __restrict__ gk, const double* __restrict__ W,
+                                                      const double* __restrict__ dt, const double* __restrict__ area,
+                                                      double coef, int32_t nC, int32_t nT, double* __restrict__ wseed,
+                                                      double* __restrict__ dtbar, double* __restrict__ qbar) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    double sdt = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        const size_t i = (size_t)k * nT + c;
+        qbar[i] += gk[i];
+        wseed[i] = gk[i] * dt[c] * coef / area[c];
+        sdt += gk[i] * W[i] * coef;
+    }
+    dtbar[c] += sdt;
+}
+
+// y (+)= a x over the real-cell rows of 4 planes
+__global__ void __launch_bounds__(kBlock) k_axpy4(const double* __restrict__ x, double a, int32_t nC, int32_t nT,
+                                                  int overwrite, double* __restrict__ y) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    for (int k = 0; k < 4; ++k) {
+        const size_t i = (size_t)k * nT + c;
+        y[i] = overwrite ? a * x[i] : y[i] + a * x[i];
+    }
+}
+
+// acc[k][j] += ghost[k*nT + cell_j] over the farfield ghost cells
+__global__ void __launch_bounds__(kBlock) k_ghost_acc(const double* __restrict__ ghost, const int32_t* __restrict__ cells,
+                                                      int32_t n, int32_t nT, double* __restrict__ acc) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    for (int k = 0; k < 4; ++k) acc[(size_t)k * n + j] += ghost[(size_t)k * nT + cells[j]];
+}
+
+}  // namespace
+
+hipError_t launch_rev_stage(hipStream_t s, const double* gk, const double* W, const double* dt, const double* area,
+                            double coef, int32_t nC, int32_t nT, double* wseed, double* dtbar, double* qbar) {
+    hipLaunchKernelGGL(k_rev_stage, dim3((nC + kBlock - 1) / kBlock), dim3(kBlock), 0, s, gk, W, dt, area, coef, nC,
+                       nT, wseed, dtbar, qbar);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpy4(hipStream_t s, const double* x, double a, int32_t nC, int32_t nT, int overwrite, double* y) {
+    hipLaunchKernelGGL(k_axpy4, dim3((nC + kBlock - 1) / kBlock), dim3(kBlock), 0, s, x, a, nC, nT, overwrite, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_ghost_acc(hipStream_t s, const double* ghost, const int32_t* cells, int32_t n, int32_t nT,
+                            double* acc) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ghost_acc, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ghost, cells, n, nT, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_dt_vjp(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, GasT<double> g,
+                         const double* dt, const double* dtbar, double cfl, double* ebar, double* qbar) {
+    hipLaunchKernelGGL(k_dt_bar_face, dim3((F.nF + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F, dt, dtbar, cfl,
+                       ebar);
+    hipLaunchKernelGGL(k_dt_bar_cell, dim3((C.nC + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F, q, g, ebar, qbar);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_stage(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* qg, const double* q,
+                                const double* w, GasT<double> g, int second_order, double limiter_k, AdjBufs A,
+                                double* lam, double* lam_g, double* ghost, const GeoBufs* Gp) {
     const int nbc = (C.nC + kBlock - 1) / kBlock, nbf = (F.nF + kBlock - 1) / kBlock;
     const int geo = Gp != nullptr;
     const GeoBufs G = geo ? *Gp : GeoBufs{};
-    hipLaunchKernelGGL(k_adj_forward, dim3(nbc), dim3(kBlock), 0, s, C, F, q, g, second_order, limiter_k, A);
-    hipLaunchKernelGGL(k_adj_face, dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A, G, geo);
-    hipLaunchKernelGGL(k_adj_cell1, dim3(nbc), dim3(kBlock), 0, s, C, F, q, second_order, limiter_k, A, lam, G, geo);
+    hipLaunchKernelGGL(k_adj_forward, dim3(nbc), dim3(kBlock), 0, s, C, F, qg, q, g, second_order, limiter_k, A);
+    hipLaunchKernelGGL(k_adj_face, dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A, G, geo, ghost);
+    hipLaunchKernelGGL(k_adj_cell1, dim3(nbc), dim3(kBlock), 0, s, C, F, q, second_order, limiter_k, A, lam, G, geo,
+                       ghost);
     if (second_order) {
-        hipLaunchKernelGGL(k_adj_face2, dim3(nbf), dim3(kBlock), 0, s, C, F, q, g, A, G, geo);
-        hipLaunchKernelGGL(k_adj_cell2, dim3(nbc), dim3(kBlock), 0, s, C, F, A, lam);
+        hipLaunchKernelGGL(k_adj_face2, dim3(nbf), dim3(kBlock), 0, s, C, F, qg, g, A, G, geo, ghost);
+        hipLaunchKernelGGL(k_adj_cell2, dim3(nbc), dim3(kBlock), 0, s, C, F, A, lam, lam_g);
+    } else if (lam_g != lam) {
+        hipLaunchKernelGGL(k_fill0, dim3((4 * C.nT + kBlock - 1) / kBlock), dim3(kBlock), 0, s, lam_g, 4 * C.nT);
     }
     if (geo) {
         hipLaunchKernelGGL(k_geo_face, dim3(nbf), dim3(kBlock), 0, s, C, F, G);
@@ -522,6 +696,12 @@ hipError_t launch_adjoint_state(hipStream_t s, CellsT<double> C, FacesT<double> 
         hipLaunchKernelGGL(k_geo_node, dim3((G.nN + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F, G);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_adjoint_state(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* w,
+                                GasT<double> g, int second_order, double limiter_k, AdjBufs A, double* lam,
+                                const GeoBufs* Gp) {
+    return launch_adjoint_stage(s, C, F, q, q, w, g, second_order, limiter_k, A, lam, lam, nullptr, Gp);
 }
 
 }  // namespace afx

@@ -46,4 +46,26 @@ hipError_t launch_adjoint_state(hipStream_t s, CellsT<double> C, FacesT<double> 
                                 GasT<double> g, int second_order, double limiter_k, AdjBufs A, double* lam,
                                 const GeoBufs* G = nullptr);
 
+// The explicit stage residual of explicitSolver::solve (solver.h:802-828)
+// takes its gradients from the step's start state qg and everything else
+// from the stage state q (A3).  lam = (dR/dq)^T w, lam_g = (dR/dqg)^T w; with
+// ghost non-null the adjoints of the farfield ghost states (boundary data)
+// are written into its ghost rows [k*nT + ghost cell].
+hipError_t launch_adjoint_stage(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* qg, const double* q,
+                                const double* w, GasT<double> g, int second_order, double limiter_k, AdjBufs A,
+                                double* lam, double* lam_g, double* ghost, const GeoBufs* G = nullptr);
+
+// reverse of one explicit stage update qk_s+1 = q + W_s dt coef (real cells)
+hipError_t launch_rev_stage(hipStream_t s, const double* gk, const double* W, const double* dt, const double* area,
+                            double coef, int32_t nC, int32_t nT, double* wseed, double* dtbar, double* qbar);
+// y = a x (overwrite) or y += a x, over the real-cell rows of the 4 planes
+hipError_t launch_axpy4(hipStream_t s, const double* x, double a, int32_t nC, int32_t nT, int overwrite, double* y);
+// acc[k][j] += ghost[k*nT + cells[j]]
+hipError_t launch_ghost_acc(hipStream_t s, const double* ghost, const int32_t* cells, int32_t n, int32_t nT,
+                            double* acc);
+
+// dt = cfl A / sum_f max(eig_o, eig_n) len (solver.h:308-356): qbar += (d dt / d q)^T dtbar
+hipError_t launch_dt_vjp(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, GasT<double> g,
+                         const double* dt, const double* dtbar, double cfl, double* ebar, double* qbar);
+
 }  // namespace afx

@@ -1227,6 +1227,43 @@ int afx_rans_explicit_residual(afx_rans* h, const double* qk, double* qW) {
     return AFX_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// explicitSolver::solve (solver.h:802-828) for four variables, from qin to
+// qout (they may alias): dt from qin, stage states qk, gradients from qin
+// (A3), limiters and fluxes from qk, qk = qin + qW dt alpha_s relax.  With
+// keep non-null the stage residuals go to keep[0..2] and the stage states
+// qk_1, qk_2 to keep[3], keep[4] (for the reverse sweep).
+int explicit_step64(afx_rans* h, const double* qin, double* qout, double relax, double* const* keep) {
+    static const double alpha[3] = {0.25, 0.5, 1.};
+    hipStream_t s = h->stream;
+    const int64_t nq = (int64_t)NV * h->nT;
+    int rc;
+    HIPCHK(launch_dt<double>(s, h->cells64(), h->faces64(), qin, h->gas64(), h->cfl, h->dt.p));
+    HIPCHK(launch_copy(s, qin, h->qk.p, nq));
+    double* qs = h->qk.p;
+    for (int st = 0; st < 3; ++st) {
+        if (h->needs_gradients() && (rc = run_gradients64(h, qin, qs))) return rc;
+        double* W = keep ? keep[st] : h->qW.p;
+        HIPCHK(launch_face_flux<double>(s, h->cells64(), h->faces64(), qs, qin, h->gx.p, h->gy.p, h->lim.p,
+                                        h->gas64(), h->opts.viscosity, h->opts.second_order, nullptr, h->flux.p));
+        HIPCHK(launch_gather<double>(s, h->cells64(), h->flux.p, h->nF, 1, W));
+        double* next = qs;
+        if (keep) {
+            next = st < 2 ? keep[3 + st] : h->qk.p;
+            if (st < 2) HIPCHK(launch_copy(s, qs, next, nq));   // ghost rows of the stage state
+        }
+        HIPCHK(launch_rk_update(s, qin, W, h->dt.p, alpha[st], relax, h->nC, h->nT, NV, next));
+        qs = next;
+    }
+    if (qs != qout) HIPCHK(launch_copy(s, qs, qout, nq));
+    return AFX_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int afx_rans_explicit_step(afx_rans* h, double relaxation, double* err) {
     int rc = check_handle(h);
     if (rc) return rc;
@@ -1254,22 +1291,10 @@ int afx_rans_explicit_step(afx_rans* h, double relaxation, double* err) {
         HIPCHK(hipStreamSynchronize(s));
         return AFX_OK;
     }
-    static const double alpha[3] = {0.25, 0.5, 1.};
-    hipStream_t s = h->stream;
-    const int64_t nq = (int64_t)NV * h->nT;
-    HIPCHK(launch_dt<double>(s, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->cfl, h->dt.p));
-    HIPCHK(launch_copy(s, h->q.p, h->qk.p, nq));
-    for (double ai : alpha) {
-        if (h->needs_gradients() && (rc = run_gradients64(h, h->q.p, h->qk.p))) return rc;
-        HIPCHK(launch_face_flux<double>(s, h->cells64(), h->faces64(), h->qk.p, h->q.p, h->gx.p, h->gy.p, h->lim.p,
-                                        h->gas64(), h->opts.viscosity, h->opts.second_order, nullptr, h->flux.p));
-        HIPCHK(launch_gather<double>(s, h->cells64(), h->flux.p, h->nF, 1, h->qW.p));
-        HIPCHK(launch_rk_update(s, h->q.p, h->qW.p, h->dt.p, ai, relaxation, h->nC, h->nT, NV, h->qk.p));
-    }
-    HIPCHK(launch_copy(s, h->qk.p, h->q.p, nq));
+    if ((rc = explicit_step64(h, h->q.p, h->q.p, relaxation, nullptr))) return rc;
     h->q32_valid = false;
     if (err) return norm_of(h, h->qW.p, h->nT, NV, err);
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipStreamSynchronize(h->stream));
     return AFX_OK;
 }
 
@@ -1638,6 +1663,154 @@ int afx_rans_adjoint_nodes(afx_rans* h, const double* w, double* lambda, double*
     if (lambda && (rc = download_aos(h, h->a_lam.p, lambda))) return rc;
     HIPCHK(hipMemcpyAsync(xbar, h->g_xbar.p, 2 * (size_t)h->nN * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    return AFX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// get_wall_profile's CL and CD (post.h:301-387) at the device state q and the
+// gradient dCL/dq in the device layout (owner cells of the patch's faces)
+int wall_cl_seed(afx_rans* h, const char* patch, const double* qdev, double* cl, double* cd, std::vector<double>& seed) {
+    const Mesh& m = h->m;
+    std::vector<int32_t> bsel, cells;
+    for (int32_t b = 0; b < h->nB; ++b)
+        if (m.names[m.boundaryNameIdx[b]] == patch) {
+            bsel.push_back(b);
+            cells.push_back(h->c_old2new[m.edgesCells[2 * (size_t)m.boundaryEdges[b]]]);
+        }
+    if (bsel.empty()) return fail(AFX_EARG, std::string("no boundary faces on patch '") + patch + "'");
+    DBuf<int32_t> dc;
+    DBuf<double> st;
+    HIPCHK(dc.alloc(cells.size()));
+    HIPCHK(dc.upload(cells, h->stream));
+    HIPCHK(st.alloc(cells.size() * NV));
+    HIPCHK(launch_gather_states(h->stream, qdev, h->nT, NV, dc.p, (int32_t)cells.size(), st.p));
+    std::vector<double> q(cells.size() * NV);
+    HIPCHK(hipMemcpyAsync(q.data(), st.p, q.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    double v[4];
+    boundary_variables(h, v);
+    const double p_inf = v[3], mach = v[0], aoa = v[1], gm = h->gas.gamma;
+    double xmin = 0., xmax = 0.;
+    for (size_t k = 0; k < bsel.size(); ++k) {
+        const double x = m.edgesCentersX[m.boundaryEdges[bsel[k]]];
+        if (k == 0) xmin = xmax = x;
+        else { xmin = std::min(xmin, x); xmax = std::max(xmax, x); }
+    }
+    const double ext = xmax - xmin, sa = std::sin(aoa), ca = std::cos(aoa);
+    double fx = 0., fy = 0.;
+    seed.assign((size_t)NV * h->nT, 0.0);
+    for (size_t k = 0; k < bsel.size(); ++k) {
+        const uint32_t e = m.boundaryEdges[bsel[k]];
+        const double* qc = &q[k * NV];
+        const double p = (gm - 1) * (qc[3] - 0.5 / qc[0] * (qc[1] * qc[1] + qc[2] * qc[2]));
+        const double cp = 2. / (gm * mach * mach) * (p / p_inf - 1.);
+        const double nx = m.edgesNormalsX[e], ny = m.edgesNormalsY[e], L = m.edgesLengths[e];
+        fx += cp * nx * L / ext;
+        fy += cp * ny * L / ext;
+        const double dcl_dp = (-nx * sa + ny * ca) * L / ext * 2. / (gm * mach * mach * p_inf);
+        const double dp[4] = {(gm - 1) * 0.5 * (qc[1] * qc[1] + qc[2] * qc[2]) / (qc[0] * qc[0]),
+                              -(gm - 1) * qc[1] / qc[0], -(gm - 1) * qc[2] / qc[0], gm - 1};
+        for (int j = 0; j < 4; ++j) seed[(size_t)j * h->nT + cells[k]] += dcl_dp * dp[j];
+    }
+    *cl = -fx * sa + fy * ca;
+    *cd = fx * ca + fy * sa;
+    return AFX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int afx_rans_dcl_dalpha(afx_rans* h, const char* patch, int32_t steps, int32_t every, double relaxation, double* cl,
+                        double* dcl) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!patch || !cl || !dcl) return fail(AFX_EARG, "null argument");
+    if (steps < 1 || every < 1) return fail(AFX_EARG, "steps and the checkpoint interval must be >= 1");
+    if (h->sa()) return fail(AFX_EUNSUP, "dCL/dalpha covers the four-variable inviscid residual");
+    if ((rc = adjoint_check(h))) return rc;
+    hipStream_t s = h->stream;
+    const int32_t nC = h->nC, nT = h->nT;
+    const size_t nq = (size_t)NV * nT;
+    const int nck = (steps + every - 1) / every;
+    // farfield ghosts (their state is the alpha-dependent boundary data)
+    const Mesh& m = h->m;
+    std::vector<int32_t> far;
+    std::vector<double> dfar;   // d(rho u, rho v) / d angle per farfield ghost
+    for (int32_t b = 0; b < h->nB; ++b)
+        if (h->bface_type[b] == AFX_BC_FARFIELD) {
+            far.push_back(h->c_old2new[m.edgesCells[2 * (size_t)m.boundaryEdges[b] + 1]]);
+            const double* v = &h->bface_vars[4 * (size_t)b];   // mach, angle, T, p (core.h:61-84)
+            const double c = std::sqrt(h->gas.gamma * h->gas.R * v[2]), rho = v[3] / (h->gas.R * v[2]);
+            dfar.push_back(-rho * v[0] * c * std::sin(v[1]));
+            dfar.push_back(rho * v[0] * c * std::cos(v[1]));
+        }
+    const int32_t nfar = (int32_t)far.size();
+    DBuf<double> ck, seg, keepb, g, qb, lam, lamg, wseed, dtb, eb, gh, gacc;
+    DBuf<int32_t> dfarc;
+    HIPCHK(ck.alloc(nck * nq));
+    HIPCHK(seg.alloc((size_t)std::min(every, steps) * nq));
+    HIPCHK(keepb.alloc(5 * nq));
+    for (DBuf<double>* b : {&g, &qb, &lam, &lamg, &wseed, &gh}) HIPCHK(b->alloc(nq));
+    HIPCHK(dtb.alloc(nT));
+    HIPCHK(eb.alloc(h->nF));
+    HIPCHK(gacc.alloc(4 * (size_t)std::max(nfar, 1)));
+    HIPCHK(dfarc.alloc(std::max(nfar, 1)));
+    if (nfar) HIPCHK(dfarc.upload(far, s));
+    HIPCHK(hipMemsetAsync(wseed.p, 0, nq * 8, s));
+    HIPCHK(hipMemsetAsync(gacc.p, 0, 4 * (size_t)std::max(nfar, 1) * 8, s));
+    double* keep[5];
+    for (int k = 0; k < 5; ++k) keep[k] = keepb.p + k * nq;
+
+    // forward: steps explicit steps of the handle's state, checkpoints every `every`
+    for (int32_t n = 0; n < steps; ++n) {
+        if (n % every == 0) HIPCHK(launch_copy(s, h->q.p, ck.p + (size_t)(n / every) * nq, (int64_t)nq));
+        if ((rc = explicit_step64(h, h->q.p, h->q.p, relaxation, nullptr))) return rc;
+    }
+    h->q32_valid = false;
+    std::vector<double> seed;
+    double cd = 0.0;
+    if ((rc = wall_cl_seed(h, patch, h->q.p, cl, &cd, seed))) return rc;
+    HIPCHK(g.upload(seed, s));
+
+    // reverse sweep, one checkpoint segment at a time (states recomputed)
+    static const double alpha[3] = {0.25, 0.5, 1.};
+    AdjBufs A{h->gx.p, h->gy.p, h->lim.p, h->a_act.p, h->a_jmax.p, h->a_jmin.p, h->a_bL.p, h->a_bR.p,
+              h->a_lgx.p, h->a_lgy.p, h->a_nbv.p, h->a_nbj.p, h->a_co.p, h->a_cn.p};
+    for (int k = nck - 1; k >= 0; --k) {
+        const int32_t start = k * every, end = std::min(steps, start + every);
+        HIPCHK(launch_copy(s, ck.p + (size_t)k * nq, seg.p, (int64_t)nq));
+        for (int32_t n = start; n < end - 1; ++n)
+            if ((rc = explicit_step64(h, seg.p + (size_t)(n - start) * nq, seg.p + (size_t)(n - start + 1) * nq,
+                                      relaxation, nullptr)))
+                return rc;
+        for (int32_t n = end - 1; n >= start; --n) {
+            double* qn = seg.p + (size_t)(n - start) * nq;
+            if ((rc = explicit_step64(h, qn, h->qk.p, relaxation, keep))) return rc;   // dt, W_s, qk_1, qk_2
+            HIPCHK(hipMemsetAsync(qb.p, 0, nq * 8, s));
+            HIPCHK(hipMemsetAsync(dtb.p, 0, (size_t)nT * 8, s));
+            for (int st = 2; st >= 0; --st) {
+                HIPCHK(launch_rev_stage(s, g.p, keep[st], h->dt.p, h->area.p, alpha[st] * relaxation, nC, nT,
+                                        wseed.p, dtb.p, qb.p));
+                const double* qs = st == 0 ? qn : keep[2 + st];
+                HIPCHK(launch_adjoint_stage(s, h->cells64(), h->faces64(), qn, qs, wseed.p, h->gas64(),
+                                            h->opts.second_order, h->opts.limiter_k, A, lam.p, lamg.p, gh.p));
+                HIPCHK(launch_ghost_acc(s, gh.p, dfarc.p, nfar, nT, gacc.p));
+                HIPCHK(launch_axpy4(s, lamg.p, 1.0, nC, nT, 0, qb.p));
+                HIPCHK(launch_axpy4(s, lam.p, 1.0, nC, nT, st > 0 ? 1 : 0, st > 0 ? g.p : qb.p));
+            }
+            HIPCHK(launch_dt_vjp(s, h->cells64(), h->faces64(), qn, h->gas64(), h->dt.p, dtb.p, h->cfl, eb.p, qb.p));
+            HIPCHK(launch_copy(s, qb.p, g.p, (int64_t)nq));
+        }
+    }
+    std::vector<double> ga(4 * (size_t)std::max(nfar, 1));
+    HIPCHK(hipMemcpyAsync(ga.data(), gacc.p, ga.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // CL = -fx sin a + fy cos a: explicit part -CD, plus the boundary data
+    double d = -cd;
+    for (int32_t j = 0; j < nfar; ++j) d += ga[(size_t)1 * nfar + j] * dfar[2 * j] + ga[(size_t)2 * nfar + j] * dfar[2 * j + 1];
+    *dcl = d;
     return AFX_OK;
 }
 

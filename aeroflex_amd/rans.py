@@ -106,6 +106,7 @@ def _load():
         "afx_rans_time_residual": (C.c_int, [P, i32, i32, dp]),
         "afx_rans_adjoint": (C.c_int, [P, dp, dp]),
         "afx_rans_adjoint_nodes": (C.c_int, [P, dp, dp, dp]),
+        "afx_rans_dcl_dalpha": (C.c_int, [P, C.c_char_p, i32, i32, f64, dp, dp]),
         "afx_rans_time_adjoint": (C.c_int, [P, i32, dp]),
         "afx_batch_create": (C.c_int, [i32, P, f64, C.POINTER(P)]),
         "afx_batch_run": (C.c_int, [P, i32, i32]),
@@ -495,6 +496,14 @@ class Solver:
         xb = np.empty(2 * nn)
         _check(lib.afx_rans_adjoint_nodes(self.h, _dptr(w), _dptr(lam), _dptr(xb)))
         return lam, xb.reshape(-1, 2)
+
+    def dcl_dalpha(self, steps=300, checkpoint_every=25, relaxation=1.0, patch="wall"):
+        """Run `steps` explicit steps from the current state and return (CL,
+        dCL/dangle) by reverse accumulation with checkpoints."""
+        cl, d = C.c_double(), C.c_double()
+        _check(lib.afx_rans_dcl_dalpha(self.h, patch.encode(), steps, checkpoint_every, relaxation, C.byref(cl),
+                                       C.byref(d)))
+        return cl.value, d.value
 
     def time_adjoint(self, reps=20):
         ms = C.c_double()

@@ -231,6 +231,17 @@ int afx_rans_adjoint(afx_rans* h, const double* w, double* lambda);
  * 744-787 (normals, lengths, midpoints, centroids, areas, ghost centroids)
  * and the Green-Gauss weight of solver.h:436-444.  lambda is nullable. */
 int afx_rans_adjoint_nodes(afx_rans* h, const double* w, double* lambda, double* xbar);
+/* dCL/dalpha through pseudo-time (no counterpart in the reference): runs
+ * `steps` explicit steps (explicitSolver::solve, solver.h:802-828, with the
+ * handle's CFL) from the current state, which becomes the result as after
+ * the same number of afx_rans_explicit_step calls, returns CL of the patch
+ * (get_wall_profile, post.h:301-387) and its derivative with respect to the
+ * farfield angle in radians (boundary_variables::angle, core.h:61-84), the
+ * starting state held fixed.  Reverse accumulation through every stage
+ * (A3 gradients, local time step), states stored every `checkpoint_every`
+ * steps and recomputed per segment.  Inviscid, Green-Gauss, four variables. */
+int afx_rans_dcl_dalpha(afx_rans* h, const char* patch, int32_t steps, int32_t checkpoint_every, double relaxation,
+                        double* cl, double* dcl_dalpha);
 int afx_rans_time_adjoint(afx_rans* h, int32_t reps, double* ms);
 int afx_rans_time_residual(afx_rans* h, int32_t precision, int32_t reps, double* ms);
 /* Batched explicit pseudo-time stepping of many solvers (conditions and/or

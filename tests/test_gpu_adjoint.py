@@ -176,3 +176,39 @@ def test_node_adjoint_single_nodes_on_jittered_mesh():
             fd = float(w @ (rp - rm)) / (2 * h)
             noise = 50 * 2.2e-16 * float(np.abs(w) @ np.abs(rp)) / (2 * h)
             assert abs(xb[node, d] - fd) <= 1e-5 * abs(fd) + noise, (node, d, xb[node, d], fd, noise)
+
+
+# ------------------------------------------------ dCL/dalpha through steps --
+def cl_after_steps(s, q, angle, steps, mach=0.5):
+    from aeroflex_amd import rans
+    far = rans.BoundaryVariables(mach=mach, angle=angle)
+    s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition("slip-wall")})
+    s.set_q(q)
+    s.refill_bcs()
+    for _ in range(steps):
+        s.explicit_solve(1.0)
+    return s.get_wall_profile("wall")[0]
+
+
+@pytest.mark.parametrize("path,second_order", [(QUAD, False), (QUAD, True), (TRI, True)])
+def test_dcl_dalpha_matches_central_differences(path, second_order):
+    """dCL/dangle by reverse accumulation through 60 explicit steps with
+    checkpoints every 7 steps (a ragged last segment) against central
+    differences of CL after the same 60 steps (h = 1e-6 rad); the
+    checkpoint interval does not change the result."""
+    from aeroflex_amd import rans
+    print(f"[adjoint] dcl/dalpha {os.path.basename(path)} second_order={second_order}", flush=True)
+    angle = 2 * 0.01745
+    s, o, q = make_pair(path, wall="slip-wall", second_order=second_order, angle=angle)
+    s.set_cfl(1.0)
+    steps = 60
+    s.set_q(q)
+    cl, d = s.dcl_dalpha(steps=steps, checkpoint_every=7)
+    s.set_q(q)
+    cl1, d1 = s.dcl_dalpha(steps=steps, checkpoint_every=steps)
+    assert cl1 == cl and d1 == d
+    cl0 = cl_after_steps(s, q, angle, steps)
+    assert abs(cl0 - cl) <= 1e-12 * abs(cl)
+    h = 1e-6
+    fd = (cl_after_steps(s, q, angle + h, steps) - cl_after_steps(s, q, angle - h, steps)) / (2 * h)
+    assert abs(d - fd) <= 1e-6 * abs(fd), (d, fd)
