@@ -172,9 +172,10 @@ __global__ void __launch_bounds__(kBlock) k_adj_forward(CellsT<double> C, FacesT
 }
 
 // pass 2: per face, adjoints of the reconstructed states
+template <bool GEO, bool GHOST>
 __global__ void __launch_bounds__(kBlock) k_adj_face(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                      const double* __restrict__ w, GasT<double> g, int second_order,
-                                                     AdjBufs A, GeoBufs G, int geo, double* __restrict__ ghost) {
+                                                     AdjBufs A, GeoBufs G, double* __restrict__ ghost) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F.nF) return;
     const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
@@ -203,7 +204,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_face(CellsT<double> C, FacesT<do
         roe_flux_vjp<double>(g.gamma, nx, ny, qL, qb, bF, bL, bB);
         face_vars_vjp(g, kind, nx, ny, qL, qR, bB, bL);
         for (int k = 0; k < 4; ++k) bR[k] = 0.0;
-        if (ghost && kind == K_FARFIELD) {   // the ghost's first adjoint write of this evaluation
+        if (GHOST && kind == K_FARFIELD) {   // the ghost's first adjoint write of this evaluation
             double gb[4] = {0.0, 0.0, 0.0, 0.0};
             face_vars_vjp_bc(g, kind, nx, ny, qL, qR, bB, gb);
             for (int k = 0; k < 4; ++k) ghost[k * nT + n] = gb[k];
@@ -213,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_face(CellsT<double> C, FacesT<do
         A.bL[k * nF + f] = bL[k];
         A.bR[k * nF + f] = bR[k];
     }
-    if (!geo) return;
+    if (!GEO) return;
     // metric adjoints of F(qL, qR; n) len: length, normal, reconstruction offsets
     double fl[4], qRuse[4];
     if (kind == K_INTERNAL) {
@@ -250,9 +251,10 @@ __global__ void __launch_bounds__(kBlock) k_adj_face(CellsT<double> C, FacesT<do
 }
 
 // pass 3: per cell, state / gradient / limiter adjoints
+template <bool GEO, bool GHOST>
 __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                       int second_order, double limiter_k, AdjBufs A,
-                                                      double* __restrict__ lam, GeoBufs G, int geo,
+                                                      double* __restrict__ lam, GeoBufs G,
                                                       double* __restrict__ ghost) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C.nC) return;
@@ -290,11 +292,11 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<d
             const int j = dqg > 1e-16 ? A.jmax[k * nC + c] : A.jmin[k * nC + c];
             const double D = (j >= 0 ? q[k * nT + j] : qc) - qc;
             const double Ka = limiter_k * sqrt(C.area[c]), K3a = Ka * Ka * Ka;
-            double d_dqg, d_D, d_K;
-            (void)limiter_phi_d<double>(dqg, D, K3a, &d_dqg, &d_D, &d_K);
+            double d_dqg, d_D, d_K = 0.0;
+            (void)limiter_phi_d<double>(dqg, D, K3a, &d_dqg, &d_D, GEO ? &d_K : nullptr);
             lgx[k] += lphi[k] * d_dqg * dx;
             lgy[k] += lphi[k] * d_dqg * dy;
-            if (geo) {
+            if (GEO) {
                 // K3a = (K sqrt(A))^3, and the offset of the limiting face
                 abar += lphi[k] * d_K * 1.5 * K3a / C.area[c];
                 const size_t side = (e >> 31) ? 6 : 4;
@@ -304,7 +306,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<d
             if (j >= 0) {   // D = q_j - q_c (a limiter on q_c itself has D = 0 and no net dependence)
                 lq[k] -= lphi[k] * d_D;
                 if (j < C.nC) { nbv = lphi[k] * d_D; nbj = j; }
-                else if (ghost) ghost[k * nT + j] += lphi[k] * d_D;   // farfield ghost (its owner is this cell)
+                else if (GHOST) ghost[k * nT + j] += lphi[k] * d_D;   // farfield ghost (its owner is this cell)
             }
         }
         lam[k * nT + c] = lq[k];
@@ -313,7 +315,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<d
         A.nbv[k * nC + c] = nbv;
         A.nbj[k * nC + c] = nbj;
     }
-    if (geo) {
+    if (GEO) {
         // gradient = S / A
         if (second_order)
             for (int k = 0; k < 4; ++k) abar -= (lgx[k] * gx[k] + lgy[k] * gy[k]) / C.area[c];
@@ -322,8 +324,9 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<d
 }
 
 // pass 4: transpose of the Green-Gauss face values
+template <bool GEO, bool GHOST>
 __global__ void __launch_bounds__(kBlock) k_adj_face2(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
-                                                      GasT<double> g, AdjBufs A, GeoBufs G, int geo,
+                                                      GasT<double> g, AdjBufs A, GeoBufs G,
                                                       double* __restrict__ ghost) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F.nF) return;
@@ -350,7 +353,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_face2(CellsT<double> C, FacesT<d
             qn[k] = q[k * nT + n];
         }
         face_vars_vjp(g, kind, nx, ny, qo, qn, a, co);
-        if (ghost && kind == K_FARFIELD) {
+        if (GHOST && kind == K_FARFIELD) {
             double gb[4] = {0.0, 0.0, 0.0, 0.0};
             face_vars_vjp_bc(g, kind, nx, ny, qo, qn, a, gb);
             for (int k = 0; k < 4; ++k) ghost[k * nT + n] += gb[k];
@@ -360,7 +363,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_face2(CellsT<double> C, FacesT<d
         A.co[k * nF + f] = co[k];
         A.cn[k * nF + f] = cn[k];
     }
-    if (!geo) return;
+    if (!GEO) return;
     // face value fv (as in pass 1) and the metric adjoints of fv n len / A
     double qo[4], qn[4], qv[4], fv[4];
     for (int k = 0; k < 4; ++k) { qo[k] = q[k * nT + o]; qn[k] = q[k * nT + n]; }
@@ -674,27 +677,39 @@ hipError_t launch_dt_vjp(hipStream_t s, CellsT<double> C, FacesT<double> F, cons
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint_stage(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* qg, const double* q,
-                                const double* w, GasT<double> g, int second_order, double limiter_k, AdjBufs A,
-                                double* lam, double* lam_g, double* ghost, const GeoBufs* Gp) {
+namespace {
+template <bool GEO, bool GHOST>
+void adjoint_passes(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* qg, const double* q,
+                    const double* w, GasT<double> g, int second_order, double limiter_k, AdjBufs A, double* lam,
+                    double* lam_g, double* ghost, const GeoBufs& G) {
     const int nbc = (C.nC + kBlock - 1) / kBlock, nbf = (F.nF + kBlock - 1) / kBlock;
-    const int geo = Gp != nullptr;
-    const GeoBufs G = geo ? *Gp : GeoBufs{};
     hipLaunchKernelGGL(k_adj_forward, dim3(nbc), dim3(kBlock), 0, s, C, F, qg, q, g, second_order, limiter_k, A);
-    hipLaunchKernelGGL(k_adj_face, dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A, G, geo, ghost);
-    hipLaunchKernelGGL(k_adj_cell1, dim3(nbc), dim3(kBlock), 0, s, C, F, q, second_order, limiter_k, A, lam, G, geo,
+    hipLaunchKernelGGL((k_adj_face<GEO, GHOST>), dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A, G,
                        ghost);
+    hipLaunchKernelGGL((k_adj_cell1<GEO, GHOST>), dim3(nbc), dim3(kBlock), 0, s, C, F, q, second_order, limiter_k, A,
+                       lam, G, ghost);
     if (second_order) {
-        hipLaunchKernelGGL(k_adj_face2, dim3(nbf), dim3(kBlock), 0, s, C, F, qg, g, A, G, geo, ghost);
+        hipLaunchKernelGGL((k_adj_face2<GEO, GHOST>), dim3(nbf), dim3(kBlock), 0, s, C, F, qg, g, A, G, ghost);
         hipLaunchKernelGGL(k_adj_cell2, dim3(nbc), dim3(kBlock), 0, s, C, F, A, lam, lam_g);
     } else if (lam_g != lam) {
         hipLaunchKernelGGL(k_fill0, dim3((4 * C.nT + kBlock - 1) / kBlock), dim3(kBlock), 0, s, lam_g, 4 * C.nT);
     }
-    if (geo) {
+    if (GEO) {
         hipLaunchKernelGGL(k_geo_face, dim3(nbf), dim3(kBlock), 0, s, C, F, G);
         hipLaunchKernelGGL(k_geo_cell, dim3(nbc), dim3(kBlock), 0, s, C, F, G);
         hipLaunchKernelGGL(k_geo_node, dim3((G.nN + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F, G);
     }
+}
+}  // namespace
+
+hipError_t launch_adjoint_stage(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* qg, const double* q,
+                                const double* w, GasT<double> g, int second_order, double limiter_k, AdjBufs A,
+                                double* lam, double* lam_g, double* ghost, const GeoBufs* Gp) {
+    const GeoBufs G = Gp ? *Gp : GeoBufs{};
+    if (Gp && ghost) return hipErrorInvalidValue;   // no caller needs both
+    if (Gp) adjoint_passes<true, false>(s, C, F, qg, q, w, g, second_order, limiter_k, A, lam, lam_g, nullptr, G);
+    else if (ghost) adjoint_passes<false, true>(s, C, F, qg, q, w, g, second_order, limiter_k, A, lam, lam_g, ghost, G);
+    else adjoint_passes<false, false>(s, C, F, qg, q, w, g, second_order, limiter_k, A, lam, lam_g, nullptr, G);
     return hipGetLastError();
 }
 

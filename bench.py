@@ -374,11 +374,96 @@ def adjoint_check(rans, cells=500_000, n_dir=64, h=1e-7):
         fd = float(w @ (rp - rm)) / (2 * h * scale)
         ad = float(lam @ v)
         worst = max(worst, abs(ad - fd) / max(abs(fd), 1e-3 * float(np.linalg.norm(lam))))
-    return {"cells": s.n_real, "model": "inviscid, Green-Gauss, second order (active-branch limiter)",
-            "adjoint_ms": round(t_adj, 4), "primal_fp64_ms": round(t_primal, 4),
-            "adjoint_over_primal": round(t_adj / t_primal, 2), "fd_directions": n_dir, "fd_step": h,
-            "fd_max_rel_err": float(f"{worst:.3e}"),
-            "kernels": "5 passes: afx::k_adj_forward, k_adj_face, k_adj_cell1, k_adj_face2, k_adj_cell2"}
+    out = {"cells": s.n_real, "model": "inviscid, Green-Gauss, second order (active-branch limiter)",
+           "adjoint_ms": round(t_adj, 4), "primal_fp64_ms": round(t_primal, 4),
+           "adjoint_over_primal": round(t_adj / t_primal, 2), "fd_directions": n_dir, "fd_step": h,
+           "fd_max_rel_err": float(f"{worst:.3e}"),
+           "kernels": "5 passes: afx::k_adj_forward, k_adj_face, k_adj_cell1, k_adj_face2, k_adj_cell2"}
+    out["node_adjoint"] = node_adjoint_check(rans, m, s, q, w)
+    out["dcl_dalpha"] = dcl_dalpha_check(rans, s, q)
+    return out
+
+
+def node_adjoint_check(rans, m, s, q, w, n_dir=2, h=1e-6):
+    """(dR/dX)^T w over the node coordinates on the same mesh: wall time of one
+    evaluation (with its state adjoint and the host copy of xbar), and central
+    differences of w . R along random node displacements scaled by the local
+    cell size, each on a mesh rebuilt from the moved nodes."""
+    s.set_q(q)
+    s.adjoint_nodes(w)
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        s.set_q(q)
+        _, xb = s.adjoint_nodes(w)
+    t_nodes = (time.perf_counter() - t0) / reps
+    c = m.counts()
+    nr = c["real_cells"]
+    x, y = m.get("nodesX"), m.get("nodesY")
+    cells = m.get("cellsNodes").reshape(-1, 4)[:nr].astype(np.int32)
+    nverts = np.where(m.get("cellsIsTriangle")[:nr] != 0, 3, 4).astype(np.int32)
+    en = m.get("edgesNodes").reshape(-1, 2)
+    bsegs = en[m.get("boundaryEdges")].astype(np.int32)
+    bkeys = m.get("boundaryEdgesIntTag").astype(np.int32)
+    names = dict(zip((int(k) for k in bkeys), m.boundary_names()))
+    far = rans.BoundaryVariables(mach=MACH, angle=np.deg2rad(ALPHA_DEG))
+
+    def residual(dxy):
+        mm = rans.Mesh.from_elements(x + dxy[:, 0], y + dxy[:, 1], cells, nverts, bsegs, bkeys, names)
+        ss = rans.Solver(mm, viscosity_model="inviscid", second_order=True, gradient_scheme="green-gauss",
+                         limiter_k=5.0)
+        ss.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition("slip-wall")})
+        ss.init()
+        ss.set_q(q)
+        return ss.fillRhoRHS(rans.AFX_FP64)[0]
+
+    # each node moves by h times its local length (sqrt of the smallest
+    # incident cell area): a uniform relative perturbation, so the 1e-4 wall
+    # cells stay in the linear range of their limiter branches
+    area = m.get("cellsAreas")[:nr]
+    scale = np.full(len(x), np.inf)
+    for k in range(4):
+        idx = np.nonzero(nverts > k)[0]
+        np.minimum.at(scale, cells[idx, k], np.sqrt(area[idx]))
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for _ in range(n_dir):
+        V = rng.standard_normal(xb.shape) * scale[:, None]
+        fd = float(w @ (residual(h * V) - residual(-h * V))) / (2 * h)
+        ad = float(np.sum(xb * V))
+        worst = max(worst, abs(ad - fd) / max(abs(fd), 1e-3 * float(np.linalg.norm(xb) * np.linalg.norm(V))))
+    return {"nodes": int(c["nodes"]), "wall_ms_per_eval": round(t_nodes * 1e3, 3), "fd_directions": n_dir,
+            "fd_step": h, "fd_max_rel_err": float(f"{worst:.3e}"),
+            "kernels": "metric adjoints in passes 2-4, then afx::k_geo_face, k_geo_cell, k_geo_node"}
+
+
+def dcl_dalpha_check(rans, s, q, steps=300, every=25, h=1e-6):
+    """dCL/d(angle) through `steps` explicit steps (CFL 1) from the perturbed
+    state, checkpoints every `every` steps, against central differences of
+    CL after the same steps at angle +- h."""
+    s.set_cfl(1.0)
+    angle = float(np.deg2rad(ALPHA_DEG))
+    s.set_q(q)
+    t0 = time.perf_counter()
+    cl, d = s.dcl_dalpha(steps=steps, checkpoint_every=every, patch="wall")
+    t_rev = time.perf_counter() - t0
+
+    def cl_at(a):
+        far = rans.BoundaryVariables(mach=MACH, angle=a)
+        s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition("slip-wall")})
+        s.set_q(q)
+        s.refill_bcs()
+        for _ in range(steps):
+            s.explicit_solve(1.0)
+        return s.get_wall_profile("wall")[0]
+
+    t0 = time.perf_counter()
+    cp = cl_at(angle + h)
+    t_fwd = time.perf_counter() - t0
+    fd = (cp - cl_at(angle - h)) / (2 * h)
+    return {"steps": steps, "checkpoint_every": every, "cl": cl, "dcl_dangle": d, "fd_dcl_dangle": fd,
+            "fd_step": h, "rel_err": float(f"{abs(d - fd) / abs(fd):.3e}"),
+            "wall_s_forward_and_reverse": round(t_rev, 3), "wall_s_forward_only": round(t_fwd, 3)}
 
 
 def aeroelastic_bench(n_chord=64, n_span=256, n_cases=32, alpha_max=8.0, cpu=True):
@@ -499,6 +584,8 @@ def main():
                     help="skip the vortex-lattice / beam coupling section")
     ap.add_argument("--only-aeroelastic", action="store_true",
                     help="run only the vortex-lattice / beam coupling section (profiling runs)")
+    ap.add_argument("--only-adjoint", action="store_true",
+                    help="run only the adjoint section (state, node coordinates, dCL/dalpha)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -507,6 +594,10 @@ def main():
 
     if args.only_aeroelastic:
         print(json.dumps({"aeroelastic": aeroelastic_bench(cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.only_adjoint:
+        from aeroflex_amd import rans
+        print(json.dumps({"adjoint": adjoint_check(rans)}), flush=True)
         return
     from aeroflex_amd import rans
     ev = HipEvents()
