@@ -648,6 +648,38 @@ __global__ void k_forces(const double* __restrict__ corner, const double* __rest
     for (int d = 0; d < 3; ++d) sforce[3 * s + d] = F[d];
 }
 
+// nonlinear::steady::buildRHS (solver.cpp:303-316): rhs_i = -V(alpha_s).n_i
+// with the local angle of the ring's station
+__global__ void k_rhs_stations(const double* __restrict__ normal, const int* __restrict__ rst, int nr, double vinf,
+                               const double* __restrict__ alpha_deg, double* __restrict__ rhs) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nr) return;
+    const double a = alpha_deg[rst[k]] * M_PI / 180.0;
+    rhs[k] = -(vinf * cos(a) * normal[3 * k] + vinf * sin(a) * normal[3 * k + 2]);
+}
+
+// Kutta-Joukowski with each station's local stream (wingStation::computeForces
+// with inertial_stream() = V(alpha_s))
+__global__ void k_forces_stations(const double* __restrict__ corner, const double* __restrict__ gamma,
+                                  const int* __restrict__ soff, int ns, double rho, double vinf,
+                                  const double* __restrict__ alpha_deg, double* __restrict__ sforce) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const double a = alpha_deg[s] * M_PI / 180.0;
+    const double vx = vinf * cos(a), vz = vinf * sin(a);
+    double prev = 0.0, F[3] = {0.0, 0.0, 0.0};
+    for (int k = soff[s]; k < soff[s + 1]; ++k) {
+        const double* p = corner + 12 * k;
+        const double dl0 = p[3] - p[0], dl1 = p[4] - p[1], dl2 = p[5] - p[2];
+        const double g = rho * (gamma[k] - prev);
+        F[0] += (0.0 * dl2 - vz * dl1) * g;
+        F[1] += (vz * dl0 - vx * dl2) * g;
+        F[2] += (vx * dl1 - 0.0 * dl0) * g;
+        prev = gamma[k];
+    }
+    for (int d = 0; d < 3; ++d) sforce[3 * s + d] = F[d];
+}
+
 __global__ void k_panel_reset(unsigned long long* keys, unsigned* abort_flag, int* info) {
     for (int q = threadIdx.x; q < 2 * MAX_PANEL_WG; q += blockDim.x) keys[q] = 0ull;
     if (threadIdx.x == 0) {
@@ -877,10 +909,12 @@ struct afx_vlm {
     afx_dense* dense = nullptr;
     double *d_nodes = nullptr, *d_corner = nullptr, *d_wcorner = nullptr, *d_colloc = nullptr,
            *d_normal = nullptr, *d_rhs = nullptr, *d_gamma = nullptr, *d_rforce = nullptr, *d_sforce = nullptr;
-    int *d_rn = nullptr, *d_soff = nullptr;
+    int *d_rn = nullptr, *d_soff = nullptr, *d_rst = nullptr;
+    double* d_salpha = nullptr;
     ~afx_vlm() {
         for (void* p : {(void*)d_nodes, (void*)d_corner, (void*)d_wcorner, (void*)d_colloc, (void*)d_normal,
-                        (void*)d_rhs, (void*)d_gamma, (void*)d_rforce, (void*)d_sforce, (void*)d_rn, (void*)d_soff})
+                        (void*)d_rhs, (void*)d_gamma, (void*)d_rforce, (void*)d_sforce, (void*)d_rn, (void*)d_soff,
+                        (void*)d_rst, (void*)d_salpha})
             if (p) (void)hipFree(p);
         delete dense;
     }
@@ -984,10 +1018,16 @@ int afx_vlm_create(int32_t n_nodes, const double* xyz, int32_t n_stations, const
     VCHK_C(dalloc(&h->d_sforce, 3 * (size_t)n_stations));
     VCHK_C(dalloc(&h->d_rn, 4 * (size_t)nr));
     VCHK_C(dalloc(&h->d_soff, n_stations + 1));
+    VCHK_C(dalloc(&h->d_rst, nr));
+    VCHK_C(dalloc(&h->d_salpha, n_stations));
+    std::vector<int> rst(nr);
+    for (int st = 0; st < n_stations; ++st)
+        for (int k = soff[st]; k < soff[st + 1]; ++k) rst[k] = st;
     hipStream_t s = h->dense->s;
     VCHK_C(hipMemcpyAsync(h->d_nodes, xyz, 3 * (size_t)n_nodes * sizeof(double), hipMemcpyHostToDevice, s));
     VCHK_C(hipMemcpyAsync(h->d_rn, ring_nodes, 4 * (size_t)nr * sizeof(int), hipMemcpyHostToDevice, s));
     VCHK_C(hipMemcpyAsync(h->d_soff, soff, (n_stations + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    VCHK_C(hipMemcpyAsync(h->d_rst, rst.data(), nr * sizeof(int), hipMemcpyHostToDevice, s));
     VCHK_C(hipStreamSynchronize(s));
 #undef VCHK_C
     *out = h;
@@ -1120,6 +1160,160 @@ int afx_vlm_forces(afx_vlm* h, double* ring_force, double* station_force, double
         coeffs[1] = cy;
         coeffs[2] = cd;
     }
+    return AFX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+constexpr double VLM_CLA_DEG = 0.109662271123215082635482531259185634553432464599609375;   // 2 pi per radian
+
+// mlinterp 1D (libs/headeronly/mlinterp): linear inside, end values outside
+double interp1(const double* xd, const double* yd, int n, double x) {
+    if (n == 1 || x <= xd[0]) return yd[0];
+    if (x >= xd[n - 1]) return yd[n - 1];
+    int lo = 0, hi = n - 2;
+    while (lo <= hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (x < xd[mid]) hi = mid - 1;
+        else if (x >= xd[mid + 1]) lo = mid + 1;
+        else {
+            const double w = (xd[mid + 1] - x) / (xd[mid + 1] - xd[mid]);
+            return w * yd[mid] + (1.0 - w) * yd[mid + 1];
+        }
+    }
+    return yd[n - 1];
+}
+
+// database::table::coefficients (database.cpp:192-242): each section's
+// airfoil at alpha, then across the span
+void polar_coeffs(const afx_polar& P, double alpha, double span, double* out3) {
+    std::vector<double> c[3];
+    for (int q = 0; q < P.n_sections; ++q) {
+        const int a = P.section_airfoil[q];
+        const int o = P.offsets[a], n = P.offsets[a + 1] - o;
+        c[0].push_back(interp1(P.alpha_deg + o, P.cl + o, n, alpha));
+        c[1].push_back(interp1(P.alpha_deg + o, P.cd + o, n, alpha));
+        c[2].push_back(interp1(P.alpha_deg + o, P.cmy + o, n, alpha));
+    }
+    for (int k = 0; k < 3; ++k) out3[k] = interp1(P.section_span, c[k].data(), P.n_sections, span);
+}
+}  // namespace
+
+extern "C" {
+
+int afx_vlm_nonlinear_solve(afx_vlm* h, const afx_polar* P, double relaxation, double tol, int32_t max_iter,
+                            afx_nonlinear_info* info, double* station_alpha, double* station_cl) {
+    if (!h || !P) return api_fail(AFX_EARG, "null argument");
+    if (!h->flow_set) return api_fail(AFX_ESTATE, "flow not set");
+    if (P->n_airfoils < 1 || P->n_sections < 1 || !P->offsets || !P->alpha_deg || !P->cl || !P->cd || !P->cmy ||
+        !P->section_airfoil || !P->section_span)
+        return api_fail(AFX_EARG, "polar table is empty");
+    for (int a = 0; a < P->n_airfoils; ++a)
+        if (P->offsets[a + 1] <= P->offsets[a]) return api_fail(AFX_EARG, "an airfoil polar has no rows");
+    for (int q = 0; q < P->n_sections; ++q)
+        if (P->section_airfoil[q] < 0 || P->section_airfoil[q] >= P->n_airfoils)
+            return api_fail(AFX_EARG, "section airfoil index out of range");
+    const afx_vlm_flow& f = h->flow;
+    const int ns = h->ns;
+    int rc = afx_vlm_build(h);
+    if (rc) return rc;
+    if ((rc = afx_vlm_factor(h))) return rc;
+    // station geometry (wingStation / wing, surface.cpp): area = sum of ring
+    // areas (panel.cpp:64-72), leading-edge segment of the first ring, span
+    // location (wing::computeSpan) normalised from the first station
+    std::vector<double> area(ns, 0.0), le(3 * (size_t)ns), spanloc(ns);
+    auto node = [&](int id, int d) { return h->xyz[3 * (size_t)id + d]; };
+    double span = 0.0;
+    for (int st = 0; st < ns; ++st) {
+        for (int k = h->soff[st]; k < h->soff[st + 1]; ++k) {
+            double c[3] = {0, 0, 0};
+            for (int a = 0; a < 4; ++a)
+                for (int d = 0; d < 3; ++d) c[d] += node(h->rn[4 * k + a], d) / 4.0;
+            for (int a = 0; a < 4; ++a) {
+                const int i0 = h->rn[4 * k + a], i1 = h->rn[4 * k + ((a + 1) & 3)];
+                const double dl[3] = {node(i1, 0) - node(i0, 0), node(i1, 1) - node(i0, 1), node(i1, 2) - node(i0, 2)};
+                const double ce[3] = {c[0] - node(i0, 0), c[1] - node(i0, 1), c[2] - node(i0, 2)};
+                const double x = dl[1] * ce[2] - dl[2] * ce[1], y = dl[2] * ce[0] - dl[0] * ce[2],
+                             z = dl[0] * ce[1] - dl[1] * ce[0];
+                area[st] += 0.5 * std::sqrt(x * x + y * y + z * z);
+            }
+        }
+        const int k0 = h->soff[st];
+        for (int d = 0; d < 3; ++d) le[3 * st + d] = node(h->rn[4 * k0 + 1], d) - node(h->rn[4 * k0], d);
+        const double len = std::sqrt(le[3 * st] * le[3 * st] + le[3 * st + 1] * le[3 * st + 1] +
+                                     le[3 * st + 2] * le[3 * st + 2]);
+        span += len;
+        spanloc[st] = span - 0.5 * len;
+    }
+    for (int st = ns - 1; st >= 0; --st) spanloc[st] = (spanloc[st] - spanloc[0]) / span;
+    const double q = 0.5 * f.rho * f.vinf * f.vinf;
+    std::vector<double> alpha(ns, f.alpha_deg), sf(3 * (size_t)ns), clinv(ns);
+    hipStream_t s = h->dense->s;
+    auto lift_axis = [&](int st, double* ax) {   // unit(V(alpha_s) x dl_LE)
+        const double a = alpha[st] * M_PI / 180.0, vx = std::cos(a), vz = std::sin(a);
+        const double* d = &le[3 * st];
+        ax[0] = 0.0 * d[2] - vz * d[1];
+        ax[1] = vz * d[0] - vx * d[2];
+        ax[2] = vx * d[1] - 0.0 * d[0];
+        const double n = std::sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+        for (int k = 0; k < 3; ++k) ax[k] /= n;
+    };
+    afx_nonlinear_info inf{};
+    int iter = 0;
+    double residual = 0.0;
+    do {
+        VCHK(hipMemcpyAsync(h->d_salpha, alpha.data(), ns * sizeof(double), hipMemcpyHostToDevice, s));
+        k_rhs_stations<<<(h->nr + 127) / 128, 128, 0, s>>>(h->d_normal, h->d_rst, h->nr, f.vinf, h->d_salpha, h->d_rhs);
+        VCHK(hipGetLastError());
+        afx_solve_info si{};
+        if ((rc = dense_solve_dev(h->dense, h->d_rhs, h->d_gamma, 0.0, 30, iter > 0, &si))) return rc;
+        if (!si.converged) return api_fail(AFX_EARG, "vortex-lattice refinement did not converge");
+        h->solved = h->have_gamma = true;
+        k_forces_stations<<<(ns + 127) / 128, 128, 0, s>>>(h->d_corner, h->d_gamma, h->d_soff, ns, f.rho, f.vinf,
+                                                           h->d_salpha, h->d_sforce);
+        VCHK(hipGetLastError());
+        VCHK(hipMemcpyAsync(sf.data(), h->d_sforce, sf.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        VCHK(hipStreamSynchronize(s));
+        residual = 0.0;
+        for (int st = 0; st < ns; ++st) {
+            double ax[3];
+            lift_axis(st, ax);
+            clinv[st] = (sf[3 * st] * ax[0] + sf[3 * st + 1] * ax[1] + sf[3 * st + 2] * ax[2]) / (q * area[st]);
+            const double aoa_eff = clinv[st] / VLM_CLA_DEG - alpha[st] + f.alpha_deg;
+            double c3[3];
+            polar_coeffs(*P, aoa_eff, spanloc[st], c3);
+            alpha[st] += relaxation * (c3[0] - clinv[st]) / VLM_CLA_DEG;
+            const double r = (c3[0] - clinv[st]) / c3[0];
+            residual += r * r;
+        }
+        residual = std::sqrt(residual);
+        ++iter;
+    } while (residual > tol && iter <= max_iter);
+    // nonlinear::steady::computeForces (solver.cpp:434-476): viscous section
+    // coefficients at the final effective angles
+    const double A = f.alpha_deg * M_PI / 180.0;
+    const double lx = -std::sin(A), lz = std::cos(A);
+    double cl = 0.0, cd = 0.0, cy = 0.0;
+    for (int st = 0; st < ns; ++st) {
+        const double aoa_eff = clinv[st] / VLM_CLA_DEG - alpha[st] + f.alpha_deg;
+        double c3[3], ax[3];
+        polar_coeffs(*P, aoa_eff, spanloc[st], c3);
+        lift_axis(st, ax);
+        const double scl = c3[0] * (ax[0] * lx + ax[2] * lz);
+        cl += scl * area[st] / f.sref;
+        cy += c3[0] * ax[1] * area[st] / f.sref;
+        cd += c3[1] * area[st] / f.sref;
+        if (station_cl) station_cl[st] = scl;
+    }
+    if (station_alpha) std::copy(alpha.begin(), alpha.end(), station_alpha);
+    inf.iterations = iter;
+    inf.converged = residual <= tol;
+    inf.residual = residual;
+    inf.cl = cl;
+    inf.cd = cd;
+    inf.cy = cy;
+    if (info) *info = inf;
     return AFX_OK;
 }
 

@@ -39,6 +39,16 @@ class Flow(C.Structure):
                 ("wake_length", _f64), ("core_radius", _f64)]
 
 
+class Polar(C.Structure):
+    _fields_ = [("n_airfoils", _i32), ("offsets", C.POINTER(_i32)), ("alpha_deg", _dp), ("cl", _dp), ("cd", _dp),
+                ("cmy", _dp), ("n_sections", _i32), ("section_airfoil", C.POINTER(_i32)), ("section_span", _dp)]
+
+
+class NonlinearInfo(C.Structure):
+    _fields_ = [("iterations", _i32), ("converged", _i32), ("residual", _f64), ("cl", _f64), ("cd", _f64),
+                ("cy", _f64)]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [("iterations", _i32), ("converged", _i32), ("backward_error", _f64), ("contraction", _f64)]
 
@@ -74,6 +84,8 @@ def _load():
         "afx_vlm_solve": (C.c_int, [_P, _f64, _i32, _i32, _dp, C.POINTER(SolveInfo)]),
         "afx_vlm_forces": (C.c_int, [_P, _dp, _dp, _dp]),
         "afx_vlm_get_aic": (C.c_int, [_P, _dp]),
+        "afx_vlm_nonlinear_solve": (C.c_int, [_P, C.POINTER(Polar), _f64, _f64, _i32, C.POINTER(NonlinearInfo), _dp,
+                                              _dp]),
         "afx_vlm_get_system": (C.c_int, [_P, _dp, _dp, _dp, _dp]),
         "afx_dense_create": (C.c_int, [_i32, C.POINTER(_P)]),
         "afx_dense_destroy": (C.c_int, [_P]),
@@ -176,6 +188,16 @@ class VLM:
         c = np.zeros(3)
         _chk(_lib.afx_vlm_forces(self.h, _d(rf), _d(sf), _d(c)))
         return rf, sf, {"cl": c[0], "cy": c[1], "cd_kj": c[2]}
+
+    def nonlinear_solve(self, polar, relaxation=1.0, tol=1e-6, max_iter=100):
+        """nonlinear::steady::solve with a polar table (read_polar).  Returns
+        (info, station alpha [deg], station viscous cl)."""
+        p = _polar_struct(polar)
+        ns = len(self.soff) - 1
+        a, c = np.zeros(ns), np.zeros(ns)
+        info = NonlinearInfo()
+        _chk(_lib.afx_vlm_nonlinear_solve(self.h, C.byref(p), relaxation, tol, max_iter, C.byref(info), _d(a), _d(c)))
+        return info, a, c
 
     def aic(self):
         a = np.zeros((self.n, self.n))
@@ -310,3 +332,78 @@ def wing_beam(xyz, station_offsets, n_chord, elastic_axis=0.4, ea=1e5, ei_flap=(
     props[:, 4] = -1.0
     clamped = np.array([int(np.argmin(np.abs(bx[:, 1])))], np.int32)
     return bx, en, props, clamped
+
+
+def read_polar(database_path, locations_path, surface=0):
+    """database::table::importAirfoils / importLocations (database.cpp): the
+    AIRFOIL blocks (alpha CL CD CMy rows) of one file and the SURFACE block of
+    the locations file; returns the arrays of afx_polar."""
+    foils, order, cur = {}, [], None
+    for line in open(database_path):
+        t = line.split()
+        if not t or t[0].startswith("#"):
+            continue
+        if t[0] == "AIRFOIL":
+            cur = t[1]
+            foils[cur] = []
+            order.append(cur)
+        elif t[0] == "END":
+            cur = None
+        elif cur is not None:
+            foils[cur].append([float(v) for v in t[:4]])
+    sections, spans, inside = [], [], False
+    for line in open(locations_path):
+        t = line.split()
+        if not t or t[0].startswith("#"):
+            continue
+        if t[0] == "SURFACE":
+            inside = int(t[1]) == surface
+        elif t[0] == "END":
+            inside = False
+        elif inside:
+            sections.append(order.index(t[0]))
+            spans.append(float(t[1]))
+    rows = [np.asarray(foils[n]) for n in order]
+    off = np.cumsum([0] + [len(r) for r in rows]).astype(np.int32)
+    tab = np.concatenate(rows)
+    return {"offsets": off, "alpha_deg": tab[:, 0].copy(), "cl": tab[:, 1].copy(), "cd": tab[:, 2].copy(),
+            "cmy": tab[:, 3].copy(), "section_airfoil": np.asarray(sections, np.int32),
+            "section_span": np.asarray(spans, np.float64)}
+
+
+def _polar_struct(p):
+    keep = {k: np.ascontiguousarray(v) for k, v in p.items()}
+    st = Polar(len(keep["offsets"]) - 1, _i(keep["offsets"]), _d(keep["alpha_deg"]), _d(keep["cl"]), _d(keep["cd"]),
+               _d(keep["cmy"]), len(keep["section_airfoil"]), _i(keep["section_airfoil"]), _d(keep["section_span"]))
+    st._keep = keep
+    return st
+
+
+def read_lattice(path):
+    """The reference's vortex-lattice geometry file (NODE / VORTEX /
+    WINGSTATION / WING records, vlm/src/input.cpp): returns (xyz, rings,
+    station_offsets) with the stations in wing order and each station's rings
+    as listed (leading to trailing edge)."""
+    nodes, vort, stations, wings = {}, {}, {}, []
+    for line in open(path):
+        t = line.split()
+        if not t:
+            continue
+        if t[0] == "NODE":
+            nodes[int(t[1])] = [float(v) for v in t[2:5]]
+        elif t[0] == "VORTEX":
+            vort[int(t[1])] = [int(v) for v in t[2:6]]
+        elif t[0] == "WINGSTATION":
+            stations[int(t[1])] = [int(v) for v in t[2:]]
+        elif t[0] == "WING":
+            wings.append([int(v) for v in t[2:]])
+    ids = sorted(nodes)
+    pos = {n: i for i, n in enumerate(ids)}
+    xyz = np.array([nodes[n] for n in ids])
+    rings, soff = [], [0]
+    for w in wings:
+        for st in w:
+            for v in stations[st]:
+                rings.append([pos[n] for n in vort[v]])
+            soff.append(len(rings))
+    return xyz, np.array(rings, np.int32), np.array(soff, np.int32)

@@ -116,6 +116,43 @@ int afx_vlm_forces(afx_vlm* h, double* ring_force, double* station_force, double
 int afx_vlm_get_aic(afx_vlm* h, double* aic);
 int afx_vlm_get_system(afx_vlm* h, double* rhs, double* gamma, double* colloc, double* normal);
 
+/* ------------------------------------------------ nonlinear (viscous) --- */
+
+/* database::table (common/database/database.cpp:192-242): airfoil polars
+ * (alpha in degrees, CL, CD, CMy; linear interpolation clamped at the ends,
+ * mlinterp) placed at normalised span locations of the wing. */
+typedef struct afx_polar {
+    int32_t n_airfoils;
+    const int32_t* offsets;       /* n_airfoils + 1: rows of airfoil a are [offsets[a], offsets[a+1]) */
+    const double* alpha_deg;      /* increasing within each airfoil */
+    const double* cl;
+    const double* cd;
+    const double* cmy;
+    int32_t n_sections;
+    const int32_t* section_airfoil;
+    const double* section_span;   /* increasing, 0 at the wing's first station, 1 at its end */
+} afx_polar;
+
+typedef struct afx_nonlinear_info {
+    int32_t iterations;
+    int32_t converged;
+    double residual;              /* sqrt(sum_s ((cl_visc - cl_inv) / cl_visc)^2) of the last iteration */
+    double cl, cd, cy;            /* viscous coefficients (nonlinear::steady::computeForces) */
+} afx_nonlinear_info;
+
+/* nonlinear::steady::solve (vlm/src/solver.cpp:330-388, iterate :390-423):
+ * the matrix is built and factored once at the flow's alpha; each iteration
+ * solves Gamma for rhs_i = -V(alpha_s).n_i with the station's local angle
+ * alpha_s, takes each station's inviscid cl, interpolates the viscous cl at
+ * alpha_eff = cl_inv / CLa - alpha_s + alpha (CLa = 2 pi per radian in
+ * degrees, VLM_CL_ALPHA_DEG) and moves alpha_s by relaxation (cl_visc -
+ * cl_inv) / CLa, until the residual is <= tol or max_iter is passed.  The
+ * handle's stations form one wing, in their given order.  station_alpha,
+ * station_cl (nullable): n_stations each, final local angles (degrees) and
+ * viscous section lift coefficients. */
+int afx_vlm_nonlinear_solve(afx_vlm* h, const afx_polar* polar, double relaxation, double tol, int32_t max_iter,
+                            afx_nonlinear_info* info, double* station_alpha, double* station_cl);
+
 /* ------------------------------------------- dense FP32-LU / FP64 solve --- */
 
 /* The solver behind afx_vlm_solve for any dense matrix (no reference

@@ -18,6 +18,8 @@
 //                 BiCGSTAB: dense LU with partial pivoting in FP64
 //   ovl_forces    surface.cpp:164-201 (wingStation::computeForces), solver.cpp:
 //                 112-128 and surface.cpp:262-277 (CL summed over stations)
+//   ovl_nonlinear nonlinear::steady::solve / iterate / computeForces
+//                 (solver.cpp:330-476), database.cpp:192-242 (polars)
 //   ovl_beam_*    element.hpp:77-160 (set_K_local, set_T_Rotation,
 //                 set_K_elem_global), model.hpp:103-131 (assembly),
 //                 225-238 (linear solve) with SPC1 as row/column zeroing
@@ -353,6 +355,109 @@ void ovl_transfer_disp(int nn, const double* x0, int n, const double* bxyz, int 
         }
         store3(xyz + 3 * k, p + u);
     }
+}
+
+// nonlinear::steady (vlm/src/solver.cpp:330-476) with exact FP64 solves and
+// the polar interpolation of database.cpp:192-242 (mlinterp: linear, end
+// values outside).  Polar rows: per airfoil [off[a], off[a+1]) of
+// (alpha_deg, cl, cd, cmy); sections (airfoil, span 0..1).  out = {iterations,
+// residual, CL, CD, CY}; alpha_s and station cl (ns each) returned.
+static double lin1(const double* xd, const double* yd, int n, double x) {
+    if (n == 1 || x <= xd[0]) return yd[0];
+    if (x >= xd[n - 1]) return yd[n - 1];
+    int m = 0;
+    while (!(x >= xd[m] && x < xd[m + 1])) ++m;   // linear scan (the reference bisects; same interval)
+    double w = (xd[m + 1] - x) / (xd[m + 1] - xd[m]);
+    return w * yd[m] + (1 - w) * yd[m + 1];
+}
+
+static void polar(int na, const int32_t* off, const double* pa, const double* pcl, const double* pcd, int nsec,
+                  const int32_t* sa, const double* sspan, double alpha, double span, double* out) {
+    (void)na;
+    std::vector<double> c0, c1;
+    for (int q = 0; q < nsec; ++q) {
+        int a = sa[q], o = off[a], n = off[a + 1] - o;
+        c0.push_back(lin1(pa + o, pcl + o, n, alpha));
+        c1.push_back(lin1(pa + o, pcd + o, n, alpha));
+    }
+    out[0] = lin1(sspan, c0.data(), nsec, span);
+    out[1] = lin1(sspan, c1.data(), nsec, span);
+}
+
+void ovl_nonlinear(int nn, const double* xyz, int ns, const int32_t* soff, const int32_t* rn, double aoa, double vinf,
+                   double rho, double sref, double cref, int na, const int32_t* off, const double* pa, const double* pcl,
+                   const double* pcd, int nsec, const int32_t* sa, const double* sspan, double relax, double tol,
+                   int max_iter, double* alpha_s, double* cl_s, double* out) {
+    Lattice L{nn, soff[ns], ns, xyz, soff, rn};
+    const int N = L.nr;
+    const double CLA = 0.109662271123215082635482531259185634553432464599609375;
+    std::vector<double> A((size_t)N * N), rhs(N), g(N);
+    ovl_build(nn, xyz, ns, soff, rn, aoa, vinf, 100.0 * cref, 0.0, A.data(), rhs.data());
+    std::vector<V3> nor(N);
+    for (int k = 0; k < N; ++k) {
+        V3 c;
+        double a;
+        geometry(L, k, aoa, c, nor[k], a);
+    }
+    std::vector<double> area(ns, 0.0), spanloc(ns), clinv(ns);
+    std::vector<V3> le(ns);
+    double span = 0;
+    for (int s = 0; s < ns; ++s) {
+        for (int k = soff[s]; k < soff[s + 1]; ++k) {
+            V3 c, n;
+            double a;
+            geometry(L, k, aoa, c, n, a);
+            area[s] += a;
+        }
+        le[s] = L.node(soff[s], 1) - L.node(soff[s], 0);
+        span += nrm(le[s]);
+        spanloc[s] = span - 0.5 * nrm(le[s]);
+    }
+    const double root = spanloc[0];
+    for (int s = 0; s < ns; ++s) spanloc[s] = (spanloc[s] - root) / span;
+    for (int s = 0; s < ns; ++s) alpha_s[s] = aoa;
+    const double q = 0.5 * rho * vinf * vinf;
+    auto axis = [&](int s) { V3 v = cross(freestream(alpha_s[s], 1.0), le[s]); return (1.0 / nrm(v)) * v; };
+    int iter = 0;
+    double res = 0;
+    do {
+        for (int s = 0; s < ns; ++s)
+            for (int k = soff[s]; k < soff[s + 1]; ++k) rhs[k] = -dot(freestream(alpha_s[s], vinf), nor[k]);
+        ovl_solve(N, A.data(), rhs.data(), g.data());
+        res = 0;
+        for (int s = 0; s < ns; ++s) {
+            V3 V = freestream(alpha_s[s], vinf), F{0, 0, 0};
+            double prev = 0;
+            for (int k = soff[s]; k < soff[s + 1]; ++k) {
+                F = F + (rho * (g[k] - prev)) * cross(V, L.node(k, 1) - L.node(k, 0));
+                prev = g[k];
+            }
+            clinv[s] = dot(F, axis(s)) / (q * area[s]);
+            double c2[2];
+            polar(na, off, pa, pcl, pcd, nsec, sa, sspan, clinv[s] / CLA - alpha_s[s] + aoa, spanloc[s], c2);
+            alpha_s[s] += relax * (c2[0] - clinv[s]) / CLA;
+            res += ((c2[0] - clinv[s]) / c2[0]) * ((c2[0] - clinv[s]) / c2[0]);
+        }
+        res = std::sqrt(res);
+        ++iter;
+    } while (res > tol && iter <= max_iter);
+    double a = aoa * M_PI / 180.0;
+    V3 lift{-std::sin(a), 0.0, std::cos(a)};
+    double cl = 0, cd = 0, cy = 0;
+    for (int s = 0; s < ns; ++s) {
+        double c2[2];
+        polar(na, off, pa, pcl, pcd, nsec, sa, sspan, clinv[s] / CLA - alpha_s[s] + aoa, spanloc[s], c2);
+        V3 ax = axis(s);
+        cl_s[s] = c2[0] * dot(ax, lift);
+        cl += cl_s[s] * area[s] / sref;
+        cy += c2[0] * ax.y * area[s] / sref;
+        cd += c2[1] * area[s] / sref;
+    }
+    out[0] = iter;
+    out[1] = res;
+    out[2] = cl;
+    out[3] = cd;
+    out[4] = cy;
 }
 
 // fixed-point coupling with exact FP64 solves; out = {iterations, tip, CL}

@@ -22,6 +22,8 @@ for _name, _args in {
     "ovl_beam_solve": [_i, _dp, _i, _ip, _dp, _i, _ip, _dp, _dp],
     "ovl_transfer_loads": [_i, _dp, _dp, _i, _dp, _i, _ip, _dp],
     "ovl_transfer_disp": [_i, _dp, _i, _dp, _i, _ip, _dp, _dp],
+    "ovl_nonlinear": [_i, _dp, _i, _ip, _ip, _f, _f, _f, _f, _f, _i, _ip, _dp, _dp, _dp, _i, _ip, _dp, _f, _f, _i,
+                      _dp, _dp, _dp],
     "ovl_aeroelastic": [_i, _dp, _i, _ip, _ip, _f, _f, _f, _f, _f, _i, _dp, _i, _ip, _dp, _i, _ip, _f, _i, _f,
                         _dp, _dp],
 }.items():
@@ -136,3 +138,19 @@ def aeroelastic(xyz, rings, soff, alpha, beam, vinf=1.0, rho=1.0, sref=1.0, cref
                          len(bxyz), _d(bxyz), len(en), _n(en), _d(props), len(clamped), _n(clamped), tol, max_iter,
                          relax, _d(disp), _d(out))
     return {"iterations": int(out[0]), "tip": out[1], "cl": out[2], "disp": disp.reshape(-1, 6)}
+
+
+def nonlinear(xyz, rings, soff, alpha, polar, vinf=1.0, rho=1.0, sref=1.0, cref=1.0, relax=1.0, tol=1e-6,
+              max_iter=100):
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    rings = np.ascontiguousarray(rings, np.int32)
+    soff = np.ascontiguousarray(soff, np.int32)
+    p = {k: np.ascontiguousarray(v) for k, v in polar.items()}
+    ns = len(soff) - 1
+    a, c, out = np.zeros(ns), np.zeros(ns), np.zeros(5)
+    _lib.ovl_nonlinear(len(xyz), _d(xyz), ns, _n(soff), _n(rings), alpha, vinf, rho, sref, cref,
+                       len(p["offsets"]) - 1, _n(p["offsets"]), _d(p["alpha_deg"]), _d(p["cl"]), _d(p["cd"]),
+                       len(p["section_airfoil"]), _n(p["section_airfoil"]), _d(p["section_span"]), relax, tol,
+                       max_iter, _d(a), _d(c), _d(out))
+    return {"iterations": int(out[0]), "residual": out[1], "cl": out[2], "cd": out[3], "cy": out[4],
+            "alpha": a, "station_cl": c}

@@ -196,3 +196,50 @@ def test_full_size_wing_solve():
     tan_half = math.tan(math.radians(25.0)) - 4 / 10 * 0.25 * 0.6 / 1.4
     helm = 2 * math.pi * 10 / (2 + math.sqrt(100 * (1 + tan_half ** 2) + 4))
     assert abs(slope - helm) / helm < 0.03, (slope, helm)
+
+
+# ------------------------------------------------ nonlinear (viscous polar) --
+GOLD_VLM = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "vlm")
+
+
+@pytest.mark.parametrize("alpha,relax", [(13.0, 0.5), (16.0, 0.3)])
+def test_nonlinear_stall_matches_oracle(vlm, ov, alpha, relax):
+    print(f"[vlm] nonlinear alpha={alpha}", flush=True)
+    polar = vlm.read_polar(f"{GOLD_VLM}/database_stall.dat", f"{GOLD_VLM}/locations.dat")
+    xyz, rn, so, ref = small_wing(vlm, 6, 20)
+    h = vlm.VLM(xyz, rn, so)
+    h.set_flow(alpha, sref=ref["sref"], cref=ref["cref"])
+    info, a, c = h.nonlinear_solve(polar, relaxation=relax, tol=1e-9, max_iter=300)
+    r = ov.nonlinear(xyz, rn, so, alpha, polar, sref=ref["sref"], cref=ref["cref"], relax=relax, tol=1e-9,
+                     max_iter=300)
+    assert info.converged and info.iterations == r["iterations"]
+    np.testing.assert_allclose(a, r["alpha"], rtol=0, atol=1e-8)
+    np.testing.assert_allclose(c, r["station_cl"], rtol=0, atol=1e-8)
+    assert abs(info.cl - r["cl"]) <= 1e-9 and abs(info.cy - r["cy"]) <= 1e-9
+
+
+def test_nonlinear_on_the_reference_lattice_file(vlm, ov):
+    """examples/vlm/wing_0.dat (two wings of 4 rings) with the stall polar."""
+    polar = vlm.read_polar(f"{GOLD_VLM}/database_stall.dat", f"{GOLD_VLM}/locations.dat")
+    xyz, rn, so = vlm.read_lattice(f"{GOLD_VLM}/wing_0.dat")
+    h = vlm.VLM(xyz, rn, so)
+    h.set_flow(15.0, sref=4.0, cref=1.0)
+    info, a, c = h.nonlinear_solve(polar, relaxation=0.5, tol=1e-10, max_iter=500)
+    r = ov.nonlinear(xyz, rn, so, 15.0, polar, sref=4.0, cref=1.0, relax=0.5, tol=1e-10, max_iter=500)
+    assert info.converged
+    np.testing.assert_allclose(a, r["alpha"], rtol=0, atol=1e-8)
+    assert abs(info.cl - r["cl"]) <= 1e-9 * max(1.0, abs(r["cl"]))
+
+
+def test_nonlinear_thin_airfoil_polar_reproduces_the_linear_cl(vlm):
+    polar = vlm.read_polar(f"{GOLD_VLM}/database.dat", f"{GOLD_VLM}/locations.dat")
+    import test_oracle_vlm as t
+    xyz, rn, so, sref = t.lattice(4, 16, 8.0, 1.0)
+    h = vlm.VLM(xyz, rn, so)
+    h.set_flow(4.0, sref=sref, cref=1.0)
+    info, a, c = h.nonlinear_solve(polar)
+    h.build()
+    h.factor()
+    h.solve()
+    _, _, lin = h.forces()
+    assert info.iterations == 1 and abs(info.cl - lin["cl"]) <= 1e-10 * abs(lin["cl"])

@@ -263,3 +263,43 @@ def test_coupled_wing_converges_and_bends_up():
     # the swept wing washes out as it bends: less lift than the rigid wing
     cl_rigid, _ = _cl(xyz, rings, soff, sref, 5.0)
     assert r["cl"] < cl_rigid
+
+
+# ------------------------------------------------- nonlinear (viscous polar) --
+GOLD = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "vlm")
+
+
+def _polar(name):
+    from aeroflex_amd import vlm as V
+    return V.read_polar(f"{GOLD}/{name}", f"{GOLD}/locations.dat", surface=0)
+
+
+def test_polar_reader_on_the_reference_files():
+    p = _polar("database.dat")
+    assert list(p["offsets"]) == [0, 12] and p["alpha_deg"][0] == -20 and p["alpha_deg"][-1] == 35
+    np.testing.assert_allclose(p["cl"], 0.109662271123215 * p["alpha_deg"], rtol=1e-14, atol=1e-15)
+    assert list(p["section_airfoil"]) == [0, 0] and list(p["section_span"]) == [0.0, 1.0]
+    s = _polar("database_stall.dat")
+    assert len(s["alpha_deg"]) == 19 and abs(s["cl"][s["alpha_deg"] == 15][0] - 1.51139581863892) < 1e-15
+
+
+def test_nonlinear_with_the_thin_airfoil_polar_is_the_linear_solution():
+    # CL = 2 pi alpha is the lattice's own section slope: no correction, one
+    # iteration, viscous CL = inviscid CL (on an unswept rectangle, where the
+    # station lift axes are the flow's)
+    xyz, rings, soff, sref = lattice(3, 12, 6.0, 1.0)
+    r = ov.nonlinear(xyz, rings, soff, 5.0, _polar("database.dat"), sref=sref)
+    cl_lin, _ = _cl(xyz, rings, soff, sref, 5.0)
+    assert r["iterations"] == 1 and r["residual"] < 1e-14
+    np.testing.assert_allclose(r["alpha"], 5.0, rtol=0, atol=1e-13)
+    assert abs(r["cl"] - cl_lin) <= 1e-12 * abs(cl_lin)
+
+
+def test_nonlinear_stall_lowers_lift_and_converges():
+    xyz, rings, soff, sref = lattice(3, 16, 8.0, 1.0)
+    r = ov.nonlinear(xyz, rings, soff, 16.0, _polar("database_stall.dat"), sref=sref, relax=0.5, tol=1e-8,
+                     max_iter=400)
+    cl_lin, _ = _cl(xyz, rings, soff, sref, 16.0)
+    assert r["residual"] <= 1e-8 and r["cl"] < cl_lin
+    a = r["alpha"]
+    np.testing.assert_allclose(a, a[::-1], rtol=1e-8)
