@@ -198,7 +198,12 @@ public:
 // rans::implicitSolver (solver.h:852-1213)
 class implicitSolver : public solver {
 public:
-    implicitSolver(const mesh& m, const gas& g, std::string viscosity_model) : solver(m, g, viscosity_model) {}
+    implicitSolver(const mesh& m, const gas& g, std::string viscosity_model) : solver(m, g, viscosity_model) {
+#ifdef RANS_MATRIX_FREE
+        // the same compile-time switch as solver.h:863-868
+        check(afx_rans_set_matrix_free(h_, 1));
+#endif
+    }
     void fill() override { check(afx_rans_implicit_fill(h_)); }
     int compute() override {
         int32_t st = -1;

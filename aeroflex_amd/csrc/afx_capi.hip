@@ -141,6 +141,8 @@ struct afx_rans {
     double gm_tol = 1e-2;
     bool lhs_ready = false;
     int lhs_bad = 0;
+    bool mf = false;             // matrix-free operator (RANS_MATRIX_FREE build)
+    DBuf<double> mfFb;           // its unperturbed face fluxes at the current state
     int64_t lin_iters = 0;
     int kblk = 0;
     std::vector<int32_t> t_lg;   // device cell of each local cell (unflagged)
@@ -395,6 +397,7 @@ int check_handle(afx_rans* h, bool need_bcs = true) {
 
 // ------------------------------------------------------------ implicit step --
 int ensure_implicit(afx_rans* h) {
+    if (h->mf && !h->mfFb.p) HIPCHK(h->mfFb.alloc((size_t)4 * h->nF));
     if (h->Jf.p) return AFX_OK;
     const size_t n = (size_t)NV * h->nT;
     const int m = h->gm_restart;
@@ -430,6 +433,20 @@ int gmres(afx_rans* h, const double* b, double* x, bool* converged) {
     double* V = h->kV.p;
     double* hd = h->kh.p;            // [0, m+2): first pass, [m+2, 2m+4): second pass + norm
     *converged = false;
+    // the operator and the right preconditioner: the assembled matrix with
+    // block Jacobi, or the matrix-free product with none (solver.h:868)
+    auto apply = [&](const double* in, double* out) -> hipError_t {
+        if (h->mf)
+            return launch_mf_apply(s, h->cells64(), h->faces64(), h->q.p, h->dt.p, h->mfFb.p, h->gas64(),
+                                   h->opts.viscosity, in, out);
+        return launch_lhs_apply(s, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, in, out);
+    };
+    auto precond = [&](const double* in, double* out, const double** res) -> hipError_t {
+        if (h->mf) { *res = in; return hipSuccess; }
+        *res = out;
+        return launch_block_jacobi(s, h->nC, h->nT, h->Dinv.p, in, out);
+    };
+    const double* pz = nullptr;
     HIPCHK(hipMemsetAsync(x, 0, n * 8, s));
     int rc;
     if ((rc = dots(h, b, n, 1, b, n, hd))) return rc;
@@ -450,8 +467,8 @@ int gmres(afx_rans* h, const double* b, double* x, bool* converged) {
         bool conv = false;
         for (int j = 0; j < m; ++j) {
             double* vj = V + (size_t)j * n;
-            HIPCHK(launch_block_jacobi(s, h->nC, h->nT, h->Dinv.p, vj, h->kz.p));
-            HIPCHK(launch_lhs_apply(s, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, h->kz.p, h->kw.p));
+            HIPCHK(precond(vj, h->kz.p, &pz));
+            HIPCHK(apply(pz, h->kw.p));
             if ((rc = dots(h, V, n, j + 1, h->kw.p, n, hd))) return rc;
             HIPCHK(launch_multiaxpy(s, V, n, j + 1, hd, h->kw.p, n));
             if ((rc = dots(h, V, n, j + 1, h->kw.p, n, hd + m + 2))) return rc;
@@ -490,13 +507,13 @@ int gmres(afx_rans* h, const double* b, double* x, bool* converged) {
         HIPCHK(hipMemcpyAsync(hd, y.data(), jend * 8, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemsetAsync(h->ktmp.p, 0, n * 8, s));
         HIPCHK(launch_lincomb(s, V, n, jend, hd, h->ktmp.p, n));
-        HIPCHK(launch_block_jacobi(s, h->nC, h->nT, h->Dinv.p, h->ktmp.p, h->kz.p));
-        HIPCHK(launch_axpy(s, 1.0, h->kz.p, x, n));
+        HIPCHK(precond(h->ktmp.p, h->kz.p, &pz));
+        HIPCHK(launch_axpy(s, 1.0, pz, x, n));
         h->lin_iters += jend;
         if (conv) { *converged = true; HIPCHK(hipStreamSynchronize(s)); return AFX_OK; }
         if (total >= h->gm_maxit) { HIPCHK(hipStreamSynchronize(s)); return AFX_OK; }
         // restart from the true residual
-        HIPCHK(launch_lhs_apply(s, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, x, h->kw.p));
+        HIPCHK(apply(x, h->kw.p));
         HIPCHK(launch_copy(s, b, h->kr.p, n));
         HIPCHK(launch_axpy(s, -1.0, h->kw.p, h->kr.p, n));
         if ((rc = dots(h, h->kr.p, n, 1, h->kr.p, n, hd))) return rc;
@@ -1316,6 +1333,16 @@ int afx_rans_implicit_fill(afx_rans* h) {
     if ((rc = ensure_implicit(h))) return rc;
     hipStream_t s = h->stream;
     HIPCHK(launch_dt<double>(s, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->cfl, h->dt.p));
+    if (h->mf) {
+        // fillRhoLHS of the matrix-free build assembles nothing: the operator
+        // reads q and dt when applied.  (Its gradient evaluation feeds no
+        // output of the two-argument fluxes, so it is skipped.)
+        HIPCHK(hipStreamSynchronize(s));
+        h->lhs_ready = true;
+        h->lhs_bad = 0;
+        h->q32_valid = false;
+        return AFX_OK;
+    }
     if (h->opts.viscosity != 0 && (rc = run_gradients64(h, h->q.p, h->q.p))) return rc;
     HIPCHK(launch_face_jacobian(s, h->cells64(), h->faces64(), h->q.p, h->gx.p, h->gy.p, h->gas64(),
                                 h->opts.viscosity, h->Jf.p));
@@ -1346,11 +1373,16 @@ int afx_rans_implicit_solve(afx_rans* h, double relaxation, double tol, int32_t 
     const int64_t n = (int64_t)NV * h->nT;
     h->lin_iters = 0;
     auto residual_norm = [&](double* e) -> int {
+        // fillRhoRHS starts with calc_dt (solver.h:1081); only the
+        // matrix-free operator reads dt after the fill
+        if (h->mf) HIPCHK(launch_dt<double>(s, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->cfl, h->dt.p));
         int r2 = enqueue_residual(h, AFX_FP64);
         if (r2) return r2;
         return norm_of(h, h->R.p, h->nT, NV, e);
     };
     auto step = [&](bool* ok) -> int {
+        if (h->mf)
+            HIPCHK(launch_mf_base(s, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->opts.viscosity, h->mfFb.p));
         int r2 = gmres(h, h->R.p, h->qW.p, ok);
         if (r2) return r2;
         if (*ok) {
@@ -1381,8 +1413,34 @@ int afx_rans_lhs_apply(afx_rans* h, const double* x, double* y) {
     int rc = check_handle(h);
     if (rc) return rc;
     if (!h->lhs_ready) return fail(AFX_ESTATE, "call afx_rans_implicit_fill first");
+    if (h->mf) return fail(AFX_ESTATE, "no assembled matrix in matrix-free mode (afx_rans_matrix_free_apply)");
     if ((rc = upload_aos(h, x, h->ktmp.p))) return rc;
     HIPCHK(launch_lhs_apply(h->stream, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, h->ktmp.p, h->kw.p));
+    return download_aos(h, h->kw.p, y);
+}
+
+int afx_rans_set_matrix_free(afx_rans* h, int32_t enable) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (enable != 0 && enable != 1) return fail(AFX_EARG, "enable must be 0 or 1");
+    if (h->sa()) return fail(AFX_EUNSUP, "the implicit step is not available with the Spalart-Allmaras model");
+    h->mf = enable != 0;
+    // the GMRES settings each build of implicitSolver sets (solver.h:906-914)
+    h->gm_tol = h->mf ? 1e-3 : 1e-2;
+    h->gm_maxit = h->mf ? 1000 : 500;
+    h->lhs_ready = false;
+    return AFX_OK;
+}
+
+int afx_rans_matrix_free_apply(afx_rans* h, const double* x, double* y) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!x || !y) return fail(AFX_EARG, "null vector");
+    if (!h->mf || !h->lhs_ready) return fail(AFX_ESTATE, "call afx_rans_set_matrix_free(h, 1) and afx_rans_implicit_fill first");
+    if ((rc = upload_aos(h, x, h->ktmp.p))) return rc;
+    HIPCHK(launch_mf_base(h->stream, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->opts.viscosity, h->mfFb.p));
+    HIPCHK(launch_mf_apply(h->stream, h->cells64(), h->faces64(), h->q.p, h->dt.p, h->mfFb.p, h->gas64(),
+                           h->opts.viscosity, h->ktmp.p, h->kw.p));
     return download_aos(h, h->kw.p, y);
 }
 

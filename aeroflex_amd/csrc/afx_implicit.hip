@@ -248,6 +248,74 @@ __global__ void __launch_bounds__(kBlock) k_axpy(double a, const double* __restr
     if (i < n) y[i] += a * x[i];
 }
 
+// ---------------------------------------------- matrix-free operator (a18) --
+// RansMatrix * x (ransMatrix.h:138-178, the RANS_MATRIX_FREE build): the
+// operator calls each face's flux with two arguments, i.e. at cell values and
+// zero gradients, with one step h = 1e-6 along the whole 4-vector of the
+// perturbed side and no face length.  Per face it adds
+// (F(q_L + h x_L, q_R) - F)/h to the owner row and, for two-sided faces,
+// subtracts (F(q_L, q_R + h x_R) - F)/h from the neighbour row: every row sees
+// only its own block of x.  Real rows start from x A / dt, ghost rows are x.
+
+// F(q_L, q_R) per face at the current state (SoA Fb[k * nF + f])
+__global__ void __launch_bounds__(kBlock) k_mf_base(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
+                                                    GasT<double> g, int viscous, double* __restrict__ Fb) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F.nF) return;
+    const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
+    const int o = F.c0[f], n = F.c1[f], kind = F.kind[f];
+    double qL[4], qR[4], f0[4];
+    const double z[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k) { qL[k] = q[k * nT + o]; qR[k] = q[k * nT + n]; }
+    face_flux<double>(g, kind, F.nx[f], F.ny[f], qL, qR, (viscous == 1) && (kind == K_INTERNAL), z, z, f0);
+    for (int k = 0; k < 4; ++k) Fb[k * nF + f] = f0[k];
+}
+
+// one thread per cell; the cell's faces in reference face order, as the
+// reference's edge loop reaches them
+__global__ void __launch_bounds__(kBlock) k_mf_apply(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
+                                                     const double* __restrict__ dt, const double* __restrict__ Fb,
+                                                     GasT<double> g, int viscous, const double* __restrict__ x,
+                                                     double* __restrict__ y) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C.nT) return;
+    const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
+    double xc[4];
+    for (int k = 0; k < 4; ++k) xc[k] = x[k * nT + c];
+    if (c >= C.nC) {
+        for (int k = 0; k < 4; ++k) y[k * nT + c] = xc[k];
+        return;
+    }
+    constexpr double h = 1e-6;
+    const double z[4] = {0, 0, 0, 0};
+    double acc[4], qc[4], qp[4], qo[4], fp[4];
+    for (int k = 0; k < 4; ++k) {
+        acc[k] = xc[k] * C.area[c] / dt[c];
+        qc[k] = q[k * nT + c];
+        qp[k] = qc[k] + h * xc[k];
+    }
+    for (int s = 0; s < 4; ++s) {
+        uint32_t e = C.cface[(size_t)s * C.nC + c];
+        if (e == 0xFFFFFFFFu) break;
+        const int f = (int)(e & 0x7FFFFFFFu);
+        const int kind = F.kind[f];
+        const bool lam = (viscous == 1) && (kind == K_INTERNAL);
+        if (e >> 31) {   // neighbour row of a two-sided face
+            if (kind != K_INTERNAL) continue;
+            const int o = F.c0[f];
+            for (int k = 0; k < 4; ++k) qo[k] = q[k * nT + o];
+            face_flux<double>(g, kind, F.nx[f], F.ny[f], qo, qp, lam, z, z, fp);
+            for (int k = 0; k < 4; ++k) acc[k] -= (fp[k] - Fb[k * nF + f]) / h;
+        } else {         // owner row
+            const int n = F.c1[f];
+            for (int k = 0; k < 4; ++k) qo[k] = q[k * nT + n];
+            face_flux<double>(g, kind, F.nx[f], F.ny[f], qp, qo, lam, z, z, fp);
+            for (int k = 0; k < 4; ++k) acc[k] += (fp[k] - Fb[k * nF + f]) / h;
+        }
+    }
+    for (int k = 0; k < 4; ++k) y[k * nT + c] = acc[k];
+}
+
 // ---------------------------------------------------------------- launchers --
 hipError_t launch_face_jacobian(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* gx,
                                 const double* gy, GasT<double> g, int viscous, double* Jf) {
@@ -290,6 +358,17 @@ hipError_t launch_scale(hipStream_t s, double a, const double* x, double* y, int
 }
 hipError_t launch_axpy(hipStream_t s, double a, const double* x, double* y, int64_t n) {
     hipLaunchKernelGGL(k_axpy, dim3(nblocks(n)), dim3(kBlock), 0, s, a, x, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_mf_base(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, GasT<double> g,
+                          int viscous, double* Fb) {
+    hipLaunchKernelGGL(k_mf_base, dim3(nblocks(F.nF)), dim3(kBlock), 0, s, C, F, q, g, viscous, Fb);
+    return hipGetLastError();
+}
+hipError_t launch_mf_apply(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* dt,
+                           const double* Fb, GasT<double> g, int viscous, const double* x, double* y) {
+    hipLaunchKernelGGL(k_mf_apply, dim3(nblocks(C.nT)), dim3(kBlock), 0, s, C, F, q, dt, Fb, g, viscous, x, y);
     return hipGetLastError();
 }
 

@@ -17,5 +17,10 @@ hipError_t launch_multiaxpy(hipStream_t s, const double* V, int64_t ld, int m, c
 hipError_t launch_lincomb(hipStream_t s, const double* V, int64_t ld, int m, const double* y, double* x, int64_t n);
 hipError_t launch_scale(hipStream_t s, double a, const double* x, double* y, int64_t n);
 hipError_t launch_axpy(hipStream_t s, double a, const double* x, double* y, int64_t n);
+// matrix-free operator of the RANS_MATRIX_FREE build (ransMatrix.h:138-178)
+hipError_t launch_mf_base(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, GasT<double> g,
+                          int viscous, double* Fb);
+hipError_t launch_mf_apply(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* dt,
+                           const double* Fb, GasT<double> g, int viscous, const double* x, double* y);
 
 }  // namespace afx

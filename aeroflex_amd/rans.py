@@ -90,6 +90,8 @@ def _load():
         "afx_rans_implicit_compute": (C.c_int, [P, C.POINTER(i32)]),
         "afx_rans_implicit_solve": (C.c_int, [P, f64, f64, i32, i32, dp]),
         "afx_rans_lhs_apply": (C.c_int, [P, dp, dp]),
+        "afx_rans_set_matrix_free": (C.c_int, [P, C.c_int32]),
+        "afx_rans_matrix_free_apply": (C.c_int, [P, dp, dp]),
         "afx_rans_linear_iterations": (C.c_int, [P, C.POINTER(i64)]),
         "afx_rans_wall_forces": (C.c_int, [P, C.c_char_p, dp, dp, dp]),
         "afx_rans_wall_dist": (C.c_int, [P, dp]),
@@ -395,6 +397,18 @@ class Solver:
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.empty_like(x)
         _check(lib.afx_rans_lhs_apply(self.h, _dptr(x), _dptr(y)))
+        return y
+
+    def set_matrix_free(self, enable=True):
+        """The RANS_MATRIX_FREE build of implicitSolver (solver.h:863-925):
+        RansMatrix operator, unpreconditioned GMRES (tol 1e-3, 1000 its)."""
+        _check(lib.afx_rans_set_matrix_free(self.h, 1 if enable else 0))
+
+    def matrix_free_apply(self, x):
+        """RansMatrix * x (ransMatrix.h:150-178) at the current q and dt."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        _check(lib.afx_rans_matrix_free_apply(self.h, _dptr(x), _dptr(y)))
         return y
 
     def linear_iterations(self):

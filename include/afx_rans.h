@@ -183,6 +183,23 @@ int afx_rans_implicit_solve(afx_rans* h, double relaxation, double tol, int32_t 
                             double* err);
 /* y = LHS x with the matrix of the last fill (AoS, 4*n_cells); for tests */
 int afx_rans_lhs_apply(afx_rans* h, const double* x, double* y);
+/* The RANS_MATRIX_FREE build of implicitSolver (solver.h:863-868, 911-914,
+ * 920-925): enable = 1 replaces the assembled matrix by the matrix-free
+ * operator RansMatrix (ransMatrix.h:138-178) and the block-Jacobi GMRES by
+ * unpreconditioned GMRES with that build's tolerance 1e-3 and 1000 iterations
+ * (afx_rans_set_linear_solver afterwards overrides them); enable = 0 returns
+ * to the assembled form (1e-2, 500).  Either way the next implicit step needs
+ * a new afx_rans_implicit_fill.  As in that build, the operator reads the
+ * current q and the dt of the latest residual, so it follows the state
+ * through the rhs_iterations of afx_rans_implicit_solve. */
+int afx_rans_set_matrix_free(afx_rans* h, int32_t enable);
+/* y = RansMatrix x (ransMatrix.h:150-178) at the current q and dt (AoS,
+ * 4*n_cells): x A/dt on real rows and x on ghost rows, plus per face
+ * (F(q_L + 1e-6 x_L, q_R) - F(q_L, q_R))/1e-6 on the owner row and, for
+ * interior faces, minus (F(q_L, q_R + 1e-6 x_R) - F)/1e-6 on the neighbour
+ * row, fluxes at cell values without gradients or face lengths.  Needs
+ * matrix-free mode and a fill. */
+int afx_rans_matrix_free_apply(afx_rans* h, const double* x, double* y);
 /* iterations of the last linear solves (sum over the last implicit_solve) */
 int afx_rans_linear_iterations(afx_rans* h, int64_t* iters);
 

@@ -932,6 +932,35 @@ struct Solver {
             for (int j = 0; j < 4; ++j) { rows.push_back(4 * i + j); cols.push_back(4 * i + j); vals.push_back(1.0); }
     }
 
+    // ransMatrix.h:138-178 (RansMatrix * x, RANS_MATRIX_FREE build) at the
+    // member q and dt: real rows x * A / dt, ghost rows x; per edge the
+    // two-argument flux (zero gradients), one step h = 1e-6 along the whole
+    // perturbed 4-vector, no edge length
+    void mf_apply(const double* x, double* y) const {
+        const int64_t nT = m.nCells(), nR = m.nRealCells;
+        for (int64_t i = 0; i < nR; ++i)
+            for (int k = 0; k < 4; ++k) y[4 * i + k] = x[4 * i + k] * m.cellsAreas[i] / dt[i];
+        for (int64_t i = nR; i < nT; ++i)
+            for (int k = 0; k < 4; ++k) y[4 * i + k] = x[4 * i + k];
+        const double h = 1e-6;
+        const double zero[4] = {0, 0, 0, 0};
+        for (u32 e = 0; e < m.nFaces(); ++e) {
+            const u32 i = m.edgesCells[2 * e], j = m.edgesCells[2 * e + 1];
+            const double* qL = &q[4 * (size_t)i];
+            const double* qR = &q[4 * (size_t)j];
+            double FB[4], F1[4], qp[4];
+            face_flux(ff[e], qL, qR, zero, zero, FB);
+            for (int k = 0; k < 4; ++k) qp[k] = qL[k] + h * x[4 * (size_t)i + k];
+            face_flux(ff[e], qp, qR, zero, zero, F1);
+            for (int k = 0; k < 4; ++k) y[4 * (size_t)i + k] += (F1[k] - FB[k]) / h;
+            if (ff[e].two_sided()) {
+                for (int k = 0; k < 4; ++k) qp[k] = qR[k] + h * x[4 * (size_t)j + k];
+                face_flux(ff[e], qL, qp, zero, zero, F1);
+                for (int k = 0; k < 4; ++k) y[4 * (size_t)j + k] -= (F1[k] - FB[k]) / h;
+            }
+        }
+    }
+
     // physics.h:533-578 (calc_convective_jacobian) for face e at first-order
     // states as in fillRhoLHS (solver.h:1030-1043), times the face length
     void face_jacobian(u32 e, double J[64]) {
@@ -1504,6 +1533,7 @@ void orc_solver_wall_profile(void* h, const char* patch, double* out3) {
     ((Solver*)h)->wall_profile(patch, &out3[0], &out3[1], &out3[2]);
 }
 void orc_solver_face_jacobian(void* h, int e, double* J) { ((Solver*)h)->face_jacobian((u32)e, J); }
+void orc_solver_mf_apply(void* h, const double* x, double* y) { ((Solver*)h)->mf_apply(x, y); }
 
 // fillRhoLHS triplets: returns the count; call with null buffers to size
 static thread_local std::vector<int64_t> g_rows, g_cols;

@@ -43,6 +43,7 @@ def _load():
         "orc_solver_uniform_residual": (C.c_double, [P]),
         "orc_solver_wall_profile": (None, [P, C.c_char_p, dp]),
         "orc_solver_face_jacobian": (None, [P, C.c_int, dp]),
+        "orc_solver_mf_apply": (None, [P, dp, dp]),
         "orc_solver_fill_lhs": (C.c_long, [P, P, P, P]),
         "orc_solver_wall_dist": (None, [P]),
         "orc_solver_vec": (C.c_long, [P, C.c_char_p, dp, dp]),
@@ -196,6 +197,13 @@ class OSolver:
         J = np.zeros(64)
         lib.orc_solver_face_jacobian(self.h, e, _d(J))
         return J.reshape(8, 8)
+
+    def mf_apply(self, x):
+        """RansMatrix * x (ransMatrix.h:150-178) at the member q and dt."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros_like(x)
+        lib.orc_solver_mf_apply(self.h, _d(x), _d(y))
+        return y
 
     def fill_lhs(self):
         """fillRhoLHS (solver.h:979-1075) as a scipy CSR matrix (4N x 4N)."""

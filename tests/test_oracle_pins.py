@@ -545,3 +545,78 @@ def test_face_jacobian_matches_central_differences():
             d = (orc.face_flux(0, qp, qq[c1], nx[e], ny[e]) - orc.face_flux(0, qm, qq[c1], nx[e], ny[e])) / (2 * hh)
             assert np.allclose(J[:4, i], d * L[e], rtol=1e-4, atol=1e-6 * L[e])
             assert np.array_equal(J[4:, i], -J[:4, i])
+
+
+def test_matrix_free_operator_is_block_diagonal_own_state_derivative():
+    """RansMatrix * x (ransMatrix.h:150-178): a row sees only its own block of
+    x (the neighbour's perturbation enters the other row only), starts from
+    x A / dt, and adds the derivative of each face flux along x_c without the
+    face length: +dF/dq_L x on the owner row, -dF/dq_R x on the neighbour row.
+    Checked against central differences of the face flux on an interior cell,
+    and ghost rows against the identity."""
+    m = orc.OMesh.read(TRI)
+    o = solver_on(m, cfl=3.0)
+    o.init()
+    o.refill_bcs()
+    cnt = m.counts()
+    nt, nr = cnt["cells"], cnt["real_cells"]
+    rng = np.random.default_rng(5)
+    q = o.vec("q")
+    q[: 4 * nr] *= 1 + 0.05 * rng.standard_normal(4 * nr)
+    o.vec("q", q)
+    o.calc_dt()
+    dt = o.vec("dt")
+    area = m.get("cellsAreas")
+    ec = m.get("edgesCells").reshape(-1, 2)
+    nx, ny = m.get("edgesNormalsX"), m.get("edgesNormalsY")
+    qq = q.reshape(-1, 4)
+    ghost_faces = set(np.nonzero(ec[:, 1] >= nr)[0])
+    interior = [c for c in range(nr) if not any(f in ghost_faces for f in np.nonzero((ec == c).any(axis=1))[0])]
+    for c in interior[:: max(1, len(interior) // 6)][:6]:
+        xc = rng.standard_normal(4)
+        x = np.zeros(4 * nt)
+        x[4 * c: 4 * c + 4] = xc
+        y = o.mf_apply(x).reshape(-1, 4)
+        others = np.delete(y, c, axis=0)
+        assert np.array_equal(others, np.zeros_like(others))
+        expect = xc * area[c] / dt[c]
+        eps = 1e-5
+        for f in np.nonzero((ec == c).any(axis=1))[0]:
+            a, b = ec[f]
+            if a == c:
+                d = (orc.face_flux(0, qq[a] + eps * xc, qq[b], nx[f], ny[f])
+                     - orc.face_flux(0, qq[a] - eps * xc, qq[b], nx[f], ny[f])) / (2 * eps)
+                expect = expect + d
+            else:
+                d = (orc.face_flux(0, qq[a], qq[b] + eps * xc, nx[f], ny[f])
+                     - orc.face_flux(0, qq[a], qq[b] - eps * xc, nx[f], ny[f])) / (2 * eps)
+                expect = expect - d
+        assert np.allclose(y[c], expect, rtol=1e-4, atol=1e-6 * np.max(np.abs(expect)))
+    # ghost rows are identity rows and receive nothing from one-sided faces
+    x = np.zeros(4 * nt)
+    x[4 * nr:] = rng.standard_normal(4 * (nt - nr))
+    y = o.mf_apply(x)
+    assert np.array_equal(y[4 * nr:], x[4 * nr:])
+    assert np.array_equal(y[: 4 * nr], np.zeros(4 * nr))
+
+
+def test_matrix_free_operator_ignores_viscosity():
+    """The operator calls the two-argument flux (zero gradients), so the
+    laminar stress vanishes and the laminar operator equals the inviscid one."""
+    m = orc.OMesh.read(QUAD)
+    ops = []
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(4 * m.counts()["cells"])
+    for visc in (0, 1):
+        o = orc.OSolver(m, mu_L=7e-4, viscous=visc)
+        o.set_bcs({"farfield": ("farfield", (0.5, 0.02, 1.0, 1.0)), "wall": ("wall", (0.2, 0.0, 1.0, 1.0))})
+        o.options(cfl=2.0)
+        o.init()
+        o.refill_bcs()
+        q = o.vec("q")
+        nr = 4 * m.counts()["real_cells"]
+        q[:nr] *= 1 + 0.05 * np.random.default_rng(3).standard_normal(nr)
+        o.vec("q", q)
+        o.calc_dt()
+        ops.append(o.mf_apply(x))
+    assert np.array_equal(ops[0], ops[1])
