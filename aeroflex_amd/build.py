@@ -24,6 +24,7 @@ UNITS = [
     ("afx_k32.hip", ["-ffp-contract=fast"]),    # FP32 throughput form
     ("afx_implicit.hip", ["-ffp-contract=off"]),
     ("afx_adjoint.hip", ["-ffp-contract=off"]),
+    ("afx_mapper.hip", ["-ffp-contract=off"]),   # multigrid level prolongation
     ("afx_capi.hip", ["-ffp-contract=off"]),
     ("afx_vlm.hip", ["-ffp-contract=fast"]),    # vortex lattice, FP32 LU, FP64 refinement
 ]

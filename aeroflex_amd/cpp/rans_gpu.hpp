@@ -10,6 +10,8 @@
 //   rans::implicitSolver (solver.h:852-970)     rans::gpu::implicitSolver
 //   rans::explicitSolver (solver.h:721-742)     rans::gpu::explicitSolver
 //   multigrid<T>::run_solver (multigrid.h:183-293) rans::gpu::run_solver
+//   multigrid<T>::gen_mapper (multigrid.h:96-163)  rans::gpu::mapper
+//   multigrid<T>::run (multigrid.h:296-363)        rans::gpu::multigrid_run
 //   get_wall_profile (post.h:301-387)           rans::gpu::get_wall_profile
 //
 // Every call goes through the C ABI in include/afx_rans.h; errors come back
@@ -272,6 +274,53 @@ inline int run_solver(solver& s, const Settings& st, std::vector<double>& residu
         residuals.push_back(err);
     } while ((err > st.tolerance) && (i < st.max_iterations) && !(stop && stop()));
     return 0;
+}
+
+// multigrid<T>::gen_mapper (multigrid.h:96-163), built on the device: the
+// inverse-distance map from a coarse solver's cells to a fine one's.  The two
+// solvers must outlive the mapper.
+class mapper {
+    afx_mapper* h_ = nullptr;
+    solver* coarse_ = nullptr;
+    solver* fine_ = nullptr;
+
+public:
+    mapper(solver& coarse, solver& fine) : coarse_(&coarse), fine_(&fine) {
+        check(afx_mapper_create(coarse.handle(), fine.handle(), &h_));
+    }
+    mapper(const mapper&) = delete;
+    mapper& operator=(const mapper&) = delete;
+    mapper(mapper&& o) noexcept : h_(o.h_), coarse_(o.coarse_), fine_(o.fine_) { o.h_ = nullptr; }
+    ~mapper() { if (h_) afx_mapper_destroy(h_); }
+    // solvers[i].get_q() = mappers[i-1] * solvers[i-1].get_q() (multigrid.h:334)
+    void prolongate() { check(afx_mapper_prolongate(h_, coarse_->handle(), fine_->handle())); }
+};
+
+// multigrid<T>::run (multigrid.h:296-363) over levels listed coarse to fine,
+// each constructed with its boundary conditions and options as the
+// multigrid constructor does (multigrid.h:70-77): level 0 from freestream
+// (reinit) and its boundary states, then each finer level from the mapped
+// solution of the previous one.  Returns the level the loop stopped on;
+// state receives run_solver's result for it.
+template <class S>
+S& multigrid_run(std::vector<S*>& levels, const Settings& st, std::vector<double>& residuals, int* state = nullptr,
+                 bool reinit = true, const std::function<bool()>& stop = nullptr) {
+    std::vector<mapper> mappers;
+    mappers.reserve(levels.size());
+    for (size_t i = 0; i + 1 < levels.size(); ++i) mappers.emplace_back(*levels[i], *levels[i + 1]);
+    if (reinit) levels[0]->init();
+    levels[0]->refill_bcs();
+    for (size_t i = 0; i < levels.size(); ++i) {
+        if (i > 0) {
+            levels[i - 1]->bcs_from_internal();
+            mappers[i - 1].prolongate();
+            levels[i]->refill_bcs();
+        }
+        int rc = run_solver(*levels[i], st, residuals, stop);
+        if (state) *state = rc;
+        if (rc || (stop && stop())) return *levels[i];
+    }
+    return *levels.back();
 }
 
 }  // namespace gpu

@@ -23,6 +23,7 @@
 #include "afx_tiles.hpp"
 #include "afx_implicit.hpp"
 #include "afx_adjoint_kernels.hpp"
+#include "afx_mapper.hpp"
 
 using namespace afx;
 
@@ -2106,6 +2107,103 @@ int afx_rans_destroy(afx_rans* h) {
     if (!h) return AFX_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     delete h;
+    return AFX_OK;
+}
+
+// ------------------------------------------------- multigrid prolongation --
+struct afx_mapper {
+    const afx_rans* coarse = nullptr;
+    const afx_rans* fine = nullptr;
+    int32_t nc = 0, nf = 0;
+    int64_t nnz = 0;
+    DBuf<int64_t> rowptr;           // reference fine order, nf + 1
+    DBuf<int32_t> cols, cols_dev;   // coarse reference index / coarse device index
+    DBuf<double> vals;
+    DBuf<int32_t> fnew2old;         // fine device cell -> reference cell
+};
+
+int afx_mapper_create(const afx_rans* coarse, const afx_rans* fine, afx_mapper** out) {
+    if (!out) return fail(AFX_EARG, "null output");
+    *out = nullptr;
+    if (!coarse || !fine) return fail(AFX_EARG, "null solver handle");
+    if (coarse->nv != fine->nv) return fail(AFX_EARG, "the two levels carry different numbers of variables");
+    auto* mp = new afx_mapper();
+    auto bail = [&](int rc) { delete mp; return rc; };
+#define HIPCHK_M(x)                                                                                   \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess) return bail(fail(AFX_EHIP, std::string(#x) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+    mp->coarse = coarse;
+    mp->fine = fine;
+    mp->nc = coarse->nT;
+    mp->nf = fine->nT;
+    hipStream_t s = fine->stream;
+    // centroids and areas in reference order, ghosts included (multigrid.h:101-102
+    // loop over cellsAreas.size())
+    const Mesh& mc = coarse->m;
+    const Mesh& mf = fine->m;
+    DBuf<double> cx, cy, ca, fx, fy, box;
+    DBuf<int32_t> cnt, o2n;
+    const int32_t ntiles = (mp->nc + 255) / 256;
+    HIPCHK_M(cx.alloc(mp->nc)); HIPCHK_M(cx.upload(mc.cellsCentersX.data(), mp->nc, s));
+    HIPCHK_M(cy.alloc(mp->nc)); HIPCHK_M(cy.upload(mc.cellsCentersY.data(), mp->nc, s));
+    HIPCHK_M(ca.alloc(mp->nc)); HIPCHK_M(ca.upload(mc.cellsAreas.data(), mp->nc, s));
+    HIPCHK_M(fx.alloc(mp->nf)); HIPCHK_M(fx.upload(mf.cellsCentersX.data(), mp->nf, s));
+    HIPCHK_M(fy.alloc(mp->nf)); HIPCHK_M(fy.upload(mf.cellsCentersY.data(), mp->nf, s));
+    HIPCHK_M(box.alloc((size_t)4 * ntiles));
+    HIPCHK_M(cnt.alloc(mp->nf));
+    HIPCHK_M(launch_map_tile_boxes(s, cx.p, cy.p, ca.p, mp->nc, box.p));
+    HIPCHK_M(launch_map_count(s, fx.p, fy.p, mp->nf, cx.p, cy.p, ca.p, mp->nc, box.p, cnt.p));
+    std::vector<int32_t> hc(mp->nf);
+    HIPCHK_M(hipMemcpyAsync(hc.data(), cnt.p, (size_t)mp->nf * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK_M(hipStreamSynchronize(s));
+    std::vector<int64_t> rp((size_t)mp->nf + 1, 0);
+    for (int32_t i = 0; i < mp->nf; ++i) rp[i + 1] = rp[i] + hc[i];
+    mp->nnz = rp.back();
+    HIPCHK_M(mp->rowptr.alloc(rp.size())); HIPCHK_M(mp->rowptr.upload(rp, s));
+    HIPCHK_M(mp->cols.alloc(mp->nnz));
+    HIPCHK_M(mp->cols_dev.alloc(mp->nnz));
+    HIPCHK_M(mp->vals.alloc(mp->nnz));
+    HIPCHK_M(launch_map_fill(s, fx.p, fy.p, mp->nf, cx.p, cy.p, ca.p, mp->nc, box.p, mp->rowptr.p, mp->cols.p,
+                             mp->vals.p));
+    HIPCHK_M(o2n.alloc(mp->nc)); HIPCHK_M(o2n.upload(coarse->c_old2new, s));
+    HIPCHK_M(launch_map_cols(s, mp->cols.p, o2n.p, mp->nnz, mp->cols_dev.p));
+    HIPCHK_M(mp->fnew2old.alloc(mp->nf)); HIPCHK_M(mp->fnew2old.upload(fine->c_new2old, s));
+    HIPCHK_M(hipStreamSynchronize(s));
+#undef HIPCHK_M
+    *out = mp;
+    return AFX_OK;
+}
+
+int afx_mapper_destroy(afx_mapper* mp) {
+    if (!mp) return AFX_OK;
+    if (mp->fine) (void)hipStreamSynchronize(mp->fine->stream);
+    delete mp;
+    return AFX_OK;
+}
+
+int afx_mapper_get(const afx_mapper* mp, int64_t* nnz, int64_t* row_ptr, int32_t* cols, double* weights) {
+    if (!mp) return fail(AFX_EARG, "null mapper");
+    if (nnz) *nnz = mp->nnz;
+    hipStream_t s = mp->fine->stream;
+    if (row_ptr) HIPCHK(hipMemcpyAsync(row_ptr, mp->rowptr.p, ((size_t)mp->nf + 1) * 8, hipMemcpyDeviceToHost, s));
+    if (cols) HIPCHK(hipMemcpyAsync(cols, mp->cols.p, (size_t)mp->nnz * 4, hipMemcpyDeviceToHost, s));
+    if (weights) HIPCHK(hipMemcpyAsync(weights, mp->vals.p, (size_t)mp->nnz * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return AFX_OK;
+}
+
+int afx_mapper_prolongate(const afx_mapper* mp, afx_rans* coarse, afx_rans* fine) {
+    if (!mp) return fail(AFX_EARG, "null mapper");
+    if (coarse != mp->coarse || fine != mp->fine)
+        return fail(AFX_EARG, "the mapper was generated for another pair of solvers");
+    // the coarse state must be complete before the fine stream reads it
+    HIPCHK(hipStreamSynchronize(coarse->stream));
+    HIPCHK(launch_map_prolong(fine->stream, mp->rowptr.p, mp->fnew2old.p, mp->cols_dev.p, mp->vals.p, coarse->q.p,
+                              coarse->nT, fine->q.p, fine->nT, fine->nv));
+    HIPCHK(hipStreamSynchronize(fine->stream));
+    fine->q32_valid = false;
     return AFX_OK;
 }
 

@@ -91,6 +91,10 @@ def _load():
         "afx_rans_implicit_solve": (C.c_int, [P, f64, f64, i32, i32, dp]),
         "afx_rans_lhs_apply": (C.c_int, [P, dp, dp]),
         "afx_rans_set_matrix_free": (C.c_int, [P, C.c_int32]),
+        "afx_mapper_create": (C.c_int, [P, P, C.POINTER(P)]),
+        "afx_mapper_destroy": (C.c_int, [P]),
+        "afx_mapper_get": (C.c_int, [P, C.POINTER(C.c_int64), P, P, P]),
+        "afx_mapper_prolongate": (C.c_int, [P, P, P]),
         "afx_rans_matrix_free_apply": (C.c_int, [P, dp, dp]),
         "afx_rans_linear_iterations": (C.c_int, [P, C.POINTER(i64)]),
         "afx_rans_wall_forces": (C.c_int, [P, C.c_char_p, dp, dp, dp]),
@@ -610,6 +614,61 @@ class Settings:
         self.relaxation, self.tolerance = relaxation, tolerance
         self.rhs_iterations, self.max_iterations = rhs_iterations, max_iterations
         self.airfoil_name = airfoil_name
+
+
+class Mapper:
+    """multigrid<T>::gen_mapper (multigrid.h:96-163), built on the device:
+    the inverse-distance map from the coarse solver's cells to the fine
+    solver's cells.  Both solvers must outlive it."""
+
+    def __init__(self, coarse, fine):
+        self.h = None
+        self.coarse, self.fine = coarse, fine
+        h = C.c_void_p()
+        _check(lib.afx_mapper_create(coarse.h, fine.h, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and lib is not None:
+            lib.afx_mapper_destroy(self.h)
+            self.h = None
+
+    def csr(self):
+        """(row_ptr, cols, weights) in reference cell order."""
+        nnz = C.c_int64()
+        _check(lib.afx_mapper_get(self.h, C.byref(nnz), None, None, None))
+        rp = np.empty(self.fine.n_cells + 1, np.int64)
+        cols = np.empty(nnz.value, np.int32)
+        w = np.empty(nnz.value)
+        _check(lib.afx_mapper_get(self.h, C.byref(nnz), rp.ctypes.data, cols.ctypes.data, w.ctypes.data))
+        return rp, cols, w
+
+    def prolongate(self):
+        """q_fine = M q_coarse (multigrid.h:334), every variable."""
+        _check(lib.afx_mapper_prolongate(self.h, self.coarse.h, self.fine.h))
+
+
+def multigrid_run(solvers, settings, reinit=True, mappers=None, residuals=None, stop=None):
+    """multigrid<implicitSolver>::run (multigrid.h:317-363): level 0 from
+    freestream (reinit) and its boundary states, then for each finer level
+    the coarse solution mapped over (bcs_from_internal, q_fine = M q_coarse,
+    refill_bcs) and solved by run_solver.  Returns (solver, state) of the
+    last level run; state 1 when a level's solver failed."""
+    if mappers is None:
+        mappers = [Mapper(solvers[i], solvers[i + 1]) for i in range(len(solvers) - 1)]
+    residuals = [] if residuals is None else residuals
+    if reinit:
+        solvers[0].init()
+    solvers[0].refill_bcs()
+    for i, s in enumerate(solvers):
+        if i > 0:
+            solvers[i - 1].bcs_from_internal()
+            mappers[i - 1].prolongate()
+            s.refill_bcs()
+        state, _ = run_solver(s, settings, residuals, stop)
+        if state or (stop and stop()):
+            return s, state
+    return solvers[-1], 0
 
 
 def run_solver(s, settings, residuals=None, stop=None):

@@ -45,6 +45,7 @@ extern "C" {
 typedef struct afx_mesh afx_mesh;
 typedef struct afx_rans afx_rans;
 typedef struct afx_batch afx_batch;
+typedef struct afx_mapper afx_mapper;
 
 /* core.h:27-46 (rans::gas; cp is fixed to 1 as in the reference) */
 typedef struct afx_gas {
@@ -275,6 +276,24 @@ int afx_batch_create(int32_t n, afx_rans* const* members, double relaxation, afx
 int afx_batch_run(afx_batch* b, int32_t steps, int32_t graph_steps);
 int afx_batch_residual_norms(afx_batch* b, int32_t precision, double* norms);
 int afx_batch_stream(afx_batch* b, void** stream);
+
+/* ------------------------------------------------ multigrid prolongation --
+ * multigrid<T>::gen_mapper (multigrid.h:96-163): the sparse map from a coarse
+ * solver's state to a fine one's.  Fine cell i (ghosts included) takes every
+ * coarse cell j (ghosts included) whose centroid distance satisfies
+ * d2 < 2 A_j, weighted s_ij / sum_j s_ij with s_ij = 1 / max(0.1 sqrt(A_j),
+ * sqrt(d2)); a fine cell that takes none maps to zero, as in the reference.
+ * Built on the device from the two handles' meshes (the handles must outlive
+ * the mapper).  get: the map in reference cell order as CSR, row_ptr
+ * (n_fine_cells + 1), cols (coarse cell), weights; each output nullable, nnz
+ * first to size them.  prolongate: q_fine = M q_coarse for every variable the
+ * handles carry (multigrid.h:334); the caller runs bcs_from_internal on the
+ * coarse solver before and refill_bcs on the fine one after, as
+ * multigrid<T>::run does (multigrid.h:330-336). */
+int afx_mapper_create(const afx_rans* coarse, const afx_rans* fine, afx_mapper** out);
+int afx_mapper_destroy(afx_mapper* m);
+int afx_mapper_get(const afx_mapper* m, int64_t* nnz, int64_t* row_ptr, int32_t* cols, double* weights);
+int afx_mapper_prolongate(const afx_mapper* m, afx_rans* coarse, afx_rans* fine);
 int afx_batch_destroy(afx_batch* b);
 /* The handle's hipStream_t (as void*), for event timing by the caller. */
 int afx_rans_stream(afx_rans* h, void** stream);

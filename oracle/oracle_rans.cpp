@@ -1535,6 +1535,48 @@ void orc_solver_wall_profile(void* h, const char* patch, double* out3) {
 void orc_solver_face_jacobian(void* h, int e, double* J) { ((Solver*)h)->face_jacobian((u32)e, J); }
 void orc_solver_mf_apply(void* h, const double* x, double* y) { ((Solver*)h)->mf_apply(x, y); }
 
+// multigrid<T>::gen_mapper (multigrid.h:96-163) on centroid / area arrays in
+// reference order (ghosts included): first the row scales over all coarse
+// cells in order, then the weights s_ij / scales_i.  CSR out; call with null
+// buffers to get nnz.
+static thread_local std::vector<int64_t> g_mrow;
+static thread_local std::vector<int32_t> g_mcol;
+static thread_local std::vector<double> g_mval;
+long orc_gen_mapper(int nf, const double* fx, const double* fy, int nc, const double* cx, const double* cy,
+                    const double* ca, int64_t* row_ptr, int32_t* cols, double* vals) {
+    if (row_ptr) {
+        std::memcpy(row_ptr, g_mrow.data(), g_mrow.size() * 8);
+        std::memcpy(cols, g_mcol.data(), g_mcol.size() * 4);
+        std::memcpy(vals, g_mval.data(), g_mval.size() * 8);
+        return (long)g_mval.size();
+    }
+    std::vector<double> scales(nf);
+    for (int i = 0; i < nf; ++i) {
+        const double xi = fx[i], yi = fy[i];
+        scales[i] = 0;
+        for (int j = 0; j < nc; ++j) {
+            const double d2 = (xi - cx[j]) * (xi - cx[j]) + (yi - cy[j]) * (yi - cy[j]);
+            const double r2 = ca[j];
+            if (d2 < 2 * r2) scales[i] += 1 / std::max(0.1 * std::sqrt(r2), std::sqrt(d2));
+        }
+    }
+    g_mrow.assign(1, 0); g_mcol.clear(); g_mval.clear();
+    for (int i = 0; i < nf; ++i) {
+        const double xi = fx[i], yi = fy[i];
+        for (int j = 0; j < nc; ++j) {
+            const double d2 = (xi - cx[j]) * (xi - cx[j]) + (yi - cy[j]) * (yi - cy[j]);
+            const double r2 = ca[j];
+            if (d2 < 2 * r2) {
+                const double si = 1 / std::max(0.1 * std::sqrt(r2), std::sqrt(d2));
+                g_mcol.push_back(j);
+                g_mval.push_back(si / scales[i]);
+            }
+        }
+        g_mrow.push_back((int64_t)g_mval.size());
+    }
+    return (long)g_mval.size();
+}
+
 // fillRhoLHS triplets: returns the count; call with null buffers to size
 static thread_local std::vector<int64_t> g_rows, g_cols;
 static thread_local std::vector<double> g_vals;

@@ -44,6 +44,7 @@ def _load():
         "orc_solver_wall_profile": (None, [P, C.c_char_p, dp]),
         "orc_solver_face_jacobian": (None, [P, C.c_int, dp]),
         "orc_solver_mf_apply": (None, [P, dp, dp]),
+        "orc_gen_mapper": (C.c_long, [C.c_int, dp, dp, C.c_int, dp, dp, dp, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_solver_fill_lhs": (C.c_long, [P, P, P, P]),
         "orc_solver_wall_dist": (None, [P]),
         "orc_solver_vec": (C.c_long, [P, C.c_char_p, dp, dp]),
@@ -252,3 +253,19 @@ def freestream(mach, angle, T=1.0, p=1.0, gamma=1.4, R=0.71428571428):
     out = np.zeros(4)
     lib.orc_freestream(gamma, R, mach, angle, T, p, _d(out))
     return out
+
+
+def gen_mapper(fx, fy, cx, cy, ca):
+    """multigrid::gen_mapper (multigrid.h:96-163) as a scipy CSR matrix
+    (n_fine x n_coarse, one weight per cell pair; the reference repeats it
+    on the 4 variables)."""
+    import scipy.sparse as sp
+    fx, fy, cx, cy, ca = (np.ascontiguousarray(a, np.float64) for a in (fx, fy, cx, cy, ca))
+    nnz = lib.orc_gen_mapper(len(fx), _d(fx), _d(fy), len(cx), _d(cx), _d(cy), _d(ca), None, None, None)
+    rp = np.empty(len(fx) + 1, np.int64)
+    cols = np.empty(nnz, np.int32)
+    vals = np.empty(nnz)
+    lib.orc_gen_mapper(len(fx), _d(fx), _d(fy), len(cx), _d(cx), _d(cy), _d(ca), rp.ctypes.data, cols.ctypes.data,
+                       vals.ctypes.data)
+    return sp.csr_matrix((vals, cols, rp), shape=(len(fx), len(cx)))
+

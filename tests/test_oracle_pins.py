@@ -620,3 +620,45 @@ def test_matrix_free_operator_ignores_viscosity():
         o.calc_dt()
         ops.append(o.mf_apply(x))
     assert np.array_equal(ops[0], ops[1])
+
+
+def test_gen_mapper_hand_computed_weights():
+    """multigrid::gen_mapper (multigrid.h:96-163) on three coarse cells, each
+    row worked by hand: a coarse cell j is taken when d2 < 2 A_j (strict), its
+    weight is 1/max(0.1 sqrt(A_j), sqrt(d2)) over the row's sum; a fine cell
+    out of reach of every coarse cell gets an empty row."""
+    cx, cy, ca = [0.0, 1.0, 5.0], [0.0, 0.0, 5.0], [1.0, 1.0, 0.01]
+    fx = [0.5, 0.0, 5.1, 3.0, 1.0]
+    fy = [0.0, 0.0, 5.0, 3.0, 1.0]
+    M = orc.gen_mapper(fx, fy, cx, cy, ca).toarray()
+    expect = np.array([
+        [0.5, 0.5, 0.0],            # d = 0.5 to both unit cells: 1/0.5 each
+        [10 / 11, 1 / 11, 0.0],     # d = 0 -> 1/(0.1*1) = 10; d = 1 -> 1
+        [0.0, 0.0, 1.0],            # only the small cell (d2 = 0.01 < 0.02)
+        [0.0, 0.0, 0.0],            # nothing within reach
+        [0.0, 1.0, 0.0],            # d2 = 2 to cell 0 is not < 2; d2 = 1 to cell 1
+    ])
+    assert np.allclose(M, expect, rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("coarse,fine", [(TRI, QUAD), (QUAD, TRI)])
+def test_gen_mapper_rows_are_partitions_of_unity(coarse, fine):
+    """Between the reference's two coarse airfoil meshes every row's weights
+    are positive and sum to one, so a uniform state maps to itself.  The
+    triangle mesh's farfield is a circle of 5 chords about mid-chord, the quad mesh's much further
+    out: fine cells beyond the coarse domain get empty rows (state 0, as in the
+    reference), all others are covered."""
+    mc, mf = orc.OMesh.read(coarse), orc.OMesh.read(fine)
+    M = orc.gen_mapper(mf.get("cellsCentersX"), mf.get("cellsCentersY"), mc.get("cellsCentersX"),
+                       mc.get("cellsCentersY"), mc.get("cellsAreas"))
+    assert np.all(M.data > 0)
+    sums = np.asarray(M.sum(axis=1)).ravel()
+    nonempty = np.diff(M.indptr) > 0
+    assert np.allclose(sums[nonempty], 1.0, rtol=0, atol=4e-15)
+    # distances from mid-chord, the centre of the triangle mesh's farfield circle
+    rf = np.hypot(mf.get("cellsCentersX") - 0.5, mf.get("cellsCentersY"))
+    nrc = mc.counts()["real_cells"]
+    rc = np.hypot(mc.get("cellsCentersX")[:nrc] - 0.5, mc.get("cellsCentersY")[:nrc])
+    inside = rf < 0.98 * rc.max()
+    assert nonempty[inside].all()
+    assert not nonempty[rf > 1.1 * rc.max()].any()
