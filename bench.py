@@ -563,6 +563,59 @@ def aeroelastic_bench(n_chord=64, n_span=256, n_cases=32, alpha_max=8.0, cpu=Tru
     return out
 
 
+def multigrid_setup_bench(rans, coarse_cells=500_000, fine_cells=2_000_000, cpu=True, cpu_rows=1000):
+    """Full-multigrid set-up at scale: multigrid::gen_mapper (multigrid.h:96-163)
+    between a 500k-cell and a 2M-cell jittered mesh of the benchmark generator
+    (device build, wall clock around the synchronous call), and one
+    prolongation q_fine = M q_coarse.  CPU leg: the oracle's restatement
+    (all N_fine x N_coarse pairs, one core) on the first cpu_rows fine cells,
+    scaled to all of them."""
+    def mesh(n, seed):
+        ni, nj = mesh_dims(n)
+        return rans.Mesh.jittered(ni, nj, r_far=20.0, thickness=0.12, first_h=1e-4, jitter=0.1, quad_frac=0.25,
+                                  seed=seed)
+
+    def solver(m):
+        s = rans.Solver(m, viscosity_model="inviscid")
+        far = rans.BoundaryVariables(mach=MACH, angle=np.deg2rad(ALPHA_DEG))
+        s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far),
+                   "wall": rans.BoundaryCondition("slip-wall")})
+        s.init()
+        return s
+
+    mc, mf = mesh(coarse_cells, 1), mesh(fine_cells, 5)
+    sc, sf = solver(mc), solver(mf)
+    rans.Mapper(sc, sf)    # first build: module load and allocator warm-up
+    t0 = time.perf_counter()
+    mp = rans.Mapper(sc, sf)
+    t_build = time.perf_counter() - t0
+    rp, cols, w = mp.csr()
+    mp.prolongate()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        mp.prolongate()
+    t_prol = (time.perf_counter() - t0) / 10
+    out = {"coarse_cells": sc.n_cells, "fine_cells": sf.n_cells, "nnz": int(len(w)),
+           "empty_rows": int(np.sum(np.diff(rp) == 0)), "gen_mapper_ms": round(t_build * 1e3, 2),
+           "prolongate_ms": round(t_prol * 1e3, 3)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import orc
+        fx, fy = mf.get("cellsCentersX"), mf.get("cellsCentersY")
+        cx, cy, ca = mc.get("cellsCentersX"), mc.get("cellsCentersY"), mc.get("cellsAreas")
+        t0 = time.perf_counter()
+        ref = orc.gen_mapper(fx[:cpu_rows], fy[:cpu_rows], cx, cy, ca)
+        t_cpu = time.perf_counter() - t0
+        k = int(rp[cpu_rows])
+        out["vs_oracle_rows"] = {"rows": cpu_rows, "cols_equal": bool(np.array_equal(cols[:k], ref.indices)),
+                                 "max_weight_diff": float(np.max(np.abs(w[:k] - ref.data))) if k else 0.0}
+        out["cpu_baseline"] = {"kind": "port", "cores": 1,
+                               "sample": f"oracle gen_mapper on {cpu_rows} fine rows x all coarse cells, "
+                                         f"{t_cpu:.1f} s, scaled by fine rows",
+                               "gen_mapper_s": round(t_cpu * sf.n_cells / cpu_rows, 1)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -584,6 +637,9 @@ def main():
                     help="skip the vortex-lattice / beam coupling section")
     ap.add_argument("--only-aeroelastic", action="store_true",
                     help="run only the vortex-lattice / beam coupling section (profiling runs)")
+    ap.add_argument("--no-multigrid", dest="multigrid", action="store_false",
+                    help="skip the multigrid set-up section (level map at 500k -> 2M cells)")
+    ap.add_argument("--only-multigrid", action="store_true", help="run only the multigrid set-up section")
     ap.add_argument("--only-adjoint", action="store_true",
                     help="run only the adjoint section (state, node coordinates, dCL/dalpha)")
     args = ap.parse_args()
@@ -594,6 +650,10 @@ def main():
 
     if args.only_aeroelastic:
         print(json.dumps({"aeroelastic": aeroelastic_bench(cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.only_multigrid:
+        from aeroflex_amd import rans
+        print(json.dumps({"multigrid_setup": multigrid_setup_bench(rans, cpu=not args.no_cpu)}), flush=True)
         return
     if args.only_adjoint:
         from aeroflex_amd import rans
@@ -693,6 +753,8 @@ def main():
         out["batched_sweep"] = batched_sweep(rans, args.sweep_steps)
     if args.aeroelastic:
         out["aeroelastic"] = aeroelastic_bench(cpu=not args.no_cpu)
+    if args.multigrid:
+        out["multigrid_setup"] = multigrid_setup_bench(rans, cpu=not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(rans, args.model)
     else:
