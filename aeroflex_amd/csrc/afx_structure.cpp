@@ -79,7 +79,11 @@ bool frame_axes(const V3& a, const V3& b, const V3& v, double lam[3][3], double&
     return true;
 }
 
-int beam_assemble(afx_beam* b) {
+// Assemble the banded global stiffness and factor it into Lout.  rot (nullable):
+// per element the 3 x 3 rotation from element to global axes (row-major), as
+// MODEL::set_K_global takes it from the element's current mid-node quaternion
+// (model.hpp:103-131); null: the initial axes, Lambda^T.
+int beam_assemble(const afx_beam* b, const std::vector<std::array<double, 9>>* rot, std::vector<double>& Lout) {
     const int nd = 6 * b->n, w = b->bw + 1;
     std::vector<double> K((size_t)nd * w, 0.0);   // lower band, same indexing as L
     auto addK = [&](int i, int j, double v) {
@@ -93,6 +97,9 @@ int beam_assemble(afx_beam* b) {
         if (!frame_axes(b->x[n1], b->x[n2], {p[4], p[5], p[6]}, lam, L))
             return api_fail(AFX_EARG, "beam element " + std::to_string(e) +
                                           ": zero length or orientation vector along the element");
+        if (rot)
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) lam[r][c] = (*rot)[e][3 * c + r];   // global -> element = R^T
         double Kl[12][12], T[12][12] = {}, Kg[12][12];
         frame_local(L, p[0], p[1], p[2], p[3], Kl);
         for (int blk = 0; blk < 4; ++blk)
@@ -125,8 +132,8 @@ int beam_assemble(afx_beam* b) {
             if (fi || fj) K[(size_t)i * w + (i - j)] = (i == j) ? 1.0 : 0.0;
         }
     // banded Cholesky K = L L^T
-    b->L.assign(K.size(), 0.0);
-    auto Lr = [&](int i, int j) -> double& { return b->L[(size_t)i * w + (i - j)]; };
+    Lout.assign(K.size(), 0.0);
+    auto Lr = [&](int i, int j) -> double& { return Lout[(size_t)i * w + (i - j)]; };
     for (int i = 0; i < nd; ++i) {
         const int j0 = std::max(0, i - b->bw);
         for (int j = j0; j <= i; ++j) {
@@ -145,9 +152,9 @@ int beam_assemble(afx_beam* b) {
     return AFX_OK;
 }
 
-void beam_solve_impl(const afx_beam* b, const double* f, double* u) {
+void beam_solve_with(const afx_beam* b, const std::vector<double>& Lf, const double* f, double* u) {
     const int nd = 6 * b->n, w = b->bw + 1;
-    auto L = [&](int i, int j) { return b->L[(size_t)i * w + (i - j)]; };
+    auto L = [&](int i, int j) { return Lf[(size_t)i * w + (i - j)]; };
     std::vector<double> y(nd);
     for (int i = 0; i < nd; ++i) {
         double s = b->fixed[i / 6] ? 0.0 : f[i];   // apply_SPC1_Forces
@@ -160,6 +167,8 @@ void beam_solve_impl(const afx_beam* b, const double* f, double* u) {
         u[i] = s / L(i, i);
     }
 }
+
+void beam_solve_impl(const afx_beam* b, const double* f, double* u) { beam_solve_with(b, b->L, f, u); }
 
 }  // namespace
 
@@ -309,7 +318,7 @@ int afx_beam_create(int32_t n_nodes, const double* xyz, int32_t n_elems, const i
             }
             b->fixed[clamped[k]] = 1;
         }
-        int rc = beam_assemble(b);
+        int rc = beam_assemble(b, nullptr, b->L);
         if (rc) { delete b; return rc; }
         *out = b;
         return AFX_OK;
@@ -326,6 +335,215 @@ int afx_beam_destroy(afx_beam* b) {
 int afx_beam_solve(afx_beam* b, const double* f, double* u) {
     if (!b || !f || !u) return api_fail(AFX_EARG, "null argument");
     beam_solve_impl(b, f, u);
+    return AFX_OK;
+}
+
+// ------------------------------------------------- nonlinear beam ----
+// MODEL::get_NonLin_Solve (model.hpp:240-339) with the corotational CBAR of
+// element.hpp:111-370.  Quaternions are (w, x, y, z) with Eigen's Hamilton
+// product; rotation matrices and the conversions are those of the element.
+namespace {
+struct Quat { double w, x, y, z; };
+
+Quat qmul(const Quat& a, const Quat& b) {
+    return {a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+            a.w * b.y + a.y * b.w + a.z * b.x - a.x * b.z, a.w * b.z + a.z * b.w + a.x * b.y - a.y * b.x};
+}
+Quat qconj(const Quat& a) { return {a.w, -a.x, -a.y, -a.z}; }
+
+// get_Rotation_Matrix_From_Quaternion (element.hpp:224-237), row-major
+std::array<double, 9> qmat(const Quat& q) {
+    const double s = q.w, x = q.x, y = q.y, z = q.z;
+    return {1. - 2. * (y * y + z * z), 2. * (x * y - s * z), 2. * (x * z + s * y),
+            2. * (x * y + s * z), 1. - 2. * (x * x + z * z), 2. * (y * z - s * x),
+            2. * (x * z - s * y), 2. * (y * z + s * x), 1. - 2. * (x * x + y * y)};
+}
+
+// set_qe_From_Rotation_Matrix (element.hpp:159-211) of the element axes Lambda
+Quat q_from_axes(const double m[3][3]) {
+    const double tr = m[0][0] + m[1][1] + m[2][2];
+    double c, s, vx, vy, vz;
+    if (tr > 0.0) {
+        c = 0.5 / std::sqrt(tr + 1.0);
+        s = 0.25 / c;
+        vx = (m[1][2] - m[2][1]) * c; vy = (m[2][0] - m[0][2]) * c; vz = (m[0][1] - m[1][0]) * c;
+    } else if (m[0][0] > m[1][1] && m[0][0] > m[2][2]) {
+        c = 2.0 * std::sqrt(1.0 + m[0][0] - m[1][1] - m[2][2]);
+        s = (m[1][2] - m[2][1]) / c; vx = 0.25 * c; vy = (m[1][0] + m[0][1]) / c; vz = (m[2][0] + m[0][2]) / c;
+    } else if (m[1][1] > m[2][2]) {
+        c = 2.0 * std::sqrt(1.0 + m[1][1] - m[0][0] - m[2][2]);
+        s = (m[2][0] - m[0][2]) / c; vx = (m[1][0] + m[0][1]) / c; vy = 0.25 * c; vz = (m[2][1] + m[1][2]) / c;
+    } else {
+        c = 2.0 * std::sqrt(1.0 + m[2][2] - m[0][0] - m[1][1]);
+        s = (m[0][1] - m[1][0]) / c; vx = (m[2][0] + m[0][2]) / c; vy = (m[2][1] + m[1][2]) / c; vz = 0.25 * c;
+    }
+    return {s, vx, vy, vz};
+}
+
+// set_Quaternion_Map (model.hpp:341-366): increment rotations read as roll, pitch, yaw
+Quat q_from_increment(double rx, double ry, double rz) {
+    const double cr = std::cos(rx * 0.5), sr = std::sin(rx * 0.5);
+    const double cp = std::cos(ry * 0.5), sp = std::sin(ry * 0.5);
+    const double cy = std::cos(rz * 0.5), sy = std::sin(rz * 0.5);
+    return {cr * cp * cy + sr * sp * sy, sr * cp * cy - cr * sp * sy, cr * sp * cy + sr * cp * sy,
+            cr * cp * sy - sr * sp * cy};
+}
+
+// set_qmid_From_Interpolation (element.hpp:239-266): the half-way slerp
+Quat q_half(const Quat& a, const Quat& b) {
+    if (std::fabs(a.w - b.w) < 1e-12 && std::fabs(a.x - b.x) < 1e-12 && std::fabs(a.y - b.y) < 1e-12 &&
+        std::fabs(a.z - b.z) < 1e-12)
+        return a;
+    const double na = std::sqrt(a.w * a.w + a.x * a.x + a.y * a.y + a.z * a.z);
+    const double nb = std::sqrt(b.w * b.w + b.x * b.x + b.y * b.y + b.z * b.z);
+    const double th = std::acos((a.w * b.w + a.x * b.x + a.y * b.y + a.z * b.z) / (na * nb));
+    if (!(std::fabs(th) > 1e-14)) return a;
+    const double c = std::sin(0.5 * th) / std::sin(th);
+    return {c * a.w + c * b.w, c * a.x + c * b.x, c * a.y + c * b.y, c * a.z + c * b.z};
+}
+
+// get_Euler_Angles_From_Local_Rotation (element.hpp:273-287)
+V3 euler_of(const Quat& q) {
+    const double s = q.w, a = q.x, b = q.y, c = q.z;
+    return {std::atan2(2 * s * a + 2 * b * c, 1 - 2 * (a * a + b * b)), std::asin(2 * s * b - 2 * c * a),
+            std::atan2(2 * s * c + 2 * a * b, 1 - 2 * (b * b + c * c))};
+}
+
+V3 matvec(const std::array<double, 9>& R, const V3& v) {
+    return {R[0] * v[0] + R[1] * v[1] + R[2] * v[2], R[3] * v[0] + R[4] * v[1] + R[5] * v[2],
+            R[6] * v[0] + R[7] * v[1] + R[8] * v[2]};
+}
+V3 matTvec(const std::array<double, 9>& R, const V3& v) {
+    return {R[0] * v[0] + R[3] * v[1] + R[6] * v[2], R[1] * v[0] + R[4] * v[1] + R[7] * v[2],
+            R[2] * v[0] + R[5] * v[1] + R[8] * v[2]};
+}
+
+struct CoElem {
+    int n1, n2;
+    double Kl[12][12];
+    V3 r1, r2;                  // node - midpoint in the initial configuration
+    Quat qe, q1, q2, qmid;
+    double u1[6], u2[6];        // accumulated global displacements of the end nodes
+};
+
+// CBAR::get_Force_In_GlobalRef (element.hpp:316-367)
+void element_force(const CoElem& E, double F[12]) {
+    const Quat q1p = qmul(qconj(E.qmid), E.q1), q2p = qmul(qconj(E.qmid), E.q2);
+    const V3 t1 = euler_of(q1p), t2 = euler_of(q2p);
+    const auto Rec = qmat(qmul(E.qmid, qconj(E.qe)));
+    const auto Rgc = qmat(E.qmid);
+    V3 dr1 = matvec(Rec, E.r1), dr2 = matvec(Rec, E.r2);
+    for (int k = 0; k < 3; ++k) { dr1[k] -= E.r1[k]; dr2[k] -= E.r2[k]; }
+    V3 a1, a2;
+    for (int k = 0; k < 3; ++k) {
+        const double um = 0.5 * (E.u1[k] + E.u2[k]);
+        a1[k] = E.u1[k] - um - dr1[k];
+        a2[k] = E.u2[k] - um - dr2[k];
+    }
+    const V3 d1 = matTvec(Rgc, a1), d2 = matTvec(Rgc, a2);
+    const double D[12] = {d1[0], d1[1], d1[2], t1[0], t1[1], t1[2], d2[0], d2[1], d2[2], t2[0], t2[1], t2[2]};
+    double Fl[12];
+    for (int i = 0; i < 12; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < 12; ++j) s += E.Kl[i][j] * D[j];
+        Fl[i] = s;
+    }
+    for (int blk = 0; blk < 4; ++blk) {
+        const V3 g = matvec(Rgc, {Fl[3 * blk], Fl[3 * blk + 1], Fl[3 * blk + 2]});
+        for (int k = 0; k < 3; ++k) F[3 * blk + k] = g[k];
+    }
+}
+}  // namespace
+
+int afx_beam_solve_nonlinear(afx_beam* b, const double* f, int32_t n_steps, double tol, double damping,
+                             int32_t max_iter, double* u, afx_beam_nl_info* info) {
+    if (!b || !f || !u) return api_fail(AFX_EARG, "null argument");
+    if (n_steps < 1 || !(tol > 0.0) || !(damping > 0.0) || max_iter < 1)
+        return api_fail(AFX_EARG, "need n_steps >= 1, tol > 0, damping > 0 and max_iter >= 1");
+    const int n = b->n, nd = 6 * n;
+    std::vector<CoElem> el(b->ne);
+    for (int e = 0; e < b->ne; ++e) {
+        CoElem& E = el[e];
+        E.n1 = b->en[2 * e];
+        E.n2 = b->en[2 * e + 1];
+        const double* p = &b->props[7 * e];
+        double lam[3][3], L;
+        frame_axes(b->x[E.n1], b->x[E.n2], {p[4], p[5], p[6]}, lam, L);
+        frame_local(L, p[0], p[1], p[2], p[3], E.Kl);
+        for (int k = 0; k < 3; ++k) {
+            const double mid = 0.5 * (b->x[E.n1][k] + b->x[E.n2][k]);
+            E.r1[k] = b->x[E.n1][k] - mid;
+            E.r2[k] = b->x[E.n2][k] - mid;
+        }
+        E.qe = E.q1 = E.q2 = E.qmid = q_from_axes(lam);
+        for (int k = 0; k < 6; ++k) E.u1[k] = E.u2[k] = 0.0;
+    }
+    // external loads with the clamped dofs zeroed (set_K_Final_sparse)
+    std::vector<double> F(f, f + nd), Fint(nd, 0.0), Fd(nd), da(nd), df(nd);
+    for (int i = 0; i < n; ++i)
+        if (b->fixed[i])
+            for (int k = 0; k < 6; ++k) F[6 * i + k] = 0.0;
+    std::fill(u, u + nd, 0.0);
+    std::vector<Quat> qmap(n, Quat{0, 0, 0, 0});
+    std::vector<std::array<double, 9>> rot(b->ne);
+    std::vector<double> Lf;
+    auto factor = [&]() -> int {
+        for (int e = 0; e < b->ne; ++e) rot[e] = qmat(el[e].qmid);
+        return beam_assemble(b, &rot, Lf);
+    };
+    int iters = 0;
+    double res = 0.0;
+    bool ok = true;
+    for (int step = 1; step <= n_steps && ok; ++step) {
+        const double lf = (double)step / (double)n_steps;
+        for (int i = 0; i < nd; ++i) Fd[i] = lf * F[i] - Fint[i];
+        int rc = factor();
+        if (rc) return rc;
+        beam_solve_with(b, Lf, Fd.data(), da.data());
+        do {
+            for (int i = 0; i < nd; ++i) u[i] += da[i];
+            for (int i = 0; i < n; ++i) qmap[i] = q_from_increment(da[6 * i + 3], da[6 * i + 4], da[6 * i + 5]);
+            for (auto& E : el) {
+                for (int k = 0; k < 6; ++k) { E.u1[k] += da[6 * E.n1 + k]; E.u2[k] += da[6 * E.n2 + k]; }
+                E.q1 = qmul(qmap[E.n1], E.q1);
+                E.q2 = qmul(qmap[E.n2], E.q2);
+                E.qmid = q_half(E.q1, E.q2);
+                double Fe[12];
+                element_force(E, Fe);
+                for (int k = 0; k < 6; ++k) { Fint[6 * E.n1 + k] += Fe[k]; Fint[6 * E.n2 + k] += Fe[6 + k]; }
+            }
+            for (int i = 0; i < n; ++i)
+                if (b->fixed[i])
+                    for (int k = 0; k < 6; ++k) Fint[6 * i + k] = 0.0;
+            // Rotate_Ext_Loads (model.hpp:193-205): the loads follow each node's increment
+            for (int i = 0; i < n; ++i) {
+                const auto R = qmat(qmap[i]);
+                const V3 fo = matvec(R, {F[6 * i], F[6 * i + 1], F[6 * i + 2]});
+                const V3 mo = matvec(R, {F[6 * i + 3], F[6 * i + 4], F[6 * i + 5]});
+                for (int k = 0; k < 3; ++k) { F[6 * i + k] = fo[k]; F[6 * i + 3 + k] = mo[k]; }
+            }
+            for (int i = 0; i < nd; ++i) Fd[i] = lf * F[i] - Fint[i];
+            for (int i = 0; i < n; ++i)
+                if (b->fixed[i])
+                    for (int k = 0; k < 6; ++k) F[6 * i + k] = 0.0;
+            rc = factor();
+            if (rc) return rc;
+            beam_solve_with(b, Lf, Fd.data(), df.data());
+            double s2 = 0.0;
+            for (int i = 0; i < nd; ++i) s2 += df[i] * df[i];
+            res = std::sqrt(s2);
+            std::fill(Fint.begin(), Fint.end(), 0.0);
+            const double a = (res < 100. * tol) ? damping : 1.0;
+            for (int i = 0; i < nd; ++i) da[i] = a * df[i];
+            ++iters;
+            if (!std::isfinite(res) || (iters >= max_iter && res > tol)) { ok = false; break; }
+        } while (res > tol);
+    }
+    if (info) {
+        info->iterations = iters;
+        info->converged = ok && res <= tol ? 1 : 0;
+        info->residual = res;
+    }
     return AFX_OK;
 }
 

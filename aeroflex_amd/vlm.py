@@ -96,6 +96,8 @@ def _load():
         "afx_beam_create": (C.c_int, [_i32, _dp, _i32, ip, _dp, _i32, ip, C.POINTER(_P)]),
         "afx_beam_destroy": (C.c_int, [_P]),
         "afx_beam_solve": (C.c_int, [_P, _dp, _dp]),
+        "afx_beam_solve_nonlinear": (C.c_int, [_P, _dp, C.c_int32, C.c_double, C.c_double, C.c_int32, _dp,
+                                               C.c_void_p]),
         "afx_aeroelastic_create": (C.c_int, [_P, _P, C.POINTER(_P)]),
         "afx_aeroelastic_destroy": (C.c_int, [_P]),
         "afx_default_coupling_opts": (None, [C.POINTER(CouplingOpts)]),
@@ -247,8 +249,13 @@ class Dense:
         return lu, piv
 
 
+class BeamNonlinearInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("residual", C.c_double)]
+
+
 class Beam:
-    """Linear Euler-Bernoulli frame (CBAR elements, SPC1 clamps)."""
+    """Euler-Bernoulli frame (CBAR elements, SPC1 clamps): linear solve and
+    the corotational load-stepping solve."""
 
     def __init__(self, xyz, elem_nodes, props, clamped):
         self.xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
@@ -270,6 +277,71 @@ class Beam:
         u = np.zeros(6 * len(self.xyz))
         _chk(_lib.afx_beam_solve(self.h, _d(f), _d(u)))
         return u.reshape(-1, 6)
+
+    def solve_nonlinear(self, load, n_steps=10, tol=1e-6, damping=1.0, max_iter=1000):
+        """MODEL::get_NonLin_Solve (model.hpp:240-339); defaults of the
+        structure settings (structure.hpp:21-23).  Returns (disp, info)."""
+        f = np.ascontiguousarray(load, np.float64).reshape(-1)
+        u = np.zeros(6 * len(self.xyz))
+        info = BeamNonlinearInfo()
+        _chk(_lib.afx_beam_solve_nonlinear(self.h, _d(f), n_steps, tol, damping, max_iter, _d(u),
+                                           C.cast(C.byref(info), C.c_void_p)))
+        return u.reshape(-1, 6), info
+
+
+def read_structure(path):
+    """The reference's structure input (MODEL::read_data_file, model.hpp:368-617):
+    comma-separated GRID, MAT1, PBAR, CBAR, SPC1, FORCE and MOMENT cards, other
+    lines ignored.  Nodes are numbered in GRID order, elements taken in CBAR id
+    order (the reference's std::map); SPC1 must clamp all six dofs (code
+    123456).  Returns (xyz, elem_nodes, props, clamped, load) for Beam and its
+    solves: props per element {E A, E Iy, E Iz, G J, V}, with PBAR's fields
+    A, I1 (= Iz), I2 (= Iy), J; load 6 per node (FORCE / MOMENT = scale times
+    direction)."""
+    grids, mats, pbars, cbars, spcs, loads = {}, {}, {}, {}, [], []
+    order = []
+    for raw in open(path):
+        line = raw.replace(" ", "").rstrip("\r\n")
+        if not line:
+            continue
+        f = line.split(",")
+        card = f[0]
+        if card == "GRID":
+            gid = int(f[1])
+            grids[gid] = (float(f[3]), float(f[4]), float(f[5]))
+            order.append(gid)
+        elif card == "MAT1":
+            mats[int(f[1])] = (float(f[2]), float(f[3]))
+        elif card == "PBAR":
+            pbars[int(f[1])] = (int(f[2]), float(f[3]), float(f[4]), float(f[5]), float(f[6]))
+        elif card == "CBAR":
+            cbars[int(f[1])] = (int(f[2]), int(f[3]), int(f[4]), float(f[5]), float(f[6]), float(f[7]))
+        elif card == "SPC1":
+            code = f[2]
+            for tok in f[3:]:
+                if tok:
+                    spcs.append((int(tok), code))
+        elif card in ("FORCE", "MOMENT"):
+            loads.append((card, int(f[2]), float(f[4]), float(f[5]), float(f[6]), float(f[7])))
+    index = {g: i for i, g in enumerate(order)}
+    xyz = np.array([grids[g] for g in order], np.float64)
+    en, props = [], []
+    for cid in sorted(cbars):
+        pid, a, b, vx, vy, vz = cbars[cid]
+        mid, A, Iz, Iy, J = pbars[pid]
+        E, G = mats[mid]
+        en.append((index[a], index[b]))
+        props.append((E * A, E * Iy, E * Iz, G * J, vx, vy, vz))
+    clamped = []
+    for g, code in spcs:
+        if sorted(code) != list("123456"):
+            raise ValueError(f"SPC1 code {code} on node {g}: only full clamps (123456) are supported")
+        clamped.append(index[g])
+    load = np.zeros((len(order), 6))
+    for card, g, scale, dx, dy, dz in loads:
+        off = 0 if card == "FORCE" else 3
+        load[index[g], off:off + 3] += scale * np.array([dx, dy, dz])
+    return xyz, np.array(en, np.int32), np.array(props), np.array(clamped, np.int32), load
 
 
 def default_coupling_opts():

@@ -180,6 +180,25 @@ int afx_beam_destroy(afx_beam* h);
 /* MODEL::get_Lin_Solve (model.hpp:225-238): K u = f, 6 dofs per node
  * (Fx, Fy, Fz, Mx, My, Mz) -> (ux, uy, uz, rx, ry, rz) */
 int afx_beam_solve(afx_beam* h, const double* load, double* disp);
+/* MODEL::get_NonLin_Solve (model.hpp:240-339) with the corotational CBAR
+ * (element.hpp:111-370): n_steps load steps of (step / n_steps) load; each
+ * solves with the stiffness re-assembled from every element's current
+ * mid-node rotation (set_K_global, model.hpp:103-131), updates node rotations
+ * by quaternions built from the rotation increments read as roll, pitch, yaw,
+ * recomputes internal forces in each element's corotated frame, turns the
+ * loads with their node's increment (follower loads, Rotate_Ext_Loads), and
+ * iterates until ||du|| <= tol; once ||du|| < 100 tol the increment is scaled
+ * by damping.  As in the reference the iteration count is cumulative over the
+ * steps; max_iter bounds it (the reference has no bound).  disp: accumulated
+ * increments, 6 per node (rotations as summed increments, like the
+ * reference's Dep). */
+typedef struct afx_beam_nl_info {
+    int32_t iterations;
+    int32_t converged;
+    double residual;               /* ||du|| of the last iteration */
+} afx_beam_nl_info;
+int afx_beam_solve_nonlinear(afx_beam* h, const double* load, int32_t n_steps, double tol, double damping,
+                             int32_t max_iter, double* disp, afx_beam_nl_info* info);
 
 /* -------------------------------------------------------- aeroelastic --- */
 

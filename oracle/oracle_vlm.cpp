@@ -23,6 +23,8 @@
 //   ovl_beam_*    element.hpp:77-160 (set_K_local, set_T_Rotation,
 //                 set_K_elem_global), model.hpp:103-131 (assembly),
 //                 225-238 (linear solve) with SPC1 as row/column zeroing
+//   ovl_beam_nonlinear  model.hpp:240-366 (get_NonLin_Solve, Rotate_Ext_Loads,
+//                 set_Quaternion_Map), element.hpp:111-370 (corotational CBAR)
 //
 // Pinning (tests/test_oracle_vlm.py): segment and ring closed forms of the
 // Biot-Savart law, Helmbold's lifting-surface lift slope, symmetry, Kutta-
@@ -142,7 +144,7 @@ struct Beam {
     std::vector<char> fixed;
 };
 
-void element_global(const Beam& B, int e, double Kg[12][12]) {
+void element_local(const Beam& B, int e, double Kout[12][12], double lout[3][3]) {
     V3 N1 = load3(B.xyz + 3 * B.en[2 * e]), N2 = load3(B.xyz + 3 * B.en[2 * e + 1]);
     const double* p = B.props + 7 * e;
     double L = nrm(N2 - N1);
@@ -167,6 +169,15 @@ void element_global(const Beam& B, int e, double Kg[12][12]) {
     ez = (1.0 / nrm(ez)) * ez;
     V3 ey = -1.0 * cross(ex, ez);
     double lam[3][3] = {{ex.x, ex.y, ex.z}, {ey.x, ey.y, ey.z}, {ez.x, ez.y, ez.z}};
+    for (int i = 0; i < 12; ++i)
+        for (int j = 0; j < 12; ++j) Kout[i][j] = Kl[i][j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) lout[i][j] = lam[i][j];
+}
+
+void element_global(const Beam& B, int e, double Kg[12][12]) {
+    double Kl[12][12], lam[3][3];
+    element_local(B, e, Kl, lam);
     double T[12][12] = {};
     for (int b = 0; b < 4; ++b)
         for (int r = 0; r < 3; ++r)
@@ -461,6 +472,236 @@ void ovl_nonlinear(int nn, const double* xyz, int ns, const int32_t* soff, const
 }
 
 // fixed-point coupling with exact FP64 solves; out = {iterations, tip, CL}
+// MODEL::get_NonLin_Solve (model.hpp:240-339) with the corotational CBAR
+// (element.hpp:111-370): quaternion helpers follow element.hpp:159-287 and
+// model.hpp:193-220, 341-366; Eigen's quaternion product is the Hamilton
+// product; the stiffness is re-assembled as Rot K_local Rot^T with Rot the
+// element's mid-node rotation (set_K_global) and solved densely with the
+// clamped rows and columns replaced by identity (set_K_Final_sparse).
+// Dep starts at zero (its setZero is commented out in the reference).
+// out3: iterations, converged (0/1), last residual.
+namespace {
+struct Q4 { double w, x, y, z; };
+Q4 qprod(Q4 a, Q4 b) {
+    return {a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+            a.w * b.y + a.y * b.w + a.z * b.x - a.x * b.z, a.w * b.z + a.z * b.w + a.x * b.y - a.y * b.x};
+}
+Q4 qconjugate(Q4 a) { return {a.w, -a.x, -a.y, -a.z}; }
+void qrotation(Q4 q, double M[3][3]) {
+    const double s = q.w, x = q.x, y = q.y, z = q.z;
+    M[0][0] = 1. - 2. * (y * y + z * z); M[0][1] = 2. * (x * y - s * z); M[0][2] = 2. * (x * z + s * y);
+    M[1][0] = 2. * (x * y + s * z); M[1][1] = 1. - 2. * (x * x + z * z); M[1][2] = 2. * (y * z - s * x);
+    M[2][0] = 2. * (x * z - s * y); M[2][1] = 2. * (y * z + s * x); M[2][2] = 1. - 2. * (x * x + y * y);
+}
+Q4 qe_from(const double m[3][3]) {
+    const double trace = m[0][0] + m[1][1] + m[2][2];
+    double cst, s, vx, vy, vz;
+    if (trace > 0.0) {
+        cst = 0.5 / std::sqrt(trace + 1.0);
+        s = 0.25 / cst;
+        vx = (m[1][2] - m[2][1]) * cst;
+        vy = (m[2][0] - m[0][2]) * cst;
+        vz = (m[0][1] - m[1][0]) * cst;
+    } else if (m[0][0] > m[1][1] && m[0][0] > m[2][2]) {
+        cst = 2.0 * std::sqrt(1.0 + m[0][0] - m[1][1] - m[2][2]);
+        s = (m[1][2] - m[2][1]) / cst;
+        vx = 0.25 * cst;
+        vy = (m[1][0] + m[0][1]) / cst;
+        vz = (m[2][0] + m[0][2]) / cst;
+    } else if (m[1][1] > m[2][2]) {
+        cst = 2.0 * std::sqrt(1.0 + m[1][1] - m[0][0] - m[2][2]);
+        s = (m[2][0] - m[0][2]) / cst;
+        vx = (m[1][0] + m[0][1]) / cst;
+        vy = 0.25 * cst;
+        vz = (m[2][1] + m[1][2]) / cst;
+    } else {
+        cst = 2.0 * std::sqrt(1.0 + m[2][2] - m[0][0] - m[1][1]);
+        s = (m[0][1] - m[1][0]) / cst;
+        vx = (m[2][0] + m[0][2]) / cst;
+        vy = (m[2][1] + m[1][2]) / cst;
+        vz = 0.25 * cst;
+    }
+    return {s, vx, vy, vz};
+}
+struct CoBar {
+    int n1, n2;
+    double Kl[12][12];
+    V3 r1, r2;
+    Q4 qe, q1, q2, qmid;
+    double u1[6], u2[6], umid[6];
+};
+}  // namespace
+
+void ovl_beam_nonlinear(int n, const double* xyz, int ne, const int32_t* en, const double* props, int nclamped,
+                        const int32_t* clamped, const double* f, int max_load_step, double tol, double amor,
+                        int max_iter, double* dep, double* out3) {
+    Beam B{n, ne, xyz, en, props, std::vector<char>(n, 0)};
+    for (int k = 0; k < nclamped; ++k) B.fixed[clamped[k]] = 1;
+    const int nd = 6 * n;
+    std::vector<CoBar> bars(ne);
+    for (int e = 0; e < ne; ++e) {
+        CoBar& c = bars[e];
+        c.n1 = en[2 * e];
+        c.n2 = en[2 * e + 1];
+        double lam[3][3];
+        element_local(B, e, c.Kl, lam);
+        V3 N1 = load3(xyz + 3 * c.n1), N2 = load3(xyz + 3 * c.n2);
+        V3 Nmid = 0.5 * (N1 + N2);
+        c.r1 = N1 - Nmid;
+        c.r2 = N2 - Nmid;
+        c.qe = qe_from(lam);
+        c.q1 = c.q2 = c.qmid = c.qe;
+        for (int k = 0; k < 6; ++k) c.u1[k] = c.u2[k] = c.umid[k] = 0;
+    }
+    std::vector<double> Forces(f, f + nd), Fint(nd, 0.0), Fdiff(nd), damor(nd), dfull(nd);
+    for (int i = 0; i < nd; ++i)
+        if (B.fixed[i / 6]) Forces[i] = 0;
+    for (int i = 0; i < nd; ++i) dep[i] = 0;
+    std::vector<Q4> qmap(n, Q4{0, 0, 0, 0});
+    // set_K_global + set_K_Final_sparse + get_Solve
+    auto solve = [&](const std::vector<double>& rhs, std::vector<double>& x) {
+        std::vector<double> K((size_t)nd * nd, 0.0);
+        for (auto& c : bars) {
+            double D[3][3];
+            qrotation(c.qmid, D);
+            double Rot[12][12] = {};
+            for (int b = 0; b < 4; ++b)
+                for (int r = 0; r < 3; ++r)
+                    for (int cc = 0; cc < 3; ++cc) Rot[3 * b + r][3 * b + cc] = D[r][cc];
+            double T[12][12];
+            for (int i = 0; i < 12; ++i)
+                for (int j = 0; j < 12; ++j) {
+                    double s = 0;
+                    for (int a = 0; a < 12; ++a)
+                        for (int b = 0; b < 12; ++b) s += Rot[i][a] * c.Kl[a][b] * Rot[j][b];
+                    T[i][j] = s;
+                }
+            for (int j = 0; j < 6; ++j)
+                for (int k = 0; k < 6; ++k) {
+                    K[(size_t)(6 * c.n1 + j) * nd + 6 * c.n1 + k] += T[j][k];
+                    K[(size_t)(6 * c.n2 + j) * nd + 6 * c.n2 + k] += T[6 + j][6 + k];
+                    K[(size_t)(6 * c.n1 + j) * nd + 6 * c.n2 + k] += T[j][6 + k];
+                    K[(size_t)(6 * c.n2 + j) * nd + 6 * c.n1 + k] += T[6 + j][k];
+                }
+        }
+        for (int i = 0; i < nd; ++i)
+            if (B.fixed[i / 6]) {
+                for (int j = 0; j < nd; ++j) K[(size_t)i * nd + j] = K[(size_t)j * nd + i] = 0.0;
+                K[(size_t)i * nd + i] = 1.0;
+            }
+        x.assign(nd, 0.0);
+        lu_solve(nd, K, rhs, x.data());
+    };
+    int iters = 0;
+    double residu = 0;
+    bool ok = true;
+    for (double step = 1.; step <= (double)max_load_step && ok; step++) {
+        for (int i = 0; i < nd; ++i) Fdiff[i] = (step / max_load_step) * Forces[i] - Fint[i];
+        solve(Fdiff, damor);
+        do {
+            for (int i = 0; i < nd; ++i) dep[i] += damor[i];
+            for (int i = 0; i < n; ++i) {   // set_Quaternion_Map
+                const double rx = damor[6 * i + 3], ry = damor[6 * i + 4], rz = damor[6 * i + 5];
+                const double cr = std::cos(rx * 0.5), sr = std::sin(rx * 0.5);
+                const double cp = std::cos(ry * 0.5), sp = std::sin(ry * 0.5);
+                const double cy = std::cos(rz * 0.5), sy = std::sin(rz * 0.5);
+                qmap[i] = {cr * cp * cy + sr * sp * sy, sr * cp * cy - cr * sp * sy, cr * sp * cy + sr * cp * sy,
+                           cr * cp * sy - sr * sp * cy};
+            }
+            for (auto& c : bars) {
+                // set_u_i
+                for (int k = 0; k < 6; ++k) {
+                    c.u1[k] += damor[6 * c.n1 + k];
+                    c.u2[k] += damor[6 * c.n2 + k];
+                    c.umid[k] = 0.5 * (c.u1[k] + c.u2[k]);
+                }
+                // set_q1_And_q2, set_qmid_From_Interpolation
+                c.q1 = qprod(qmap[c.n1], c.q1);
+                c.q2 = qprod(qmap[c.n2], c.q2);
+                if (std::fabs(c.q1.w - c.q2.w) < 1e-12 && std::fabs(c.q1.x - c.q2.x) < 1e-12 &&
+                    std::fabs(c.q1.y - c.q2.y) < 1e-12 && std::fabs(c.q1.z - c.q2.z) < 1e-12) {
+                    c.qmid = c.q1;
+                } else {
+                    const double n1 = std::sqrt(c.q1.w * c.q1.w + c.q1.x * c.q1.x + c.q1.y * c.q1.y + c.q1.z * c.q1.z);
+                    const double n2 = std::sqrt(c.q2.w * c.q2.w + c.q2.x * c.q2.x + c.q2.y * c.q2.y + c.q2.z * c.q2.z);
+                    const double theta =
+                        std::acos((c.q1.w * c.q2.w + c.q1.x * c.q2.x + c.q1.y * c.q2.y + c.q1.z * c.q2.z) / (n1 * n2));
+                    if (std::fabs(theta) > 1e-14) {
+                        const double cst = std::sin(0.5 * theta) / std::sin(theta);
+                        c.qmid = {cst * c.q1.w + cst * c.q2.w, cst * c.q1.x + cst * c.q2.x,
+                                  cst * c.q1.y + cst * c.q2.y, cst * c.q1.z + cst * c.q2.z};
+                    } else {
+                        c.qmid = c.q1;
+                    }
+                }
+                // set_Quaternion_Local_Rotations, get_Force_In_GlobalRef
+                Q4 qp[2] = {qprod(qconjugate(c.qmid), c.q1), qprod(qconjugate(c.qmid), c.q2)};
+                double th[2][3];
+                for (int a = 0; a < 2; ++a) {
+                    const double s = qp[a].w, aa = qp[a].x, bb = qp[a].y, cc = qp[a].z;
+                    th[a][0] = std::atan2(2 * s * aa + 2 * bb * cc, 1 - 2 * (aa * aa + bb * bb));
+                    th[a][1] = std::asin(2 * s * bb - 2 * cc * aa);
+                    th[a][2] = std::atan2(2 * s * cc + 2 * aa * bb, 1 - 2 * (bb * bb + cc * cc));
+                }
+                double Rec[3][3], Rgc[3][3];
+                qrotation(qprod(c.qmid, qconjugate(c.qe)), Rec);
+                qrotation(c.qmid, Rgc);
+                const double r[2][3] = {{c.r1.x, c.r1.y, c.r1.z}, {c.r2.x, c.r2.y, c.r2.z}};
+                const double* u[2] = {c.u1, c.u2};
+                double Dl[12];
+                for (int a = 0; a < 2; ++a) {
+                    double w[3];
+                    for (int i = 0; i < 3; ++i) {
+                        const double dr = Rec[i][0] * r[a][0] + Rec[i][1] * r[a][1] + Rec[i][2] * r[a][2] - r[a][i];
+                        w[i] = u[a][i] - c.umid[i] - dr;
+                    }
+                    for (int i = 0; i < 3; ++i) Dl[6 * a + i] = Rgc[0][i] * w[0] + Rgc[1][i] * w[1] + Rgc[2][i] * w[2];
+                    for (int i = 0; i < 3; ++i) Dl[6 * a + 3 + i] = th[a][i];
+                }
+                double Fl[12], Fg[12];
+                for (int i = 0; i < 12; ++i) {
+                    double s = 0;
+                    for (int j = 0; j < 12; ++j) s += c.Kl[i][j] * Dl[j];
+                    Fl[i] = s;
+                }
+                for (int b = 0; b < 4; ++b)
+                    for (int i = 0; i < 3; ++i)
+                        Fg[3 * b + i] = Rgc[i][0] * Fl[3 * b] + Rgc[i][1] * Fl[3 * b + 1] + Rgc[i][2] * Fl[3 * b + 2];
+                for (int k = 0; k < 6; ++k) {
+                    Fint[6 * c.n1 + k] += Fg[k];
+                    Fint[6 * c.n2 + k] += Fg[6 + k];
+                }
+            }
+            for (int i = 0; i < nd; ++i)   // apply_SPC1_Forces
+                if (B.fixed[i / 6]) Fint[i] = 0;
+            for (int i = 0; i < n; ++i) {  // Rotate_Ext_Loads
+                double D[3][3];
+                qrotation(qmap[i], D);
+                double tmp[6];
+                for (int r = 0; r < 3; ++r) {
+                    tmp[r] = D[r][0] * Forces[6 * i] + D[r][1] * Forces[6 * i + 1] + D[r][2] * Forces[6 * i + 2];
+                    tmp[3 + r] = D[r][0] * Forces[6 * i + 3] + D[r][1] * Forces[6 * i + 4] + D[r][2] * Forces[6 * i + 5];
+                }
+                for (int k = 0; k < 6; ++k) Forces[6 * i + k] = tmp[k];
+            }
+            for (int i = 0; i < nd; ++i) Fdiff[i] = (step / max_load_step) * Forces[i] - Fint[i];
+            for (int i = 0; i < nd; ++i)
+                if (B.fixed[i / 6]) Forces[i] = 0;
+            solve(Fdiff, dfull);
+            double s2 = 0;
+            for (int i = 0; i < nd; ++i) s2 += dfull[i] * dfull[i];
+            residu = std::sqrt(s2);
+            std::fill(Fint.begin(), Fint.end(), 0.0);
+            for (int i = 0; i < nd; ++i) damor[i] = residu < 100. * tol ? amor * dfull[i] : dfull[i];
+            ++iters;
+            if (!std::isfinite(residu) || (iters >= max_iter && residu > tol)) { ok = false; break; }
+        } while (residu > tol);
+    }
+    out3[0] = iters;
+    out3[1] = ok && residu <= tol ? 1 : 0;
+    out3[2] = residu;
+}
+
 void ovl_aeroelastic(int nn, const double* x0, int ns, const int32_t* soff, const int32_t* rn, double aoa,
                      double vinf, double rho, double sref, double cref, int n, const double* bxyz, int ne,
                      const int32_t* en, const double* props, int nclamped, const int32_t* clamped, double tol,

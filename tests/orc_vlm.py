@@ -24,6 +24,7 @@ for _name, _args in {
     "ovl_transfer_disp": [_i, _dp, _i, _dp, _i, _ip, _dp, _dp],
     "ovl_nonlinear": [_i, _dp, _i, _ip, _ip, _f, _f, _f, _f, _f, _i, _ip, _dp, _dp, _dp, _i, _ip, _dp, _f, _f, _i,
                       _dp, _dp, _dp],
+    "ovl_beam_nonlinear": [_i, _dp, _i, _ip, _dp, _i, _ip, _dp, _i, _f, _f, _i, _dp, _dp],
     "ovl_aeroelastic": [_i, _dp, _i, _ip, _ip, _f, _f, _f, _f, _f, _i, _dp, _i, _ip, _dp, _i, _ip, _f, _i, _f,
                         _dp, _dp],
 }.items():
@@ -154,3 +155,17 @@ def nonlinear(xyz, rings, soff, alpha, polar, vinf=1.0, rho=1.0, sref=1.0, cref=
                        max_iter, _d(a), _d(c), _d(out))
     return {"iterations": int(out[0]), "residual": out[1], "cl": out[2], "cd": out[3], "cy": out[4],
             "alpha": a, "station_cl": c}
+
+
+def beam_nonlinear(xyz, en, props, clamped, load, n_steps=10, tol=1e-6, damping=1.0, max_iter=1000):
+    """MODEL::get_NonLin_Solve (model.hpp:240-339): (disp (n, 6), info dict)."""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    en = np.ascontiguousarray(en, np.int32)
+    props = np.ascontiguousarray(props, np.float64)
+    clamped = np.ascontiguousarray(clamped, np.int32)
+    f = np.ascontiguousarray(load, np.float64).reshape(-1)
+    u, out = np.zeros(6 * len(xyz)), np.zeros(3)
+    _lib.ovl_beam_nonlinear(len(xyz), _d(xyz), len(en), _n(en), _d(props), len(clamped), _n(clamped), _d(f),
+                            n_steps, tol, damping, max_iter, _d(u), _d(out))
+    return u.reshape(-1, 6), {"iterations": int(out[0]), "converged": bool(out[1]), "residual": out[2]}
+

@@ -2,6 +2,7 @@
 the wing lattice generator and the banded-Cholesky frame model, against the
 oracle and the cantilever closed forms."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -72,3 +73,56 @@ def test_beam_rejects_mechanism_and_bad_input():
         V.Beam(x, en, np.tile([1.0, 1.0, 1.0, 1.0, 0.0, 1.0, 0.0], (2, 1)), [0])
     with pytest.raises(V.VlmError, match="invalid node"):
         V.Beam(x, [[0, 1], [1, 1]], props, [0])
+
+
+GOLDEN_STRUCT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "structure")
+
+
+def test_read_structure_moment_file():
+    """MODEL::read_data_file (model.hpp:368-617) on examples/structure/Moment.txt:
+    13 GRIDs along x, 12 CBARs with PBAR 10 / MAT1 11, SPC1 on node 100,
+    MOMENT 35e3 about z on node 112."""
+    xyz, en, props, cl, load = V.read_structure(os.path.join(GOLDEN_STRUCT, "Moment.txt"))
+    assert xyz.shape == (13, 3) and np.allclose(xyz[:, 0], np.arange(13) * 25.0)
+    assert en.tolist() == [[k, k + 1] for k in range(12)]
+    assert np.allclose(props, [140e3 * 100, 140e3 * 120, 140e3 * 120, 70e3 * 120, 0, 1, 0])
+    assert cl.tolist() == [0]
+    expect = np.zeros((13, 6))
+    expect[12, 5] = 35e3
+    assert np.array_equal(load, expect)
+
+
+@pytest.mark.parametrize("case", ["moment_txt", "inclined_3d"])
+def test_nonlinear_beam_matches_oracle(case):
+    """afx_beam_solve_nonlinear against the oracle's restatement of
+    get_NonLin_Solve: the reference's tip-moment example, and a cantilever
+    inclined in 3D under a follower tip force, torque and moment."""
+    if case == "moment_txt":
+        xyz, en, props, cl, load = V.read_structure(os.path.join(GOLDEN_STRUCT, "Moment.txt"))
+        steps = 10
+    else:
+        ne = 20
+        d = np.array([0.3, 1.0, 0.2]) / np.linalg.norm([0.3, 1.0, 0.2])
+        xyz = np.outer(np.linspace(0.0, 10.0, ne + 1), d)
+        en = np.c_[np.arange(ne), np.arange(1, ne + 1)]
+        props = np.tile([1e6, 2e3, 5e3, 1.5e3, 1.0, 0.0, 0.0], (ne, 1))
+        cl = np.array([0])
+        load = np.zeros((ne + 1, 6))
+        load[-1] = [3.0, 1.0, 20.0, 15.0, 5.0, 2.0]
+        steps = 5
+    b = V.Beam(xyz, en, props, cl)
+    u, info = b.solve_nonlinear(load, n_steps=steps, tol=1e-8)
+    uo, io = ov.beam_nonlinear(xyz, en, props, cl, load, n_steps=steps, tol=1e-8)
+    assert info.converged and io["converged"]
+    assert np.max(np.abs(u - uo)) <= 1e-7 * np.max(np.abs(uo))
+    assert abs(info.iterations - io["iterations"]) <= 1
+
+
+def test_nonlinear_beam_rejects_bad_settings():
+    xyz, en, props, cl, load = V.read_structure(os.path.join(GOLDEN_STRUCT, "Moment.txt"))
+    b = V.Beam(xyz, en, props, cl)
+    for kw in ({"n_steps": 0}, {"tol": 0.0}, {"damping": 0.0}, {"max_iter": 0}):
+        with pytest.raises(V.VlmError):
+            b.solve_nonlinear(load, **kw)
+    u, info = b.solve_nonlinear(load, n_steps=1, tol=1e-14, max_iter=3)
+    assert not info.converged and info.iterations == 3

@@ -303,3 +303,47 @@ def test_nonlinear_stall_lowers_lift_and_converges():
     assert r["residual"] <= 1e-8 and r["cl"] < cl_lin
     a = r["alpha"]
     np.testing.assert_allclose(a, a[::-1], rtol=1e-8)
+
+
+def cantilever_x(ne, L=300.0, EA=1.4e7, EI=1.68e7, GJ=8.4e6):
+    """The Moment.txt cantilever (E 140e3, A 100, I 120, G 70e3) with ne
+    elements along x, V = (0, 1, 0), clamped at x = 0."""
+    x = np.c_[np.linspace(0, L, ne + 1), np.zeros(ne + 1), np.zeros(ne + 1)]
+    en = np.c_[np.arange(ne), np.arange(1, ne + 1)]
+    props = np.tile([EA, EI, EI, GJ, 0.0, 1.0, 0.0], (ne, 1))
+    return x, en, props, [0]
+
+
+def test_nonlinear_beam_tip_moment_is_the_elastica():
+    """get_NonLin_Solve (model.hpp:240-339) on the reference's geometric-
+    nonlinear example (examples/structure/Moment.txt, SOL 64): a follower tip
+    moment M bends the cantilever into a circular arc of radius EI/M, so the
+    tip rotates by ML/EI = 0.625 and sits at (R sin 0.625, R (1 - cos 0.625)),
+    R = 480; the linear answer (93.75) is 3% off.  The corotational elements
+    reach the arc at second order in the element size."""
+    L, M, EI = 300.0, 35e3, 1.68e7
+    R, th = EI / M, M * L / EI
+    errs = []
+    for ne in (12, 24, 48):
+        x, en, props, cl = cantilever_x(ne)
+        load = np.zeros((ne + 1, 6))
+        load[-1, 5] = M
+        u, info = ov.beam_nonlinear(x, en, props, cl, load, n_steps=10, tol=1e-6)
+        assert info["converged"]
+        assert abs(u[-1, 5] - th) <= 1e-9
+        errs.append(np.hypot(u[-1, 0] - (R * math.sin(th) - L), u[-1, 1] - R * (1 - math.cos(th))))
+        assert np.all(u[:, 2:5] == 0.0)
+    assert errs[0] < 0.04 and errs[2] < 2.2e-3
+    assert 3.8 < errs[0] / errs[1] < 4.2 and 3.8 < errs[1] / errs[2] < 4.2
+
+
+def test_nonlinear_beam_small_load_is_the_linear_solution():
+    """At a load 1e-5 of a tip force plus torque the corotational solve agrees
+    with the linear frame model to the size of the neglected terms."""
+    x, en, props, cl = cantilever_x(16)
+    load = np.zeros((17, 6))
+    load[-1] = np.array([0.0, 2.0, 3.0, 40.0, 0.0, 0.0]) * 1e-5
+    lin = ov.beam_solve(x, en, props, cl, load)
+    u, info = ov.beam_nonlinear(x, en, props, cl, load, n_steps=2, tol=1e-14)
+    assert info["converged"]
+    assert np.max(np.abs(u - lin)) <= 1e-6 * np.max(np.abs(lin))
