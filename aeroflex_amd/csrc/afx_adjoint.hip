@@ -640,6 +640,51 @@ __global__ void __launch_bounds__(kBlock) k_axpy4(const double* __restrict__ x, 
     }
 }
 
+// Facc += G.fbar (8 planes over faces), Aacc += G.abar (real cells)
+__global__ void __launch_bounds__(kBlock) k_acc_metrics(const double* __restrict__ fbar, const double* __restrict__ abar,
+                                                        int32_t nF, int32_t nC, double* __restrict__ facc,
+                                                        double* __restrict__ aacc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nF)
+        for (int k = 0; k < 8; ++k) facc[(size_t)k * nF + i] += fbar[(size_t)k * nF + i];
+    if (i < nC) aacc[i] += abar[i];
+}
+
+// metric part of the local time step's VJP: dt_c = cfl A_c / S_c with
+// S_c = sum_f lambda_f len_f and lambda_f = max over the face's sides of
+// c + |u n_x + v n_y| (the owner only on boundary faces).  With
+// sbar_c = -dtbar_c dt_c^2 / (cfl A_c): lenbar_f += lambda_f (sbar_o + sbar_n),
+// (nbar_x, nbar_y)_f += len_f (sbar_o + sbar_n) sign(V) (u, v) of the active
+// side.  (The area term dtbar dt / A cancels the residual's division by the
+// area in the stage update, so neither is added.)
+__global__ void __launch_bounds__(kBlock) k_dt_bar_geo(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
+                                                       GasT<double> g, const double* __restrict__ dt,
+                                                       const double* __restrict__ dtbar, double cfl,
+                                                       double* __restrict__ facc) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F.nF) return;
+    const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
+    const int o = F.c0[f], n = F.c1[f];
+    const double nx = F.nx[f], ny = F.ny[f];
+    double sb = -dtbar[o] * dt[o] * dt[o] / (cfl * C.area[o]);
+    double qa[4], d[4];
+    for (int k = 0; k < 4; ++k) qa[k] = q[k * nT + o];
+    double lam = eig_and_grad(g, nx, ny, qa, d);
+    if (F.kind[f] == K_INTERNAL) {
+        sb += -dtbar[n] * dt[n] * dt[n] / (cfl * C.area[n]);
+        double qn[4];
+        for (int k = 0; k < 4; ++k) qn[k] = q[k * nT + n];
+        const double en = eig_and_grad(g, nx, ny, qn, d);
+        if (lam < en) { lam = en; for (int k = 0; k < 4; ++k) qa[k] = qn[k]; }
+    }
+    const double u = qa[1] / qa[0], v = qa[2] / qa[0];
+    const double sv = (u * nx + v * ny) < 0.0 ? -1.0 : 1.0;
+    const double e = F.len[f] * sb;
+    facc[0 * nF + f] += e * sv * u;
+    facc[1 * nF + f] += e * sv * v;
+    facc[2 * nF + f] += lam * sb;
+}
+
 // acc[k][j] += ghost[k*nT + cell_j] over the farfield ghost cells
 __global__ void __launch_bounds__(kBlock) k_ghost_acc(const double* __restrict__ ghost, const int32_t* __restrict__ cells,
                                                       int32_t n, int32_t nT, double* __restrict__ acc) {
@@ -649,6 +694,27 @@ __global__ void __launch_bounds__(kBlock) k_ghost_acc(const double* __restrict__
 }
 
 }  // namespace
+
+hipError_t launch_acc_metrics(hipStream_t s, const GeoBufs& G, int32_t nF, int32_t nC, double* facc, double* aacc) {
+    const int n = nF > nC ? nF : nC;
+    hipLaunchKernelGGL(k_acc_metrics, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, G.fbar, G.abar, nF, nC,
+                       facc, aacc);
+    return hipGetLastError();
+}
+
+hipError_t launch_dt_vjp_geo(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, GasT<double> g,
+                             const double* dt, const double* dtbar, double cfl, double* facc) {
+    hipLaunchKernelGGL(k_dt_bar_geo, dim3((F.nF + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F, q, g, dt, dtbar,
+                       cfl, facc);
+    return hipGetLastError();
+}
+
+hipError_t launch_geo_to_nodes(hipStream_t s, CellsT<double> C, FacesT<double> F, const GeoBufs& G) {
+    hipLaunchKernelGGL(k_geo_face, dim3((F.nF + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F, G);
+    hipLaunchKernelGGL(k_geo_cell, dim3((C.nC + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F, G);
+    hipLaunchKernelGGL(k_geo_node, dim3((G.nN + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F, G);
+    return hipGetLastError();
+}
 
 hipError_t launch_rev_stage(hipStream_t s, const double* gk, const double* W, const double* dt, const double* area,
                             double coef, int32_t nC, int32_t nT, double* wseed, double* dtbar, double* qbar) {
@@ -681,7 +747,7 @@ namespace {
 template <bool GEO, bool GHOST>
 void adjoint_passes(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* qg, const double* q,
                     const double* w, GasT<double> g, int second_order, double limiter_k, AdjBufs A, double* lam,
-                    double* lam_g, double* ghost, const GeoBufs& G) {
+                    double* lam_g, double* ghost, const GeoBufs& G, bool to_nodes = true) {
     const int nbc = (C.nC + kBlock - 1) / kBlock, nbf = (F.nF + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_adj_forward, dim3(nbc), dim3(kBlock), 0, s, C, F, qg, q, g, second_order, limiter_k, A);
     hipLaunchKernelGGL((k_adj_face<GEO, GHOST>), dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A, G,
@@ -694,7 +760,7 @@ void adjoint_passes(hipStream_t s, CellsT<double> C, FacesT<double> F, const dou
     } else if (lam_g != lam) {
         hipLaunchKernelGGL(k_fill0, dim3((4 * C.nT + kBlock - 1) / kBlock), dim3(kBlock), 0, s, lam_g, 4 * C.nT);
     }
-    if (GEO) {
+    if (GEO && to_nodes) {
         hipLaunchKernelGGL(k_geo_face, dim3(nbf), dim3(kBlock), 0, s, C, F, G);
         hipLaunchKernelGGL(k_geo_cell, dim3(nbc), dim3(kBlock), 0, s, C, F, G);
         hipLaunchKernelGGL(k_geo_node, dim3((G.nN + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F, G);
@@ -710,6 +776,13 @@ hipError_t launch_adjoint_stage(hipStream_t s, CellsT<double> C, FacesT<double> 
     if (Gp) adjoint_passes<true, false>(s, C, F, qg, q, w, g, second_order, limiter_k, A, lam, lam_g, nullptr, G);
     else if (ghost) adjoint_passes<false, true>(s, C, F, qg, q, w, g, second_order, limiter_k, A, lam, lam_g, ghost, G);
     else adjoint_passes<false, false>(s, C, F, qg, q, w, g, second_order, limiter_k, A, lam, lam_g, nullptr, G);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_stage_metrics(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* qg,
+                                        const double* q, const double* w, GasT<double> g, int second_order,
+                                        double limiter_k, AdjBufs A, double* lam, double* lam_g, const GeoBufs& G) {
+    adjoint_passes<true, false>(s, C, F, qg, q, w, g, second_order, limiter_k, A, lam, lam_g, nullptr, G, false);
     return hipGetLastError();
 }
 

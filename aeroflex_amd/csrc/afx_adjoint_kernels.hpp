@@ -55,6 +55,19 @@ hipError_t launch_adjoint_stage(hipStream_t s, CellsT<double> C, FacesT<double> 
                                 const double* w, GasT<double> g, int second_order, double limiter_k, AdjBufs A,
                                 double* lam, double* lam_g, double* ghost, const GeoBufs* G = nullptr);
 
+// launch_adjoint_stage with the metric adjoints of the stage left in G.fbar /
+// G.abar (overwritten) and not converted to node adjoints
+hipError_t launch_adjoint_stage_metrics(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* qg,
+                                        const double* q, const double* w, GasT<double> g, int second_order,
+                                        double limiter_k, AdjBufs A, double* lam, double* lam_g, const GeoBufs& G);
+// facc += G.fbar (8 x nF), aacc += G.abar (nC)
+hipError_t launch_acc_metrics(hipStream_t s, const GeoBufs& G, int32_t nF, int32_t nC, double* facc, double* aacc);
+// facc += metric adjoints (n, len) of the local time step for dtbar
+hipError_t launch_dt_vjp_geo(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, GasT<double> g,
+                             const double* dt, const double* dtbar, double cfl, double* facc);
+// G.xbar from the metric adjoints in G.fbar / G.abar
+hipError_t launch_geo_to_nodes(hipStream_t s, CellsT<double> C, FacesT<double> F, const GeoBufs& G);
+
 // reverse of one explicit stage update qk_s+1 = q + W_s dt coef (real cells)
 hipError_t launch_rev_stage(hipStream_t s, const double* gk, const double* W, const double* dt, const double* area,
                             double coef, int32_t nC, int32_t nT, double* wseed, double* dtbar, double* qbar);

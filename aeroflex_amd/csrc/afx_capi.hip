@@ -1730,7 +1730,8 @@ int afx_rans_adjoint_nodes(afx_rans* h, const double* w, double* lambda, double*
 namespace {
 // get_wall_profile's CL and CD (post.h:301-387) at the device state q and the
 // gradient dCL/dq in the device layout (owner cells of the patch's faces)
-int wall_cl_seed(afx_rans* h, const char* patch, const double* qdev, double* cl, double* cd, std::vector<double>& seed) {
+int wall_cl_seed(afx_rans* h, const char* patch, const double* qdev, double* cl, double* cd, std::vector<double>& seed,
+                 std::vector<std::pair<uint32_t, double>>* face_cp = nullptr) {
     const Mesh& m = h->m;
     std::vector<int32_t> bsel, cells;
     for (int32_t b = 0; b < h->nB; ++b)
@@ -1765,6 +1766,7 @@ int wall_cl_seed(afx_rans* h, const char* patch, const double* qdev, double* cl,
         const double* qc = &q[k * NV];
         const double p = (gm - 1) * (qc[3] - 0.5 / qc[0] * (qc[1] * qc[1] + qc[2] * qc[2]));
         const double cp = 2. / (gm * mach * mach) * (p / p_inf - 1.);
+        if (face_cp) face_cp->push_back({e, cp});
         const double nx = m.edgesNormalsX[e], ny = m.edgesNormalsY[e], L = m.edgesLengths[e];
         fx += cp * nx * L / ext;
         fy += cp * ny * L / ext;
@@ -1870,6 +1872,126 @@ int afx_rans_dcl_dalpha(afx_rans* h, const char* patch, int32_t steps, int32_t e
     double d = -cd;
     for (int32_t j = 0; j < nfar; ++j) d += ga[(size_t)1 * nfar + j] * dfar[2 * j] + ga[(size_t)2 * nfar + j] * dfar[2 * j + 1];
     *dcl = d;
+    return AFX_OK;
+}
+
+int afx_rans_dcl_dnodes(afx_rans* h, const char* patch, int32_t steps, int32_t every, double relaxation, double* cl,
+                        double* xbar) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!patch || !cl || !xbar) return fail(AFX_EARG, "null argument");
+    if (steps < 1 || every < 1) return fail(AFX_EARG, "steps and the checkpoint interval must be >= 1");
+    if (h->sa()) return fail(AFX_EUNSUP, "dCL/dX covers the four-variable inviscid residual");
+    if ((rc = adjoint_check(h))) return rc;
+    if ((rc = geo_check(h))) return rc;
+    hipStream_t s = h->stream;
+    const int32_t nC = h->nC, nT = h->nT, nF = h->nF;
+    const size_t nq = (size_t)NV * nT;
+    const int nck = (steps + every - 1) / every;
+    DBuf<double> ck, seg, keepb, g, qb, lam, lamg, wseed, dtb, eb, facc, aacc;
+    HIPCHK(ck.alloc(nck * nq));
+    HIPCHK(seg.alloc((size_t)std::min(every, steps) * nq));
+    HIPCHK(keepb.alloc(5 * nq));
+    for (DBuf<double>* b : {&g, &qb, &lam, &lamg, &wseed}) HIPCHK(b->alloc(nq));
+    HIPCHK(dtb.alloc(nT));
+    HIPCHK(eb.alloc(nF));
+    HIPCHK(facc.alloc(8 * (size_t)nF));
+    HIPCHK(aacc.alloc(nC));
+    HIPCHK(hipMemsetAsync(wseed.p, 0, nq * 8, s));
+    HIPCHK(hipMemsetAsync(aacc.p, 0, (size_t)nC * 8, s));
+    double* keep[5];
+    for (int k = 0; k < 5; ++k) keep[k] = keepb.p + k * nq;
+
+    // forward: steps explicit steps of the handle's state, checkpoints every `every`
+    for (int32_t n = 0; n < steps; ++n) {
+        if (n % every == 0) HIPCHK(launch_copy(s, h->q.p, ck.p + (size_t)(n / every) * nq, (int64_t)nq));
+        if ((rc = explicit_step64(h, h->q.p, h->q.p, relaxation, nullptr))) return rc;
+    }
+    h->q32_valid = false;
+    std::vector<double> seed;
+    std::vector<std::pair<uint32_t, double>> fcp;
+    double cd = 0.0;
+    if ((rc = wall_cl_seed(h, patch, h->q.p, cl, &cd, seed, &fcp))) return rc;
+    HIPCHK(g.upload(seed, s));
+    // CL's own metric dependence (post.h:301-387): fx, fy = sum cp (n_x, n_y) len / ext
+    const Mesh& m = h->m;
+    double v4[4];
+    boundary_variables(h, v4);
+    const double sa = std::sin(v4[1]), ca = std::cos(v4[1]);
+    double xmin = 0, xmax = 0;
+    for (size_t k = 0; k < fcp.size(); ++k) {
+        const double x = m.edgesCentersX[fcp[k].first];
+        if (k == 0 || x < xmin) xmin = x;
+        if (k == 0 || x > xmax) xmax = x;
+    }
+    // faces at the extremes; a symmetric section has two at each (upper and
+    // lower surface), where the extent's subgradient is split evenly - the
+    // derivative central differences see
+    std::vector<uint32_t> emin, emax;
+    for (auto& fc : fcp) {
+        if (m.edgesCentersX[fc.first] == xmin) emin.push_back(fc.first);
+        if (m.edgesCentersX[fc.first] == xmax) emax.push_back(fc.first);
+    }
+    const double ext = xmax - xmin;
+    {
+        std::vector<double> f0(8 * (size_t)nF, 0.0);
+        for (auto& [e, cp] : fcp) {
+            const size_t f = (size_t)h->f_old2new[e];
+            const double L = m.edgesLengths[e], nx = m.edgesNormalsX[e], ny = m.edgesNormalsY[e];
+            f0[0 * (size_t)nF + f] += -sa * cp * L / ext;
+            f0[1 * (size_t)nF + f] += ca * cp * L / ext;
+            f0[2 * (size_t)nF + f] += (-sa * nx + ca * ny) * cp / ext;
+        }
+        HIPCHK(facc.upload(f0, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+
+    // reverse sweep as in afx_rans_dcl_dalpha, with each stage's metric
+    // adjoints and the time step's accumulated over all stages and steps
+    static const double alpha[3] = {0.25, 0.5, 1.};
+    AdjBufs A{h->gx.p, h->gy.p, h->lim.p, h->a_act.p, h->a_jmax.p, h->a_jmin.p, h->a_bL.p, h->a_bR.p,
+              h->a_lgx.p, h->a_lgy.p, h->a_nbv.p, h->a_nbj.p, h->a_co.p, h->a_cn.p};
+    GeoBufs G{h->g_fbar.p, h->g_abar.p, h->g_fnode.p, h->g_cnode.p, h->g_nodex.p, h->g_nodey.p, h->g_fend.p,
+              h->g_fcen.p, h->g_cvert.p, h->g_noff.p, h->g_nent.p, h->nN, h->g_xbar.p};
+    for (int k = nck - 1; k >= 0; --k) {
+        const int32_t start = k * every, end = std::min(steps, start + every);
+        HIPCHK(launch_copy(s, ck.p + (size_t)k * nq, seg.p, (int64_t)nq));
+        for (int32_t n = start; n < end - 1; ++n)
+            if ((rc = explicit_step64(h, seg.p + (size_t)(n - start) * nq, seg.p + (size_t)(n - start + 1) * nq,
+                                      relaxation, nullptr)))
+                return rc;
+        for (int32_t n = end - 1; n >= start; --n) {
+            double* qn = seg.p + (size_t)(n - start) * nq;
+            if ((rc = explicit_step64(h, qn, h->qk.p, relaxation, keep))) return rc;   // dt, W_s, qk_1, qk_2
+            HIPCHK(hipMemsetAsync(qb.p, 0, nq * 8, s));
+            HIPCHK(hipMemsetAsync(dtb.p, 0, (size_t)nT * 8, s));
+            for (int st = 2; st >= 0; --st) {
+                HIPCHK(launch_rev_stage(s, g.p, keep[st], h->dt.p, h->area.p, alpha[st] * relaxation, nC, nT,
+                                        wseed.p, dtb.p, qb.p));
+                const double* qs = st == 0 ? qn : keep[2 + st];
+                HIPCHK(launch_adjoint_stage_metrics(s, h->cells64(), h->faces64(), qn, qs, wseed.p, h->gas64(),
+                                                    h->opts.second_order, h->opts.limiter_k, A, lam.p, lamg.p, G));
+                HIPCHK(launch_acc_metrics(s, G, nF, nC, facc.p, aacc.p));
+                HIPCHK(launch_axpy4(s, lamg.p, 1.0, nC, nT, 0, qb.p));
+                HIPCHK(launch_axpy4(s, lam.p, 1.0, nC, nT, st > 0 ? 1 : 0, st > 0 ? g.p : qb.p));
+            }
+            HIPCHK(launch_dt_vjp(s, h->cells64(), h->faces64(), qn, h->gas64(), h->dt.p, dtb.p, h->cfl, eb.p, qb.p));
+            HIPCHK(launch_dt_vjp_geo(s, h->cells64(), h->faces64(), qn, h->gas64(), h->dt.p, dtb.p, h->cfl,
+                                     facc.p));
+            HIPCHK(launch_copy(s, qb.p, g.p, (int64_t)nq));
+        }
+    }
+    HIPCHK(launch_copy(s, facc.p, G.fbar, 8 * (int64_t)nF));
+    HIPCHK(launch_copy(s, aacc.p, G.abar, nC));
+    HIPCHK(launch_geo_to_nodes(s, h->cells64(), h->faces64(), G));
+    HIPCHK(hipMemcpyAsync(xbar, G.xbar, 2 * (size_t)h->nN * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // the chord extent: x of the extreme wall-face midpoints, CL proportional to 1 / ext
+    const double eb_ = -(*cl) / ext;
+    for (uint32_t e : emax)
+        for (int end = 0; end < 2; ++end) xbar[2 * (size_t)m.edgesNodes[2 * (size_t)e + end]] += 0.5 * eb_ / emax.size();
+    for (uint32_t e : emin)
+        for (int end = 0; end < 2; ++end) xbar[2 * (size_t)m.edgesNodes[2 * (size_t)e + end]] -= 0.5 * eb_ / emin.size();
     return AFX_OK;
 }
 

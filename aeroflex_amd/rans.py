@@ -113,6 +113,7 @@ def _load():
         "afx_rans_adjoint": (C.c_int, [P, dp, dp]),
         "afx_rans_adjoint_nodes": (C.c_int, [P, dp, dp, dp]),
         "afx_rans_dcl_dalpha": (C.c_int, [P, C.c_char_p, i32, i32, f64, dp, dp]),
+        "afx_rans_dcl_dnodes": (C.c_int, [P, C.c_char_p, i32, i32, f64, dp, dp]),
         "afx_rans_time_adjoint": (C.c_int, [P, i32, dp]),
         "afx_batch_create": (C.c_int, [i32, P, f64, C.POINTER(P)]),
         "afx_batch_run": (C.c_int, [P, i32, i32]),
@@ -522,6 +523,17 @@ class Solver:
         _check(lib.afx_rans_dcl_dalpha(self.h, patch.encode(), steps, checkpoint_every, relaxation, C.byref(cl),
                                        C.byref(d)))
         return cl.value, d.value
+
+    def dcl_dnodes(self, steps=300, checkpoint_every=25, relaxation=1.0, patch="wall"):
+        """Run `steps` explicit steps from the current state and return (CL,
+        dCL/dX) by reverse accumulation with checkpoints; dCL/dX has shape
+        (n_nodes, 2) in the mesh's node order."""
+        cl = C.c_double()
+        nn = int(self.mesh.counts()["nodes"])
+        xb = np.empty(2 * nn)
+        _check(lib.afx_rans_dcl_dnodes(self.h, patch.encode(), steps, checkpoint_every, relaxation, C.byref(cl),
+                                       _dptr(xb)))
+        return cl.value, xb.reshape(-1, 2)
 
     def time_adjoint(self, reps=20):
         ms = C.c_double()

@@ -260,6 +260,17 @@ int afx_rans_adjoint_nodes(afx_rans* h, const double* w, double* lambda, double*
  * steps and recomputed per segment.  Inviscid, Green-Gauss, four variables. */
 int afx_rans_dcl_dalpha(afx_rans* h, const char* patch, int32_t steps, int32_t checkpoint_every, double relaxation,
                         double* cl, double* dcl_dalpha);
+/* dCL/dX through pseudo-time (no counterpart in the reference): as
+ * afx_rans_dcl_dalpha, the derivative of CL after `steps` explicit steps
+ * with respect to the mesh-node coordinates, the starting state held fixed:
+ * xbar[2 i + d] for coordinate d of node i (reference node order).  It
+ * covers every metric of the residual (as afx_rans_adjoint_nodes), the
+ * local time step's face lengths and normals, and CL's own wall normals,
+ * lengths and chord extent (post.h:301-387); where several wall faces
+ * share the extreme midpoint x (the two surfaces of a symmetric section)
+ * the extent's derivative is split evenly among them. */
+int afx_rans_dcl_dnodes(afx_rans* h, const char* patch, int32_t steps, int32_t checkpoint_every, double relaxation,
+                        double* cl, double* xbar);
 int afx_rans_time_adjoint(afx_rans* h, int32_t reps, double* ms);
 int afx_rans_time_residual(afx_rans* h, int32_t precision, int32_t reps, double* ms);
 /* Batched explicit pseudo-time stepping of many solvers (conditions and/or

@@ -212,3 +212,71 @@ def test_dcl_dalpha_matches_central_differences(path, second_order):
     h = 1e-6
     fd = (cl_after_steps(s, q, angle + h, steps) - cl_after_steps(s, q, angle - h, steps)) / (2 * h)
     assert abs(d - fd) <= 1e-6 * abs(fd), (d, fd)
+
+
+# ----------------------------------------------------- dCL/dX through steps --
+def cl_after_steps_moved(el, dxy, q, steps, second_order, angle=2 * 0.01745, mach=0.5):
+    from aeroflex_amd import rans
+    x, y, cells, nverts, bsegs, bkeys, names = el
+    m = rans.Mesh.from_elements(x + dxy[:, 0], y + dxy[:, 1], cells, nverts, bsegs, bkeys, names)
+    s = rans.Solver(m, second_order=second_order, gradient_scheme="green-gauss", limiter_k=5.0)
+    far = rans.BoundaryVariables(mach=mach, angle=angle)
+    s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition("slip-wall")})
+    s.init()
+    s.set_cfl(1.0)
+    s.set_q(q)
+    s.refill_bcs()
+    for _ in range(steps):
+        s.explicit_solve(1.0)
+    return s.get_wall_profile("wall")[0]
+
+
+@pytest.mark.parametrize("path,second_order", [(QUAD, False), (QUAD, True), (TRI, True)])
+def test_dcl_dnodes_matches_central_differences(path, second_order):
+    """dCL/dX by reverse accumulation through 40 explicit steps with
+    checkpoints every 7 steps against central differences of CL after the
+    same 40 steps on meshes rebuilt from moved nodes (h = 1e-8 along a
+    random unit direction over all nodes, wall and farfield included, and
+    along the wall nodes only); the checkpoint interval does not change it.
+    At second order the limiter's min / max switches make larger steps
+    straddle kinks (on the triangle mesh along wall displacements: 2e-3
+    relative at h = 1e-6, 7e-4 at 1e-7, 7e-8 at 1e-8), hence the small h;
+    the bound adds the quotient's rounding, 50 eps |CL| / 2h."""
+    print(f"[adjoint] dcl/dX {os.path.basename(path)} second_order={second_order}", flush=True)
+    angle = 2 * 0.01745
+    s, o, q = make_pair(path, wall="slip-wall", second_order=second_order, angle=angle)
+    s.set_cfl(1.0)
+    steps = 40
+    s.set_q(q)
+    cl, xb = s.dcl_dnodes(steps=steps, checkpoint_every=7)
+    s.set_q(q)
+    cl1, xb1 = s.dcl_dnodes(steps=steps, checkpoint_every=steps)
+    assert cl1 == cl and np.array_equal(xb1, xb)
+    el = elements_of(s.mesh)
+    assert abs(cl_after_steps_moved(el, np.zeros_like(xb), q, steps, second_order) - cl) <= 1e-12 * abs(cl)
+    rng = np.random.default_rng(23)
+    on_wall = np.array(s.mesh.boundary_names()) == "wall"
+    wall_nodes = np.unique(el[4][on_wall].ravel())
+    # CL divides by the chord extent max - min of the wall-face midpoints; where
+    # a second face lies within 1e-6 of an extreme without tying it exactly (the
+    # triangle mesh: 7e-11 and 2e-9) the extent is not differentiable at the
+    # scale of h, so those faces' nodes stay put (exact ties are kept: the
+    # derivative splits evenly between them)
+    xc = s.mesh.get("edgesCentersX")[s.mesh.get("boundaryEdges")[on_wall]]
+    near = [(0 < abs(x - xc.max()) < 1e-6) or (0 < abs(x - xc.min()) < 1e-6) for x in xc]
+    extreme = (xc == xc.max()) | (xc == xc.min())
+    frozen = np.unique(el[4][on_wall][np.array(near) | (extreme & np.any(near))].ravel()) if any(near) else []
+    dirs = [rng.standard_normal(xb.shape)]
+    vw = np.zeros_like(xb)
+    vw[wall_nodes] = rng.standard_normal((len(wall_nodes), 2))
+    dirs.append(vw)
+    for V in dirs:
+        V[frozen] = 0.0
+    h = 1e-8
+    noise = 50 * 2.2e-16 * abs(cl) / (2 * h)
+    for V in dirs:
+        V = V / np.linalg.norm(V)
+        fd = (cl_after_steps_moved(el, h * V, q, steps, second_order)
+              - cl_after_steps_moved(el, -h * V, q, steps, second_order)) / (2 * h)
+        ad = float(np.sum(xb * V))
+        assert abs(ad - fd) <= 1e-6 * abs(fd) + noise, (ad, fd)
