@@ -125,6 +125,9 @@ public:
     solver(const mesh& m, const gas& g, std::string viscosity_model) : viscosity_model_(viscosity_model) {
         afx_default_opts(&opts_);
         opts_.viscosity = viscosity_code(viscosity_model);
+#ifdef RANS_MICHALAK_LIMITER
+        opts_.limiter = 1;   // the same compile-time switch as solver.h:557
+#endif
         afx_gas ag{g.R, g.mu_L, g.Pr_L, g.Pr_T, g.gamma};
         check(afx_rans_create(m.handle(), &ag, &opts_, &h_));
         nR_ = m.nRealCells();

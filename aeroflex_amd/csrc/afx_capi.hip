@@ -180,6 +180,10 @@ struct afx_rans {
     }
     bool sa() const { return nv == 5; }
     bool needs_gradients() const { return opts.second_order || opts.viscosity != 0; }
+    bool michalak() const { return opts.second_order && opts.limiter == 1; }
+    // second-order flag as the kernels take it: bit 0 second order, bit 1 the
+    // Michalak limiter build
+    int so() const { return opts.second_order ? (opts.limiter == 1 ? 3 : 1) : 0; }
     TilesF32 tiles32() const {
         return TilesF32{t_cell0.p, t_loff.p, t_ng.p, t_foff.p, t_nft.p, t_goff.p, t_Lglob.p, t_slots.p,
                         t_fck.p, t_geo4.p, tl.ntiles, tl.lmax, tl.gmax, tl.ftmax};
@@ -299,13 +303,13 @@ int ensure_q32(afx_rans* h) {
 int run_gradients64(afx_rans* h, const double* qg, const double* ql) {
     HIPCHK(launch_set_walls<double>(h->stream, h->w_owner.p, h->w_ghost.p, h->n_walls, const_cast<double*>(ql), h->nT, NV));
     HIPCHK(launch_grad_lim<double>(h->stream, h->cells64(), h->faces64(), qg, ql, h->gas64(), h->opts.gradient,
-                                   h->opts.second_order, h->opts.limiter_k, h->gx.p, h->gy.p, h->lim.p));
+                                   h->so(), h->opts.limiter_k, h->gx.p, h->gy.p, h->lim.p));
     return AFX_OK;
 }
 int run_gradients32(afx_rans* h) {
     HIPCHK(launch_set_walls<float>(h->stream, h->w_owner.p, h->w_ghost.p, h->n_walls, h->q32.p, h->nT, NV));
     HIPCHK(launch_grad_lim<float>(h->stream, h->cells32(), h->faces32(), h->q32.p, h->q32.p, h->gas32(),
-                                  h->opts.gradient, h->opts.second_order, (float)h->opts.limiter_k, h->gx32.p,
+                                  h->opts.gradient, h->so(), (float)h->opts.limiter_k, h->gx32.p,
                                   h->gy32.p, h->lim32.p));
     return AFX_OK;
 }
@@ -346,10 +350,10 @@ int enqueue_residual(afx_rans* h, int precision) {
     if (precision == AFX_FP64 && fused && h->tile64_ok) {
         HIPCHK(launch_residual_tile<double>(h->stream, h->tiles<double>(h->t_geo64.p), h->q.p, h->nT, h->nC, h->area.p,
                                            h->lsq.p, h->xc.p, h->yc.p, h->wall_d.p, h->gas64(), h->opts.gradient,
-                                           h->opts.second_order, h->opts.limiter_k, h->opts.viscosity, h->R.p, h->q.p));
+                                           h->so(), h->opts.limiter_k, h->opts.viscosity, h->R.p, h->q.p));
         return AFX_OK;
     }
-    if (precision == AFX_FP32 && fused && h->tile32_ok) {
+    if (precision == AFX_FP32 && fused && h->tile32_ok && !h->michalak()) {
         int rc = ensure_q32(h);
         if (rc) return rc;
         const double k = h->opts.limiter_k;
@@ -361,6 +365,8 @@ int enqueue_residual(afx_rans* h, int precision) {
         h->r32_aos = true;
         return AFX_OK;
     }
+    if (h->sa() && precision == AFX_FP32 && h->michalak())
+        return fail(AFX_EUNSUP, "the FP32 Spalart-Allmaras residual has no Michalak-limiter form");
     if (h->sa())
         return fail(AFX_EUNSUP, "the Spalart-Allmaras residual runs in the fused tile kernels only "
                                 "(renumber = 1, fused = 1, tiles within the LDS limit)");
@@ -738,6 +744,7 @@ int create_impl(const Mesh& mesh_in, const afx_gas* g, const afx_opts* o, const 
     if (opts.viscosity < 0 || opts.viscosity > 2)
         return fail(AFX_EUNSUP, "viscosity model must be 0 (inviscid), 1 (laminar) or 2 (Spalart-Allmaras)");
     if (opts.gradient != 0 && opts.gradient != 1) return fail(AFX_EARG, "gradient must be 0 (least-squares) or 1 (green-gauss)");
+    if (opts.limiter != 0 && opts.limiter != 1) return fail(AFX_EARG, "limiter must be 0 (default) or 1 (Michalak)");
     auto h = new afx_rans();
     try {
         h->m = mesh_in;
@@ -1078,9 +1085,11 @@ int afx_rans_set_options(afx_rans* h, const afx_opts* o) {
     if (rc) return rc;
     if (!o) return fail(AFX_EARG, "null options");
     if (o->gradient != 0 && o->gradient != 1) return fail(AFX_EARG, "gradient must be 0 (least-squares) or 1 (green-gauss)");
+    if (o->limiter != 0 && o->limiter != 1) return fail(AFX_EARG, "limiter must be 0 (default) or 1 (Michalak)");
     h->opts.second_order = o->second_order;
     h->opts.gradient = o->gradient;
     h->opts.limiter_k = o->limiter_k;
+    h->opts.limiter = o->limiter;
     return AFX_OK;
 }
 
@@ -1297,7 +1306,7 @@ int afx_rans_explicit_step(afx_rans* h, double relaxation, double* err) {
         for (double ai : alpha5) {
             HIPCHK(launch_residual_tile<double>(s, h->tiles<double>(h->t_geo64.p), h->qk.p, h->nT, h->nC, h->area.p,
                                                 h->lsq.p, h->xc.p, h->yc.p, h->wall_d.p, h->gas64(), h->opts.gradient,
-                                                h->opts.second_order, h->opts.limiter_k, h->opts.viscosity, h->R.p,
+                                                h->so(), h->opts.limiter_k, h->opts.viscosity, h->R.p,
                                                 h->qk.p));
             HIPCHK(launch_rk_update_area(s, h->q.p, h->R.p, h->area.p, h->dt.p, ai * relaxation, h->nC, h->nT, h->nv,
                                          h->qk.p, h->qW.p));
@@ -1621,6 +1630,7 @@ namespace {
 int adjoint_check(afx_rans* h) {
     if (h->opts.viscosity != 0) return fail(AFX_EUNSUP, "the state adjoint covers the inviscid residual");
     if (h->opts.gradient != 1) return fail(AFX_EUNSUP, "the state adjoint covers Green-Gauss gradients");
+    if (h->michalak()) return fail(AFX_EUNSUP, "the state adjoint covers the default limiter, not the Michalak build");
     if (!h->a_w.p) {
         const size_t c4 = (size_t)4 * h->nC, t4 = (size_t)4 * h->nT, f4 = (size_t)4 * h->nF;
         for (DBuf<double>* b : {&h->a_w, &h->a_lam, &h->a_lgx, &h->a_lgy}) HIPCHK(b->alloc(t4));
@@ -2123,6 +2133,8 @@ int afx_batch_create(int32_t n, afx_rans* const* members, double relaxation, afx
         int rc = check_handle(h);
         if (!rc && !(h->tile32_ok && h->opts.fused && h->iarea32.p))
             rc = fail(AFX_EUNSUP, "batch members need the fused tile kernels (renumber = 1, fused = 1)");
+        if (!rc && h->michalak())
+            rc = fail(AFX_EUNSUP, "batched stepping runs the default limiter (the FP32 fused kernel), not the Michalak build");
         if (!rc && i > 0 && h->sa() != members[0]->sa())
             rc = fail(AFX_EARG, "batch members must all use the same number of variables");
         if (!rc && h->sa() && !h->wd_ready) rc = compute_wall_distance(h);

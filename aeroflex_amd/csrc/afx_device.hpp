@@ -39,6 +39,45 @@ __device__ __forceinline__ T limiter_phi(T dqg, T dmax, T dmin, T K3a) {
     return (pos || neg) ? l : T(1.0);
 }
 
+// physics.h:583-592: michalak_limiter with RANS_YT = 2, the cubic
+// a y^3 + b y^2 + y joining phi(y_t) = 1 with zero slope (a = 0, b = -1/4)
+template <typename T>
+__device__ __forceinline__ T michalak(T y) {
+    constexpr double yt = 2.0;
+    constexpr double a = 1.0 / (yt * yt) - 2.0 / (yt * yt * yt);
+    constexpr double b = -3.0 / 2.0 * a * yt - 0.5 / yt;
+    return y >= T(yt) ? T(1.0) : T(a) * y * y * y + T(b) * y * y + y;
+}
+
+// solver.h:557-573, the RANS_MICHALAK_LIMITER build: a smooth switch sig
+// from 1 (range of the neighbours' values below K3a) to 0 (above 2 K3a)
+// blends the unlimited value with the Michalak limiter of the selected bound
+template <typename T>
+__device__ __forceinline__ T limiter_michalak(T dqg, T dmax, T dmin, T K3a) {
+    const T dd = (dmax - dmin) * (dmax - dmin);
+    T sig;
+    if (dd <= K3a) {
+        sig = T(1.0);
+    } else if (dd <= 2 * K3a) {
+        const T y = ddiv<T>(dd, K3a) - T(1.0);
+        sig = T(2.0) * y * y * y - T(3.0) * y * y + T(1.0);
+    } else {
+        sig = T(0.0);
+    }
+    T lim = T(1.0);
+    if (sig < T(1.0)) {
+        if (dqg > T(1e-14)) lim = michalak<T>(ddiv<T>(dmax, dqg));
+        else if (dqg < T(-1e-14)) lim = michalak<T>(ddiv<T>(dmin, dqg));
+    }
+    return sig + (T(1.0) - sig) * lim;
+}
+
+// the limiter of the selected build: second_order bit 1 set = Michalak
+template <typename T>
+__device__ __forceinline__ T limiter_value(int second_order, T dqg, T dmax, T dmin, T K3a) {
+    return (second_order & 2) ? limiter_michalak<T>(dqg, dmax, dmin, K3a) : limiter_phi<T>(dqg, dmax, dmin, K3a);
+}
+
 // physics.h:230-257: stresses and heat flux at the face from the face-averaged
 // state and the face gradients, with viscosity mu and conductivity kc as
 // arguments (laminar: mu_L, cp mu_L/Pr_L; SA: mu_L + mu_t, cp (mu_L/Pr_L + mu_t/Pr_T))
@@ -466,7 +505,7 @@ __global__ void __launch_bounds__(kBlock) k_grad_lim(CellsT<T> C, FacesT<T> F, c
             T dqg = sx[k] * dx + sy[k] * dy;
             T dmax = qmax[k] - qc[k];
             T dmin = qmin[k] - qc[k];
-            const T l = limiter_phi<T>(dqg, dmax, dmin, K3a);
+            const T l = limiter_value<T>(second_order, dqg, dmax, dmin, K3a);
             lm[k] = (l < lm[k]) ? l : lm[k];
         }
     }
@@ -717,7 +756,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
                     const T dqg = sx[k] * dx + sy[k] * dy;
                     const T dmax = qmax[k] - qc[k];
                     const T dmin = qmin[k] - qc[k];
-                    const T l = limiter_phi<T>(dqg, dmax, dmin, K3a);
+                    const T l = limiter_value<T>(second_order, dqg, dmax, dmin, K3a);
                     lm[k] = (l < lm[k]) ? l : lm[k];
                 }
             }

@@ -43,7 +43,7 @@ class AfxGas(C.Structure):
 class AfxOpts(C.Structure):
     _fields_ = [("viscosity", C.c_int32), ("second_order", C.c_int32), ("gradient", C.c_int32),
                 ("limiter_k", C.c_double), ("renumber", C.c_int32), ("tile_cells", C.c_int32),
-                ("fused", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("fused", C.c_int32), ("limiter", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 def _load():
@@ -269,13 +269,15 @@ class Mesh:
 # "spallart-allmaras" the one rans::Settings offers (core.h:176,224)
 VISCOSITY = {"inviscid": 0, "laminar": 1, "spalart-allmaras": 2, "spallart-allmaras": 2}
 GRADIENT = {"least-squares": 0, "green-gauss": 1}
+LIMITERS = {"default": 0, "michalak": 1}
 
 
 class Solver:
     """Device residual behind the rans::solver interface (solver.h:40-175)."""
 
     def __init__(self, mesh, gas=None, viscosity_model="inviscid", second_order=True,
-                 gradient_scheme="green-gauss", limiter_k=5.0, renumber=True, tile_cells=0, fused=True):
+                 gradient_scheme="green-gauss", limiter_k=5.0, renumber=True, tile_cells=0, fused=True,
+                 limiter="default"):
         if viscosity_model not in VISCOSITY:
             raise RansError(f"unsupported viscosity model '{viscosity_model}'")
         self.gas = gas if gas is not None else Gas()
@@ -289,6 +291,9 @@ class Solver:
         o.renumber = int(renumber)
         o.tile_cells = int(tile_cells)
         o.fused = int(fused)
+        if limiter not in LIMITERS:
+            raise RansError(f"unknown limiter '{limiter}' (default, michalak)")
+        o.limiter = LIMITERS[limiter]
         self._opts = o
         g = self.gas.c_struct()
         h = C.c_void_p()
@@ -309,6 +314,14 @@ class Solver:
 
     def set_gradient_scheme(self, s):
         self._opts.gradient = GRADIENT[s]
+        self._push_opts()
+
+    def set_limiter(self, name):
+        """The reference's limiter builds: "default" (solver.h:575-583) or
+        "michalak" (RANS_MICHALAK_LIMITER, solver.h:557-573)."""
+        if name not in LIMITERS:
+            raise RansError(f"unknown limiter '{name}' (default, michalak)")
+        self._opts.limiter = LIMITERS[name]
         self._push_opts()
 
     def set_limiter_k(self, k):

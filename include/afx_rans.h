@@ -71,7 +71,11 @@ typedef struct afx_opts {
     int32_t tile_cells;    /* cells per tile of the fused residual kernel (0: automatic, the
                               largest of 256, 224, ... whose FP64 tile fits 64 KiB of LDS) */
     int32_t fused;         /* 1: fused tile kernel for the residual (default), 0: three-kernel path */
-    int32_t reserved[5];
+    int32_t limiter;       /* 0: the reference's default limiter (solver.h:575-583, default);
+                              1: its RANS_MICHALAK_LIMITER build (solver.h:557-573, physics.h:583-592).
+                              With 1 the FP32 residual runs the three-kernel path; the adjoint,
+                              dCL entries and batched stepping return AFX_EUNSUP */
+    int32_t reserved[4];
 } afx_opts;
 
 const char* afx_last_error(void);
@@ -134,8 +138,9 @@ int afx_rans_create(const afx_mesh* m, const afx_gas* g, const afx_opts* o, afx_
  * from bcs.at(), solver.h:220). */
 int afx_rans_set_bcs(afx_rans* h, int32_t n, const char* const* names, const int32_t* types,
                      const double* farfield_vars /* 4*n: mach, angle, T, p */);
-/* set_second_order / set_gradient_scheme / set_limiter_k (solver.h:135,149-152,162);
- * viscosity and renumber are fixed at creation and ignored here */
+/* set_second_order / set_gradient_scheme / set_limiter_k (solver.h:135,149-152,162)
+ * and the limiter build; viscosity and renumber are fixed at creation and
+ * ignored here */
 int afx_rans_set_options(afx_rans* h, const afx_opts* o);
 int afx_rans_init(afx_rans* h);                 /* solver::init (solver.h:615-631) */
 int afx_rans_refill_bcs(afx_rans* h);           /* solver::refill_bcs (solver.h:259-274) */

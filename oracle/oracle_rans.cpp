@@ -518,6 +518,40 @@ static void face_flux(const FaceFlux& F, const double* q_L, const double* q_R,
     roe_flux(inv, q_L, qb, zero, zero, f);
 }
 
+// one (cell, face, variable) limiter value of calc_limiters (solver.h:545-584):
+// the default build (solver.h:575-583) or RANS_MICHALAK_LIMITER
+// (solver.h:557-573 with michalak_limiter, physics.h:580-592, RANS_YT 2)
+static double limiter_value(bool michalak, double dqg, double dmax, double dmin, double K3a) {
+    double lim = 1.0;
+    if (michalak) {
+        const double dMaxMin2 = (dmax - dmin) * (dmax - dmin);
+        double sig;
+        if (dMaxMin2 <= K3a) {
+            sig = 1.;
+        } else if (dMaxMin2 <= 2 * K3a) {
+            const double y = (dMaxMin2 / K3a - 1.0);
+            sig = 2.0 * y * y * y - 3.0 * y * y + 1.0;
+        } else {
+            sig = 0.;
+        }
+        if (sig < 1.0) {
+            constexpr double YT = 2.0;
+            constexpr double a = 1.0 / (YT * YT) - 2.0 / (YT * YT * YT);
+            constexpr double b = -3.0 / 2.0 * a * YT - 0.5 / YT;
+            auto mich = [&](double y) { return y >= YT ? 1.0 : a * y * y * y + b * y * y + y; };
+            if (dqg > 1e-14) lim = mich(dmax / dqg);
+            else if (dqg < -1e-14) lim = mich(dmin / dqg);
+            else lim = 1.0;
+        }
+        return sig + (1.0 - sig) * lim;
+    }
+    if (dqg > 1e-16)
+        lim = 1 / dqg * ((dmax * dmax + K3a) * dqg + 2 * dqg * dqg * dmax) / (dmax * dmax + 2 * dqg * dqg + dmax * dqg + K3a);
+    else if (dqg < -1e-16)
+        lim = 1 / dqg * ((dmin * dmin + K3a) * dqg + 2 * dqg * dqg * dmin) / (dmin * dmin + 2 * dqg * dqg + dmin * dqg + K3a);
+    return lim;
+}
+
 // ---------------------------------------------------------------- solver ----
 struct BcSpec { int type; BVars vars; };  // type: BC_FARFIELD / BC_SLIP / BC_WALL / -1 other
 
@@ -527,6 +561,7 @@ struct Solver {
     int viscous = 0;              // 0 inviscid, 1 laminar
     bool viscous_model_named_non_inviscid = false;  // viscosity_model != "inviscid"
     bool second_order = true;
+    bool michalak = false;        // RANS_MICHALAK_LIMITER build
     bool gg = true;               // green-gauss (else least-squares)
     double limiter_k = 5.;
     double cfl = 1;
@@ -747,11 +782,7 @@ struct Solver {
                     double dmin = q_min[4 * id + k] - q_[4 * id + k];
                     const double Ka = limiter_k * sqrt_area;
                     const double K3a = Ka * Ka * Ka;
-                    double lim = 1.0;
-                    if (dqg > 1e-16)
-                        lim = 1 / dqg * ((dmax * dmax + K3a) * dqg + 2 * dqg * dqg * dmax) / (dmax * dmax + 2 * dqg * dqg + dmax * dqg + K3a);
-                    else if (dqg < -1e-16)
-                        lim = 1 / dqg * ((dmin * dmin + K3a) * dqg + 2 * dqg * dqg * dmin) / (dmin * dmin + 2 * dqg * dqg + dmin * dqg + K3a);
+                    const double lim = limiter_value(michalak, dqg, dmax, dmin, K3a);
                     limiters[4 * id + k] = std::min(limiters[4 * id + k], lim);
                 }
             }
@@ -1152,11 +1183,7 @@ struct SASolver {
                 const double K3a = Ka * Ka * Ka;
                 const double dqg = gnx[id] * dx + gny[id] * dy;
                 const double dmax = mx[id] - nu[id], dmin = mn[id] - nu[id];
-                double lim = 1.0;
-                if (dqg > 1e-16)
-                    lim = 1 / dqg * ((dmax * dmax + K3a) * dqg + 2 * dqg * dqg * dmax) / (dmax * dmax + 2 * dqg * dqg + dmax * dqg + K3a);
-                else if (dqg < -1e-16)
-                    lim = 1 / dqg * ((dmin * dmin + K3a) * dqg + 2 * dqg * dqg * dmin) / (dmin * dmin + 2 * dqg * dqg + dmin * dqg + K3a);
+                const double lim = limiter_value(S.michalak, dqg, dmax, dmin, K3a);
                 lnu[id] = std::min(lnu[id], lim);
             }
         }
@@ -1534,6 +1561,7 @@ void orc_solver_wall_profile(void* h, const char* patch, double* out3) {
 }
 void orc_solver_face_jacobian(void* h, int e, double* J) { ((Solver*)h)->face_jacobian((u32)e, J); }
 void orc_solver_mf_apply(void* h, const double* x, double* y) { ((Solver*)h)->mf_apply(x, y); }
+void orc_solver_set_michalak(void* h, int on) { ((Solver*)h)->michalak = on != 0; }
 
 // multigrid<T>::gen_mapper (multigrid.h:96-163) on centroid / area arrays in
 // reference order (ghosts included): first the row scales over all coarse

@@ -32,13 +32,13 @@ def perturbed_state(n_real, n_cells, ghost_q, qinf, seed=2, amp=0.01):
 
 def make_pair(path=QUAD, viscosity="inviscid", wall="slip-wall", second_order=True, gradient="green-gauss",
               limiter_k=5.0, mach=0.5, angle=2 * 0.01745, mu_L=1e-5, seed=2, renumber=True, mesh=None, omesh=None,
-              fused=True, tile_cells=0):
+              fused=True, tile_cells=0, limiter="default"):
     from aeroflex_amd import rans
     m = mesh if mesh is not None else rans.Mesh.read_file(path)
     om = omesh if omesh is not None else orc.OMesh.read(path)
     gas = rans.Gas(mu_L=mu_L)
     s = rans.Solver(m, gas=gas, viscosity_model=viscosity, second_order=second_order, gradient_scheme=gradient,
-                    limiter_k=limiter_k, renumber=renumber, fused=fused, tile_cells=tile_cells)
+                    limiter_k=limiter_k, renumber=renumber, fused=fused, tile_cells=tile_cells, limiter=limiter)
     far = rans.BoundaryVariables(mach=mach, angle=angle)
     s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition(wall)})
     s.init()
@@ -50,6 +50,7 @@ def make_pair(path=QUAD, viscosity="inviscid", wall="slip-wall", second_order=Tr
     o = orc.OSolver(om, mu_L=mu_L, viscous=1 if viscosity == "laminar" else 0)
     o.set_bcs({"farfield": ("farfield", (mach, angle, 1.0, 1.0)), "wall": (wall, (0.2, 0.0, 1.0, 1.0))})
     o.options(second_order=second_order, green_gauss=gradient == "green-gauss", limiter_k=limiter_k, cfl=1.0)
+    o.set_michalak(1 if limiter == "michalak" else 0)
     o.vec("q", q)
     return s, o, q
 

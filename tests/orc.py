@@ -44,6 +44,7 @@ def _load():
         "orc_solver_wall_profile": (None, [P, C.c_char_p, dp]),
         "orc_solver_face_jacobian": (None, [P, C.c_int, dp]),
         "orc_solver_mf_apply": (None, [P, dp, dp]),
+        "orc_solver_set_michalak": (None, [P, C.c_int]),
         "orc_gen_mapper": (C.c_long, [C.c_int, dp, dp, C.c_int, dp, dp, dp, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_solver_fill_lhs": (C.c_long, [P, P, P, P]),
         "orc_solver_wall_dist": (None, [P]),

@@ -149,3 +149,56 @@ def test_fused_matches_three_kernel_path_fp32():
     R1, _ = s1.fillRhoRHS(rans.AFX_FP32)
     R2, _ = s2.fillRhoRHS(rans.AFX_FP32)
     assert np.linalg.norm(R1 - R2) / np.linalg.norm(R2) <= 1e-5
+
+
+# ------------------------------------------ RANS_MICHALAK_LIMITER build --
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("path,limiter_k", [(QUAD, 5.0), (TRI, 5.0), (TRI, 0.3)])
+def test_residual_fp64_michalak(path, limiter_k, fused):
+    """The Michalak limiter build (solver.h:557-573): FP64 residual, fused and
+    three-kernel paths, against the oracle; limiter_k 0.3 puts many cells in
+    the blended range of the switch."""
+    from aeroflex_amd import rans
+    s, o, q = make_pair(path, limiter_k=limiter_k, fused=fused, limiter="michalak")
+    R, _ = s.fillRhoRHS(rans.AFX_FP64)
+    o.fill_rhs()
+    Ro = o.vec("RhoVector")
+    assert rel_max(R, Ro) <= FP64_TOL
+    # and it differs from the default build
+    s.set_limiter("default")
+    R0, _ = s.fillRhoRHS(rans.AFX_FP64)
+    assert rel_max(R0, Ro) > 1e-6
+
+
+@pytest.mark.parametrize("path", [QUAD, TRI])
+def test_limiters_michalak_fp64(path):
+    from aeroflex_amd import rans
+    s, o, q = make_pair(path, limiter_k=0.3, limiter="michalak")
+    gx, gy, lim = s.gradients(rans.AFX_FP64)
+    o.set_walls()
+    o.calc_gradients()
+    o.calc_limiters()
+    nr = 4 * s.n_real
+    ol = o.vec("limiters")[:nr]
+    assert np.max(np.abs(lim[:nr] - ol)) <= 1e-12
+    assert np.min(ol) < 0.9 and np.mean((ol > 0) & (ol < 1)) > 0.05
+
+
+def test_residual_fp32_michalak_uses_the_three_kernel_path():
+    from aeroflex_amd import rans
+    s, o, q = make_pair(TRI, limiter_k=0.3, limiter="michalak")
+    R, _ = s.fillRhoRHS(rans.AFX_FP32)
+    o.fill_rhs()
+    Ro = o.vec("RhoVector")
+    assert np.linalg.norm(R - Ro) / np.linalg.norm(Ro) <= 1e-4
+
+
+def test_michalak_build_refuses_adjoint_and_batch():
+    from aeroflex_amd import rans
+    s, o, q = make_pair(QUAD, limiter="michalak")
+    w = np.zeros(4 * s.n_cells)
+    w[: 4 * s.n_real] = 1.0
+    with pytest.raises(rans.RansError, match="Michalak"):
+        s.adjoint(w)
+    with pytest.raises(rans.RansError, match="Michalak"):
+        rans.Batch([s])

@@ -662,3 +662,65 @@ def test_gen_mapper_rows_are_partitions_of_unity(coarse, fine):
     inside = rf < 0.98 * rc.max()
     assert nonempty[inside].all()
     assert not nonempty[rf > 1.1 * rc.max()].any()
+
+
+def michalak_setup(limiter_k):
+    m = orc.OMesh.read(TRI)
+    o = solver_on(m, limiter_k=limiter_k)
+    o.set_michalak(1)
+    c = m.counts()
+    nr = c["real_cells"]
+    o.init()
+    o.refill_bcs()
+    rng = np.random.default_rng(5)
+    q = o.vec("q")
+    q[: 4 * nr] *= 1 + 0.02 * rng.standard_normal(4 * nr)
+    o.vec("q", q)
+    o.set_walls()
+    o.calc_gradients()
+    o.calc_limiters()
+    return m, o, nr, rng
+
+
+def test_michalak_limiter_reduces_to_the_michalak_gooch_cubic_at_k0():
+    """RANS_MICHALAK_LIMITER (solver.h:557-573): with limiter_k = 0 the switch
+    sig is 0 wherever the neighbours' values differ, so each limiter is the
+    min over faces of the Michalak & Ollivier-Gooch (2008) cubic with y_t = 2:
+    the cubic through P(0) = 0 with P'(0) = 1 that meets 1 with zero slope at
+    y_t, worked out here from those four conditions, then 1 beyond y_t."""
+    m, o, nr, rng = michalak_setup(0.0)
+    # P(y) = y + b y^2 + a y^3, P(2) = 1, P'(2) = 0
+    a, b = np.linalg.solve([[8.0, 4.0], [12.0, 4.0]], [1.0 - 2.0, -1.0])
+    P = lambda y: 1.0 if y >= 2 else y + b * y * y + a * y ** 3
+    q = o.vec("q").reshape(-1, 4)
+    gx, gy = o.vec("gx").reshape(-1, 4), o.vec("gy").reshape(-1, 4)
+    lim = o.vec("limiters").reshape(-1, 4)
+    ce = m.get("cellsEdges").reshape(-1, 4)[:nr]
+    ec = m.get("edgesCells").reshape(-1, 2)
+    xf, yf = m.get("edgesCentersX"), m.get("edgesCentersY")
+    xc, yc = m.get("cellsCentersX"), m.get("cellsCentersY")
+    for i in rng.choice(nr, 400, replace=False):
+        fs = ce[i][ce[i] != 0xFFFFFFFF]
+        nb = np.where(ec[fs, 0] == i, ec[fs, 1], ec[fs, 0])
+        qmax = np.maximum(q[i], q[nb].max(0))
+        qmin = np.minimum(q[i], q[nb].min(0))
+        expect = np.ones(4)
+        for f in fs:
+            d = gx[i] * (xf[f] - xc[i]) + gy[i] * (yf[f] - yc[i])
+            for k in range(4):
+                if qmax[k] == qmin[k] or abs(d[k]) <= 1e-14:
+                    continue
+                D = (qmax[k] if d[k] > 0 else qmin[k]) - q[i, k]
+                expect[k] = min(expect[k], P(D / d[k]))
+        assert np.allclose(lim[i], expect, rtol=1e-12, atol=1e-14)
+
+
+def test_michalak_limiter_switch_is_off_for_smooth_ranges():
+    """With a limiter_k so large that (q_max - q_min)^2 <= (K sqrt(A))^3 in
+    every cell, sig = 1 and the build applies no limiting at all, unlike the
+    default limiter at the same K (which stays below 1 wherever dqg != 0)."""
+    m, o, nr, rng = michalak_setup(1e4)
+    assert np.array_equal(o.vec("limiters")[: 4 * nr], np.ones(4 * nr))
+    o.set_michalak(0)
+    o.calc_limiters()
+    assert np.min(o.vec("limiters")[: 4 * nr]) < 1.0
