@@ -757,7 +757,7 @@ struct Solver {
                 for (int k = 0; k < 4; ++k) { gx[4 * i + k] *= 0; gy[4 * i + k] *= 0; }
         }
     }
-    // solver.h:517-593 (default Venkatakrishnan-type branch, no RANS_MICHALAK_LIMITER)
+    // solver.h:517-593, the default branch or the RANS_MICHALAK_LIMITER build (limiter_value)
     void calc_limiters(const std::vector<double>& q_) {
         for (size_t i = 0; i < limiters.size(); ++i) { limiters[i] = 1; q_min[i] = q_[i]; q_max[i] = q_[i]; }
         for (u32 e = 0; e < m.nFaces(); ++e) {
