@@ -331,13 +331,17 @@ __global__ __launch_bounds__(128) void k_tri_inv(const float* __restrict__ A, in
 // Each workgroup takes a ticket; ticket t owns block row t (forward) or
 // nblk - 1 - t (backward), so a workgroup only waits on blocks owned by
 // workgroups that started before it.  It accumulates T(b, c) x_c over the
-// blocks c it depends on as their flags are published (the block of T is
-// loaded before the wait), applies the inverse of its diagonal block and
-// publishes x_b.  Every wait is bounded (abort flag, the grid still drains).
+// blocks c it depends on (the block of T is loaded before the wait), applies
+// the inverse of its diagonal block and publishes x_b.  Publication is one
+// 64-bit word per element, the value in the low half and the call's epoch in
+// the high half: a consumer thread polls the element it needs, so the value
+// arrives with its own flag in a single round trip (no separate flag, fence
+// or reload).  Every wait is bounded (abort flag, the grid still drains).
 template <bool LOWER>
 __global__ __launch_bounds__(256) void k_trsv(const float* __restrict__ A, int lda, const float* __restrict__ Tinv,
                                               float* __restrict__ y, int n, int nblk, unsigned* ticket,
-                                              unsigned base, unsigned* flags, unsigned epoch, unsigned* abort_flag) {
+                                              unsigned base, unsigned long long* yx, unsigned epoch,
+                                              unsigned* abort_flag) {
     __shared__ float xs[LU_NB];
     __shared__ float part[2][LU_NB];
     __shared__ int s_b;
@@ -361,20 +365,27 @@ __global__ __launch_bounds__(256) void k_trsv(const float* __restrict__ A, int l
             const int k = 2 * q + half;
             a[q] = (k < nc && r < nb) ? A[(size_t)(c0 + k) * lda + b0 + r] : 0.f;
         }
-        if (t == 0) {
-            unsigned spins = 0;
-            while (__hip_atomic_load(&flags[c], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-                if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
-                if (++spins > SPIN_LIMIT) {
-                    __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
+        if (t < LU_NB) {
+            float v = 0.f;
+            if (t < nc) {
+                unsigned spins = 0;
+                unsigned long long w = 0ull;
+                while (true) {
+                    w = __hip_atomic_load(&yx[c0 + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((unsigned)(w >> 32) == epoch) break;
+                    if ((spins & 31u) == 0u &&
+                        __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+                        break;
+                    if (++spins > SPIN_LIMIT) {
+                        __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
                 }
-                __builtin_amdgcn_s_sleep(1);
+                v = __uint_as_float((unsigned)(w & 0xFFFFFFFFull));
             }
+            xs[t] = v;
         }
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        if (t < LU_NB) xs[t] = t < nc ? y[c0 + t] : 0.f;
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < LU_NB / 2; ++q) acc += a[q] * xs[2 * q + half];
@@ -389,10 +400,12 @@ __global__ __launch_bounds__(256) void k_trsv(const float* __restrict__ A, int l
     for (int q = 0; q < LU_NB / 2; ++q) s += tinv[q] * xs[2 * q + half];
     part[half][r] = s;
     __syncthreads();
-    if (t < nb) y[b0 + t] = part[0][t] + part[1][t];
-    __threadfence();
-    __syncthreads();
-    if (t == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (t < nb) {
+        const float xv = part[0][t] + part[1][t];
+        y[b0 + t] = xv;
+        __hip_atomic_store(&yx[b0 + t], ((unsigned long long)epoch << 32) | (unsigned long long)__float_as_uint(xv),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // x += (double) y
@@ -712,7 +725,8 @@ struct afx_dense {
     unsigned* pabort = nullptr;
     int* info = nullptr;
     float *Linv = nullptr, *Uinv = nullptr;   // inverses of the 128 x 128 diagonal blocks
-    unsigned *tk = nullptr, *flags = nullptr, *abort = nullptr;   // triangular-solve tickets and block flags
+    unsigned *tk = nullptr, *abort = nullptr;   // triangular-solve tickets and abort flag
+    unsigned long long* yx = nullptr;            // published solution elements: value | epoch << 32
     unsigned tk_base = 0, epoch = 0;
     int nblk = 0;
     double anorm = -1.0;       // |A|inf of the current FP64 matrix (-1: not yet known)
@@ -721,7 +735,7 @@ struct afx_dense {
         for (void* p : {(void*)A64, (void*)A32, (void*)ipiv, (void*)perm, (void*)b, (void*)x, (void*)r,
                         (void*)rowabs, (void*)norms, (void*)y, (void*)keys, (void*)s_vals,
                         (void*)s_diag, (void*)pabort, (void*)info, (void*)Linv, (void*)Uinv, (void*)tk,
-                        (void*)flags, (void*)abort})
+                        (void*)yx, (void*)abort})
             if (p) (void)hipFree(p);
         if (rb) rocblas_destroy_handle(rb);
         if (s) (void)hipStreamDestroy(s);
@@ -764,10 +778,10 @@ int dense_init(afx_dense* h, int n) {
     VCHK(dalloc(&h->Linv, (size_t)h->nblk * LU_NB * LU_NB));
     VCHK(dalloc(&h->Uinv, (size_t)h->nblk * LU_NB * LU_NB));
     VCHK(dalloc(&h->tk, 1));
-    VCHK(dalloc(&h->flags, h->nblk));
+    VCHK(dalloc(&h->yx, h->n));
     VCHK(dalloc(&h->abort, 1));
     VCHK(hipMemsetAsync(h->tk, 0, sizeof(unsigned), h->s));
-    VCHK(hipMemsetAsync(h->flags, 0, h->nblk * sizeof(unsigned), h->s));
+    VCHK(hipMemsetAsync(h->yx, 0, (size_t)h->n * sizeof(unsigned long long), h->s));
     VCHK(hipMemsetAsync(h->abort, 0, sizeof(unsigned), h->s));
     VCHK(hipStreamSynchronize(h->s));
     return AFX_OK;
@@ -842,10 +856,10 @@ int dense_factor(afx_dense* h, int32_t* info_out) {
 int dense_lu_solve(afx_dense* h) {
     hipStream_t s = h->s;
     const int nb = h->nblk;
-    k_trsv<true><<<nb, 256, 0, s>>>(h->A32, h->lda, h->Linv, h->y, h->n, nb, h->tk, h->tk_base, h->flags,
+    k_trsv<true><<<nb, 256, 0, s>>>(h->A32, h->lda, h->Linv, h->y, h->n, nb, h->tk, h->tk_base, h->yx,
                                     ++h->epoch, h->abort);
     h->tk_base += (unsigned)nb;
-    k_trsv<false><<<nb, 256, 0, s>>>(h->A32, h->lda, h->Uinv, h->y, h->n, nb, h->tk, h->tk_base, h->flags,
+    k_trsv<false><<<nb, 256, 0, s>>>(h->A32, h->lda, h->Uinv, h->y, h->n, nb, h->tk, h->tk_base, h->yx,
                                      ++h->epoch, h->abort);
     h->tk_base += (unsigned)nb;
     VCHK(hipGetLastError());
