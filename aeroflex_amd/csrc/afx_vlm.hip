@@ -86,9 +86,25 @@ __device__ inline void wave_argmax(float& v, int& i) {
 // slower workgroup still reads for column j (it cannot post j + 2 before
 // every workgroup has posted j + 1).  Every wait is bounded: a workgroup that
 // polls longer than SPIN_LIMIT raises `abort` and every workgroup returns.
+__device__ inline unsigned long long tagged(unsigned seq, float v) {
+    return ((unsigned long long)seq << 32) | (unsigned long long)__float_as_uint(v);
+}
+// wait (bounded) for a value | sequence word of this sequence; false on timeout
+__device__ inline bool wait_tagged(const unsigned long long* p, unsigned seq, float& v) {
+    unsigned spins = 0;
+    unsigned long long w;
+    while (((w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 32) != seq) {
+        if (++spins > SPIN_LIMIT) { v = 0.f; return false; }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    v = __uint_as_float((unsigned)(w & 0xFFFFFFFFull));
+    return true;
+}
+
 __global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda, int m, int nbc, int j0,
                                                   int* __restrict__ ipiv, unsigned long long* keys,
-                                                  float* __restrict__ s_vals, float* __restrict__ s_diag,
+                                                  unsigned long long* __restrict__ s_vals,
+                                                  unsigned long long* __restrict__ s_diag,
                                                   unsigned* abort_flag, int* __restrict__ info) {
     __shared__ float slab[LU_NB * PANEL_R];   // panel column c at slab[c * PANEL_R]
     __shared__ float urow[LU_NB];
@@ -125,14 +141,17 @@ __global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda
         }
         __syncthreads();
         const int lrow = s_lrow;
+        // candidate (and diagonal) rows as value | sequence words: a reader
+        // checks each word's sequence, so no fence orders them before the key
         if (lrow >= 0)
             for (int c = t; c < nbc; c += 256)
-                s_vals[((size_t)par * MAX_PANEL_WG + w) * LU_NB + c] = slab[c * PANEL_R + lrow];
+                __hip_atomic_store(&s_vals[((size_t)par * MAX_PANEL_WG + w) * LU_NB + c],
+                                   tagged(seq, slab[c * PANEL_R + lrow]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int jw = j / PANEL_R;
         if (w == jw)
-            for (int c = t; c < nbc; c += 256) s_diag[par * LU_NB + c] = slab[c * PANEL_R + (j - rbase)];
-        __threadfence();
-        __syncthreads();
+            for (int c = t; c < nbc; c += 256)
+                __hip_atomic_store(&s_diag[par * LU_NB + c], tagged(seq, slab[c * PANEL_R + (j - rbase)]),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (t == 0) {
             const unsigned long long key =
                 lrow >= 0 ? make_key(rv[0], (unsigned)ri[0], seq) : make_key(0.f, NO_ROW, seq);
@@ -163,7 +182,6 @@ __global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         wave_argmax(best, bi);
         if ((t & 63) == 0) { rv[t >> 6] = best; ri[t >> 6] = bi; }
         __syncthreads();
@@ -174,25 +192,38 @@ __global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda
         __syncthreads();
         const int p = s_win;                 // panel-relative pivot row, >= j
         const int pw = p / PANEL_R;
-        for (int c = t; c < nbc; c += 256) urow[c] = s_vals[((size_t)par * MAX_PANEL_WG + pw) * LU_NB + c];
-        if (w == 0 && t == 0) ipiv[j0 + j] = j0 + p;
-        __syncthreads();
-        if (p != j) {
-            if (w == pw)
-                for (int c = t; c < nbc; c += 256) slab[c * PANEL_R + (p - rbase)] = s_diag[par * LU_NB + c];
-            if (w == jw)
-                for (int c = t; c < nbc; c += 256) slab[c * PANEL_R + (j - rbase)] = urow[c];
+        int late = 0;
+        for (int c = t; c < nbc; c += 256) {
+            float v;
+            late |= !wait_tagged(&s_vals[((size_t)par * MAX_PANEL_WG + pw) * LU_NB + c], seq, v);
+            urow[c] = v;
         }
+        if (w == 0 && t == 0) ipiv[j0 + j] = j0 + p;
+        if (p != j && w == pw)
+            for (int c = t; c < nbc; c += 256) {
+                float v;
+                late |= !wait_tagged(&s_diag[par * LU_NB + c], seq, v);
+                slab[c * PANEL_R + (p - rbase)] = v;
+            }
+        if (__syncthreads_or(late)) {
+            if (t == 0) __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        if (p != j && w == jw)
+            for (int c = t; c < nbc; c += 256) slab[c * PANEL_R + (j - rbase)] = urow[c];
         __syncthreads();
         const float piv = urow[j];
         if (piv != 0.f) {
             if (t < rows && rbase + t > j) slab[j * PANEL_R + t] /= piv;
             __syncthreads();
-            const int ncols = nbc - j - 1;
-            for (int idx = t; idx < ncols * PANEL_R; idx += 256) {
-                const int cc = idx / PANEL_R, r = idx - cc * PANEL_R;
-                const int c = j + 1 + cc;
-                if (r < rows && rbase + r > j) slab[c * PANEL_R + r] -= slab[j * PANEL_R + r] * urow[c];
+            // rank-1 update: two threads per row, the multiplier in a register
+            // and the pivot row broadcast from LDS, columns interleaved
+            if (t < 2 * PANEL_R) {
+                const int r = t < PANEL_R ? t : t - PANEL_R, h2 = t < PANEL_R ? 0 : 1;
+                if (r < rows && rbase + r > j) {
+                    const float l = slab[j * PANEL_R + r];
+                    for (int c = j + 1 + h2; c < nbc; c += 2) slab[c * PANEL_R + r] -= l * urow[c];
+                }
             }
         } else if (w == 0 && t == 0 && *info == 0) {
             *info = j0 + j + 1;
@@ -693,9 +724,15 @@ __global__ void k_forces_stations(const double* __restrict__ corner, const doubl
     for (int d = 0; d < 3; ++d) sforce[3 * s + d] = F[d];
 }
 
-__global__ void k_panel_reset(unsigned long long* keys, unsigned* abort_flag, int* info) {
-    for (int q = threadIdx.x; q < 2 * MAX_PANEL_WG; q += blockDim.x) keys[q] = 0ull;
-    if (threadIdx.x == 0) {
+__global__ void k_panel_reset(unsigned long long* keys, unsigned long long* s_vals, unsigned long long* s_diag,
+                              unsigned* abort_flag, int* info) {
+    const int stride = gridDim.x * blockDim.x, i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int q = i0; q < 2 * MAX_PANEL_WG; q += stride) keys[q] = 0ull;
+    // the row words carry the column sequence j0 + j + 1, which repeats from one
+    // factorisation to the next: clear them with the keys
+    for (int q = i0; q < 2 * MAX_PANEL_WG * LU_NB; q += stride) s_vals[q] = 0ull;
+    for (int q = i0; q < 2 * LU_NB; q += stride) s_diag[q] = 0ull;
+    if (i0 == 0) {
         *abort_flag = 0u;
         *info = 0;
     }
@@ -720,8 +757,8 @@ struct afx_dense {
     double *b = nullptr, *x = nullptr, *r = nullptr, *rowabs = nullptr, *norms = nullptr;
     float* y = nullptr;
     unsigned long long* keys = nullptr;
-    float* s_vals = nullptr;
-    float* s_diag = nullptr;
+    unsigned long long* s_vals = nullptr;   // candidate rows, value | column sequence
+    unsigned long long* s_diag = nullptr;
     unsigned* pabort = nullptr;
     int* info = nullptr;
     float *Linv = nullptr, *Uinv = nullptr;   // inverses of the 128 x 128 diagonal blocks
@@ -794,7 +831,7 @@ int dense_factor(afx_dense* h, int32_t* info_out) {
     dim3 tg((n + 63) / 64, (n + 63) / 64);
     k_to_cm32<<<tg, 256, 0, s>>>(h->A64, n, h->A32, lda);
     VCHK(hipGetLastError());
-    k_panel_reset<<<1, 256, 0, s>>>(h->keys, h->pabort, h->info);
+    k_panel_reset<<<64, 256, 0, s>>>(h->keys, h->s_vals, h->s_diag, h->pabort, h->info);
     VCHK(hipGetLastError());
     for (int j0 = 0; j0 < n; j0 += LU_NB) {
         const int nbc = std::min(LU_NB, n - j0);
