@@ -64,13 +64,40 @@ __device__ inline void argmax_combine(float& v, int& i, float ov, int oi) {
     if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
 }
 
-__device__ inline void wave_argmax(float& v, int& i) {
-    for (int off = 32; off > 0; off >>= 1) {
-        const float ov = __shfl_xor(v, off);
-        const int oi = __shfl_xor(i, off);
-        argmax_combine(v, i, ov, oi);
-    }
+// (|a|, row) as one ordered 64-bit key: larger |a| first, then the lower
+// row; |a| < 0 (no candidate) maps to 0.  |a| >= 0 (or +inf) orders as its bits.
+__device__ inline unsigned long long argmax_key(float v, int i) {
+    return v < 0.f ? 0ull : ((unsigned long long)__float_as_uint(v) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)i);
 }
+__device__ inline void argmax_unkey(unsigned long long k, float& v, int& i) {
+    v = k ? __uint_as_float((unsigned)(k >> 32)) : -1.f;
+    i = k ? (int)(0xFFFFFFFFu - (unsigned)(k & 0xFFFFFFFFull)) : INT_MAX;
+}
+// one step of a wavefront max of 64-bit keys through DPP lane moves
+template <int CTRL, int ROW_MASK>
+__device__ inline unsigned long long dpp_max_step(unsigned long long k) {
+    const int lo = (int)(unsigned)k, hi = (int)(unsigned)(k >> 32);
+    const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xF, false);
+    const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xF, false);
+    const unsigned long long o = ((unsigned long long)(unsigned)ohi << 32) | (unsigned long long)(unsigned)olo;
+    return o > k ? o : k;
+}
+// maximum over the wavefront's 64 lanes, returned in every lane: pairs and
+// quads (quad_perm), half rows and rows (mirrors), then rows 1 and 3 from
+// lane 15 of the row below (row_bcast15) and rows 2 and 3 from lane 31
+// (row_bcast31); lane 63 then holds the maximum
+__device__ inline unsigned long long wave_max_key(unsigned long long k) {
+    k = dpp_max_step<0xB1, 0xF>(k);    // quad_perm [1,0,3,2]
+    k = dpp_max_step<0x4E, 0xF>(k);    // quad_perm [2,3,0,1]
+    k = dpp_max_step<0x141, 0xF>(k);   // row_half_mirror
+    k = dpp_max_step<0x140, 0xF>(k);   // row_mirror
+    k = dpp_max_step<0x142, 0xA>(k);   // row_bcast15
+    k = dpp_max_step<0x143, 0xC>(k);   // row_bcast31
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)k, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(k >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ inline void wave_argmax(float& v, int& i) { argmax_unkey(wave_max_key(argmax_key(v, i)), v, i); }
 
 // --------------------------------------------------- panel factorisation --
 // Unblocked LU with partial pivoting of the m x nbc panel whose top-left
@@ -222,6 +249,7 @@ __global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda
                 const int r = t < PANEL_R ? t : t - PANEL_R, h2 = t < PANEL_R ? 0 : 1;
                 if (r < rows && rbase + r > j) {
                     const float l = slab[j * PANEL_R + r];
+#pragma unroll 4
                     for (int c = j + 1 + h2; c < nbc; c += 2) slab[c * PANEL_R + r] -= l * urow[c];
                 }
             }
