@@ -108,10 +108,16 @@ __device__ inline void wave_argmax(float& v, int& i) { argmax_unkey(wave_max_key
 // polls the keys of all workgroups: once every key carries this column's
 // sequence, all workgroups know the same pivot without any counter, read the
 // pivot row, perform the interchange on the rows they own, scale their part
-// of the column and update their part of the panel.  Keys and rows alternate
-// between two buffers, so posting for column j + 1 cannot overwrite what a
-// slower workgroup still reads for column j (it cannot post j + 2 before
-// every workgroup has posted j + 1).  Every wait is bounded: a workgroup that
+// of the column and update their part of the panel.  The update is ordered so
+// that the next column's key leaves early: column j + 1 is updated first, its
+// candidate found, the two rows to be posted (candidate and diagonal) finish
+// step j, the key is posted, and only then does the rest of the rank-1 update
+// run, while the other workgroups' keys are in flight.  Every element still
+// takes the same operations as in the plain right-looking order.  Keys and
+// rows alternate between two buffers, so posting for column j + 1 cannot
+// overwrite what a slower workgroup still reads for column j (it cannot post
+// j + 2 before every workgroup has posted j + 1, which each does only after
+// reading column j's pivot row).  Every wait is bounded: a workgroup that
 // polls longer than SPIN_LIMIT raises `abort` and every workgroup returns.
 __device__ inline unsigned long long tagged(unsigned seq, float v) {
     return ((unsigned long long)seq << 32) | (unsigned long long)__float_as_uint(v);
@@ -126,6 +132,21 @@ __device__ inline bool wait_tagged(const unsigned long long* p, unsigned seq, fl
     }
     v = __uint_as_float((unsigned)(w & 0xFFFFFFFFull));
     return true;
+}
+
+// the workgroup's candidate (largest |a|, lowest row) from one (|a|, row) per
+// thread; leaves it in rv[0], ri[0] and returns its workgroup-local row, or -1
+__device__ inline int block_candidate(float best, int bi, float* rv, int* ri, int* s_lrow, int rbase) {
+    const int t = threadIdx.x;
+    wave_argmax(best, bi);
+    if ((t & 63) == 0) { rv[t >> 6] = best; ri[t >> 6] = bi; }
+    __syncthreads();
+    if (t == 0) {
+        for (int q = 1; q < 4; ++q) argmax_combine(rv[0], ri[0], rv[q], ri[q]);
+        *s_lrow = ri[0] != INT_MAX ? ri[0] - rbase : -1;
+    }
+    __syncthreads();
+    return *s_lrow;
 }
 
 __global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda, int m, int nbc, int j0,
@@ -149,33 +170,18 @@ __global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda
     }
     __syncthreads();
 
-    for (int j = 0; j < nbc; ++j) {
+    // column j's candidate row (local row lrow, or none) and, from the
+    // workgroup holding panel row j, the diagonal row, as value | sequence
+    // words (a reader checks each word's sequence, so no fence orders them
+    // before the key), then the key
+    auto post = [&](int j, int lrow) {
         const int par = j & 1;
         const unsigned seq = (unsigned)(j0 + j + 1);
-        float best = -1.f;
-        int bi = INT_MAX;
-        if (t < rows && rbase + t >= j) {
-            best = fabsf(slab[j * PANEL_R + t]);
-            if (!(best == best)) best = INFINITY;   // a NaN entry is taken as the largest
-            bi = rbase + t;
-        }
-        wave_argmax(best, bi);
-        if ((t & 63) == 0) { rv[t >> 6] = best; ri[t >> 6] = bi; }
-        __syncthreads();
-        if (t == 0) {
-            for (int q = 1; q < 4; ++q) argmax_combine(rv[0], ri[0], rv[q], ri[q]);
-            s_lrow = ri[0] != INT_MAX ? ri[0] - rbase : -1;
-        }
-        __syncthreads();
-        const int lrow = s_lrow;
-        // candidate (and diagonal) rows as value | sequence words: a reader
-        // checks each word's sequence, so no fence orders them before the key
         if (lrow >= 0)
             for (int c = t; c < nbc; c += 256)
                 __hip_atomic_store(&s_vals[((size_t)par * MAX_PANEL_WG + w) * LU_NB + c],
                                    tagged(seq, slab[c * PANEL_R + lrow]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const int jw = j / PANEL_R;
-        if (w == jw)
+        if (w == j / PANEL_R)
             for (int c = t; c < nbc; c += 256)
                 __hip_atomic_store(&s_diag[par * LU_NB + c], tagged(seq, slab[c * PANEL_R + (j - rbase)]),
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -184,8 +190,22 @@ __global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda
                 lrow >= 0 ? make_key(rv[0], (unsigned)ri[0], seq) : make_key(0.f, NO_ROW, seq);
             __hip_atomic_store(&keys[par * MAX_PANEL_WG + w], key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+    };
+    // |a| of a candidate entry; a NaN is taken as the largest
+    auto mag = [](float a) { const float v = fabsf(a); return v == v ? v : INFINITY; };
+
+    {
+        const bool live = t < rows;
+        post(0, block_candidate(live ? mag(slab[t]) : -1.f, live ? rbase + t : INT_MAX, rv, ri, &s_lrow, rbase));
+    }
+
+    for (int j = 0; j < nbc; ++j) {
+        const int par = j & 1;
+        const unsigned seq = (unsigned)(j0 + j + 1);
 
         // gather: every key of this column's sequence
+        float best = -1.f;
+        int bi = INT_MAX;
         unsigned spins = 0;
         for (;;) {
             int ok = 1;
@@ -218,7 +238,7 @@ __global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda
         }
         __syncthreads();
         const int p = s_win;                 // panel-relative pivot row, >= j
-        const int pw = p / PANEL_R;
+        const int pw = p / PANEL_R, jw = j / PANEL_R;
         int late = 0;
         for (int c = t; c < nbc; c += 256) {
             float v;
@@ -240,24 +260,49 @@ __global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda
             for (int c = t; c < nbc; c += 256) slab[c * PANEL_R + (j - rbase)] = urow[c];
         __syncthreads();
         const float piv = urow[j];
-        if (piv != 0.f) {
-            if (t < rows && rbase + t > j) slab[j * PANEL_R + t] /= piv;
-            __syncthreads();
-            // rank-1 update: two threads per row, the multiplier in a register
-            // and the pivot row broadcast from LDS, columns interleaved
-            if (t < 2 * PANEL_R) {
-                const int r = t < PANEL_R ? t : t - PANEL_R, h2 = t < PANEL_R ? 0 : 1;
-                if (r < rows && rbase + r > j) {
-                    const float l = slab[j * PANEL_R + r];
-#pragma unroll 4
-                    for (int c = j + 1 + h2; c < nbc; c += 2) slab[c * PANEL_R + r] -= l * urow[c];
-                }
+        const bool upd = piv != 0.f;
+        if (!upd && w == 0 && t == 0 && *info == 0) *info = j0 + j + 1;
+        const bool below = t < rows && rbase + t > j;   // rows that step j updates
+        if (j + 1 == nbc) {
+            if (upd && below) slab[j * PANEL_R + t] /= piv;
+            break;
+        }
+        // one thread per row: scale column j, update column j + 1, and offer
+        // the result as column j + 1's candidate
+        float c1 = -1.f;
+        if (below) {
+            float a1 = slab[(j + 1) * PANEL_R + t];
+            if (upd) {
+                const float l = slab[j * PANEL_R + t] / piv;
+                slab[j * PANEL_R + t] = l;
+                a1 -= l * urow[j + 1];
+                slab[(j + 1) * PANEL_R + t] = a1;
             }
-        } else if (w == 0 && t == 0 && *info == 0) {
-            *info = j0 + j + 1;
+            c1 = mag(a1);
+        }
+        const int lrow1 = block_candidate(c1, below ? rbase + t : INT_MAX, rv, ri, &s_lrow, rbase);
+        // the rows posted for column j + 1 finish step j first
+        const int drow1 = w == (j + 1) / PANEL_R ? j + 1 - rbase : -1;
+        if (upd)
+            for (int c = j + 2 + t; c < nbc; c += 256) {
+                if (lrow1 >= 0) slab[c * PANEL_R + lrow1] -= slab[j * PANEL_R + lrow1] * urow[c];
+                if (drow1 >= 0 && drow1 != lrow1) slab[c * PANEL_R + drow1] -= slab[j * PANEL_R + drow1] * urow[c];
+            }
+        __syncthreads();
+        post(j + 1, lrow1);
+        // the rest of the rank-1 update: two threads per row, the multiplier
+        // in a register and the pivot row broadcast from LDS, columns interleaved
+        if (upd && t < 2 * PANEL_R) {
+            const int r = t < PANEL_R ? t : t - PANEL_R, h2 = t < PANEL_R ? 0 : 1;
+            if (r < rows && rbase + r > j && r != lrow1 && r != drow1) {
+                const float l = slab[j * PANEL_R + r];
+#pragma unroll 4
+                for (int c = j + 2 + h2; c < nbc; c += 2) slab[c * PANEL_R + r] -= l * urow[c];
+            }
         }
         __syncthreads();
     }
+    __syncthreads();
 
     for (int idx = t; idx < nbc * PANEL_R; idx += 256) {
         const int c = idx / PANEL_R, r = idx - c * PANEL_R;
@@ -288,8 +333,11 @@ __device__ inline float readlane_f(float v, int lane) {
 }
 
 // A(j0:j0+nbc, j0+nbc:n) <- L11^-1 A(j0:j0+nbc, j0+nbc:n), L11 unit lower
-// (strsm L,L,N,U).  One wavefront per column, two rows per lane, L11 in LDS.
-constexpr int TRSM_CPW = 4;   // columns per wavefront
+// (strsm L,L,N,U).  Each wavefront solves TRSM_CPW columns together, two rows
+// per lane, L11 in LDS: the columns' substitution chains are independent, so
+// interleaving them inside the k loop hides the latency of each step
+// (readlane of x_k after its last update) behind the other columns' work.
+constexpr int TRSM_CPW = 8;   // columns per wavefront
 __global__ __launch_bounds__(256) void k_trsm_llnu(float* __restrict__ A, int lda, int n, int j0, int nbc) {
     __shared__ float Ls[LU_NB * LU_NB];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -299,19 +347,39 @@ __global__ __launch_bounds__(256) void k_trsm_llnu(float* __restrict__ A, int ld
     }
     __syncthreads();
     const int c0 = j0 + nbc + (blockIdx.x * 4 + wv) * TRSM_CPW;
+    float b0[TRSM_CPW], b1[TRSM_CPW];
+#pragma unroll
     for (int q = 0; q < TRSM_CPW; ++q) {
-        const int col = c0 + q;
-        if (col >= n) break;
-        float* cp = A + (size_t)col * lda + j0;
-        float b0 = lane < nbc ? cp[lane] : 0.f;
-        float b1 = lane + 64 < nbc ? cp[lane + 64] : 0.f;
-        for (int k = 0; k < nbc; ++k) {
-            const float xk = k < 64 ? readlane_f(b0, k) : readlane_f(b1, k - 64);
-            if (lane > k && lane < nbc) b0 -= Ls[k * LU_NB + lane] * xk;
-            if (lane + 64 > k && lane + 64 < nbc) b1 -= Ls[k * LU_NB + lane + 64] * xk;
+        const float* cp = A + (size_t)(c0 + q) * lda + j0;
+        const bool live = c0 + q < n;
+        b0[q] = live && lane < nbc ? cp[lane] : 0.f;
+        b1[q] = live && lane + 64 < nbc ? cp[lane + 64] : 0.f;
+    }
+    for (int k = 0; k < nbc; ++k) {
+        // only rows r > k (and r < nbc) take -L(r, k) x_k
+        const bool u0 = lane > k && lane < nbc, u1 = lane + 64 > k && lane + 64 < nbc;
+        const float l0 = Ls[k * LU_NB + lane], l1 = Ls[k * LU_NB + lane + 64];
+        if (k < 64) {
+#pragma unroll
+            for (int q = 0; q < TRSM_CPW; ++q) {
+                const float xk = readlane_f(b0[q], k);
+                b0[q] = u0 ? b0[q] - l0 * xk : b0[q];
+                b1[q] = u1 ? b1[q] - l1 * xk : b1[q];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < TRSM_CPW; ++q) {
+                const float xk = readlane_f(b1[q], k - 64);
+                b1[q] = u1 ? b1[q] - l1 * xk : b1[q];
+            }
         }
-        if (lane < nbc) cp[lane] = b0;
-        if (lane + 64 < nbc) cp[lane + 64] = b1;
+    }
+#pragma unroll
+    for (int q = 0; q < TRSM_CPW; ++q) {
+        if (c0 + q >= n) break;
+        float* cp = A + (size_t)(c0 + q) * lda + j0;
+        if (lane < nbc) cp[lane] = b0[q];
+        if (lane + 64 < nbc) cp[lane + 64] = b1[q];
     }
 }
 
