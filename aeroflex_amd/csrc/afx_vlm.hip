@@ -310,13 +310,12 @@ __global__ __launch_bounds__(256) void k_panel_lu(float* __restrict__ A, int lda
     }
 }
 
-// row interchanges of the panel [j0, j0 + nbc) applied to every column
-// outside it (LAPACK slaswp), one thread per column
-__global__ __launch_bounds__(256) void k_laswp(float* __restrict__ A, int lda, int n, int j0, int nbc,
+// row interchanges of the panel [j0, j0 + nbc) applied to the columns
+// [cbeg, cend) (LAPACK slaswp), one thread per column
+__global__ __launch_bounds__(256) void k_laswp(float* __restrict__ A, int lda, int cbeg, int cend, int j0, int nbc,
                                                const int* __restrict__ ipiv) {
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    const int col = c < j0 ? c : c + nbc;
-    if (col >= n) return;
+    const int col = cbeg + blockIdx.x * 256 + threadIdx.x;
+    if (col >= cend) return;
     float* cp = A + (size_t)col * lda;
     for (int k = 0; k < nbc; ++k) {
         const int p = ipiv[j0 + k];
@@ -332,13 +331,14 @@ __device__ inline float readlane_f(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// A(j0:j0+nbc, j0+nbc:n) <- L11^-1 A(j0:j0+nbc, j0+nbc:n), L11 unit lower
+// A(j0:j0+nbc, cbeg:cend) <- L11^-1 A(j0:j0+nbc, cbeg:cend), L11 unit lower
 // (strsm L,L,N,U).  Each wavefront solves TRSM_CPW columns together, two rows
 // per lane, L11 in LDS: the columns' substitution chains are independent, so
 // interleaving them inside the k loop hides the latency of each step
 // (readlane of x_k after its last update) behind the other columns' work.
 constexpr int TRSM_CPW = 8;   // columns per wavefront
-__global__ __launch_bounds__(256) void k_trsm_llnu(float* __restrict__ A, int lda, int n, int j0, int nbc) {
+__global__ __launch_bounds__(256) void k_trsm_llnu(float* __restrict__ A, int lda, int cbeg, int cend, int j0,
+                                                   int nbc) {
     __shared__ float Ls[LU_NB * LU_NB];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     for (int idx = t; idx < nbc * nbc; idx += 256) {
@@ -346,12 +346,12 @@ __global__ __launch_bounds__(256) void k_trsm_llnu(float* __restrict__ A, int ld
         Ls[k * LU_NB + r] = A[(size_t)(j0 + k) * lda + j0 + r];
     }
     __syncthreads();
-    const int c0 = j0 + nbc + (blockIdx.x * 4 + wv) * TRSM_CPW;
+    const int c0 = cbeg + (blockIdx.x * 4 + wv) * TRSM_CPW;
     float b0[TRSM_CPW], b1[TRSM_CPW];
 #pragma unroll
     for (int q = 0; q < TRSM_CPW; ++q) {
         const float* cp = A + (size_t)(c0 + q) * lda + j0;
-        const bool live = c0 + q < n;
+        const bool live = c0 + q < cend;
         b0[q] = live && lane < nbc ? cp[lane] : 0.f;
         b1[q] = live && lane + 64 < nbc ? cp[lane + 64] : 0.f;
     }
@@ -376,7 +376,7 @@ __global__ __launch_bounds__(256) void k_trsm_llnu(float* __restrict__ A, int ld
     }
 #pragma unroll
     for (int q = 0; q < TRSM_CPW; ++q) {
-        if (c0 + q >= n) break;
+        if (c0 + q >= cend) break;
         float* cp = A + (size_t)(c0 + q) * lda + j0;
         if (lane < nbc) cp[lane] = b0[q];
         if (lane + 64 < nbc) cp[lane + 64] = b1[q];
@@ -846,6 +846,11 @@ struct afx_dense {
     int n = 0, lda = 0;
     hipStream_t s = nullptr;
     rocblas_handle rb = nullptr;
+    // look-ahead of the factorisation: the update of the columns past the
+    // next panel runs on s2 while s factors that panel
+    hipStream_t s2 = nullptr;
+    rocblas_handle rb2 = nullptr;
+    hipEvent_t ev_panel = nullptr, ev_far = nullptr;
     double* A64 = nullptr;     // row-major n x n
     float* A32 = nullptr;      // column-major lda x n, LU in place
     int* ipiv = nullptr;
@@ -871,7 +876,11 @@ struct afx_dense {
                         (void*)yx, (void*)abort})
             if (p) (void)hipFree(p);
         if (rb) rocblas_destroy_handle(rb);
+        if (rb2) rocblas_destroy_handle(rb2);
+        if (ev_panel) (void)hipEventDestroy(ev_panel);
+        if (ev_far) (void)hipEventDestroy(ev_far);
         if (s) (void)hipStreamDestroy(s);
+        if (s2) (void)hipStreamDestroy(s2);
     }
 };
 
@@ -891,6 +900,11 @@ int dense_init(afx_dense* h, int n) {
     VCHK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
     RBCHK(rocblas_create_handle(&h->rb));
     RBCHK(rocblas_set_stream(h->rb, h->s));
+    VCHK(hipStreamCreateWithFlags(&h->s2, hipStreamNonBlocking));
+    RBCHK(rocblas_create_handle(&h->rb2));
+    RBCHK(rocblas_set_stream(h->rb2, h->s2));
+    VCHK(hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming));
+    VCHK(hipEventCreateWithFlags(&h->ev_far, hipEventDisableTiming));
     const size_t nn = (size_t)n;
     VCHK(dalloc(&h->A64, nn * nn));
     VCHK(dalloc(&h->A32, (size_t)h->lda * nn));
@@ -929,33 +943,52 @@ int dense_factor(afx_dense* h, int32_t* info_out) {
     VCHK(hipGetLastError());
     k_panel_reset<<<64, 256, 0, s>>>(h->keys, h->s_vals, h->s_diag, h->pabort, h->info);
     VCHK(hipGetLastError());
+    // Right-looking blocked LU with a look-ahead of one panel.  After panel
+    // k (columns [j0, c1)) is factored on s, s2 swaps the rows of the columns
+    // left of it and past the next panel ([0, j0) and [c2, n)) and applies
+    // step k's update to [c2, n), while s applies it to the next panel's
+    // columns [c1, c2) alone and factors that panel.  s waits for the far
+    // update of step k - 1 before its own update of step k, since that one
+    // brought [c1, c2) up to step k - 1; the column sets the two streams touch
+    // at the same time are disjoint.
+    const float alpha = -1.f, beta = 1.f;
+    float* A = h->A32;
     for (int j0 = 0; j0 < n; j0 += LU_NB) {
         const int nbc = std::min(LU_NB, n - j0);
         const int m = n - j0;
         const int nwg = (m + PANEL_R - 1) / PANEL_R;
-        float* A = h->A32;
         int ldai = lda, mi = m, nbci = nbc, j0i = j0;
         void* args[] = {&A, &ldai, &mi, &nbci, &j0i, &h->ipiv, &h->keys, &h->s_vals, &h->s_diag, &h->pabort,
                         &h->info};
         VCHK(hipLaunchCooperativeKernel((const void*)k_panel_lu, dim3(nwg), dim3(256), args, 0, s));
-        const int others = n - nbc;
-        if (others > 0) {
-            k_laswp<<<(others + 255) / 256, 256, 0, s>>>(h->A32, lda, n, j0, nbc, h->ipiv);
+        VCHK(hipEventRecord(h->ev_panel, s));
+        const int c1 = j0 + nbc, c2 = std::min(n, c1 + LU_NB), rest = n - c1;
+        const float* A21 = A + (size_t)j0 * lda + c1;
+        auto update = [&](hipStream_t st, rocblas_handle rb, int cb, int ce) -> int {
+            k_laswp<<<(ce - cb + 255) / 256, 256, 0, st>>>(A, lda, cb, ce, j0, nbc, h->ipiv);
+            k_trsm_llnu<<<(ce - cb + 4 * TRSM_CPW - 1) / (4 * TRSM_CPW), 256, 0, st>>>(A, lda, cb, ce, j0, nbc);
+            VCHK(hipGetLastError());
+            RBCHK(rocblas_sgemm(rb, rocblas_operation_none, rocblas_operation_none, rest, ce - cb, nbc, &alpha, A21,
+                                lda, A + (size_t)cb * lda + j0, lda, &beta, A + (size_t)cb * lda + c1, lda));
+            return AFX_OK;
+        };
+        VCHK(hipStreamWaitEvent(h->s2, h->ev_panel, 0));
+        if (j0 > 0) {
+            k_laswp<<<(j0 + 255) / 256, 256, 0, h->s2>>>(A, lda, 0, j0, j0, nbc, h->ipiv);
             VCHK(hipGetLastError());
         }
-        const int rest = n - j0 - nbc;
-        if (rest > 0) {
-            const int wgs = (rest + 4 * TRSM_CPW - 1) / (4 * TRSM_CPW);
-            k_trsm_llnu<<<wgs, 256, 0, s>>>(h->A32, lda, n, j0, nbc);
-            VCHK(hipGetLastError());
-            const float alpha = -1.f, beta = 1.f;
-            float* A21 = h->A32 + (size_t)j0 * lda + j0 + nbc;
-            float* A12 = h->A32 + (size_t)(j0 + nbc) * lda + j0;
-            float* A22 = h->A32 + (size_t)(j0 + nbc) * lda + j0 + nbc;
-            RBCHK(rocblas_sgemm(h->rb, rocblas_operation_none, rocblas_operation_none, rest, rest, nbc, &alpha, A21,
-                                lda, A12, lda, &beta, A22, lda));
+        if (c2 < n) {
+            const int rc = update(h->s2, h->rb2, c2, n);
+            if (rc != AFX_OK) return rc;
         }
+        if (c1 < n) {
+            if (j0 > 0) VCHK(hipStreamWaitEvent(s, h->ev_far, 0));   // step k - 1 on [c1, c2)
+            const int rc = update(s, h->rb, c1, c2);
+            if (rc != AFX_OK) return rc;
+        }
+        VCHK(hipEventRecord(h->ev_far, h->s2));
     }
+    VCHK(hipStreamWaitEvent(s, h->ev_far, 0));
     unsigned aborted = 0;
     int info = 0;
     VCHK(hipMemcpyAsync(&aborted, h->pabort, sizeof(unsigned), hipMemcpyDeviceToHost, s));
