@@ -30,8 +30,9 @@
 //   FD convective Jacobian                pinned: vs analytic-free central FD
 //   SA source terms (own, no reference)   pinned: hand-evaluated terms and
 //                                         nu~=0 reduction to laminar
-//   get_uniform_residual (stale qW, A8)   parity unpinned beyond freestream
-//                                         preservation (no golden vectors exist)
+//   get_uniform_residual (stale qW, A8)   pinned: slip-wall channel closed
+//                                         form 2 rho v sabs(c~) |L|; second
+//                                         call doubles it
 // ============================================================================
 #include <cmath>
 #include <cstdint>

@@ -478,6 +478,38 @@ def test_uniform_residual_and_stale_accumulation():
     assert e2 == 2 * e1
 
 
+def test_uniform_residual_slip_wall_closed_form():
+    """get_uniform_residual on a channel of rectangles (slip walls on top and
+    bottom, farfield at both ends) with the freestream at angle a: exact
+    freestream fluxes cancel around each closed cell, so only the wall faces
+    leave a residual.  At a slip wall the ghost mirrors the normal velocity
+    (u_n -> -u_n), the Roe average has u~_n = 0, H~ = H and sound speed
+    c~^2 = c^2 + (gamma - 1) u_n^2 / 2, and only the two acoustic waves carry
+    the jump, with eigenvalue magnitude lam = sabs(c~) = sqrt(c~^2 + 1e-4)
+    (the smoothed |.| of physics.h:84, no Harten correction at that size):
+    F_wall = (0, (p + rho u_n^2 + rho u_n lam) n, 0).  Top and bottom walls
+    together leave -2 rho v lam L on the y-momentum row of each cell of width
+    L, so |R| = 2 rho v lam |L| with v = V sin(a)."""
+    xs = np.cumsum(np.r_[0.0, np.random.default_rng(5).uniform(0.3, 1.7, 9)])
+    m = orc.OMesh.build(*strip_mesh(xs, height=0.8))
+    mach, a, gamma = 0.6, 0.3, 1.4
+    o = solver_on(m, mach=mach, angle=a, second_order=False)
+    e1 = o.uniform_residual()
+    q = orc.freestream(mach, a)
+    rho, u, v = q[0], q[1] / q[0], q[2] / q[0]
+    p = (gamma - 1) * (q[3] - 0.5 * rho * (u * u + v * v))
+    c2 = gamma * p / rho
+    ct2 = c2 + 0.5 * (gamma - 1) * v * v
+    lam = np.sqrt(ct2 + 1e-4)
+    L = np.diff(xs)
+    expect = 2 * rho * v * lam * np.linalg.norm(L)
+    assert abs(e1 - expect) <= 1e-12 * expect, (e1, expect)
+    # the plausible slips (the cell sound speed c for c~, |c~| unsmoothed)
+    # are far outside that tolerance at this angle
+    for wrong in (np.sqrt(c2 + 1e-4), np.sqrt(ct2)):
+        assert abs(2 * rho * v * wrong * np.linalg.norm(L) - expect) > 1e-5 * expect
+
+
 def test_wall_profile_lower_surface_pressure():
     """cp = 1 on the lower surface, 0 elsewhere, alpha = 0: the lift is the
     lower surface's x-extent over the chord, the drag its (closed) y-extent."""
