@@ -104,27 +104,40 @@ __device__ __forceinline__ double roe_dn(const GasT<double>& g, int kind, double
 }
 
 // pass 1: gradients, limiters and the active branch of every (cell, variable)
-__global__ void __launch_bounds__(kBlock) k_adj_forward(CellsT<double> C, FacesT<double> F, const double* __restrict__ qg,
-                                                        const double* __restrict__ q, GasT<double> g, int second_order,
-                                                        double limiter_k, AdjBufs A) {
+__global__ void __launch_bounds__(kBlock)
+k_adj_forward(CellsT<double> C, FacesT<double> F, const double* qg, const double* q, GasT<double> g, int second_order,
+              double limiter_k, AdjBufs A) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C.nC) return;
     const size_t nT = (size_t)C.nT, nC = (size_t)C.nC;
+    // gradients and limiters of one state (qg == q): the neighbour extrema
+    // come from the values the gradient loop already holds
+    const bool same = qg == q;
     double qc[4], sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0};
-    for (int k = 0; k < 4; ++k) qc[k] = q[k * nT + c];
+    double mn[4], mx[4];
+    int jmn[4] = {-1, -1, -1, -1}, jmx[4] = {-1, -1, -1, -1};
+    for (int k = 0; k < 4; ++k) { qc[k] = q[k * nT + c]; mn[k] = qc[k]; mx[k] = qc[k]; }
     for (int s = 0; s < 4; ++s) {
         const uint32_t e = C.cface[(size_t)s * nC + c];
         if (e == 0xFFFFFFFFu) break;
         const int f = (int)(e & 0x7FFFFFFFu);
-        const int o = F.c0[f], n = F.c1[f];
+        const int o = F.c0[f], n = F.c1[f], kind = F.kind[f];
         double qo[4], qn[4], qv[4];
         for (int k = 0; k < 4; ++k) { qo[k] = qg[k * nT + o]; qn[k] = qg[k * nT + n]; }
-        face_vars<double>(g, F.kind[f], F.nx[f], F.ny[f], qo, qn, qv);
+        face_vars<double>(g, kind, F.nx[f], F.ny[f], qo, qn, qv);
         const double gw = F.gw[f], L = F.len[f];
         for (int k = 0; k < 4; ++k) {
             const double fk = (qo[k] * (1.0 - gw) + qv[k] * gw) * L;
             if (e >> 31) { sx[k] -= fk * F.nx[f]; sy[k] -= fk * F.ny[f]; }
             else { sx[k] += fk * F.nx[f]; sy[k] += fk * F.ny[f]; }
+        }
+        if (same && second_order && kind != K_WALL && kind != K_SLIP) {   // wall ghost = owner copy
+            const int other = (e >> 31) ? o : n;
+            for (int k = 0; k < 4; ++k) {
+                const double v = (e >> 31) ? qo[k] : qn[k];
+                if (v < mn[k]) { mn[k] = v; jmn[k] = other; }
+                if (mx[k] < v) { mx[k] = v; jmx[k] = other; }
+            }
         }
     }
     const double Ar = C.area[c];
@@ -133,22 +146,20 @@ __global__ void __launch_bounds__(kBlock) k_adj_forward(CellsT<double> C, FacesT
         A.gy[k * nT + c] = sy[k] / Ar;
     }
     if (!second_order) return;
-    double mn[4], mx[4];
-    int jmn[4] = {-1, -1, -1, -1}, jmx[4] = {-1, -1, -1, -1};
-    for (int k = 0; k < 4; ++k) { mn[k] = qc[k]; mx[k] = qc[k]; }
-    for (int s = 0; s < 4; ++s) {
-        const uint32_t e = C.cface[(size_t)s * nC + c];
-        if (e == 0xFFFFFFFFu) break;
-        const int f = (int)(e & 0x7FFFFFFFu);
-        const int kind = F.kind[f];
-        if (kind == K_WALL || kind == K_SLIP) continue;   // ghost = owner copy
-        const int other = (e >> 31) ? F.c0[f] : F.c1[f];
-        for (int k = 0; k < 4; ++k) {
-            const double v = q[k * nT + other];
-            if (v < mn[k]) { mn[k] = v; jmn[k] = other; }
-            if (mx[k] < v) { mx[k] = v; jmx[k] = other; }
+    if (!same)
+        for (int s = 0; s < 4; ++s) {
+            const uint32_t e = C.cface[(size_t)s * nC + c];
+            if (e == 0xFFFFFFFFu) break;
+            const int f = (int)(e & 0x7FFFFFFFu);
+            const int kind = F.kind[f];
+            if (kind == K_WALL || kind == K_SLIP) continue;   // ghost = owner copy
+            const int other = (e >> 31) ? F.c0[f] : F.c1[f];
+            for (int k = 0; k < 4; ++k) {
+                const double v = q[k * nT + other];
+                if (v < mn[k]) { mn[k] = v; jmn[k] = other; }
+                if (mx[k] < v) { mx[k] = v; jmx[k] = other; }
+            }
         }
-    }
     const double Ka = limiter_k * sqrt(Ar), K3a = Ka * Ka * Ka;
     double lm[4] = {1, 1, 1, 1};
     int act[4] = {-1, -1, -1, -1};
@@ -171,15 +182,25 @@ __global__ void __launch_bounds__(kBlock) k_adj_forward(CellsT<double> C, FacesT
     }
 }
 
-// pass 2: per face, adjoints of the reconstructed states
-template <bool GEO, bool GHOST>
+// pass 2: per face, adjoints of the reconstructed states.  PART 1 takes the
+// interior faces (boundary code compiled out, so the kernel needs far fewer
+// registers), PART 2 the boundary faces from the list A.bface.
+template <bool GEO, bool GHOST, int PART>
 __global__ void __launch_bounds__(kBlock) k_adj_face(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                      const double* __restrict__ w, GasT<double> g, int second_order,
                                                      AdjBufs A, GeoBufs G, double* __restrict__ ghost) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= F.nF) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int f;
+    if constexpr (PART == 1) {
+        f = i;
+        if (f >= F.nF || F.kind[f] != K_INTERNAL) return;
+    } else {
+        if (i >= A.nbf) return;
+        f = A.bface[i];
+    }
     const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
-    const int o = F.c0[f], n = F.c1[f], kind = F.kind[f];
+    const int o = F.c0[f], n = F.c1[f];
+    const int kind = PART == 1 ? (int)K_INTERNAL : (int)F.kind[f];
     const double nx = F.nx[f], ny = F.ny[f], L = F.len[f];
     double qL[4], qR[4];
     for (int k = 0; k < 4; ++k) { qL[k] = q[k * nT + o]; qR[k] = q[k * nT + n]; }
@@ -750,8 +771,11 @@ void adjoint_passes(hipStream_t s, CellsT<double> C, FacesT<double> F, const dou
                     double* lam_g, double* ghost, const GeoBufs& G, bool to_nodes = true) {
     const int nbc = (C.nC + kBlock - 1) / kBlock, nbf = (F.nF + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_adj_forward, dim3(nbc), dim3(kBlock), 0, s, C, F, qg, q, g, second_order, limiter_k, A);
-    hipLaunchKernelGGL((k_adj_face<GEO, GHOST>), dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A, G,
+    hipLaunchKernelGGL((k_adj_face<GEO, GHOST, 1>), dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A, G,
                        ghost);
+    if (A.nbf > 0)
+        hipLaunchKernelGGL((k_adj_face<GEO, GHOST, 2>), dim3((A.nbf + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F,
+                           q, w, g, second_order, A, G, ghost);
     hipLaunchKernelGGL((k_adj_cell1<GEO, GHOST>), dim3(nbc), dim3(kBlock), 0, s, C, F, q, second_order, limiter_k, A,
                        lam, G, ghost);
     if (second_order) {

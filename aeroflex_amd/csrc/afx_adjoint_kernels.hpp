@@ -20,6 +20,8 @@ struct AdjBufs {
     int32_t* nbj;
     double* co;      // [4][nF] Green-Gauss transpose: owner / neighbour parts
     double* cn;
+    const int32_t* bface;   // [nbf] the boundary faces (device index, ascending)
+    int32_t nbf;
 };
 
 // work arrays of the adjoint with respect to the mesh nodes

@@ -126,6 +126,7 @@ struct afx_rans {
     DBuf<double> a_w, a_lam, a_bL, a_bR, a_lgx, a_lgy, a_nbv, a_co, a_cn;
     DBuf<int8_t> a_act;
     DBuf<int32_t> a_jmax, a_jmin, a_nbj;
+    DBuf<int32_t> a_bface;   // device indices of the boundary faces, ascending
     // adjoint with respect to the mesh nodes: metric adjoints, node data in
     // device face / cell order, node incidence lists (CSR)
     DBuf<double> g_fbar, g_abar, g_nodex, g_nodey, g_fend, g_fcen, g_cvert, g_xbar;
@@ -1639,8 +1640,19 @@ int adjoint_check(afx_rans* h) {
         HIPCHK(h->a_act.alloc(c4));
         HIPCHK(h->a_jmax.alloc(c4)); HIPCHK(h->a_jmin.alloc(c4)); HIPCHK(h->a_nbj.alloc(c4));
         HIPCHK(hipMemsetAsync(h->a_lam.p, 0, t4 * 8, h->stream));
+        std::vector<int32_t> bf;
+        bf.reserve(h->m.boundaryEdges.size());
+        for (uint32_t e : h->m.boundaryEdges) bf.push_back(h->f_old2new[e]);
+        std::sort(bf.begin(), bf.end());
+        if (!bf.empty()) { HIPCHK(h->a_bface.alloc(bf.size())); HIPCHK(h->a_bface.upload(bf, h->stream)); }
+        HIPCHK(hipStreamSynchronize(h->stream));
     }
     return AFX_OK;
+}
+AdjBufs adj_bufs(const afx_rans* h) {
+    return AdjBufs{h->gx.p, h->gy.p, h->lim.p, h->a_act.p, h->a_jmax.p, h->a_jmin.p, h->a_bL.p, h->a_bR.p,
+                   h->a_lgx.p, h->a_lgy.p, h->a_nbv.p, h->a_nbj.p, h->a_co.p, h->a_cn.p,
+                   h->a_bface.p, (int32_t)h->m.boundaryEdges.size()};
 }
 int geo_check(afx_rans* h) {
     if (h->g_fbar.p) return AFX_OK;
@@ -1692,8 +1704,7 @@ int geo_check(afx_rans* h) {
     return AFX_OK;
 }
 int enqueue_adjoint(afx_rans* h) {
-    AdjBufs A{h->gx.p, h->gy.p, h->lim.p, h->a_act.p, h->a_jmax.p, h->a_jmin.p, h->a_bL.p, h->a_bR.p,
-              h->a_lgx.p, h->a_lgy.p, h->a_nbv.p, h->a_nbj.p, h->a_co.p, h->a_cn.p};
+    const AdjBufs A = adj_bufs(h);
     HIPCHK(launch_adjoint_state(h->stream, h->cells64(), h->faces64(), h->q.p, h->a_w.p, h->gas64(),
                                 h->opts.second_order, h->opts.limiter_k, A, h->a_lam.p));
     return AFX_OK;
@@ -1722,8 +1733,7 @@ int afx_rans_adjoint_nodes(afx_rans* h, const double* w, double* lambda, double*
     if ((rc = geo_check(h))) return rc;
     if ((rc = upload_aos(h, w, h->a_w.p))) return rc;
     HIPCHK(launch_set_walls<double>(h->stream, h->w_owner.p, h->w_ghost.p, h->n_walls, h->q.p, h->nT, NV));
-    AdjBufs A{h->gx.p, h->gy.p, h->lim.p, h->a_act.p, h->a_jmax.p, h->a_jmin.p, h->a_bL.p, h->a_bR.p,
-              h->a_lgx.p, h->a_lgy.p, h->a_nbv.p, h->a_nbj.p, h->a_co.p, h->a_cn.p};
+    const AdjBufs A = adj_bufs(h);
     GeoBufs G{h->g_fbar.p, h->g_abar.p, h->g_fnode.p, h->g_cnode.p, h->g_nodex.p, h->g_nodey.p, h->g_fend.p,
               h->g_fcen.p, h->g_cvert.p, h->g_noff.p, h->g_nent.p, h->nN, h->g_xbar.p};
     HIPCHK(launch_adjoint_state(h->stream, h->cells64(), h->faces64(), h->q.p, h->a_w.p, h->gas64(),
@@ -1847,8 +1857,7 @@ int afx_rans_dcl_dalpha(afx_rans* h, const char* patch, int32_t steps, int32_t e
 
     // reverse sweep, one checkpoint segment at a time (states recomputed)
     static const double alpha[3] = {0.25, 0.5, 1.};
-    AdjBufs A{h->gx.p, h->gy.p, h->lim.p, h->a_act.p, h->a_jmax.p, h->a_jmin.p, h->a_bL.p, h->a_bR.p,
-              h->a_lgx.p, h->a_lgy.p, h->a_nbv.p, h->a_nbj.p, h->a_co.p, h->a_cn.p};
+    const AdjBufs A = adj_bufs(h);
     for (int k = nck - 1; k >= 0; --k) {
         const int32_t start = k * every, end = std::min(steps, start + every);
         HIPCHK(launch_copy(s, ck.p + (size_t)k * nq, seg.p, (int64_t)nq));
@@ -1959,8 +1968,7 @@ int afx_rans_dcl_dnodes(afx_rans* h, const char* patch, int32_t steps, int32_t e
     // reverse sweep as in afx_rans_dcl_dalpha, with each stage's metric
     // adjoints and the time step's accumulated over all stages and steps
     static const double alpha[3] = {0.25, 0.5, 1.};
-    AdjBufs A{h->gx.p, h->gy.p, h->lim.p, h->a_act.p, h->a_jmax.p, h->a_jmin.p, h->a_bL.p, h->a_bR.p,
-              h->a_lgx.p, h->a_lgy.p, h->a_nbv.p, h->a_nbj.p, h->a_co.p, h->a_cn.p};
+    const AdjBufs A = adj_bufs(h);
     GeoBufs G{h->g_fbar.p, h->g_abar.p, h->g_fnode.p, h->g_cnode.p, h->g_nodex.p, h->g_nodey.p, h->g_fend.p,
               h->g_fcen.p, h->g_cvert.p, h->g_noff.p, h->g_nent.p, h->nN, h->g_xbar.p};
     for (int k = nck - 1; k >= 0; --k) {
