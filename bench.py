@@ -169,6 +169,27 @@ def time_stage(ev, s, stream, prec, stage, n):
     return ev.elapsed_ms(a, b) / n
 
 
+def fp32_accuracy(s, nc, V):
+    """The FP32 residual against the plain FP64 form on the bench mesh and
+    state (the FP64 form equals the oracle's residual to 1e-12 in the test
+    suite): relative L2 norm of the difference over the real cells' V rows,
+    and its max norm relative to max |R|."""
+    from aeroflex_amd import rans
+    R64, _ = s.fillRhoRHS(rans.AFX_FP64)
+    rows64 = [R64.reshape(-1, 4)[:nc]]
+    if V == 5:
+        rows64.append(s.nu_residual()[:nc, None])
+    R32, _ = s.fillRhoRHS(rans.AFX_FP32)
+    rows32 = [R32.reshape(-1, 4)[:nc]]
+    if V == 5:
+        rows32.append(s.nu_residual()[:nc, None])
+    a, b = np.hstack(rows64), np.hstack(rows32)
+    d = b - a
+    return {"rel_l2": float(f"{np.linalg.norm(d) / np.linalg.norm(a):.3e}"),
+            "max_abs_over_max_R": float(f"{np.max(np.abs(d)) / np.max(np.abs(a)):.3e}"),
+            "rows": "real cells, all V variables"}
+
+
 def read_traffic(path):
     """HBM bytes per residual from a committed rocprofv3 PMC summary, if any."""
     if not os.path.exists(path):
@@ -794,6 +815,7 @@ def main():
     s.synchronize()
     t64 = time_stage(ev, s, stream, FP64, 3, args.fp64_steps)
     b64 = alg_bytes(nc, nf, nb, 8, V=V)
+    acc32 = fp32_accuracy(s, nc, V)
 
     prof = "residual_fp32_sa_2M_traffic.txt" if V == 5 else "residual_fp32_2M_traffic.txt"
     traffic = read_traffic(os.path.join(ROOT, "profiles", prof))
@@ -826,6 +848,7 @@ def main():
         "fp64_plain": {"value": round(nc / (t64 * 1e-3) / 1e6, 2), "kernel_ms": round(t64, 5),
                        "kernel": f"afx::k_residual_tile<double, {V}>", "alg_bytes_per_launch": b64,
                        "roofline_frac": round(b64 / (t64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "fp32_vs_fp64": acc32,
         "fp32_three_kernel_ms": None if unf is None else {k: round(v, 5) for k, v in unf.items()},
     }
     if args.sweep_steps > 0 or args.adjoint:
