@@ -30,7 +30,8 @@ UNITS = [
 ]
 CXX_UNITS = [("afx_mesh.cpp", ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]),
              ("afx_tiles.cpp", ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]),
-             ("afx_structure.cpp", ["-O2", "-std=c++17", "-fPIC"])]
+             ("afx_structure.cpp", ["-O2", "-std=c++17", "-fPIC"]),
+             ("afx_post.cpp", ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread"])]
 
 
 def _run(cmd):
@@ -51,7 +52,7 @@ def build(verbose=True):
         objs.append(obj)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs +
-         ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
+         ["-L/opt/rocm/lib", "-lrocblas", "-pthread", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     # C++ driver over the C ABI (rans_gpu.hpp), linked against the library
     drv = os.path.join(HERE, "rans_gpu_driver")

@@ -13,6 +13,7 @@
 //   multigrid<T>::gen_mapper (multigrid.h:96-163)  rans::gpu::mapper
 //   multigrid<T>::run (multigrid.h:296-363)        rans::gpu::multigrid_run
 //   get_wall_profile (post.h:301-387)           rans::gpu::get_wall_profile
+//   save(filename, solver) (post.h:58-180)      rans::gpu::save
 //
 // Every call goes through the C ABI in include/afx_rans.h; errors come back
 // as the exceptions the reference throws at the same places
@@ -250,6 +251,10 @@ inline wallProfile get_wall_profile(solver& s, const std::string& patch) {
     check(afx_rans_wall_forces(s.handle(), patch.c_str(), &w.cl, &w.cd, &w.cm));
     return w;
 }
+
+// save(filename, solver) (post.h:58-180): the reference's ASCII VTU of the
+// current state, the same bytes
+inline void save(const std::string& filename, solver& s) { check(afx_rans_save_vtu(s.handle(), filename.c_str())); }
 
 // multigrid<implicitSolver>::run_solver (multigrid.h:240-293); returns 0 on
 // success, 1 when the solver reports failure; residuals receives the

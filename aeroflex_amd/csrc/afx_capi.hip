@@ -24,6 +24,7 @@
 #include "afx_implicit.hpp"
 #include "afx_adjoint_kernels.hpp"
 #include "afx_mapper.hpp"
+#include "afx_post.hpp"
 
 using namespace afx;
 
@@ -1535,6 +1536,19 @@ int afx_rans_wall_dist(afx_rans* h, double* wall_dist) {
     if (rc) return rc;
     if (!h->wd_ready && (rc = compute_wall_distance(h))) return rc;
     if (wall_dist) return download_cells(h, h->wall_d.p, wall_dist);
+    return AFX_OK;
+}
+
+int afx_rans_save_vtu(afx_rans* h, const char* path) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!path) return fail(AFX_EARG, "path is null");
+    if (!h->wd_ready && (rc = compute_wall_distance(h))) return rc;
+    std::vector<double> q((size_t)4 * h->nT), d(h->nT);
+    if ((rc = afx_rans_get_state(h, q.data()))) return rc;
+    if ((rc = download_cells(h, h->wall_d.p, d.data()))) return rc;
+    const std::string err = write_vtu(path, h->m, q.data(), d.data(), h->gas.gamma, h->gas.R);
+    if (!err.empty()) return fail(AFX_EIO, err);
     return AFX_OK;
 }
 

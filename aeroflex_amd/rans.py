@@ -99,6 +99,7 @@ def _load():
         "afx_rans_linear_iterations": (C.c_int, [P, C.POINTER(i64)]),
         "afx_rans_wall_forces": (C.c_int, [P, C.c_char_p, dp, dp, dp]),
         "afx_rans_wall_dist": (C.c_int, [P, dp]),
+        "afx_rans_save_vtu": (C.c_int, [P, C.c_char_p]),
         "afx_rans_counts": (C.c_int, [P, C.POINTER(i64)]),
         "afx_rans_stream": (C.c_int, [P, C.POINTER(P)]),
         "afx_rans_launch_stage": (C.c_int, [P, i32, i32, i32]),
@@ -443,6 +444,11 @@ class Solver:
         cl, cd, cm = C.c_double(), C.c_double(), C.c_double()
         _check(lib.afx_rans_wall_forces(self.h, patch.encode(), C.byref(cl), C.byref(cd), C.byref(cm)))
         return cl.value, cd.value, cm.value
+
+    def save_vtu(self, path):
+        """save(filename, solver) (post.h:58-180): the reference's ASCII VTU
+        of the current state, the same file byte for byte."""
+        _check(lib.afx_rans_save_vtu(self.h, os.fsencode(path)))
 
     def wall_dist(self):
         d = np.empty(self.n_cells)

@@ -255,6 +255,37 @@ def _cpu_worker(args):
     return nc, n, el
 
 
+def vtu_bench(rans, s, model, cpu=True):
+    """save (post.h:58-180) of the headline case's state: the reference's
+    ASCII VTU written from the device state (wall distance computed before
+    the timed call), and beside it the oracle's one-thread restatement on the
+    120k-cell CPU sample, scaled by cells."""
+    import tempfile
+    s.wall_dist()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "solution.vtu")
+        t0 = time.perf_counter()
+        s.save_vtu(path)
+        t = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        out = {"vtu_cells": s.n_real, "vtu_s": round(t, 3), "vtu_mb": round(size / 1e6, 1),
+               "vtu_threads": max(1, min(16, os.cpu_count() or 1))}
+        if cpu:
+            ni, nj = mesh_dims(120_000)
+            m = rans.Mesh.jittered(ni, nj, r_far=20.0, thickness=0.12, first_h=1e-4, jitter=0.1, quad_frac=0.25,
+                                   seed=1)
+            mp = os.path.join(d, "sample.msh")
+            m.write_msh(mp)
+            o, _, nc = _oracle_case(mp, model)
+            o.wall_dist()
+            t0 = time.perf_counter()
+            o.save_vtu(os.path.join(d, "oracle.vtu"))
+            to = time.perf_counter() - t0
+            out["cpu_baseline"] = {"kind": "port", "cores": 1, "sample": f"oracle save on {nc} cells, {to:.2f} s",
+                                   "vtu_s_at_cells": round(to * s.n_real / nc, 2)}
+    return out
+
+
 def cpu_baseline(rans, model, budget_s=12.0):
     """Oracle (serial FP64 fillRhoRHS with the same model, calc_dt included) on
     a 120k-cell sample mesh of the same generator and state: one thread, and
@@ -896,6 +927,7 @@ def main():
         out["aeroelastic"] = aeroelastic_bench(cpu=not args.no_cpu)
     if args.multigrid:
         out["multigrid_setup"] = multigrid_setup_bench(rans, cpu=not args.no_cpu)
+    out["post"] = vtu_bench(rans, s, args.model, cpu=not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(rans, args.model)
     else:

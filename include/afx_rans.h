@@ -218,6 +218,12 @@ int afx_rans_wall_forces(afx_rans* h, const char* patch, double* cl, double* cd,
 int afx_rans_wall_dist(afx_rans* h, double* wall_dist);
 /* counts[5] as afx_mesh_counts */
 int afx_rans_counts(afx_rans* h, int64_t counts[5]);
+/* save (post.h:58-180): the reference's ASCII VTU of the current state (real
+ * cells: wall distance, Mach, density, pressure, temperature, velocity),
+ * byte for byte, formatted on the host's cores.  Computes the wall distance
+ * first when it is not yet known, as multigrid<T>::run does (multigrid.h:71).
+ * Requires BCs. */
+int afx_rans_save_vtu(afx_rans* h, const char* path);
 
 /* ---------------------------------------------------- Spalart-Allmaras --
  * Replace the reference's unused SA members nu / nuW (solver.h:55,59,187-188):

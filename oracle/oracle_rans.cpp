@@ -40,6 +40,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <map>
 #include <sstream>
 #include <stdexcept>
@@ -1617,6 +1618,65 @@ long orc_solver_fill_lhs(void* h, int64_t* rows, int64_t* cols, double* vals) {
     return (long)g_vals.size();
 }
 void orc_solver_wall_dist(void* h) { Solver* s = (Solver*)h; compute_wall_dist(s->m, s->bface_type); }
+// save (post.h:58-180): the reference's ASCII VTU of the real cells, with its
+// number formats -- std::to_string (%f) for the node coordinates and indices,
+// std::fixed with precision 16 (double2string, post.h:32-36) for cell data --
+// and the solution accessors of core.h:132-163.  Returns 0, or 1 when the
+// file cannot be written.
+int orc_save_vtu(void* h, const char* path) {
+    const Solver& S = *(const Solver*)h;
+    const Mesh& m = S.m;
+    const std::vector<double>& q = S.q;
+    const double gm = S.g.gamma, R = S.g.R;
+    auto fix16 = [](double x) { std::ostringstream o; o << std::fixed << std::setprecision(16) << x; return o.str(); };
+    auto rho = [&](u32 i) { return q[4 * i]; };
+    auto p = [&](u32 i) { return (gm - 1) * (q[4 * i + 3] - 0.5 / rho(i) * (q[4 * i + 1] * q[4 * i + 1] + q[4 * i + 2] * q[4 * i + 2])); };
+    auto c = [&](u32 i) { return std::sqrt(gm * p(i) / rho(i)); };
+    std::ostringstream o;
+    o << "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"BigEndian\">\n"
+      << "  <UnstructuredGrid>\n"
+      << "    <Piece NumberOfPoints=\"" << std::to_string(m.nodesX.size()) << "\" NumberOfCells=\""
+      << std::to_string(m.nRealCells) << "\">\n"
+      << "      <Points>\n"
+      << "        <DataArray type=\"Float32\" NumberOfComponents=\"3\" Format=\"ascii\">\n";
+    for (size_t i = 0; i < m.nodesX.size(); ++i)
+        o << "          " << std::to_string(m.nodesX[i]) << " " << std::to_string(m.nodesY[i]) << " 0.0\n";
+    o << "        </DataArray>\n" << "      </Points>\n" << "      <Cells>\n"
+      << "        <DataArray type=\"Int32\" Name=\"connectivity\" Format=\"ascii\">\n" << "          ";
+    for (u32 i = 0; i < m.nRealCells; ++i) {
+        const u32 nv = m.cellsIsTriangle[i] ? 3 : 4;
+        for (u32 k = 0; k < nv; ++k) o << std::to_string(m.cellsNodes[4 * i + k]) << "  ";
+        o << "\n          ";
+    }
+    o << "        </DataArray>\n" << "        <DataArray type=\"Int32\" Name=\"offsets\" Format=\"ascii\">\n" << "          ";
+    u32 off = 0;
+    for (u32 i = 0; i < m.nRealCells; ++i) {
+        off += m.cellsIsTriangle[i] ? 3 : 4;
+        o << std::to_string(off) << "  ";
+    }
+    o << "\n" << "        </DataArray>\n" << "        <DataArray type=\"Int32\" Name=\"types\" Format=\"ascii\">\n" << "          ";
+    for (u32 i = 0; i < m.nRealCells; ++i) o << (m.cellsIsTriangle[i] ? "5  " : "9  ");
+    o << "\n" << "        </DataArray>\n" << "      </Cells>\n" << "      <CellData Scalars=\"scalars\">\n";
+    auto block = [&](const char* name, auto f) {
+        o << "        <DataArray type=\"Float32\" Name=\"" << name << "\" Format=\"ascii\">\n";
+        for (u32 i = 0; i < m.nRealCells; ++i) o << "          " << fix16(f(i)) << "\n";
+        o << "        </DataArray>\n";
+    };
+    block("Wall Distance", [&](u32 i) { return m.wall_dist[i]; });
+    block("Mach", [&](u32 i) { return std::sqrt(q[4 * i + 1] * q[4 * i + 1] + q[4 * i + 2] * q[4 * i + 2]) / (rho(i) * c(i)); });
+    block("Density", [&](u32 i) { return rho(i); });
+    block("Pressure", [&](u32 i) { return p(i); });
+    block("Temperature", [&](u32 i) { return p(i) / (R * rho(i)); });
+    o << "        <DataArray type=\"Float32\" Name=\"Velocity\" NumberOfComponents=\"3\" Format=\"ascii\">\n";
+    for (u32 i = 0; i < m.nRealCells; ++i)
+        o << "          " << fix16(q[4 * i + 1] / rho(i)) << " " << fix16(q[4 * i + 2] / rho(i)) << " 0.0\n";
+    o << "        </DataArray>\n" << "      </CellData>\n" << "    </Piece>\n" << "  </UnstructuredGrid>\n" << "</VTKFile>";
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return 1;
+    const std::string txt = o.str();
+    f.write(txt.data(), (std::streamsize)txt.size());
+    return f ? 0 : 1;
+}
 
 // vector access: name in {q, qW, dt, gx, gy, limiters, q_min, q_max, RhoVector}
 long orc_solver_vec(void* h, const char* name, double* out, const double* in) {

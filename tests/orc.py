@@ -48,6 +48,7 @@ def _load():
         "orc_gen_mapper": (C.c_long, [C.c_int, dp, dp, C.c_int, dp, dp, dp, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_solver_fill_lhs": (C.c_long, [P, P, P, P]),
         "orc_solver_wall_dist": (None, [P]),
+        "orc_save_vtu": (C.c_int, [P, C.c_char_p]),
         "orc_solver_vec": (C.c_long, [P, C.c_char_p, dp, dp]),
         "orc_face_flux": (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                  C.c_double, dp, dp, dp, dp, dp]),
@@ -189,6 +190,11 @@ class OSolver:
     def __getattr__(self, name):
         fn = getattr(lib, "orc_solver_" + name)
         return lambda *a: fn(self.h, *a)
+
+    def save_vtu(self, path):
+        """save (post.h:58-180) of the current state and wall distance."""
+        if lib.orc_save_vtu(self.h, os.fsencode(str(path))) != 0:
+            raise OSError(f"cannot write {path}")
 
     def wall_profile(self, patch="wall"):
         out = np.zeros(3)

@@ -756,3 +756,60 @@ def test_michalak_limiter_switch_is_off_for_smooth_ranges():
     o.set_michalak(0)
     o.calc_limiters()
     assert np.min(o.vec("limiters")[: 4 * nr]) < 1.0
+
+
+def test_vtu_round_trip_and_format(tmp_path):
+    """save (post.h:58-180) on a channel of rectangles whose cells carry
+    chosen primitives (rho, u, v, p): the file gives back rho, (u, v), p, the
+    ideal-gas temperature p / (R rho) and the Mach number |V| / sqrt(gamma p /
+    rho) of those primitives; node lines are std::to_string's "%f" and cell
+    lines std::fixed precision 16, character for character; the connectivity,
+    offsets and types blocks follow the reference's layout (two spaces after
+    every entry, quads as type 9)."""
+    import xml.etree.ElementTree as ET
+    xs = np.cumsum(np.r_[0.0, np.random.default_rng(8).uniform(0.3, 1.7, 6)])
+    mesh = orc.OMesh.build(*strip_mesh(xs, height=0.7))
+    o = solver_on(mesh)
+    o.init()
+    rng = np.random.default_rng(9)
+    nc = mesh.counts()["real_cells"]
+    prim = np.column_stack([rng.uniform(0.5, 2, nc), rng.uniform(-0.6, 0.6, nc), rng.uniform(-0.6, 0.6, nc),
+                            rng.uniform(0.5, 2, nc)])
+    q = o.vec("q").reshape(-1, 4)
+    q[:nc] = np.array([prim_to_cons(*row) for row in prim])
+    o.vec("q", q.reshape(-1))
+    o.wall_dist()
+    path = tmp_path / "out.vtu"
+    o.save_vtu(path)
+    text = path.read_text()
+    root = ET.fromstring(text)
+    piece = root.find("UnstructuredGrid/Piece")
+    nodes_x, nodes_y = mesh.get("nodesX"), mesh.get("nodesY")
+    assert int(piece.get("NumberOfPoints")) == len(nodes_x) and int(piece.get("NumberOfCells")) == nc
+    arrays = {a.get("Name"): a.text for a in piece.iter("DataArray") if a.get("Name")}
+    assert piece.find("Points/DataArray").text == "\n" + "".join(
+        f"          {x:f} {y:f} 0.0\n" for x, y in zip(nodes_x, nodes_y)) + "        "
+    cells = mesh.get("cellsNodes").reshape(-1, 4)[:nc]
+    assert arrays["connectivity"] == "\n          " + "".join(
+        "".join(f"{n}  " for n in row) + "\n          " for row in cells) + "        "
+    assert arrays["offsets"] == "\n          " + "".join(f"{4 * (i + 1)}  " for i in range(nc)) + "\n        "
+    assert arrays["types"] == "\n          " + "9  " * nc + "\n        "
+
+    def col(name, k=0):
+        rows = arrays[name].strip().split("\n")
+        return np.array([float(r.split()[k]) for r in rows]), [r.strip().split()[k] for r in rows]
+
+    gamma, R = 1.4, 0.71428571428
+    rho, u, v, p = prim.T
+    for name, want, tol in (("Density", rho, 1e-16), ("Pressure", p, 1e-14), ("Temperature", p / (R * rho), 1e-14),
+                            ("Mach", np.hypot(u, v) / np.sqrt(gamma * p / rho), 1e-14)):
+        got, _ = col(name)
+        assert np.max(np.abs(got - want) / np.abs(want)) <= tol, name
+    ux, _ = col("Velocity", 0)
+    vy, _ = col("Velocity", 1)
+    assert np.max(np.abs(ux - u)) <= 1e-15 and np.max(np.abs(vy - v)) <= 1e-15
+    _, dens_txt = col("Density")
+    assert dens_txt == [f"{x:.16f}" for x in q[:nc, 0]]
+    wd, _ = col("Wall Distance")
+    assert np.array_equal(wd, np.array([float(f"{d:.16f}") for d in o.vec("wall_dist")[:nc]]))
+    assert text.endswith("    </Piece>\n  </UnstructuredGrid>\n</VTKFile>")
