@@ -14,6 +14,7 @@
 //   multigrid<T>::run (multigrid.h:296-363)        rans::gpu::multigrid_run
 //   get_wall_profile (post.h:301-387)           rans::gpu::get_wall_profile
 //   save(filename, solver) (post.h:58-180)      rans::gpu::save
+//   CpProfile (post.h:182-290)                  rans::gpu::CpProfile
 //
 // Every call goes through the C ABI in include/afx_rans.h; errors come back
 // as the exceptions the reference throws at the same places
@@ -251,6 +252,31 @@ inline wallProfile get_wall_profile(solver& s, const std::string& patch) {
     check(afx_rans_wall_forces(s.handle(), patch.c_str(), &w.cl, &w.cd, &w.cm));
     return w;
 }
+
+// CpProfile (post.h:182-290) without its plotting arrays: face centres and
+// cp of a patch's faces in boundary order, and the moment point of
+// calc_chord_coords (a quarter of the way from the leftmost face centre to
+// the rightmost)
+struct CpProfile {
+    double x_moment = 0., y_moment = 0.;
+    std::vector<double> x, y, cp;
+    void calc_chord_coords(solver& s, const std::string& af) {
+        calc_cp(s, af);
+        size_t l = 0, r = 0;
+        for (size_t k = 0; k < x.size(); ++k) {
+            if (x[k] < x[l]) l = k;
+            if (x[k] > x[r]) r = k;
+        }
+        x_moment = (x[r] - x[l]) * 0.25 + x[l];
+        y_moment = (y[r] - y[l]) * 0.25 + y[l];
+    }
+    void calc_cp(solver& s, const std::string& af) {
+        int64_t n = 0;
+        check(afx_rans_cp_profile(s.handle(), af.c_str(), &n, nullptr, nullptr, nullptr));
+        x.resize((size_t)n); y.resize((size_t)n); cp.resize((size_t)n);
+        check(afx_rans_cp_profile(s.handle(), af.c_str(), &n, x.data(), y.data(), cp.data()));
+    }
+};
 
 // save(filename, solver) (post.h:58-180): the reference's ASCII VTU of the
 // current state, the same bytes

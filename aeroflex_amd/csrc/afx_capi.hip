@@ -1475,12 +1475,13 @@ int afx_rans_uniform_residual(afx_rans* h, double* err) {
     return AFX_OK;
 }
 
-int afx_rans_wall_forces(afx_rans* h, const char* patch, double* cl, double* cd, double* cm) {
-    int rc = check_handle(h);
-    if (rc) return rc;
+// boundary segments of a patch (reference order) and their owners' states
+// (NV per face), gathered on the device
+static int gather_patch(afx_rans* h, const char* patch, std::vector<int32_t>& bsel, std::vector<double>& st) {
     if (!patch) return fail(AFX_EARG, "null patch name");
     const Mesh& m = h->m;
-    std::vector<int32_t> bsel, cells;
+    std::vector<int32_t> cells;
+    bsel.clear();
     for (int32_t b = 0; b < h->nB; ++b)
         if (m.names[m.boundaryNameIdx[b]] == patch) {
             bsel.push_back(b);
@@ -1495,9 +1496,19 @@ int afx_rans_wall_forces(afx_rans* h, const char* patch, double* cl, double* cd,
     }
     HIPCHK(h->patch_cells.upload(cells, h->stream));
     HIPCHK(launch_gather_states(h->stream, h->q.p, h->nT, NV, h->patch_cells.p, (int32_t)cells.size(), h->states.p));
-    std::vector<double> st(cells.size() * NV);
+    st.assign(cells.size() * NV, 0.0);
     HIPCHK(hipMemcpyAsync(st.data(), h->states.p, st.size() * 8, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    return AFX_OK;
+}
+
+int afx_rans_wall_forces(afx_rans* h, const char* patch, double* cl, double* cd, double* cm) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    const Mesh& m = h->m;
+    std::vector<int32_t> bsel;
+    std::vector<double> st;
+    if ((rc = gather_patch(h, patch, bsel, st))) return rc;
     // post.h:301-387, in the reference's boundary order
     double v[4];
     boundary_variables(h, v);
@@ -1528,6 +1539,35 @@ int afx_rans_wall_forces(afx_rans* h, const char* patch, double* cl, double* cd,
     if (cd) *cd = fx * std::cos(aoa) + fy * std::sin(aoa);
     if (cl) *cl = -fx * std::sin(aoa) + fy * std::cos(aoa);
     if (cm) *cm = mm;
+    return AFX_OK;
+}
+
+int afx_rans_cp_profile(afx_rans* h, const char* patch, int64_t* n, double* x, double* y, double* cp) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!n) return fail(AFX_EARG, "n is null");
+    std::vector<int32_t> bsel;
+    std::vector<double> st;
+    if ((rc = gather_patch(h, patch, bsel, st))) return rc;
+    const int64_t cap = *n;
+    *n = (int64_t)bsel.size();
+    if (!x && !y && !cp) return AFX_OK;
+    if (!x || !y || !cp) return fail(AFX_EARG, "x, y and cp are all null (query) or all set");
+    if (cap < *n) return fail(AFX_EARG, "arrays hold " + std::to_string(cap) + " faces, the patch has " +
+                                             std::to_string(*n));
+    // CpProfile::calc_cp (post.h:246-290): the owner's pressure, first order
+    const Mesh& m = h->m;
+    double v[4];
+    boundary_variables(h, v);
+    const double p_inf = v[3], mach_inf = v[0], gm = h->gas.gamma;
+    for (size_t k = 0; k < bsel.size(); ++k) {
+        const uint32_t e = m.boundaryEdges[bsel[k]];
+        const double* q = &st[k * NV];
+        const double p = (gm - 1) * (q[3] - 0.5 / q[0] * (q[1] * q[1] + q[2] * q[2]));
+        x[k] = m.edgesCentersX[e];
+        y[k] = m.edgesCentersY[e];
+        cp[k] = 2. / (gm * mach_inf * mach_inf) * (p / p_inf - 1.);
+    }
     return AFX_OK;
 }
 

@@ -100,6 +100,7 @@ def _load():
         "afx_rans_wall_forces": (C.c_int, [P, C.c_char_p, dp, dp, dp]),
         "afx_rans_wall_dist": (C.c_int, [P, dp]),
         "afx_rans_save_vtu": (C.c_int, [P, C.c_char_p]),
+        "afx_rans_cp_profile": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_int64), dp, dp, dp]),
         "afx_rans_counts": (C.c_int, [P, C.POINTER(i64)]),
         "afx_rans_stream": (C.c_int, [P, C.POINTER(P)]),
         "afx_rans_launch_stage": (C.c_int, [P, i32, i32, i32]),
@@ -444,6 +445,15 @@ class Solver:
         cl, cd, cm = C.c_double(), C.c_double(), C.c_double()
         _check(lib.afx_rans_wall_forces(self.h, patch.encode(), C.byref(cl), C.byref(cd), C.byref(cm)))
         return cl.value, cd.value, cm.value
+
+    def cp_profile(self, patch="wall"):
+        """CpProfile::calc_cp (post.h:246-290): face centres x, y and cp of
+        the patch's boundary faces, in the reference's boundary order."""
+        n = C.c_int64(0)
+        _check(lib.afx_rans_cp_profile(self.h, patch.encode(), C.byref(n), None, None, None))
+        x, y, cp = np.empty(n.value), np.empty(n.value), np.empty(n.value)
+        _check(lib.afx_rans_cp_profile(self.h, patch.encode(), C.byref(n), _dptr(x), _dptr(y), _dptr(cp)))
+        return x, y, cp
 
     def save_vtu(self, path):
         """save(filename, solver) (post.h:58-180): the reference's ASCII VTU

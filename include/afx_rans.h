@@ -213,6 +213,12 @@ int afx_rans_linear_iterations(afx_rans* h, int64_t* iters);
 int afx_rans_uniform_residual(afx_rans* h, double* err);
 /* get_wall_profile (post.h:301-387): CL, CD, CM on the named patch */
 int afx_rans_wall_forces(afx_rans* h, const char* patch, double* cl, double* cd, double* cm);
+/* CpProfile::calc_chord_coords / calc_cp (post.h:200-290): per boundary face
+ * of the named patch, in the reference's boundary order, the face centre
+ * (x, y) and cp = 2 / (gamma M_inf^2) (p / p_inf - 1) of the owner cell's
+ * state.  *n: in, the arrays' capacity; out, the patch's face count.  With x,
+ * y and cp all null only *n is returned. */
+int afx_rans_cp_profile(afx_rans* h, const char* patch, int64_t* n, double* x, double* y, double* cp);
 /* mesh::compute_wall_dist (mesh.h:794-830) on the device; wall_dist out
  * (n_cells, nullable).  Requires BCs. */
 int afx_rans_wall_dist(afx_rans* h, double* wall_dist);

@@ -1618,6 +1618,30 @@ long orc_solver_fill_lhs(void* h, int64_t* rows, int64_t* cols, double* vals) {
     return (long)g_vals.size();
 }
 void orc_solver_wall_dist(void* h) { Solver* s = (Solver*)h; compute_wall_dist(s->m, s->bface_type); }
+// CpProfile::calc_cp (post.h:246-290): per boundary face of the patch, in
+// boundary order, the face centre and cp = 2/(gamma M_inf^2) (p/p_inf - 1)
+// of the owner cell (sol.p, core.h:145-147); returns the face count (arrays
+// may be null to count only)
+int orc_cp_profile(void* h, const char* patch, double* x, double* y, double* cp) {
+    const Solver& S = *(const Solver*)h;
+    const Mesh& m = S.m;
+    const BVars far = S.get_boundary_variables();
+    int k = 0;
+    for (size_t b = 0; b < m.boundaryEdges.size(); ++b) {
+        if (m.boundaryEdgesPhysicals[b] != patch) continue;
+        const u32 e = m.boundaryEdges[b];
+        const u32 c = m.edgesCells[2 * e];
+        const double* q = &S.q[4 * c];
+        const double p = (S.g.gamma - 1) * (q[3] - 0.5 / q[0] * (q[1] * q[1] + q[2] * q[2]));
+        if (x) {
+            x[k] = m.edgesCentersX[e];
+            y[k] = m.edgesCentersY[e];
+            cp[k] = 2. / (S.g.gamma * far.mach * far.mach) * (p / far.p - 1.);
+        }
+        ++k;
+    }
+    return k;
+}
 // save (post.h:58-180): the reference's ASCII VTU of the real cells, with its
 // number formats -- std::to_string (%f) for the node coordinates and indices,
 // std::fixed with precision 16 (double2string, post.h:32-36) for cell data --

@@ -49,6 +49,7 @@ def _load():
         "orc_solver_fill_lhs": (C.c_long, [P, P, P, P]),
         "orc_solver_wall_dist": (None, [P]),
         "orc_save_vtu": (C.c_int, [P, C.c_char_p]),
+        "orc_cp_profile": (C.c_int, [P, C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "orc_solver_vec": (C.c_long, [P, C.c_char_p, dp, dp]),
         "orc_face_flux": (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                  C.c_double, dp, dp, dp, dp, dp]),
@@ -190,6 +191,13 @@ class OSolver:
     def __getattr__(self, name):
         fn = getattr(lib, "orc_solver_" + name)
         return lambda *a: fn(self.h, *a)
+
+    def cp_profile(self, patch="wall"):
+        """CpProfile::calc_cp (post.h:246-290): x, y, cp per patch face."""
+        n = lib.orc_cp_profile(self.h, patch.encode(), None, None, None)
+        x, y, cp = np.empty(n), np.empty(n), np.empty(n)
+        lib.orc_cp_profile(self.h, patch.encode(), x.ctypes.data, y.ctypes.data, cp.ctypes.data)
+        return x, y, cp
 
     def save_vtu(self, path):
         """save (post.h:58-180) of the current state and wall distance."""

@@ -813,3 +813,32 @@ def test_vtu_round_trip_and_format(tmp_path):
     wd, _ = col("Wall Distance")
     assert np.array_equal(wd, np.array([float(f"{d:.16f}") for d in o.vec("wall_dist")[:nc]]))
     assert text.endswith("    </Piece>\n  </UnstructuredGrid>\n</VTKFile>")
+
+
+def test_cp_profile_closed_forms():
+    """CpProfile::calc_cp (post.h:246-290): cp = 2 / (gamma M^2) (p / p_inf -
+    1) of the owner cell.  The freestream state gives cp = 0 on every wall
+    face; the freestream velocity at pressure p_inf (1 + k) gives
+    2 k / (gamma M^2); the positions are the wall faces' centres, in the
+    mesh's boundary order."""
+    m = orc.OMesh.read(TRI)
+    mach = 0.5
+    o = solver_on(m, mach=mach, angle=0.03)
+    o.init()
+    o.refill_bcs()
+    x, y, cp = o.cp_profile("wall")
+    names = m.boundary_names()
+    be = m.get("boundaryEdges")
+    walls = [be[b] for b in range(len(names)) if names[b] == "wall"]
+    assert len(cp) == len(walls) > 0
+    assert np.array_equal(x, m.get("edgesCentersX")[walls]) and np.array_equal(y, m.get("edgesCentersY")[walls])
+    assert np.max(np.abs(cp)) <= 1e-14
+    q = o.vec("q").reshape(-1, 4)
+    k = 0.37
+    nc = m.counts()["real_cells"]
+    # raise the pressure by (1 + k) at fixed density and velocity: rhoE += k p_inf / (gamma - 1)
+    q[:nc, 3] += k * 1.0 / 0.4
+    o.vec("q", q.reshape(-1))
+    _, _, cp2 = o.cp_profile("wall")
+    want = 2 * k / (1.4 * mach * mach)
+    assert np.max(np.abs(cp2 - want)) <= 1e-13 * want
