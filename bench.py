@@ -882,6 +882,7 @@ def main():
     t64 = time_stage(ev, s, stream, FP64, 3, args.fp64_steps)
     b64 = alg_bytes(nc, nf, nb, 8, V=V)
     acc32 = fp32_accuracy(s, nc, V)
+    post = vtu_bench(rans, s, args.model, cpu=not args.no_cpu)
 
     prof = "residual_fp32_sa_2M_traffic.txt" if V == 5 else "residual_fp32_2M_traffic.txt"
     traffic = read_traffic(os.path.join(ROOT, "profiles", prof))
@@ -927,7 +928,7 @@ def main():
         out["aeroelastic"] = aeroelastic_bench(cpu=not args.no_cpu)
     if args.multigrid:
         out["multigrid_setup"] = multigrid_setup_bench(rans, cpu=not args.no_cpu)
-    out["post"] = vtu_bench(rans, s, args.model, cpu=not args.no_cpu)
+    out["post"] = post
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(rans, args.model)
     else:
