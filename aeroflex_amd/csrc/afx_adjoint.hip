@@ -283,12 +283,15 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<d
     double lq[4] = {0, 0, 0, 0}, lgx[4] = {0, 0, 0, 0}, lgy[4] = {0, 0, 0, 0}, lphi[4] = {0, 0, 0, 0};
     double gx[4], gy[4], ph[4];
     for (int k = 0; k < 4; ++k) { gx[k] = A.gx[k * nT + c]; gy[k] = A.gy[k * nT + c]; ph[k] = A.lim[k * nT + c]; }
+    uint32_t es[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};   // the slots' words and offsets, kept
+    double dxs[4] = {0, 0, 0, 0}, dys[4] = {0, 0, 0, 0};                   // for the limiter's active slot
     for (int s = 0; s < 4; ++s) {
         const uint32_t e = C.cface[(size_t)s * nC + c];
         if (e == 0xFFFFFFFFu) break;
         const int f = (int)(e & 0x7FFFFFFFu);
         const bool nb = (e >> 31) != 0;
         const double dx = F.xf[f] - C.xc[c], dy = F.yf[f] - C.yc[c];
+        es[s] = e; dxs[s] = dx; dys[s] = dy;
         for (int k = 0; k < 4; ++k) {
             const double b = nb ? A.bR[k * nF + f] : A.bL[k * nF + f];
             lq[k] += b;
@@ -305,9 +308,10 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<d
         int nbj = -1;
         const int s = second_order ? (int)A.act[k * nC + c] : -1;
         if (s >= 0) {
-            const uint32_t e = C.cface[(size_t)s * nC + c];
+            const uint32_t e = s == 0 ? es[0] : s == 1 ? es[1] : s == 2 ? es[2] : es[3];
             const int f = (int)(e & 0x7FFFFFFFu);
-            const double dx = F.xf[f] - C.xc[c], dy = F.yf[f] - C.yc[c];
+            const double dx = s == 0 ? dxs[0] : s == 1 ? dxs[1] : s == 2 ? dxs[2] : dxs[3];
+            const double dy = s == 0 ? dys[0] : s == 1 ? dys[1] : s == 2 ? dys[2] : dys[3];
             const double qc = q[k * nT + c];
             const double dqg = gx[k] * dx + gy[k] * dy;
             const int j = dqg > 1e-16 ? A.jmax[k * nC + c] : A.jmin[k * nC + c];
