@@ -283,7 +283,8 @@ struct CpProfile {
 inline void save(const std::string& filename, solver& s) { check(afx_rans_save_vtu(s.handle(), filename.c_str())); }
 
 // multigrid<implicitSolver>::run_solver (multigrid.h:240-293); returns 0 on
-// success, 1 when the solver reports failure; residuals receives the
+// success, 1 when the solver reports failure or the residual is not finite;
+// residuals receives the
 // normalised residual of every iteration
 inline int run_solver(solver& s, const Settings& st, std::vector<double>& residuals,
                       const std::function<bool()>& stop = nullptr) {
@@ -304,7 +305,9 @@ inline int run_solver(solver& s, const Settings& st, std::vector<double>& residu
         err /= err_0;
         cfl = std::min(st.start_cfl + (i + 1) * st.slope_cfl, st.max_cfl);
         i++;
-        if (err < 0) return 1;
+        // a non-finite normalised residual is a failure (the reference's
+        // loop would read NaN as converged)
+        if (err < 0 || !std::isfinite(err)) return 1;
         residuals.push_back(err);
     } while ((err > st.tolerance) && (i < st.max_iterations) && !(stop && stop()));
     return 0;

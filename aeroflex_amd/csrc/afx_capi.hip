@@ -1403,20 +1403,30 @@ int afx_rans_implicit_solve(afx_rans* h, double relaxation, double tol, int32_t 
         }
         return AFX_OK;
     };
+    // Deviation from solver.h:1170-1213: a non-finite residual norm is the
+    // failure value -1 (the reference would return NaN, which its loop reads
+    // as converged); the state is left as it was when the norm was taken.
     double err = 0;
     bool ok = false;
+    auto failed = [&](const char*) {
+        if (err_out) *err_out = -1;
+        return AFX_OK;
+    };
     if ((rc = residual_norm(&err))) return rc;
+    if (!std::isfinite(err)) return failed("non-finite residual norm");
     if (err < tol) { if (err_out) *err_out = err; return AFX_OK; }
     const double err_ini = err;
     if ((rc = step(&ok))) return rc;
-    if (!ok) { if (err_out) *err_out = -1; return AFX_OK; }
+    if (!ok) return failed("GMRES did not converge");
     for (int i = 0; i < rhs_iterations; ++i) {
         if ((rc = residual_norm(&err))) return rc;
+        if (!std::isfinite(err)) return failed("non-finite residual norm");
         if (err < tol || err > 10 * err_ini) { if (err_out) *err_out = err; return AFX_OK; }
         if ((rc = step(&ok))) return rc;
-        if (!ok) { if (err_out) *err_out = -1; return AFX_OK; }
+        if (!ok) return failed("GMRES did not converge");
     }
     if ((rc = residual_norm(&err))) return rc;
+    if (!std::isfinite(err)) return failed("non-finite residual norm");
     if (err_out) *err_out = err;
     return AFX_OK;
 }

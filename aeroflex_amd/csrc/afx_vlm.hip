@@ -821,7 +821,7 @@ __global__ void k_forces_stations(const double* __restrict__ corner, const doubl
 }
 
 __global__ void k_panel_reset(unsigned long long* keys, unsigned long long* s_vals, unsigned long long* s_diag,
-                              unsigned* abort_flag, int* info) {
+                              unsigned* abort_flag, unsigned* trsv_abort, int* info) {
     const int stride = gridDim.x * blockDim.x, i0 = blockIdx.x * blockDim.x + threadIdx.x;
     for (int q = i0; q < 2 * MAX_PANEL_WG; q += stride) keys[q] = 0ull;
     // the row words carry the column sequence j0 + j + 1, which repeats from one
@@ -830,6 +830,7 @@ __global__ void k_panel_reset(unsigned long long* keys, unsigned long long* s_va
     for (int q = i0; q < 2 * LU_NB; q += stride) s_diag[q] = 0ull;
     if (i0 == 0) {
         *abort_flag = 0u;
+        *trsv_abort = 0u;   // a new factorisation also re-arms the triangular solves
         *info = 0;
     }
 }
@@ -941,7 +942,7 @@ int dense_factor(afx_dense* h, int32_t* info_out) {
     dim3 tg((n + 63) / 64, (n + 63) / 64);
     k_to_cm32<<<tg, 256, 0, s>>>(h->A64, n, h->A32, lda);
     VCHK(hipGetLastError());
-    k_panel_reset<<<64, 256, 0, s>>>(h->keys, h->s_vals, h->s_diag, h->pabort, h->info);
+    k_panel_reset<<<64, 256, 0, s>>>(h->keys, h->s_vals, h->s_diag, h->pabort, h->abort, h->info);
     VCHK(hipGetLastError());
     // Right-looking blocked LU with a look-ahead of one panel.  After panel
     // k (columns [j0, c1)) is factored on s, s2 swaps the rows of the columns
@@ -1056,7 +1057,13 @@ int dense_solve_dev(afx_dense* h, const double* b_dev, double* x_dev, double tol
         VCHK(hipMemcpyAsync(nrm, h->norms, sizeof(nrm), hipMemcpyDeviceToHost, s));
         VCHK(hipMemcpyAsync(&aborted, h->abort, sizeof(unsigned), hipMemcpyDeviceToHost, s));
         VCHK(hipStreamSynchronize(s));
-        if (aborted) return api_fail(AFX_EHIP, "triangular solve: a block wait timed out");
+        if (aborted) {
+            // the timed-out launch left the solution words incomplete: the
+            // factorisation is discarded, and the next factor call clears the
+            // flag (k_panel_reset)
+            h->factored = false;
+            return api_fail(AFX_EHIP, "triangular solve: a block wait timed out (factor again before solving)");
+        }
         if (need_norm) h->anorm = nrm[3];
         const double denom = h->anorm * nrm[1] + nrm[2];
         inf.backward_error = denom > 0.0 ? nrm[0] / denom : 0.0;

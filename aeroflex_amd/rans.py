@@ -23,6 +23,8 @@ import os
 
 import numpy as np
 
+from ._handle import Handle, close_all, live_handles  # noqa: F401
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libafx_rans.so")
 
@@ -182,13 +184,13 @@ class BoundaryCondition:
         self.vars_far = vars_far if vars_far is not None else BoundaryVariables()
 
 
-class Mesh:
+class Mesh(Handle):
     """rans::mesh (mesh.h:198-273), host side, reference order."""
 
     _U32 = {"edgesNodes", "edgesCells", "cellsNodes", "cellsEdges", "boundaryEdges", "boundaryEdgesIntTag"}
 
     def __init__(self, handle):
-        self.h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        self._own(C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle)
 
     @classmethod
     def read_file(cls, path):
@@ -261,10 +263,8 @@ class Mesh:
         _check(lib.afx_mesh_rcm(self.h, _vptr(p)))
         return p
 
-    def __del__(self):
-        if getattr(self, "h", None) is not None and lib is not None:
-            lib.afx_mesh_destroy(self.h)
-            self.h = None
+    def _destroy(self, h):
+        lib.afx_mesh_destroy(h)
 
 
 # "spalart-allmaras" is the name solver::set_bcs tests (solver.h:206);
@@ -274,7 +274,7 @@ GRADIENT = {"least-squares": 0, "green-gauss": 1}
 LIMITERS = {"default": 0, "michalak": 1}
 
 
-class Solver:
+class Solver(Handle):
     """Device residual behind the rans::solver interface (solver.h:40-175)."""
 
     def __init__(self, mesh, gas=None, viscosity_model="inviscid", second_order=True,
@@ -300,7 +300,7 @@ class Solver:
         g = self.gas.c_struct()
         h = C.c_void_p()
         _check(lib.afx_rans_create(mesh.h, C.byref(g), C.byref(o), C.byref(h)))
-        self.h = h
+        self._own(h)
         c = (C.c_int64 * 5)()
         _check(lib.afx_rans_counts(self.h, c))
         self.n_real, self.n_cells, self.n_faces, self.n_bfaces = c[1], c[2], c[3], c[4]
@@ -508,7 +508,7 @@ class Solver:
         g = other.gas.c_struct()
         h = C.c_void_p()
         _check(lib.afx_rans_create_like(self.h, C.byref(g), C.byref(h)))
-        other.h = h
+        other._own(h)
         other.n_real, other.n_cells, other.n_faces, other.n_bfaces = (self.n_real, self.n_cells, self.n_faces,
                                                                       self.n_bfaces)
         other.bcs = {}
@@ -575,13 +575,11 @@ class Solver:
         _check(lib.afx_rans_time_residual(self.h, precision, reps, C.byref(ms)))
         return ms.value
 
-    def __del__(self):
-        if getattr(self, "h", None) is not None and lib is not None:
-            lib.afx_rans_destroy(self.h)
-            self.h = None
+    def _destroy(self, h):
+        _check(lib.afx_rans_destroy(h))
 
 
-class Batch:
+class Batch(Handle):
     """Explicit pseudo-time steps of many solvers at once (FP32 fused
     residuals of all members in one launch per stage, FP64 updates,
     hipGraph replay).  Keeps references to its members."""
@@ -592,7 +590,7 @@ class Batch:
                                                  for m in self.members])
         h = C.c_void_p()
         _check(lib.afx_batch_create(len(self.members), arr, relaxation, C.byref(h)))
-        self.h = h
+        self._own(h)
 
     def run(self, steps, graph_steps=25):
         _check(lib.afx_batch_run(self.h, steps, graph_steps))
@@ -607,10 +605,8 @@ class Batch:
         _check(lib.afx_batch_stream(self.h, C.byref(s)))
         return s.value
 
-    def __del__(self):
-        if getattr(self, "h", None) is not None and lib is not None:
-            lib.afx_batch_destroy(self.h)
-            self.h = None
+    def _destroy(self, h):
+        _check(lib.afx_batch_destroy(h))
 
 
 def autotune_tile(make_solver, candidates=(128, 160, 192, 224, 256), precision=AFX_FP32, reps=20, replicas=1):
@@ -657,22 +653,19 @@ class Settings:
         self.airfoil_name = airfoil_name
 
 
-class Mapper:
+class Mapper(Handle):
     """multigrid<T>::gen_mapper (multigrid.h:96-163), built on the device:
     the inverse-distance map from the coarse solver's cells to the fine
     solver's cells.  Both solvers must outlive it."""
 
     def __init__(self, coarse, fine):
-        self.h = None
         self.coarse, self.fine = coarse, fine
         h = C.c_void_p()
         _check(lib.afx_mapper_create(coarse.h, fine.h, C.byref(h)))
-        self.h = h
+        self._own(h)
 
-    def __del__(self):
-        if getattr(self, "h", None) and lib is not None:
-            lib.afx_mapper_destroy(self.h)
-            self.h = None
+    def _destroy(self, h):
+        _check(lib.afx_mapper_destroy(h))
 
     def csr(self):
         """(row_ptr, cols, weights) in reference cell order."""
@@ -716,7 +709,7 @@ def run_solver(s, settings, residuals=None, stop=None):
     """multigrid<implicitSolver>::run_solver (multigrid.h:240-293) over the
     device solver: CFL ramp, first-iteration normalisation rebase, stop on
     tolerance / iteration count / stop flag.  Returns (state, residuals):
-    state 0 ok, 1 linear-solver failure."""
+    state 0 ok, 1 linear-solver failure or a non-finite residual."""
     residuals = [] if residuals is None else residuals
     cfl = settings.start_cfl
     i = 0
@@ -735,7 +728,9 @@ def run_solver(s, settings, residuals=None, stop=None):
         err /= err_0
         cfl = min(settings.start_cfl + (i + 1) * settings.slope_cfl, settings.max_cfl)
         i += 1
-        if err < 0:
+        # a non-finite normalised residual is a failure (the reference's
+        # loop would read NaN as converged)
+        if err < 0 or not np.isfinite(err):
             return 1, residuals
         residuals.append(err)
         if not (err > settings.tolerance and i < settings.max_iterations and not (stop and stop())):

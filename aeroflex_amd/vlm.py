@@ -18,6 +18,7 @@ import os
 
 import numpy as np
 
+from ._handle import Handle
 from .rans import LIB_PATH
 
 _P = C.c_void_p
@@ -141,7 +142,7 @@ def wing_lattice(n_chord, n_span, aspect_ratio=10.0, taper=0.4, sweep_deg=25.0, 
     return xyz, rn, so, {"sref": ref[0], "cref": ref[1], "span": ref[2]}
 
 
-class VLM:
+class VLM(Handle):
     """Linear steady vortex-lattice solver on the device."""
 
     def __init__(self, xyz, ring_nodes, station_offsets):
@@ -151,13 +152,11 @@ class VLM:
         h = _P()
         _chk(_lib.afx_vlm_create(len(self.xyz), _d(self.xyz), len(self.soff) - 1, _i(self.soff), _i(self.rings),
                                  C.byref(h)))
-        self.h = h
+        self._own(h)
         self.n = len(self.rings)
 
-    def __del__(self):
-        if getattr(self, "h", None):
-            _lib.afx_vlm_destroy(self.h)
-            self.h = None
+    def _destroy(self, h):
+        _chk(_lib.afx_vlm_destroy(h))
 
     def set_flow(self, alpha_deg, vinf=1.0, rho=1.0, sref=1.0, cref=1.0, wake_length=0.0, core_radius=0.0):
         self.flow = Flow(alpha_deg, vinf, rho, sref, cref, wake_length, core_radius)
@@ -212,18 +211,17 @@ class VLM:
         return rhs, g, col, nor
 
 
-class Dense:
+class Dense(Handle):
     """FP32 LU (matrix cores) + FP64 iterative refinement for a dense matrix."""
 
     def __init__(self, n):
         h = _P()
         _chk(_lib.afx_dense_create(int(n), C.byref(h)))
-        self.h, self.n = h, int(n)
+        self._own(h)
+        self.n = int(n)
 
-    def __del__(self):
-        if getattr(self, "h", None):
-            _lib.afx_dense_destroy(self.h)
-            self.h = None
+    def _destroy(self, h):
+        _chk(_lib.afx_dense_destroy(h))
 
     def set(self, a):
         a = np.ascontiguousarray(a, np.float64)
@@ -253,7 +251,7 @@ class BeamNonlinearInfo(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("residual", C.c_double)]
 
 
-class Beam:
+class Beam(Handle):
     """Euler-Bernoulli frame (CBAR elements, SPC1 clamps): linear solve and
     the corotational load-stepping solve."""
 
@@ -265,12 +263,10 @@ class Beam:
         h = _P()
         _chk(_lib.afx_beam_create(len(self.xyz), _d(self.xyz), len(self.en), _i(self.en), _d(self.props),
                                   len(self.clamped), _i(self.clamped), C.byref(h)))
-        self.h = h
+        self._own(h)
 
-    def __del__(self):
-        if getattr(self, "h", None):
-            _lib.afx_beam_destroy(self.h)
-            self.h = None
+    def _destroy(self, h):
+        _chk(_lib.afx_beam_destroy(h))
 
     def solve(self, load):
         f = np.ascontiguousarray(load, np.float64).reshape(-1)
@@ -350,16 +346,15 @@ def default_coupling_opts():
     return o
 
 
-class Aeroelastic:
+class Aeroelastic(Handle):
     def __init__(self, vlm, beam):
         h = _P()
         _chk(_lib.afx_aeroelastic_create(vlm.h, beam.h, C.byref(h)))
-        self.h, self.vlm, self.beam = h, vlm, beam
+        self.vlm, self.beam = vlm, beam
+        self._own(h)
 
-    def __del__(self):
-        if getattr(self, "h", None):
-            _lib.afx_aeroelastic_destroy(self.h)
-            self.h = None
+    def _destroy(self, h):
+        _chk(_lib.afx_aeroelastic_destroy(h))
 
     def solve(self, **kw):
         o = default_coupling_opts()

@@ -676,6 +676,11 @@ def aeroelastic_bench(n_chord=64, n_span=256, n_cases=32, alpha_max=8.0, cpu=Tru
         cases.append(info)
     t_cases = time.perf_counter() - t0
     last = cases[-1]
+    # release the coupling, beam and lattice (device matrices, streams and
+    # rocBLAS handles) before the CPU leg, dependents first
+    ae.close()
+    beam.close()
+    h.close()
     out = {
         "workload": f"VLM {n_chord}x{n_span} rings (N={N}), AR 10, taper 0.4, sweep 25 deg, cosine spacing; "
                     f"{n_span}-element Euler-Bernoulli beam; {n_cases} load cases alpha 0..{alpha_max} deg",
@@ -708,9 +713,10 @@ def aeroelastic_bench(n_chord=64, n_span=256, n_cases=32, alpha_max=8.0, cpu=Tru
         nc2, ns2 = 16, 64
         x2, r2, s2, ref2 = V.wing_lattice(nc2, ns2)
         b2 = V.wing_beam(x2, s2, nc2)
-        h2 = V.VLM(x2, r2, s2)
-        h2.set_flow(alpha_max, sref=ref2["sref"], cref=ref2["cref"])
-        i2 = V.Aeroelastic(h2, V.Beam(*b2)).solve(tol=1e-11, max_iter=80)
+        with V.VLM(x2, r2, s2) as h2, V.Beam(*b2) as bm2:
+            h2.set_flow(alpha_max, sref=ref2["sref"], cref=ref2["cref"])
+            with V.Aeroelastic(h2, bm2) as ae2:
+                i2 = ae2.solve(tol=1e-11, max_iter=80)
         t0 = time.perf_counter()
         o2 = ov.aeroelastic(x2, r2, s2, alpha_max, b2, sref=ref2["sref"], cref=ref2["cref"], tol=1e-11,
                             max_iter=80)
@@ -919,6 +925,8 @@ def main():
         "fp32_three_kernel_ms": None if unf is None else {k: round(v, 5) for k, v in unf.items()},
     }
     if args.sweep_steps > 0 or args.adjoint:
+        s.close()
+        m.close()
         del s, m
     if args.adjoint:
         out["adjoint"] = adjoint_check(rans)
@@ -937,4 +945,9 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    try:
+        main()
+    finally:
+        # every device handle is released here, before interpreter shutdown
+        if "aeroflex_amd.rans" in sys.modules:
+            sys.modules["aeroflex_amd.rans"].close_all()

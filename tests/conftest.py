@@ -8,3 +8,12 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP library)")
+
+
+def pytest_sessionfinish(session, exitstatus):
+    # release every device handle the tests left open before interpreter
+    # shutdown (solvers, batches, lattices, dense factors: streams, device
+    # memory and rocBLAS handles)
+    mod = sys.modules.get("aeroflex_amd._handle")
+    if mod is not None:
+        mod.close_all()

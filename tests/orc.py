@@ -6,7 +6,9 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle_rans.so")
+# AFX_ORACLE_LIB selects another build of the oracle (a build with a planted
+# mistake, to show that a pin fails on it)
+ORACLE_LIB = os.environ.get("AFX_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle_rans.so")
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 BC = {"farfield": 1, "slip-wall": 2, "wall": 3}
