@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -1238,6 +1239,90 @@ int afx_rans_residual(afx_rans* h, int32_t precision, double* R, double* norm2) 
         if (R && (rc = download_aos32(h, h->R32.p, R))) return rc;
         if (norm2 && (rc = norm_of32(h, h->R32.p, h->nT, NV, norm2))) return rc;
     }
+    return AFX_OK;
+}
+
+// ------------------------------------------ allocation-free residual norm --
+// afx_rans_residual_norm_dev keeps the partial sums of its norm reduction in
+// this file-scope device array instead of the handle's buffers.  Every GPU
+// holds its own copy for the life of the process.  Layout per call: the
+// partial sums of up to two reductions (kNormParts x kNormBlocks), then their
+// kNormParts totals.  Every slot a call reads is written earlier in the same
+// call (one partial per block of k_sumsq, one total per k_final_sum), so
+// nothing needs clearing; overlapping calls on one GPU share the array and
+// race.
+constexpr int kNormParts = 2;       // FP32 Spalart-Allmaras: float4 rows + the rho nu~ row
+constexpr int kNormBlocks = 1024;   // blocks of one norm reduction (the handles' nblk_red)
+constexpr int kScratchDoubles = kNormParts * kNormBlocks + kNormParts;
+static __device__ __align__(256) double scratch[kScratchDoubles];   // 16 KiB + 16 B per GPU
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxScratchDevices] = {};
+
+// device address of `scratch` on the current device (looked up once per
+// device; the symbol's host-side name is not a device address)
+int scratch_of_current_device(double** out) {
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) return fail(AFX_EUNSUP, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(AFX_EHIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return AFX_OK;
+}
+
+// out = sqrt(sum) for one part; with two parts sqrt(n0^2 + n1^2) of
+// n_k = sqrt(sum_k), the host arithmetic of afx_rans_residual
+__global__ void k_norm_from_sums(const double* __restrict__ sums, int parts, double* __restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double n0 = sqrt(sums[0]);
+        if (parts == 1) {
+            out[0] = n0;
+        } else {
+            const double n1 = sqrt(sums[1]);
+            out[0] = sqrt(n0 * n0 + n1 * n1);
+        }
+    }
+}
+}  // namespace
+
+int afx_rans_residual_norm_dev(afx_rans* h, int32_t precision, double* norm2_dev) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (precision != AFX_FP64 && precision != AFX_FP32) return fail(AFX_EARG, "precision must be AFX_FP64 or AFX_FP32");
+    if (!norm2_dev) return fail(AFX_EARG, "null device output");
+    if (h->nblk_red > kNormBlocks)
+        return fail(AFX_EUNSUP, "the norm reduction needs " + std::to_string(h->nblk_red) +
+                                    " partial sums; the allocation-free form holds " + std::to_string(kNormBlocks));
+    double* sc = nullptr;
+    if ((rc = scratch_of_current_device(&sc))) return rc;
+    double* sums = sc + kNormParts * kNormBlocks;
+    const int nb = h->nblk_red;
+    hipStream_t s = h->stream;
+    if ((rc = enqueue_residual(h, precision))) return rc;
+    int parts = 1;
+    if (precision == AFX_FP64) {
+        HIPCHK(launch_sumsq(s, h->R.p, h->nT, h->nT, h->nv, sc, nb));
+    } else if (h->r32_aos) {
+        HIPCHK(launch_sumsq_f32(s, reinterpret_cast<const float*>(h->R32v.p), (int64_t)NV * h->nT,
+                                (int64_t)NV * h->nT, 1, sc, nb));
+        if (h->sa()) {
+            HIPCHK(launch_sumsq_f32(s, h->R32n.p, h->nT, h->nT, 1, sc + kNormBlocks, nb));
+            parts = 2;
+        }
+    } else {
+        HIPCHK(launch_sumsq_f32(s, h->R32.p, h->nT, h->nT, NV, sc, nb));
+    }
+    for (int k = 0; k < parts; ++k) HIPCHK(launch_final_sum(s, sc + k * kNormBlocks, nb, sums + k));
+    hipLaunchKernelGGL(k_norm_from_sums, dim3(1), dim3(64), 0, s, sums, parts, norm2_dev);
+    HIPCHK(hipGetLastError());
     return AFX_OK;
 }
 

@@ -84,6 +84,7 @@ def _load():
         "afx_rans_calc_dt": (C.c_int, [P, dp]),
         "afx_rans_gradients": (C.c_int, [P, i32, dp, dp, dp]),
         "afx_rans_residual": (C.c_int, [P, i32, dp, dp]),
+        "afx_rans_residual_norm_dev": (C.c_int, [P, i32, P]),
         "afx_rans_explicit_residual": (C.c_int, [P, dp, dp]),
         "afx_rans_explicit_step": (C.c_int, [P, f64, dp]),
         "afx_rans_uniform_residual": (C.c_int, [P, dp]),

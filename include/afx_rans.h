@@ -161,6 +161,16 @@ int afx_rans_gradients(afx_rans* h, int32_t precision, double* gx, double* gy, d
  * only enqueues the kernels on the handle's stream and returns without
  * synchronising (inputs and outputs stay resident in HBM). */
 int afx_rans_residual(afx_rans* h, int32_t precision, double* R, double* norm2);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory or synchronise with the host (code under HIP graph capture, serving
+ * loops): afx_rans_residual(h, precision, NULL, norm2) with ||R||_2 written to
+ * the device double *norm2_dev, enqueued on the handle's stream without any
+ * allocation or host synchronisation (R stays in the handle).  The partial
+ * sums of the norm live in one static 16 KiB array per GPU, shared by every
+ * call on that GPU: calls that overlap on one GPU (different handles or
+ * streams) race.  Limit: 1024 partial sums per reduction (AFX_EUNSUP beyond).
+ * The result equals afx_rans_residual's norm2. */
+int afx_rans_residual_norm_dev(afx_rans* h, int32_t precision, double* norm2_dev);
 /* explicitSolver::calc_residual (solver.h:745-799) for a stage state qk
  * (AoS), with gradients from the current q (A3); qW out (AoS, nullable) */
 int afx_rans_explicit_residual(afx_rans* h, const double* qk, double* qW);
