@@ -2222,6 +2222,7 @@ struct afx_batch {
     int32_t nitems = 0;
     size_t lds = 0, lds_dt = 0;
     bool sa = false;
+    int viscous = 0, gg = 1, so = 1;   // the members' common model, gradient scheme and order
     double relax = 1.0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -2237,7 +2238,7 @@ namespace {
 int batch_step(afx_batch* b) {
     for (int st = 0; st < 3; ++st)
         HIPCHK(launch_residual_tile_f32_batch(b->stream, b->stage[st].p, b->items.p, b->nitems,
-                                              st == 0 ? b->lds_dt : b->lds, b->sa));
+                                              st == 0 ? b->lds_dt : b->lds, b->viscous, b->gg, b->so));
     return AFX_OK;
 }
 int batch_tables(afx_batch* b) {
@@ -2292,8 +2293,11 @@ int afx_batch_create(int32_t n, afx_rans* const* members, double relaxation, afx
             rc = fail(AFX_EUNSUP, "batch members need the fused tile kernels (renumber = 1, fused = 1)");
         if (!rc && h->michalak())
             rc = fail(AFX_EUNSUP, "batched stepping runs the default limiter (the FP32 fused kernel), not the Michalak build");
-        if (!rc && i > 0 && h->sa() != members[0]->sa())
-            rc = fail(AFX_EARG, "batch members must all use the same number of variables");
+        if (!rc && i > 0 && (h->opts.viscosity != members[0]->opts.viscosity ||
+                             h->opts.gradient != members[0]->opts.gradient ||
+                             h->opts.second_order != members[0]->opts.second_order))
+            rc = fail(AFX_EARG, "batch members must share the viscosity model, gradient scheme and order "
+                                "(one kernel instantiation per launch)");
         if (!rc && h->sa() && !h->wd_ready) rc = compute_wall_distance(h);
         if (!rc && !h->q32v_b.p) {   // the two further FP32 stage buffers
             if (h->q32v_b.alloc(h->nT) != hipSuccess || h->q32v_c.alloc(h->nT) != hipSuccess ||
@@ -2304,6 +2308,9 @@ int afx_batch_create(int32_t n, afx_rans* const* members, double relaxation, afx
         b->mem.push_back(h);
     }
     b->sa = members[0]->sa();
+    b->viscous = members[0]->opts.viscosity;
+    b->gg = members[0]->opts.gradient;
+    b->so = members[0]->opts.second_order;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         delete b;
         return fail(AFX_EHIP, "hipStreamCreateWithFlags failed");

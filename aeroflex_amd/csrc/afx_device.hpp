@@ -1015,10 +1015,35 @@ __device__ __forceinline__ uint32_t slot_of(const uint2& w, int s) {
 }
 #define F4_LOOP(expr) { expr(x); expr(y); expr(z); expr(w); }
 
+// Limiter of one variable of one cell (solver.h:536-591) from the extreme
+// face projections.  Per face the reference takes, for dqg = grad q . d_f,
+// phi(dqg, D) = (D^2 + K3a + 2 dqg D) / (D^2 + K3a + dqg D + 2 dqg^2) with
+// D = q_max - q when dqg > 1e-16, D = q_min - q when dqg < -1e-16 (else 1), and
+// the cell's limiter is the minimum over its faces, at most 1.  phi is even
+// in (dqg, D) -> (-dqg, -D).  For dqg > 0 and D >= 0, phi >= 1 exactly when
+// dqg <= D/2, and for dqg >= D/4 its derivative is proportional to
+// (D - 4 dqg)(D^2 + K3a) - 4 D dqg^2 < 0, so min(1, phi) over the faces is
+// min(1, phi(ymax, D+)) with ymax the largest projection, and likewise
+// phi(ymin, D-) for the negative side: two evaluations per variable instead
+// of one per face.  ymax, ymin start at 0 (faces at or below the 1e-16
+// threshold give 1 and are skipped).
+__device__ __forceinline__ float limiter_extremes(float ymax, float ymin, float dM, float dm, float K3a) {
+    const float DDp = fmaf(dM, dM, K3a), DDm = fmaf(dm, dm, K3a);
+    const float tp = ymax * dM, tm = ymin * dm;
+    const float pp = fmaf(2.f, tp, DDp) * frcp(fmaf(2.f * ymax, ymax, DDp + tp));
+    const float pm = fmaf(2.f, tm, DDm) * frcp(fmaf(2.f * ymin, ymin, DDm + tm));
+    float l = 1.f;
+    l = ymax > 1e-16f ? fminf(l, pp) : l;
+    l = ymin < -1e-16f ? fminf(l, pm) : l;
+    return l;
+}
+
 // one tile of the FP32 residual (the body of both the single-condition and
-// the batched kernels)
-template <bool SA, bool STAGE>
+// the batched kernels).  MODEL: 0 inviscid, 1 laminar, 2 Spalart-Allmaras;
+// GG: Green-Gauss (else least squares) gradients; SO: second order.
+template <int MODEL, bool STAGE, bool GG, bool SO>
 __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const int t, float4* smem4) {
+    constexpr bool SA = MODEL == 2;
     const TilesF32& S = A.S;
     const float4* __restrict__ q = A.q;
     const int32_t nC = A.nC;
@@ -1028,7 +1053,6 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
     const float* __restrict__ xc = A.xc;
     const float* __restrict__ yc = A.yc;
     const GasT<float> g = A.g;
-    const int gg = A.gg, second_order = A.second_order, viscous = A.viscous;
     const float K3 = A.K3;
     float4* __restrict__ R = A.R;
     float4* __restrict__ qghost = A.qghost;
@@ -1074,12 +1098,20 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
         const int gc = Lg[c];
         const float4 qc = sq[c];
         float4 qmin = qc, qmax = qc;
-        float4 sx = f4(0.f, 0.f, 0.f, 0.f), sy = f4(0.f, 0.f, 0.f, 0.f);
+        float4 sx, sy;
         const float nuc = SA ? sqn[c] : 0.f;
         float sxn = 0.f, syn = 0.f, mnn = nuc, mxn = nuc;
         float dxs[4], dys[4];
         int ns = 0;
-        if (gg) {
+        if constexpr (GG) {
+            // Green-Gauss (solver.h:428-469): face value a q_c + b q_other
+            // (interior: the weights 1 - g/2 and g/2 of physics.h:275, the
+            // same face value for both cells; boundary: 1 - g and g with the
+            // BC state), times n len / A.  Summed as
+            //   q_c sum_s a_s n_s len_s / A + sum_s b_s q_other,s n_s len_s / A.
+            const float ia = iarea[gc];
+            float ax = 0.f, ay = 0.f, bxn = 0.f, byn = 0.f;
+            float4 bx = f4(0.f, 0.f, 0.f, 0.f), by = f4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const uint32_t e = slot_of(sw, s);
@@ -1089,25 +1121,29 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
                 const uint32_t fc = fck[f];
                 const int kind = (int)(fc >> 24);
                 const int other = nb ? (int)(fc & 0xFFFu) : (int)((fc >> 12) & 0xFFFu);
-                const float4 ga = geo[2 * f], gb = geo[2 * f + 1];   // (nx, ny, len, g), (d0x, d0y, d1x, d1y)
-                dxs[s] = nb ? gb.z : gb.x;
-                dys[s] = nb ? gb.w : gb.y;
+                const float4 ga = geo[2 * f];   // (nx, ny, len, g)
+                if constexpr (SO) {
+                    const float4 gb = geo[2 * f + 1];   // (d0x, d0y, d1x, d1y)
+                    dxs[s] = nb ? gb.z : gb.x;
+                    dys[s] = nb ? gb.w : gb.y;
+                }
                 ns = s + 1;
-                const float sl = nb ? -ga.z : ga.z;
+                const float sl = (nb ? -ga.z : ga.z) * ia;
                 const float nlx = ga.x * sl, nly = ga.y * sl;
+                float wa, wb, nuo = 0.f;
+                float4 qo;
                 if (kind == K_INTERNAL) {
-                    const float4 qn = sq[other];
+                    qo = sq[other];
                     const float wn = 0.5f * ga.w, wo = 1.f - wn;
-                    const float a = nb ? wn : wo, b = nb ? wo : wn;   // weights of this cell and the other
-#define GG_INT(m) { const float fv = a * qc.m + b * qn.m; sx.m += fv * nlx; sy.m += fv * nly; \
-                    qmin.m = fminf(qmin.m, qn.m); qmax.m = fmaxf(qmax.m, qn.m); }
-                    F4_LOOP(GG_INT)
-#undef GG_INT
+                    wa = nb ? wn : wo;
+                    wb = nb ? wo : wn;
+#define MINMAX(m) { qmin.m = fminf(qmin.m, qo.m); qmax.m = fmaxf(qmax.m, qo.m); }
+                    F4_LOOP(MINMAX)
+#undef MINMAX
                     if constexpr (SA) {
-                        const float nun = sqn[other];
-                        const float fv = a * nuc + b * nun;
-                        sxn += fv * nlx; syn += fv * nly;
-                        mnn = fminf(mnn, nun); mxn = fmaxf(mxn, nun);
+                        nuo = sqn[other];
+                        mnn = fminf(mnn, nuo);
+                        mxn = fmaxf(mxn, nuo);
                     }
                 } else {
                     float qb[4] = {0.f, 0.f, 0.f, 0.f}, qv[4];
@@ -1121,23 +1157,31 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
                         if constexpr (SA) { nug = sqn[other]; mnn = fminf(mnn, nug); mxn = fmaxf(mxn, nug); }
                     }
                     face_vars<float>(g, kind, ga.x, ga.y, qa, qb, qv);
-                    const float w1 = 1.f - ga.w;
-                    const float fv0 = qa[0] * w1 + qv[0] * ga.w, fv1 = qa[1] * w1 + qv[1] * ga.w;
-                    const float fv2 = qa[2] * w1 + qv[2] * ga.w, fv3 = qa[3] * w1 + qv[3] * ga.w;
-                    sx.x += fv0 * nlx; sx.y += fv1 * nlx; sx.z += fv2 * nlx; sx.w += fv3 * nlx;
-                    sy.x += fv0 * nly; sy.y += fv1 * nly; sy.z += fv2 * nly; sy.w += fv3 * nly;
-                    if constexpr (SA) {
-                        const float vr = sa_bnd_nu<float>(kind, ga.x, ga.y, qa, nuc, qb, nug, qv);
-                        const float fv = nuc * w1 + vr * ga.w;
-                        sxn += fv * nlx; syn += fv * nly;
-                    }
+                    qo = f4(qv[0], qv[1], qv[2], qv[3]);
+                    wa = 1.f - ga.w;
+                    wb = ga.w;
+                    if constexpr (SA) nuo = sa_bnd_nu<float>(kind, ga.x, ga.y, qa, nuc, qb, nug, qv);
+                }
+                ax = fmaf(wa, nlx, ax);
+                ay = fmaf(wa, nly, ay);
+                const float bnx = wb * nlx, bny = wb * nly;
+#define BSUM(m) { bx.m = fmaf(bnx, qo.m, bx.m); by.m = fmaf(bny, qo.m, by.m); }
+                F4_LOOP(BSUM)
+#undef BSUM
+                if constexpr (SA) {
+                    bxn = fmaf(bnx, nuo, bxn);
+                    byn = fmaf(bny, nuo, byn);
                 }
             }
-            const float ia = iarea[gc];
-            sx = f4(sx.x * ia, sx.y * ia, sx.z * ia, sx.w * ia);
-            sy = f4(sy.x * ia, sy.y * ia, sy.z * ia, sy.w * ia);
-            sxn *= ia; syn *= ia;
+#define GSUM(m) { sx.m = fmaf(qc.m, ax, bx.m); sy.m = fmaf(qc.m, ay, by.m); }
+            F4_LOOP(GSUM)
+#undef GSUM
+            if constexpr (SA) {
+                sxn = fmaf(nuc, ax, bxn);
+                syn = fmaf(nuc, ay, byn);
+            }
         } else {
+            // least squares (solver.h:470-513, quirk A2)
             float rx[4] = {0.f, 0.f, 0.f, 0.f}, ry[4] = {0.f, 0.f, 0.f, 0.f};
             const float qa[4] = {qc.x, qc.y, qc.z, qc.w};
             float mn[4] = {qc.x, qc.y, qc.z, qc.w}, mx[4] = {qc.x, qc.y, qc.z, qc.w};
@@ -1190,41 +1234,27 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
         sgx[c] = sx;
         sgy[c] = sy;
         if constexpr (SA) sgn[c] = make_float2(sxn, syn);
-        if (second_order) {
+        if constexpr (SO) {
             const float K3a = K3 * a15[gc];
-            float4 lm = f4(1.f, 1.f, 1.f, 1.f);
-            const float4 dM = f4(qmax.x - qc.x, qmax.y - qc.y, qmax.z - qc.z, qmax.w - qc.w);
-            const float4 dm = f4(qmin.x - qc.x, qmin.y - qc.y, qmin.z - qc.z, qmin.w - qc.w);
-            const float4 ddM = f4(dM.x * dM.x + K3a, dM.y * dM.y + K3a, dM.z * dM.z + K3a, dM.w * dM.w + K3a);
-            const float4 ddm = f4(dm.x * dm.x + K3a, dm.y * dm.y + K3a, dm.z * dm.z + K3a, dm.w * dm.w + K3a);
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                if (s >= ns) break;
-                const float dx = dxs[s], dy = dys[s];
-                // D has the sign of dqg, so the denominator D^2 + K3a + dqg D
-                // + 2 dqg^2 is positive whenever dqg != 0 (no NaN reaches fminf)
-#define LIM(m) { const float dqg = sx.m * dx + sy.m * dy; const bool pos = dqg > 1e-16f, neg = dqg < -1e-16f; \
-                 const float D = pos ? dM.m : dm.m, DD = pos ? ddM.m : ddm.m; const float tq = dqg * D; \
-                 float phi = (DD + 2.f * tq) * frcp(DD + tq + 2.f * dqg * dqg); phi = (pos || neg) ? phi : 1.f; \
-                 lm.m = fminf(lm.m, phi); }
-                F4_LOOP(LIM)
+            float4 lm;
+#define LIM(m) { float ymax = 0.f, ymin = 0.f;                                         \
+                 _Pragma("unroll") for (int s = 0; s < 4; ++s) if (s < ns) {           \
+                     const float y = fmaf(sx.m, dxs[s], sy.m * dys[s]);                \
+                     ymax = fmaxf(ymax, y); ymin = fminf(ymin, y); }                   \
+                 lm.m = limiter_extremes(ymax, ymin, qmax.m - qc.m, qmin.m - qc.m, K3a); }
+            F4_LOOP(LIM)
 #undef LIM
-            }
             slm[c] = lm;
             if constexpr (SA) {
-                const float dM = mxn - nuc, dm = mnn - nuc;
-                const float ddM = dM * dM + K3a, ddm = dm * dm + K3a;
-                float l = 1.f;
-                for (int s = 0; s < ns; ++s) {
-                    const float dqg = sxn * dxs[s] + syn * dys[s];
-                    const bool pos = dqg > 1e-16f, neg = dqg < -1e-16f;
-                    const float D = pos ? dM : dm, DD = pos ? ddM : ddm;
-                    const float tq = dqg * D;
-                    float phi = (DD + 2.f * tq) * frcp(DD + tq + 2.f * dqg * dqg);
-                    phi = (pos || neg) ? phi : 1.f;
-                    l = fminf(l, phi);
-                }
-                slmn[c] = l;
+                float ymax = 0.f, ymin = 0.f;
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    if (s < ns) {
+                        const float y = fmaf(sxn, dxs[s], syn * dys[s]);
+                        ymax = fmaxf(ymax, y);
+                        ymin = fminf(ymin, y);
+                    }
+                slmn[c] = limiter_extremes(ymax, ymin, mxn - nuc, mnn - nuc, K3a);
             }
         }
     }
@@ -1263,14 +1293,14 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
                 sev2[fl] = kmax * (g.k + mut * frcp(g.PrT)) * frcp(rho) * ga.z * ga.z;
             }
         }
-        if (second_order) {
+        if constexpr (SO) {
             const float4 ax = sgx[o], ay = sgy[o], al = slm[o];
-#define RECL(m) qL.m += (ax.m * gb.x + ay.m * gb.y) * al.m;
+#define RECL(m) qL.m = fmaf(fmaf(ax.m, gb.x, ay.m * gb.y), al.m, qL.m);
             F4_LOOP(RECL)
 #undef RECL
             if (n < nG) {
                 const float4 bx = sgx[n], by = sgy[n], bl = slm[n];
-#define RECR(m) qR.m += (bx.m * gb.z + by.m * gb.w) * bl.m;
+#define RECR(m) qR.m = fmaf(fmaf(bx.m, gb.z, by.m * gb.w), bl.m, qR.m);
                 F4_LOOP(RECR)
 #undef RECR
             }
@@ -1278,24 +1308,26 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
         float F[5];
         if constexpr (SA) {
             const bool staged = (kind == K_INTERNAL || kind == K_FARFIELD);
-            float nL = sqn[o], nR = staged ? sqn[n] : 0.f;
-            if (second_order) {
+            const float nuo_c = sqn[o];
+            float nL = nuo_c, nR = staged ? sqn[n] : 0.f;
+            if constexpr (SO) {
                 const float2 ga_ = sgn[o];
-                nL += (ga_.x * gb.x + ga_.y * gb.y) * slmn[o];
-                if (n < nG) { const float2 gn2 = sgn[n]; nR += (gn2.x * gb.z + gn2.y * gb.w) * slmn[n]; }
+                nL = fmaf(fmaf(ga_.x, gb.x, ga_.y * gb.y), slmn[o], nL);
+                if (n < nG) { const float2 gn2 = sgn[n]; nR = fmaf(fmaf(gn2.x, gb.z, gn2.y * gb.w), slmn[n], nR); }
             }
             const float qL5[5] = {qL.x, qL.y, qL.z, qL.w, nL};
             const float qR5[5] = {qR.x, qR.y, qR.z, qR.w, nR};
             float gxa[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, gya[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+            const float4 so_ = sq[o];
             if (kind == K_INTERNAL) {
-                // face gradients along the cell connection (textbook direction)
-                const int go_ = Lg[o], gn_ = Lg[n];
-                float t0 = xc[gn_] - xc[go_], t1 = yc[gn_] - yc[go_];
-                const float il = frcp(__builtin_amdgcn_sqrtf(t0 * t0 + t1 * t1));
+                // face gradients along the cell connection (textbook
+                // direction): x_n - x_o = d_0 - d_1 from the face offsets
+                float t0 = gb.x - gb.z, t1 = gb.y - gb.w;
+                const float il = __builtin_amdgcn_rsqf(fmaf(t0, t0, t1 * t1));
                 t0 *= il; t1 *= il;
-                const float4 so = sq[o], sn = sq[n], xo = sgx[o], xn = sgx[n], yo = sgy[o], yn = sgy[n];
+                const float4 sn = sq[n], xo = sgx[o], xn = sgx[n], yo = sgy[o], yn = sgy[n];
                 const float2 go2 = sgn[o], gn2 = sgn[n];
-                const float so5[5] = {so.x, so.y, so.z, so.w, sqn[o]}, sn5[5] = {sn.x, sn.y, sn.z, sn.w, sqn[n]};
+                const float so5[5] = {so_.x, so_.y, so_.z, so_.w, nuo_c}, sn5[5] = {sn.x, sn.y, sn.z, sn.w, sqn[n]};
                 const float xo5[5] = {xo.x, xo.y, xo.z, xo.w, go2.x}, xn5[5] = {xn.x, xn.y, xn.z, xn.w, gn2.x};
                 const float yo5[5] = {yo.x, yo.y, yo.z, yo.w, go2.y}, yn5[5] = {yn.x, yn.y, yn.z, yn.w, gn2.y};
 #pragma unroll
@@ -1303,13 +1335,12 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
                     const float dir = (sn5[k] - so5[k]) * il;
                     const float gxb = 0.5f * (xo5[k] + xn5[k]);
                     const float gyb = 0.5f * (yo5[k] + yn5[k]);
-                    const float dot = gxb * t0 + gyb * t1;
-                    gxa[k] = gxb - (dot - dir) * t0;
-                    gya[k] = gyb - (dot - dir) * t1;
+                    const float corr = fmaf(gxb, t0, gyb * t1) - dir;
+                    gxa[k] = fmaf(-corr, t0, gxb);
+                    gya[k] = fmaf(-corr, t1, gyb);
                 }
             }
-            const float4 so_ = sq[o];
-            sa_face_flux_f32(g, kind, ga.x, ga.y, qL5, qR5, gxa, gya, sqn[o] * frcp(so_.x),
+            sa_face_flux_f32(g, kind, ga.x, ga.y, qL5, qR5, gxa, gya, nuo_c * frcp(so_.x),
                              gb.x * ga.x + gb.y * ga.y, F);
             sfl[fl] = f4(F[0] * ga.z, F[1] * ga.z, F[2] * ga.z, F[3] * ga.z);
             sfln[fl] = F[4] * ga.z;
@@ -1318,7 +1349,9 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
         const float qLa[4] = {qL.x, qL.y, qL.z, qL.w};
         float qRa[4] = {qR.x, qR.y, qR.z, qR.w};
         if (kind == K_INTERNAL) {
-            if (viscous == 1) {
+            if constexpr (MODEL == 1) {
+                // averaged face gradient with the reference's direction (A1:
+                // the normalised sum of the two centroids, solver.h:359-398)
                 float gxa[4], gya[4];
                 const int go_ = Lg[o], gn_ = Lg[n];
                 float t0 = xc[go_] + xc[gn_], t1 = yc[go_] + yc[gn_];
@@ -1416,48 +1449,66 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
     }
 }
 
-template <bool SA>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA ? 5 : 6)))
+template <int MODEL, bool GG, bool SO>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MODEL == 2 ? 5 : 6)))
 k_residual_tile_f32(TileArgsF32 A) {
     extern __shared__ float4 smem4[];
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= A.S.ntiles) return;
-    tile_residual_f32<SA, false>(A, t, smem4);
+    tile_residual_f32<MODEL, false, GG, SO>(A, t, smem4);
 }
 
 // batched explicit stage: residual + stage update of every (member, tile)
-template <bool SA>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SA ? 4 : 5)))
+template <int MODEL, bool GG, bool SO>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MODEL == 2 ? 4 : 5)))
 k_residual_tile_f32_batch(const TileArgsF32* __restrict__ members, const int2* __restrict__ items) {
     extern __shared__ float4 smem4[];
     const int2 it = items[xcd_tile(blockIdx.x, gridDim.x)];
     if (it.x < 0) return;
-    tile_residual_f32<SA, true>(members[it.x], it.y, smem4);
+    tile_residual_f32<MODEL, true, GG, SO>(members[it.x], it.y, smem4);
 }
+
+// the (model, gradient scheme, order) instantiation for the run-time options
+#define AFX_TILE32_DISPATCH(KERNEL, MODEL_, GG_, SO_, ...)                                                 \
+    do {                                                                                                   \
+        const int sel_ = (MODEL_) * 4 + ((GG_) ? 2 : 0) + ((SO_) ? 1 : 0);                                 \
+        switch (sel_) {                                                                                    \
+            case 0: hipLaunchKernelGGL((KERNEL<0, false, false>), __VA_ARGS__); break;                     \
+            case 1: hipLaunchKernelGGL((KERNEL<0, false, true>), __VA_ARGS__); break;                      \
+            case 2: hipLaunchKernelGGL((KERNEL<0, true, false>), __VA_ARGS__); break;                      \
+            case 3: hipLaunchKernelGGL((KERNEL<0, true, true>), __VA_ARGS__); break;                       \
+            case 4: hipLaunchKernelGGL((KERNEL<1, false, false>), __VA_ARGS__); break;                     \
+            case 5: hipLaunchKernelGGL((KERNEL<1, false, true>), __VA_ARGS__); break;                      \
+            case 6: hipLaunchKernelGGL((KERNEL<1, true, false>), __VA_ARGS__); break;                      \
+            case 7: hipLaunchKernelGGL((KERNEL<1, true, true>), __VA_ARGS__); break;                       \
+            case 8: hipLaunchKernelGGL((KERNEL<2, false, false>), __VA_ARGS__); break;                     \
+            case 9: hipLaunchKernelGGL((KERNEL<2, false, true>), __VA_ARGS__); break;                      \
+            case 10: hipLaunchKernelGGL((KERNEL<2, true, false>), __VA_ARGS__); break;                     \
+            default: hipLaunchKernelGGL((KERNEL<2, true, true>), __VA_ARGS__); break;                      \
+        }                                                                                                  \
+    } while (0)
 
 hipError_t launch_residual_tile_f32(hipStream_t s, TilesF32 S, const float4* q, int32_t nC, const float* iarea,
                                     const float* a15, const float* lsq, const float* xc, const float* yc,
                                     GasT<float> g, int gg, int second_order, float K3, int viscous, float4* R,
                                     float4* qghost, SAFields32 sa) {
+    if (viscous < 0 || viscous > 2) return hipErrorInvalidValue;
     const int nb = ((S.ntiles + 7) / 8) * 8;
     TileArgsF32 A{S, q, iarea, a15, lsq, xc, yc, R, qghost, sa, nullptr, g, K3, 0.f, nC, gg, second_order, viscous,
                   nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, 0};
-    if (viscous == 2)
-        hipLaunchKernelGGL(k_residual_tile_f32<true>, dim3(nb), dim3(kBlock), tile_lds_bytes_f32(S, 5, false), s, A);
-    else
-        hipLaunchKernelGGL(k_residual_tile_f32<false>, dim3(nb), dim3(kBlock), tile_lds_bytes_f32(S, 4, false), s, A);
+    const size_t lds = tile_lds_bytes_f32(S, viscous == 2 ? 5 : 4, false);
+    AFX_TILE32_DISPATCH(k_residual_tile_f32, viscous, gg != 0, second_order != 0, dim3(nb), dim3(kBlock), lds, s, A);
     return hipGetLastError();
 }
 
 hipError_t launch_residual_tile_f32_batch(hipStream_t s, const TileArgsF32* members, const int2* items, int32_t nitems,
-                                          size_t lds, bool sa) {
-    if (nitems <= 0 || (nitems & 7)) return hipErrorInvalidValue;
-    if (sa)
-        hipLaunchKernelGGL(k_residual_tile_f32_batch<true>, dim3(nitems), dim3(kBlock), lds, s, members, items);
-    else
-        hipLaunchKernelGGL(k_residual_tile_f32_batch<false>, dim3(nitems), dim3(kBlock), lds, s, members, items);
+                                          size_t lds, int viscous, int gg, int second_order) {
+    if (nitems <= 0 || (nitems & 7) || viscous < 0 || viscous > 2) return hipErrorInvalidValue;
+    AFX_TILE32_DISPATCH(k_residual_tile_f32_batch, viscous, gg != 0, second_order != 0, dim3(nitems), dim3(kBlock),
+                        lds, s, members, items);
     return hipGetLastError();
 }
+#undef AFX_TILE32_DISPATCH
 #endif  // AFX_FP32_UNIT
 
 // ---------------------------------------------------------------- launchers --

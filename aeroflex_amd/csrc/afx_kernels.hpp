@@ -156,9 +156,10 @@ inline size_t tile_lds_bytes_f32(const TilesF32& S, int nv, bool dt) {
 }
 // one launch over the (member, tile) work items of many meshes / conditions;
 // items padded with member -1 to a multiple of 8 blocks; lds = the largest
-// member's tile_lds_bytes_f32; sa: every member runs the SA model
+// member's tile_lds_bytes_f32; every member runs the same viscosity model
+// (0/1/2), gradient scheme (gg) and order
 hipError_t launch_residual_tile_f32_batch(hipStream_t s, const TileArgsF32* members, const int2* items, int32_t nitems,
-                                          size_t lds, bool sa);
+                                          size_t lds, int viscous, int gg, int second_order);
 
 // SA explicit step of one solver: qk = q + R / A dt coef (real cells, nv planes), qW = R / A
 hipError_t launch_rk_update_area(hipStream_t s, const double* q, const double* R, const double* area, const double* dt,

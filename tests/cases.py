@@ -57,3 +57,47 @@ def make_pair(path=QUAD, viscosity="inviscid", wall="slip-wall", second_order=Tr
 
 def rel_max(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def elements_of(mesh):
+    """Elements of a product mesh (nodes, cells, boundary segments, keys), in
+    the form orc.OMesh.build takes."""
+    c = mesh.counts()
+    nr = c["real_cells"]
+    cells = mesh.get("cellsNodes").reshape(-1, 4)[:nr].astype(np.int32)
+    nv = np.where(mesh.get("cellsIsTriangle")[:nr] == 1, 3, 4).astype(np.int32)
+    en = mesh.get("edgesNodes").reshape(-1, 2)
+    be = mesh.get("boundaryEdges")
+    bsegs = en[be].astype(np.int32)
+    bkeys = mesh.get("boundaryEdgesIntTag").astype(np.int32)
+    return mesh.get("nodesX"), mesh.get("nodesY"), cells, nv, bsegs, bkeys
+
+
+GENERATED_NAMES = {1001: "farfield", 1002: "wall", 2003: "internal"}
+
+
+def oracle_mesh_of(mesh):
+    """The oracle's mesh built from a generated product mesh's elements."""
+    return orc.OMesh.build(*elements_of(mesh), GENERATED_NAMES)
+
+
+def oracle_sa_explicit_steps(o, q, nu, area, n_real, steps, relaxation):
+    """The reference's three-stage explicit step (solver.h:802-828) driven with
+    the oracle's SA residual and SA time step: per step, dt of the stage-0
+    state, then stages 1/4, 1/2, 1 from the stage-0 state.  Returns (q, nu)."""
+    nr = n_real
+    for _ in range(steps):
+        q0, nu0 = q.copy(), nu.copy()
+        o.vec("q", q0)
+        o.sa_vec("nu", nu0)
+        dt = o.sa_calc_dt()
+        for ai in (0.25, 0.5, 1.0):
+            o.vec("q", q)
+            o.sa_vec("nu", nu)
+            R, T = o.sa_fill_rhs()
+            q, nu = o.vec("q").copy(), o.sa_vec("nu").copy()   # wall ghosts refreshed by the residual
+            w = dt[:nr] / area[:nr] * ai * relaxation
+            for k in range(4):
+                q[k:4 * nr:4] = q0[k:4 * nr:4] + R[k:4 * nr:4] * w
+            nu[:nr] = nu0[:nr] + T[:nr] * w
+    return q, nu

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import orc
+from cases import elements_of
 from cases import QUAD, TRI
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -35,19 +36,6 @@ def assert_same_mesh(a, o):
 @pytest.mark.parametrize("path", [QUAD, TRI])
 def test_reader_bitexact_vs_oracle(path):
     assert_same_mesh(rans().Mesh.read_file(path), orc.OMesh.read(path))
-
-
-def elements_of(mesh):
-    """Elements of a product mesh (nodes, cells, boundary segments)."""
-    c = mesh.counts()
-    nr = c["real_cells"]
-    cells = mesh.get("cellsNodes").reshape(-1, 4)[:nr].astype(np.int32)
-    nv = np.where(mesh.get("cellsIsTriangle")[:nr] == 1, 3, 4).astype(np.int32)
-    en = mesh.get("edgesNodes").reshape(-1, 2)
-    be = mesh.get("boundaryEdges")
-    bsegs = en[be].astype(np.int32)
-    bkeys = mesh.get("boundaryEdgesIntTag").astype(np.int32)
-    return mesh.get("nodesX"), mesh.get("nodesY"), cells, nv, bsegs, bkeys
 
 
 @pytest.mark.parametrize("kind", ["ogrid", "jittered"])
