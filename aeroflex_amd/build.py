@@ -52,7 +52,7 @@ def build(verbose=True):
         objs.append(obj)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs +
-         ["-L/opt/rocm/lib", "-lrocblas", "-pthread", "-Wl,-rpath,/opt/rocm/lib"])
+         ["-pthread", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     # C++ driver over the C ABI (rans_gpu.hpp), linked against the library
     drv = os.path.join(HERE, "rans_gpu_driver")

@@ -7,14 +7,14 @@
 //   - the FP64 influence matrix, one thread per matrix entry, four Biot-Savart
 //     segments per ring (potential.cpp:36-58, solver.cpp:139-254);
 //   - FP32 LU with partial pivoting: a cooperative panel kernel (rows of the
-//     panel in LDS, one grid barrier per column), row interchanges, a
-//     wavefront triangular solve for the block row and the trailing update
-//     as an FP32 GEMM on the matrix cores (rocBLAS sgemm);
+//     panel in LDS, pivot keys exchanged between workgroups), row
+//     interchanges, a wavefront triangular solve for the block row and the
+//     trailing update as a hand-written FP32 GEMM on the matrix cores
+//     (k_gemm_sub_f32, v_mfma_f32_32x32x2f32);
 //   - FP64 iterative refinement on the FP64 matrix with the FP32 factor as
 //     the preconditioner (replaces Eigen's BiCGSTAB, solver.cpp:41-50);
 //   - Kutta-Joukowski ring forces (surface.cpp:164-201).
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <climits>
@@ -35,12 +35,6 @@ namespace {
     do {                                                                                            \
         hipError_t e_ = (x);                                                                        \
         if (e_ != hipSuccess) return api_fail(AFX_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-#define RBCHK(x)                                                                                    \
-    do {                                                                                            \
-        rocblas_status s_ = (x);                                                                    \
-        if (s_ != rocblas_status_success)                                                           \
-            return api_fail(AFX_EHIP, std::string(#x) + ": " + rocblas_status_to_string(s_));      \
     } while (0)
 
 constexpr int LU_NB = 128;          // panel width = trailing-update depth = solve block
@@ -820,6 +814,125 @@ __global__ void k_forces_stations(const double* __restrict__ corner, const doubl
     for (int d = 0; d < 3; ++d) sforce[3 * s + d] = F[d];
 }
 
+// ------------------------------------------------- trailing update (MFMA) --
+// C -= A B in FP32 on the matrix cores, all three column-major with one
+// leading dimension ld (the LU's A22 -= A21 U12: A is m x k, B is k x n).
+// One 256-thread workgroup computes a 128 x 128 tile of C: each wavefront a
+// 64 x 64 block as 2 x 2 v_mfma_f32_32x32x2f32 accumulators (16 floats per
+// lane each).  K advances in chunks of 16 staged through two LDS buffers: the
+// A chunk k-major as in memory (rows contiguous), the B chunk transposed to
+// k-major rows padded by 4 words so its column-wise stores spread over the
+// banks; the next chunk's global loads are issued before the current chunk's
+// MFMAs.  Operand layout of the 32x32x2 instruction: lane l supplies
+// A[l % 32][l / 32] and B[l / 32][l % 32] of the 32 x 2 and 2 x 32 slices;
+// accumulator element v of lane l is C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
+// Blocks map to tiles XCD-aware (consecutive tiles of one XCD share A rows).
+constexpr int GM_TM = 128, GM_TN = 128, GM_TK = 16;
+constexpr int GM_LDB = GM_TN + 4;
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__global__ void __launch_bounds__(256) k_gemm_sub_f32(const float* __restrict__ A, const float* __restrict__ B,
+                                                       float* __restrict__ C, int ld, int m, int n, int k) {
+    __shared__ float As[2][GM_TK][GM_TM];
+    __shared__ float Bs[2][GM_TK][GM_LDB];
+    const int tiles_m = (m + GM_TM - 1) / GM_TM, tiles_n = (n + GM_TN - 1) / GM_TN;
+    const int ntiles = tiles_m * tiles_n;
+    // XCD-aware: block b runs on XCD b % 8; XCD x takes tiles [x * per, (x + 1) * per)
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = (nb + 7) / 8;
+    const int tile = (b % 8) * per + b / 8;
+    if (tile >= ntiles) return;
+    const int tm = tile % tiles_m, tn = tile / tiles_m;
+    const int row0 = tm * GM_TM, col0 = tn * GM_TN;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    const bool full = row0 + GM_TM <= m && col0 + GM_TN <= n;
+    // global -> register staging: A chunk (128 rows x 16 k): column kk = t / 16,
+    // rows 8 (t % 16) .. +8; B chunk (16 k x 128 columns): column t / 2, k
+    // 8 (t % 2) .. +8
+    const int a_k = t >> 4, a_r = (t & 15) * 8;
+    const int b_c = t >> 1, b_k = (t & 1) * 8;
+    float ra[8], rb[8];
+    auto load = [&](int kb) {
+        const int ka = kb + a_k, kbb = kb + b_k;
+        if (full && kb + GM_TK <= k) {
+            const float4* pa = reinterpret_cast<const float4*>(A + (size_t)ka * ld + row0 + a_r);
+            const float4 a0 = pa[0], a1 = pa[1];
+            ra[0] = a0.x; ra[1] = a0.y; ra[2] = a0.z; ra[3] = a0.w; ra[4] = a1.x; ra[5] = a1.y; ra[6] = a1.z; ra[7] = a1.w;
+            const float4* pb = reinterpret_cast<const float4*>(B + (size_t)(col0 + b_c) * ld + kbb);
+            const float4 b0 = pb[0], b1 = pb[1];
+            rb[0] = b0.x; rb[1] = b0.y; rb[2] = b0.z; rb[3] = b0.w; rb[4] = b1.x; rb[5] = b1.y; rb[6] = b1.z; rb[7] = b1.w;
+        } else {
+            for (int i = 0; i < 8; ++i) {
+                const int r = row0 + a_r + i;
+                ra[i] = (r < m && ka < k) ? A[(size_t)ka * ld + r] : 0.f;
+                const int c = col0 + b_c, kk = kbb + i;
+                rb[i] = (c < n && kk < k) ? B[(size_t)c * ld + kk] : 0.f;
+            }
+        }
+    };
+    auto store = [&](int buf) {
+        float4* da = reinterpret_cast<float4*>(&As[buf][a_k][a_r]);
+        da[0] = make_float4(ra[0], ra[1], ra[2], ra[3]);
+        da[1] = make_float4(ra[4], ra[5], ra[6], ra[7]);
+        for (int i = 0; i < 8; ++i) Bs[buf][b_k + i][b_c] = rb[i];
+    };
+    f32x16 acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+    const int nk = (k + GM_TK - 1) / GM_TK;
+    load(0);
+    store(0);
+    __syncthreads();
+    const int lr = lane & 31, lk = lane >> 5;
+    for (int c = 0; c < nk; ++c) {
+        const int buf = c & 1;
+        if (c + 1 < nk) load((c + 1) * GM_TK);
+#pragma unroll
+        for (int kk = 0; kk < GM_TK; kk += 2) {
+            const float a0 = As[buf][kk + lk][wm + lr], a1 = As[buf][kk + lk][wm + 32 + lr];
+            const float b0 = Bs[buf][kk + lk][wn + lr], b1 = Bs[buf][kk + lk][wn + 32 + lr];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        if (c + 1 < nk) {
+            store(buf ^ 1);   // the other buffer: its last readers finished before the previous barrier
+        }
+        __syncthreads();
+    }
+    // C -= acc
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            const int col = col0 + wn + j * 32 + lr;
+            for (int g = 0; g < 4; ++g) {
+                const int row = row0 + wm + i * 32 + 8 * g + 4 * lk;   // rows row .. row + 3
+                if (full) {
+                    float4* pc = reinterpret_cast<float4*>(C + (size_t)col * ld + row);
+                    float4 cv = *pc;
+                    cv.x -= acc[i][j][4 * g + 0];
+                    cv.y -= acc[i][j][4 * g + 1];
+                    cv.z -= acc[i][j][4 * g + 2];
+                    cv.w -= acc[i][j][4 * g + 3];
+                    *pc = cv;
+                } else if (col < n) {
+                    for (int v = 0; v < 4; ++v)
+                        if (row + v < m) C[(size_t)col * ld + row + v] -= acc[i][j][4 * g + v];
+                }
+            }
+        }
+}
+
+hipError_t launch_gemm_sub_f32(hipStream_t s, const float* A, const float* B, float* C, int ld, int m, int n, int k) {
+    if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
+    const int tiles = ((m + GM_TM - 1) / GM_TM) * ((n + GM_TN - 1) / GM_TN);
+    const int nb = ((tiles + 7) / 8) * 8;
+    hipLaunchKernelGGL(k_gemm_sub_f32, dim3(nb), dim3(256), 0, s, A, B, C, ld, m, n, k);
+    return hipGetLastError();
+}
+
 __global__ void k_panel_reset(unsigned long long* keys, unsigned long long* s_vals, unsigned long long* s_diag,
                               unsigned* abort_flag, unsigned* trsv_abort, int* info) {
     const int stride = gridDim.x * blockDim.x, i0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -846,11 +959,9 @@ hipError_t dalloc(T** p, size_t count) {
 struct afx_dense {
     int n = 0, lda = 0;
     hipStream_t s = nullptr;
-    rocblas_handle rb = nullptr;
     // look-ahead of the factorisation: the update of the columns past the
     // next panel runs on s2 while s factors that panel
     hipStream_t s2 = nullptr;
-    rocblas_handle rb2 = nullptr;
     hipEvent_t ev_panel = nullptr, ev_far = nullptr;
     double* A64 = nullptr;     // row-major n x n
     float* A32 = nullptr;      // column-major lda x n, LU in place
@@ -876,8 +987,6 @@ struct afx_dense {
                         (void*)s_diag, (void*)pabort, (void*)info, (void*)Linv, (void*)Uinv, (void*)tk,
                         (void*)yx, (void*)abort})
             if (p) (void)hipFree(p);
-        if (rb) rocblas_destroy_handle(rb);
-        if (rb2) rocblas_destroy_handle(rb2);
         if (ev_panel) (void)hipEventDestroy(ev_panel);
         if (ev_far) (void)hipEventDestroy(ev_far);
         if (s) (void)hipStreamDestroy(s);
@@ -899,11 +1008,7 @@ int dense_init(afx_dense* h, int n) {
     h->n = n;
     h->lda = ((n + 31) / 32) * 32 + 32;   // padded so that columns do not alias HBM channels
     VCHK(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
-    RBCHK(rocblas_create_handle(&h->rb));
-    RBCHK(rocblas_set_stream(h->rb, h->s));
     VCHK(hipStreamCreateWithFlags(&h->s2, hipStreamNonBlocking));
-    RBCHK(rocblas_create_handle(&h->rb2));
-    RBCHK(rocblas_set_stream(h->rb2, h->s2));
     VCHK(hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming));
     VCHK(hipEventCreateWithFlags(&h->ev_far, hipEventDisableTiming));
     const size_t nn = (size_t)n;
@@ -952,7 +1057,6 @@ int dense_factor(afx_dense* h, int32_t* info_out) {
     // update of step k - 1 before its own update of step k, since that one
     // brought [c1, c2) up to step k - 1; the column sets the two streams touch
     // at the same time are disjoint.
-    const float alpha = -1.f, beta = 1.f;
     float* A = h->A32;
     for (int j0 = 0; j0 < n; j0 += LU_NB) {
         const int nbc = std::min(LU_NB, n - j0);
@@ -965,12 +1069,12 @@ int dense_factor(afx_dense* h, int32_t* info_out) {
         VCHK(hipEventRecord(h->ev_panel, s));
         const int c1 = j0 + nbc, c2 = std::min(n, c1 + LU_NB), rest = n - c1;
         const float* A21 = A + (size_t)j0 * lda + c1;
-        auto update = [&](hipStream_t st, rocblas_handle rb, int cb, int ce) -> int {
+        auto update = [&](hipStream_t st, int cb, int ce) -> int {
             k_laswp<<<(ce - cb + 255) / 256, 256, 0, st>>>(A, lda, cb, ce, j0, nbc, h->ipiv);
             k_trsm_llnu<<<(ce - cb + 4 * TRSM_CPW - 1) / (4 * TRSM_CPW), 256, 0, st>>>(A, lda, cb, ce, j0, nbc);
             VCHK(hipGetLastError());
-            RBCHK(rocblas_sgemm(rb, rocblas_operation_none, rocblas_operation_none, rest, ce - cb, nbc, &alpha, A21,
-                                lda, A + (size_t)cb * lda + j0, lda, &beta, A + (size_t)cb * lda + c1, lda));
+            VCHK(launch_gemm_sub_f32(st, A21, A + (size_t)cb * lda + j0, A + (size_t)cb * lda + c1, lda, rest, ce - cb,
+                                     nbc));
             return AFX_OK;
         };
         VCHK(hipStreamWaitEvent(h->s2, h->ev_panel, 0));
@@ -979,12 +1083,12 @@ int dense_factor(afx_dense* h, int32_t* info_out) {
             VCHK(hipGetLastError());
         }
         if (c2 < n) {
-            const int rc = update(h->s2, h->rb2, c2, n);
+            const int rc = update(h->s2, c2, n);
             if (rc != AFX_OK) return rc;
         }
         if (c1 < n) {
             if (j0 > 0) VCHK(hipStreamWaitEvent(s, h->ev_far, 0));   // step k - 1 on [c1, c2)
-            const int rc = update(s, h->rb, c1, c2);
+            const int rc = update(s, c1, c2);
             if (rc != AFX_OK) return rc;
         }
         VCHK(hipEventRecord(h->ev_far, h->s2));
