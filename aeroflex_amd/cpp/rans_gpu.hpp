@@ -15,6 +15,10 @@
 //   get_wall_profile (post.h:301-387)           rans::gpu::get_wall_profile
 //   save(filename, solver) (post.h:58-180)      rans::gpu::save
 //   CpProfile (post.h:182-290)                  rans::gpu::CpProfile
+//   database::airfoil (database.hpp:17-52)      rans::gpu::airfoil
+//   Rans::compute_alphas (rans.h:53-57)         rans::gpu::compute_alphas
+//   Rans::run_airfoil / solve_airfoil           rans::gpu::run_airfoil / solve_airfoil
+//     (rans.h:78-123)
 //
 // Every call goes through the C ABI in include/afx_rans.h; errors come back
 // as the exceptions the reference throws at the same places
@@ -29,6 +33,7 @@
 #include <cstdint>
 #include <functional>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -313,6 +318,32 @@ inline int run_solver(solver& s, const Settings& st, std::vector<double>& residu
     return 0;
 }
 
+// multigrid<explicitSolver>::run_solver (multigrid.h:183-237): the CFL stays
+// at its start value and the residual is normalised only when positive;
+// returns 0 on success, 1 on failure or a non-finite residual
+inline int run_solver(explicitSolver& s, const Settings& st, std::vector<double>& residuals,
+                      const std::function<bool()>& stop = nullptr) {
+    int i = 0;
+    double err = 0;
+    double err_0 = s.get_uniform_residual();
+    do {
+        s.set_cfl(st.start_cfl);
+        s.fill();
+        int ok = s.compute();
+        if (ok == 0) {
+            err = s.solve(st.relaxation);
+            if ((i == 0) && (err > 2 * err_0)) err_0 = err;
+        } else {
+            err = -1;
+        }
+        if (err > 0) err /= err_0;
+        if (err < 0 || !std::isfinite(err)) return 1;
+        residuals.push_back(err);
+        i++;
+    } while ((err > st.tolerance) && (i < st.max_iterations) && !(stop && stop()));
+    return 0;
+}
+
 // multigrid<T>::gen_mapper (multigrid.h:96-163), built on the device: the
 // inverse-distance map from a coarse solver's cells to a fine one's.  The two
 // solvers must outlive the mapper.
@@ -341,10 +372,14 @@ public:
 // state receives run_solver's result for it.
 template <class S>
 S& multigrid_run(std::vector<S*>& levels, const Settings& st, std::vector<double>& residuals, int* state = nullptr,
-                 bool reinit = true, const std::function<bool()>& stop = nullptr) {
-    std::vector<mapper> mappers;
-    mappers.reserve(levels.size());
-    for (size_t i = 0; i + 1 < levels.size(); ++i) mappers.emplace_back(*levels[i], *levels[i + 1]);
+                 bool reinit = true, const std::function<bool()>& stop = nullptr,
+                 std::vector<mapper>* prebuilt = nullptr) {
+    std::vector<mapper> own;
+    if (!prebuilt) {
+        own.reserve(levels.size());
+        for (size_t i = 0; i + 1 < levels.size(); ++i) own.emplace_back(*levels[i], *levels[i + 1]);
+    }
+    std::vector<mapper>& mappers = prebuilt ? *prebuilt : own;
     if (reinit) levels[0]->init();
     levels[0]->refill_bcs();
     for (size_t i = 0; i < levels.size(); ++i) {
@@ -358,6 +393,93 @@ S& multigrid_run(std::vector<S*>& levels, const Settings& st, std::vector<double
         if (rc || (stop && stop())) return *levels[i];
     }
     return *levels.back();
+}
+
+// database::airfoil (common/database/database.hpp:17-52): the polar of one
+// section, alpha in degrees
+struct airfoil {
+    std::vector<double> alpha, cl, cd, cmy;
+};
+
+// Rans::compute_alphas (rans.h:53-57)
+inline std::vector<double> compute_alphas(double alpha_start, double alpha_end, double alpha_step) {
+    std::vector<double> out;
+    for (double a = alpha_start; a <= alpha_end; a += alpha_step) out.push_back(a);
+    return out;
+}
+
+// Rans::run_airfoil<T> (rans.h:78-106) over solvers built on the airfoil's
+// meshes (coarse to fine), each with its options set: the farfield angle is
+// db.alpha[0] * 0.01745 when level 0 is initialised, then for every alpha the
+// levels get the farfield at alpha * 0.01745, the multigrid loop runs without
+// re-initialising (warm start from the previous angle) and the wall profile
+// is appended to db; with a non-empty save_prefix the solution file
+// <save_prefix>_<alpha>.vtu is written.  Returns the last loop state.
+template <class S>
+int run_airfoil(std::vector<S*>& levels, const Settings& st, std::map<std::string, boundary_condition> bcs,
+                airfoil& db, const std::string& patch = "wall", const std::string& save_prefix = "",
+                std::vector<double>* residuals = nullptr, const std::function<bool()>& stop = nullptr) {
+    if (db.alpha.empty() || levels.empty()) return 0;
+    std::vector<double> own_res;
+    std::vector<double>& res = residuals ? *residuals : own_res;
+    std::vector<mapper> mappers;
+    mappers.reserve(levels.size());
+    for (size_t i = 0; i + 1 < levels.size(); ++i) mappers.emplace_back(*levels[i], *levels[i + 1]);
+    bcs.at("farfield").vars_far.angle = db.alpha[0] * 0.01745;
+    for (S* s : levels) s->set_bcs(bcs);
+    levels[0]->init();
+    int state = 0;
+    for (double alpha : db.alpha) {
+        bcs.at("farfield").vars_far.angle = alpha * 0.01745;   // deg to rad, rans.h:94
+        for (S* s : levels) s->set_bcs(bcs);
+        S& s = multigrid_run(levels, st, res, &state, false, stop, &mappers);
+        wallProfile wp = get_wall_profile(s, patch);
+        db.cl.push_back(wp.cl);
+        db.cd.push_back(wp.cd);
+        db.cmy.push_back(wp.cm);
+        if (!save_prefix.empty()) save(save_prefix + "_" + std::to_string(alpha) + ".vtu", s);
+        if (stop && stop()) break;
+    }
+    return state;
+}
+
+// Rans::solve_airfoil (rans.h:116-123): the solver type of the settings
+// (solver_options "explicit" / "implicit", core.h:174,202) picks the device
+// solver and loop; the levels are set up as the multigrid constructor does
+// (multigrid.h:70-77).  An unknown type is rejected (the reference returns
+// without solving).
+inline int solve_airfoil(const std::string& solver_type, const std::vector<const mesh*>& meshes, const gas& g,
+                         const std::string& viscosity_model, const std::map<std::string, boundary_condition>& bcs,
+                         const Settings& st, airfoil& db, bool second_order = true,
+                         const std::string& gradient_scheme = "green-gauss", double limiter_k = 5.0,
+                         const std::string& patch = "wall", const std::string& save_prefix = "") {
+    auto setup = [&](solver& s) {
+        s.set_bcs(bcs);
+        s.set_second_order(second_order);
+        s.set_gradient_scheme(gradient_scheme);
+        s.set_limiter_k(limiter_k);
+    };
+    if (solver_type == "implicit") {
+        std::vector<std::unique_ptr<implicitSolver>> own;
+        std::vector<implicitSolver*> levels;
+        for (const mesh* m : meshes) {
+            own.emplace_back(new implicitSolver(*m, g, viscosity_model));
+            setup(*own.back());
+            levels.push_back(own.back().get());
+        }
+        return run_airfoil(levels, st, bcs, db, patch, save_prefix);
+    }
+    if (solver_type == "explicit") {
+        std::vector<std::unique_ptr<explicitSolver>> own;
+        std::vector<explicitSolver*> levels;
+        for (const mesh* m : meshes) {
+            own.emplace_back(new explicitSolver(*m, g, viscosity_model));
+            setup(*own.back());
+            levels.push_back(own.back().get());
+        }
+        return run_airfoil(levels, st, bcs, db, patch, save_prefix);
+    }
+    throw std::invalid_argument("unknown solver type: " + solver_type + " (explicit, implicit)");
 }
 
 }  // namespace gpu

@@ -9,6 +9,17 @@
 // of multigrid<T>::run (multigrid.h:296-363) over them.
 // Prints one JSON object: iterations, residual history, CL, CD, CM (of the
 // last level run).
+//
+//   rans_gpu_driver polar <meshes> <mach> <alpha_start> <alpha_end> <alpha_step>
+//                   <iterations> [implicit|explicit] [wing_alpha_deg]
+//
+// The chain of app.cpp:168-184: Rans::compute_alphas, Rans::solve_airfoil
+// (the alpha sweep with warm starts, rans.h:78-123) into a database::airfoil,
+// then that polar as the section table of the nonlinear viscous vortex-lattice
+// solve (nonlinear::steady::solve, vlm/src/solver.cpp:330-388) of a
+// rectangular AR 8 wing (16 x 32 rings) at wing_alpha_deg (default 3).  Each
+// angle runs to a normalised residual of 1e-12 or the iteration count.
+// Prints the polar and the wing's iterations, convergence and CL.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -17,9 +28,111 @@
 #include <string>
 #include <vector>
 
+#include <afx_vlm.h>
+
 #include "rans_gpu.hpp"
 
+namespace {
+
+std::vector<std::string> split_list(const std::string& src) {
+    std::vector<std::string> out;
+    for (size_t a = 0, b; a <= src.size(); a = b + 1) {
+        b = src.find(',', a);
+        if (b == std::string::npos) b = src.size();
+        out.push_back(src.substr(a, b - a));
+    }
+    return out;
+}
+
+void load_mesh(rans::gpu::mesh& m, const std::string& src) {
+    if (src.rfind("ogrid:", 0) == 0) {
+        int ni = 128, nj = 65;
+        std::sscanf(src.c_str() + 6, "%d:%d", &ni, &nj);
+        m = rans::gpu::mesh::naca_ogrid(ni, nj);
+    } else {
+        m.read_file(src);
+    }
+}
+
+void vchk(int rc) {
+    if (rc != AFX_OK) throw std::runtime_error(afx_last_error());
+}
+
+int polar_chain(int argc, char** argv) {
+    if (argc < 9) {
+        std::fprintf(stderr, "usage: %s polar <meshes> <mach> <alpha_start> <alpha_end> <alpha_step> <iterations> "
+                             "[implicit|explicit] [wing_alpha_deg]\n", argv[0]);
+        return 2;
+    }
+    const std::vector<std::string> srcs = split_list(argv[2]);
+    const double mach = std::atof(argv[3]);
+    rans::gpu::Settings st;
+    st.max_iterations = std::atoi(argv[7]);
+    st.tolerance = 1e-12;   // each angle to its converged state
+    const std::string kind = argc > 8 ? argv[8] : "implicit";
+    const double wing_alpha = argc > 9 ? std::atof(argv[9]) : 3.0;
+    if (kind == "explicit") {
+        st.start_cfl = st.max_cfl = 0.8;
+        st.slope_cfl = 0.0;
+    }
+    std::vector<rans::gpu::mesh> meshes(srcs.size());
+    std::vector<const rans::gpu::mesh*> mp;
+    for (size_t l = 0; l < srcs.size(); ++l) {
+        load_mesh(meshes[l], srcs[l]);
+        mp.push_back(&meshes[l]);
+    }
+    std::map<std::string, rans::gpu::boundary_condition> bcs;
+    bcs["farfield"].bc_type = "farfield";
+    bcs["farfield"].vars_far = rans::gpu::boundary_variables(mach, 0.0, 1.0, 1.0);
+    bcs["wall"].bc_type = "slip-wall";
+    rans::gpu::airfoil db;
+    db.alpha = rans::gpu::compute_alphas(std::atof(argv[4]), std::atof(argv[5]), std::atof(argv[6]));
+    const int state = rans::gpu::solve_airfoil(kind, mp, rans::gpu::gas(), "inviscid", bcs, st, db);
+
+    // the polar as the wing's section table: one airfoil at both span ends
+    afx_wing_spec ws{16, 32, 8.0, 1.0, 0.0, 1.0, 1};
+    int32_t nn = 0, nr = 0, ns = 0;
+    double ref[3];
+    vchk(afx_wing_lattice(&ws, &nn, &nr, &ns, nullptr, nullptr, nullptr, ref));
+    std::vector<double> xyz(3 * (size_t)nn);
+    std::vector<int32_t> rings(4 * (size_t)nr), soff((size_t)ns + 1);
+    vchk(afx_wing_lattice(&ws, &nn, &nr, &ns, xyz.data(), rings.data(), soff.data(), ref));
+    afx_vlm* v = nullptr;
+    vchk(afx_vlm_create(nn, xyz.data(), ns, soff.data(), rings.data(), &v));
+    const int32_t offs[2] = {0, (int32_t)db.alpha.size()};
+    const int32_t sec[2] = {0, 0};
+    const double span[2] = {0.0, 1.0};
+    afx_polar pol{1, offs, db.alpha.data(), db.cl.data(), db.cd.data(), db.cmy.data(), 2, sec, span};
+    afx_vlm_flow fl{wing_alpha, 1.0, 1.0, ref[0], ref[1], 100.0 * ref[1], 0.0};
+    afx_nonlinear_info info{};
+    int rc = afx_vlm_set_flow(v, &fl);
+    if (rc == AFX_OK) rc = afx_vlm_nonlinear_solve(v, &pol, 1.0, 1e-10, 200, &info, nullptr, nullptr);
+    afx_vlm_destroy(v);
+    vchk(rc);
+    std::printf("{\"state\": %d, \"alpha\": [", state);
+    for (size_t i = 0; i < db.alpha.size(); ++i) std::printf("%s%.17g", i ? ", " : "", db.alpha[i]);
+    std::printf("], \"cl\": [");
+    for (size_t i = 0; i < db.cl.size(); ++i) std::printf("%s%.17g", i ? ", " : "", db.cl[i]);
+    std::printf("], \"cd\": [");
+    for (size_t i = 0; i < db.cd.size(); ++i) std::printf("%s%.17g", i ? ", " : "", db.cd[i]);
+    std::printf("], \"cm\": [");
+    for (size_t i = 0; i < db.cmy.size(); ++i) std::printf("%s%.17g", i ? ", " : "", db.cmy[i]);
+    std::printf("], \"wing\": {\"alpha\": %.17g, \"iterations\": %d, \"converged\": %d, \"cl\": %.17g, "
+                "\"cd\": %.17g}}\n", wing_alpha, info.iterations, info.converged, info.cl, info.cd);
+    return state;
+}
+
+}  // namespace
+
 int main(int argc, char** argv) {
+    if (argc > 1 && std::strcmp(argv[1], "polar") == 0) {
+        try {
+            return polar_chain(argc, argv);
+        } catch (std::exception& e) {
+            std::fprintf(stderr, "error: %s\n", e.what());
+            return 3;
+        }
+    }
     if (argc < 5) {
         std::fprintf(stderr, "usage: %s <mesh.msh|ogrid:NI:NJ> <mach> <alpha_deg> <iterations> "
                              "[implicit|explicit] [inviscid|laminar] [slip-wall|wall] [tolerance]\n", argv[0]);
@@ -34,12 +147,7 @@ int main(int argc, char** argv) {
         const std::string visc = argc > 6 ? argv[6] : "inviscid";
         const std::string wall = argc > 7 ? argv[7] : "slip-wall";
         st.tolerance = argc > 8 ? std::atof(argv[8]) : 1e-4;
-        std::vector<std::string> srcs;
-        for (size_t a = 0, b; a <= src.size(); a = b + 1) {
-            b = src.find(',', a);
-            if (b == std::string::npos) b = src.size();
-            srcs.push_back(src.substr(a, b - a));
-        }
+        const std::vector<std::string> srcs = split_list(src);
         rans::gpu::gas g;
         if (kind == "explicit") {
             st.start_cfl = st.max_cfl = 0.8;
@@ -54,13 +162,7 @@ int main(int argc, char** argv) {
         std::vector<std::unique_ptr<rans::gpu::solver>> owned;
         std::vector<rans::gpu::solver*> levels;
         for (size_t l = 0; l < srcs.size(); ++l) {
-            if (srcs[l].rfind("ogrid:", 0) == 0) {
-                int ni = 128, nj = 65;
-                std::sscanf(srcs[l].c_str() + 6, "%d:%d", &ni, &nj);
-                meshes[l] = rans::gpu::mesh::naca_ogrid(ni, nj);
-            } else {
-                meshes[l].read_file(srcs[l]);
-            }
+            load_mesh(meshes[l], srcs[l]);
             if (kind == "explicit") owned.emplace_back(new rans::gpu::explicitSolver(meshes[l], g, visc));
             else owned.emplace_back(new rans::gpu::implicitSolver(meshes[l], g, visc));
             rans::gpu::solver* s = owned.back().get();

@@ -1,7 +1,9 @@
 // Adjoint of the FP64 residual with respect to the state: lambda = (dR/dq)^T w
-// for the inviscid residual of fillRhoRHS (solver.h:1079-1152) with
-// Green-Gauss gradients (solver.h:428-469), the limiter of solver.h:517-593
-// and MUSCL reconstruction, first or second order.
+// for the inviscid and laminar residual of fillRhoRHS (solver.h:1079-1152)
+// with Green-Gauss gradients (solver.h:428-469), the limiter of
+// solver.h:517-593 and MUSCL reconstruction, first or second order.  The
+// laminar terms (physics.h:230-258 at the face gradient of
+// average_gradients, solver.h:359-398) add their adjoints in passes 2 and 3.
 //
 // Five atomics-free passes over the device layout (cells SoA q[k*nT+i]):
 //   1 cells  gradients, limiters and the limiter's active branch (the face
@@ -34,6 +36,7 @@ namespace afx {
 
 template <> __device__ __forceinline__ Dual1 ddiv<Dual1>(Dual1 a, Dual1 b) { return a / b; }
 template <> __device__ __forceinline__ Dual1 dsqrt<Dual1>(Dual1 x) { return dual_sqrt(x); }
+template <> __device__ __forceinline__ Dual1 dcbrt<Dual1>(Dual1 x) { return dual_cbrt(x); }
 
 namespace {
 
@@ -101,6 +104,32 @@ __device__ __forceinline__ double roe_dn(const GasT<double>& g, int kind, double
     double s = 0.0;
     for (int k = 0; k < 4; ++k) s += bF[k] * f[k].d;
     return s;
+}
+
+// VJP of the laminar terms (viscous_terms, physics.h:230-258) of a face with
+// respect to its reconstructed states and its face gradients: bL, bR +=
+// (d F_v / d q)^T bF and bgx, bgy = (d F_v / d grad)^T bF, by sixteen
+// forward-mode sweeps of the same device code
+__device__ void viscous_vjp(const GasT<double>& g, double nx, double ny, const double* qL, const double* qR,
+                            const double* gxa, const double* gya, const double* bF, double* bL, double* bR,
+                            double* bgx, double* bgy) {
+    const GasT<Dual1> gd = dual_gas(g);
+    for (int j = 0; j < 16; ++j) {
+        Dual1 a[4], b[4], gx[4], gy[4];
+        for (int k = 0; k < 4; ++k) {
+            a[k] = Dual1(qL[k], j == k ? 1.0 : 0.0);
+            b[k] = Dual1(qR[k], j == 4 + k ? 1.0 : 0.0);
+            gx[k] = Dual1(gxa[k], j == 8 + k ? 1.0 : 0.0);
+            gy[k] = Dual1(gya[k], j == 12 + k ? 1.0 : 0.0);
+        }
+        Dual1 f1(0.0), f2(0.0), f3(0.0);
+        viscous_terms<Dual1>(gd, Dual1(nx), Dual1(ny), a, b, gx, gy, gd.mu, gd.k, f1, f2, f3);
+        const double v = bF[1] * f1.d + bF[2] * f2.d + bF[3] * f3.d;
+        if (j < 4) bL[j] += v;
+        else if (j < 8) bR[j - 4] += v;
+        else if (j < 12) bgx[j - 8] = v;
+        else bgy[j - 12] = v;
+    }
 }
 
 // pass 1: gradients, limiters and the active branch of every (cell, variable)
@@ -219,6 +248,31 @@ __global__ void __launch_bounds__(kBlock) k_adj_face(CellsT<double> C, FacesT<do
         bF[k] = kind == K_INTERNAL ? L * (w[k * nT + n] - w[k * nT + o]) : -L * w[k * nT + o];
     if (kind == K_INTERNAL) {
         roe_flux_vjp<double>(g.gamma, nx, ny, qL, qR, bF, bL, bR);
+        if (A.vgx) {
+            // laminar terms at the face gradient of average_gradients
+            // (solver.h:359-398, A1): g = gbar - (gbar . t - dir) t with t the
+            // normalised sum of the centroids and dir = (q_0 - q_1) / |x_0 + x_1|
+            double t0 = C.xc[o] + C.xc[n], t1 = C.yc[o] + C.yc[n];
+            const double l = sqrt(t0 * t0 + t1 * t1);
+            t0 /= l;
+            t1 /= l;
+            double gxa[4], gya[4], bgx[4], bgy[4];
+            for (int k = 0; k < 4; ++k) {
+                const double dir = (q[k * nT + o] - q[k * nT + n]) / l;
+                const double gxb = (A.gx[k * nT + o] + A.gx[k * nT + n]) * 0.5;
+                const double gyb = (A.gy[k * nT + o] + A.gy[k * nT + n]) * 0.5;
+                const double dot = gxb * t0 + gyb * t1;
+                gxa[k] = gxb - (dot - dir) * t0;
+                gya[k] = gyb - (dot - dir) * t1;
+            }
+            viscous_vjp(g, nx, ny, qL, qR, gxa, gya, bF, bL, bR, bgx, bgy);
+            for (int k = 0; k < 4; ++k) {
+                const double bt = bgx[k] * t0 + bgy[k] * t1;   // the adjoint of dir
+                A.vgx[k * nF + f] = 0.5 * (bgx[k] - t0 * bt);
+                A.vgy[k * nF + f] = 0.5 * (bgy[k] - t1 * bt);
+                A.vdq[k * nF + f] = bt / l;
+            }
+        }
     } else {
         double qb[4], bB[4];
         face_vars<double>(g, kind, nx, ny, qL, qR, qb);
@@ -301,6 +355,12 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<d
                 lgy[k] += b * ph[k] * dy;
             }
         }
+        if (A.vgx && F.kind[f] == K_INTERNAL)   // laminar terms: face gradient and difference quotient
+            for (int k = 0; k < 4; ++k) {
+                lgx[k] += A.vgx[k * nF + f];
+                lgy[k] += A.vgy[k * nF + f];
+                lq[k] += nb ? -A.vdq[k * nF + f] : A.vdq[k * nF + f];
+            }
     }
     double abar = 0.0;
     for (int k = 0; k < 4; ++k) {
@@ -782,7 +842,7 @@ void adjoint_passes(hipStream_t s, CellsT<double> C, FacesT<double> F, const dou
                            q, w, g, second_order, A, G, ghost);
     hipLaunchKernelGGL((k_adj_cell1<GEO, GHOST>), dim3(nbc), dim3(kBlock), 0, s, C, F, q, second_order, limiter_k, A,
                        lam, G, ghost);
-    if (second_order) {
+    if (second_order || A.vgx) {   // gradients enter the residual (reconstruction or laminar terms)
         hipLaunchKernelGGL((k_adj_face2<GEO, GHOST>), dim3(nbf), dim3(kBlock), 0, s, C, F, qg, g, A, G, ghost);
         hipLaunchKernelGGL(k_adj_cell2, dim3(nbc), dim3(kBlock), 0, s, C, F, A, lam, lam_g);
     } else if (lam_g != lam) {
@@ -818,6 +878,84 @@ hipError_t launch_adjoint_state(hipStream_t s, CellsT<double> C, FacesT<double> 
                                 GasT<double> g, int second_order, double limiter_k, AdjBufs A, double* lam,
                                 const GeoBufs* Gp) {
     return launch_adjoint_stage(s, C, F, q, q, w, g, second_order, limiter_k, A, lam, lam, nullptr, Gp);
+}
+
+// ------------------------------------------------------------------ tangent --
+// Tangent of the FP64 residual: the fused tile kernel k_residual_tile
+// instantiated with Dual1, so R and (dR/dq) v come out of the same operations
+// as the plain residual (gradients, limiter on its active branch, Roe, laminar
+// and Spalart-Allmaras terms, cell sums).  The value parts reproduce the FP64
+// residual bit for bit; the derivative parts are the exact directional
+// derivative of the piecewise-smooth residual.
+namespace {
+__global__ void __launch_bounds__(kBlock) k_to_dual(const double* __restrict__ x, int64_t n, Dual1* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = Dual1(x[i]);
+}
+// q + eps v over nv planes of nT cells; ghost cells (i >= nC) keep a zero
+// derivative: their states are boundary data
+__global__ void __launch_bounds__(kBlock) k_dual_state(const double* __restrict__ q, const double* __restrict__ v,
+                                                       int32_t nT, int32_t nC, int nv, Dual1* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nv * nT) return;
+    const int32_t c = (int32_t)(i % nT);
+    out[i] = Dual1(q[i], c < nC ? v[i] : 0.0);
+}
+__global__ void __launch_bounds__(kBlock) k_dual_parts(const Dual1* __restrict__ x, int64_t n, double* __restrict__ val,
+                                                       double* __restrict__ der) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (val) val[i] = x[i].v;
+    der[i] = x[i].d;
+}
+inline unsigned nblk(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+}  // namespace
+
+hipError_t launch_to_dual(hipStream_t s, const double* x, int64_t n, Dual1* out) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_to_dual, dim3(nblk(n)), dim3(kBlock), 0, s, x, n, out);
+    return hipGetLastError();
+}
+hipError_t launch_dual_state(hipStream_t s, const double* q, const double* v, int32_t nT, int32_t nC, int nv,
+                             Dual1* out) {
+    hipLaunchKernelGGL(k_dual_state, dim3(nblk((int64_t)nv * nT)), dim3(kBlock), 0, s, q, v, nT, nC, nv, out);
+    return hipGetLastError();
+}
+hipError_t launch_dual_parts(hipStream_t s, const Dual1* x, int64_t n, double* val, double* der) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dual_parts, dim3(nblk(n)), dim3(kBlock), 0, s, x, n, val, der);
+    return hipGetLastError();
+}
+size_t tile_lds_bytes_tangent(int32_t lmax, int32_t gmax, int32_t ftmax, int nv) {
+    return tile_lds_bytes<Dual1>(lmax, gmax, ftmax, nv);
+}
+hipError_t launch_residual_tile_tangent(hipStream_t s, TilesT<Dual1> S, const Dual1* q, int32_t nT, int32_t nC,
+                                        const Dual1* area, const Dual1* lsq, const Dual1* xc, const Dual1* yc,
+                                        const Dual1* wd, GasT<double> g64, int gg, int so, double limiter_k,
+                                        int viscous, Dual1* R) {
+    const int nb = ((S.ntiles + 7) / 8) * 8;
+    const GasT<Dual1> g{Dual1(g64.gamma), Dual1(g64.R), Dual1(g64.mu), Dual1(g64.k), Dual1(g64.cp), Dual1(g64.PrL),
+                        Dual1(g64.PrT)};
+    const int nv = viscous == 2 ? 5 : 4;
+    const size_t lds = tile_lds_bytes<Dual1>(S.lmax, S.gmax, S.ftmax, nv);
+    hipError_t e;
+    // the state array is also the ghost array: the kernel refreshes the
+    // wall/slip ghosts of its tiles from their owners (set_walls_from_internal)
+    Dual1* qg = const_cast<Dual1*>(q);
+    if (nv == 5) {
+        if ((e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_residual_tile<Dual1, 5>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL((k_residual_tile<Dual1, 5>), dim3(nb), dim3(kBlock), lds, s, S, q, nT, nC, area, lsq, xc,
+                           yc, wd, g, gg, so, Dual1(limiter_k), viscous, R, qg);
+    } else {
+        if ((e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_residual_tile<Dual1, 4>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL((k_residual_tile<Dual1, 4>), dim3(nb), dim3(kBlock), lds, s, S, q, nT, nC, area, lsq, xc,
+                           yc, wd, g, gg, so, Dual1(limiter_k), viscous, R, qg);
+    }
+    return hipGetLastError();
 }
 
 }  // namespace afx

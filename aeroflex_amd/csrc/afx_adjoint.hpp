@@ -5,9 +5,10 @@
 //                  bqL = (dF/dqL)^T bF and bqR = (dF/dqR)^T bF: the flux is
 //                  re-evaluated forward, then swept backwards operation by
 //                  operation (derivatives written out by hand);
-//   Dual1          forward-mode number for the 4x4 Jacobians of the boundary
-//                  states (face_vars, physics.h:267-530), evaluated by the
-//                  same templated device code as the residual;
+//   Dual1          forward-mode number: the 4x4 Jacobians of the boundary
+//                  states (face_vars, physics.h:267-530) and the tangent of
+//                  the whole residual (dR/dq) v, evaluated by the same
+//                  templated device code as the residual;
 //   limiter_phi_d  value and partial derivatives of the limiter function of
 //                  solver.h:578-584 in its selected branch.
 #pragma once
@@ -166,9 +167,16 @@ __host__ __device__ inline Dual1& operator-=(Dual1& a, Dual1 b) { return a = a -
 __host__ __device__ inline Dual1& operator*=(Dual1& a, Dual1 b) { return a = a * b; }
 __host__ __device__ inline bool operator<(Dual1 a, Dual1 b) { return a.v < b.v; }
 __host__ __device__ inline bool operator>(Dual1 a, Dual1 b) { return a.v > b.v; }
+__host__ __device__ inline bool operator<=(Dual1 a, Dual1 b) { return a.v <= b.v; }
+__host__ __device__ inline bool operator>=(Dual1 a, Dual1 b) { return a.v >= b.v; }
 __host__ __device__ inline Dual1 dual_sqrt(Dual1 a) {
     const double s = sqrt(a.v);
     return Dual1(s, a.d / (2.0 * s));
+}
+// x^(1/3) and its derivative x^(1/3) / (3 x) (x != 0)
+__host__ __device__ inline Dual1 dual_cbrt(Dual1 a) {
+    const double c = cbrt(a.v);
+    return Dual1(c, a.d * c / (3.0 * a.v));
 }
 
 // limiter function of solver.h:578-584 in its selected branch (dqg != 0):

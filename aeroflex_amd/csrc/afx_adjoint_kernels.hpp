@@ -1,8 +1,24 @@
 // Launcher of the state adjoint of the FP64 residual (afx_adjoint.hip).
 #pragma once
 #include "afx_kernels.hpp"
+#include "afx_adjoint.hpp"
 
 namespace afx {
+
+// Tangent of the FP64 residual (forward mode, Dual1 instantiation of the fused
+// tile kernel).  launch_to_dual: Dual1 copies of geometry arrays (zero
+// derivative); launch_dual_state: (q, v) -> q + eps v over nv planes, ghost
+// cells with a zero derivative; launch_dual_parts: value and derivative
+// planes of a Dual1 array (val may be null).
+hipError_t launch_to_dual(hipStream_t s, const double* x, int64_t n, Dual1* out);
+hipError_t launch_dual_state(hipStream_t s, const double* q, const double* v, int32_t nT, int32_t nC, int nv,
+                             Dual1* out);
+hipError_t launch_dual_parts(hipStream_t s, const Dual1* x, int64_t n, double* val, double* der);
+size_t tile_lds_bytes_tangent(int32_t lmax, int32_t gmax, int32_t ftmax, int nv);
+hipError_t launch_residual_tile_tangent(hipStream_t s, TilesT<Dual1> S, const Dual1* q, int32_t nT, int32_t nC,
+                                        const Dual1* area, const Dual1* lsq, const Dual1* xc, const Dual1* yc,
+                                        const Dual1* wd, GasT<double> g64, int gg, int so, double limiter_k,
+                                        int viscous, Dual1* R);
 
 // work arrays of one adjoint evaluation ([4][nT] / [4][nC] / [4][nF] planes)
 struct AdjBufs {
@@ -22,6 +38,13 @@ struct AdjBufs {
     double* cn;
     const int32_t* bface;   // [nbf] the boundary faces (device index, ascending)
     int32_t nbf;
+    // laminar residual (null otherwise): per interior face, half the adjoint of
+    // the averaged face gradient (owed to both cells' gradients) and the
+    // adjoint of the face's difference quotient divided by |x_0 + x_1| (owed to
+    // the owner's state, with the opposite sign to the neighbour's)
+    double* vgx;     // [4][nF]
+    double* vgy;
+    double* vdq;
 };
 
 // work arrays of the adjoint with respect to the mesh nodes

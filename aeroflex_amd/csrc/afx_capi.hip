@@ -129,6 +129,7 @@ struct afx_rans {
     DBuf<int8_t> a_act;
     DBuf<int32_t> a_jmax, a_jmin, a_nbj;
     DBuf<int32_t> a_bface;   // device indices of the boundary faces, ascending
+    DBuf<double> a_vgx, a_vgy, a_vdq;   // laminar terms' face-gradient / difference-quotient adjoints
     // adjoint with respect to the mesh nodes: metric adjoints, node data in
     // device face / cell order, node incidence lists (CSR)
     DBuf<double> g_fbar, g_abar, g_nodex, g_nodey, g_fend, g_fcen, g_cvert, g_xbar;
@@ -147,6 +148,9 @@ struct afx_rans {
     int lhs_bad = 0;
     bool mf = false;             // matrix-free operator (RANS_MATRIX_FREE build)
     DBuf<double> mfFb;           // its unperturbed face fluxes at the current state
+    // tangent (Dual1 tile kernel): geometry copies, state, residual, direction
+    DBuf<Dual1> tg_geo, tg_area, tg_lsq, tg_xc, tg_yc, tg_wd, tg_q, tg_R;
+    DBuf<double> tg_v, tg_out;
     int64_t lin_iters = 0;
     int kblk = 0;
     std::vector<int32_t> t_lg;   // device cell of each local cell (unflagged)
@@ -1777,8 +1781,15 @@ int afx_rans_launch_stage(afx_rans* h, int32_t precision, int32_t stage, int32_t
 }
 
 namespace {
-int adjoint_check(afx_rans* h) {
-    if (h->opts.viscosity != 0) return fail(AFX_EUNSUP, "the state adjoint covers the inviscid residual");
+// laminar_ok: the state adjoint covers the laminar residual too; the adjoints
+// with respect to the mesh nodes and through pseudo-time cover the inviscid one
+int adjoint_check(afx_rans* h, bool laminar_ok = false) {
+    if (h->opts.viscosity == 2)
+        return fail(AFX_EUNSUP, "the adjoint covers the inviscid and laminar residuals, not the Spalart-Allmaras "
+                                "model (use afx_rans_tangent for its directional derivatives)");
+    if (h->opts.viscosity == 1 && !laminar_ok)
+        return fail(AFX_EUNSUP, "the adjoints with respect to the mesh nodes and through pseudo-time cover the "
+                                "inviscid residual");
     if (h->opts.gradient != 1) return fail(AFX_EUNSUP, "the state adjoint covers Green-Gauss gradients");
     if (h->michalak()) return fail(AFX_EUNSUP, "the state adjoint covers the default limiter, not the Michalak build");
     if (!h->a_w.p) {
@@ -1796,12 +1807,18 @@ int adjoint_check(afx_rans* h) {
         if (!bf.empty()) { HIPCHK(h->a_bface.alloc(bf.size())); HIPCHK(h->a_bface.upload(bf, h->stream)); }
         HIPCHK(hipStreamSynchronize(h->stream));
     }
+    if (h->opts.viscosity == 1 && !h->a_vgx.p) {
+        const size_t f4 = (size_t)4 * h->nF;
+        for (DBuf<double>* b : {&h->a_vgx, &h->a_vgy, &h->a_vdq}) HIPCHK(b->alloc(f4));
+    }
     return AFX_OK;
 }
 AdjBufs adj_bufs(const afx_rans* h) {
+    const bool lam = h->opts.viscosity == 1;
     return AdjBufs{h->gx.p, h->gy.p, h->lim.p, h->a_act.p, h->a_jmax.p, h->a_jmin.p, h->a_bL.p, h->a_bR.p,
                    h->a_lgx.p, h->a_lgy.p, h->a_nbv.p, h->a_nbj.p, h->a_co.p, h->a_cn.p,
-                   h->a_bface.p, (int32_t)h->m.boundaryEdges.size()};
+                   h->a_bface.p, (int32_t)h->m.boundaryEdges.size(), lam ? h->a_vgx.p : nullptr,
+                   lam ? h->a_vgy.p : nullptr, lam ? h->a_vdq.p : nullptr};
 }
 int geo_check(afx_rans* h) {
     if (h->g_fbar.p) return AFX_OK;
@@ -1864,7 +1881,7 @@ int afx_rans_adjoint(afx_rans* h, const double* w, double* lambda) {
     int rc = check_handle(h);
     if (rc) return rc;
     if (!w || !lambda) return fail(AFX_EARG, "null vector");
-    if ((rc = adjoint_check(h))) return rc;
+    if ((rc = adjoint_check(h, true))) return rc;
     if ((rc = upload_aos(h, w, h->a_w.p))) return rc;
     // wall/slip ghosts as fillRhoRHS sees them (they enter no flux, but the
     // state is then the one the residual reads)
@@ -2162,11 +2179,110 @@ int afx_rans_dcl_dnodes(afx_rans* h, const char* patch, int32_t steps, int32_t e
     return AFX_OK;
 }
 
+namespace {
+// buffers and geometry copies of the tangent (first call), the wall distance
+// it reads (every call: it follows the BCs)
+int tangent_prepare(afx_rans* h) {
+    if (!h->opts.fused || !h->tile64_ok)
+        return fail(AFX_EUNSUP, "the tangent runs in the fused tile kernel (renumber = 1, fused = 1, tiles within the "
+                                "LDS limit)");
+    const int nv = h->nv;
+    const size_t lds = tile_lds_bytes_tangent(h->tl.lmax, h->tl.gmax, h->tl.ftmax, nv);
+    int dev = 0, maxl = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&maxl, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if ((size_t)maxl < lds)
+        return fail(AFX_EUNSUP, "the tangent's tiles need " + std::to_string(lds) + " bytes of LDS per workgroup; the "
+                                "device allows " + std::to_string(maxl) + " (use smaller tiles)");
+    int rc;
+    if (h->sa() && !h->wd_ready && (rc = compute_wall_distance(h))) return rc;
+    hipStream_t s = h->stream;
+    const int64_t nT = h->nT;
+    if (!h->tg_q.p) {
+        HIPCHK(h->tg_geo.alloc(h->t_geo64.n));
+        HIPCHK(h->tg_area.alloc(h->area.n));
+        HIPCHK(h->tg_lsq.alloc(h->lsq.n));
+        HIPCHK(h->tg_xc.alloc(h->xc.n));
+        HIPCHK(h->tg_yc.alloc(h->yc.n));
+        HIPCHK(h->tg_wd.alloc(h->wall_d.n));
+        HIPCHK(h->tg_q.alloc((size_t)nv * nT));
+        HIPCHK(h->tg_R.alloc((size_t)nv * nT));
+        HIPCHK(h->tg_v.alloc((size_t)nv * nT));
+        HIPCHK(h->tg_out.alloc((size_t)nv * nT));
+        HIPCHK(launch_to_dual(s, h->t_geo64.p, (int64_t)h->t_geo64.n, h->tg_geo.p));
+        HIPCHK(launch_to_dual(s, h->area.p, (int64_t)h->area.n, h->tg_area.p));
+        HIPCHK(launch_to_dual(s, h->lsq.p, (int64_t)h->lsq.n, h->tg_lsq.p));
+        HIPCHK(launch_to_dual(s, h->xc.p, (int64_t)h->xc.n, h->tg_xc.p));
+        HIPCHK(launch_to_dual(s, h->yc.p, (int64_t)h->yc.n, h->tg_yc.p));
+        HIPCHK(hipMemsetAsync(h->tg_v.p, 0, (size_t)nv * nT * 8, s));
+    }
+    HIPCHK(launch_to_dual(s, h->wall_d.p, (int64_t)h->wall_d.n, h->tg_wd.p));   // follows the BCs
+    return AFX_OK;
+}
+// (q, v in tg_v) -> tg_out = (dR/dq) v, on the handle's stream
+int enqueue_tangent(afx_rans* h) {
+    hipStream_t s = h->stream;
+    const int nv = h->nv;
+    const int64_t nT = h->nT;
+    HIPCHK(launch_dual_state(s, h->q.p, h->tg_v.p, h->nT, h->nC, nv, h->tg_q.p));
+    HIPCHK(launch_residual_tile_tangent(s, h->tiles<Dual1>(h->tg_geo.p), h->tg_q.p, h->nT, h->nC, h->tg_area.p,
+                                        h->tg_lsq.p, h->tg_xc.p, h->tg_yc.p, h->tg_wd.p, h->gas64(), h->opts.gradient,
+                                        h->so(), h->opts.limiter_k, h->opts.viscosity, h->tg_R.p));
+    HIPCHK(launch_dual_parts(s, h->tg_R.p, (int64_t)nv * nT, nullptr, h->tg_out.p));
+    return AFX_OK;
+}
+}  // namespace
+
+int afx_rans_tangent(afx_rans* h, const double* v, const double* v_nu, double* jv, double* jv_nu) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!v || !jv) return fail(AFX_EARG, "null direction or output");
+    if (!h->sa() && (v_nu || jv_nu)) return fail(AFX_EARG, "rho nu~ direction / output only with the Spalart-Allmaras model");
+    if ((rc = tangent_prepare(h))) return rc;
+    hipStream_t s = h->stream;
+    const int nv = h->nv;
+    const int64_t nT = h->nT;
+    if ((rc = upload_aos(h, v, h->tg_v.p))) return rc;
+    if (nv == 5) {
+        std::vector<double> tmp(h->nT, 0.0);
+        if (v_nu)
+            for (int32_t i = 0; i < h->nT; ++i) tmp[i] = v_nu[h->c_new2old[i]];
+        HIPCHK(hipMemcpyAsync(h->tg_v.p + 4 * nT, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    if ((rc = enqueue_tangent(h))) return rc;
+    if ((rc = download_aos(h, h->tg_out.p, jv))) return rc;
+    if (jv_nu && (rc = download_cells(h, h->tg_out.p + 4 * nT, jv_nu))) return rc;
+    return AFX_OK;
+}
+
+int afx_rans_time_tangent(afx_rans* h, int32_t reps, double* ms) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (reps < 1 || !ms) return fail(AFX_EARG, "reps must be >= 1 and ms non-null");
+    if ((rc = tangent_prepare(h))) return rc;
+    if ((rc = enqueue_tangent(h))) return rc;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, h->stream));
+    for (int32_t i = 0; i < reps; ++i)
+        if ((rc = enqueue_tangent(h))) return rc;
+    HIPCHK(hipEventRecord(b, h->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, a, b));
+    HIPCHK(hipEventDestroy(a));
+    HIPCHK(hipEventDestroy(b));
+    *ms = (double)t / reps;
+    return AFX_OK;
+}
+
 int afx_rans_time_adjoint(afx_rans* h, int32_t reps, double* ms) {
     int rc = check_handle(h);
     if (rc) return rc;
     if (reps < 1 || !ms) return fail(AFX_EARG, "reps must be >= 1 and ms non-null");
-    if ((rc = adjoint_check(h))) return rc;
+    if ((rc = adjoint_check(h, true))) return rc;
     if ((rc = enqueue_adjoint(h))) return rc;
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));

@@ -116,6 +116,8 @@ def _load():
         "afx_rans_create_like": (C.c_int, [P, C.POINTER(AfxGas), C.POINTER(P)]),
         "afx_rans_time_residual": (C.c_int, [P, i32, i32, dp]),
         "afx_rans_adjoint": (C.c_int, [P, dp, dp]),
+        "afx_rans_tangent": (C.c_int, [P, dp, dp, dp, dp]),
+        "afx_rans_time_tangent": (C.c_int, [P, i32, dp]),
         "afx_rans_adjoint_nodes": (C.c_int, [P, dp, dp, dp]),
         "afx_rans_dcl_dalpha": (C.c_int, [P, C.c_char_p, i32, i32, f64, dp, dp]),
         "afx_rans_dcl_dnodes": (C.c_int, [P, C.c_char_p, i32, i32, f64, dp, dp]),
@@ -525,6 +527,27 @@ class Solver(Handle):
         c.set_cfl(getattr(self, "cfl", 1.0))
         return c
 
+    def tangent(self, v, v_nu=None):
+        """(dR/dq) v of fillRhoRHS at the current state (every model), in the
+        q(4*i+k) layout; with the SA model also the rho nu~ direction v_nu and
+        row.  Returns jv, or (jv, jv_nu) with the SA model."""
+        v = np.ascontiguousarray(v, np.float64)
+        if v.shape != (4 * self.n_cells,):
+            raise RansError("tangent direction has the wrong length")
+        jv = np.empty(4 * self.n_cells)
+        sa = self._opts.viscosity == 2
+        if sa:
+            vn = None if v_nu is None else np.ascontiguousarray(v_nu, np.float64)
+            if vn is not None and vn.shape != (self.n_cells,):
+                raise RansError("rho nu~ direction has the wrong length")
+            jn = np.empty(self.n_cells)
+            _check(lib.afx_rans_tangent(self.h, _dptr(v), None if vn is None else _dptr(vn), _dptr(jv), _dptr(jn)))
+            return jv, jn
+        if v_nu is not None:
+            raise RansError("rho nu~ direction only with the Spalart-Allmaras model")
+        _check(lib.afx_rans_tangent(self.h, _dptr(v), None, _dptr(jv), None))
+        return jv
+
     def adjoint(self, w):
         """(dR/dq)^T w of fillRhoRHS at the current state (inviscid, Green-Gauss)."""
         w = np.ascontiguousarray(w, np.float64)
@@ -564,6 +587,12 @@ class Solver(Handle):
         _check(lib.afx_rans_dcl_dnodes(self.h, patch.encode(), steps, checkpoint_every, relaxation, C.byref(cl),
                                        _dptr(xb)))
         return cl.value, xb.reshape(-1, 2)
+
+    def time_tangent(self, reps=20):
+        """Average ms of one tangent evaluation (HIP events)."""
+        ms = C.c_double()
+        _check(lib.afx_rans_time_tangent(self.h, reps, C.byref(ms)))
+        return ms.value
 
     def time_adjoint(self, reps=20):
         ms = C.c_double()
@@ -683,12 +712,107 @@ class Mapper(Handle):
         _check(lib.afx_mapper_prolongate(self.h, self.coarse.h, self.fine.h))
 
 
-def multigrid_run(solvers, settings, reinit=True, mappers=None, residuals=None, stop=None):
+DEG_TO_RAD = 0.01745   # the degree factor of Rans::run_airfoil (rans.h:88,94)
+SOLVER_TYPES = ("explicit", "implicit")   # rans::Settings::solver_options (core.h:174)
+
+
+class Airfoil:
+    """database::airfoil (common/database/database.hpp:17-52): the polar of
+    one section, alpha in degrees with cl, cd, cmy per angle."""
+
+    def __init__(self, alpha=()):
+        self.alpha = list(alpha)
+        self.cl, self.cd, self.cmy = [], [], []
+
+
+def compute_alphas(alpha_start, alpha_end, alpha_step):
+    """Rans::compute_alphas (rans.h:53-57): alpha_start, alpha_start + step,
+    ... while <= alpha_end, accumulated as the reference does."""
+    out = []
+    a = alpha_start
+    while a <= alpha_end:
+        out.append(a)
+        a += alpha_step
+    return out
+
+
+def solve_airfoil(levels, settings, db, solver_type="implicit", patch="wall", save_prefix=None, stop=None,
+                  residuals=None):
+    """Rans::solve_airfoil / run_airfoil (rans.h:78-123) over device solvers
+    built on the airfoil's meshes (coarse to fine, BCs set): the farfield
+    angle is db.alpha[0] * 0.01745 when level 0 is initialised, then for
+    each alpha (warm-started from the previous angle's state) every level
+    gets the farfield at alpha * 0.01745, the multigrid loop runs without
+    re-initialising, and the wall profile's CL, CD, CM are appended to db
+    (with the solution file <save_prefix>_<alpha>.vtu when save_prefix is
+    given).  solver_type: "implicit" or "explicit" (settings.solver_type(),
+    core.h:174,202).  Returns db."""
+    if solver_type not in SOLVER_TYPES:
+        raise RansError(f"unknown solver type '{solver_type}' ({', '.join(SOLVER_TYPES)})")
+    loop = run_solver if solver_type == "implicit" else run_solver_explicit
+
+    def at_angle(alpha):
+        for s in levels:
+            bcs = dict(s.bcs)
+            far = bcs["farfield"].vars_far
+            bcs["farfield"] = BoundaryCondition("farfield", BoundaryVariables(far.mach, alpha * DEG_TO_RAD,
+                                                                              far.T, far.p))
+            s.set_bcs(bcs)
+
+    if not db.alpha:
+        return db
+    mappers = [Mapper(levels[i], levels[i + 1]) for i in range(len(levels) - 1)]
+    try:
+        at_angle(db.alpha[0])
+        levels[0].init()
+        for alpha in db.alpha:
+            at_angle(alpha)
+            s, _ = multigrid_run(levels, settings, reinit=False, mappers=mappers, residuals=residuals, stop=stop,
+                                 loop=loop)
+            cl, cd, cm = s.get_wall_profile(patch)
+            db.cl.append(cl)
+            db.cd.append(cd)
+            db.cmy.append(cm)
+            if save_prefix is not None:
+                s.save_vtu(f"{save_prefix}_{alpha}.vtu")
+            if stop and stop():
+                break
+    finally:
+        for mp in reversed(mappers):
+            mp.close()
+    return db
+
+
+def run_solver_explicit(s, settings, residuals=None, stop=None):
+    """multigrid<explicitSolver>::run_solver (multigrid.h:183-237): the CFL
+    stays at its start value, the residual is normalised only when positive,
+    the loop stops on tolerance / iteration count / stop flag.  Returns
+    (state, residuals): state 0 ok, 1 failure or a non-finite residual."""
+    residuals = [] if residuals is None else residuals
+    i = 0
+    err_0 = s.get_uniform_residual()
+    while True:
+        s.set_cfl(settings.start_cfl)
+        err = s.explicit_solve(settings.relaxation)
+        if i == 0 and err > 2 * err_0:
+            err_0 = err
+        if err > 0:
+            err /= err_0
+        if err < 0 or not np.isfinite(err):
+            return 1, residuals
+        residuals.append(err)
+        i += 1
+        if not (err > settings.tolerance and i < settings.max_iterations and not (stop and stop())):
+            return 0, residuals
+
+
+def multigrid_run(solvers, settings, reinit=True, mappers=None, residuals=None, stop=None, loop=None):
     """multigrid<implicitSolver>::run (multigrid.h:317-363): level 0 from
     freestream (reinit) and its boundary states, then for each finer level
     the coarse solution mapped over (bcs_from_internal, q_fine = M q_coarse,
-    refill_bcs) and solved by run_solver.  Returns (solver, state) of the
-    last level run; state 1 when a level's solver failed."""
+    refill_bcs) and solved by run_solver (or `loop`, e.g. run_solver_explicit
+    for multigrid<explicitSolver>::run).  Returns (solver, state) of the last
+    level run; state 1 when a level's solver failed."""
     if mappers is None:
         mappers = [Mapper(solvers[i], solvers[i + 1]) for i in range(len(solvers) - 1)]
     residuals = [] if residuals is None else residuals
@@ -700,7 +824,7 @@ def multigrid_run(solvers, settings, reinit=True, mappers=None, residuals=None, 
             solvers[i - 1].bcs_from_internal()
             mappers[i - 1].prolongate()
             s.refill_bcs()
-        state, _ = run_solver(s, settings, residuals, stop)
+        state, _ = (loop or run_solver)(s, settings, residuals, stop)
         if state or (stop and stop()):
             return s, state
     return solvers[-1], 0

@@ -438,6 +438,32 @@ def read_polar(database_path, locations_path, surface=0):
             "section_span": np.asarray(spans, np.float64)}
 
 
+def polar_from_airfoils(airfoils, sections):
+    """The polar table of database::table (common/database/database.cpp:192-242)
+    from airfoil polars computed by the RANS alpha sweep (rans.Airfoil, as
+    app.cpp:168-176 fills vlm.database.airfoils from Rans::solve_airfoil)
+    instead of a database file.  airfoils: {name: Airfoil} (alpha in degrees,
+    increasing); sections: [(airfoil name, span location in [0, 1])] along
+    the wing, as a locations file's SURFACE block lists them.  Returns the
+    arrays of afx_polar (the format of read_polar)."""
+    order = list(airfoils)
+    rows = []
+    for n in order:
+        a = airfoils[n]
+        if not (len(a.alpha) == len(a.cl) == len(a.cd) == len(a.cmy)) or not a.alpha:
+            raise VlmError(f"airfoil '{n}': the polar needs alpha, cl, cd and cmy for every angle")
+        t = np.column_stack([a.alpha, a.cl, a.cd, a.cmy]).astype(np.float64)
+        if np.any(np.diff(t[:, 0]) <= 0):
+            raise VlmError(f"airfoil '{n}': the angles must increase")
+        rows.append(t)
+    off = np.cumsum([0] + [len(r) for r in rows]).astype(np.int32)
+    tab = np.concatenate(rows)
+    sec = np.asarray([order.index(name) for name, _ in sections], np.int32)
+    span = np.asarray([float(x) for _, x in sections], np.float64)
+    return {"offsets": off, "alpha_deg": tab[:, 0].copy(), "cl": tab[:, 1].copy(), "cd": tab[:, 2].copy(),
+            "cmy": tab[:, 3].copy(), "section_airfoil": sec, "section_span": span}
+
+
 def _polar_struct(p):
     keep = {k: np.ascontiguousarray(v) for k, v in p.items()}
     st = Polar(len(keep["offsets"]) - 1, _i(keep["offsets"]), _d(keep["alpha_deg"]), _d(keep["cl"]), _d(keep["cd"]),

@@ -190,6 +190,34 @@ def fp32_accuracy(s, nc, V):
             "rows": "real cells, all V variables"}
 
 
+def fp32_forces(rans, s, steps=20, cfl=0.3, relax=0.8):
+    """CL and CD (get_wall_profile, post.h:301-387) after `steps` explicit
+    three-stage pseudo-time steps from the bench state, with FP64 residuals
+    and FP64 updates (the plain form) and with FP32 residuals and FP64 updates
+    (the batched launch with one member), on the headline mesh.  The device's
+    FP64 steps match the oracle's to 1e-10 in CL/CD in the test suite.  The
+    solver's state is restored afterwards."""
+    q0, nu0 = s.get_q(), s.get_nu()
+    s32 = s.clone()
+    s.set_cfl(cfl)
+    s32.set_cfl(cfl)
+    try:
+        for _ in range(steps):
+            s.explicit_solve(relax)
+        cl64, cd64, _ = s.get_wall_profile("wall")
+        with rans.Batch([s32], relaxation=relax) as b:
+            b.run(steps, 0)
+        cl32, cd32, _ = s32.get_wall_profile("wall")
+    finally:
+        s32.close()
+        s.set_q(q0)
+        s.set_nu(nu0)
+    return {"steps": steps, "cfl": cfl, "relaxation": relax, "cl_fp64": cl64, "cd_fp64": cd64, "cl_fp32": cl32,
+            "cd_fp32": cd32, "cl_rel_err": float(f"{abs(cl32 - cl64) / abs(cl64):.3e}"),
+            "cd_rel_err": float(f"{abs(cd32 - cd64) / abs(cd64):.3e}"),
+            "note": "FP32 residual + FP64 update vs FP64 residual + FP64 update, same start state and steps"}
+
+
 def read_traffic(path):
     """HBM bytes per residual from a committed rocprofv3 PMC summary, if any."""
     if not os.path.exists(path):
@@ -440,6 +468,7 @@ def adjoint_check(rans, cells=500_000, n_dir=64, h=1e-7):
     s.set_q(q)
     t_primal = s.time_residual(rans.AFX_FP64, 50)
     t_adj = s.time_adjoint(20)
+    t_tan = s.time_tangent(20)
     rng = np.random.default_rng(3)
 
     def unit():
@@ -461,11 +490,21 @@ def adjoint_check(rans, cells=500_000, n_dir=64, h=1e-7):
         fd = float(w @ (rp - rm)) / (2 * h * scale)
         ad = float(lam @ v)
         worst = max(worst, abs(ad - fd) / max(abs(fd), 1e-3 * float(np.linalg.norm(lam))))
+    # tangent/adjoint dot-product test on the same state: <w, J v> = <J^T w, v>
+    dots = []
+    for _ in range(4):
+        v = unit()
+        s.set_q(q)
+        jv = s.tangent(v)
+        dots.append(abs(float(w @ jv) - float(lam @ v)) / (float(np.linalg.norm(w)) * float(np.linalg.norm(jv))))
     out = {"cells": s.n_real, "model": "inviscid, Green-Gauss, second order (active-branch limiter)",
            "adjoint_ms": round(t_adj, 4), "primal_fp64_ms": round(t_primal, 4),
            "adjoint_over_primal": round(t_adj / t_primal, 2), "fd_directions": n_dir, "fd_step": h,
            "fd_max_rel_err": float(f"{worst:.3e}"),
-           "kernels": "5 passes: afx::k_adj_forward, k_adj_face, k_adj_cell1, k_adj_face2, k_adj_cell2"}
+           "kernels": "5 passes: afx::k_adj_forward, k_adj_face, k_adj_cell1, k_adj_face2, k_adj_cell2",
+           "tangent_ms": round(t_tan, 4), "tangent_over_primal": round(t_tan / t_primal, 2),
+           "tangent_kernel": "afx::k_residual_tile<afx::Dual1, 4> (forward mode of the FP64 tile residual)",
+           "dot_product_max_rel": float(f"{max(dots):.3e}"), "dot_product_tests": len(dots)}
     out["node_adjoint"] = node_adjoint_check(rans, m, s, q, w)
     out["dcl_dalpha"] = dcl_dalpha_check(rans, s, q)
     out["dcl_dnodes"] = dcl_dnodes_check(rans, m, s, q)
@@ -689,7 +728,7 @@ def aeroelastic_bench(n_chord=64, n_span=256, n_cases=32, alpha_max=8.0, cpu=Tru
         "lu_factor_ms": round(t_factor * 1e3, 2),
         "lu_factor_first_call_ms": round(t_factor_first * 1e3, 2),
         "lu_tflops": round(2.0 / 3.0 * N ** 3 / t_factor / 1e12, 2),
-        "lu_dtype": "f32 (rocBLAS sgemm on MFMA for the trailing update) + f64 refinement",
+        "lu_dtype": "f32 (hand-written MFMA GEMM for the trailing update) + f64 refinement",
         "solve_ms": round(t_solve * 1e3, 3),
         "solve_refinement_steps": sinfo.iterations,
         "solve_backward_error": sinfo.backward_error,
@@ -888,6 +927,7 @@ def main():
     t64 = time_stage(ev, s, stream, FP64, 3, args.fp64_steps)
     b64 = alg_bytes(nc, nf, nb, 8, V=V)
     acc32 = fp32_accuracy(s, nc, V)
+    forces32 = fp32_forces(rans, s) if V == 5 else None
     post = vtu_bench(rans, s, args.model, cpu=not args.no_cpu)
 
     prof = "residual_fp32_sa_2M_traffic.txt" if V == 5 else "residual_fp32_2M_traffic.txt"
@@ -915,13 +955,14 @@ def main():
                    "seeds": {"mesh": 1, "state": 2}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"afx::k_residual_tile_f32<{'true' if V == 5 else 'false'}>",
+                     "kernel": f"afx::k_residual_tile_f32<{2 if V == 5 else 0}, true, true>",
                      "alg_bytes_per_launch": b32,
                      "kernel_ms": round(res32_ms, 5), "timed_region_ms_per_step": round(ev_ms / args.steps, 5)},
         "fp64_plain": {"value": round(nc / (t64 * 1e-3) / 1e6, 2), "kernel_ms": round(t64, 5),
                        "kernel": f"afx::k_residual_tile<double, {V}>", "alg_bytes_per_launch": b64,
                        "roofline_frac": round(b64 / (t64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "fp32_vs_fp64": acc32,
+        "fp32_vs_fp64_forces": forces32,
         "fp32_three_kernel_ms": None if unf is None else {k: round(v, 5) for k, v in unf.items()},
     }
     if args.sweep_steps > 0 or args.adjoint:

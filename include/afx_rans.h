@@ -265,16 +265,31 @@ int afx_rans_nu_residual(afx_rans* h, double* Rnu);
 int afx_rans_create_like(const afx_rans* base, const afx_gas* g, afx_rans** out);
 /* Adjoint of the residual (no counterpart in the reference): lambda =
  * (dR/dq)^T w for R = fillRhoRHS (solver.h:1079-1152) at the current state,
- * for the inviscid residual with Green-Gauss gradients, first or second order
- * (the limiter differentiated on its active branch).  w, lambda: reference
+ * for the inviscid and laminar residuals with Green-Gauss gradients, first or
+ * second order (the limiter differentiated on its active branch; the laminar
+ * terms of physics.h:230-258 at the face gradient of solver.h:359-398).  w, lambda: reference
  * AoS layout 4 * n_cells; ghost rows of w are ignored and of lambda zero.
  * time_adjoint: average ms of one adjoint evaluation (HIP events). */
 int afx_rans_adjoint(afx_rans* h, const double* w, double* lambda);
+/* Tangent of the residual (no counterpart in the reference): jv = (dR/dq) v
+ * for R = fillRhoRHS (solver.h:1079-1152) at the current state, for every
+ * model (inviscid, laminar, Spalart-Allmaras), gradient scheme and order, the
+ * limiter differentiated on its active branch.  v, jv: reference AoS layout
+ * 4 * n_cells (ghost rows of v are ignored: ghost states are boundary data;
+ * ghost rows of jv are zero); with the Spalart-Allmaras model v_nu / jv_nu
+ * (n_cells each, nullable) carry the rho nu~ direction and row.  Forward-mode
+ * instantiation of the fused FP64 tile kernel; needs the fused kernel and
+ * twice its LDS per workgroup (AFX_EUNSUP otherwise). */
+int afx_rans_tangent(afx_rans* h, const double* v, const double* v_nu, double* jv, double* jv_nu);
+/* Average ms of one tangent evaluation on the device (HIP events; the last
+ * direction, no host transfers). */
+int afx_rans_time_tangent(afx_rans* h, int32_t reps, double* ms);
 /* The same evaluation plus the adjoint with respect to the mesh nodes:
  * xbar[2 i + d] = ((dR/dX)^T w) for coordinate d of node i (reference node
  * order, mesh.h nodesX / nodesY), through the metrics of mesh.h:378-453 and
  * 744-787 (normals, lengths, midpoints, centroids, areas, ghost centroids)
- * and the Green-Gauss weight of solver.h:436-444.  lambda is nullable. */
+ * and the Green-Gauss weight of solver.h:436-444.  lambda is nullable.
+ * Inviscid residual (AFX_EUNSUP for the laminar and SA models). */
 int afx_rans_adjoint_nodes(afx_rans* h, const double* w, double* lambda, double* xbar);
 /* dCL/dalpha through pseudo-time (no counterpart in the reference): runs
  * `steps` explicit steps (explicitSolver::solve, solver.h:802-828, with the

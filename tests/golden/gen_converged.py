@@ -53,7 +53,8 @@ def build_mesh(spec):
                            {1001: "farfield", 1002: "wall", 2003: "internal"})
 
 
-def run(name, cfg, start_cfl=40.0, slope_cfl=50.0, max_cfl=100.0, relax=0.9, rhs_iterations=5, floor=1e-11):
+def run(name, cfg, start_cfl=40.0, slope_cfl=50.0, max_cfl=100.0, relax=0.9, rhs_iterations=5, floor=1e-11,
+        save=True):
     m = build_mesh(cfg["mesh"])
     o = orc.OSolver(m, viscous=cfg["viscous"])
     o.set_bcs({"farfield": ("farfield", (cfg["mach"], cfg["angle"], 1.0, 1.0)),
@@ -92,12 +93,37 @@ def run(name, cfg, start_cfl=40.0, slope_cfl=50.0, max_cfl=100.0, relax=0.9, rhs
         if hist[-1] < floor:
             break
     cl, cd, cm = o.wall_profile("wall")
-    np.savez_compressed(os.path.join(HERE, f"converged_{name}.npz"), q=o.vec("q"), cl=cl, cd=cd, cm=cm,
-                        residuals=np.array(hist))
+    if save:
+        np.savez_compressed(os.path.join(HERE, f"converged_{name}.npz"), q=o.vec("q"), cl=cl, cd=cd, cm=cm,
+                            residuals=np.array(hist))
     print(f"{name}: {len(hist)} iterations, residual {hist[-1]:.3e}, CL {cl:.12f} CD {cd:.12e} CM {cm:.12e}")
+    return cl, cd, cm, hist[-1]
+
+
+# Polars of converged states (the alpha sweep of Rans::run_airfoil, rans.h:78-106,
+# in the reference's degree factor 0.01745): the step of the reference's
+# examples/conf.ini (3 deg) from -2 deg, so that the polar also covers the
+# low effective angles of a finite wing's tip stations
+POLARS = {
+    "naca0012q_coarse_m05_inviscid": dict(mesh=("msh", "naca0012q_coarse.msh"), mach=0.5,
+                                          alphas=(-2.0, 1.0, 4.0, 7.0), viscous=0, wall="slip-wall", iters=400),
+}
+
+
+def run_polar(name, cfg):
+    rows = []
+    for a in cfg["alphas"]:
+        c = dict(cfg, angle=a * 0.01745)
+        rows.append(run(f"{name}_a{a:g}", c, save=False))
+    rows = np.array(rows)
+    np.savez_compressed(os.path.join(HERE, f"polar_{name}.npz"), alpha=np.array(cfg["alphas"]), cl=rows[:, 0],
+                        cd=rows[:, 1], cm=rows[:, 2], final_residual=rows[:, 3])
 
 
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(CASES)
+    names = sys.argv[1:] or list(CASES) + [f"polar:{n}" for n in POLARS]
     for n in names:
-        run(n, CASES[n])
+        if n.startswith("polar:"):
+            run_polar(n[6:], POLARS[n[6:]])
+        else:
+            run(n, CASES[n])

@@ -84,10 +84,19 @@ def test_adjoint_against_oracle_differences():
 
 
 def test_adjoint_rejects_other_models():
+    """The reverse passes cover the inviscid and laminar residuals; the SA
+    model, and the node / pseudo-time adjoints of the laminar residual, are
+    refused."""
     from aeroflex_amd import rans
-    s, _, _ = make_pair(TRI, viscosity="laminar", wall="wall", mu_L=1e-3)
+    import test_gpu_sa
+    s, _ = test_gpu_sa.sa_pair(TRI)
     with pytest.raises(rans.RansError):
         s.adjoint(np.zeros(4 * s.n_cells))
+    s.close()
+    s, _, _ = make_pair(TRI, viscosity="laminar", wall="wall", mu_L=1e-3)
+    s.adjoint(np.zeros(4 * s.n_cells))
+    with pytest.raises(rans.RansError):
+        s.adjoint_nodes(np.zeros(4 * s.n_cells))
 
 
 # ------------------------------------------------------ node coordinates --
