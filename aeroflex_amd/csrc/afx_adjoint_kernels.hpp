@@ -20,7 +20,8 @@ hipError_t launch_residual_tile_tangent(hipStream_t s, TilesT<Dual1> S, const Du
                                         const Dual1* wd, GasT<double> g64, int gg, int so, double limiter_k,
                                         int viscous, Dual1* R);
 
-// work arrays of one adjoint evaluation ([4][nT] / [4][nC] / [4][nF] planes)
+// work arrays of one adjoint evaluation ([4][nT] / [4][nC] / [4][nF] planes;
+// five planes each with the Spalart-Allmaras model)
 struct AdjBufs {
     double* gx;      // [4][nT] cell gradients
     double* gy;
@@ -70,6 +71,11 @@ struct GeoBufs {
 hipError_t launch_adjoint_state(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* w,
                                 GasT<double> g, int second_order, double limiter_k, AdjBufs A, double* lam,
                                 const GeoBufs* G = nullptr);
+// state adjoint of the Spalart-Allmaras residual (five planes per array of A,
+// vgx / vgy / vdq required; wd: wall distance per device cell)
+hipError_t launch_adjoint_state_sa(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q,
+                                   const double* w, const double* wd, GasT<double> g, int second_order,
+                                   double limiter_k, AdjBufs A, double* lam);
 
 // The explicit stage residual of explicitSolver::solve (solver.h:802-828)
 // takes its gradients from the step's start state qg and everything else

@@ -1,0 +1,409 @@
+// State adjoint of the FP64 Spalart-Allmaras residual: lambda = (dR/dq)^T w
+// over the five rows (rho, rho u, rho v, rho E, rho nu~) of the fused tile
+// residual with the SA model (k_residual_tile<double, 5>), Green-Gauss
+// gradients (solver.h:428-469), the default limiter (solver.h:517-593),
+// first or second order.
+//
+// The same five atomics-free passes as the inviscid and laminar adjoint
+// (afx_adjoint.hip), over five variables:
+//   1 cells  gradients, limiters and the limiter's active branch
+//   2 faces  reconstruction, then the adjoint of the five-variable face flux
+//            (sa_face_flux: Roe, upwind rho nu~ convection, viscous terms with
+//            mu_L + mu_t at the face gradients, nu~ diffusion, the no-slip
+//            wall's nu~ diffusion) with respect to the reconstructed states,
+//            the face gradients and the owner's cell-centre nu~
+//   3 cells  gather: direct state adjoint, gradient adjoint (reconstruction,
+//            face gradients, and the cell source sa_cell_source with its
+//            unlimited gradients), limiter adjoint on the active branch
+//   4 faces  transpose of the Green-Gauss face values (boundary faces: the
+//            face states of face_vars and sa_bnd_nu)
+//   5 cells  gather of pass 4 and of the neighbours' limiter parts
+// The local Jacobians of the flux, the source and the boundary face values
+// come from forward-mode sweeps (one per input) of the same device functions
+// the residual evaluates, so they are exact; the structure around them is
+// transposed by hand.  Wall / slip ghost states enter nothing (their face
+// states are functions of the owner's); farfield ghost states are boundary
+// data and get no adjoint.
+#include "afx_device.hpp"
+#include "afx_adjoint.hpp"
+#include "afx_adjoint_kernels.hpp"
+
+namespace afx {
+
+template <> __device__ __forceinline__ Dual1 ddiv<Dual1>(Dual1 a, Dual1 b) { return a / b; }
+template <> __device__ __forceinline__ Dual1 dsqrt<Dual1>(Dual1 x) { return dual_sqrt(x); }
+template <> __device__ __forceinline__ Dual1 dcbrt<Dual1>(Dual1 x) { return dual_cbrt(x); }
+
+namespace {
+
+constexpr int NS = 5;
+
+__device__ __forceinline__ GasT<Dual1> sa_dual_gas(const GasT<double>& g) {
+    return GasT<Dual1>{Dual1(g.gamma), Dual1(g.R), Dual1(g.mu), Dual1(g.k), Dual1(g.cp), Dual1(g.PrL), Dual1(g.PrT)};
+}
+
+// Green-Gauss face value of the five variables on a boundary face from the
+// owner's state qo (5) and the ghost's qn (5): face_vars for the mean flow,
+// sa_bnd_nu for rho nu~ (as k_residual_tile stages them)
+template <typename T>
+__device__ __forceinline__ void bnd_face_value(const GasT<T>& g, int kind, T nx, T ny, const T* qo, const T* qn,
+                                               T* qv) {
+    face_vars<T>(g, kind, nx, ny, qo, qn, qv);
+    qv[4] = sa_bnd_nu<T>(kind, nx, ny, qo, qo[4], qn, qn[4], qv);
+}
+
+// pass 1: gradients, limiters and the active branch of every (cell, variable)
+__global__ void __launch_bounds__(kBlock) k_sadj_forward(CellsT<double> C, FacesT<double> F,
+                                                         const double* __restrict__ q, GasT<double> g,
+                                                         int second_order, double limiter_k, AdjBufs A) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C.nC) return;
+    const size_t nT = (size_t)C.nT, nC = (size_t)C.nC;
+    double qc[NS], sx[NS], sy[NS], mn[NS], mx[NS];
+    int jmn[NS], jmx[NS];
+    for (int k = 0; k < NS; ++k) {
+        qc[k] = q[k * nT + c];
+        mn[k] = mx[k] = qc[k];
+        sx[k] = sy[k] = 0.0;
+        jmn[k] = jmx[k] = -1;
+    }
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t e = C.cface[(size_t)s * nC + c];
+        if (e == 0xFFFFFFFFu) break;
+        const int f = (int)(e & 0x7FFFFFFFu);
+        const int o = F.c0[f], n = F.c1[f], kind = F.kind[f];
+        const double nx = F.nx[f], ny = F.ny[f];
+        double qo[NS], qn[NS], qv[NS];
+        for (int k = 0; k < NS; ++k) { qo[k] = q[k * nT + o]; qn[k] = q[k * nT + n]; }
+        face_vars<double>(g, kind, nx, ny, qo, qn, qv);
+        qv[4] = kind == K_INTERNAL ? (qo[4] + qn[4]) * 0.5 : sa_bnd_nu<double>(kind, nx, ny, qo, qo[4], qn, qn[4], qv);
+        const double gw = F.gw[f], L = F.len[f];
+        for (int k = 0; k < NS; ++k) {
+            const double fk = (qo[k] * (1.0 - gw) + qv[k] * gw) * L;
+            if (e >> 31) { sx[k] -= fk * nx; sy[k] -= fk * ny; }
+            else { sx[k] += fk * nx; sy[k] += fk * ny; }
+        }
+        if (second_order && kind != K_WALL && kind != K_SLIP) {   // wall ghost = owner copy
+            const int other = (e >> 31) ? o : n;
+            for (int k = 0; k < NS; ++k) {
+                const double v = (e >> 31) ? qo[k] : qn[k];
+                if (v < mn[k]) { mn[k] = v; jmn[k] = other; }
+                if (mx[k] < v) { mx[k] = v; jmx[k] = other; }
+            }
+        }
+    }
+    const double Ar = C.area[c];
+    for (int k = 0; k < NS; ++k) {
+        sx[k] /= Ar;
+        sy[k] /= Ar;
+        A.gx[k * nT + c] = sx[k];
+        A.gy[k * nT + c] = sy[k];
+    }
+    if (!second_order) return;
+    const double Ka = limiter_k * sqrt(Ar), K3a = Ka * Ka * Ka;
+    double lm[NS];
+    int act[NS];
+    for (int k = 0; k < NS; ++k) { lm[k] = 1.0; act[k] = -1; }
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t e = C.cface[(size_t)s * nC + c];
+        if (e == 0xFFFFFFFFu) break;
+        const int f = (int)(e & 0x7FFFFFFFu);
+        const double dx = F.xf[f] - C.xc[c], dy = F.yf[f] - C.yc[c];
+        for (int k = 0; k < NS; ++k) {
+            const double dqg = sx[k] * dx + sy[k] * dy;
+            const double l = limiter_phi<double>(dqg, mx[k] - qc[k], mn[k] - qc[k], K3a);
+            if (l < lm[k]) { lm[k] = l; act[k] = s; }
+        }
+    }
+    for (int k = 0; k < NS; ++k) {
+        A.lim[k * nT + c] = lm[k];
+        A.act[k * nC + c] = (int8_t)act[k];
+        A.jmax[k * nC + c] = jmx[k];
+        A.jmin[k * nC + c] = jmn[k];
+    }
+}
+
+// pass 2: per face, adjoints of the reconstructed states (bL, bR), of the
+// face gradients (vgx, vgy: half of them, owed to both cells' gradients) and
+// of the direct cell-state dependences (vdq: the face gradient's difference
+// quotient, owner sign, and the wall diffusion's cell-centre nu~).  PART 1:
+// interior faces; PART 2: the boundary faces of A.bface.
+template <int PART>
+__global__ void __launch_bounds__(kBlock) k_sadj_face(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
+                                                      const double* __restrict__ w, GasT<double> g, int second_order,
+                                                      AdjBufs A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int f;
+    if constexpr (PART == 1) {
+        f = i;
+        if (f >= F.nF || F.kind[f] != K_INTERNAL) return;
+    } else {
+        if (i >= A.nbf) return;
+        f = A.bface[i];
+    }
+    const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
+    const int o = F.c0[f], n = F.c1[f];
+    const int kind = PART == 1 ? (int)K_INTERNAL : (int)F.kind[f];
+    const double nx = F.nx[f], ny = F.ny[f], L = F.len[f];
+    const double d0x = F.xf[f] - C.xc[o], d0y = F.yf[f] - C.yc[o];
+    double qL[NS], qR[NS];
+    for (int k = 0; k < NS; ++k) { qL[k] = q[k * nT + o]; qR[k] = q[k * nT + n]; }
+    if (second_order) {
+        for (int k = 0; k < NS; ++k)
+            qL[k] += (A.gx[k * nT + o] * d0x + A.gy[k * nT + o] * d0y) * A.lim[k * nT + o];
+        if (kind == K_INTERNAL) {
+            const double d1x = F.xf[f] - C.xc[n], d1y = F.yf[f] - C.yc[n];
+            for (int k = 0; k < NS; ++k)
+                qR[k] += (A.gx[k * nT + n] * d1x + A.gy[k * nT + n] * d1y) * A.lim[k * nT + n];
+        }
+    }
+    double bF[NS], bL[NS], bR[NS], vgx[NS], vgy[NS], vdq[NS];
+    for (int k = 0; k < NS; ++k) {
+        bF[k] = kind == K_INTERNAL ? L * (w[k * nT + n] - w[k * nT + o]) : -L * w[k * nT + o];
+        bL[k] = bR[k] = vgx[k] = vgy[k] = vdq[k] = 0.0;
+    }
+    const GasT<Dual1> gd = sa_dual_gas(g);
+    if (kind == K_INTERNAL) {
+        // face gradients of the SA model: textbook direction t = (x_n - x_o) /
+        // l and derivative (q_n - q_o) / l, g = gbar - (gbar . t - dq / l) t
+        double t0 = C.xc[n] - C.xc[o], t1 = C.yc[n] - C.yc[o];
+        const double l = sqrt(t0 * t0 + t1 * t1);
+        t0 /= l;
+        t1 /= l;
+        double gxa[NS], gya[NS];
+        for (int k = 0; k < NS; ++k) {
+            const double dir = (q[k * nT + n] - q[k * nT + o]) / l;
+            const double gxb = (A.gx[k * nT + o] + A.gx[k * nT + n]) * 0.5;
+            const double gyb = (A.gy[k * nT + o] + A.gy[k * nT + n]) * 0.5;
+            const double dot = gxb * t0 + gyb * t1;
+            gxa[k] = gxb - (dot - dir) * t0;
+            gya[k] = gyb - (dot - dir) * t1;
+        }
+        // one sweep per input: qL (5), qR (5), face gradients x (5), y (5)
+        double bgx[NS], bgy[NS];
+        for (int j = 0; j < 4 * NS; ++j) {
+            Dual1 a[NS], b[NS], gx[NS], gy[NS], fo[NS];
+            for (int k = 0; k < NS; ++k) {
+                a[k] = Dual1(qL[k], j == k ? 1.0 : 0.0);
+                b[k] = Dual1(qR[k], j == NS + k ? 1.0 : 0.0);
+                gx[k] = Dual1(gxa[k], j == 2 * NS + k ? 1.0 : 0.0);
+                gy[k] = Dual1(gya[k], j == 3 * NS + k ? 1.0 : 0.0);
+            }
+            sa_face_flux<Dual1>(gd, K_INTERNAL, Dual1(nx), Dual1(ny), a, b, gx, gy, Dual1(0.0), Dual1(1.0), fo);
+            double v = 0.0;
+            for (int k = 0; k < NS; ++k) v += bF[k] * fo[k].d;
+            if (j < NS) bL[j] = v;
+            else if (j < 2 * NS) bR[j - NS] = v;
+            else if (j < 3 * NS) bgx[j - 2 * NS] = v;
+            else bgy[j - 3 * NS] = v;
+        }
+        for (int k = 0; k < NS; ++k) {
+            const double bt = bgx[k] * t0 + bgy[k] * t1;   // the adjoint of dq / l
+            vgx[k] = 0.5 * (bgx[k] - t0 * bt);
+            vgy[k] = 0.5 * (bgy[k] - t1 * bt);
+            vdq[k] = -bt / l;                              // owner's share; the neighbour takes -vdq
+        }
+    } else {
+        // boundary face: the flux of the reconstructed owner state and the
+        // ghost's boundary data; the wall diffusion reads the owner's
+        // cell-centre nu~ = (rho nu~) / rho and its normal distance d0 . n
+        const double qo0 = q[o], qo4 = q[4 * nT + o];
+        const double nut_c = qo4 / qo0, dn = d0x * nx + d0y * ny;
+        double bnut = 0.0;
+        for (int j = 0; j <= NS; ++j) {
+            Dual1 a[NS], b[NS], z[NS], fo[NS];
+            for (int k = 0; k < NS; ++k) {
+                a[k] = Dual1(qL[k], j == k ? 1.0 : 0.0);
+                b[k] = Dual1(qR[k]);
+                z[k] = Dual1(0.0);
+            }
+            sa_face_flux<Dual1>(gd, kind, Dual1(nx), Dual1(ny), a, b, z, z, Dual1(nut_c, j == NS ? 1.0 : 0.0),
+                                Dual1(dn), fo);
+            double v = 0.0;
+            for (int k = 0; k < NS; ++k) v += bF[k] * fo[k].d;
+            if (j < NS) bL[j] = v;
+            else bnut = v;
+        }
+        vdq[0] = -bnut * qo4 / (qo0 * qo0);
+        vdq[4] = bnut / qo0;
+    }
+    for (int k = 0; k < NS; ++k) {
+        A.bL[k * nF + f] = bL[k];
+        A.bR[k * nF + f] = bR[k];
+        A.vgx[k * nF + f] = vgx[k];
+        A.vgy[k * nF + f] = vgy[k];
+        A.vdq[k * nF + f] = vdq[k];
+    }
+}
+
+// pass 3: per cell, state / gradient / limiter adjoints, with the cell
+// source's adjoint (w_rho nu~ A dS/dq, dS/d grad)
+__global__ void __launch_bounds__(kBlock) k_sadj_cell1(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
+                                                       const double* __restrict__ w, const double* __restrict__ wd,
+                                                       GasT<double> g, int second_order, double limiter_k, AdjBufs A,
+                                                       double* __restrict__ lam) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C.nC) return;
+    const size_t nT = (size_t)C.nT, nC = (size_t)C.nC, nF = (size_t)F.nF;
+    double lq[NS], lgx[NS], lgy[NS], lphi[NS], gx[NS], gy[NS], ph[NS], qc[NS];
+    for (int k = 0; k < NS; ++k) {
+        lq[k] = lgx[k] = lgy[k] = lphi[k] = 0.0;
+        gx[k] = A.gx[k * nT + c];
+        gy[k] = A.gy[k * nT + c];
+        ph[k] = second_order ? A.lim[k * nT + c] : 1.0;
+        qc[k] = q[k * nT + c];
+    }
+    uint32_t es[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    double dxs[4] = {0, 0, 0, 0}, dys[4] = {0, 0, 0, 0};
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t e = C.cface[(size_t)s * nC + c];
+        if (e == 0xFFFFFFFFu) break;
+        const int f = (int)(e & 0x7FFFFFFFu);
+        const bool nb = (e >> 31) != 0;
+        const double dx = F.xf[f] - C.xc[c], dy = F.yf[f] - C.yc[c];
+        es[s] = e; dxs[s] = dx; dys[s] = dy;
+        for (int k = 0; k < NS; ++k) {
+            const double b = nb ? A.bR[k * nF + f] : A.bL[k * nF + f];
+            lq[k] += b;
+            if (second_order) {
+                lphi[k] += b * (gx[k] * dx + gy[k] * dy);
+                lgx[k] += b * ph[k] * dx;
+                lgy[k] += b * ph[k] * dy;
+            }
+            lgx[k] += A.vgx[k * nF + f];
+            lgy[k] += A.vgy[k * nF + f];
+            lq[k] += nb ? -A.vdq[k * nF + f] : A.vdq[k * nF + f];
+        }
+    }
+    // the source term A S(q_c, grad q_c, d_c) of the rho nu~ row
+    const double ws = w[4 * nT + c] * C.area[c];
+    if (ws != 0.0) {
+        const GasT<Dual1> gd = sa_dual_gas(g);
+        for (int j = 0; j < 3 * NS; ++j) {
+            Dual1 a[NS], bx[NS], by[NS];
+            for (int k = 0; k < NS; ++k) {
+                a[k] = Dual1(qc[k], j == k ? 1.0 : 0.0);
+                bx[k] = Dual1(gx[k], j == NS + k ? 1.0 : 0.0);
+                by[k] = Dual1(gy[k], j == 2 * NS + k ? 1.0 : 0.0);
+            }
+            const double v = ws * sa_cell_source<Dual1>(gd, a, bx, by, Dual1(wd[c])).d;
+            if (j < NS) lq[j] += v;
+            else if (j < 2 * NS) lgx[j - NS] += v;
+            else lgy[j - 2 * NS] += v;
+        }
+    }
+    for (int k = 0; k < NS; ++k) {
+        double nbv = 0.0;
+        int nbj = -1;
+        const int s = second_order ? (int)A.act[k * nC + c] : -1;
+        if (s >= 0) {
+            const uint32_t e = s == 0 ? es[0] : s == 1 ? es[1] : s == 2 ? es[2] : es[3];
+            (void)e;
+            const double dx = s == 0 ? dxs[0] : s == 1 ? dxs[1] : s == 2 ? dxs[2] : dxs[3];
+            const double dy = s == 0 ? dys[0] : s == 1 ? dys[1] : s == 2 ? dys[2] : dys[3];
+            const double dqg = gx[k] * dx + gy[k] * dy;
+            const int j = dqg > 1e-16 ? A.jmax[k * nC + c] : A.jmin[k * nC + c];
+            const double D = (j >= 0 ? q[k * nT + j] : qc[k]) - qc[k];
+            const double Ka = limiter_k * sqrt(C.area[c]), K3a = Ka * Ka * Ka;
+            double d_dqg, d_D;
+            (void)limiter_phi_d<double>(dqg, D, K3a, &d_dqg, &d_D, nullptr);
+            lgx[k] += lphi[k] * d_dqg * dx;
+            lgy[k] += lphi[k] * d_dqg * dy;
+            if (j >= 0) {   // D = q_j - q_c
+                lq[k] -= lphi[k] * d_D;
+                if (j < C.nC) { nbv = lphi[k] * d_D; nbj = j; }   // a farfield ghost is boundary data
+            }
+        }
+        lam[k * nT + c] = lq[k];
+        A.lgx[k * nT + c] = lgx[k];
+        A.lgy[k * nT + c] = lgy[k];
+        A.nbv[k * nC + c] = nbv;
+        A.nbj[k * nC + c] = nbj;
+    }
+}
+
+// pass 4: transpose of the Green-Gauss face values
+__global__ void __launch_bounds__(kBlock) k_sadj_face2(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
+                                                       GasT<double> g, AdjBufs A) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F.nF) return;
+    const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
+    const int o = F.c0[f], n = F.c1[f], kind = F.kind[f];
+    const double nx = F.nx[f], ny = F.ny[f], L = F.len[f], gw = F.gw[f];
+    const bool two = kind == K_INTERNAL;
+    double lfv[NS], co[NS], cn[NS];
+    for (int k = 0; k < NS; ++k) {
+        double v = (A.lgx[k * nT + o] * nx + A.lgy[k * nT + o] * ny) * L / C.area[o];
+        if (two) v -= (A.lgx[k * nT + n] * nx + A.lgy[k * nT + n] * ny) * L / C.area[n];
+        lfv[k] = v;
+    }
+    if (two) {   // face value q_o (1 - gw) + (q_o + q_n) gw / 2, every variable
+        for (int k = 0; k < NS; ++k) { co[k] = lfv[k] * (1.0 - 0.5 * gw); cn[k] = lfv[k] * (0.5 * gw); }
+    } else {     // q_o (1 - gw) + face value (q_o; ghost) gw
+        const GasT<Dual1> gd = sa_dual_gas(g);
+        Dual1 qn[NS];
+        double qo[NS];
+        for (int k = 0; k < NS; ++k) {
+            qo[k] = q[k * nT + o];
+            qn[k] = Dual1(q[k * nT + n]);
+            co[k] = lfv[k] * (1.0 - gw);
+            cn[k] = 0.0;
+        }
+        for (int j = 0; j < NS; ++j) {
+            Dual1 a[NS], v[NS];
+            for (int k = 0; k < NS; ++k) a[k] = Dual1(qo[k], j == k ? 1.0 : 0.0);
+            bnd_face_value<Dual1>(gd, kind, Dual1(nx), Dual1(ny), a, qn, v);
+            double s = 0.0;
+            for (int k = 0; k < NS; ++k) s += lfv[k] * gw * v[k].d;
+            co[j] += s;
+        }
+    }
+    for (int k = 0; k < NS; ++k) {
+        A.co[k * nF + f] = co[k];
+        A.cn[k * nF + f] = cn[k];
+    }
+}
+
+// pass 5: gather of the Green-Gauss transpose and the neighbours' limiter parts
+__global__ void __launch_bounds__(kBlock) k_sadj_cell2(CellsT<double> C, FacesT<double> F, AdjBufs A,
+                                                       double* __restrict__ lam) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C.nC) return;
+    const size_t nT = (size_t)C.nT, nC = (size_t)C.nC, nF = (size_t)F.nF;
+    double acc[NS];
+    for (int k = 0; k < NS; ++k) acc[k] = lam[k * nT + c];
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t e = C.cface[(size_t)s * nC + c];
+        if (e == 0xFFFFFFFFu) break;
+        const int f = (int)(e & 0x7FFFFFFFu);
+        const bool nb = (e >> 31) != 0;
+        const int other = nb ? F.c0[f] : F.c1[f];
+        for (int k = 0; k < NS; ++k) {
+            acc[k] += nb ? A.cn[k * nF + f] : A.co[k * nF + f];
+            if (other < C.nC && A.nbj[k * nC + other] == c) acc[k] += A.nbv[k * nC + other];
+        }
+    }
+    for (int k = 0; k < NS; ++k) lam[k * nT + c] = acc[k];
+}
+
+}  // namespace
+
+hipError_t launch_adjoint_state_sa(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q,
+                                   const double* w, const double* wd, GasT<double> g, int second_order,
+                                   double limiter_k, AdjBufs A, double* lam) {
+    if (!A.vgx || !A.vgy || !A.vdq) return hipErrorInvalidValue;
+    const int nbc = (C.nC + kBlock - 1) / kBlock, nbf = (F.nF + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_sadj_forward, dim3(nbc), dim3(kBlock), 0, s, C, F, q, g, second_order, limiter_k, A);
+    hipLaunchKernelGGL(k_sadj_face<1>, dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A);
+    if (A.nbf > 0)
+        hipLaunchKernelGGL(k_sadj_face<2>, dim3((A.nbf + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F, q, w, g,
+                           second_order, A);
+    hipLaunchKernelGGL(k_sadj_cell1, dim3(nbc), dim3(kBlock), 0, s, C, F, q, w, wd, g, second_order, limiter_k, A,
+                       lam);
+    // the gradients always enter (face gradients of the viscous terms, the source)
+    hipLaunchKernelGGL(k_sadj_face2, dim3(nbf), dim3(kBlock), 0, s, C, F, q, g, A);
+    hipLaunchKernelGGL(k_sadj_cell2, dim3(nbc), dim3(kBlock), 0, s, C, F, A, lam);
+    return hipGetLastError();
+}
+
+}  // namespace afx

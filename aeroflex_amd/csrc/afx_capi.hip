@@ -104,7 +104,7 @@ struct afx_rans {
 
     // fused tile residual
     TileLayout tl;
-    DBuf<int32_t> t_cell0, t_loff, t_ng, t_foff, t_nft, t_goff, t_Lglob;
+    DBuf<int32_t> t_cell0, t_loff, t_ng, t_foff, t_nft, t_goff, t_Lglob, t_nbf;
     DBuf<uint64_t> t_slots;
     DBuf<uint32_t> t_fcell;
     DBuf<int8_t> t_fkind;
@@ -139,11 +139,37 @@ struct afx_rans {
     DBuf<float> q32n_b, q32n_c;
 
     // implicit step: Jacobian blocks, diagonal blocks and their inverses,
-    // GMRES work space
-    DBuf<double> Jf, Dblk, Dinv, kV, kw, kz, kr, ktmp, kh, kpart;
-    DBuf<int> kbad;
+    // GMRES work space and state (kg: the GmDev arrays, kflags its flags)
+    DBuf<double> Jf, Dblk, Dinv, kV, kw, kz, kr, ktmp, kpart, kg;
+    DBuf<int> kbad, kflags;
     int32_t gm_restart = 30, gm_maxit = 500;
     double gm_tol = 1e-2;
+    // preconditioner: 0 block Jacobi, 1 multicolour block DILU (colour per
+    // real cell, cells by colour, colour offsets on the host, Dt^-1 blocks)
+    int precond = 1;
+    DBuf<uint8_t> colour;
+    DBuf<int32_t> col_cells;
+    std::vector<int32_t> col_off;
+    DBuf<double> Dti;
+    GmDev gm_dev() const {
+        const int m = gm_restart;
+        double* b = kg.p;
+        GmDev G;
+        G.hdr = b;
+        G.H = b + 8;
+        G.cs = G.H + (size_t)(m + 1) * m;
+        G.sn = G.cs + m;
+        G.g = G.sn + m;
+        G.y = G.g + (m + 1);
+        G.h1 = G.y + m;
+        G.h2 = G.h1 + (m + 1);
+        G.flags = kflags.p;
+        G.tol = gm_tol;
+        G.m = m;
+        G.maxit = gm_maxit;
+        return G;
+    }
+    static size_t gm_doubles(int m) { return 8 + (size_t)(m + 1) * m + 2 * (size_t)m + (m + 1) + m + 2 * (size_t)(m + 1); }
     bool lhs_ready = false;
     int lhs_bad = 0;
     bool mf = false;             // matrix-free operator (RANS_MATRIX_FREE build)
@@ -193,7 +219,7 @@ struct afx_rans {
     int so() const { return opts.second_order ? (opts.limiter == 1 ? 3 : 1) : 0; }
     TilesF32 tiles32() const {
         return TilesF32{t_cell0.p, t_loff.p, t_ng.p, t_foff.p, t_nft.p, t_goff.p, t_Lglob.p, t_slots.p,
-                        t_fck.p, t_geo4.p, tl.ntiles, tl.lmax, tl.gmax, tl.ftmax};
+                        t_fck.p, t_geo4.p, t_nbf.p, tl.ntiles, tl.lmax, tl.gmax, tl.ftmax};
     }
     template <typename T>
     TilesT<T> tiles(const T* geo) const {
@@ -424,117 +450,129 @@ int ensure_implicit(afx_rans* h) {
     HIPCHK(h->kz.alloc(n));
     HIPCHK(h->kr.alloc(n));
     HIPCHK(h->ktmp.alloc(n));
-    HIPCHK(h->kh.alloc(2 * (size_t)(m + 2)));
     HIPCHK(h->kpart.alloc((size_t)(m + 2) * h->kblk));
+    HIPCHK(h->kg.alloc(afx_rans::gm_doubles(m)));
+    HIPCHK(h->kflags.alloc(8));
     HIPCHK(h->kbad.alloc(1));
+    HIPCHK(h->Dti.alloc((size_t)16 * h->nC));
     HIPCHK(hipMemsetAsync(h->Jf.p, 0, (size_t)32 * h->nF * 8, h->stream));
-    return AFX_OK;
-}
-
-// dot products of w with the first m vectors of V (stride ld) into dev[0..m)
-int dots(afx_rans* h, const double* V, int64_t ld, int m, const double* w, int64_t n, double* dev) {
-    HIPCHK(launch_multidot(h->stream, V, ld, m, w, n, h->kpart.p, h->kblk, dev));
+    HIPCHK(hipMemsetAsync(h->kflags.p, 0, 8 * sizeof(int), h->stream));
+    // greedy colouring of the real cells in device order over their
+    // real-cell face neighbours (at most four), for the DILU sweeps
+    {
+        const int32_t nC = h->nC;
+        std::vector<uint32_t> cf((size_t)4 * nC);
+        std::vector<int32_t> c0(h->nF), c1(h->nF);
+        HIPCHK(hipMemcpyAsync(cf.data(), h->cface.p, cf.size() * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(c0.data(), h->c0.p, c0.size() * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(c1.data(), h->c1.p, c1.size() * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        std::vector<uint8_t> col(nC, 0xFF);
+        int ncol = 0;
+        for (int32_t i = 0; i < nC; ++i) {
+            unsigned used = 0;
+            for (int sl = 0; sl < 4; ++sl) {
+                const uint32_t e = cf[(size_t)sl * nC + i];
+                if (e == 0xFFFFFFFFu) break;
+                const int32_t f = (int32_t)(e & 0x7FFFFFFFu);
+                const int32_t j = (e >> 31) ? c0[f] : c1[f];
+                if (j < nC && col[j] != 0xFF) used |= 1u << col[j];
+            }
+            int c = 0;
+            while (used & (1u << c)) ++c;   // at most 4 neighbours: c <= 4
+            col[i] = (uint8_t)c;
+            ncol = std::max(ncol, c + 1);
+        }
+        h->col_off.assign(ncol + 1, 0);
+        for (int32_t i = 0; i < nC; ++i) h->col_off[col[i] + 1]++;
+        for (int c = 0; c < ncol; ++c) h->col_off[c + 1] += h->col_off[c];
+        std::vector<int32_t> cells(nC), pos(h->col_off.begin(), h->col_off.end() - 1);
+        for (int32_t i = 0; i < nC; ++i) cells[pos[col[i]]++] = i;
+        HIPCHK(h->colour.alloc(std::max<size_t>(nC, 1)));
+        HIPCHK(h->colour.upload(col, h->stream));
+        HIPCHK(h->col_cells.alloc(std::max<size_t>(nC, 1)));
+        HIPCHK(h->col_cells.upload(cells, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
     return AFX_OK;
 }
 
 // Right-preconditioned restarted GMRES (classical Gram-Schmidt applied twice)
-// for LHS x = b; b and x are device vectors of length 4*nT.  Returns AFX_OK
-// and *converged, or a negative error code.
+// for LHS x = b; b and x are device vectors of length 4*nT.  The Arnoldi
+// state stays on the device (GmDev): a restart cycle of m steps is enqueued
+// at once, the kernels of a step do nothing once the iteration has
+// converged or broken down, and the host reads the flags once per cycle.
+// Returns AFX_OK and *converged, or a negative error code.
 int gmres(afx_rans* h, const double* b, double* x, bool* converged) {
     const int64_t n = (int64_t)NV * h->nT;
     const int m = h->gm_restart;
+    const int nb = h->kblk;
     hipStream_t s = h->stream;
     double* V = h->kV.p;
-    double* hd = h->kh.p;            // [0, m+2): first pass, [m+2, 2m+4): second pass + norm
+    double* part = h->kpart.p;
+    const GmDev G = h->gm_dev();
+    const int* active = G.flags;
     *converged = false;
     // the operator and the right preconditioner: the assembled matrix with
-    // block Jacobi, or the matrix-free product with none (solver.h:868)
-    auto apply = [&](const double* in, double* out) -> hipError_t {
+    // block Jacobi or block DILU, or the matrix-free product with none
+    // (solver.h:868)
+    auto apply = [&](const double* in, double* out, const int* act) -> hipError_t {
         if (h->mf)
             return launch_mf_apply(s, h->cells64(), h->faces64(), h->q.p, h->dt.p, h->mfFb.p, h->gas64(),
-                                   h->opts.viscosity, in, out);
-        return launch_lhs_apply(s, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, in, out);
+                                   h->opts.viscosity, in, out, act);
+        return launch_lhs_apply(s, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, in, out, act);
     };
-    auto precond = [&](const double* in, double* out, const double** res) -> hipError_t {
+    auto precond = [&](const double* in, double* out, const double** res, const int* act) -> hipError_t {
         if (h->mf) { *res = in; return hipSuccess; }
         *res = out;
-        return launch_block_jacobi(s, h->nC, h->nT, h->Dinv.p, in, out);
+        if (h->precond == 1)
+            return launch_dilu_apply(s, h->cells64(), h->faces64(), h->colour.p, h->col_cells.p, h->col_off.data(),
+                                     (int)h->col_off.size() - 1, h->Jf.p, h->Dti.p, in, out, act);
+        return launch_block_jacobi(s, h->nC, h->nT, h->Dinv.p, in, out, act);
+    };
+    // |r|^2 of the cycle's start vector into hdr[3], then the cycle's set-up
+    auto begin = [&](int first) -> hipError_t {
+        hipError_t e = launch_gs_pass(s, nullptr, n, 0, nullptr, h->kr.p, n, false, part, nb, nullptr);
+        if (e == hipSuccess) e = launch_gm_reduce(s, part, nb, 0, 1, G.hdr + 3, nullptr);
+        if (e == hipSuccess) e = launch_gm_begin(s, G, first);
+        if (e == hipSuccess) e = launch_scale_dev(s, G.hdr + 2, h->kr.p, V, n, active);
+        return e;
     };
     const double* pz = nullptr;
     HIPCHK(hipMemsetAsync(x, 0, n * 8, s));
-    int rc;
-    if ((rc = dots(h, b, n, 1, b, n, hd))) return rc;
-    double beta0 = 0;
-    HIPCHK(hipMemcpyAsync(&beta0, hd, 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    beta0 = std::sqrt(beta0);
-    if (!(beta0 > 0)) { *converged = true; return AFX_OK; }
     HIPCHK(launch_copy(s, b, h->kr.p, n));
-    double beta = beta0;
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hv(2 * (m + 2));
-    int total = 0;
+    HIPCHK(begin(1));
+    int flags[4] = {0, 0, 0, 0};
+    int total_before = 0;
     while (true) {
-        HIPCHK(launch_scale(s, 1.0 / beta, h->kr.p, V, n));
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = beta;
-        int jend = 0;
-        bool conv = false;
         for (int j = 0; j < m; ++j) {
             double* vj = V + (size_t)j * n;
-            HIPCHK(precond(vj, h->kz.p, &pz));
-            HIPCHK(apply(pz, h->kw.p));
-            if ((rc = dots(h, V, n, j + 1, h->kw.p, n, hd))) return rc;
-            HIPCHK(launch_multiaxpy(s, V, n, j + 1, hd, h->kw.p, n));
-            if ((rc = dots(h, V, n, j + 1, h->kw.p, n, hd + m + 2))) return rc;
-            HIPCHK(launch_multiaxpy(s, V, n, j + 1, hd + m + 2, h->kw.p, n));
-            if ((rc = dots(h, h->kw.p, n, 1, h->kw.p, n, hd + m + 2 + j + 1))) return rc;
-            HIPCHK(hipMemcpyAsync(hv.data(), hd, hv.size() * 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hv[i] + hv[m + 2 + i];
-            const double nrm = std::sqrt(std::max(hv[m + 2 + j + 1], 0.0));
-            H[(size_t)(j + 1) * m + j] = nrm;
-            for (int i = 0; i < j; ++i) {
-                const double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
-                H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
-                H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * bb;
-            }
-            const double a = H[(size_t)j * m + j], bb = H[(size_t)(j + 1) * m + j];
-            const double r = std::hypot(a, bb);
-            cs[j] = r > 0 ? a / r : 1.0;
-            sn[j] = r > 0 ? bb / r : 0.0;
-            H[(size_t)j * m + j] = r;
-            H[(size_t)(j + 1) * m + j] = 0.0;
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = cs[j] * g[j];
-            ++total;
-            jend = j + 1;
-            if (std::fabs(g[j + 1]) <= h->gm_tol * beta0) { conv = true; break; }
-            if (!(nrm > 0) || total >= h->gm_maxit) break;
-            HIPCHK(launch_scale(s, 1.0 / nrm, h->kw.p, V + (size_t)(j + 1) * n, n));
+            HIPCHK(precond(vj, h->kz.p, &pz, active));
+            HIPCHK(apply(pz, h->kw.p, active));
+            HIPCHK(launch_gs_pass(s, V, n, j + 1, nullptr, h->kw.p, n, true, part, nb, active));
+            HIPCHK(launch_gm_reduce(s, part, nb, 0, j + 1, G.h1, active));
+            HIPCHK(launch_gs_pass(s, V, n, j + 1, G.h1, h->kw.p, n, true, part, nb, active));
+            HIPCHK(launch_gm_reduce(s, part, nb, 0, j + 1, G.h2, active));
+            HIPCHK(launch_gs_pass(s, V, n, j + 1, G.h2, h->kw.p, n, false, part, nb, active));
+            HIPCHK(launch_gm_reduce(s, part, nb, j + 1, 1, G.hdr + 3, active));
+            HIPCHK(launch_gm_step(s, G, j));
+            if (j + 1 < m) HIPCHK(launch_scale_dev(s, G.hdr + 2, h->kw.p, V + (size_t)(j + 1) * n, n, active));
         }
-        // y = H^-1 g (upper triangular), x += M^-1 (V y)
-        for (int i = jend - 1; i >= 0; --i) {
-            double t = g[i];
-            for (int k = i + 1; k < jend; ++k) t -= H[(size_t)i * m + k] * y[k];
-            y[i] = t / H[(size_t)i * m + i];
-        }
-        HIPCHK(hipMemcpyAsync(hd, y.data(), jend * 8, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemsetAsync(h->ktmp.p, 0, n * 8, s));
-        HIPCHK(launch_lincomb(s, V, n, jend, hd, h->ktmp.p, n));
-        HIPCHK(precond(h->ktmp.p, h->kz.p, &pz));
+        // y = H^-1 g, x += M^-1 (V y)
+        HIPCHK(launch_gm_solve(s, G));
+        HIPCHK(launch_lincomb_dev(s, V, n, G.flags + 3, G.y, h->ktmp.p, n));
+        HIPCHK(precond(h->ktmp.p, h->kz.p, &pz, nullptr));
         HIPCHK(launch_axpy(s, 1.0, pz, x, n));
-        h->lin_iters += jend;
-        if (conv) { *converged = true; HIPCHK(hipStreamSynchronize(s)); return AFX_OK; }
-        if (total >= h->gm_maxit) { HIPCHK(hipStreamSynchronize(s)); return AFX_OK; }
-        // restart from the true residual
-        HIPCHK(apply(x, h->kw.p));
-        HIPCHK(launch_copy(s, b, h->kr.p, n));
-        HIPCHK(launch_axpy(s, -1.0, h->kw.p, h->kr.p, n));
-        if ((rc = dots(h, h->kr.p, n, 1, h->kr.p, n, hd))) return rc;
-        HIPCHK(hipMemcpyAsync(&beta, hd, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(flags, G.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        beta = std::sqrt(beta);
-        if (beta <= h->gm_tol * beta0) { *converged = true; return AFX_OK; }
+        h->lin_iters += flags[2] - total_before;
+        total_before = flags[2];
+        if (flags[1]) { *converged = true; return AFX_OK; }
+        if (flags[2] >= h->gm_maxit) return AFX_OK;
+        // restart from the true residual
+        HIPCHK(apply(x, h->kw.p, nullptr));
+        HIPCHK(launch_sub(s, b, h->kw.p, h->kr.p, n));
+        HIPCHK(begin(0));   // converged here: the next cycle's kernels do nothing
     }
 }
 
@@ -966,6 +1004,7 @@ int create_impl(const Mesh& mesh_in, const afx_gas* g, const afx_opts* o, const 
         HIPCHK_C(h->t_ng.alloc(tl.ng.size())); HIPCHK_C(h->t_ng.upload(tl.ng, s));
         HIPCHK_C(h->t_foff.alloc(tl.foff.size())); HIPCHK_C(h->t_foff.upload(tl.foff, s));
         HIPCHK_C(h->t_nft.alloc(tl.nft.size())); HIPCHK_C(h->t_nft.upload(tl.nft, s));
+        HIPCHK_C(h->t_nbf.alloc(tl.nbf.size())); HIPCHK_C(h->t_nbf.upload(tl.nbf, s));
         HIPCHK_C(h->t_goff.alloc(tl.goff.size())); HIPCHK_C(h->t_goff.upload(tl.goff, s));
         HIPCHK_C(h->t_Lglob.alloc(lg.size())); HIPCHK_C(h->t_Lglob.upload(lg, s));
         HIPCHK_C(h->t_slots.alloc(tl.slots.size())); HIPCHK_C(h->t_slots.upload(tl.slots, s));
@@ -1419,11 +1458,21 @@ int afx_rans_explicit_step(afx_rans* h, double relaxation, double* err) {
 int afx_rans_set_linear_solver(afx_rans* h, int32_t restart, double tol, int32_t maxit) {
     int rc = check_handle(h, false);
     if (rc) return rc;
-    if (restart < 1 || restart > 200 || !(tol > 0) || maxit < 1) return fail(AFX_EARG, "bad linear solver settings");
+    if (restart < 1 || restart > 32 || !(tol > 0) || maxit < 1)
+        return fail(AFX_EARG, "bad linear solver settings (restart 1..32, tol > 0, maxit >= 1)");
     if (h->Jf.p && restart != h->gm_restart) return fail(AFX_ESTATE, "restart length is fixed after the first fill");
     h->gm_restart = restart;
     h->gm_tol = tol;
     h->gm_maxit = maxit;
+    return AFX_OK;
+}
+
+int afx_rans_set_preconditioner(afx_rans* h, int32_t kind) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (kind != 0 && kind != 1) return fail(AFX_EARG, "preconditioner: 0 block Jacobi, 1 multicolour block DILU");
+    h->precond = kind;
+    h->lhs_ready = false;
     return AFX_OK;
 }
 
@@ -1449,6 +1498,9 @@ int afx_rans_implicit_fill(afx_rans* h) {
                                 h->opts.viscosity, h->Jf.p));
     HIPCHK(hipMemsetAsync(h->kbad.p, 0, sizeof(int), s));
     HIPCHK(launch_assemble_diag(s, h->cells64(), h->faces64(), h->dt.p, h->Jf.p, h->Dblk.p, h->Dinv.p, h->kbad.p));
+    if (h->precond == 1)
+        HIPCHK(launch_dilu_setup(s, h->cells64(), h->faces64(), h->colour.p, h->col_cells.p, h->col_off.data(),
+                                 (int)h->col_off.size() - 1, h->Dblk.p, h->Jf.p, h->Dti.p, h->kbad.p));
     HIPCHK(hipMemcpyAsync(&h->lhs_bad, h->kbad.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     h->lhs_ready = true;
@@ -1528,6 +1580,34 @@ int afx_rans_lhs_apply(afx_rans* h, const double* x, double* y) {
     if ((rc = upload_aos(h, x, h->ktmp.p))) return rc;
     HIPCHK(launch_lhs_apply(h->stream, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, h->ktmp.p, h->kw.p));
     return download_aos(h, h->kw.p, y);
+}
+
+int afx_rans_precond_apply(afx_rans* h, const double* x, double* y) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!x || !y) return fail(AFX_EARG, "null vector");
+    if (!h->lhs_ready) return fail(AFX_ESTATE, "call afx_rans_implicit_fill first");
+    if (h->mf) return fail(AFX_ESTATE, "the matrix-free mode has no preconditioner");
+    if ((rc = upload_aos(h, x, h->ktmp.p))) return rc;
+    if (h->precond == 1)
+        HIPCHK(launch_dilu_apply(h->stream, h->cells64(), h->faces64(), h->colour.p, h->col_cells.p,
+                                 h->col_off.data(), (int)h->col_off.size() - 1, h->Jf.p, h->Dti.p, h->ktmp.p, h->kw.p));
+    else
+        HIPCHK(launch_block_jacobi(h->stream, h->nC, h->nT, h->Dinv.p, h->ktmp.p, h->kw.p));
+    return download_aos(h, h->kw.p, y);
+}
+
+int afx_rans_cell_colours(afx_rans* h, int32_t* colours, int32_t* ncolours) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (!h->colour.p) return fail(AFX_ESTATE, "no colouring before the first afx_rans_implicit_fill");
+    std::vector<uint8_t> col(h->nC);
+    HIPCHK(hipMemcpyAsync(col.data(), h->colour.p, col.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (colours)
+        for (int32_t i = 0; i < h->nC; ++i) colours[h->c_new2old[i]] = col[i];
+    if (ncolours) *ncolours = (int32_t)h->col_off.size() - 1;
+    return AFX_OK;
 }
 
 int afx_rans_set_matrix_free(afx_rans* h, int32_t enable) {
@@ -1783,17 +1863,21 @@ int afx_rans_launch_stage(afx_rans* h, int32_t precision, int32_t stage, int32_t
 namespace {
 // laminar_ok: the state adjoint covers the laminar residual too; the adjoints
 // with respect to the mesh nodes and through pseudo-time cover the inviscid one
-int adjoint_check(afx_rans* h, bool laminar_ok = false) {
-    if (h->opts.viscosity == 2)
-        return fail(AFX_EUNSUP, "the adjoint covers the inviscid and laminar residuals, not the Spalart-Allmaras "
-                                "model (use afx_rans_tangent for its directional derivatives)");
-    if (h->opts.viscosity == 1 && !laminar_ok)
+// state_adjoint: the state adjoint (every model); otherwise the adjoints with
+// respect to the mesh nodes and through pseudo-time (inviscid only)
+int adjoint_check(afx_rans* h, bool state_adjoint = false) {
+    if (h->opts.viscosity != 0 && !state_adjoint)
         return fail(AFX_EUNSUP, "the adjoints with respect to the mesh nodes and through pseudo-time cover the "
                                 "inviscid residual");
     if (h->opts.gradient != 1) return fail(AFX_EUNSUP, "the state adjoint covers Green-Gauss gradients");
     if (h->michalak()) return fail(AFX_EUNSUP, "the state adjoint covers the default limiter, not the Michalak build");
+    if (h->sa() && !h->wd_ready) {
+        int rc = compute_wall_distance(h);
+        if (rc) return rc;
+    }
     if (!h->a_w.p) {
-        const size_t c4 = (size_t)4 * h->nC, t4 = (size_t)4 * h->nT, f4 = (size_t)4 * h->nF;
+        // planes: one per variable (five with the Spalart-Allmaras model)
+        const size_t c4 = (size_t)h->nv * h->nC, t4 = (size_t)h->nv * h->nT, f4 = (size_t)h->nv * h->nF;
         for (DBuf<double>* b : {&h->a_w, &h->a_lam, &h->a_lgx, &h->a_lgy}) HIPCHK(b->alloc(t4));
         for (DBuf<double>* b : {&h->a_bL, &h->a_bR, &h->a_co, &h->a_cn}) HIPCHK(b->alloc(f4));
         HIPCHK(h->a_nbv.alloc(c4));
@@ -1807,14 +1891,14 @@ int adjoint_check(afx_rans* h, bool laminar_ok = false) {
         if (!bf.empty()) { HIPCHK(h->a_bface.alloc(bf.size())); HIPCHK(h->a_bface.upload(bf, h->stream)); }
         HIPCHK(hipStreamSynchronize(h->stream));
     }
-    if (h->opts.viscosity == 1 && !h->a_vgx.p) {
-        const size_t f4 = (size_t)4 * h->nF;
+    if (h->opts.viscosity != 0 && !h->a_vgx.p) {
+        const size_t f4 = (size_t)h->nv * h->nF;
         for (DBuf<double>* b : {&h->a_vgx, &h->a_vgy, &h->a_vdq}) HIPCHK(b->alloc(f4));
     }
     return AFX_OK;
 }
 AdjBufs adj_bufs(const afx_rans* h) {
-    const bool lam = h->opts.viscosity == 1;
+    const bool lam = h->opts.viscosity != 0;
     return AdjBufs{h->gx.p, h->gy.p, h->lim.p, h->a_act.p, h->a_jmax.p, h->a_jmin.p, h->a_bL.p, h->a_bR.p,
                    h->a_lgx.p, h->a_lgy.p, h->a_nbv.p, h->a_nbj.p, h->a_co.p, h->a_cn.p,
                    h->a_bface.p, (int32_t)h->m.boundaryEdges.size(), lam ? h->a_vgx.p : nullptr,
@@ -1871,24 +1955,44 @@ int geo_check(afx_rans* h) {
 }
 int enqueue_adjoint(afx_rans* h) {
     const AdjBufs A = adj_bufs(h);
-    HIPCHK(launch_adjoint_state(h->stream, h->cells64(), h->faces64(), h->q.p, h->a_w.p, h->gas64(),
-                                h->opts.second_order, h->opts.limiter_k, A, h->a_lam.p));
+    if (h->sa())
+        HIPCHK(launch_adjoint_state_sa(h->stream, h->cells64(), h->faces64(), h->q.p, h->a_w.p, h->wall_d.p,
+                                       h->gas64(), h->opts.second_order, h->opts.limiter_k, A, h->a_lam.p));
+    else
+        HIPCHK(launch_adjoint_state(h->stream, h->cells64(), h->faces64(), h->q.p, h->a_w.p, h->gas64(),
+                                    h->opts.second_order, h->opts.limiter_k, A, h->a_lam.p));
     return AFX_OK;
 }
 }  // namespace
 
-int afx_rans_adjoint(afx_rans* h, const double* w, double* lambda) {
+int afx_rans_adjoint_nu(afx_rans* h, const double* w, const double* w_nu, double* lambda, double* lambda_nu) {
     int rc = check_handle(h);
     if (rc) return rc;
     if (!w || !lambda) return fail(AFX_EARG, "null vector");
+    if (!h->sa() && (w_nu || lambda_nu))
+        return fail(AFX_EARG, "rho nu~ seed / output only with the Spalart-Allmaras model");
     if ((rc = adjoint_check(h, true))) return rc;
     if ((rc = upload_aos(h, w, h->a_w.p))) return rc;
+    if (h->sa()) {
+        std::vector<double> tmp(h->nT, 0.0);
+        if (w_nu)
+            for (int32_t i = 0; i < h->nT; ++i) tmp[i] = w_nu[h->c_new2old[i]];
+        HIPCHK(hipMemcpyAsync(h->a_w.p + 4 * (size_t)h->nT, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice,
+                              h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
     // wall/slip ghosts as fillRhoRHS sees them (they enter no flux, but the
     // state is then the one the residual reads)
-    HIPCHK(launch_set_walls<double>(h->stream, h->w_owner.p, h->w_ghost.p, h->n_walls, h->q.p, h->nT, NV));
+    HIPCHK(launch_set_walls<double>(h->stream, h->w_owner.p, h->w_ghost.p, h->n_walls, h->q.p, h->nT, h->nv));
     if ((rc = enqueue_adjoint(h))) return rc;
     h->q32_valid = false;
-    return download_aos(h, h->a_lam.p, lambda);
+    if ((rc = download_aos(h, h->a_lam.p, lambda))) return rc;
+    if (lambda_nu && (rc = download_cells(h, h->a_lam.p + 4 * (size_t)h->nT, lambda_nu))) return rc;
+    return AFX_OK;
+}
+
+int afx_rans_adjoint(afx_rans* h, const double* w, double* lambda) {
+    return afx_rans_adjoint_nu(h, w, nullptr, lambda, nullptr);
 }
 
 int afx_rans_adjoint_nodes(afx_rans* h, const double* w, double* lambda, double* xbar) {
@@ -2215,6 +2319,8 @@ int tangent_prepare(afx_rans* h) {
         HIPCHK(launch_to_dual(s, h->xc.p, (int64_t)h->xc.n, h->tg_xc.p));
         HIPCHK(launch_to_dual(s, h->yc.p, (int64_t)h->yc.n, h->tg_yc.p));
         HIPCHK(hipMemsetAsync(h->tg_v.p, 0, (size_t)nv * nT * 8, s));
+        // the tile kernel writes the real rows only: ghost rows of J v stay 0
+        HIPCHK(hipMemsetAsync(h->tg_R.p, 0, (size_t)nv * nT * sizeof(Dual1), s));
     }
     HIPCHK(launch_to_dual(s, h->wall_d.p, (int64_t)h->wall_d.n, h->tg_wd.p));   // follows the BCs
     return AFX_OK;

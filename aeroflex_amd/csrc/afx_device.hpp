@@ -858,34 +858,42 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// Roe flux of physics.h:160-228 with shared reciprocals (FP32)
+// Roe flux of physics.h:160-228 (FP32), times 2 sc: sc = 1/2 gives the flux,
+// sc = len/2 the flux times the face length.  Reciprocals and roots share
+// reciprocal square roots: 1/rho = rsq(rho)^2, sqrt(rho) = rho rsq(rho),
+// c = c^2 rsq(c^2), 1/c^2 = rsq(c^2)^2, 1/(2 d) = 10 rsq(c^2) (d = c/20).
 __device__ __forceinline__ void roe_flux_f32(float gamma, float nx, float ny, const float* qL, const float* qR,
-                                             float* f) {
+                                             float* f, float sc = 0.5f) {
     const float gm1 = gamma - 1.f;
     const float rL = qL[0], ruL = qL[1], rvL = qL[2], reL = qL[3];
     const float rR = qR[0], ruR = qR[1], rvR = qR[2], reR = qR[3];
-    const float irL = frcp(rL), irR = frcp(rR);
+    const float rsL = __builtin_amdgcn_rsqf(rL), rsR = __builtin_amdgcn_rsqf(rR);
+    const float irL = rsL * rsL, irR = rsR * rsR;
     const float uL = ruL * irL, vL = rvL * irL, uR = ruR * irR, vR = rvR * irR;
     const float VL = uL * nx + vL * ny, VR = uR * nx + vR * ny;
     const float pL = gm1 * (reL - 0.5f * (ruL * uL + rvL * vL));
     const float pR = gm1 * (reR - 0.5f * (ruR * uR + rvR * vR));
-    float f0 = 0.5f * (VL * rL + VR * rR);
-    float f1 = 0.5f * (VL * ruL + VR * ruR + (pL + pR) * nx);
-    float f2 = 0.5f * (VL * rvL + VR * rvR + (pL + pR) * ny);
-    float f3 = 0.5f * (VL * (reL + pL) + VR * (reR + pR));
-    const float srL = __builtin_amdgcn_sqrtf(rL), srR = __builtin_amdgcn_sqrtf(rR);
+    const float hL = reL + pL, hR = reR + pR;   // rho H per side
+    // central part (twice the average of the two sides' fluxes)
+    const float ps = pL + pR;
+    float f0 = VL * rL + VR * rR;
+    float f1 = VL * ruL + VR * ruR + ps * nx;
+    float f2 = VL * rvL + VR * rvR + ps * ny;
+    float f3 = VL * hL + VR * hR;
+    const float srL = rL * rsL, srR = rR * rsR;
     const float is = frcp(srL + srR);
     const float wL = srL * is, wR = srR * is;
     const float rho = srL * srR;
     const float u = uL * wL + uR * wR;
     const float v = vL * wL + vR * wR;
-    const float h = (reL + pL) * irL * wL + (reR + pR) * irR * wR;
+    const float h = (hL * irL) * wL + (hR * irR) * wR;
     const float q2 = u * u + v * v;
     const float c2 = gm1 * (h - 0.5f * q2);
-    const float c = __builtin_amdgcn_sqrtf(c2);
+    const float rc = __builtin_amdgcn_rsqf(c2);
+    const float c = c2 * rc, ic2 = rc * rc;
     const float V = u * nx + v * ny;
     const float d = 0.05f * c;
-    const float i2d = frcp(2.f * d), dd = d * d;
+    const float i2d = 10.f * rc, dd = d * d;
     float lcm = __builtin_amdgcn_sqrtf((V - c) * (V - c) + 1e-4f);
     float lc = __builtin_amdgcn_sqrtf(V * V + 1e-4f);
     float lcp = __builtin_amdgcn_sqrtf((V + c) * (V + c) + 1e-4f);
@@ -893,18 +901,19 @@ __device__ __forceinline__ void roe_flux_f32(float gamma, float nx, float ny, co
     lc = lc > d ? lc : (lc * lc + dd) * i2d;
     lcp = lcp > d ? lcp : (lcp * lcp + dd) * i2d;
     const float dp = pR - pL, dV = VR - VL;
-    const float ic2 = frcp(c2);
     const float rcdV = rho * c * dV;
-    const float kF1 = lcm * (dp - rcdV) * (0.5f * ic2);
-    const float kF5 = lcp * (dp + rcdV) * (0.5f * ic2);
+    const float hic2 = 0.5f * ic2;
+    const float kF1 = lcm * (dp - rcdV) * hic2;
+    const float kF5 = lcp * (dp + rcdV) * hic2;
     const float k0 = lc * ((rR - rL) - dp * ic2);
     const float k1 = lc * rho;
     const float du = uR - uL, dv = vR - vL;
-    f0 -= 0.5f * (kF1 + k0 + kF5);
-    f1 -= 0.5f * (kF1 * (u - c * nx) + k0 * u + k1 * (du - dV * nx) + kF5 * (u + c * nx));
-    f2 -= 0.5f * (kF1 * (v - c * ny) + k0 * v + k1 * (dv - dV * ny) + kF5 * (v + c * ny));
-    f3 -= 0.5f * (kF1 * (h - c * V) + k0 * q2 * 0.5f + k1 * (u * du + v * dv - V * dV) + kF5 * (h + c * V));
-    f[0] = f0; f[1] = f1; f[2] = f2; f[3] = f3;
+    const float cnx = c * nx, cny = c * ny, cV = c * V;
+    f0 -= kF1 + k0 + kF5;
+    f1 -= kF1 * (u - cnx) + k0 * u + k1 * (du - dV * nx) + kF5 * (u + cnx);
+    f2 -= kF1 * (v - cny) + k0 * v + k1 * (dv - dV * ny) + kF5 * (v + cny);
+    f3 -= kF1 * (h - cV) + k0 * q2 * 0.5f + k1 * (u * du + v * dv - V * dV) + kF5 * (h + cV);
+    f[0] = sc * f0; f[1] = sc * f1; f[2] = sc * f2; f[3] = sc * f3;
 }
 
 // physics.h:230-257 in FP32 with one reciprocal of the face density: the
@@ -966,34 +975,44 @@ __device__ __forceinline__ float sa_source_f32(float rho, float nut, float imu, 
     return rho * (nut < 0.f ? neg : pos);
 }
 
-// five-variable face flux of sa_face_flux with the FP32 Roe kernel above
-__device__ __forceinline__ void sa_face_flux_f32(const GasT<float>& g, int kind, float nx, float ny, const float* qL,
-                                                 const float* qR, const float* gx, const float* gy, float nut_c,
-                                                 float dn, float* f) {
+// five-variable face flux of sa_face_flux with the FP32 Roe kernel above,
+// times the face length L.  gx2, gy2: twice the face gradients (interior
+// faces; the halves are taken into the viscosities).
+__device__ __forceinline__ void sa_face_flux_f32(const GasT<float>& g, int kind, float nx, float ny, float L,
+                                                 const float* qL, const float* qR, const float* gx2, const float* gy2,
+                                                 float nut_c, float dn, float* f) {
     if (kind != K_INTERNAL) {
         float qb[4];
         face_vars<float>(g, kind, nx, ny, qL, qR, qb);
         const float nb = sa_bnd_nu<float>(kind, nx, ny, qL, qL[4], qR, qR[4], qb);
-        roe_flux_f32(g.gamma, nx, ny, qL, qb, f);
-        f[4] = f[0] * (f[0] >= 0.f ? qL[4] * frcp(qL[0]) : nb * frcp(qb[0]));
-        if (kind == K_WALL) f[4] += g.mu * float(1.0 / sa::sigma) * nut_c * frcp(dn);
+        roe_flux_f32(g.gamma, nx, ny, qL, qb, f, 0.5f * L);
+        const bool up = f[0] >= 0.f;
+        f[4] = f[0] * (up ? qL[4] : nb) * frcp(up ? qL[0] : qb[0]);
+        if (kind == K_WALL) f[4] += L * g.mu * float(1.0 / sa::sigma) * nut_c * frcp(dn);
         return;
     }
-    roe_flux_f32(g.gamma, nx, ny, qL, qR, f);
-    f[4] = f[0] * (f[0] >= 0.f ? qL[4] * frcp(qL[0]) : qR[4] * frcp(qR[0]));
+    roe_flux_f32(g.gamma, nx, ny, qL, qR, f, 0.5f * L);
+    const bool up = f[0] >= 0.f;
+    f[4] = f[0] * (up ? qL[4] : qR[4]) * frcp(up ? qL[0] : qR[0]);
     const float nc = 0.5f * (qL[4] + qR[4]);
     const float chi = nc * frcp(g.mu);
     const float c3 = chi * chi * chi;
-    const float mut = chi > 0.f ? nc * c3 * frcp(c3 + float(sa::cv1_3)) : 0.f;
-    const float fn = chi >= 0.f ? 1.f : (float(sa::cn1) + c3) * frcp(float(sa::cn1) - c3);
+    // mu_t = nc chi^3 / (chi^3 + cv1^3) for chi > 0, f_n = (cn1 + chi^3) /
+    // (cn1 - chi^3) for chi < 0: one reciprocal serves the side taken
+    const bool pos = chi > 0.f;
+    const float rden = frcp(pos ? c3 + float(sa::cv1_3) : float(sa::cn1) - c3);
+    const float mut = pos ? nc * c3 * rden : 0.f;
+    const float fn = chi >= 0.f ? 1.f : (float(sa::cn1) + c3) * rden;
     const float kc = g.k + mut * (g.cp * frcp(g.PrT));
+    const float nxl = nx * L, nyl = ny * L;
     float f1 = f[1], f2 = f[2], f3 = f[3];
-    viscous_terms_f32(g.gamma - 1.f, frcp(g.R), nx, ny, qL, qR, gx, gy, g.mu + mut, kc, f1, f2, f3);
+    viscous_terms_f32(g.gamma - 1.f, frcp(g.R), nxl, nyl, qL, qR, gx2, gy2, 0.5f * (g.mu + mut), 0.5f * kc, f1, f2,
+                      f3);
     f[1] = f1; f[2] = f2; f[3] = f3;
     const float iqc0 = frcp(0.5f * (qL[0] + qR[0]));
     const float nut = nc * iqc0;
-    const float gvx = (gx[4] - nut * gx[0]) * iqc0, gvy = (gy[4] - nut * gy[0]) * iqc0;
-    f[4] -= (g.mu + nc * fn) * float(1.0 / sa::sigma) * (nx * gvx + ny * gvy);
+    const float gvx = (gx2[4] - nut * gx2[0]) * iqc0, gvy = (gy2[4] - nut * gy2[0]) * iqc0;
+    f[4] -= (g.mu + nc * fn) * float(0.5 / sa::sigma) * (nxl * gvx + nyl * gvy);
 }
 
 // sa_cell_source in FP32
@@ -1040,8 +1059,11 @@ __device__ __forceinline__ float limiter_extremes(float ymax, float ymin, float 
 
 // one tile of the FP32 residual (the body of both the single-condition and
 // the batched kernels).  MODEL: 0 inviscid, 1 laminar, 2 Spalart-Allmaras;
-// GG: Green-Gauss (else least squares) gradients; SO: second order.
-template <int MODEL, bool STAGE, bool GG, bool SO>
+// GG: Green-Gauss (else least squares) gradients; SO: second order; BND: the
+// tile's face list holds boundary faces (without, every face is taken as
+// interior at compile time: no kind tests, no boundary states, no ghost
+// writes).
+template <int MODEL, bool STAGE, bool GG, bool SO, bool BND>
 __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const int t, float4* smem4) {
     constexpr bool SA = MODEL == 2;
     const TilesF32& S = A.S;
@@ -1101,8 +1123,7 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
         float4 sx, sy;
         const float nuc = SA ? sqn[c] : 0.f;
         float sxn = 0.f, syn = 0.f, mnn = nuc, mxn = nuc;
-        float dxs[4], dys[4];
-        int ns = 0;
+        float dxs[4] = {0.f, 0.f, 0.f, 0.f}, dys[4] = {0.f, 0.f, 0.f, 0.f};   // 0 past the last slot
         if constexpr (GG) {
             // Green-Gauss (solver.h:428-469): face value a q_c + b q_other
             // (interior: the weights 1 - g/2 and g/2 of physics.h:275, the
@@ -1119,7 +1140,7 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
                 const int f = (int)(e & 0x7FFFu);
                 const bool nb = (e >> 15) != 0;
                 const uint32_t fc = fck[f];
-                const int kind = (int)(fc >> 24);
+                const int kind = BND ? (int)(fc >> 24) : (int)K_INTERNAL;
                 const int other = nb ? (int)(fc & 0xFFFu) : (int)((fc >> 12) & 0xFFFu);
                 const float4 ga = geo[2 * f];   // (nx, ny, len, g)
                 if constexpr (SO) {
@@ -1127,7 +1148,6 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
                     dxs[s] = nb ? gb.z : gb.x;
                     dys[s] = nb ? gb.w : gb.y;
                 }
-                ns = s + 1;
                 const float sl = (nb ? -ga.z : ga.z) * ia;
                 const float nlx = ga.x * sl, nly = ga.y * sl;
                 float wa, wb, nuo = 0.f;
@@ -1192,13 +1212,12 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
                 const int f = (int)(e & 0x7FFFu);
                 const bool nb = (e >> 15) != 0;
                 const uint32_t fc = fck[f];
-                const int kind = (int)(fc >> 24);
+                const int kind = BND ? (int)(fc >> 24) : (int)K_INTERNAL;
                 const int other = nb ? (int)(fc & 0xFFFu) : (int)((fc >> 12) & 0xFFFu);
                 const float4 ga = geo[2 * f], gb = geo[2 * f + 1];
                 const float dcx = nb ? gb.z : gb.x, dcy = nb ? gb.w : gb.y;
                 const float dox = nb ? gb.x : gb.z, doy = nb ? gb.y : gb.w;
                 dxs[s] = dcx; dys[s] = dcy;
-                ns = s + 1;
                 float qb[4] = {0.f, 0.f, 0.f, 0.f}, qv[4];
                 const bool walls = (kind == K_WALL || kind == K_SLIP);
                 if (!walls) { const float4 t4 = sq[other]; qb[0] = t4.x; qb[1] = t4.y; qb[2] = t4.z; qb[3] = t4.w; }
@@ -1238,7 +1257,7 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
             const float K3a = K3 * a15[gc];
             float4 lm;
 #define LIM(m) { float ymax = 0.f, ymin = 0.f;                                         \
-                 _Pragma("unroll") for (int s = 0; s < 4; ++s) if (s < ns) {           \
+                 _Pragma("unroll") for (int s = 0; s < 4; ++s) {                       \
                      const float y = fmaf(sx.m, dxs[s], sy.m * dys[s]);                \
                      ymax = fmaxf(ymax, y); ymin = fminf(ymin, y); }                   \
                  lm.m = limiter_extremes(ymax, ymin, qmax.m - qc.m, qmin.m - qc.m, K3a); }
@@ -1248,12 +1267,11 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
             if constexpr (SA) {
                 float ymax = 0.f, ymin = 0.f;
 #pragma unroll
-                for (int s = 0; s < 4; ++s)
-                    if (s < ns) {
-                        const float y = fmaf(sxn, dxs[s], syn * dys[s]);
-                        ymax = fmaxf(ymax, y);
-                        ymin = fminf(ymin, y);
-                    }
+                for (int s = 0; s < 4; ++s) {
+                    const float y = fmaf(sxn, dxs[s], syn * dys[s]);
+                    ymax = fmaxf(ymax, y);
+                    ymin = fminf(ymin, y);
+                }
                 slmn[c] = limiter_extremes(ymax, ymin, mxn - nuc, mnn - nuc, K3a);
             }
         }
@@ -1264,7 +1282,7 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
     for (int fl = threadIdx.x; fl < nFT; fl += blockDim.x) {
         const uint32_t fc = fck[fl];
         const int o = (int)(fc & 0xFFFu), n = (int)((fc >> 12) & 0xFFFu);
-        const int kind = (int)(fc >> 24);
+        const int kind = BND ? (int)(fc >> 24) : (int)K_INTERNAL;
         const float4 ga = geo[2 * fl], gb = geo[2 * fl + 1];
         float4 qL = sq[o];
         float4 qR = (kind == K_INTERNAL || kind == K_FARFIELD) ? sq[n] : f4(0.f, 0.f, 0.f, 0.f);
@@ -1298,7 +1316,7 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
 #define RECL(m) qL.m = fmaf(fmaf(ax.m, gb.x, ay.m * gb.y), al.m, qL.m);
             F4_LOOP(RECL)
 #undef RECL
-            if (n < nG) {
+            if (!BND || n < nG) {
                 const float4 bx = sgx[n], by = sgy[n], bl = slm[n];
 #define RECR(m) qR.m = fmaf(fmaf(bx.m, gb.z, by.m * gb.w), bl.m, qR.m);
                 F4_LOOP(RECR)
@@ -1313,18 +1331,21 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
             if constexpr (SO) {
                 const float2 ga_ = sgn[o];
                 nL = fmaf(fmaf(ga_.x, gb.x, ga_.y * gb.y), slmn[o], nL);
-                if (n < nG) { const float2 gn2 = sgn[n]; nR = fmaf(fmaf(gn2.x, gb.z, gn2.y * gb.w), slmn[n], nR); }
+                if (!BND || n < nG) { const float2 gn2 = sgn[n]; nR = fmaf(fmaf(gn2.x, gb.z, gn2.y * gb.w), slmn[n], nR); }
             }
             const float qL5[5] = {qL.x, qL.y, qL.z, qL.w, nL};
             const float qR5[5] = {qR.x, qR.y, qR.z, qR.w, nR};
             float gxa[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, gya[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
             const float4 so_ = sq[o];
             if (kind == K_INTERNAL) {
-                // face gradients along the cell connection (textbook
-                // direction): x_n - x_o = d_0 - d_1 from the face offsets
+                // twice the face gradients along the cell connection
+                // (textbook direction): x_n - x_o = d_0 - d_1 from the face
+                // offsets; 2 g = S - (S . t - 2 dq/l) t with S the sum of
+                // the two cells' gradients
                 float t0 = gb.x - gb.z, t1 = gb.y - gb.w;
                 const float il = __builtin_amdgcn_rsqf(fmaf(t0, t0, t1 * t1));
                 t0 *= il; t1 *= il;
+                const float il2 = 2.f * il;
                 const float4 sn = sq[n], xo = sgx[o], xn = sgx[n], yo = sgy[o], yn = sgy[n];
                 const float2 go2 = sgn[o], gn2 = sgn[n];
                 const float so5[5] = {so_.x, so_.y, so_.z, so_.w, nuo_c}, sn5[5] = {sn.x, sn.y, sn.z, sn.w, sqn[n]};
@@ -1332,51 +1353,52 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
                 const float yo5[5] = {yo.x, yo.y, yo.z, yo.w, go2.y}, yn5[5] = {yn.x, yn.y, yn.z, yn.w, gn2.y};
 #pragma unroll
                 for (int k = 0; k < 5; ++k) {
-                    const float dir = (sn5[k] - so5[k]) * il;
-                    const float gxb = 0.5f * (xo5[k] + xn5[k]);
-                    const float gyb = 0.5f * (yo5[k] + yn5[k]);
-                    const float corr = fmaf(gxb, t0, gyb * t1) - dir;
-                    gxa[k] = fmaf(-corr, t0, gxb);
-                    gya[k] = fmaf(-corr, t1, gyb);
+                    const float dir2 = (sn5[k] - so5[k]) * il2;
+                    const float Sx = xo5[k] + xn5[k], Sy = yo5[k] + yn5[k];
+                    const float corr2 = fmaf(Sx, t0, fmaf(Sy, t1, -dir2));
+                    gxa[k] = fmaf(-corr2, t0, Sx);
+                    gya[k] = fmaf(-corr2, t1, Sy);
                 }
             }
-            sa_face_flux_f32(g, kind, ga.x, ga.y, qL5, qR5, gxa, gya, nuo_c * frcp(so_.x),
+            sa_face_flux_f32(g, kind, ga.x, ga.y, ga.z, qL5, qR5, gxa, gya, nuo_c * frcp(so_.x),
                              gb.x * ga.x + gb.y * ga.y, F);
-            sfl[fl] = f4(F[0] * ga.z, F[1] * ga.z, F[2] * ga.z, F[3] * ga.z);
-            sfln[fl] = F[4] * ga.z;
+            sfl[fl] = f4(F[0], F[1], F[2], F[3]);
+            sfln[fl] = F[4];
             continue;
         }
         const float qLa[4] = {qL.x, qL.y, qL.z, qL.w};
         float qRa[4] = {qR.x, qR.y, qR.z, qR.w};
         if (kind == K_INTERNAL) {
             if constexpr (MODEL == 1) {
-                // averaged face gradient with the reference's direction (A1:
-                // the normalised sum of the two centroids, solver.h:359-398)
+                // twice the averaged face gradient with the reference's
+                // direction (A1: the normalised sum of the two centroids,
+                // solver.h:359-398)
                 float gxa[4], gya[4];
                 const int go_ = Lg[o], gn_ = Lg[n];
                 float t0 = xc[go_] + xc[gn_], t1 = yc[go_] + yc[gn_];
-                const float il = frcp(__builtin_amdgcn_sqrtf(t0 * t0 + t1 * t1));
+                const float il = __builtin_amdgcn_rsqf(t0 * t0 + t1 * t1);
                 t0 *= il; t1 *= il;
+                const float il2 = 2.f * il;
                 const float4 so = sq[o], sn = sq[n], xo = sgx[o], xn = sgx[n], yo = sgy[o], yn = sgy[n];
                 for (int k = 0; k < 4; ++k) {
-                    const float dir = (get(so, k) - get(sn, k)) * il;
-                    const float gxb = 0.5f * (get(xo, k) + get(xn, k));
-                    const float gyb = 0.5f * (get(yo, k) + get(yn, k));
-                    const float dot = gxb * t0 + gyb * t1;
-                    gxa[k] = gxb - (dot - dir) * t0;
-                    gya[k] = gyb - (dot - dir) * t1;
+                    const float dir2 = (get(so, k) - get(sn, k)) * il2;
+                    const float Sx = get(xo, k) + get(xn, k), Sy = get(yo, k) + get(yn, k);
+                    const float corr2 = fmaf(Sx, t0, fmaf(Sy, t1, -dir2));
+                    gxa[k] = fmaf(-corr2, t0, Sx);
+                    gya[k] = fmaf(-corr2, t1, Sy);
                 }
-                roe_flux_f32(g.gamma, ga.x, ga.y, qLa, qRa, F);
-                viscous_terms_f32(g.gamma - 1.f, frcp(g.R), ga.x, ga.y, qLa, qRa, gxa, gya, g.mu, g.k, F[1], F[2], F[3]);
+                roe_flux_f32(g.gamma, ga.x, ga.y, qLa, qRa, F, 0.5f * ga.z);
+                viscous_terms_f32(g.gamma - 1.f, frcp(g.R), ga.x * ga.z, ga.y * ga.z, qLa, qRa, gxa, gya, 0.5f * g.mu,
+                                  0.5f * g.k, F[1], F[2], F[3]);
             } else {
-                roe_flux_f32(g.gamma, ga.x, ga.y, qLa, qRa, F);
+                roe_flux_f32(g.gamma, ga.x, ga.y, qLa, qRa, F, 0.5f * ga.z);
             }
         } else {
             float qb[4];
             face_vars<float>(g, kind, ga.x, ga.y, qLa, qRa, qb);
-            roe_flux_f32(g.gamma, ga.x, ga.y, qLa, qb, F);
+            roe_flux_f32(g.gamma, ga.x, ga.y, qLa, qb, F, 0.5f * ga.z);
         }
-        sfl[fl] = f4(F[0] * ga.z, F[1] * ga.z, F[2] * ga.z, F[3] * ga.z);
+        sfl[fl] = f4(F[0], F[1], F[2], F[3]);
     }
     __syncthreads();
 
@@ -1401,7 +1423,7 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
             a2 += (double)(nb ? v.z : -v.z);
             a3 += (double)(nb ? v.w : -v.w);
             if constexpr (SA) { const float vn = sfln[fl]; a4 += (double)(nb ? vn : -vn); }
-            if (qghost) {
+            if (BND && qghost) {
                 const uint32_t fc = fck[fl];
                 const int kind = (int)(fc >> 24);
                 if (kind == K_WALL || kind == K_SLIP) {
@@ -1455,7 +1477,8 @@ k_residual_tile_f32(TileArgsF32 A) {
     extern __shared__ float4 smem4[];
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= A.S.ntiles) return;
-    tile_residual_f32<MODEL, false, GG, SO>(A, t, smem4);
+    if (A.S.nbf[t]) tile_residual_f32<MODEL, false, GG, SO, true>(A, t, smem4);
+    else tile_residual_f32<MODEL, false, GG, SO, false>(A, t, smem4);
 }
 
 // batched explicit stage: residual + stage update of every (member, tile)
@@ -1465,7 +1488,9 @@ k_residual_tile_f32_batch(const TileArgsF32* __restrict__ members, const int2* _
     extern __shared__ float4 smem4[];
     const int2 it = items[xcd_tile(blockIdx.x, gridDim.x)];
     if (it.x < 0) return;
-    tile_residual_f32<MODEL, true, GG, SO>(members[it.x], it.y, smem4);
+    const TileArgsF32& M = members[it.x];
+    if (M.S.nbf[it.y]) tile_residual_f32<MODEL, true, GG, SO, true>(M, it.y, smem4);
+    else tile_residual_f32<MODEL, true, GG, SO, false>(M, it.y, smem4);
 }
 
 // the (model, gradient scheme, order) instantiation for the run-time options

@@ -1,8 +1,10 @@
 // Implicit pseudo-time step on the device (FP64): the finite-difference flux
 // Jacobian of implicitSolver::fillRhoLHS (solver.h:979-1075,
-// physics.h:533-578) in 4x4 blocks, its block-Jacobi preconditioner, the
-// operator product and the Krylov vector kernels used by the restarted
-// GMRES driver in afx_capi.hip.
+// physics.h:533-578) in 4x4 blocks, its block-Jacobi and multicolour block
+// DILU preconditioners, the operator product and the kernels of the
+// device-resident restarted GMRES driven from afx_capi.hip (the Arnoldi
+// coefficients, Givens rotations and convergence flags live in device
+// memory; the host reads the flags once per restart cycle).
 //
 // Layout: vectors are SoA [k*nT + i] over real and ghost cells (ghost rows of
 // the reference's matrix are identity rows with zero right-hand side, so their
@@ -127,7 +129,8 @@ __global__ void __launch_bounds__(kBlock) k_assemble_diag(CellsT<double> C, Face
 // y = LHS x (ghost rows identity; ghost columns dropped since x_ghost = 0)
 __global__ void __launch_bounds__(kBlock) k_lhs_apply(CellsT<double> C, FacesT<double> F, const double* __restrict__ D,
                                                       const double* __restrict__ Jf, const double* __restrict__ x,
-                                                      double* __restrict__ y) {
+                                                      double* __restrict__ y, const int* __restrict__ active) {
+    if (active && *active == 0) return;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C.nT) return;
     const size_t nT = (size_t)C.nT;
@@ -163,7 +166,9 @@ __global__ void __launch_bounds__(kBlock) k_lhs_apply(CellsT<double> C, FacesT<d
 
 // z = Dinv x (ghost rows identity)
 __global__ void __launch_bounds__(kBlock) k_block_jacobi(int32_t nC, int32_t nT, const double* __restrict__ Dinv,
-                                                         const double* __restrict__ x, double* __restrict__ z) {
+                                                         const double* __restrict__ x, double* __restrict__ z,
+                                                         const int* __restrict__ active) {
+    if (active && *active == 0) return;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nT) return;
     const size_t n = (size_t)nT;
@@ -177,75 +182,271 @@ __global__ void __launch_bounds__(kBlock) k_block_jacobi(int32_t nC, int32_t nT,
     for (int r = 0; r < 4; ++r) z[r * n + c] = B[4 * r] * xc[0] + B[4 * r + 1] * xc[1] + B[4 * r + 2] * xc[2] + B[4 * r + 3] * xc[3];
 }
 
-// partial[j*nblk + b] = sum over block b of V_j . w, j < m (V_j contiguous, stride ld)
-__global__ void __launch_bounds__(kBlock) k_multidot(const double* __restrict__ V, int64_t ld, int m, const double* __restrict__ w,
-                                                     int64_t n, double* __restrict__ partial) {
-    __shared__ double red[kBlock / 64];
-    for (int j = 0; j < m; ++j) {
-        const double* v = V + (size_t)j * ld;
-        double s = 0.0;
-        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-            s += v[i] * w[i];
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int k = 0; k < kBlock / 64; ++k) t += red[k];
-            partial[(size_t)j * gridDim.x + blockIdx.x] = t;
-        }
-        __syncthreads();
-    }
-}
-
-// out[j] = sum_b partial[j*nblk + b]
-__global__ void k_multidot_final(const double* __restrict__ partial, int nblk, int m, double* __restrict__ out) {
-    __shared__ double red[kBlock / 64];
-    for (int j = 0; j < m; ++j) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += partial[(size_t)j * nblk + i];
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int k = 0; k < kBlock / 64; ++k) t += red[k];
-            out[j] = t;
-        }
-        __syncthreads();
-    }
-}
-
-// w -= sum_j h[j] V_j   (h on the device)
-__global__ void __launch_bounds__(kBlock) k_multiaxpy(const double* __restrict__ V, int64_t ld, int m,
-                                                      const double* __restrict__ h, double* __restrict__ w, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double acc = w[i];
-    for (int j = 0; j < m; ++j) acc -= h[j] * V[(size_t)j * ld + i];
-    w[i] = acc;
-}
-
-// x += sum_j y[j] V_j   (y on the device)
-__global__ void __launch_bounds__(kBlock) k_lincomb(const double* __restrict__ V, int64_t ld, int m,
-                                                    const double* __restrict__ y, double* __restrict__ x, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double acc = x[i];
-    for (int j = 0; j < m; ++j) acc += y[j] * V[(size_t)j * ld + i];
-    x[i] = acc;
-}
-
-// y = a * x
-__global__ void __launch_bounds__(kBlock) k_scale(double a, const double* __restrict__ x, double* __restrict__ y, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = a * x[i];
-}
-
 // y += a * x
 __global__ void __launch_bounds__(kBlock) k_axpy(double a, const double* __restrict__ x, double* __restrict__ y, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] += a * x[i];
+}
+
+// ------------------------------------------- multicolour block DILU (f1) --
+// The reference preconditions GMRES with an incomplete LU (IncompleteLUT,
+// solver.h:868-910).  On the device: block DILU in a colour order of the
+// real cells (no two face neighbours share a colour), so each colour is one
+// parallel sweep.  With A_ij the off-diagonal block of row i, column j:
+//   Dt_i = D_i - sum_{j before i} A_ij Dt_j^-1 A_ji          (set-up, by colour)
+//   M = (Dt + L) Dt^-1 (Dt + U),  L / U: the blocks A_ij with j before / after i.
+// Off-diagonal blocks from the face Jacobians: for face f with owner o and
+// neighbour n, A_on = J01(f) and A_no = -J00(f); boundary faces have none.
+
+// the block A_ij (sgn, J) of cell i across its slot entry e, and the other cell
+__device__ __forceinline__ const double* offdiag(const FacesT<double>& F, const double* Jf, uint32_t e, int* other,
+                                                  double* sgn) {
+    const int f = (int)(e & 0x7FFFFFFFu);
+    const double* J = Jf + (size_t)32 * f;
+    if (e >> 31) { *other = F.c0[f]; *sgn = -1.0; return J; }        // i is the neighbour: -J00
+    *other = F.c1[f]; *sgn = 1.0; return J + 16;                       // i is the owner: J01
+}
+// the transposed-position block A_ji of the same face
+__device__ __forceinline__ const double* offdiag_t(const double* Jf, uint32_t e, double* sgn) {
+    const int f = (int)(e & 0x7FFFFFFFu);
+    const double* J = Jf + (size_t)32 * f;
+    if (e >> 31) { *sgn = 1.0; return J + 16; }
+    *sgn = -1.0; return J;
+}
+
+__global__ void __launch_bounds__(kBlock) k_dilu_setup(CellsT<double> C, FacesT<double> F, const uint8_t* __restrict__ colour,
+                                                       int c, const int32_t* __restrict__ cells, int32_t ncells,
+                                                       const double* __restrict__ D, const double* __restrict__ Jf,
+                                                       double* __restrict__ Dti, int* __restrict__ bad) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncells) return;
+    const int i = cells[k];
+    double B[16];
+    for (int r = 0; r < 16; ++r) B[r] = D[(size_t)16 * i + r];
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t e = C.cface[(size_t)s * C.nC + i];
+        if (e == 0xFFFFFFFFu) break;
+        if (F.kind[e & 0x7FFFFFFFu] != K_INTERNAL) continue;
+        int j;
+        double sa, sb;
+        const double* Aij = offdiag(F, Jf, e, &j, &sa);
+        if (colour[j] >= c) continue;
+        const double* Aji = offdiag_t(Jf, e, &sb);
+        const double* Pj = Dti + (size_t)16 * j;
+        double T[16];   // Dt_j^-1 A_ji
+        for (int r = 0; r < 4; ++r)
+            for (int q = 0; q < 4; ++q)
+                T[4 * r + q] = sb * (Pj[4 * r] * Aji[q] + Pj[4 * r + 1] * Aji[4 + q] + Pj[4 * r + 2] * Aji[8 + q] +
+                                     Pj[4 * r + 3] * Aji[12 + q]);
+        for (int r = 0; r < 4; ++r)
+            for (int q = 0; q < 4; ++q)
+                B[4 * r + q] -= sa * (Aij[4 * r] * T[q] + Aij[4 * r + 1] * T[4 + q] + Aij[4 * r + 2] * T[8 + q] +
+                                      Aij[4 * r + 3] * T[12 + q]);
+    }
+    double X[16];
+    if (!inv4(B, X)) {
+        atomicAdd(bad, 1);
+        for (int r = 0; r < 16; ++r) X[r] = 0.0;
+    }
+    for (int r = 0; r < 16; ++r) Dti[(size_t)16 * i + r] = X[r];
+}
+
+// forward sweep of colour c: y_i = Dt_i^-1 (b_i - sum_{j before i} A_ij y_j)
+// backward sweep of colour c (in place): y_i -= Dt_i^-1 sum_{j after i} A_ij y_j
+template <bool FORWARD>
+__global__ void __launch_bounds__(kBlock) k_dilu_sweep(CellsT<double> C, FacesT<double> F, const uint8_t* __restrict__ colour,
+                                                       int c, const int32_t* __restrict__ cells, int32_t ncells,
+                                                       const double* __restrict__ Jf, const double* __restrict__ Dti,
+                                                       const double* __restrict__ b, double* __restrict__ y,
+                                                       const int* __restrict__ active) {
+    if (active && *active == 0) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncells) return;
+    const size_t nT = (size_t)C.nT;
+    const int i = cells[k];
+    double acc[4];
+    for (int r = 0; r < 4; ++r) acc[r] = FORWARD ? b[r * nT + i] : 0.0;
+    bool any = FORWARD;
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t e = C.cface[(size_t)s * C.nC + i];
+        if (e == 0xFFFFFFFFu) break;
+        if (F.kind[e & 0x7FFFFFFFu] != K_INTERNAL) continue;
+        int j;
+        double sa;
+        const double* A = offdiag(F, Jf, e, &j, &sa);
+        if (FORWARD ? colour[j] >= c : colour[j] <= c) continue;
+        any = true;
+        double yj[4];
+        for (int r = 0; r < 4; ++r) yj[r] = sa * y[r * nT + j];
+        for (int r = 0; r < 4; ++r)
+            acc[r] += (FORWARD ? -1.0 : 1.0) * (A[4 * r] * yj[0] + A[4 * r + 1] * yj[1] + A[4 * r + 2] * yj[2] + A[4 * r + 3] * yj[3]);
+    }
+    if (!any) return;
+    const double* P = Dti + (size_t)16 * i;
+    for (int r = 0; r < 4; ++r) {
+        const double t = P[4 * r] * acc[0] + P[4 * r + 1] * acc[1] + P[4 * r + 2] * acc[2] + P[4 * r + 3] * acc[3];
+        if (FORWARD) y[r * nT + i] = t;
+        else y[r * nT + i] -= t;
+    }
+}
+
+// ghost rows are identity rows: z = x there
+__global__ void __launch_bounds__(kBlock) k_copy_ghost_rows(int32_t nC, int32_t nT, const double* __restrict__ x,
+                                                            double* __restrict__ z, const int* __restrict__ active) {
+    if (active && *active == 0) return;
+    const int c = nC + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nT) return;
+    for (int k = 0; k < 4; ++k) z[(size_t)k * nT + c] = x[(size_t)k * nT + c];
+}
+
+// --------------------------------------------- device-resident GMRES (f1) --
+// One Arnoldi step j with classical Gram-Schmidt applied twice, as three
+// passes over the basis: (1) h1 = V^T w; (2) w -= V h1 and h2 = V^T w in the
+// same pass; (3) w -= V h2 and |w|^2.  Each pass writes per-block partial
+// sums that k_gm_reduce adds up.
+constexpr int kGsMax = 32;
+
+template <bool UPDATE, bool DOTS>
+__global__ void __launch_bounds__(kBlock) k_gs_pass(const double* __restrict__ V, int64_t ld, int m,
+                                                    const double* __restrict__ h, double* __restrict__ w, int64_t n,
+                                                    double* __restrict__ partial, const int* __restrict__ active) {
+    if (active && *active == 0) return;
+    __shared__ double red[kBlock / 64][kGsMax + 1];
+    double acc[kGsMax + 1];
+#pragma unroll
+    for (int k = 0; k <= kGsMax; ++k) acc[k] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double v[kGsMax];
+        double wi = w[i];
+#pragma unroll
+        for (int k = 0; k < kGsMax; ++k) {
+            if (k < m && (UPDATE || DOTS)) {
+                v[k] = V[(size_t)k * ld + i];
+                if (UPDATE) wi -= h[k] * v[k];
+            }
+        }
+        if (UPDATE) w[i] = wi;
+#pragma unroll
+        for (int k = 0; k < kGsMax; ++k)
+            if (DOTS && k < m) acc[k] += v[k] * wi;
+        acc[kGsMax] += wi * wi;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k <= kGsMax; ++k) {
+        if (k == kGsMax || (DOTS && k < m)) {
+            double t = acc[k];
+            for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+            if (lane == 0) red[wave][k] = t;
+        }
+    }
+    __syncthreads();
+    const int k = threadIdx.x;
+    if (k <= m && (DOTS || k == m)) {
+        const int src = k == m ? kGsMax : k;
+        double t = 0.0;
+        for (int q = 0; q < kBlock / 64; ++q) t += red[q][src];
+        partial[(size_t)k * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+// out[k] = sum_b partial[(first + k) * nblk + b], k < cnt; one block, a wavefront per k
+__global__ void __launch_bounds__(kBlock) k_gm_reduce(const double* __restrict__ partial, int nblk, int first, int cnt,
+                                                      double* __restrict__ out, const int* __restrict__ active) {
+    if (active && *active == 0) return;
+    const int lane = threadIdx.x & 63;
+    for (int k = threadIdx.x >> 6; k < cnt; k += kBlock / 64) {
+        const double* p = partial + (size_t)(first + k) * nblk;
+        double t = 0.0;
+        for (int b = lane; b < nblk; b += 64) t += p[b];
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+        if (lane == 0) out[k] = t;
+    }
+}
+
+// start of a cycle from the residual norm^2 in hdr[3]: the first cycle sets
+// beta0 (zero right-hand side: converged at once); a restart tests the true
+// residual against tol beta0
+__global__ void k_gm_begin(GmDev G, int first) {
+    if (threadIdx.x != 0) return;
+    const double beta = __builtin_sqrt(G.hdr[3]);
+    if (first) { G.hdr[0] = beta; G.flags[1] = 0; G.flags[2] = 0; }
+    G.hdr[1] = beta;
+    G.flags[3] = 0;
+    for (int k = 0; k <= G.m; ++k) G.g[k] = 0.0;
+    G.g[0] = beta;
+    const bool done = first ? !(beta > 0.0) : (beta <= G.tol * G.hdr[0]);
+    if (done) { G.flags[1] = 1; G.flags[0] = 0; return; }
+    G.flags[0] = 1;
+    G.hdr[2] = 1.0 / beta;
+}
+
+// Hessenberg column j from the two passes and |w|, the previous Givens
+// rotations and a new one, the residual estimate |g_{j+1}|, the flags and the
+// scale of the next basis vector
+__global__ void k_gm_step(GmDev G, int j) {
+    if (threadIdx.x != 0 || G.flags[0] == 0) return;
+    const int m = G.m;
+    double* H = G.H;
+    for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = G.h1[i] + G.h2[i];
+    const double nrm = __builtin_sqrt(G.hdr[3] > 0.0 ? G.hdr[3] : 0.0);
+    H[(size_t)(j + 1) * m + j] = nrm;
+    for (int i = 0; i < j; ++i) {
+        const double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
+        H[(size_t)i * m + j] = G.cs[i] * a + G.sn[i] * bb;
+        H[(size_t)(i + 1) * m + j] = -G.sn[i] * a + G.cs[i] * bb;
+    }
+    const double a = H[(size_t)j * m + j], bb = H[(size_t)(j + 1) * m + j];
+    const double r = hypot(a, bb);
+    G.cs[j] = r > 0 ? a / r : 1.0;
+    G.sn[j] = r > 0 ? bb / r : 0.0;
+    H[(size_t)j * m + j] = r;
+    H[(size_t)(j + 1) * m + j] = 0.0;
+    G.g[j + 1] = -G.sn[j] * G.g[j];
+    G.g[j] = G.cs[j] * G.g[j];
+    G.flags[2] += 1;
+    G.flags[3] = j + 1;
+    if (fabs(G.g[j + 1]) <= G.tol * G.hdr[0]) { G.flags[1] = 1; G.flags[0] = 0; return; }
+    if (!(nrm > 0) || G.flags[2] >= G.maxit) { G.flags[0] = 0; return; }
+    G.hdr[2] = 1.0 / nrm;
+}
+
+// y = H^-1 g over the cycle's jend columns (upper triangular)
+__global__ void k_gm_solve(GmDev G) {
+    if (threadIdx.x != 0) return;
+    const int m = G.m, jend = G.flags[3];
+    for (int i = jend - 1; i >= 0; --i) {
+        double t = G.g[i];
+        for (int k = i + 1; k < jend; ++k) t -= G.H[(size_t)i * m + k] * G.y[k];
+        G.y[i] = t / G.H[(size_t)i * m + i];
+    }
+}
+
+// y = a x with the scale a on the device
+__global__ void __launch_bounds__(kBlock) k_scale_dev(const double* __restrict__ a, const double* __restrict__ x,
+                                                      double* __restrict__ y, int64_t n, const int* __restrict__ active) {
+    if (active && *active == 0) return;
+    const double s = *a;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = s * x[i];
+}
+
+// x = sum_{k < jend} y_k V_k, jend on the device
+__global__ void __launch_bounds__(kBlock) k_lincomb_dev(const double* __restrict__ V, int64_t ld, const int* __restrict__ jend,
+                                                        const double* __restrict__ y, double* __restrict__ x, int64_t n) {
+    const int m = *jend;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int k = 0; k < m; ++k) acc += y[k] * V[(size_t)k * ld + i];
+    x[i] = acc;
+}
+
+// r = b - w
+__global__ void __launch_bounds__(kBlock) k_sub(const double* __restrict__ b, const double* __restrict__ w,
+                                                double* __restrict__ r, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = b[i] - w[i];
 }
 
 // ---------------------------------------------- matrix-free operator (a18) --
@@ -276,7 +477,8 @@ __global__ void __launch_bounds__(kBlock) k_mf_base(CellsT<double> C, FacesT<dou
 __global__ void __launch_bounds__(kBlock) k_mf_apply(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                      const double* __restrict__ dt, const double* __restrict__ Fb,
                                                      GasT<double> g, int viscous, const double* __restrict__ x,
-                                                     double* __restrict__ y) {
+                                                     double* __restrict__ y, const int* __restrict__ active) {
+    if (active && *active == 0) return;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C.nT) return;
     const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
@@ -328,32 +530,93 @@ hipError_t launch_assemble_diag(hipStream_t s, CellsT<double> C, FacesT<double> 
     return hipGetLastError();
 }
 hipError_t launch_lhs_apply(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* D, const double* Jf,
-                            const double* x, double* y) {
-    hipLaunchKernelGGL(k_lhs_apply, dim3(nblocks(C.nT)), dim3(kBlock), 0, s, C, F, D, Jf, x, y);
+                            const double* x, double* y, const int* active) {
+    hipLaunchKernelGGL(k_lhs_apply, dim3(nblocks(C.nT)), dim3(kBlock), 0, s, C, F, D, Jf, x, y, active);
     return hipGetLastError();
 }
-hipError_t launch_block_jacobi(hipStream_t s, int32_t nC, int32_t nT, const double* Dinv, const double* x, double* z) {
-    hipLaunchKernelGGL(k_block_jacobi, dim3(nblocks(nT)), dim3(kBlock), 0, s, nC, nT, Dinv, x, z);
+hipError_t launch_block_jacobi(hipStream_t s, int32_t nC, int32_t nT, const double* Dinv, const double* x, double* z,
+                               const int* active) {
+    hipLaunchKernelGGL(k_block_jacobi, dim3(nblocks(nT)), dim3(kBlock), 0, s, nC, nT, Dinv, x, z, active);
     return hipGetLastError();
 }
-hipError_t launch_multidot(hipStream_t s, const double* V, int64_t ld, int m, const double* w, int64_t n,
-                           double* partial, int nblk, double* out) {
-    hipLaunchKernelGGL(k_multidot, dim3(nblk), dim3(kBlock), 0, s, V, ld, m, w, n, partial);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_multidot_final, dim3(1), dim3(kBlock), 0, s, partial, nblk, m, out);
+hipError_t launch_dilu_setup(hipStream_t s, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
+                             const int32_t* cells, const int32_t* coff, int ncolours, const double* D, const double* Jf,
+                             double* Dti, int* bad) {
+    for (int c = 0; c < ncolours; ++c) {
+        const int32_t n = coff[c + 1] - coff[c];
+        if (n == 0) continue;
+        hipLaunchKernelGGL(k_dilu_setup, dim3(nblocks(n)), dim3(kBlock), 0, s, C, F, colour, c, cells + coff[c], n, D,
+                           Jf, Dti, bad);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+hipError_t launch_dilu_apply(hipStream_t s, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
+                             const int32_t* cells, const int32_t* coff, int ncolours, const double* Jf,
+                             const double* Dti, const double* x, double* z, const int* active) {
+    hipError_t e;
+    if (C.nT > C.nC) {
+        hipLaunchKernelGGL(k_copy_ghost_rows, dim3(nblocks(C.nT - C.nC)), dim3(kBlock), 0, s, C.nC, C.nT, x, z, active);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    for (int c = 0; c < ncolours; ++c) {
+        const int32_t n = coff[c + 1] - coff[c];
+        if (n == 0) continue;
+        hipLaunchKernelGGL(k_dilu_sweep<true>, dim3(nblocks(n)), dim3(kBlock), 0, s, C, F, colour, c, cells + coff[c], n,
+                           Jf, Dti, x, z, active);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    for (int c = ncolours - 2; c >= 0; --c) {
+        const int32_t n = coff[c + 1] - coff[c];
+        if (n == 0) continue;
+        hipLaunchKernelGGL(k_dilu_sweep<false>, dim3(nblocks(n)), dim3(kBlock), 0, s, C, F, colour, c, cells + coff[c],
+                           n, Jf, Dti, x, z, active);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+hipError_t launch_gs_pass(hipStream_t s, const double* V, int64_t ld, int m, const double* h, double* w, int64_t n,
+                          bool dots, double* partial, int nblk, const int* active) {
+    if (m > kGsMax) return hipErrorInvalidValue;
+    if (h && dots)
+        hipLaunchKernelGGL((k_gs_pass<true, true>), dim3(nblk), dim3(kBlock), 0, s, V, ld, m, h, w, n, partial, active);
+    else if (h)
+        hipLaunchKernelGGL((k_gs_pass<true, false>), dim3(nblk), dim3(kBlock), 0, s, V, ld, m, h, w, n, partial, active);
+    else if (dots)
+        hipLaunchKernelGGL((k_gs_pass<false, true>), dim3(nblk), dim3(kBlock), 0, s, V, ld, m, h, w, n, partial, active);
+    else
+        hipLaunchKernelGGL((k_gs_pass<false, false>), dim3(nblk), dim3(kBlock), 0, s, V, ld, m, h, w, n, partial, active);
     return hipGetLastError();
 }
-hipError_t launch_multiaxpy(hipStream_t s, const double* V, int64_t ld, int m, const double* h, double* w, int64_t n) {
-    hipLaunchKernelGGL(k_multiaxpy, dim3(nblocks(n)), dim3(kBlock), 0, s, V, ld, m, h, w, n);
+hipError_t launch_gm_reduce(hipStream_t s, const double* partial, int nblk, int first, int cnt, double* out,
+                            const int* active) {
+    hipLaunchKernelGGL(k_gm_reduce, dim3(1), dim3(kBlock), 0, s, partial, nblk, first, cnt, out, active);
     return hipGetLastError();
 }
-hipError_t launch_lincomb(hipStream_t s, const double* V, int64_t ld, int m, const double* y, double* x, int64_t n) {
-    hipLaunchKernelGGL(k_lincomb, dim3(nblocks(n)), dim3(kBlock), 0, s, V, ld, m, y, x, n);
+hipError_t launch_gm_begin(hipStream_t s, GmDev G, int first) {
+    hipLaunchKernelGGL(k_gm_begin, dim3(1), dim3(64), 0, s, G, first);
     return hipGetLastError();
 }
-hipError_t launch_scale(hipStream_t s, double a, const double* x, double* y, int64_t n) {
-    hipLaunchKernelGGL(k_scale, dim3(nblocks(n)), dim3(kBlock), 0, s, a, x, y, n);
+hipError_t launch_gm_step(hipStream_t s, GmDev G, int j) {
+    hipLaunchKernelGGL(k_gm_step, dim3(1), dim3(64), 0, s, G, j);
+    return hipGetLastError();
+}
+hipError_t launch_gm_solve(hipStream_t s, GmDev G) {
+    hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(64), 0, s, G);
+    return hipGetLastError();
+}
+hipError_t launch_scale_dev(hipStream_t s, const double* a, const double* x, double* y, int64_t n, const int* active) {
+    hipLaunchKernelGGL(k_scale_dev, dim3(nblocks(n)), dim3(kBlock), 0, s, a, x, y, n, active);
+    return hipGetLastError();
+}
+hipError_t launch_lincomb_dev(hipStream_t s, const double* V, int64_t ld, const int* jend, const double* y, double* x,
+                              int64_t n) {
+    hipLaunchKernelGGL(k_lincomb_dev, dim3(nblocks(n)), dim3(kBlock), 0, s, V, ld, jend, y, x, n);
+    return hipGetLastError();
+}
+hipError_t launch_sub(hipStream_t s, const double* b, const double* w, double* r, int64_t n) {
+    hipLaunchKernelGGL(k_sub, dim3(nblocks(n)), dim3(kBlock), 0, s, b, w, r, n);
     return hipGetLastError();
 }
 hipError_t launch_axpy(hipStream_t s, double a, const double* x, double* y, int64_t n) {
@@ -367,8 +630,8 @@ hipError_t launch_mf_base(hipStream_t s, CellsT<double> C, FacesT<double> F, con
     return hipGetLastError();
 }
 hipError_t launch_mf_apply(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* dt,
-                           const double* Fb, GasT<double> g, int viscous, const double* x, double* y) {
-    hipLaunchKernelGGL(k_mf_apply, dim3(nblocks(C.nT)), dim3(kBlock), 0, s, C, F, q, dt, Fb, g, viscous, x, y);
+                           const double* Fb, GasT<double> g, int viscous, const double* x, double* y, const int* active) {
+    hipLaunchKernelGGL(k_mf_apply, dim3(nblocks(C.nT)), dim3(kBlock), 0, s, C, F, q, dt, Fb, g, viscous, x, y, active);
     return hipGetLastError();
 }
 

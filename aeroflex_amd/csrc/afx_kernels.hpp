@@ -99,6 +99,7 @@ struct TilesF32 {
     const uint64_t* slots;
     const uint32_t* fck;
     const float4* geo;
+    const int32_t* nbf;      // per tile: boundary faces in its face list (0: interior tile)
     int32_t ntiles;
     int32_t lmax, gmax, ftmax;
 };

@@ -155,11 +155,14 @@ TileLayout build_tiles(const Mesh& m, const std::vector<uint32_t>& rcm, int32_t 
             }
             L.slots.push_back(word);
         }
+        int32_t nbf = 0;
         for (int32_t f : Fc) {
             uint32_t a = m.edgesCells[2 * (size_t)f], b = m.edgesCells[2 * (size_t)f + 1];
             L.fcell.push_back((uint32_t)loc[a] | ((uint32_t)loc[b] << 16));
             L.fref.push_back(f);
+            nbf += (int32_t)b >= nC || (int32_t)a >= nC;
         }
+        L.nbf.push_back(nbf);
         for (int32_t c : Lc) L.L_ref.push_back(c);
         L.ng.push_back(nG);
         L.nft.push_back(nft);

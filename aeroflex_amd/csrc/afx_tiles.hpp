@@ -30,6 +30,7 @@ struct TileLayout {
     std::vector<int32_t> ng;             // ntiles
     std::vector<int32_t> foff;           // ntiles + 1 (into per-face arrays)
     std::vector<int32_t> nft;            // ntiles
+    std::vector<int32_t> nbf;            // ntiles: faces of the tile's list with a ghost side
     std::vector<int32_t> goff;           // ntiles + 1 (into slots)
     std::vector<int32_t> L_ref;          // reference cell index of each local cell
     std::vector<uint64_t> slots;         // per G cell: 4 x u16 (bit 15 = neighbour side, 0xFFFF pad)

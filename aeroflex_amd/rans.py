@@ -89,6 +89,9 @@ def _load():
         "afx_rans_explicit_step": (C.c_int, [P, f64, dp]),
         "afx_rans_uniform_residual": (C.c_int, [P, dp]),
         "afx_rans_set_linear_solver": (C.c_int, [P, i32, f64, i32]),
+        "afx_rans_set_preconditioner": (C.c_int, [P, i32]),
+        "afx_rans_precond_apply": (C.c_int, [P, P, P]),
+        "afx_rans_cell_colours": (C.c_int, [P, P, P]),
         "afx_rans_implicit_fill": (C.c_int, [P]),
         "afx_rans_implicit_compute": (C.c_int, [P, C.POINTER(i32)]),
         "afx_rans_implicit_solve": (C.c_int, [P, f64, f64, i32, i32, dp]),
@@ -116,6 +119,7 @@ def _load():
         "afx_rans_create_like": (C.c_int, [P, C.POINTER(AfxGas), C.POINTER(P)]),
         "afx_rans_time_residual": (C.c_int, [P, i32, i32, dp]),
         "afx_rans_adjoint": (C.c_int, [P, dp, dp]),
+        "afx_rans_adjoint_nu": (C.c_int, [P, dp, dp, dp, dp]),
         "afx_rans_tangent": (C.c_int, [P, dp, dp, dp, dp]),
         "afx_rans_time_tangent": (C.c_int, [P, i32, dp]),
         "afx_rans_adjoint_nodes": (C.c_int, [P, dp, dp, dp]),
@@ -400,6 +404,16 @@ class Solver(Handle):
     def set_linear_solver(self, restart=30, tol=1e-2, maxit=500):
         _check(lib.afx_rans_set_linear_solver(self.h, restart, tol, maxit))
 
+    PRECONDITIONERS = {"block-jacobi": 0, "dilu": 1}
+
+    def set_preconditioner(self, kind="dilu"):
+        """The GMRES preconditioner standing in for the reference's
+        IncompleteLUT (solver.h:886): "block-jacobi" or "dilu" (multicolour
+        block DILU, the default)."""
+        if kind not in self.PRECONDITIONERS:
+            raise RansError(f"unknown preconditioner {kind!r}")
+        _check(lib.afx_rans_set_preconditioner(self.h, self.PRECONDITIONERS[kind]))
+
     def fill(self):
         """fillRhoLHS (solver.h:973-1075)."""
         _check(lib.afx_rans_implicit_fill(self.h))
@@ -421,6 +435,21 @@ class Solver(Handle):
         y = np.empty_like(x)
         _check(lib.afx_rans_lhs_apply(self.h, _dptr(x), _dptr(y)))
         return y
+
+    def precond_apply(self, x):
+        """M^-1 x with the preconditioner of the last fill (test hook)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        _check(lib.afx_rans_precond_apply(self.h, _dptr(x), _dptr(y)))
+        return y
+
+    def cell_colours(self):
+        """(colour per real cell in reference order, number of colours) of the
+        DILU sweeps (test hook)."""
+        col = np.empty(self.n_real, dtype=np.int32)
+        n = C.c_int32()
+        _check(lib.afx_rans_cell_colours(self.h, col.ctypes.data_as(C.c_void_p), C.byref(n)))
+        return col, n.value
 
     def set_matrix_free(self, enable=True):
         """The RANS_MATRIX_FREE build of implicitSolver (solver.h:863-925):
@@ -548,12 +577,25 @@ class Solver(Handle):
         _check(lib.afx_rans_tangent(self.h, _dptr(v), None, _dptr(jv), None))
         return jv
 
-    def adjoint(self, w):
-        """(dR/dq)^T w of fillRhoRHS at the current state (inviscid, Green-Gauss)."""
+    def adjoint(self, w, w_nu=None):
+        """(dR/dq)^T w of fillRhoRHS at the current state (every model,
+        Green-Gauss), in the q(4*i+k) layout; with the SA model also the
+        rho nu~ seed w_nu and row.  Returns lambda, or (lambda, lambda_nu) with
+        the SA model."""
         w = np.ascontiguousarray(w, np.float64)
         if w.shape != (4 * self.n_cells,):
             raise RansError("adjoint seed has the wrong length")
         lam = np.empty(4 * self.n_cells)
+        if self._opts.viscosity == 2:
+            wn = None if w_nu is None else np.ascontiguousarray(w_nu, np.float64)
+            if wn is not None and wn.shape != (self.n_cells,):
+                raise RansError("rho nu~ seed has the wrong length")
+            ln = np.empty(self.n_cells)
+            _check(lib.afx_rans_adjoint_nu(self.h, _dptr(w), None if wn is None else _dptr(wn), _dptr(lam),
+                                           _dptr(ln)))
+            return lam, ln
+        if w_nu is not None:
+            raise RansError("rho nu~ seed only with the Spalart-Allmaras model")
         _check(lib.afx_rans_adjoint(self.h, _dptr(w), _dptr(lam)))
         return lam
 

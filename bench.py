@@ -31,14 +31,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E peak, GB/s
-TILE_CELLS = 192        # cells per tile of the fused residual (tile-size sweep on MI355X)
+TILE_CELLS = 200        # cells per tile of the fused residual (tile-size sweep on MI355X: 160-256)
 
 
 class HipEvents:
-    """hipEvent timing on an arbitrary stream (the library's)."""
+    """hipEvent timing on an arbitrary stream (the library's).  The HIP
+    functions are looked up through the library's handle, i.e. in the HIP
+    runtime the library is linked against."""
 
     def __init__(self):
-        self.hip = C.CDLL("libamdhip64.so")
+        from aeroflex_amd.rans import LIB_PATH
+        self.hip = C.CDLL(LIB_PATH)
         self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
         self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
         self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
@@ -451,6 +454,107 @@ def batched_sweep(rans, steps, graph_steps=25, n_cond=64, n_mesh=24, cond_cells=
     }
 
 
+def implicit_bench(rans, cells=2_000_000, cfls=(40.0, 10.0, 2.0), relaxation=0.9, rhs_iterations=5):
+    """One implicit pseudo-time iteration of the reference loop
+    (multigrid.h:262-268: fillRhoLHS, then implicitSolver::solve with the
+    reference's rhs_iterations 5 and relaxation 0.9, solver.h:973-1213) at its
+    start CFL 40 and at 10 and 2, on the bench's 2M-cell jittered mesh,
+    inviscid (there is no SA Jacobian) with a slip wall, from the perturbed
+    freestream state.  Per preconditioner and CFL: fill and solve wall times,
+    GMRES iterations, time per GMRES iteration, and the returned residual norm
+    (-1: the linear solve failed or the state went non-finite, as the
+    reference's step does from this state at CFL 40)."""
+    ni, nj = mesh_dims(cells)
+    m = rans.Mesh.jittered(ni, nj, r_far=20.0, thickness=0.12, first_h=1e-4, jitter=0.1, quad_frac=0.25, seed=1)
+    far = rans.BoundaryVariables(mach=MACH, angle=np.deg2rad(ALPHA_DEG))
+    out = {"cells": None, "relaxation": relaxation, "rhs_iterations": rhs_iterations,
+           "gmres": "restart 30, tol 1e-2, maxit 500 (solver.h:906-910), device-resident Arnoldi state"}
+    for precond, cfl in [(p, c) for p in ("dilu", "block-jacobi") for c in cfls]:
+        s = rans.Solver(m, viscosity_model="inviscid", second_order=True, gradient_scheme="green-gauss",
+                        limiter_k=5.0)
+        s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition("slip-wall")})
+        s.init()
+        s.refill_bcs()
+        q, _ = perturbed(s.get_q(), None, s.n_real, far.get_conservative(s.gas), s.gas.gamma,
+                         np.random.default_rng(2))
+        out["cells"] = s.n_real
+        s.set_preconditioner(precond)
+        s.set_cfl(cfl)
+        # untimed: one fill and a single linear solve (allocations, first launches)
+        s.set_q(q)
+        s.fill()
+        s.solve(relaxation, 0.0, 0)
+        s.set_q(q)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.fill()
+        s.synchronize()
+        t_fill = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        err = s.solve(relaxation, 0.0, rhs_iterations)
+        s.synchronize()
+        t_solve = time.perf_counter() - t0
+        its = s.linear_iterations()
+        out[f"{precond}_cfl{cfl:g}"] = {"fill_ms": round(t_fill * 1e3, 2), "solve_ms": round(t_solve * 1e3, 2),
+                                         "iteration_ms": round((t_fill + t_solve) * 1e3, 2),
+                                         "gmres_iterations": int(its),
+                                         "ms_per_gmres_iteration": round(t_solve * 1e3 / max(its, 1), 3),
+                                         "residual_norm": err}
+        s.close()
+    m.close()
+    return out
+
+
+def sa_adjoint_check(rans, cells=500_000, n_pairs=3):
+    """State adjoint of the SA residual (the headline model) on a 500k-cell
+    mesh of the same generator (Green-Gauss, second order, M 0.7 alpha 1.5
+    deg Re 1e6 + 1% perturbations, no-slip wall): time against the FP64
+    residual and the tangent, and the tangent/adjoint dot-product test over the
+    five rows, |<w, Jv> - <J^T w, v>| / (|w| |Jv|)."""
+    ni, nj = mesh_dims(cells)
+    m = rans.Mesh.jittered(ni, nj, r_far=20.0, thickness=0.12, first_h=1e-4, jitter=0.1, quad_frac=0.25, seed=1)
+    s = rans.Solver(m, gas=rans.Gas(mu_L=freestream_mu(rans)), viscosity_model="spalart-allmaras",
+                    second_order=True, gradient_scheme="green-gauss", limiter_k=5.0)
+    far = rans.BoundaryVariables(mach=MACH, angle=np.deg2rad(ALPHA_DEG))
+    s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition("wall")})
+    s.init()
+    s.refill_bcs()
+    q, nu = perturbed(s.get_q(), s.get_nu(), s.n_real, far.get_conservative(s.gas), s.gas.gamma,
+                      np.random.default_rng(2))
+    s.set_q(q)
+    s.set_nu(nu)
+    t_primal = s.time_residual(rans.AFX_FP64, 50)
+    t_adj = s.time_adjoint(20)
+    t_tan = s.time_tangent(20)
+    nr, nt = s.n_real, s.n_cells
+    rng = np.random.default_rng(4)
+
+    def seed(nv):
+        x = np.zeros(nv * nt)
+        x[:nv * nr] = rng.standard_normal(nv * nr)
+        return x
+
+    worst = 0.0
+    for _ in range(n_pairs):
+        v, vn, w, wn = seed(4), seed(1), seed(4), seed(1)
+        s.set_q(q)
+        s.set_nu(nu)
+        jv, jn = s.tangent(v, vn)
+        s.set_q(q)
+        s.set_nu(nu)
+        lam, ln = s.adjoint(w, w_nu=wn)
+        lhs, rhs = float(w @ jv) + float(wn @ jn), float(lam @ v) + float(ln @ vn)
+        scale = float(np.sqrt(w @ w + wn @ wn) * np.sqrt(jv @ jv + jn @ jn))
+        worst = max(worst, abs(lhs - rhs) / scale)
+    s.close()
+    m.close()
+    return {"cells": nr, "model": "Spalart-Allmaras, Green-Gauss, second order, no-slip wall",
+            "adjoint_ms": round(t_adj, 4), "primal_fp64_ms": round(t_primal, 4),
+            "adjoint_over_primal": round(t_adj / t_primal, 2), "tangent_ms": round(t_tan, 4),
+            "dot_product_max_rel": float(f"{worst:.3e}"), "dot_product_tests": n_pairs,
+            "kernels": "5 passes: afx::k_sadj_forward, k_sadj_face, k_sadj_cell1, k_sadj_face2, k_sadj_cell2"}
+
+
 def adjoint_check(rans, cells=500_000, n_dir=64, h=1e-7):
     """State adjoint of the FP64 residual on a 500k-cell mesh of the same
     generator (inviscid, Green-Gauss, second order, M 0.7 alpha 1.5 deg + 1%
@@ -855,6 +959,9 @@ def main():
     ap.add_argument("--no-multigrid", dest="multigrid", action="store_false",
                     help="skip the multigrid set-up section (level map at 500k -> 2M cells)")
     ap.add_argument("--only-multigrid", action="store_true", help="run only the multigrid set-up section")
+    ap.add_argument("--no-implicit", dest="implicit", action="store_false",
+                    help="skip the implicit-iteration timing")
+    ap.add_argument("--only-implicit", action="store_true", help="run only the implicit-iteration timing")
     ap.add_argument("--only-adjoint", action="store_true",
                     help="run only the adjoint section (state, node coordinates, dCL/dalpha)")
     args = ap.parse_args()
@@ -872,7 +979,13 @@ def main():
         return
     if args.only_adjoint:
         from aeroflex_amd import rans
-        print(json.dumps({"adjoint": adjoint_check(rans)}), flush=True)
+        adj = adjoint_check(rans)
+        adj["spalart_allmaras"] = sa_adjoint_check(rans)
+        print(json.dumps({"adjoint": adj}), flush=True)
+        return
+    if args.only_implicit:
+        from aeroflex_amd import rans
+        print(json.dumps({"implicit": implicit_bench(rans)}), flush=True)
         return
     from aeroflex_amd import rans
     ev = HipEvents()
@@ -965,12 +1078,15 @@ def main():
         "fp32_vs_fp64_forces": forces32,
         "fp32_three_kernel_ms": None if unf is None else {k: round(v, 5) for k, v in unf.items()},
     }
-    if args.sweep_steps > 0 or args.adjoint:
+    if args.sweep_steps > 0 or args.adjoint or args.implicit:
         s.close()
         m.close()
         del s, m
     if args.adjoint:
         out["adjoint"] = adjoint_check(rans)
+        out["adjoint"]["spalart_allmaras"] = sa_adjoint_check(rans)
+    if args.implicit:
+        out["implicit"] = implicit_bench(rans)
     if args.sweep_steps > 0:
         out["batched_sweep"] = batched_sweep(rans, args.sweep_steps)
     if args.aeroelastic:

@@ -179,10 +179,19 @@ int afx_rans_explicit_residual(afx_rans* h, const double* qk, double* qW);
 int afx_rans_explicit_step(afx_rans* h, double relaxation, double* err);
 /* ---- implicit pseudo-time step (implicitSolver, solver.h:852-1213) ----
  * The linear solver replacing Eigen's GMRES + IncompleteLUT (solver.h:886,
- * 907-910): restarted GMRES(restart) with a block-Jacobi (4x4) right
- * preconditioner, FP64, stopping at ||b - A x|| <= tol ||b|| or after maxit
- * iterations (defaults 30, 1e-2, 500 as the reference sets them). */
+ * 907-910): restarted GMRES(restart) with a 4x4-block right preconditioner,
+ * FP64, stopping at ||b - A x|| <= tol ||b|| or after maxit iterations
+ * (defaults 30, 1e-2, 500 as the reference sets them; restart at most 32).
+ * The Arnoldi state stays on the device; the host synchronises once per
+ * restart cycle. */
 int afx_rans_set_linear_solver(afx_rans* h, int32_t restart, double tol, int32_t maxit);
+/* The preconditioner standing in for IncompleteLUT (solver.h:886, fill 2,
+ * drop tolerance 1e-3): kind 0 block Jacobi (the inverse 4x4 diagonal
+ * blocks), kind 1 (default) block DILU in a greedy colour order of the real
+ * cells, M = (Dt + L) Dt^-1 (Dt + U) with Dt_i = D_i - sum over earlier
+ * neighbours j of A_ij Dt_j^-1 A_ji; one parallel sweep per colour.  The next
+ * implicit step needs a new afx_rans_implicit_fill. */
+int afx_rans_set_preconditioner(afx_rans* h, int32_t kind);
 /* implicitSolver::fill -> fillRhoLHS (solver.h:973-1075): calc_dt, gradients
  * when viscous, FD flux Jacobian blocks (physics.h:533-578), A/dt diagonal */
 int afx_rans_implicit_fill(afx_rans* h);
@@ -199,9 +208,15 @@ int afx_rans_implicit_solve(afx_rans* h, double relaxation, double tol, int32_t 
                             double* err);
 /* y = LHS x with the matrix of the last fill (AoS, 4*n_cells); for tests */
 int afx_rans_lhs_apply(afx_rans* h, const double* x, double* y);
+/* y = M^-1 x with the preconditioner of the last fill (AoS, 4*n_cells; ghost
+ * rows identity); for tests */
+int afx_rans_precond_apply(afx_rans* h, const double* x, double* y);
+/* colour of each real cell (reference order, n_real entries, nullable) in the
+ * DILU sweeps and the number of colours; for tests */
+int afx_rans_cell_colours(afx_rans* h, int32_t* colours, int32_t* ncolours);
 /* The RANS_MATRIX_FREE build of implicitSolver (solver.h:863-868, 911-914,
  * 920-925): enable = 1 replaces the assembled matrix by the matrix-free
- * operator RansMatrix (ransMatrix.h:138-178) and the block-Jacobi GMRES by
+ * operator RansMatrix (ransMatrix.h:138-178) and the preconditioned GMRES by
  * unpreconditioned GMRES with that build's tolerance 1e-3 and 1000 iterations
  * (afx_rans_set_linear_solver afterwards overrides them); enable = 0 returns
  * to the assembled form (1e-2, 500).  Either way the next implicit step needs
@@ -265,12 +280,19 @@ int afx_rans_nu_residual(afx_rans* h, double* Rnu);
 int afx_rans_create_like(const afx_rans* base, const afx_gas* g, afx_rans** out);
 /* Adjoint of the residual (no counterpart in the reference): lambda =
  * (dR/dq)^T w for R = fillRhoRHS (solver.h:1079-1152) at the current state,
- * for the inviscid and laminar residuals with Green-Gauss gradients, first or
- * second order (the limiter differentiated on its active branch; the laminar
- * terms of physics.h:230-258 at the face gradient of solver.h:359-398).  w, lambda: reference
- * AoS layout 4 * n_cells; ghost rows of w are ignored and of lambda zero.
+ * for every viscosity model with Green-Gauss gradients, first or second order
+ * (the limiter differentiated on its active branch; the laminar terms of
+ * physics.h:230-258 at the face gradient of solver.h:359-398; with the
+ * Spalart-Allmaras model the five-variable face flux and the cell source).
+ * w, lambda: reference AoS layout 4 * n_cells; ghost rows of w are ignored
+ * and of lambda zero.  With the Spalart-Allmaras model this is
+ * afx_rans_adjoint_nu with a zero rho nu~ seed.
  * time_adjoint: average ms of one adjoint evaluation (HIP events). */
 int afx_rans_adjoint(afx_rans* h, const double* w, double* lambda);
+/* The same with the rho nu~ row of the Spalart-Allmaras model: seed w_nu and
+ * output lambda_nu (n_cells each, reference order, nullable; only with that
+ * model). */
+int afx_rans_adjoint_nu(afx_rans* h, const double* w, const double* w_nu, double* lambda, double* lambda_nu);
 /* Tangent of the residual (no counterpart in the reference): jv = (dR/dq) v
  * for R = fillRhoRHS (solver.h:1079-1152) at the current state, for every
  * model (inviscid, laminar, Spalart-Allmaras), gradient scheme and order, the

@@ -84,19 +84,22 @@ def test_adjoint_against_oracle_differences():
 
 
 def test_adjoint_rejects_other_models():
-    """The reverse passes cover the inviscid and laminar residuals; the SA
-    model, and the node / pseudo-time adjoints of the laminar residual, are
-    refused."""
+    """The state adjoint covers every model; the node / pseudo-time adjoints
+    cover the inviscid residual, so the laminar and SA ones are refused, and a
+    rho nu~ seed is refused without the SA model."""
     from aeroflex_amd import rans
     import test_gpu_sa
     s, _ = test_gpu_sa.sa_pair(TRI)
+    s.adjoint(np.zeros(4 * s.n_cells))
     with pytest.raises(rans.RansError):
-        s.adjoint(np.zeros(4 * s.n_cells))
+        s.adjoint_nodes(np.zeros(4 * s.n_cells))
     s.close()
     s, _, _ = make_pair(TRI, viscosity="laminar", wall="wall", mu_L=1e-3)
     s.adjoint(np.zeros(4 * s.n_cells))
     with pytest.raises(rans.RansError):
         s.adjoint_nodes(np.zeros(4 * s.n_cells))
+    with pytest.raises(rans.RansError):
+        s.adjoint(np.zeros(4 * s.n_cells), w_nu=np.zeros(s.n_cells))
 
 
 # ------------------------------------------------------ node coordinates --

@@ -52,10 +52,15 @@ def _solver_quad(rans):
 
 
 class _DeviceDouble:
-    """One double of device memory allocated by the caller (hipMalloc)."""
+    """One double of device memory allocated by the caller (hipMalloc).  The
+    HIP runtime functions are looked up through the library's own handle
+    (its dependency scope), so they are those of the runtime the library is
+    linked against even when another copy of libamdhip64 is loaded in the
+    process."""
 
     def __init__(self):
-        self.hip = C.CDLL("libamdhip64.so")
+        from aeroflex_amd.rans import LIB_PATH
+        self.hip = C.CDLL(LIB_PATH)
         self.hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         self.hip.hipFree.argtypes = [C.c_void_p]
         self.hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
