@@ -18,12 +18,13 @@
 //   4 faces  transpose of the Green-Gauss face values (boundary faces: the
 //            face states of face_vars and sa_bnd_nu)
 //   5 cells  gather of pass 4 and of the neighbours' limiter parts
-// The local Jacobians of the flux, the source and the boundary face values
-// come from forward-mode sweeps (one per input) of the same device functions
-// the residual evaluates, so they are exact; the structure around them is
-// transposed by hand.  Wall / slip ghost states enter nothing (their face
-// states are functions of the owner's); farfield ghost states are boundary
-// data and get no adjoint.
+// The Roe part of the flux goes through the hand-written reverse sweep of the
+// inviscid adjoint; the diffusive part (sa_face_diffusion), the cell source,
+// the boundary faces and their face values through forward-mode sweeps (one
+// per input) of the same device functions the residual evaluates, so every
+// local Jacobian is exact; the structure around them is transposed by hand.
+// Wall / slip ghost states enter nothing (their face states are functions of
+// the owner's); farfield ghost states are boundary data and get no adjoint.
 #include "afx_device.hpp"
 #include "afx_adjoint.hpp"
 #include "afx_adjoint_kernels.hpp"
@@ -179,7 +180,21 @@ __global__ void __launch_bounds__(kBlock) k_sadj_face(CellsT<double> C, FacesT<d
             gxa[k] = gxb - (dot - dir) * t0;
             gya[k] = gyb - (dot - dir) * t1;
         }
-        // one sweep per input: qL (5), qR (5), face gradients x (5), y (5)
+        // Roe and the rho nu~ convection f4 = f0 s (s = nu~ of the upwind
+        // side): the hand-written reverse sweep of the Roe flux with the
+        // mass-flux adjoint bF0 + bF4 s, then s's own dependence
+        double fr[4];
+        roe_flux<double>(g, nx, ny, qL, qR, false, nullptr, nullptr, fr);
+        const bool up = fr[0] >= 0.0;
+        const double s = up ? qL[4] / qL[0] : qR[4] / qR[0];
+        const double bRoe[4] = {bF[0] + bF[4] * s, bF[1], bF[2], bF[3]};
+        roe_flux_vjp<double>(g.gamma, nx, ny, qL, qR, bRoe, bL, bR);
+        double* bu = up ? bL : bR;
+        const double* qu = up ? qL : qR;
+        bu[4] += bF[4] * fr[0] / qu[0];
+        bu[0] -= bF[4] * fr[0] * qu[4] / (qu[0] * qu[0]);
+        // the diffusive part (sa_face_diffusion): one forward sweep per input,
+        // qL (5), qR (5), face gradients x (5), y (5)
         double bgx[NS], bgy[NS];
         for (int j = 0; j < 4 * NS; ++j) {
             Dual1 a[NS], b[NS], gx[NS], gy[NS], fo[NS];
@@ -188,12 +203,13 @@ __global__ void __launch_bounds__(kBlock) k_sadj_face(CellsT<double> C, FacesT<d
                 b[k] = Dual1(qR[k], j == NS + k ? 1.0 : 0.0);
                 gx[k] = Dual1(gxa[k], j == 2 * NS + k ? 1.0 : 0.0);
                 gy[k] = Dual1(gya[k], j == 3 * NS + k ? 1.0 : 0.0);
+                fo[k] = Dual1(0.0);
             }
-            sa_face_flux<Dual1>(gd, K_INTERNAL, Dual1(nx), Dual1(ny), a, b, gx, gy, Dual1(0.0), Dual1(1.0), fo);
+            sa_face_diffusion<Dual1>(gd, Dual1(nx), Dual1(ny), a, b, gx, gy, fo);
             double v = 0.0;
-            for (int k = 0; k < NS; ++k) v += bF[k] * fo[k].d;
-            if (j < NS) bL[j] = v;
-            else if (j < 2 * NS) bR[j - NS] = v;
+            for (int k = 1; k < NS; ++k) v += bF[k] * fo[k].d;
+            if (j < NS) bL[j] += v;
+            else if (j < 2 * NS) bR[j - NS] += v;
             else if (j < 3 * NS) bgx[j - 2 * NS] = v;
             else bgy[j - 3 * NS] = v;
         }
@@ -275,21 +291,27 @@ __global__ void __launch_bounds__(kBlock) k_sadj_cell1(CellsT<double> C, FacesT<
             lq[k] += nb ? -A.vdq[k * nF + f] : A.vdq[k * nF + f];
         }
     }
-    // the source term A S(q_c, grad q_c, d_c) of the rho nu~ row
+    // the source term A S(q_c, grad q_c, d_c) of the rho nu~ row: one forward
+    // sweep per input it reads, the state (5) and the gradients of rho, rho v
+    // and rho nu~ in x and of rho, rho u and rho nu~ in y (vorticity and
+    // grad nu~; sa_cell_source reads no other gradient)
     const double ws = w[4 * nT + c] * C.area[c];
     if (ws != 0.0) {
         const GasT<Dual1> gd = sa_dual_gas(g);
-        for (int j = 0; j < 3 * NS; ++j) {
+        constexpr int kx[3] = {0, 2, 4}, ky[3] = {0, 1, 4};
+        for (int j = 0; j < NS + 6; ++j) {
             Dual1 a[NS], bx[NS], by[NS];
             for (int k = 0; k < NS; ++k) {
                 a[k] = Dual1(qc[k], j == k ? 1.0 : 0.0);
-                bx[k] = Dual1(gx[k], j == NS + k ? 1.0 : 0.0);
-                by[k] = Dual1(gy[k], j == 2 * NS + k ? 1.0 : 0.0);
+                bx[k] = Dual1(gx[k]);
+                by[k] = Dual1(gy[k]);
             }
+            if (j >= NS && j < NS + 3) bx[kx[j - NS]].d = 1.0;
+            if (j >= NS + 3) by[ky[j - NS - 3]].d = 1.0;
             const double v = ws * sa_cell_source<Dual1>(gd, a, bx, by, Dual1(wd[c])).d;
             if (j < NS) lq[j] += v;
-            else if (j < 2 * NS) lgx[j - NS] += v;
-            else lgy[j - 2 * NS] += v;
+            else if (j < NS + 3) lgx[kx[j - NS]] += v;
+            else lgy[ky[j - NS - 3]] += v;
         }
     }
     for (int k = 0; k < NS; ++k) {

@@ -304,6 +304,28 @@ __device__ __forceinline__ T sa_bnd_nu(int kind, T nx, T ny, const T* qL, T nL, 
     const T io = qL[1] * nx + qL[2] * ny;
     return qb[0] * (io < T(0) ? ddiv<T>(nR, qR[0]) : ddiv<T>(nL, qL[0]));
 }
+// the diffusive part of an interior face's five-variable flux, subtracted
+// from f[1..4]: the mean-flow viscous terms with mu_L + mu_t and
+// k = cp (mu_L/Pr_L + mu_t/Pr_T), and the nu~ diffusion (mu_L + rho nu~ f_n) /
+// sigma grad nu~ . n at the face gradients gx, gy of the 5 variables
+template <typename T>
+__device__ __forceinline__ void sa_face_diffusion(const GasT<T>& g, T nx, T ny, const T* qL, const T* qR, const T* gx,
+                                                  const T* gy, T* f) {
+    const T nc = T(0.5) * (qL[4] + qR[4]);
+    const T chi = ddiv<T>(nc, g.mu);
+    const T mut = nc * sa_fv1<T>(chi);
+    const T mu = g.mu + mut;
+    const T kc = g.cp * (ddiv<T>(g.mu, g.PrL) + ddiv<T>(mut, g.PrT));
+    T f1 = f[1], f2 = f[2], f3 = f[3];
+    viscous_terms<T>(g, nx, ny, qL, qR, gx, gy, mu, kc, f1, f2, f3);
+    f[1] = f1; f[2] = f2; f[3] = f3;
+    const T qc0 = T(0.5) * (qL[0] + qR[0]);
+    const T nut = ddiv<T>(nc, qc0);
+    const T gvx = ddiv<T>(gx[4] - nut * gx[0], qc0), gvy = ddiv<T>(gy[4] - nut * gy[0], qc0);
+    const T dcoef = ddiv<T>(g.mu + nc * sa_fn<T>(chi), T(sa::sigma));
+    f[4] -= dcoef * (nx * gvx + ny * gvy);
+}
+
 // five-variable face flux: Roe on the mean flow, upwind convection of rho nu~
 // by the sign of the mass flux; internal faces add the viscous terms with
 // mu_L + mu_t and the nu~ diffusion; a no-slip wall face adds the diffusion
@@ -324,19 +346,7 @@ __device__ __forceinline__ void sa_face_flux(const GasT<T>& g, int kind, T nx, T
     }
     roe_flux<T>(g, nx, ny, qL, qR, false, gx, gy, f);
     f[4] = f[0] * (f[0] >= T(0) ? ddiv<T>(qL[4], qL[0]) : ddiv<T>(qR[4], qR[0]));
-    const T nc = T(0.5) * (qL[4] + qR[4]);
-    const T chi = ddiv<T>(nc, g.mu);
-    const T mut = nc * sa_fv1<T>(chi);
-    const T mu = g.mu + mut;
-    const T kc = g.cp * (ddiv<T>(g.mu, g.PrL) + ddiv<T>(mut, g.PrT));
-    T f1 = f[1], f2 = f[2], f3 = f[3];
-    viscous_terms<T>(g, nx, ny, qL, qR, gx, gy, mu, kc, f1, f2, f3);
-    f[1] = f1; f[2] = f2; f[3] = f3;
-    const T qc0 = T(0.5) * (qL[0] + qR[0]);
-    const T nut = ddiv<T>(nc, qc0);
-    const T gvx = ddiv<T>(gx[4] - nut * gx[0], qc0), gvy = ddiv<T>(gy[4] - nut * gy[0], qc0);
-    const T dcoef = ddiv<T>(g.mu + nc * sa_fn<T>(chi), T(sa::sigma));
-    f[4] -= dcoef * (nx * gvx + ny * gvy);
+    sa_face_diffusion<T>(g, nx, ny, qL, qR, gx, gy, f);
 }
 // source of cell with state q (5), unlimited gradients gx/gy (5) and wall
 // distance d: vorticity from the momentum gradients, grad nu~ from grad(rho nu~)

@@ -555,7 +555,7 @@ def sa_adjoint_check(rans, cells=500_000, n_pairs=3):
             "kernels": "5 passes: afx::k_sadj_forward, k_sadj_face, k_sadj_cell1, k_sadj_face2, k_sadj_cell2"}
 
 
-def adjoint_check(rans, cells=500_000, n_dir=64, h=1e-7):
+def adjoint_check(rans, cells=500_000, n_dir=64, h=1e-7, node_fd_dirs=2):
     """State adjoint of the FP64 residual on a 500k-cell mesh of the same
     generator (inviscid, Green-Gauss, second order, M 0.7 alpha 1.5 deg + 1%
     perturbations): time against the FP64 residual, and central differences
@@ -609,7 +609,7 @@ def adjoint_check(rans, cells=500_000, n_dir=64, h=1e-7):
            "tangent_ms": round(t_tan, 4), "tangent_over_primal": round(t_tan / t_primal, 2),
            "tangent_kernel": "afx::k_residual_tile<afx::Dual1, 4> (forward mode of the FP64 tile residual)",
            "dot_product_max_rel": float(f"{max(dots):.3e}"), "dot_product_tests": len(dots)}
-    out["node_adjoint"] = node_adjoint_check(rans, m, s, q, w)
+    out["node_adjoint"] = node_adjoint_check(rans, m, s, q, w, n_dir=node_fd_dirs)
     out["dcl_dalpha"] = dcl_dalpha_check(rans, s, q)
     out["dcl_dnodes"] = dcl_dnodes_check(rans, m, s, q)
     return out
@@ -658,11 +658,15 @@ def node_adjoint_check(rans, m, s, q, w, n_dir=2, h=1e-6):
         np.minimum.at(scale, cells[idx, k], np.sqrt(area[idx]))
     rng = np.random.default_rng(5)
     worst = 0.0
-    for _ in range(n_dir):
+    t0 = time.perf_counter()
+    for i in range(n_dir):
         V = rng.standard_normal(xb.shape) * scale[:, None]
         fd = float(w @ (residual(h * V) - residual(-h * V))) / (2 * h)
         ad = float(np.sum(xb * V))
         worst = max(worst, abs(ad - fd) / max(abs(fd), 1e-3 * float(np.linalg.norm(xb) * np.linalg.norm(V))))
+        if n_dir > 2:   # a long check: progress on stderr
+            print(f"node FD direction {i + 1}/{n_dir}: worst relative error so far {worst:.3e} "
+                  f"({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
     return {"nodes": int(c["nodes"]), "wall_ms_per_eval": round(t_nodes * 1e3, 3), "fd_directions": n_dir,
             "fd_step": h, "fd_max_rel_err": float(f"{worst:.3e}"),
             "kernels": "metric adjoints in passes 2-4, then afx::k_geo_face, k_geo_cell, k_geo_node"}
@@ -962,6 +966,9 @@ def main():
     ap.add_argument("--no-implicit", dest="implicit", action="store_false",
                     help="skip the implicit-iteration timing")
     ap.add_argument("--only-implicit", action="store_true", help="run only the implicit-iteration timing")
+    ap.add_argument("--node-fd-dirs", type=int, default=64,
+                    help="random node displacements of the node adjoint's central-difference check (each rebuilds "
+                         "the 500k-cell mesh and solver twice)")
     ap.add_argument("--only-adjoint", action="store_true",
                     help="run only the adjoint section (state, node coordinates, dCL/dalpha)")
     args = ap.parse_args()
@@ -979,7 +986,7 @@ def main():
         return
     if args.only_adjoint:
         from aeroflex_amd import rans
-        adj = adjoint_check(rans)
+        adj = adjoint_check(rans, node_fd_dirs=args.node_fd_dirs)
         adj["spalart_allmaras"] = sa_adjoint_check(rans)
         print(json.dumps({"adjoint": adj}), flush=True)
         return
@@ -1083,7 +1090,7 @@ def main():
         m.close()
         del s, m
     if args.adjoint:
-        out["adjoint"] = adjoint_check(rans)
+        out["adjoint"] = adjoint_check(rans, node_fd_dirs=args.node_fd_dirs)
         out["adjoint"]["spalart_allmaras"] = sa_adjoint_check(rans)
     if args.implicit:
         out["implicit"] = implicit_bench(rans)
