@@ -342,7 +342,20 @@ def cpu_baseline(rans, model, budget_s=12.0):
     out["all_cores"] = {"value": round(rate, 4), "unit": "Mcell-residuals/s", "cores": cores, "kind": "port",
                         "sample": f"{cores} processes, each the serial oracle on the same sample for "
                                   f"{budget_s:.0f} s, rates summed"}
+    out["cpu_model"] = cpu_model()
     return out
+
+
+def cpu_model():
+    """The host CPU's model name (first entry of /proc/cpuinfo), if readable."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 SWEEP_CFL = 0.3
@@ -430,6 +443,15 @@ def batched_sweep(rans, steps, graph_steps=25, n_cond=64, n_mesh=24, cond_cells=
     conds[0].explicit_solve(0.8)             # warm-up of the plain form
     reset()
     t_plain = timed(lambda: [c.explicit_solve(0.8) for c in conds for _ in range(steps)])
+    # the same FP32 form one condition at a time (a batch of one member
+    # each, replayed from its own graph): what batching alone buys
+    singles = [rans.Batch([c], relaxation=0.8) for c in conds]
+    for b in singles:
+        b.run(graph_steps, graph_steps)
+    reset()
+    t_single = timed(lambda: [b.run(steps, graph_steps) for b in singles])
+    for b in singles:
+        b.close()
     cells_all = sum(m.n_real for m in conds + uneven)
     cells_cond = sum(m.n_real for m in conds)
     return {
@@ -441,6 +463,8 @@ def batched_sweep(rans, steps, graph_steps=25, n_cond=64, n_mesh=24, cond_cells=
         "batched_64_conditions_s": round(t_cond, 4),
         "plain_64_sequential_s": round(t_plain, 4),
         "speedup_vs_plain_sequential": round(t_plain / t_cond, 2),
+        "fp32_64_sequential_s": round(t_single, 4),
+        "speedup_vs_fp32_sequential": round(t_single / t_cond, 2),
         "plain_Mcell_res_per_s": round(3 * steps * cells_cond / t_plain / 1e6, 1),
         "batched_64_Mcell_res_per_s": round(3 * steps * cells_cond / t_cond / 1e6, 1),
         "finite_final_residuals": bool(np.all(np.isfinite(norms))),
@@ -449,7 +473,8 @@ def batched_sweep(rans, steps, graph_steps=25, n_cond=64, n_mesh=24, cond_cells=
         "note": f"explicit three-stage pseudo-time steps (SA, CFL {SWEEP_CFL}, relaxation 0.8) from freestream; "
                 "plain = FP64 fused residual "
                 "+ FP64 update per condition with a norm per step, batched = one launch per stage over all "
-                "members (FP32 residual, FP64 stage update), replayed from a hipGraph; tiles autotuned on the "
+                "members (FP32 residual, FP64 stage update), replayed from a hipGraph; fp32 sequential = the "
+                "batched kernel on one condition at a time, each from its own graph; tiles autotuned on the "
                 "batched step",
     }
 
