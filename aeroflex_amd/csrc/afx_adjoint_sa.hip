@@ -269,15 +269,14 @@ __global__ void __launch_bounds__(kBlock) k_sadj_cell1(CellsT<double> C, FacesT<
         ph[k] = second_order ? A.lim[k * nT + c] : 1.0;
         qc[k] = q[k * nT + c];
     }
-    uint32_t es[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
-    double dxs[4] = {0, 0, 0, 0}, dys[4] = {0, 0, 0, 0};
+    double dxs[4] = {0, 0, 0, 0}, dys[4] = {0, 0, 0, 0};   // the slots' offsets, for the limiter's active slot
     for (int s = 0; s < 4; ++s) {
         const uint32_t e = C.cface[(size_t)s * nC + c];
         if (e == 0xFFFFFFFFu) break;
         const int f = (int)(e & 0x7FFFFFFFu);
         const bool nb = (e >> 31) != 0;
         const double dx = F.xf[f] - C.xc[c], dy = F.yf[f] - C.yc[c];
-        es[s] = e; dxs[s] = dx; dys[s] = dy;
+        dxs[s] = dx; dys[s] = dy;
         for (int k = 0; k < NS; ++k) {
             const double b = nb ? A.bR[k * nF + f] : A.bL[k * nF + f];
             lq[k] += b;
@@ -319,8 +318,6 @@ __global__ void __launch_bounds__(kBlock) k_sadj_cell1(CellsT<double> C, FacesT<
         int nbj = -1;
         const int s = second_order ? (int)A.act[k * nC + c] : -1;
         if (s >= 0) {
-            const uint32_t e = s == 0 ? es[0] : s == 1 ? es[1] : s == 2 ? es[2] : es[3];
-            (void)e;
             const double dx = s == 0 ? dxs[0] : s == 1 ? dxs[1] : s == 2 ? dxs[2] : dxs[3];
             const double dy = s == 0 ? dys[0] : s == 1 ? dys[1] : s == 2 ? dys[2] : dys[3];
             const double dqg = gx[k] * dx + gy[k] * dy;

@@ -224,7 +224,8 @@ struct afx_rans {
     template <typename T>
     TilesT<T> tiles(const T* geo) const {
         return TilesT<T>{t_cell0.p, t_loff.p, t_ng.p, t_foff.p, t_nft.p, t_goff.p, t_Lglob.p, t_slots.p,
-                         t_fcell.p, t_fkind.p, geo, (int64_t)tl.fref.size(), tl.ntiles, tl.lmax, tl.gmax, tl.ftmax};
+                         t_fcell.p, t_fkind.p, geo, t_nbf.p, (int64_t)tl.fref.size(), tl.ntiles, tl.lmax, tl.gmax,
+                         tl.ftmax};
     }
 };
 

@@ -627,17 +627,18 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
     return (b & 7) * per + (b >> 3);
 }
 
-template <typename T, int NV>
-__global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* __restrict__ q, int32_t nT, int32_t nC,
-                                                          const T* __restrict__ area, const T* __restrict__ lsq,
-                                                          const T* __restrict__ xc, const T* __restrict__ yc,
-                                                          const T* __restrict__ wd, GasT<T> g, int gg,
-                                                          int second_order, T limiter_k, int viscous,
-                                                          T* __restrict__ R, T* __restrict__ qghost) {
+// BND: the tile's face list holds boundary faces; without, every face is
+// taken as interior at compile time (no kind tests, no boundary states, no
+// ghost writes) - the same operations on the same values, so the same bits
+template <typename T, int NV, bool BND>
+__device__ __forceinline__ void residual_tile_body(const TilesT<T>& S, const T* __restrict__ q, int32_t nT, int32_t nC,
+                                                   const T* __restrict__ area, const T* __restrict__ lsq,
+                                                   const T* __restrict__ xc, const T* __restrict__ yc,
+                                                   const T* __restrict__ wd, const GasT<T>& g, int gg,
+                                                   int second_order, T limiter_k, int viscous,
+                                                   T* __restrict__ R, T* __restrict__ qghost, const int t,
+                                                   unsigned char* smem_raw) {
     constexpr bool SA = (NV == 5);
-    extern __shared__ unsigned char smem_raw[];
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    if (t >= S.ntiles) return;
     const int c0g = S.cell0[t], nTt = S.cell0[t + 1] - c0g;
     const int lo = S.loff[t], nL = S.loff[t + 1] - lo;
     const int nG = S.ng[t];
@@ -679,7 +680,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
                 T qo[4], qn[4], qv[NV];
                 for (int k = 0; k < 4; ++k) { qo[k] = sq[k * LM + o]; qn[k] = sq[k * LM + n]; }
                 const T nx = S.geo[f], ny = S.geo[NF + f], L = S.geo[2 * NF + f], gf = S.geo[3 * NF + f];
-                const int kind = S.fkind[f];
+                const int kind = BND ? (int)S.fkind[f] : (int)K_INTERNAL;
                 face_vars<T>(g, kind, nx, ny, qo, qn, qv);
                 T vo[NV];
                 for (int k = 0; k < 4; ++k) vo[k] = qo[k];
@@ -711,7 +712,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
                 const T dox = nb_side ? S.geo[4 * NF + f] : S.geo[6 * NF + f];
                 const T doy = nb_side ? S.geo[5 * NF + f] : S.geo[7 * NF + f];
                 const T dx = dcx - dox, dy = dcy - doy;   // x_other - x_cell
-                const int kind = S.fkind[f];
+                const int kind = BND ? (int)S.fkind[f] : (int)K_INTERNAL;
                 T qn[4], qv[NV];
                 for (int k = 0; k < 4; ++k) qn[k] = sq[k * LM + other];
                 face_vars<T>(g, kind, S.geo[f], S.geo[NF + f], qc, qn, qv);
@@ -740,7 +741,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
                 const uint32_t e = (uint32_t)(sw >> (16 * s)) & 0xFFFFu;
                 if (e == 0xFFFFu) break;
                 const int f = fo + (int)(e & 0x7FFFu);
-                const int kind = S.fkind[f];
+                const int kind = BND ? (int)S.fkind[f] : (int)K_INTERNAL;
                 // a wall/slip ghost holds the owner's state (set_walls_from_internal)
                 if (kind == K_WALL || kind == K_SLIP) continue;
                 const uint32_t fc = S.fcell[f];
@@ -780,7 +781,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
         const int f = fo + fl;
         const uint32_t fc = S.fcell[f];
         const int o = (int)(fc & 0xFFFFu), n = (int)(fc >> 16);
-        const int kind = S.fkind[f];
+        const int kind = BND ? (int)S.fkind[f] : (int)K_INTERNAL;
         const T nx = S.geo[f], ny = S.geo[NF + f], L = S.geo[2 * NF + f];
         T qL[NV], qR[NV];
         for (int k = 0; k < NV; ++k) { qL[k] = sq[k * LM + o]; qR[k] = sq[k * LM + n]; }
@@ -788,7 +789,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
             const T d0x = S.geo[4 * NF + f], d0y = S.geo[5 * NF + f];
             for (int k = 0; k < NV; ++k)
                 qL[k] = qL[k] + (sgx[k * GM + o] * d0x + sgy[k * GM + o] * d0y) * slm[k * GM + o];
-            if (n < nG) {
+            if (!BND || n < nG) {
                 const T d1x = S.geo[6 * NF + f], d1y = S.geo[7 * NF + f];
                 for (int k = 0; k < NV; ++k)
                     qR[k] = qR[k] + (sgx[k * GM + n] * d1x + sgy[k * GM + n] * d1y) * slm[k * GM + n];
@@ -837,7 +838,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
             const int fl = (int)(e & 0x7FFFu);
             if (e >> 15) for (int k = 0; k < NV; ++k) acc[k].add(sfl[k * FM + fl]);
             else for (int k = 0; k < NV; ++k) acc[k].sub(sfl[k * FM + fl]);
-            if (qghost) {
+            if (BND && qghost) {
                 const int kind = S.fkind[fo + fl];
                 if (kind == K_WALL || kind == K_SLIP) {
                     const int gh = -1 - Lg[(int)(S.fcell[fo + fl] >> 16)];
@@ -855,6 +856,23 @@ __global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* 
     }
 }
 
+template <typename T, int NV>
+__global__ void __launch_bounds__(kBlock) k_residual_tile(TilesT<T> S, const T* __restrict__ q, int32_t nT, int32_t nC,
+                                                          const T* __restrict__ area, const T* __restrict__ lsq,
+                                                          const T* __restrict__ xc, const T* __restrict__ yc,
+                                                          const T* __restrict__ wd, GasT<T> g, int gg,
+                                                          int second_order, T limiter_k, int viscous,
+                                                          T* __restrict__ R, T* __restrict__ qghost) {
+    extern __shared__ unsigned char smem_raw[];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= S.ntiles) return;
+    if (S.nbf && S.nbf[t] == 0)
+        residual_tile_body<T, NV, false>(S, q, nT, nC, area, lsq, xc, yc, wd, g, gg, second_order, limiter_k, viscous, R,
+                                         qghost, t, smem_raw);
+    else
+        residual_tile_body<T, NV, true>(S, q, nT, nC, area, lsq, xc, yc, wd, g, gg, second_order, limiter_k, viscous, R,
+                                        qghost, t, smem_raw);
+}
 
 // ---------------------------------------------------------------------------
 // FP32 throughput form of the fused tile residual.  Same algorithm as
@@ -1090,6 +1108,12 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
     float4* __restrict__ qghost = A.qghost;
     const SAFields32 sa = A.sa;
     float* __restrict__ dt_out = STAGE ? A.dt : nullptr;   // the local time step belongs to the stage form
+    // wavefront roles rotate with the tile: the waves of a workgroup sit on
+    // the CU's SIMDs in order, so the partial last chunk of each phase falls
+    // on a different SIMD for consecutive tiles instead of always the same
+    constexpr int kWaves = kBlock / 64;
+    static_assert((kWaves & (kWaves - 1)) == 0, "power-of-two wavefronts per workgroup");
+    const int tid = ((((int)threadIdx.x >> 6) + t) & (kWaves - 1)) * 64 + ((int)threadIdx.x & 63);
     const int c0g = S.cell0[t], nTt = S.cell0[t + 1] - c0g;
     const int lo = S.loff[t], nL = S.loff[t + 1] - lo;
     const int nG = S.ng[t];
@@ -1114,7 +1138,7 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
     const float4* geo = S.geo + 2 * (int64_t)fo;
 
     // 1. stage states
-    for (int i = threadIdx.x; i < nL; i += blockDim.x) {
+    for (int i = tid; i < nL; i += blockDim.x) {
         const int gi = Lg[i];
         if (gi >= 0) {   // wall/slip ghosts (flagged < 0) are never read
             sq[i] = q[gi];
@@ -1125,7 +1149,7 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
 
     // 2. gradients and limiters of T u H1
     const uint2* slots2 = reinterpret_cast<const uint2*>(S.slots) + go;
-    for (int c = threadIdx.x; c < nG; c += blockDim.x) {
+    for (int c = tid; c < nG; c += blockDim.x) {
         const uint2 sw = slots2[c];
         const int gc = Lg[c];
         const float4 qc = sq[c];
@@ -1289,7 +1313,7 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
     __syncthreads();
 
     // 3. fluxes of the faces touching T
-    for (int fl = threadIdx.x; fl < nFT; fl += blockDim.x) {
+    for (int fl = tid; fl < nFT; fl += blockDim.x) {
         const uint32_t fc = fck[fl];
         const int o = (int)(fc & 0xFFFu), n = (int)((fc >> 12) & 0xFFFu);
         const int kind = BND ? (int)(fc >> 24) : (int)K_INTERNAL;
@@ -1413,7 +1437,7 @@ __device__ __forceinline__ void tile_residual_f32(const TileArgsF32& A, const in
     __syncthreads();
 
     // 4. signed sums in reference face order (exact FP64 accumulation of the FP32 fluxes)
-    for (int c = threadIdx.x; c < nTt; c += blockDim.x) {
+    for (int c = tid; c < nTt; c += blockDim.x) {
         const uint2 sw = slots2[c];
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
         float lsum = 0.f, vsum = 0.f;

@@ -126,7 +126,10 @@ __global__ void __launch_bounds__(kBlock) k_assemble_diag(CellsT<double> C, Face
     for (int k = 0; k < 16; ++k) Dinv[(size_t)16 * c + k] = X[k];
 }
 
-// y = LHS x (ghost rows identity; ghost columns dropped since x_ghost = 0)
+// y = LHS x (ghost rows identity; ghost columns dropped since x_ghost = 0).
+// The cell's four slot words, then its faces' kinds and neighbours, then the
+// blocks and the neighbours' x are loaded as independent batches, so the
+// gathers of all slots are in flight together.
 __global__ void __launch_bounds__(kBlock) k_lhs_apply(CellsT<double> C, FacesT<double> F, const double* __restrict__ D,
                                                       const double* __restrict__ Jf, const double* __restrict__ x,
                                                       double* __restrict__ y, const int* __restrict__ active) {
@@ -138,28 +141,37 @@ __global__ void __launch_bounds__(kBlock) k_lhs_apply(CellsT<double> C, FacesT<d
         for (int k = 0; k < 4; ++k) y[k * nT + c] = x[k * nT + c];
         return;
     }
+    uint32_t e[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) e[s] = C.cface[(size_t)s * C.nC + c];
+    int other[4];
+    bool use[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        use[s] = false;
+        other[s] = 0;
+        if (e[s] != 0xFFFFFFFFu) {
+            const int f = (int)(e[s] & 0x7FFFFFFFu);
+            use[s] = F.kind[f] == K_INTERNAL;
+            other[s] = (e[s] >> 31) ? F.c0[f] : F.c1[f];
+        }
+    }
     double xc[4], acc[4];
     for (int k = 0; k < 4; ++k) xc[k] = x[k * nT + c];
     const double* B = D + (size_t)16 * c;
     for (int r = 0; r < 4; ++r) acc[r] = B[4 * r] * xc[0] + B[4 * r + 1] * xc[1] + B[4 * r + 2] * xc[2] + B[4 * r + 3] * xc[3];
+#pragma unroll
     for (int s = 0; s < 4; ++s) {
-        uint32_t e = C.cface[(size_t)s * C.nC + c];
-        if (e == 0xFFFFFFFFu) break;
-        const int f = (int)(e & 0x7FFFFFFFu);
-        if (F.kind[f] != K_INTERNAL) continue;
-        const double* J = Jf + (size_t)32 * f;
-        if (e >> 31) {   // this cell is the neighbour: block (c1, c0) = -J00
-            const int o = F.c0[f];
-            double xo[4];
-            for (int k = 0; k < 4; ++k) xo[k] = x[k * nT + o];
-            for (int r = 0; r < 4; ++r) acc[r] -= J[4 * r] * xo[0] + J[4 * r + 1] * xo[1] + J[4 * r + 2] * xo[2] + J[4 * r + 3] * xo[3];
-        } else {         // owner: block (c0, c1) = J01
-            const int n = F.c1[f];
-            double xn[4];
-            for (int k = 0; k < 4; ++k) xn[k] = x[k * nT + n];
-            for (int r = 0; r < 4; ++r)
-                acc[r] += J[16 + 4 * r] * xn[0] + J[16 + 4 * r + 1] * xn[1] + J[16 + 4 * r + 2] * xn[2] + J[16 + 4 * r + 3] * xn[3];
-        }
+        if (!use[s]) continue;
+        const int f = (int)(e[s] & 0x7FFFFFFFu);
+        const bool nb = (e[s] >> 31) != 0;
+        // neighbour row: block (c1, c0) = -J00; owner row: block (c0, c1) = J01
+        const double* J = Jf + (size_t)32 * f + (nb ? 0 : 16);
+        const double sg = nb ? -1.0 : 1.0;
+        double xo[4];
+        for (int k = 0; k < 4; ++k) xo[k] = sg * x[k * nT + other[s]];
+        for (int r = 0; r < 4; ++r)
+            acc[r] += J[4 * r] * xo[0] + J[4 * r + 1] * xo[1] + J[4 * r + 2] * xo[2] + J[4 * r + 3] * xo[3];
     }
     for (int k = 0; k < 4; ++k) y[k * nT + c] = acc[k];
 }
@@ -264,20 +276,37 @@ __global__ void __launch_bounds__(kBlock) k_dilu_sweep(CellsT<double> C, FacesT<
     if (k >= ncells) return;
     const size_t nT = (size_t)C.nT;
     const int i = cells[k];
+    uint32_t e[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) e[s] = C.cface[(size_t)s * C.nC + i];
+    int other[4];
+    bool use[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        use[s] = false;
+        other[s] = 0;
+        if (e[s] != 0xFFFFFFFFu) {
+            const int f = (int)(e[s] & 0x7FFFFFFFu);
+            if (F.kind[f] == K_INTERNAL) {
+                other[s] = (e[s] >> 31) ? F.c0[f] : F.c1[f];
+                const int cj = colour[other[s]];
+                use[s] = FORWARD ? cj < c : cj > c;
+            }
+        }
+    }
     double acc[4];
     for (int r = 0; r < 4; ++r) acc[r] = FORWARD ? b[r * nT + i] : 0.0;
     bool any = FORWARD;
+#pragma unroll
     for (int s = 0; s < 4; ++s) {
-        const uint32_t e = C.cface[(size_t)s * C.nC + i];
-        if (e == 0xFFFFFFFFu) break;
-        if (F.kind[e & 0x7FFFFFFFu] != K_INTERNAL) continue;
-        int j;
-        double sa;
-        const double* A = offdiag(F, Jf, e, &j, &sa);
-        if (FORWARD ? colour[j] >= c : colour[j] <= c) continue;
+        if (!use[s]) continue;
         any = true;
+        const int f = (int)(e[s] & 0x7FFFFFFFu);
+        const bool nb = (e[s] >> 31) != 0;
+        const double* A = Jf + (size_t)32 * f + (nb ? 0 : 16);   // A_ij: -J00 (i the neighbour) or J01
+        const double sa = nb ? -1.0 : 1.0;
         double yj[4];
-        for (int r = 0; r < 4; ++r) yj[r] = sa * y[r * nT + j];
+        for (int r = 0; r < 4; ++r) yj[r] = sa * y[r * nT + other[s]];
         for (int r = 0; r < 4; ++r)
             acc[r] += (FORWARD ? -1.0 : 1.0) * (A[4 * r] * yj[0] + A[4 * r + 1] * yj[1] + A[4 * r + 2] * yj[2] + A[4 * r + 3] * yj[3]);
     }

@@ -63,6 +63,7 @@ struct TilesT {
     const uint32_t* fcell;   // per local face: c0 | c1 << 16 (local cells)
     const int8_t* fkind;     // per local face
     const T* geo;            // 8 x nFtot: nx, ny, len, gw, d0x, d0y, d1x, d1y (d = x_f - x_cell)
+    const int32_t* nbf;      // ntiles: boundary faces in the tile's list (0: interior tile); nullable
     int64_t nFtot;
     int32_t ntiles;
     int32_t lmax, gmax, ftmax;
