@@ -6,8 +6,8 @@ handle is released by close(), by leaving a `with` block, or, for handles the
 caller never closed, by close_all(), which runs from Python's atexit at the
 start of interpreter shutdown.  Releasing from there, rather than from
 __del__ while modules are being torn down, keeps every hipFree, stream and
-rocBLAS-handle destruction ahead of the HIP runtime's own exit handlers and of
-any profiler that finalises at process exit.
+event destruction ahead of the HIP runtime's own exit handlers and of any
+profiler that finalises at process exit.
 
 close_all() releases live handles newest first: a handle that refers to other
 handles (a batch to its solvers, a mapper to its two solvers, a coupling to
