@@ -245,6 +245,7 @@ public:
 struct Settings {
     double start_cfl = 40., slope_cfl = 50., max_cfl = 100., relaxation = 0.9, tolerance = 1e-4;
     int rhs_iterations = 5, max_iterations = 300;
+    std::string airfoil_name = "wall";   // the patch of the loop's Cp profile (core.h:196)
 };
 
 struct wallProfile {
@@ -289,10 +290,13 @@ inline void save(const std::string& filename, solver& s) { check(afx_rans_save_v
 
 // multigrid<implicitSolver>::run_solver (multigrid.h:240-293); returns 0 on
 // success, 1 when the solver reports failure or the residual is not finite;
-// residuals receives the
-// normalised residual of every iteration
+// residuals receives the normalised residual of every iteration.  With a
+// profile, profile->calc_cp(s, st.airfoil_name) after every
+// profile_every-th iteration (the reference refreshes it every iteration,
+// multigrid.h:286; the gather runs on the device).
 inline int run_solver(solver& s, const Settings& st, std::vector<double>& residuals,
-                      const std::function<bool()>& stop = nullptr) {
+                      const std::function<bool()>& stop = nullptr, CpProfile* profile = nullptr,
+                      int profile_every = 1) {
     double cfl = st.start_cfl;
     int i = 0;
     double err = 0;
@@ -313,6 +317,7 @@ inline int run_solver(solver& s, const Settings& st, std::vector<double>& residu
         // a non-finite normalised residual is a failure (the reference's
         // loop would read NaN as converged)
         if (err < 0 || !std::isfinite(err)) return 1;
+        if (profile && i % profile_every == 0) profile->calc_cp(s, st.airfoil_name);
         residuals.push_back(err);
     } while ((err > st.tolerance) && (i < st.max_iterations) && !(stop && stop()));
     return 0;
@@ -320,9 +325,12 @@ inline int run_solver(solver& s, const Settings& st, std::vector<double>& residu
 
 // multigrid<explicitSolver>::run_solver (multigrid.h:183-237): the CFL stays
 // at its start value and the residual is normalised only when positive;
-// returns 0 on success, 1 on failure or a non-finite residual
+// returns 0 on success, 1 on failure or a non-finite residual.  A profile is
+// refreshed as in the implicit loop (an addition: the reference's explicit
+// loop refreshes none).
 inline int run_solver(explicitSolver& s, const Settings& st, std::vector<double>& residuals,
-                      const std::function<bool()>& stop = nullptr) {
+                      const std::function<bool()>& stop = nullptr, CpProfile* profile = nullptr,
+                      int profile_every = 1) {
     int i = 0;
     double err = 0;
     double err_0 = s.get_uniform_residual();
@@ -340,6 +348,7 @@ inline int run_solver(explicitSolver& s, const Settings& st, std::vector<double>
         if (err < 0 || !std::isfinite(err)) return 1;
         residuals.push_back(err);
         i++;
+        if (profile && i % profile_every == 0) profile->calc_cp(s, st.airfoil_name);
     } while ((err > st.tolerance) && (i < st.max_iterations) && !(stop && stop()));
     return 0;
 }
@@ -369,11 +378,12 @@ public:
 // multigrid constructor does (multigrid.h:70-77): level 0 from freestream
 // (reinit) and its boundary states, then each finer level from the mapped
 // solution of the previous one.  Returns the level the loop stopped on;
-// state receives run_solver's result for it.
+// state receives run_solver's result for it; profile / profile_every are
+// handed to run_solver.
 template <class S>
 S& multigrid_run(std::vector<S*>& levels, const Settings& st, std::vector<double>& residuals, int* state = nullptr,
                  bool reinit = true, const std::function<bool()>& stop = nullptr,
-                 std::vector<mapper>* prebuilt = nullptr) {
+                 std::vector<mapper>* prebuilt = nullptr, CpProfile* profile = nullptr, int profile_every = 1) {
     std::vector<mapper> own;
     if (!prebuilt) {
         own.reserve(levels.size());
@@ -388,7 +398,7 @@ S& multigrid_run(std::vector<S*>& levels, const Settings& st, std::vector<double
             mappers[i - 1].prolongate();
             levels[i]->refill_bcs();
         }
-        int rc = run_solver(*levels[i], st, residuals, stop);
+        int rc = run_solver(*levels[i], st, residuals, stop, profile, profile_every);
         if (state) *state = rc;
         if (rc || (stop && stop())) return *levels[i];
     }

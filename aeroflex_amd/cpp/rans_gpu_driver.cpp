@@ -8,7 +8,8 @@
 // A comma-separated list of meshes (coarse to fine) runs the multigrid loop
 // of multigrid<T>::run (multigrid.h:296-363) over them.
 // Prints one JSON object: iterations, residual history, CL, CD, CM (of the
-// last level run).
+// last level run), and the size and minimum of the wall's Cp profile, which
+// the loop refreshes every iteration (multigrid.h:286).
 //
 //   rans_gpu_driver polar <meshes> <mach> <alpha_start> <alpha_end> <alpha_step>
 //                   <iterations> [implicit|explicit] [wing_alpha_deg]
@@ -20,6 +21,7 @@
 // rectangular AR 8 wing (16 x 32 rings) at wing_alpha_deg (default 3).  Each
 // angle runs to a normalised residual of 1e-12 or the iteration count.
 // Prints the polar and the wing's iterations, convergence and CL.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -174,11 +176,16 @@ int main(int argc, char** argv) {
         }
         std::vector<double> res;
         int state = 0;
-        rans::gpu::solver& last = rans::gpu::multigrid_run(levels, st, res, &state);
+        // the wall's Cp profile refreshed every iteration, as multigrid.h:286 does
+        rans::gpu::CpProfile prof;
+        rans::gpu::solver& last = rans::gpu::multigrid_run(levels, st, res, &state, true, nullptr, nullptr, &prof);
         rans::gpu::wallProfile w = rans::gpu::get_wall_profile(last, "wall");
+        double cp_min = 0.0;
+        for (double c : prof.cp) cp_min = std::min(cp_min, c);
         std::printf("{\"state\": %d, \"iterations\": %zu, \"residuals\": [", state, res.size());
         for (size_t i = 0; i < res.size(); ++i) std::printf("%s%.17g", i ? ", " : "", res[i]);
-        std::printf("], \"cl\": %.17g, \"cd\": %.17g, \"cm\": %.17g}\n", w.cl, w.cd, w.cm);
+        std::printf("], \"cl\": %.17g, \"cd\": %.17g, \"cm\": %.17g, \"cp_faces\": %zu, \"cp_min\": %.17g}\n",
+                    w.cl, w.cd, w.cm, prof.cp.size(), cp_min);
         return state;
     } catch (std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());

@@ -725,6 +725,21 @@ class Settings:
         self.airfoil_name = airfoil_name
 
 
+class CpProfile:
+    """CpProfile (post.h:182-290) without the GUI's plotting arrays: the wall
+    patch's face centres x, y and cp, refreshed by calc_cp from the device
+    state (a gather on the device, then one copy of three short vectors).
+    `updates` counts the refreshes."""
+
+    def __init__(self):
+        self.x = self.y = self.cp = np.empty(0)
+        self.updates = 0
+
+    def calc_cp(self, s, patch="wall"):
+        self.x, self.y, self.cp = s.cp_profile(patch)
+        self.updates += 1
+
+
 class Mapper(Handle):
     """multigrid<T>::gen_mapper (multigrid.h:96-163), built on the device:
     the inverse-distance map from the coarse solver's cells to the fine
@@ -825,11 +840,14 @@ def solve_airfoil(levels, settings, db, solver_type="implicit", patch="wall", sa
     return db
 
 
-def run_solver_explicit(s, settings, residuals=None, stop=None):
+def run_solver_explicit(s, settings, residuals=None, stop=None, profile=None, profile_every=1):
     """multigrid<explicitSolver>::run_solver (multigrid.h:183-237): the CFL
     stays at its start value, the residual is normalised only when positive,
-    the loop stops on tolerance / iteration count / stop flag.  Returns
-    (state, residuals): state 0 ok, 1 failure or a non-finite residual."""
+    the loop stops on tolerance / iteration count / stop flag.  With a
+    CpProfile, calc_cp on settings.airfoil_name after every profile_every-th
+    iteration, as the implicit loop does (an addition here: the reference's
+    explicit loop refreshes no profile).  Returns (state, residuals): state 0
+    ok, 1 failure or a non-finite residual."""
     residuals = [] if residuals is None else residuals
     i = 0
     err_0 = s.get_uniform_residual()
@@ -844,11 +862,14 @@ def run_solver_explicit(s, settings, residuals=None, stop=None):
             return 1, residuals
         residuals.append(err)
         i += 1
+        if profile is not None and i % profile_every == 0:
+            profile.calc_cp(s, settings.airfoil_name)
         if not (err > settings.tolerance and i < settings.max_iterations and not (stop and stop())):
             return 0, residuals
 
 
-def multigrid_run(solvers, settings, reinit=True, mappers=None, residuals=None, stop=None, loop=None):
+def multigrid_run(solvers, settings, reinit=True, mappers=None, residuals=None, stop=None, loop=None, profile=None,
+                  profile_every=1):
     """multigrid<implicitSolver>::run (multigrid.h:317-363): level 0 from
     freestream (reinit) and its boundary states, then for each finer level
     the coarse solution mapped over (bcs_from_internal, q_fine = M q_coarse,
@@ -866,17 +887,20 @@ def multigrid_run(solvers, settings, reinit=True, mappers=None, residuals=None, 
             solvers[i - 1].bcs_from_internal()
             mappers[i - 1].prolongate()
             s.refill_bcs()
-        state, _ = (loop or run_solver)(s, settings, residuals, stop)
+        state, _ = (loop or run_solver)(s, settings, residuals, stop, profile, profile_every)
         if state or (stop and stop()):
             return s, state
     return solvers[-1], 0
 
 
-def run_solver(s, settings, residuals=None, stop=None):
+def run_solver(s, settings, residuals=None, stop=None, profile=None, profile_every=1):
     """multigrid<implicitSolver>::run_solver (multigrid.h:240-293) over the
     device solver: CFL ramp, first-iteration normalisation rebase, stop on
-    tolerance / iteration count / stop flag.  Returns (state, residuals):
-    state 0 ok, 1 linear-solver failure or a non-finite residual."""
+    tolerance / iteration count / stop flag; with a CpProfile, calc_cp on
+    settings.airfoil_name after every profile_every-th iteration (the
+    reference: every iteration, multigrid.h:286).  Returns (state,
+    residuals): state 0 ok, 1 linear-solver failure or a non-finite
+    residual."""
     residuals = [] if residuals is None else residuals
     cfl = settings.start_cfl
     i = 0
@@ -899,6 +923,8 @@ def run_solver(s, settings, residuals=None, stop=None):
         # loop would read NaN as converged)
         if err < 0 or not np.isfinite(err):
             return 1, residuals
+        if profile is not None and i % profile_every == 0:
+            profile.calc_cp(s, settings.airfoil_name)
         residuals.append(err)
         if not (err > settings.tolerance and i < settings.max_iterations and not (stop and stop())):
             return 0, residuals

@@ -301,6 +301,15 @@ def vtu_bench(rans, s, model, cpu=True):
         size = os.path.getsize(path)
         out = {"vtu_cells": s.n_real, "vtu_s": round(t, 3), "vtu_mb": round(size / 1e6, 1),
                "vtu_threads": max(1, min(16, os.cpu_count() or 1))}
+        # CpProfile::calc_cp (post.h:246-290), refreshed every iteration by
+        # the reference's loop: a gather of the wall faces on the device
+        s.cp_profile("wall")
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            x, _, cp = s.cp_profile("wall")
+        out["cp_profile_ms"] = round((time.perf_counter() - t0) * 1e3 / reps, 3)
+        out["cp_profile_faces"] = int(len(cp))
         if cpu:
             ni, nj = mesh_dims(120_000)
             m = rans.Mesh.jittered(ni, nj, r_far=20.0, thickness=0.12, first_h=1e-4, jitter=0.1, quad_frac=0.25,

@@ -221,6 +221,25 @@ def test_cpp_driver_matches_python_mirror():
     assert r["iterations"] == len(res)
     assert np.allclose(r["residuals"], res, rtol=1e-12, atol=0)
     assert abs(r["cl"] - s.get_wall_profile("wall")[0]) <= 1e-14
+    # the driver refreshes the wall's Cp profile every iteration (multigrid.h:286)
+    x, y, cp = s.cp_profile("wall")
+    assert r["cp_faces"] == len(cp) and abs(r["cp_min"] - min(0.0, cp.min())) <= 1e-12
+
+
+@pytest.mark.parametrize("every", [1, 4])
+def test_loop_refreshes_the_cp_profile(every):
+    """The iteration loop refreshes the CpProfile after every (or every
+    n-th) iteration, as multigrid.h:286 does each iteration; the last refresh
+    after the last iteration equals the final state's profile."""
+    from aeroflex_amd import rans
+    s = case_solver(rans.Mesh.read_file(QUAD), 0.5, 1.25 * 0.01745)
+    prof = rans.CpProfile()
+    st = rans.Settings(tolerance=1e-13, max_iterations=12)
+    state, res = rans.run_solver(s, st, profile=prof, profile_every=every)
+    assert state == 0 and len(res) == 12
+    assert prof.updates == len(res) // every
+    x, y, cp = s.cp_profile("wall")
+    assert np.array_equal(prof.cp, cp) and np.array_equal(prof.x, x)
 
 
 def test_cpp_driver_errors_are_exceptions():
