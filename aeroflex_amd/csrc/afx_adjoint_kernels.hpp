@@ -71,6 +71,10 @@ struct GeoBufs {
 hipError_t launch_adjoint_state(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* w,
                                 GasT<double> g, int second_order, double limiter_k, AdjBufs A, double* lam,
                                 const GeoBufs* G = nullptr);
+// Green-Gauss gradients of the five variables of the Spalart-Allmaras model
+// (pass 1 of its adjoint without the limiters): gx, gy 5 planes, real cells
+hipError_t launch_sa_gradients(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, GasT<double> g,
+                              double* gx, double* gy);
 // state adjoint of the Spalart-Allmaras residual (five planes per array of A,
 // vgx / vgy / vdq required; wd: wall distance per device cell)
 hipError_t launch_adjoint_state_sa(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q,

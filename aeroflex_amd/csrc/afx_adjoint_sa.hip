@@ -407,6 +407,16 @@ __global__ void __launch_bounds__(kBlock) k_sadj_cell2(CellsT<double> C, FacesT<
 
 }  // namespace
 
+hipError_t launch_sa_gradients(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, GasT<double> g,
+                              double* gx, double* gy) {
+    AdjBufs A{};
+    A.gx = gx;
+    A.gy = gy;
+    const int nbc = (C.nC + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_sadj_forward, dim3(nbc), dim3(kBlock), 0, s, C, F, q, g, 0, 0.0, A);
+    return hipGetLastError();
+}
+
 hipError_t launch_adjoint_state_sa(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q,
                                    const double* w, const double* wd, GasT<double> g, int second_order,
                                    double limiter_k, AdjBufs A, double* lam) {

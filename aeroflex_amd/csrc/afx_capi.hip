@@ -97,6 +97,7 @@ struct afx_rans {
     DBuf<double> q, qk, qW, dt, gx, gy, lim, R, flux;
     DBuf<float> q32, gx32, gy32, lim32, R32, flux32;
     DBuf<double> partial, sum1, uni64, wall_d, states;
+    DBuf<double> sa_wd;   // the Spalart-Allmaras model's wall distance (compute_wall_distance)
     DBuf<float> uni32;
     DBuf<int32_t> w_owner, w_ghost, b_owner, b_ghost, patch_cells;
     int32_t n_walls = 0;
@@ -350,23 +351,37 @@ int run_gradients32(afx_rans* h) {
 
 // mesh::compute_wall_dist (mesh.h:794-830) on the device: distance from each
 // cell centroid to the nearest wall / slip-wall face centre, capped at 1.
-// With the SA model also fills the FP32 (area, distance) pairs.
+// With the SA model also the model's distance (sa_wd): to the nearest point
+// of the same faces' segments, which the face centres overestimate up to a/2
+// times on wall cells of aspect ratio a (and the model's destruction ~ 1/d^2
+// with it); and the FP32 (area, model distance) pairs.
 int compute_wall_distance(afx_rans* h) {
     const Mesh& m = h->m;
     std::vector<double> wx, wy;
+    std::vector<double4> seg;
     for (int32_t b = 0; b < h->nB; ++b)
         if (h->bface_type[b] == AFX_BC_WALL || h->bface_type[b] == AFX_BC_SLIP) {
-            wx.push_back(m.edgesCentersX[m.boundaryEdges[b]]);
-            wy.push_back(m.edgesCentersY[m.boundaryEdges[b]]);
+            const uint32_t e = m.boundaryEdges[b];
+            wx.push_back(m.edgesCentersX[e]);
+            wy.push_back(m.edgesCentersY[e]);
+            const uint32_t na = m.edgesNodes[2 * (size_t)e], nb = m.edgesNodes[2 * (size_t)e + 1];
+            seg.push_back(make_double4(m.nodesX[na], m.nodesY[na], m.nodesX[nb] - m.nodesX[na],
+                                       m.nodesY[nb] - m.nodesY[na]));
         }
     DBuf<double> dwx, dwy;
     HIPCHK(dwx.alloc(wx.size())); HIPCHK(dwx.upload(wx, h->stream));
     HIPCHK(dwy.alloc(wy.size())); HIPCHK(dwy.upload(wy, h->stream));
     HIPCHK(launch_wall_dist(h->stream, h->xc.p, h->yc.p, h->nT, dwx.p, dwy.p, (int32_t)wx.size(), h->wall_d.p));
+    DBuf<double4> dseg;
+    if (h->sa()) {
+        if (!h->sa_wd.p) HIPCHK(h->sa_wd.alloc(h->nT));
+        HIPCHK(dseg.alloc(seg.size())); HIPCHK(dseg.upload(seg, h->stream));
+        HIPCHK(launch_wall_dist_seg(h->stream, h->xc.p, h->yc.p, h->nT, dseg.p, (int32_t)seg.size(), h->sa_wd.p));
+    }
     HIPCHK(hipStreamSynchronize(h->stream));
     if (h->sa_ad32.p) {
         std::vector<double> d(h->nT);
-        HIPCHK(hipMemcpyAsync(d.data(), h->wall_d.p, d.size() * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(d.data(), h->sa_wd.p, d.size() * 8, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         std::vector<float2> ad(h->nT);
         for (int32_t i = 0; i < h->nT; ++i) ad[i] = make_float2((float)m.cellsAreas[h->c_new2old[i]], (float)d[i]);
@@ -383,7 +398,7 @@ int enqueue_residual(afx_rans* h, int precision) {
     h->last_precision = precision;
     if (precision == AFX_FP64 && fused && h->tile64_ok) {
         HIPCHK(launch_residual_tile<double>(h->stream, h->tiles<double>(h->t_geo64.p), h->q.p, h->nT, h->nC, h->area.p,
-                                           h->lsq.p, h->xc.p, h->yc.p, h->wall_d.p, h->gas64(), h->opts.gradient,
+                                           h->lsq.p, h->xc.p, h->yc.p, h->sa_wd.p, h->gas64(), h->opts.gradient,
                                            h->so(), h->opts.limiter_k, h->opts.viscosity, h->R.p, h->q.p));
         return AFX_OK;
     }
@@ -437,15 +452,48 @@ int check_handle(afx_rans* h, bool need_bcs = true) {
 
 
 // ------------------------------------------------------------ implicit step --
+// the Spalart-Allmaras implicit step: the assembled operator (no matrix-free
+// form), Green-Gauss gradients (those its Jacobian freezes), the fused
+// residual, the wall distance of the source
+int sa_implicit_check(afx_rans* h) {
+    if (h->mf) return fail(AFX_EUNSUP, "the matrix-free operator covers the four-variable residual");
+    if (h->opts.gradient != 1) return fail(AFX_EUNSUP, "the Spalart-Allmaras implicit step covers Green-Gauss gradients");
+    if (!h->tile64_ok) return fail(AFX_EUNSUP, "the Spalart-Allmaras residual needs the fused tile kernels");
+    if (!h->wd_ready) return compute_wall_distance(h);
+    return AFX_OK;
+}
+// an operand of the implicit operator into ktmp: x AoS (4 per cell), x_nu the
+// rho nu~ row (the Spalart-Allmaras model only, where it is required)
+int upload_operand(afx_rans* h, const double* x, const double* x_nu, const double* y_nu) {
+    if (h->sa() != (x_nu != nullptr) || h->sa() != (y_nu != nullptr))
+        return fail(AFX_EARG, h->sa() ? "the Spalart-Allmaras operator needs the rho nu~ rows x_nu and y_nu"
+                                      : "rho nu~ rows only with the Spalart-Allmaras model");
+    int rc = upload_aos(h, x, h->ktmp.p);
+    if (rc || !x_nu) return rc;
+    std::vector<double> tmp(h->nT);
+    for (int32_t i = 0; i < h->nT; ++i) tmp[i] = x_nu[h->c_new2old[i]];
+    HIPCHK(hipMemcpyAsync(h->ktmp.p + 4 * (size_t)h->nT, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return AFX_OK;
+}
+int download_operand(afx_rans* h, double* y, double* y_nu) {
+    int rc = download_aos(h, h->kw.p, y);
+    if (rc || !y_nu) return rc;
+    return download_cells(h, h->kw.p + 4 * (size_t)h->nT, y_nu);
+}
+
 int ensure_implicit(afx_rans* h) {
     if (h->mf && !h->mfFb.p) HIPCHK(h->mfFb.alloc((size_t)4 * h->nF));
     if (h->Jf.p) return AFX_OK;
-    const size_t n = (size_t)NV * h->nT;
+    // blocks of nv x nv (five with the Spalart-Allmaras model), vectors of nv planes
+    const int nv = h->nv;
+    const size_t nn = (size_t)nv * nv;
+    const size_t n = (size_t)nv * h->nT;
     const int m = h->gm_restart;
     h->kblk = 512;
-    HIPCHK(h->Jf.alloc((size_t)32 * h->nF));
-    HIPCHK(h->Dblk.alloc((size_t)16 * h->nC));
-    HIPCHK(h->Dinv.alloc((size_t)16 * h->nC));
+    HIPCHK(h->Jf.alloc(2 * nn * h->nF));
+    HIPCHK(h->Dblk.alloc(nn * h->nC));
+    HIPCHK(h->Dinv.alloc(nn * h->nC));
     HIPCHK(h->kV.alloc((size_t)(m + 1) * n));
     HIPCHK(h->kw.alloc(n));
     HIPCHK(h->kz.alloc(n));
@@ -455,8 +503,8 @@ int ensure_implicit(afx_rans* h) {
     HIPCHK(h->kg.alloc(afx_rans::gm_doubles(m)));
     HIPCHK(h->kflags.alloc(8));
     HIPCHK(h->kbad.alloc(1));
-    HIPCHK(h->Dti.alloc((size_t)16 * h->nC));
-    HIPCHK(hipMemsetAsync(h->Jf.p, 0, (size_t)32 * h->nF * 8, h->stream));
+    HIPCHK(h->Dti.alloc(nn * h->nC));
+    HIPCHK(hipMemsetAsync(h->Jf.p, 0, 2 * nn * h->nF * 8, h->stream));
     HIPCHK(hipMemsetAsync(h->kflags.p, 0, 8 * sizeof(int), h->stream));
     // greedy colouring of the real cells in device order over their
     // real-cell face neighbours (at most four), for the DILU sweeps
@@ -499,13 +547,13 @@ int ensure_implicit(afx_rans* h) {
 }
 
 // Right-preconditioned restarted GMRES (classical Gram-Schmidt applied twice)
-// for LHS x = b; b and x are device vectors of length 4*nT.  The Arnoldi
+// for LHS x = b; b and x are device vectors of nv planes of nT.  The Arnoldi
 // state stays on the device (GmDev): a restart cycle of m steps is enqueued
 // at once, the kernels of a step do nothing once the iteration has
 // converged or broken down, and the host reads the flags once per cycle.
 // Returns AFX_OK and *converged, or a negative error code.
 int gmres(afx_rans* h, const double* b, double* x, bool* converged) {
-    const int64_t n = (int64_t)NV * h->nT;
+    const int64_t n = (int64_t)h->nv * h->nT;
     const int m = h->gm_restart;
     const int nb = h->kblk;
     hipStream_t s = h->stream;
@@ -521,15 +569,15 @@ int gmres(afx_rans* h, const double* b, double* x, bool* converged) {
         if (h->mf)
             return launch_mf_apply(s, h->cells64(), h->faces64(), h->q.p, h->dt.p, h->mfFb.p, h->gas64(),
                                    h->opts.viscosity, in, out, act);
-        return launch_lhs_apply(s, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, in, out, act);
+        return launch_lhs_apply(s, h->nv, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, in, out, act);
     };
     auto precond = [&](const double* in, double* out, const double** res, const int* act) -> hipError_t {
         if (h->mf) { *res = in; return hipSuccess; }
         *res = out;
         if (h->precond == 1)
-            return launch_dilu_apply(s, h->cells64(), h->faces64(), h->colour.p, h->col_cells.p, h->col_off.data(),
-                                     (int)h->col_off.size() - 1, h->Jf.p, h->Dti.p, in, out, act);
-        return launch_block_jacobi(s, h->nC, h->nT, h->Dinv.p, in, out, act);
+            return launch_dilu_apply(s, h->nv, h->cells64(), h->faces64(), h->colour.p, h->col_cells.p,
+                                     h->col_off.data(), (int)h->col_off.size() - 1, h->Jf.p, h->Dti.p, in, out, act);
+        return launch_block_jacobi(s, h->nv, h->nC, h->nT, h->Dinv.p, in, out, act);
     };
     // |r|^2 of the cycle's start vector into hdr[3], then the cycle's set-up
     auto begin = [&](int first) -> hipError_t {
@@ -1436,7 +1484,7 @@ int afx_rans_explicit_step(afx_rans* h, double relaxation, double* err) {
         HIPCHK(launch_copy(s, h->q.p, h->qk.p, nq));
         for (double ai : alpha5) {
             HIPCHK(launch_residual_tile<double>(s, h->tiles<double>(h->t_geo64.p), h->qk.p, h->nT, h->nC, h->area.p,
-                                                h->lsq.p, h->xc.p, h->yc.p, h->wall_d.p, h->gas64(), h->opts.gradient,
+                                                h->lsq.p, h->xc.p, h->yc.p, h->sa_wd.p, h->gas64(), h->opts.gradient,
                                                 h->so(), h->opts.limiter_k, h->opts.viscosity, h->R.p,
                                                 h->qk.p));
             HIPCHK(launch_rk_update_area(s, h->q.p, h->R.p, h->area.p, h->dt.p, ai * relaxation, h->nC, h->nT, h->nv,
@@ -1480,10 +1528,10 @@ int afx_rans_set_preconditioner(afx_rans* h, int32_t kind) {
 int afx_rans_implicit_fill(afx_rans* h) {
     int rc = check_handle(h);
     if (rc) return rc;
-    if (h->sa()) return fail(AFX_EUNSUP, "the implicit step is not available with the Spalart-Allmaras model");
+    if (h->sa() && (rc = sa_implicit_check(h))) return rc;
     if ((rc = ensure_implicit(h))) return rc;
     hipStream_t s = h->stream;
-    HIPCHK(launch_dt<double>(s, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->cfl, h->dt.p));
+    HIPCHK(launch_dt<double>(s, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->cfl, h->dt.p, h->sa() ? 1 : 0));
     if (h->mf) {
         // fillRhoLHS of the matrix-free build assembles nothing: the operator
         // reads q and dt when applied.  (Its gradient evaluation feeds no
@@ -1494,14 +1542,26 @@ int afx_rans_implicit_fill(afx_rans* h) {
         h->q32_valid = false;
         return AFX_OK;
     }
-    if (h->opts.viscosity != 0 && (rc = run_gradients64(h, h->q.p, h->q.p))) return rc;
-    HIPCHK(launch_face_jacobian(s, h->cells64(), h->faces64(), h->q.p, h->gx.p, h->gy.p, h->gas64(),
-                                h->opts.viscosity, h->Jf.p));
+    SaSource src{nullptr, nullptr, nullptr, nullptr, h->gas64()};
+    if (h->sa()) {
+        // wall / slip ghosts copy their owners (all five planes), then the
+        // unlimited Green-Gauss gradients of the five variables
+        HIPCHK(launch_set_walls<double>(s, h->w_owner.p, h->w_ghost.p, h->n_walls, h->q.p, h->nT, h->nv));
+        HIPCHK(launch_sa_gradients(s, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->gx.p, h->gy.p));
+        HIPCHK(launch_face_jacobian_sa(s, h->cells64(), h->faces64(), h->q.p, h->gx.p, h->gy.p, h->gas64(), h->Jf.p));
+        src = SaSource{h->q.p, h->gx.p, h->gy.p, h->sa_wd.p, h->gas64()};
+    } else {
+        if (h->opts.viscosity != 0 && (rc = run_gradients64(h, h->q.p, h->q.p))) return rc;
+        HIPCHK(launch_face_jacobian(s, h->cells64(), h->faces64(), h->q.p, h->gx.p, h->gy.p, h->gas64(),
+                                    h->opts.viscosity, h->Jf.p));
+    }
     HIPCHK(hipMemsetAsync(h->kbad.p, 0, sizeof(int), s));
-    HIPCHK(launch_assemble_diag(s, h->cells64(), h->faces64(), h->dt.p, h->Jf.p, h->Dblk.p, h->Dinv.p, h->kbad.p));
+    HIPCHK(launch_assemble_diag(s, h->nv, h->cells64(), h->faces64(), h->dt.p, h->Jf.p, src, h->Dblk.p, h->Dinv.p,
+                                h->kbad.p));
     if (h->precond == 1)
-        HIPCHK(launch_dilu_setup(s, h->cells64(), h->faces64(), h->colour.p, h->col_cells.p, h->col_off.data(),
-                                 (int)h->col_off.size() - 1, h->Dblk.p, h->Jf.p, h->Dti.p, h->kbad.p));
+        HIPCHK(launch_dilu_setup(s, h->nv, h->cells64(), h->faces64(), h->colour.p, h->col_cells.p,
+                                 h->col_off.data(), (int)h->col_off.size() - 1, h->Dblk.p, h->Jf.p, h->Dti.p,
+                                 h->kbad.p));
     HIPCHK(hipMemcpyAsync(&h->lhs_bad, h->kbad.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     h->lhs_ready = true;
@@ -1524,7 +1584,7 @@ int afx_rans_implicit_solve(afx_rans* h, double relaxation, double tol, int32_t 
     if (!h->lhs_ready) return fail(AFX_ESTATE, "call afx_rans_implicit_fill first");
     if (precision != AFX_FP64) return fail(AFX_EUNSUP, "the implicit step uses the FP64 residual");
     hipStream_t s = h->stream;
-    const int64_t n = (int64_t)NV * h->nT;
+    const int64_t n = (int64_t)h->nv * h->nT;
     h->lin_iters = 0;
     auto residual_norm = [&](double* e) -> int {
         // fillRhoRHS starts with calc_dt (solver.h:1081); only the
@@ -1532,7 +1592,7 @@ int afx_rans_implicit_solve(afx_rans* h, double relaxation, double tol, int32_t 
         if (h->mf) HIPCHK(launch_dt<double>(s, h->cells64(), h->faces64(), h->q.p, h->gas64(), h->cfl, h->dt.p));
         int r2 = enqueue_residual(h, AFX_FP64);
         if (r2) return r2;
-        return norm_of(h, h->R.p, h->nT, NV, e);
+        return norm_of(h, h->R.p, h->nT, h->nv, e);
     };
     auto step = [&](bool* ok) -> int {
         if (h->mf)
@@ -1573,29 +1633,38 @@ int afx_rans_implicit_solve(afx_rans* h, double relaxation, double tol, int32_t 
     return AFX_OK;
 }
 
-int afx_rans_lhs_apply(afx_rans* h, const double* x, double* y) {
+int afx_rans_lhs_apply_nu(afx_rans* h, const double* x, const double* x_nu, double* y, double* y_nu) {
     int rc = check_handle(h);
     if (rc) return rc;
+    if (!x || !y) return fail(AFX_EARG, "null vector");
     if (!h->lhs_ready) return fail(AFX_ESTATE, "call afx_rans_implicit_fill first");
     if (h->mf) return fail(AFX_ESTATE, "no assembled matrix in matrix-free mode (afx_rans_matrix_free_apply)");
-    if ((rc = upload_aos(h, x, h->ktmp.p))) return rc;
-    HIPCHK(launch_lhs_apply(h->stream, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, h->ktmp.p, h->kw.p));
-    return download_aos(h, h->kw.p, y);
+    if ((rc = upload_operand(h, x, x_nu, y_nu))) return rc;
+    HIPCHK(launch_lhs_apply(h->stream, h->nv, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, h->ktmp.p, h->kw.p));
+    return download_operand(h, y, y_nu);
 }
 
-int afx_rans_precond_apply(afx_rans* h, const double* x, double* y) {
+int afx_rans_lhs_apply(afx_rans* h, const double* x, double* y) {
+    return afx_rans_lhs_apply_nu(h, x, nullptr, y, nullptr);
+}
+
+int afx_rans_precond_apply_nu(afx_rans* h, const double* x, const double* x_nu, double* y, double* y_nu) {
     int rc = check_handle(h);
     if (rc) return rc;
     if (!x || !y) return fail(AFX_EARG, "null vector");
     if (!h->lhs_ready) return fail(AFX_ESTATE, "call afx_rans_implicit_fill first");
     if (h->mf) return fail(AFX_ESTATE, "the matrix-free mode has no preconditioner");
-    if ((rc = upload_aos(h, x, h->ktmp.p))) return rc;
+    if ((rc = upload_operand(h, x, x_nu, y_nu))) return rc;
     if (h->precond == 1)
-        HIPCHK(launch_dilu_apply(h->stream, h->cells64(), h->faces64(), h->colour.p, h->col_cells.p,
+        HIPCHK(launch_dilu_apply(h->stream, h->nv, h->cells64(), h->faces64(), h->colour.p, h->col_cells.p,
                                  h->col_off.data(), (int)h->col_off.size() - 1, h->Jf.p, h->Dti.p, h->ktmp.p, h->kw.p));
     else
-        HIPCHK(launch_block_jacobi(h->stream, h->nC, h->nT, h->Dinv.p, h->ktmp.p, h->kw.p));
-    return download_aos(h, h->kw.p, y);
+        HIPCHK(launch_block_jacobi(h->stream, h->nv, h->nC, h->nT, h->Dinv.p, h->ktmp.p, h->kw.p));
+    return download_operand(h, y, y_nu);
+}
+
+int afx_rans_precond_apply(afx_rans* h, const double* x, double* y) {
+    return afx_rans_precond_apply_nu(h, x, nullptr, y, nullptr);
 }
 
 int afx_rans_cell_colours(afx_rans* h, int32_t* colours, int32_t* ncolours) {
@@ -1615,7 +1684,8 @@ int afx_rans_set_matrix_free(afx_rans* h, int32_t enable) {
     int rc = check_handle(h, false);
     if (rc) return rc;
     if (enable != 0 && enable != 1) return fail(AFX_EARG, "enable must be 0 or 1");
-    if (h->sa()) return fail(AFX_EUNSUP, "the implicit step is not available with the Spalart-Allmaras model");
+    if (h->sa() && enable)
+        return fail(AFX_EUNSUP, "the matrix-free operator covers the four-variable residual");
     h->mf = enable != 0;
     // the GMRES settings each build of implicitSolver sets (solver.h:906-914)
     h->gm_tol = h->mf ? 1e-3 : 1e-2;
@@ -1756,6 +1826,15 @@ int afx_rans_wall_dist(afx_rans* h, double* wall_dist) {
     if (rc) return rc;
     if (!h->wd_ready && (rc = compute_wall_distance(h))) return rc;
     if (wall_dist) return download_cells(h, h->wall_d.p, wall_dist);
+    return AFX_OK;
+}
+
+int afx_rans_sa_wall_dist(afx_rans* h, double* dist) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!h->sa()) return fail(AFX_EUNSUP, "the model's wall distance belongs to the Spalart-Allmaras model");
+    if (!h->wd_ready && (rc = compute_wall_distance(h))) return rc;
+    if (dist) return download_cells(h, h->sa_wd.p, dist);
     return AFX_OK;
 }
 
@@ -1957,7 +2036,7 @@ int geo_check(afx_rans* h) {
 int enqueue_adjoint(afx_rans* h) {
     const AdjBufs A = adj_bufs(h);
     if (h->sa())
-        HIPCHK(launch_adjoint_state_sa(h->stream, h->cells64(), h->faces64(), h->q.p, h->a_w.p, h->wall_d.p,
+        HIPCHK(launch_adjoint_state_sa(h->stream, h->cells64(), h->faces64(), h->q.p, h->a_w.p, h->sa_wd.p,
                                        h->gas64(), h->opts.second_order, h->opts.limiter_k, A, h->a_lam.p));
     else
         HIPCHK(launch_adjoint_state(h->stream, h->cells64(), h->faces64(), h->q.p, h->a_w.p, h->gas64(),
@@ -2309,7 +2388,7 @@ int tangent_prepare(afx_rans* h) {
         HIPCHK(h->tg_lsq.alloc(h->lsq.n));
         HIPCHK(h->tg_xc.alloc(h->xc.n));
         HIPCHK(h->tg_yc.alloc(h->yc.n));
-        HIPCHK(h->tg_wd.alloc(h->wall_d.n));
+        HIPCHK(h->tg_wd.alloc(h->wall_d.n));   // the model's distance (sa_wd) with SA, else unused
         HIPCHK(h->tg_q.alloc((size_t)nv * nT));
         HIPCHK(h->tg_R.alloc((size_t)nv * nT));
         HIPCHK(h->tg_v.alloc((size_t)nv * nT));
@@ -2323,7 +2402,8 @@ int tangent_prepare(afx_rans* h) {
         // the tile kernel writes the real rows only: ghost rows of J v stay 0
         HIPCHK(hipMemsetAsync(h->tg_R.p, 0, (size_t)nv * nT * sizeof(Dual1), s));
     }
-    HIPCHK(launch_to_dual(s, h->wall_d.p, (int64_t)h->wall_d.n, h->tg_wd.p));   // follows the BCs
+    if (h->sa())   // follows the BCs
+        HIPCHK(launch_to_dual(s, h->sa_wd.p, (int64_t)h->sa_wd.n, h->tg_wd.p));
     return AFX_OK;
 }
 // (q, v in tg_v) -> tg_out = (dR/dq) v, on the handle's stream

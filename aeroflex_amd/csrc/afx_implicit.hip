@@ -6,12 +6,16 @@
 // coefficients, Givens rotations and convergence flags live in device
 // memory; the host reads the flags once per restart cycle).
 //
+// Blocks are N x N with N = 4 (the reference's four variables) or N = 5 (the
+// Spalart-Allmaras model, whose Jacobian is this build's own: the reference's
+// SA branch is empty, physics.h:259-261).
+//
 // Layout: vectors are SoA [k*nT + i] over real and ghost cells (ghost rows of
 // the reference's matrix are identity rows with zero right-hand side, so their
 // unknowns stay zero and the ghost columns of boundary faces are dropped).
-// Per face Jf[32*f + ...]: J00 = dF/dq_owner * len (row-major 4x4), then
-// J01 = dF/dq_neighbour * len.  Per real cell D[16*c + ...] = A/dt I + sum of
-// the cell's own blocks, Dinv its inverse.
+// Per face Jf[2NN*f + ...]: J00 = dF/dq_owner * len (row-major N x N), then
+// J01 = dF/dq_neighbour * len.  Per real cell D[NN*c + ...] = A/dt I + sum of
+// the cell's own blocks (+ the SA source row), Dinv its inverse.
 #include "afx_device.hpp"
 #include "afx_implicit.hpp"
 
@@ -71,74 +75,180 @@ __global__ void __launch_bounds__(kBlock) k_face_jacobian(CellsT<double> C, Face
     }
 }
 
-// 4x4 inverse by Gauss-Jordan with partial pivoting; returns false if singular
-__device__ bool inv4(const double* A, double* X) {
-    double M[4][8];
-    for (int r = 0; r < 4; ++r)
-        for (int c = 0; c < 4; ++c) { M[r][c] = A[4 * r + c]; M[r][c + 4] = (r == c) ? 1.0 : 0.0; }
-    for (int c = 0; c < 4; ++c) {
+// Spalart-Allmaras: 5x5 blocks by the same rule (forward differences,
+// h = max(1e-6, |v|*1e-6), the perturbed component restored by subtraction,
+// first-order cell states, times the face length) of the five-variable face
+// flux sa_face_flux.  The cell gradients gx, gy (5 planes) stay frozen; the
+// face gradients' directional parts (v_n - v_o)/l, l along the cell
+// connection, and the no-slip wall's cell-centre nu~ follow the perturbed
+// states, so the viscous and nu~ diffusion couplings enter the blocks.  The
+// rho nu~ column's step floor is 1e-6 mu_L instead of 1e-6 (rho nu~ is of the
+// order mu_L chi; the absolute floor would perturb it by O(1)).
+__global__ void __launch_bounds__(kBlock) k_face_jacobian_sa(CellsT<double> C, FacesT<double> F,
+                                                             const double* __restrict__ q,
+                                                             const double* __restrict__ gx,
+                                                             const double* __restrict__ gy, GasT<double> g,
+                                                             double* __restrict__ Jf) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F.nF) return;
+    const size_t nT = (size_t)C.nT;
+    const int o = F.c0[f], n = F.c1[f];
+    const int kind = F.kind[f];
+    const double nx = F.nx[f], ny = F.ny[f], L = F.len[f];
+    const bool internal = kind == K_INTERNAL;
+    double a[5], b[5], gxb[5], gyb[5];
+    for (int k = 0; k < 5; ++k) {
+        a[k] = q[k * nT + o];
+        b[k] = q[k * nT + n];
+        gxb[k] = gyb[k] = 0.0;
+    }
+    double t0 = 0.0, t1 = 0.0, l = 1.0;
+    if (internal) {
+        t0 = C.xc[n] - C.xc[o];
+        t1 = C.yc[n] - C.yc[o];
+        l = __builtin_sqrt(t0 * t0 + t1 * t1);
+        t0 /= l; t1 /= l;
+        for (int k = 0; k < 5; ++k) {
+            gxb[k] = (gx[k * nT + o] + gx[k * nT + n]) * 0.5;
+            gyb[k] = (gy[k * nT + o] + gy[k * nT + n]) * 0.5;
+        }
+    }
+    const double dn = (F.xf[f] - C.xc[o]) * nx + (F.yf[f] - C.yc[o]) * ny;
+    auto flux = [&](double* out) {
+        double gxa[5] = {0, 0, 0, 0, 0}, gya[5] = {0, 0, 0, 0, 0};
+        if (internal)
+            for (int k = 0; k < 5; ++k) {
+                const double dir = (b[k] - a[k]) / l;
+                const double dot = gxb[k] * t0 + gyb[k] * t1;
+                gxa[k] = gxb[k] - (dot - dir) * t0;
+                gya[k] = gyb[k] - (dot - dir) * t1;
+            }
+        sa_face_flux<double>(g, kind, nx, ny, a, b, gxa, gya, a[4] / a[0], dn, out);
+    };
+    double f0[5], fp[5];
+    flux(f0);
+    double* J = Jf + (size_t)50 * f;
+    for (int i = 0; i < 5; ++i) {
+        const double lo = i == 4 ? 1e-6 * g.mu : 1e-6;
+        {
+            const double hi = dabs<double>(a[i]) * 1e-6;
+            const double up = (lo < hi) ? hi : lo;
+            a[i] += up;
+            flux(fp);
+            a[i] -= up;
+            for (int r = 0; r < 5; ++r) J[5 * r + i] = (fp[r] - f0[r]) / up * L;
+        }
+        {
+            const double hi = dabs<double>(b[i]) * 1e-6;
+            const double up = (lo < hi) ? hi : lo;
+            b[i] += up;
+            flux(fp);
+            b[i] -= up;
+            for (int r = 0; r < 5; ++r) J[25 + 5 * r + i] = (fp[r] - f0[r]) / up * L;
+        }
+    }
+}
+
+// N x N inverse by Gauss-Jordan with partial pivoting; returns false if singular
+template <int N>
+__device__ bool invN(const double* A, double* X) {
+    double M[N][2 * N];
+    for (int r = 0; r < N; ++r)
+        for (int c = 0; c < N; ++c) { M[r][c] = A[N * r + c]; M[r][c + N] = (r == c) ? 1.0 : 0.0; }
+    for (int c = 0; c < N; ++c) {
         int p = c;
         double best = dabs<double>(M[c][c]);
-        for (int r = c + 1; r < 4; ++r)
+        for (int r = c + 1; r < N; ++r)
             if (dabs<double>(M[r][c]) > best) { best = dabs<double>(M[r][c]); p = r; }
         if (!(best > 0.0) || !isfinite(best)) return false;
         if (p != c)
-            for (int k = 0; k < 8; ++k) { double t = M[c][k]; M[c][k] = M[p][k]; M[p][k] = t; }
+            for (int k = 0; k < 2 * N; ++k) { double t = M[c][k]; M[c][k] = M[p][k]; M[p][k] = t; }
         const double ip = 1.0 / M[c][c];
-        for (int k = 0; k < 8; ++k) M[c][k] *= ip;
-        for (int r = 0; r < 4; ++r) {
+        for (int k = 0; k < 2 * N; ++k) M[c][k] *= ip;
+        for (int r = 0; r < N; ++r) {
             if (r == c) continue;
             const double fct = M[r][c];
-            for (int k = 0; k < 8; ++k) M[r][k] -= fct * M[c][k];
+            for (int k = 0; k < 2 * N; ++k) M[r][k] -= fct * M[c][k];
         }
     }
-    for (int r = 0; r < 4; ++r)
-        for (int c = 0; c < 4; ++c) X[4 * r + c] = M[r][c + 4];
+    for (int r = 0; r < N; ++r)
+        for (int c = 0; c < N; ++c) X[N * r + c] = M[r][c + N];
     return true;
 }
 
 // diagonal blocks (solver.h:1013-1056): A/dt on the diagonal, then the
 // cell's own-state blocks of each of its faces in reference face order
-// (owner: J00, neighbour: the lower block -J01); then their inverses
+// (owner: J00, neighbour: the lower block -J01); then their inverses.
+// N = 5 adds the source row: -A dS/dv at the frozen gradients (same step
+// rule and rho nu~ floor), its diagonal entry only where >= 0 (destruction), so production stays
+// explicit and never lowers the diagonal of the nu~ row.
+template <int N>
 __global__ void __launch_bounds__(kBlock) k_assemble_diag(CellsT<double> C, FacesT<double> F, const double* __restrict__ dt,
-                                                          const double* __restrict__ Jf, double* __restrict__ D,
-                                                          double* __restrict__ Dinv, int* __restrict__ bad) {
+                                                          const double* __restrict__ Jf, SaSource src,
+                                                          double* __restrict__ D, double* __restrict__ Dinv,
+                                                          int* __restrict__ bad) {
+    constexpr int NN = N * N;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C.nC) return;
-    double B[16];
+    double B[NN];
     const double ad = C.area[c] / dt[c];
-    for (int k = 0; k < 16; ++k) B[k] = 0.0;
-    for (int k = 0; k < 4; ++k) B[5 * k] = ad;
+    for (int k = 0; k < NN; ++k) B[k] = 0.0;
+    for (int k = 0; k < N; ++k) B[(N + 1) * k] = ad;
     for (int s = 0; s < 4; ++s) {
         uint32_t e = C.cface[(size_t)s * C.nC + c];
         if (e == 0xFFFFFFFFu) break;
         const int f = (int)(e & 0x7FFFFFFFu);
-        const double* J = Jf + (size_t)32 * f;
-        if (e >> 31) for (int k = 0; k < 16; ++k) B[k] += -J[16 + k];
-        else for (int k = 0; k < 16; ++k) B[k] += J[k];
+        const double* J = Jf + (size_t)2 * NN * f;
+        if (e >> 31) for (int k = 0; k < NN; ++k) B[k] += -J[NN + k];
+        else for (int k = 0; k < NN; ++k) B[k] += J[k];
     }
-    for (int k = 0; k < 16; ++k) D[(size_t)16 * c + k] = B[k];
-    double X[16];
-    if (!inv4(B, X)) {
+    if constexpr (N == 5) {
+        if (src.q) {
+            const size_t nT = (size_t)C.nT;
+            double v[5], sx[5], sy[5];
+            for (int k = 0; k < 5; ++k) {
+                v[k] = src.q[k * nT + c];
+                sx[k] = src.gx[k * nT + c];
+                sy[k] = src.gy[k * nT + c];
+            }
+            const double d = src.wd[c], A = C.area[c];
+            const double s0 = sa_cell_source<double>(src.g, v, sx, sy, d);
+            for (int k = 0; k < 5; ++k) {
+                const double lo = k == 4 ? 1e-6 * src.g.mu : 1e-6, hi = dabs<double>(v[k]) * 1e-6;
+                const double up = (lo < hi) ? hi : lo;
+                v[k] += up;
+                const double sp = sa_cell_source<double>(src.g, v, sx, sy, d);
+                v[k] -= up;
+                double val = -A * (sp - s0) / up;
+                if (k == 4 && val < 0) val = 0;
+                B[20 + k] += val;
+            }
+        }
+    }
+    for (int k = 0; k < NN; ++k) D[(size_t)NN * c + k] = B[k];
+    double X[NN];
+    if (!invN<N>(B, X)) {
         atomicAdd(bad, 1);
-        for (int k = 0; k < 16; ++k) X[k] = 0.0;
+        for (int k = 0; k < NN; ++k) X[k] = 0.0;
     }
-    for (int k = 0; k < 16; ++k) Dinv[(size_t)16 * c + k] = X[k];
+    for (int k = 0; k < NN; ++k) Dinv[(size_t)NN * c + k] = X[k];
 }
 
 // y = LHS x (ghost rows identity; ghost columns dropped since x_ghost = 0).
 // The cell's four slot words, then its faces' kinds and neighbours, then the
 // blocks and the neighbours' x are loaded as independent batches, so the
 // gathers of all slots are in flight together.
+template <int N>
 __global__ void __launch_bounds__(kBlock) k_lhs_apply(CellsT<double> C, FacesT<double> F, const double* __restrict__ D,
                                                       const double* __restrict__ Jf, const double* __restrict__ x,
                                                       double* __restrict__ y, const int* __restrict__ active) {
+    constexpr int NN = N * N;
     if (active && *active == 0) return;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C.nT) return;
     const size_t nT = (size_t)C.nT;
     if (c >= C.nC) {
-        for (int k = 0; k < 4; ++k) y[k * nT + c] = x[k * nT + c];
+        for (int k = 0; k < N; ++k) y[k * nT + c] = x[k * nT + c];
         return;
     }
     uint32_t e[4];
@@ -156,27 +266,35 @@ __global__ void __launch_bounds__(kBlock) k_lhs_apply(CellsT<double> C, FacesT<d
             other[s] = (e[s] >> 31) ? F.c0[f] : F.c1[f];
         }
     }
-    double xc[4], acc[4];
-    for (int k = 0; k < 4; ++k) xc[k] = x[k * nT + c];
-    const double* B = D + (size_t)16 * c;
-    for (int r = 0; r < 4; ++r) acc[r] = B[4 * r] * xc[0] + B[4 * r + 1] * xc[1] + B[4 * r + 2] * xc[2] + B[4 * r + 3] * xc[3];
+    double xc[N], acc[N];
+    for (int k = 0; k < N; ++k) xc[k] = x[k * nT + c];
+    const double* B = D + (size_t)NN * c;
+    for (int r = 0; r < N; ++r) {
+        double t = 0.0;
+        for (int k = 0; k < N; ++k) t += B[N * r + k] * xc[k];
+        acc[r] = t;
+    }
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         if (!use[s]) continue;
         const int f = (int)(e[s] & 0x7FFFFFFFu);
         const bool nb = (e[s] >> 31) != 0;
         // neighbour row: block (c1, c0) = -J00; owner row: block (c0, c1) = J01
-        const double* J = Jf + (size_t)32 * f + (nb ? 0 : 16);
+        const double* J = Jf + (size_t)2 * NN * f + (nb ? 0 : NN);
         const double sg = nb ? -1.0 : 1.0;
-        double xo[4];
-        for (int k = 0; k < 4; ++k) xo[k] = sg * x[k * nT + other[s]];
-        for (int r = 0; r < 4; ++r)
-            acc[r] += J[4 * r] * xo[0] + J[4 * r + 1] * xo[1] + J[4 * r + 2] * xo[2] + J[4 * r + 3] * xo[3];
+        double xo[N];
+        for (int k = 0; k < N; ++k) xo[k] = sg * x[k * nT + other[s]];
+        for (int r = 0; r < N; ++r) {
+            double t = 0.0;
+            for (int k = 0; k < N; ++k) t += J[N * r + k] * xo[k];
+            acc[r] += t;
+        }
     }
-    for (int k = 0; k < 4; ++k) y[k * nT + c] = acc[k];
+    for (int k = 0; k < N; ++k) y[k * nT + c] = acc[k];
 }
 
 // z = Dinv x (ghost rows identity)
+template <int N>
 __global__ void __launch_bounds__(kBlock) k_block_jacobi(int32_t nC, int32_t nT, const double* __restrict__ Dinv,
                                                          const double* __restrict__ x, double* __restrict__ z,
                                                          const int* __restrict__ active) {
@@ -185,13 +303,17 @@ __global__ void __launch_bounds__(kBlock) k_block_jacobi(int32_t nC, int32_t nT,
     if (c >= nT) return;
     const size_t n = (size_t)nT;
     if (c >= nC) {
-        for (int k = 0; k < 4; ++k) z[k * n + c] = x[k * n + c];
+        for (int k = 0; k < N; ++k) z[k * n + c] = x[k * n + c];
         return;
     }
-    double xc[4];
-    for (int k = 0; k < 4; ++k) xc[k] = x[k * n + c];
-    const double* B = Dinv + (size_t)16 * c;
-    for (int r = 0; r < 4; ++r) z[r * n + c] = B[4 * r] * xc[0] + B[4 * r + 1] * xc[1] + B[4 * r + 2] * xc[2] + B[4 * r + 3] * xc[3];
+    double xc[N];
+    for (int k = 0; k < N; ++k) xc[k] = x[k * n + c];
+    const double* B = Dinv + (size_t)N * N * c;
+    for (int r = 0; r < N; ++r) {
+        double t = 0.0;
+        for (int k = 0; k < N; ++k) t += B[N * r + k] * xc[k];
+        z[r * n + c] = t;
+    }
 }
 
 // y += a * x
@@ -211,66 +333,75 @@ __global__ void __launch_bounds__(kBlock) k_axpy(double a, const double* __restr
 // neighbour n, A_on = J01(f) and A_no = -J00(f); boundary faces have none.
 
 // the block A_ij (sgn, J) of cell i across its slot entry e, and the other cell
+template <int N>
 __device__ __forceinline__ const double* offdiag(const FacesT<double>& F, const double* Jf, uint32_t e, int* other,
                                                   double* sgn) {
     const int f = (int)(e & 0x7FFFFFFFu);
-    const double* J = Jf + (size_t)32 * f;
+    const double* J = Jf + (size_t)2 * N * N * f;
     if (e >> 31) { *other = F.c0[f]; *sgn = -1.0; return J; }        // i is the neighbour: -J00
-    *other = F.c1[f]; *sgn = 1.0; return J + 16;                       // i is the owner: J01
+    *other = F.c1[f]; *sgn = 1.0; return J + N * N;                   // i is the owner: J01
 }
 // the transposed-position block A_ji of the same face
+template <int N>
 __device__ __forceinline__ const double* offdiag_t(const double* Jf, uint32_t e, double* sgn) {
     const int f = (int)(e & 0x7FFFFFFFu);
-    const double* J = Jf + (size_t)32 * f;
-    if (e >> 31) { *sgn = 1.0; return J + 16; }
+    const double* J = Jf + (size_t)2 * N * N * f;
+    if (e >> 31) { *sgn = 1.0; return J + N * N; }
     *sgn = -1.0; return J;
 }
 
+template <int N>
 __global__ void __launch_bounds__(kBlock) k_dilu_setup(CellsT<double> C, FacesT<double> F, const uint8_t* __restrict__ colour,
                                                        int c, const int32_t* __restrict__ cells, int32_t ncells,
                                                        const double* __restrict__ D, const double* __restrict__ Jf,
                                                        double* __restrict__ Dti, int* __restrict__ bad) {
+    constexpr int NN = N * N;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncells) return;
     const int i = cells[k];
-    double B[16];
-    for (int r = 0; r < 16; ++r) B[r] = D[(size_t)16 * i + r];
+    double B[NN];
+    for (int r = 0; r < NN; ++r) B[r] = D[(size_t)NN * i + r];
     for (int s = 0; s < 4; ++s) {
         const uint32_t e = C.cface[(size_t)s * C.nC + i];
         if (e == 0xFFFFFFFFu) break;
         if (F.kind[e & 0x7FFFFFFFu] != K_INTERNAL) continue;
         int j;
         double sa, sb;
-        const double* Aij = offdiag(F, Jf, e, &j, &sa);
+        const double* Aij = offdiag<N>(F, Jf, e, &j, &sa);
         if (colour[j] >= c) continue;
-        const double* Aji = offdiag_t(Jf, e, &sb);
-        const double* Pj = Dti + (size_t)16 * j;
-        double T[16];   // Dt_j^-1 A_ji
-        for (int r = 0; r < 4; ++r)
-            for (int q = 0; q < 4; ++q)
-                T[4 * r + q] = sb * (Pj[4 * r] * Aji[q] + Pj[4 * r + 1] * Aji[4 + q] + Pj[4 * r + 2] * Aji[8 + q] +
-                                     Pj[4 * r + 3] * Aji[12 + q]);
-        for (int r = 0; r < 4; ++r)
-            for (int q = 0; q < 4; ++q)
-                B[4 * r + q] -= sa * (Aij[4 * r] * T[q] + Aij[4 * r + 1] * T[4 + q] + Aij[4 * r + 2] * T[8 + q] +
-                                      Aij[4 * r + 3] * T[12 + q]);
+        const double* Aji = offdiag_t<N>(Jf, e, &sb);
+        const double* Pj = Dti + (size_t)NN * j;
+        double T[NN];   // Dt_j^-1 A_ji
+        for (int r = 0; r < N; ++r)
+            for (int q = 0; q < N; ++q) {
+                double t = 0.0;
+                for (int m = 0; m < N; ++m) t += Pj[N * r + m] * Aji[N * m + q];
+                T[N * r + q] = sb * t;
+            }
+        for (int r = 0; r < N; ++r)
+            for (int q = 0; q < N; ++q) {
+                double t = 0.0;
+                for (int m = 0; m < N; ++m) t += Aij[N * r + m] * T[N * m + q];
+                B[N * r + q] -= sa * t;
+            }
     }
-    double X[16];
-    if (!inv4(B, X)) {
+    double X[NN];
+    if (!invN<N>(B, X)) {
         atomicAdd(bad, 1);
-        for (int r = 0; r < 16; ++r) X[r] = 0.0;
+        for (int r = 0; r < NN; ++r) X[r] = 0.0;
     }
-    for (int r = 0; r < 16; ++r) Dti[(size_t)16 * i + r] = X[r];
+    for (int r = 0; r < NN; ++r) Dti[(size_t)NN * i + r] = X[r];
 }
 
 // forward sweep of colour c: y_i = Dt_i^-1 (b_i - sum_{j before i} A_ij y_j)
 // backward sweep of colour c (in place): y_i -= Dt_i^-1 sum_{j after i} A_ij y_j
-template <bool FORWARD>
+template <int N, bool FORWARD>
 __global__ void __launch_bounds__(kBlock) k_dilu_sweep(CellsT<double> C, FacesT<double> F, const uint8_t* __restrict__ colour,
                                                        int c, const int32_t* __restrict__ cells, int32_t ncells,
                                                        const double* __restrict__ Jf, const double* __restrict__ Dti,
                                                        const double* __restrict__ b, double* __restrict__ y,
                                                        const int* __restrict__ active) {
+    constexpr int NN = N * N;
     if (active && *active == 0) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncells) return;
@@ -294,8 +425,8 @@ __global__ void __launch_bounds__(kBlock) k_dilu_sweep(CellsT<double> C, FacesT<
             }
         }
     }
-    double acc[4];
-    for (int r = 0; r < 4; ++r) acc[r] = FORWARD ? b[r * nT + i] : 0.0;
+    double acc[N];
+    for (int r = 0; r < N; ++r) acc[r] = FORWARD ? b[r * nT + i] : 0.0;
     bool any = FORWARD;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -303,29 +434,33 @@ __global__ void __launch_bounds__(kBlock) k_dilu_sweep(CellsT<double> C, FacesT<
         any = true;
         const int f = (int)(e[s] & 0x7FFFFFFFu);
         const bool nb = (e[s] >> 31) != 0;
-        const double* A = Jf + (size_t)32 * f + (nb ? 0 : 16);   // A_ij: -J00 (i the neighbour) or J01
+        const double* A = Jf + (size_t)2 * NN * f + (nb ? 0 : NN);   // A_ij: -J00 (i the neighbour) or J01
         const double sa = nb ? -1.0 : 1.0;
-        double yj[4];
-        for (int r = 0; r < 4; ++r) yj[r] = sa * y[r * nT + other[s]];
-        for (int r = 0; r < 4; ++r)
-            acc[r] += (FORWARD ? -1.0 : 1.0) * (A[4 * r] * yj[0] + A[4 * r + 1] * yj[1] + A[4 * r + 2] * yj[2] + A[4 * r + 3] * yj[3]);
+        double yj[N];
+        for (int r = 0; r < N; ++r) yj[r] = sa * y[r * nT + other[s]];
+        for (int r = 0; r < N; ++r) {
+            double t = 0.0;
+            for (int m = 0; m < N; ++m) t += A[N * r + m] * yj[m];
+            acc[r] += (FORWARD ? -1.0 : 1.0) * t;
+        }
     }
     if (!any) return;
-    const double* P = Dti + (size_t)16 * i;
-    for (int r = 0; r < 4; ++r) {
-        const double t = P[4 * r] * acc[0] + P[4 * r + 1] * acc[1] + P[4 * r + 2] * acc[2] + P[4 * r + 3] * acc[3];
+    const double* P = Dti + (size_t)NN * i;
+    for (int r = 0; r < N; ++r) {
+        double t = 0.0;
+        for (int m = 0; m < N; ++m) t += P[N * r + m] * acc[m];
         if (FORWARD) y[r * nT + i] = t;
         else y[r * nT + i] -= t;
     }
 }
 
 // ghost rows are identity rows: z = x there
-__global__ void __launch_bounds__(kBlock) k_copy_ghost_rows(int32_t nC, int32_t nT, const double* __restrict__ x,
+__global__ void __launch_bounds__(kBlock) k_copy_ghost_rows(int32_t nC, int32_t nT, int nv, const double* __restrict__ x,
                                                             double* __restrict__ z, const int* __restrict__ active) {
     if (active && *active == 0) return;
     const int c = nC + blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nT) return;
-    for (int k = 0; k < 4; ++k) z[(size_t)k * nT + c] = x[(size_t)k * nT + c];
+    for (int k = 0; k < nv; ++k) z[(size_t)k * nT + c] = x[(size_t)k * nT + c];
 }
 
 // --------------------------------------------- device-resident GMRES (f1) --
@@ -553,57 +688,94 @@ hipError_t launch_face_jacobian(hipStream_t s, CellsT<double> C, FacesT<double> 
     hipLaunchKernelGGL(k_face_jacobian, dim3(nblocks(F.nF)), dim3(kBlock), 0, s, C, F, q, gx, gy, g, viscous, Jf);
     return hipGetLastError();
 }
-hipError_t launch_assemble_diag(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* dt, const double* Jf,
-                                double* D, double* Dinv, int* bad) {
-    hipLaunchKernelGGL(k_assemble_diag, dim3(nblocks(C.nC)), dim3(kBlock), 0, s, C, F, dt, Jf, D, Dinv, bad);
+hipError_t launch_face_jacobian_sa(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* gx,
+                                   const double* gy, GasT<double> g, double* Jf) {
+    hipLaunchKernelGGL(k_face_jacobian_sa, dim3(nblocks(F.nF)), dim3(kBlock), 0, s, C, F, q, gx, gy, g, Jf);
     return hipGetLastError();
 }
-hipError_t launch_lhs_apply(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* D, const double* Jf,
+hipError_t launch_assemble_diag(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const double* dt,
+                                const double* Jf, SaSource src, double* D, double* Dinv, int* bad) {
+    if (nv == 5)
+        hipLaunchKernelGGL(k_assemble_diag<5>, dim3(nblocks(C.nC)), dim3(kBlock), 0, s, C, F, dt, Jf, src, D, Dinv, bad);
+    else if (nv == 4)
+        hipLaunchKernelGGL(k_assemble_diag<4>, dim3(nblocks(C.nC)), dim3(kBlock), 0, s, C, F, dt, Jf, src, D, Dinv, bad);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+hipError_t launch_lhs_apply(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const double* D, const double* Jf,
                             const double* x, double* y, const int* active) {
-    hipLaunchKernelGGL(k_lhs_apply, dim3(nblocks(C.nT)), dim3(kBlock), 0, s, C, F, D, Jf, x, y, active);
+    if (nv == 5)
+        hipLaunchKernelGGL(k_lhs_apply<5>, dim3(nblocks(C.nT)), dim3(kBlock), 0, s, C, F, D, Jf, x, y, active);
+    else if (nv == 4)
+        hipLaunchKernelGGL(k_lhs_apply<4>, dim3(nblocks(C.nT)), dim3(kBlock), 0, s, C, F, D, Jf, x, y, active);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
-hipError_t launch_block_jacobi(hipStream_t s, int32_t nC, int32_t nT, const double* Dinv, const double* x, double* z,
-                               const int* active) {
-    hipLaunchKernelGGL(k_block_jacobi, dim3(nblocks(nT)), dim3(kBlock), 0, s, nC, nT, Dinv, x, z, active);
+hipError_t launch_block_jacobi(hipStream_t s, int nv, int32_t nC, int32_t nT, const double* Dinv, const double* x,
+                               double* z, const int* active) {
+    if (nv == 5)
+        hipLaunchKernelGGL(k_block_jacobi<5>, dim3(nblocks(nT)), dim3(kBlock), 0, s, nC, nT, Dinv, x, z, active);
+    else if (nv == 4)
+        hipLaunchKernelGGL(k_block_jacobi<4>, dim3(nblocks(nT)), dim3(kBlock), 0, s, nC, nT, Dinv, x, z, active);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
-hipError_t launch_dilu_setup(hipStream_t s, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
-                             const int32_t* cells, const int32_t* coff, int ncolours, const double* D, const double* Jf,
-                             double* Dti, int* bad) {
+template <int N>
+static hipError_t dilu_setup_n(hipStream_t s, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
+                               const int32_t* cells, const int32_t* coff, int ncolours, const double* D,
+                               const double* Jf, double* Dti, int* bad) {
     for (int c = 0; c < ncolours; ++c) {
         const int32_t n = coff[c + 1] - coff[c];
         if (n == 0) continue;
-        hipLaunchKernelGGL(k_dilu_setup, dim3(nblocks(n)), dim3(kBlock), 0, s, C, F, colour, c, cells + coff[c], n, D,
+        hipLaunchKernelGGL(k_dilu_setup<N>, dim3(nblocks(n)), dim3(kBlock), 0, s, C, F, colour, c, cells + coff[c], n, D,
                            Jf, Dti, bad);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
-hipError_t launch_dilu_apply(hipStream_t s, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
-                             const int32_t* cells, const int32_t* coff, int ncolours, const double* Jf,
-                             const double* Dti, const double* x, double* z, const int* active) {
+hipError_t launch_dilu_setup(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
+                             const int32_t* cells, const int32_t* coff, int ncolours, const double* D, const double* Jf,
+                             double* Dti, int* bad) {
+    if (nv == 5) return dilu_setup_n<5>(s, C, F, colour, cells, coff, ncolours, D, Jf, Dti, bad);
+    if (nv == 4) return dilu_setup_n<4>(s, C, F, colour, cells, coff, ncolours, D, Jf, Dti, bad);
+    return hipErrorInvalidValue;
+}
+template <int N>
+static hipError_t dilu_apply_n(hipStream_t s, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
+                               const int32_t* cells, const int32_t* coff, int ncolours, const double* Jf,
+                               const double* Dti, const double* x, double* z, const int* active) {
     hipError_t e;
     if (C.nT > C.nC) {
-        hipLaunchKernelGGL(k_copy_ghost_rows, dim3(nblocks(C.nT - C.nC)), dim3(kBlock), 0, s, C.nC, C.nT, x, z, active);
+        hipLaunchKernelGGL(k_copy_ghost_rows, dim3(nblocks(C.nT - C.nC)), dim3(kBlock), 0, s, C.nC, C.nT, N, x, z,
+                           active);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     for (int c = 0; c < ncolours; ++c) {
         const int32_t n = coff[c + 1] - coff[c];
         if (n == 0) continue;
-        hipLaunchKernelGGL(k_dilu_sweep<true>, dim3(nblocks(n)), dim3(kBlock), 0, s, C, F, colour, c, cells + coff[c], n,
-                           Jf, Dti, x, z, active);
+        hipLaunchKernelGGL((k_dilu_sweep<N, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, C, F, colour, c,
+                           cells + coff[c], n, Jf, Dti, x, z, active);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     for (int c = ncolours - 2; c >= 0; --c) {
         const int32_t n = coff[c + 1] - coff[c];
         if (n == 0) continue;
-        hipLaunchKernelGGL(k_dilu_sweep<false>, dim3(nblocks(n)), dim3(kBlock), 0, s, C, F, colour, c, cells + coff[c],
-                           n, Jf, Dti, x, z, active);
+        hipLaunchKernelGGL((k_dilu_sweep<N, false>), dim3(nblocks(n)), dim3(kBlock), 0, s, C, F, colour, c,
+                           cells + coff[c], n, Jf, Dti, x, z, active);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;
+}
+hipError_t launch_dilu_apply(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
+                             const int32_t* cells, const int32_t* coff, int ncolours, const double* Jf,
+                             const double* Dti, const double* x, double* z, const int* active) {
+    if (nv == 5) return dilu_apply_n<5>(s, C, F, colour, cells, coff, ncolours, Jf, Dti, x, z, active);
+    if (nv == 4) return dilu_apply_n<4>(s, C, F, colour, cells, coff, ncolours, Jf, Dti, x, z, active);
+    return hipErrorInvalidValue;
 }
 hipError_t launch_gs_pass(hipStream_t s, const double* V, int64_t ld, int m, const double* h, double* w, int64_t n,
                           bool dots, double* partial, int nblk, const int* active) {

@@ -6,19 +6,31 @@ namespace afx {
 
 hipError_t launch_face_jacobian(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* gx,
                                 const double* gy, GasT<double> g, int viscous, double* Jf);
-hipError_t launch_assemble_diag(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* dt, const double* Jf,
-                                double* D, double* Dinv, int* bad);
+// the Spalart-Allmaras model's 5x5 face blocks (q, gx, gy: 5 planes)
+hipError_t launch_face_jacobian_sa(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* gx,
+                                   const double* gy, GasT<double> g, double* Jf);
+// the source row of the Spalart-Allmaras diagonal blocks (q null: none)
+struct SaSource {
+    const double* q;    // 5 planes
+    const double* gx;   // unlimited cell gradients, 5 planes
+    const double* gy;
+    const double* wd;   // wall distance
+    GasT<double> g;
+};
+// nv: block size, 4 or 5 (Spalart-Allmaras)
+hipError_t launch_assemble_diag(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const double* dt,
+                                const double* Jf, SaSource src, double* D, double* Dinv, int* bad);
 // active: a device flag (nullable); the kernel does nothing while it reads 0
-hipError_t launch_lhs_apply(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* D, const double* Jf,
+hipError_t launch_lhs_apply(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const double* D, const double* Jf,
                             const double* x, double* y, const int* active = nullptr);
-hipError_t launch_block_jacobi(hipStream_t s, int32_t nC, int32_t nT, const double* Dinv, const double* x, double* z,
-                               const int* active = nullptr);
+hipError_t launch_block_jacobi(hipStream_t s, int nv, int32_t nC, int32_t nT, const double* Dinv, const double* x,
+                               double* z, const int* active = nullptr);
 // multicolour block DILU: colour per real cell, cells listed colour by colour
 // (cells[coff[c] .. coff[c+1]), coff on the host)
-hipError_t launch_dilu_setup(hipStream_t s, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
+hipError_t launch_dilu_setup(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
                              const int32_t* cells, const int32_t* coff, int ncolours, const double* D, const double* Jf,
                              double* Dti, int* bad);
-hipError_t launch_dilu_apply(hipStream_t s, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
+hipError_t launch_dilu_apply(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
                              const int32_t* cells, const int32_t* coff, int ncolours, const double* Jf,
                              const double* Dti, const double* x, double* z, const int* active = nullptr);
 

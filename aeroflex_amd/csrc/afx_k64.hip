@@ -157,6 +157,43 @@ hipError_t launch_wall_dist(hipStream_t s, const double* xc, const double* yc, i
     return hipGetLastError();
 }
 
+// The Spalart-Allmaras model's wall distance: from each cell centroid to the
+// nearest point of the wall / slip-wall face segments (a + t (b - a), t
+// clamped to [0, 1]; seg = {ax, ay, bx - ax, by - ay} per face), 1 without
+// such faces.  Segments staged through LDS tiles as in k_wall_dist.
+__global__ void __launch_bounds__(kBlock) k_wall_dist_seg(const double* __restrict__ xc, const double* __restrict__ yc,
+                                                          int32_t nT, const double4* __restrict__ seg, int32_t nw,
+                                                          double* __restrict__ d) {
+    __shared__ double4 ss[kBlock];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const double px = c < nT ? xc[c] : 0.0, py = c < nT ? yc[c] : 0.0;
+    double mind = 1.0;
+    bool first = true;
+    for (int base = 0; base < nw; base += kBlock) {
+        const int j = base + threadIdx.x;
+        if (j < nw) ss[threadIdx.x] = seg[j];
+        __syncthreads();
+        const int m = nw - base < kBlock ? nw - base : kBlock;
+        for (int k = 0; k < m; ++k) {
+            const double4 g = ss[k];
+            const double ll = g.z * g.z + g.w * g.w;
+            double t = ((px - g.x) * g.z + (py - g.y) * g.w) / ll;
+            t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+            const double ex = px - (g.x + t * g.z), ey = py - (g.y + t * g.w);
+            const double dd = __builtin_sqrt(ex * ex + ey * ey);
+            if (first) { mind = dd; first = false; }
+            else mind = (dd < mind) ? dd : mind;
+        }
+        __syncthreads();
+    }
+    if (c < nT) d[c] = mind;
+}
+hipError_t launch_wall_dist_seg(hipStream_t s, const double* xc, const double* yc, int32_t nT, const double4* seg,
+                                int32_t nw, double* d) {
+    hipLaunchKernelGGL(k_wall_dist_seg, dim3(nblocks(nT)), dim3(kBlock), 0, s, xc, yc, nT, seg, nw, d);
+    return hipGetLastError();
+}
+
 __global__ void __launch_bounds__(kBlock) k_gather_states(const double* __restrict__ q, int32_t nT, int nv,
                                                           const int32_t* __restrict__ cells, int32_t n,
                                                           double* __restrict__ out) {

@@ -207,6 +207,10 @@ hipError_t launch_rk_update(hipStream_t s, const double* q, const double* qW, co
 hipError_t launch_to_f32(hipStream_t s, const double* src, float* dst, int64_t n);
 hipError_t launch_wall_dist(hipStream_t s, const double* xc, const double* yc, int32_t nT, const double* wx,
                             const double* wy, int32_t nw, double* d);
+// the Spalart-Allmaras model's distance to the wall / slip-wall face segments
+// (seg: {ax, ay, bx - ax, by - ay} per face)
+hipError_t launch_wall_dist_seg(hipStream_t s, const double* xc, const double* yc, int32_t nT, const double4* seg,
+                                int32_t nw, double* d);
 hipError_t launch_gather_states(hipStream_t s, const double* q, int32_t nT, int nv, const int32_t* cells, int32_t n,
                                 double* out);
 

@@ -96,6 +96,8 @@ def _load():
         "afx_rans_implicit_compute": (C.c_int, [P, C.POINTER(i32)]),
         "afx_rans_implicit_solve": (C.c_int, [P, f64, f64, i32, i32, dp]),
         "afx_rans_lhs_apply": (C.c_int, [P, dp, dp]),
+        "afx_rans_lhs_apply_nu": (C.c_int, [P, dp, dp, dp, dp]),
+        "afx_rans_precond_apply_nu": (C.c_int, [P, dp, dp, dp, dp]),
         "afx_rans_set_matrix_free": (C.c_int, [P, C.c_int32]),
         "afx_mapper_create": (C.c_int, [P, P, C.POINTER(P)]),
         "afx_mapper_destroy": (C.c_int, [P]),
@@ -105,6 +107,7 @@ def _load():
         "afx_rans_linear_iterations": (C.c_int, [P, C.POINTER(i64)]),
         "afx_rans_wall_forces": (C.c_int, [P, C.c_char_p, dp, dp, dp]),
         "afx_rans_wall_dist": (C.c_int, [P, dp]),
+        "afx_rans_sa_wall_dist": (C.c_int, [P, dp]),
         "afx_rans_save_vtu": (C.c_int, [P, C.c_char_p]),
         "afx_rans_cp_profile": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_int64), dp, dp, dp]),
         "afx_rans_counts": (C.c_int, [P, C.POINTER(i64)]),
@@ -415,7 +418,8 @@ class Solver(Handle):
         _check(lib.afx_rans_set_preconditioner(self.h, self.PRECONDITIONERS[kind]))
 
     def fill(self):
-        """fillRhoLHS (solver.h:973-1075)."""
+        """fillRhoLHS (solver.h:973-1075); with the SA model the 5x5-block
+        matrix of the model (Green-Gauss gradients)."""
         _check(lib.afx_rans_implicit_fill(self.h))
 
     def compute(self):
@@ -430,18 +434,34 @@ class Solver(Handle):
         _check(lib.afx_rans_implicit_solve(self.h, relaxation, tol, rhs_iterations, precision, C.byref(e)))
         return e.value
 
-    def lhs_apply(self, x):
+    def _operator(self, fn, x, x_nu):
         x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (4 * self.n_cells,):
+            raise RansError("operand has the wrong length")
         y = np.empty_like(x)
-        _check(lib.afx_rans_lhs_apply(self.h, _dptr(x), _dptr(y)))
+        if self._opts.viscosity == 2:
+            if x_nu is None:
+                raise RansError("the Spalart-Allmaras operator needs the rho nu~ rows")
+            xn = np.ascontiguousarray(x_nu, dtype=np.float64)
+            if xn.shape != (self.n_cells,):
+                raise RansError("rho nu~ rows have the wrong length")
+            yn = np.empty_like(xn)
+            _check(fn(self.h, _dptr(x), _dptr(xn), _dptr(y), _dptr(yn)))
+            return y, yn
+        if x_nu is not None:
+            raise RansError("rho nu~ rows only with the Spalart-Allmaras model")
+        _check(fn(self.h, _dptr(x), None, _dptr(y), None))
         return y
 
-    def precond_apply(self, x):
-        """M^-1 x with the preconditioner of the last fill (test hook)."""
-        x = np.ascontiguousarray(x, dtype=np.float64)
-        y = np.empty_like(x)
-        _check(lib.afx_rans_precond_apply(self.h, _dptr(x), _dptr(y)))
-        return y
+    def lhs_apply(self, x, x_nu=None):
+        """LHS x with the matrix of the last fill (test hook); with the SA
+        model also the rho nu~ rows: returns (y, y_nu)."""
+        return self._operator(lib.afx_rans_lhs_apply_nu, x, x_nu)
+
+    def precond_apply(self, x, x_nu=None):
+        """M^-1 x with the preconditioner of the last fill (test hook); with
+        the SA model as lhs_apply."""
+        return self._operator(lib.afx_rans_precond_apply_nu, x, x_nu)
 
     def cell_colours(self):
         """(colour per real cell in reference order, number of colours) of the
@@ -493,8 +513,17 @@ class Solver(Handle):
         _check(lib.afx_rans_save_vtu(self.h, os.fsencode(path)))
 
     def wall_dist(self):
+        """mesh::compute_wall_dist (mesh.h:794-830): distance to the nearest
+        wall / slip-wall face centre."""
         d = np.empty(self.n_cells)
         _check(lib.afx_rans_wall_dist(self.h, _dptr(d)))
+        return d
+
+    def sa_wall_dist(self):
+        """The SA model's wall distance: to the nearest point of the wall /
+        slip-wall face segments."""
+        d = np.empty(self.n_cells)
+        _check(lib.afx_rans_sa_wall_dist(self.h, _dptr(d)))
         return d
 
     # -- Spalart-Allmaras: rho nu~ per cell (the reference's nu, solver.h:55)

@@ -493,32 +493,46 @@ def implicit_bench(rans, cells=2_000_000, cfls=(40.0, 10.0, 2.0), relaxation=0.9
     (multigrid.h:262-268: fillRhoLHS, then implicitSolver::solve with the
     reference's rhs_iterations 5 and relaxation 0.9, solver.h:973-1213) at its
     start CFL 40 and at 10 and 2, on the bench's 2M-cell jittered mesh,
-    inviscid (there is no SA Jacobian) with a slip wall, from the perturbed
-    freestream state.  Per preconditioner and CFL: fill and solve wall times,
-    GMRES iterations, time per GMRES iteration, and the returned residual norm
-    (-1: the linear solve failed or the state went non-finite, as the
-    reference's step does from this state at CFL 40)."""
+    inviscid with a slip wall, from the perturbed freestream state; then the
+    SA model's step (5x5 blocks, no-slip wall, Re 1e6, DILU) at CFL 10 and 2.
+    Per preconditioner and CFL: fill and solve wall times, GMRES iterations,
+    time per GMRES iteration, and the returned residual norm (-1: the linear
+    solve failed or the state went non-finite, as the reference's step does
+    from this state at CFL 40)."""
     ni, nj = mesh_dims(cells)
     m = rans.Mesh.jittered(ni, nj, r_far=20.0, thickness=0.12, first_h=1e-4, jitter=0.1, quad_frac=0.25, seed=1)
     far = rans.BoundaryVariables(mach=MACH, angle=np.deg2rad(ALPHA_DEG))
     out = {"cells": None, "relaxation": relaxation, "rhs_iterations": rhs_iterations,
            "gmres": "restart 30, tol 1e-2, maxit 500 (solver.h:906-910), device-resident Arnoldi state"}
-    for precond, cfl in [(p, c) for p in ("dilu", "block-jacobi") for c in cfls]:
-        s = rans.Solver(m, viscosity_model="inviscid", second_order=True, gradient_scheme="green-gauss",
-                        limiter_k=5.0)
-        s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition("slip-wall")})
+    runs = [("", p, c) for p in ("dilu", "block-jacobi") for c in cfls] + [("sa_", "dilu", c) for c in (10.0, 2.0)]
+    for model, precond, cfl in runs:
+        sa = model == "sa_"
+        if sa:
+            s = rans.Solver(m, gas=rans.Gas(mu_L=freestream_mu(rans)), viscosity_model="spalart-allmaras",
+                            second_order=True, gradient_scheme="green-gauss", limiter_k=5.0)
+            s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition("wall")})
+        else:
+            s = rans.Solver(m, viscosity_model="inviscid", second_order=True, gradient_scheme="green-gauss",
+                            limiter_k=5.0)
+            s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far),
+                       "wall": rans.BoundaryCondition("slip-wall")})
         s.init()
         s.refill_bcs()
-        q, _ = perturbed(s.get_q(), None, s.n_real, far.get_conservative(s.gas), s.gas.gamma,
-                         np.random.default_rng(2))
+        q, nu = perturbed(s.get_q(), s.get_nu() if sa else None, s.n_real, far.get_conservative(s.gas),
+                          s.gas.gamma, np.random.default_rng(2))
         out["cells"] = s.n_real
         s.set_preconditioner(precond)
         s.set_cfl(cfl)
+
+        def reset():
+            s.set_q(q)
+            if sa:
+                s.set_nu(nu)
         # untimed: one fill and a single linear solve (allocations, first launches)
-        s.set_q(q)
+        reset()
         s.fill()
         s.solve(relaxation, 0.0, 0)
-        s.set_q(q)
+        reset()
         s.synchronize()
         t0 = time.perf_counter()
         s.fill()
@@ -529,13 +543,65 @@ def implicit_bench(rans, cells=2_000_000, cfls=(40.0, 10.0, 2.0), relaxation=0.9
         s.synchronize()
         t_solve = time.perf_counter() - t0
         its = s.linear_iterations()
-        out[f"{precond}_cfl{cfl:g}"] = {"fill_ms": round(t_fill * 1e3, 2), "solve_ms": round(t_solve * 1e3, 2),
+        out[f"{model}{precond}_cfl{cfl:g}"] = {"fill_ms": round(t_fill * 1e3, 2), "solve_ms": round(t_solve * 1e3, 2),
                                          "iteration_ms": round((t_fill + t_solve) * 1e3, 2),
                                          "gmres_iterations": int(its),
                                          "ms_per_gmres_iteration": round(t_solve * 1e3 / max(its, 1), 3),
                                          "residual_norm": err}
         s.close()
     m.close()
+    return out
+
+
+def sa_config1(rans, iterations=500):
+    """The reference's plain FP64 case with the SA model: NACA 0012 O-grid of
+    129 x 65 nodes (16 384 triangles), farfield at 20 chords, M 0.5, alpha
+    2 deg (the reference's degree factor 0.01745), Re 1e6 (mu_L = 7e-7),
+    no-slip wall, freestream start with rho nu~ = 3 mu_L, 500 iterations of
+    the reference loop (multigrid.h:240-293: CFL 40 + 50 i up to 100,
+    relaxation 0.9, rhs_iterations 5, GMRES 1e-2), then the same at a fixed
+    CFL of 5 and with CFL 5 + i up to 20, both with rhs_iterations 0.  Per run: loop state, iterations, the
+    residual history's ends, CL/CD and the largest chi = nu~/nu_L every 10
+    iterations, time per iteration."""
+    out = {"mesh": "NACA 0012 O-grid 128 x 65 (16 384 triangles), r_far 20, first cell 2e-3",
+           "flow": "M 0.5, alpha 2 x 0.01745 rad, Re 1e6 (mu_L 7e-7), no-slip wall, rho nu~ = 3 mu_L"}
+    mesh = rans.Mesh.naca_ogrid(128, 65, r_far=20.0, first_h=2e-3)
+    far = rans.BoundaryVariables(mach=0.5, angle=2 * 0.01745)
+    mu = 7e-7
+    for name, st in (("reference_loop", rans.Settings(max_iterations=iterations, tolerance=1e-13)),
+                     ("cfl5", rans.Settings(start_cfl=5.0, slope_cfl=0.0, max_cfl=5.0, rhs_iterations=0,
+                                            max_iterations=iterations, tolerance=1e-13)),
+                     ("cfl5_to_20", rans.Settings(start_cfl=5.0, slope_cfl=1.0, max_cfl=20.0, rhs_iterations=0,
+                                                  max_iterations=iterations, tolerance=1e-13))):
+        s = rans.Solver(mesh, gas=rans.Gas(mu_L=mu), viscosity_model="spalart-allmaras")
+        s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition("wall")})
+        s.init()
+        s.refill_bcs()
+        chi, forces = [], []
+
+        class Watch:
+            updates = 0
+
+            def calc_cp(self, solver, patch):
+                self.updates += 1
+                nu = solver.get_nu()[:solver.n_real]
+                chi.append(float(f"{nu.max() / mu:.4g}"))
+                cl, cd, _ = solver.get_wall_profile("wall")
+                forces.append((float(f"{cl:.6g}"), float(f"{cd:.6g}")))
+
+        s.synchronize()
+        t0 = time.perf_counter()
+        state, res = rans.run_solver(s, st, profile=Watch(), profile_every=10)
+        s.synchronize()
+        dt = time.perf_counter() - t0
+        n = len(res)
+        out[name] = {"state": state, "iterations": n,
+                     "ms_per_iteration": round(dt * 1e3 / max(n, 1), 2),
+                     "residuals_first": [float(f"{r:.4g}") for r in res[:5]],
+                     "residuals_last": [float(f"{r:.4g}") for r in res[-5:]],
+                     "chi_max_every_10": chi, "cl_cd_every_10": forces}
+        s.close()
+    mesh.close()
     return out
 
 
@@ -1026,7 +1092,9 @@ def main():
         return
     if args.only_implicit:
         from aeroflex_amd import rans
-        print(json.dumps({"implicit": implicit_bench(rans)}), flush=True)
+        imp = implicit_bench(rans)
+        imp["sa_config1"] = sa_config1(rans)
+        print(json.dumps({"implicit": imp}), flush=True)
         return
     from aeroflex_amd import rans
     ev = HipEvents()
@@ -1128,6 +1196,7 @@ def main():
         out["adjoint"]["spalart_allmaras"] = sa_adjoint_check(rans)
     if args.implicit:
         out["implicit"] = implicit_bench(rans)
+        out["implicit"]["sa_config1"] = sa_config1(rans)
     if args.sweep_steps > 0:
         out["batched_sweep"] = batched_sweep(rans, args.sweep_steps)
     if args.aeroelastic:

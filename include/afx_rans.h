@@ -179,7 +179,8 @@ int afx_rans_explicit_residual(afx_rans* h, const double* qk, double* qW);
 int afx_rans_explicit_step(afx_rans* h, double relaxation, double* err);
 /* ---- implicit pseudo-time step (implicitSolver, solver.h:852-1213) ----
  * The linear solver replacing Eigen's GMRES + IncompleteLUT (solver.h:886,
- * 907-910): restarted GMRES(restart) with a 4x4-block right preconditioner,
+ * 907-910): restarted GMRES(restart) with a block right preconditioner
+ * (4x4 blocks; 5x5 with the Spalart-Allmaras model),
  * FP64, stopping at ||b - A x|| <= tol ||b|| or after maxit iterations
  * (defaults 30, 1e-2, 500 as the reference sets them; restart at most 32).
  * The Arnoldi state stays on the device; the host synchronises once per
@@ -193,7 +194,14 @@ int afx_rans_set_linear_solver(afx_rans* h, int32_t restart, double tol, int32_t
  * implicit step needs a new afx_rans_implicit_fill. */
 int afx_rans_set_preconditioner(afx_rans* h, int32_t kind);
 /* implicitSolver::fill -> fillRhoLHS (solver.h:973-1075): calc_dt, gradients
- * when viscous, FD flux Jacobian blocks (physics.h:533-578), A/dt diagonal */
+ * when viscous, FD flux Jacobian blocks (physics.h:533-578), A/dt diagonal.
+ * With the Spalart-Allmaras model (no counterpart: the reference's SA branch
+ * is empty, physics.h:259-261) the same rule gives 5x5 blocks of the
+ * five-variable face flux, with the cell gradients frozen and the face
+ * gradients' directional parts following the perturbed states; the diagonal
+ * blocks add the source row -A dS/dv (its diagonal entry only where >= 0);
+ * dt is the model's time step (with the viscous radius).  It needs
+ * Green-Gauss gradients and the fused residual; no matrix-free form. */
 int afx_rans_implicit_fill(afx_rans* h);
 /* implicitSolver::compute (solver.h:1160-1167): factorises the
  * preconditioner; *status = 0 on success, -1 when a diagonal block is
@@ -206,11 +214,17 @@ int afx_rans_implicit_compute(afx_rans* h, int32_t* status);
  * the residual form (AFX_FP64 or AFX_FP32); the update and solve are FP64. */
 int afx_rans_implicit_solve(afx_rans* h, double relaxation, double tol, int32_t rhs_iterations, int32_t precision,
                             double* err);
-/* y = LHS x with the matrix of the last fill (AoS, 4*n_cells); for tests */
+/* y = LHS x with the matrix of the last fill (AoS, 4*n_cells); for tests.
+ * Not with the Spalart-Allmaras model (use afx_rans_lhs_apply_nu). */
 int afx_rans_lhs_apply(afx_rans* h, const double* x, double* y);
+/* The same with the rho nu~ rows of the Spalart-Allmaras model: x_nu, y_nu
+ * (n_cells each) are required with that model and must be null otherwise. */
+int afx_rans_lhs_apply_nu(afx_rans* h, const double* x, const double* x_nu, double* y, double* y_nu);
 /* y = M^-1 x with the preconditioner of the last fill (AoS, 4*n_cells; ghost
- * rows identity); for tests */
+ * rows identity); for tests.  Not with the Spalart-Allmaras model. */
 int afx_rans_precond_apply(afx_rans* h, const double* x, double* y);
+/* The same with the rho nu~ rows (as afx_rans_lhs_apply_nu). */
+int afx_rans_precond_apply_nu(afx_rans* h, const double* x, const double* x_nu, double* y, double* y_nu);
 /* colour of each real cell (reference order, n_real entries, nullable) in the
  * DILU sweeps and the number of colours; for tests */
 int afx_rans_cell_colours(afx_rans* h, int32_t* colours, int32_t* ncolours);
@@ -247,6 +261,12 @@ int afx_rans_cp_profile(afx_rans* h, const char* patch, int64_t* n, double* x, d
 /* mesh::compute_wall_dist (mesh.h:794-830) on the device; wall_dist out
  * (n_cells, nullable).  Requires BCs. */
 int afx_rans_wall_dist(afx_rans* h, double* wall_dist);
+/* The wall distance the Spalart-Allmaras model uses (no counterpart: the
+ * reference's SA branch is empty): from each cell centroid to the nearest
+ * point of the wall / slip-wall face segments, 1 without such faces.  The
+ * face centres of mesh::compute_wall_dist overestimate it up to a/2 times on
+ * wall cells of aspect ratio a.  dist out (n_cells, nullable); SA only. */
+int afx_rans_sa_wall_dist(afx_rans* h, double* dist);
 /* counts[5] as afx_mesh_counts */
 int afx_rans_counts(afx_rans* h, int64_t counts[5]);
 /* save (post.h:58-180): the reference's ASCII VTU of the current state (real

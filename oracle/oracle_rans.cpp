@@ -30,6 +30,11 @@
 //   FD convective Jacobian                pinned: vs analytic-free central FD
 //   SA source terms (own, no reference)   pinned: hand-evaluated terms and
 //                                         nu~=0 reduction to laminar
+//   SA implicit matrix (own)              pinned: its face function equals the
+//                                         residual's first-order flux, blocks
+//                                         and source row vs central FD, the
+//                                         assembly vs a numpy assembly of the
+//                                         blocks (tests/test_oracle_sa_implicit.py)
 //   get_uniform_residual (stale qW, A8)   pinned: slip-wall channel closed
 //                                         form 2 rho v sabs(c~) |L|; second
 //                                         call doubles it
@@ -1102,11 +1107,41 @@ inline double source(double rho, double nut, double mu, double omega, double d, 
 }
 }  // namespace sa
 
+// The SA model's wall distance: from each cell centroid to the nearest point
+// of the wall and slip-wall face segments, the face set of compute_wall_dist
+// (mesh.h:794-830), which measures to the face centres instead; on a wall
+// cell of aspect ratio a that overestimates the distance up to a/2-fold, and
+// the model's destruction ~ 1/d^2 with it.  1 when there is no such face, as
+// there.  Point-to-segment: t = clamp(((p - a).(b - a)) / |b - a|^2, 0, 1),
+// d = |p - (a + t (b - a))|, a and b the face's nodes in edge order.
+static void model_wall_dist(const Mesh& m, const std::vector<int>& bface_type, std::vector<double>& dist) {
+    dist.assign(m.nCells(), 1.0);
+    for (u32 i = 0; i < m.nCells(); ++i) {
+        const double px = m.cellsCentersX[i], py = m.cellsCentersY[i];
+        double mind = 1;
+        bool first = true;
+        for (size_t j = 0; j < m.boundaryEdges.size(); ++j) {
+            if (bface_type[j] != BC_WALL && bface_type[j] != BC_SLIP) continue;
+            const u32 e = m.boundaryEdges[j];
+            const double ax = m.nodesX[m.edgesNodes[2 * e]], ay = m.nodesY[m.edgesNodes[2 * e]];
+            const double dx = m.nodesX[m.edgesNodes[2 * e + 1]] - ax, dy = m.nodesY[m.edgesNodes[2 * e + 1]] - ay;
+            const double ll = dx * dx + dy * dy;
+            double t = ((px - ax) * dx + (py - ay) * dy) / ll;
+            t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+            const double ex = px - (ax + t * dx), ey = py - (ay + t * dy);
+            const double d = std::sqrt(ex * ex + ey * ey);
+            if (first) { mind = d; first = false; } else mind = std::min(mind, d);
+        }
+        dist[i] = mind;
+    }
+}
+
 enum { SA_CONV = 1, SA_DIFF = 2, SA_SOURCE = 4, SA_MUT = 8, SA_WALL = 16, SA_ALL = 31 };
 
 struct SASolver {
     Solver* s;
     std::vector<double> nu, gnx, gny, lnu, TurbVector;   // rho nu~ per cell (the reference's nu)
+    std::vector<double> dist;                             // the model's wall distance (model_wall_dist)
     int terms = SA_ALL;                                   // for tests: which terms enter
     explicit SASolver(Solver* s_) : s(s_) {
         s->a1_sum_direction = false;   // textbook face gradients for every viscous term of the model
@@ -1190,12 +1225,19 @@ struct SASolver {
             }
         }
     }
-    // flux of face e (times nothing: caller multiplies by the length)
-    void face_flux5(u32 e, const double* qL, double nL, const double* qR, double nR, double f[5]) const {
+    // flux of face e (times nothing: caller multiplies by the length) at the
+    // face states (qL, nL), (qR, nR).  zL/znL, zR/znR: the cell-centre states
+    // of the owner and the neighbour, which give the directional part of the
+    // face gradients and the no-slip wall's cell-centre nu~ (null: the member
+    // state, as in the residual)
+    void face_flux5(u32 e, const double* qL, double nL, const double* qR, double nR, double f[5],
+                    const double* zL = nullptr, double znL = 0, const double* zR = nullptr, double znR = 0) const {
         const Solver& S = *s;
         const FaceFlux& F = S.ff[e];
         const Gas& g = S.g;
         const u32 c0 = S.m.edgesCells[2 * e], c1 = S.m.edgesCells[2 * e + 1];
+        if (!zL) { zL = &S.q[4 * c0]; znL = nu[c0]; }
+        if (!zR) { zR = &S.q[4 * c1]; znR = nu[c1]; }
         static const double zero[4] = {0, 0, 0, 0};
         FaceFlux inv = F;
         inv.kind = BC_INTERNAL;
@@ -1212,15 +1254,29 @@ struct SASolver {
                 const Mesh& m = S.m;
                 const double dn = (m.edgesCentersX[e] - m.cellsCentersX[c0]) * F.nx +
                                   (m.edgesCentersY[e] - m.cellsCentersY[c0]) * F.ny;
-                const double nut_c = nu[c0] / S.q[4 * c0];
+                const double nut_c = znL / zL[0];
                 f[4] -= (g.mu_L / sa::sigma) * ((0.0 - nut_c) / dn);
             }
             return;
         }
         roe_flux(inv, qL, qR, zero, zero, f);
         f[4] = (terms & SA_CONV) ? f[0] * (f[0] >= 0 ? nL / qL[0] : nR / qR[0]) : 0.0;
+        // face gradients: S.average_gradients with the textbook direction
+        // (a1_sum_direction is off for the model), written out here so the
+        // directional part can come from the centre states zL, zR
+        const Mesh& m = S.m;
+        double t0 = m.cellsCentersX[c1] - m.cellsCentersX[c0], t1 = m.cellsCentersY[c1] - m.cellsCentersY[c0];
+        const double l = std::sqrt(t0 * t0 + t1 * t1);
+        t0 /= l; t1 /= l;
         double gxa[4], gya[4];
-        S.average_gradients(gxa, gya, c0, c1);
+        for (int k = 0; k < 4; ++k) {
+            const double dir = (zR[k] - zL[k]) / l;
+            const double gxb = (S.gx[4 * c0 + k] + S.gx[4 * c1 + k]) * 0.5;
+            const double gyb = (S.gy[4 * c0 + k] + S.gy[4 * c1 + k]) * 0.5;
+            const double dot = gxb * t0 + gyb * t1;
+            gxa[k] = gxb - (dot - dir) * t0;
+            gya[k] = gyb - (dot - dir) * t1;
+        }
         double qc[4];
         for (int k = 0; k < 4; ++k) qc[k] = 0.5 * (qL[k] + qR[k]);
         const double nc = 0.5 * (nL + nR);
@@ -1246,11 +1302,7 @@ struct SASolver {
         if (terms & SA_DIFF) {
             // face gradient of rho nu~ (average_gradients with the textbook
             // direction), then grad nu~ = (grad(rho nu~) - nu~ grad rho) / rho
-            const Mesh& m = S.m;
-            double t0 = m.cellsCentersX[c1] - m.cellsCentersX[c0], t1 = m.cellsCentersY[c1] - m.cellsCentersY[c0];
-            const double l = std::sqrt(t0 * t0 + t1 * t1);
-            t0 /= l; t1 /= l;
-            const double dir = (nu[c1] - nu[c0]) / l;
+            const double dir = (znR - znL) / l;
             const double gxb = (gnx[c0] + gnx[c1]) * 0.5, gyb = (gny[c0] + gny[c1]) * 0.5;
             const double dot = gxb * t0 + gyb * t1;
             const double gfx = gxb - (dot - dir) * t0, gfy = gyb - (dot - dir) * t1;
@@ -1260,14 +1312,13 @@ struct SASolver {
             f[4] -= dcoef * (F.nx * gvx + F.ny * gvy);
         }
     }
-    // fillRhoRHS (solver.h:1079-1152) with the fifth equation; RhoVector and
-    // TurbVector are the un-divided residuals, ghost rows zero
-    void fill_rhs() {
+    // the part of fillRhoRHS before the limiters (solver.h:1083-1085): wall
+    // and slip ghosts copy their owners (mean flow and rho nu~), then the
+    // unlimited gradients of the five variables
+    void prepare() {
         Solver& S = *s;
         const Mesh& m = S.m;
-        const Gas& g = S.g;
-        const u32 nR = m.nRealCells, nT = m.nCells();
-        S.calc_dt();
+        if (dist.empty()) model_wall_dist(m, S.bface_type, dist);
         S.set_walls_from_internal(S.q);
         for (size_t b = 0; b < m.boundaryEdges.size(); ++b)
             if (S.bface_type[b] == BC_WALL || S.bface_type[b] == BC_SLIP) {
@@ -1277,6 +1328,16 @@ struct SASolver {
         if (!S.gg && S.lsq.empty()) S.calc_least_squares_matrices();
         S.calc_gradients();
         gradients();
+    }
+    // fillRhoRHS (solver.h:1079-1152) with the fifth equation; RhoVector and
+    // TurbVector are the un-divided residuals, ghost rows zero
+    void fill_rhs() {
+        Solver& S = *s;
+        const Mesh& m = S.m;
+        const Gas& g = S.g;
+        const u32 nR = m.nRealCells, nT = m.nCells();
+        S.calc_dt();
+        prepare();
         if (S.second_order) { S.calc_limiters(S.q); limiters(); }
         std::fill(S.RhoVector.begin(), S.RhoVector.end(), 0.0);
         std::fill(TurbVector.begin(), TurbVector.end(), 0.0);
@@ -1351,17 +1412,117 @@ struct SASolver {
             S.dt[i] = S.cfl * m.cellsAreas[i] / (sc[i] + 4.0 * (sv[i] / m.cellsAreas[i]));
     }
     // per-volume source of cell i from its unlimited gradients
-    double cell_source(u32 i) const {
+    double cell_source(u32 i) const { return source_at(i, &s->q[4 * (size_t)i], nu[i]); }
+    // the same at the cell state (qi, ni) with the cell's gradients
+    double source_at(u32 i, const double* qi, double ni) const {
         const Solver& S = *s;
-        const double* qi = &S.q[4 * i];
         const double* gxi = &S.gx[4 * i];
         const double* gyi = &S.gy[4 * i];
-        const double rho = qi[0], nut = nu[i] / rho;
+        const double rho = qi[0], nut = ni / rho;
         const double dudy = (gyi[1] - qi[1] / rho * gyi[0]) / rho;
         const double dvdx = (gxi[2] - qi[2] / rho * gxi[0]) / rho;
         const double omega = std::fabs(dvdx - dudy);
         const double gvx = (gnx[i] - nut * gxi[0]) / rho, gvy = (gny[i] - nut * gyi[0]) / rho;
-        return sa::source(rho, nut, S.g.mu_L, omega, S.m.wall_dist[i], gvx, gvy);
+        return sa::source(rho, nut, S.g.mu_L, omega, dist[i], gvx, gvy);
+    }
+
+    // ---- implicit step of the model.  The reference's SA branch is empty
+    // (physics.h:259-261), so this Jacobian is the build's own, made by the
+    // reference's rule for the mean flow (calc_convective_jacobian,
+    // physics.h:533-578: forward differences, h = max(1e-6, 1e-6 |v|), the
+    // perturbed component restored by subtraction, first-order cell states,
+    // times the face length) applied to the five-variable face flux.  The
+    // cell gradients stay frozen; the face gradients' directional parts
+    // (v_n - v_o)/l and the no-slip wall's cell-centre nu~ follow the
+    // perturbed states, so the viscous and nu~ diffusion couplings along the
+    // cell connection enter the blocks.  The rho nu~ column's step floor is
+    // 1e-6 mu_L instead of 1e-6: rho nu~ is of the order mu_L chi, so the
+    // absolute floor would perturb it by O(1) wherever chi is O(1).
+    // J: 10 x 10 row-major; rows 0-4 the owner's flux, rows 5-9 their
+    // negatives; columns 0-4 the owner's variables, 5-9 the neighbour's.
+    void face_jacobian5(u32 e, double J[100]) const {
+        const Solver& S = *s;
+        const u32 c0 = S.m.edgesCells[2 * e], c1 = S.m.edgesCells[2 * e + 1];
+        double a[5], b[5], f0[5], fp[5];
+        for (int k = 0; k < 4; ++k) { a[k] = S.q[4 * (size_t)c0 + k]; b[k] = S.q[4 * (size_t)c1 + k]; }
+        a[4] = nu[c0];
+        b[4] = nu[c1];
+        auto flux = [&](double* out) { face_flux5(e, a, a[4], b, b[4], out, a, a[4], b, b[4]); };
+        flux(f0);
+        const double L = S.m.edgesLengths[e];
+        for (int i = 0; i < 5; ++i) {
+            for (int side = 0; side < 2; ++side) {
+                double* v = side ? b : a;
+                const double up = std::max(i == 4 ? 1e-6 * S.g.mu_L : 1e-6, std::fabs(v[i]) * 1e-6);
+                v[i] += up;
+                flux(fp);
+                v[i] -= up;
+                for (int r = 0; r < 5; ++r) {
+                    const double d = (fp[r] - f0[r]) / up;
+                    J[10 * r + 5 * side + i] = d * L;
+                    J[10 * (r + 5) + 5 * side + i] = -d * L;
+                }
+            }
+        }
+    }
+    // row 4 of cell i's own block from the source: -A dS/dv_k at frozen
+    // gradients (same step rule and rho nu~ floor), the LHS of R_4 = ... + A S.  The diagonal
+    // entry is kept only where it is >= 0 (the source's net derivative is
+    // destruction): production stays explicit, so the source never lowers the
+    // diagonal of the nu~ row.
+    void source_row(u32 i, double out[5]) const {
+        const Solver& S = *s;
+        double v[5];
+        for (int k = 0; k < 4; ++k) v[k] = S.q[4 * (size_t)i + k];
+        v[4] = nu[i];
+        const double A = S.m.cellsAreas[i];
+        const double s0 = source_at(i, v, v[4]);
+        for (int k = 0; k < 5; ++k) {
+            const double up = std::max(k == 4 ? 1e-6 * S.g.mu_L : 1e-6, std::fabs(v[k]) * 1e-6);
+            v[k] += up;
+            const double sp = source_at(i, v, v[4]);
+            v[k] -= up;
+            out[k] = -A * (sp - s0) / up;
+        }
+        if (out[4] < 0) out[4] = 0;
+    }
+    // the 5N x 5N matrix of the implicit step in the AoS order [5 i + k] as
+    // COO triplets (duplicates summed by the caller): A/dt (the model's time
+    // step) on the real diagonal, the face blocks in face order as fillRhoLHS
+    // places them (solver.h:1013-1056), the source rows, identity ghost rows
+    void fill_lhs(std::vector<int64_t>& rows, std::vector<int64_t>& cols, std::vector<double>& vals) {
+        const Solver& S = *s;
+        const Mesh& m = S.m;
+        calc_dt();
+        prepare();
+        rows.clear(); cols.clear(); vals.clear();
+        const int64_t nT = m.nCells(), nR = m.nRealCells;
+        auto put = [&](int64_t r, int64_t c, double v) { rows.push_back(r); cols.push_back(c); vals.push_back(v); };
+        for (int64_t i = 0; i < nR; ++i)
+            for (int j = 0; j < 5; ++j) put(5 * i + j, 5 * i + j, m.cellsAreas[i] / S.dt[i]);
+        double J[100];
+        for (u32 e = 0; e < m.nFaces(); ++e) {
+            const int64_t k0 = 5 * (int64_t)m.edgesCells[2 * e], k1 = 5 * (int64_t)m.edgesCells[2 * e + 1];
+            face_jacobian5(e, J);
+            for (int i = 0; i < 5; ++i)
+                for (int j = 0; j < 5; ++j) {
+                    put(k0 + i, k0 + j, J[10 * i + j]);
+                    put(k0 + i, k1 + j, J[10 * i + j + 5]);
+                    if (S.ff[e].two_sided()) {
+                        put(k1 + i, k0 + j, J[10 * (i + 5) + j]);
+                        put(k1 + i, k1 + j, J[10 * (i + 5) + j + 5]);
+                    }
+                }
+        }
+        if (terms & SA_SOURCE) {
+            double r[5];
+            for (int64_t i = 0; i < nR; ++i) {
+                source_row((u32)i, r);
+                for (int k = 0; k < 5; ++k) put(5 * i + 4, 5 * i + k, r[k]);
+            }
+        }
+        for (int64_t i = nR; i < nT; ++i)
+            for (int j = 0; j < 5; ++j) put(5 * i + j, 5 * i + j, 1.0);
     }
 };
 
@@ -1534,23 +1695,41 @@ void* orc_sa_create(void* solver_h) { return new SASolver((Solver*)solver_h); }
 void orc_sa_free(void* h) { delete (SASolver*)h; }
 void orc_sa_set_terms(void* h, int terms) { ((SASolver*)h)->terms = terms; }
 void orc_sa_calc_dt(void* h) { ((SASolver*)h)->calc_dt(); }
+static void sa_wall_dist(SASolver* a);
 void orc_sa_fill_rhs(void* h) {
     SASolver* a = (SASolver*)h;
-    bool have_d = false;
-    for (double d : a->s->m.wall_dist) have_d = have_d || d != 0.0;
-    if (!have_d) compute_wall_dist(a->s->m, a->s->bface_type);
+    sa_wall_dist(a);
     a->fill_rhs();
 }
 long orc_sa_vec(void* h, const char* name, double* out, const double* in) {
     SASolver* a = (SASolver*)h;
     std::string n(name);
     std::vector<double>* v = nullptr;
+    if (n == "dist") sa_wall_dist(a);
     if (n == "nu") v = &a->nu; else if (n == "TurbVector") v = &a->TurbVector; else if (n == "gnx") v = &a->gnx;
+    else if (n == "dist") v = &a->dist;
     else if (n == "gny") v = &a->gny; else if (n == "lnu") v = &a->lnu;
     if (!v) return -1;
     if (in) std::memcpy(v->data(), in, v->size() * 8);
     if (out) std::memcpy(out, v->data(), v->size() * 8);
     return (long)v->size();
+}
+static void sa_wall_dist(SASolver* a) {
+    if (a->dist.empty()) model_wall_dist(a->s->m, a->s->bface_type, a->dist);
+}
+void orc_sa_prepare(void* h) { sa_wall_dist((SASolver*)h); ((SASolver*)h)->prepare(); }
+void orc_sa_face_jacobian(void* h, int e, double* J) { ((SASolver*)h)->face_jacobian5((u32)e, J); }
+void orc_sa_source_row(void* h, int i, double* out) { ((SASolver*)h)->source_row((u32)i, out); }
+double orc_sa_cell_source(void* h, int i, const double* q5) {
+    SASolver* a = (SASolver*)h;
+    return q5 ? a->source_at((u32)i, q5, q5[4]) : a->cell_source((u32)i);
+}
+// the five-variable face flux at face states qL5, qR5 with centre states
+// zL5, zR5 (null: the member states)
+void orc_sa_face_flux(void* h, int e, const double* qL5, const double* qR5, const double* zL5, const double* zR5,
+                      double* f) {
+    SASolver* a = (SASolver*)h;
+    a->face_flux5((u32)e, qL5, qL5[4], qR5, qR5[4], f, zL5, zL5 ? zL5[4] : 0.0, zR5, zR5 ? zR5[4] : 0.0);
 }
 double orc_sa_source(double rho, double nut, double mu, double omega, double d, double gnx, double gny) {
     return sa::source(rho, nut, mu, omega, d, gnx, gny);
@@ -1618,6 +1797,18 @@ long orc_solver_fill_lhs(void* h, int64_t* rows, int64_t* cols, double* vals) {
     return (long)g_vals.size();
 }
 void orc_solver_wall_dist(void* h) { Solver* s = (Solver*)h; compute_wall_dist(s->m, s->bface_type); }
+// the implicit step's matrix of the SA model, as orc_solver_fill_lhs
+long orc_sa_fill_lhs(void* h, int64_t* rows, int64_t* cols, double* vals) {
+    if (!rows) {
+        sa_wall_dist((SASolver*)h);
+        ((SASolver*)h)->fill_lhs(g_rows, g_cols, g_vals);
+        return (long)g_vals.size();
+    }
+    std::memcpy(rows, g_rows.data(), g_rows.size() * 8);
+    std::memcpy(cols, g_cols.data(), g_cols.size() * 8);
+    std::memcpy(vals, g_vals.data(), g_vals.size() * 8);
+    return (long)g_vals.size();
+}
 // CpProfile::calc_cp (post.h:246-290): per boundary face of the patch, in
 // boundary order, the face centre and cp = 2/(gamma M_inf^2) (p/p_inf - 1)
 // of the owner cell (sol.p, core.h:145-147); returns the face count (arrays

@@ -69,6 +69,12 @@ def _load():
         "orc_sa_source": (C.c_double, [C.c_double] * 7),
         "orc_sa_fw": (C.c_double, [C.c_double]),
         "orc_sa_fv1": (C.c_double, [C.c_double]),
+        "orc_sa_prepare": (None, [P]),
+        "orc_sa_face_jacobian": (None, [P, C.c_int, dp]),
+        "orc_sa_source_row": (None, [P, C.c_int, dp]),
+        "orc_sa_cell_source": (C.c_double, [P, C.c_int, dp]),
+        "orc_sa_face_flux": (None, [P, C.c_int, dp, dp, dp, dp, dp]),
+        "orc_sa_fill_lhs": (C.c_long, [P, P, P, P]),
     }
     for n, (r, a) in sig.items():
         f = getattr(lib, n)
@@ -167,6 +173,47 @@ class OSolver:
         """fillRhoRHS with the SA equation: (RhoVector (4N), TurbVector (N))."""
         lib.orc_sa_fill_rhs(self.sa)
         return self.vec("RhoVector"), self.sa_vec("TurbVector")
+
+    def sa_prepare(self):
+        """Wall and slip ghosts from their owners, unlimited gradients of the
+        five variables (and the wall distance once)."""
+        lib.orc_sa_prepare(self.sa)
+
+    def sa_face_jacobian(self, e):
+        """The SA implicit step's 10 x 10 face block of face e (times its length)."""
+        J = np.zeros(100)
+        lib.orc_sa_face_jacobian(self.sa, e, _d(J))
+        return J.reshape(10, 10)
+
+    def sa_source_row(self, i):
+        """-A dS/dv of cell i (diagonal entry clipped at 0)."""
+        out = np.zeros(5)
+        lib.orc_sa_source_row(self.sa, i, _d(out))
+        return out
+
+    def sa_cell_source(self, i, q5=None):
+        """Per-volume source of cell i at its state (or at q5) with its gradients."""
+        return lib.orc_sa_cell_source(self.sa, i, None if q5 is None else _d(np.ascontiguousarray(q5, np.float64)))
+
+    def sa_face_flux(self, e, qL5, qR5, zL5=None, zR5=None):
+        """Five-variable face flux (without the length) at face states qL5, qR5
+        with centre states zL5, zR5 (None: the member states)."""
+        f = np.zeros(5)
+        a = [np.ascontiguousarray(v, np.float64) if v is not None else None for v in (qL5, qR5, zL5, zR5)]
+        lib.orc_sa_face_flux(self.sa, e, _d(a[0]), _d(a[1]), None if a[2] is None else _d(a[2]),
+                             None if a[3] is None else _d(a[3]), _d(f))
+        return f
+
+    def sa_fill_lhs(self):
+        """The SA implicit step's matrix as a scipy CSR matrix (5N x 5N, AoS [5 i + k])."""
+        import scipy.sparse as sp
+        n = lib.orc_sa_fill_lhs(self.sa, None, None, None)
+        r = np.empty(n, np.int64)
+        c = np.empty(n, np.int64)
+        v = np.empty(n)
+        lib.orc_sa_fill_lhs(self.sa, _v(r), _v(c), _v(v))
+        N = 5 * self.n_cells
+        return sp.coo_matrix((v, (r, c)), shape=(N, N)).tocsr()
 
     def set_bcs(self, bcs):
         """bcs: {name: (type_str, (mach, angle, T, p))}"""

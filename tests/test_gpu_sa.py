@@ -143,5 +143,17 @@ def test_sa_init_and_refill_freestream_nu():
     s.init()
     s.refill_bcs()
     assert np.allclose(s.get_nu(), 4.0 * 2e-5, rtol=0, atol=0)
-    with pytest.raises(rans.RansError):   # the implicit step has no SA Jacobian
-        s.fill()
+    with pytest.raises(rans.RansError):   # the matrix-free operator is the four-variable one
+        s.set_matrix_free(True)
+
+
+@pytest.mark.parametrize("path,wall", [(TRI, "wall"), (QUAD, "slip-wall")])
+def test_sa_wall_distance_matches_oracle(path, wall):
+    """The model's wall distance (to the nearest point of the wall face
+    segments) on the device equals the oracle's bit for bit; the reference's
+    distance to the face centres stays what afx_rans_wall_dist returns."""
+    s, o = sa_pair(path, wall=wall)
+    d = s.sa_wall_dist()
+    assert np.array_equal(d, o.sa_vec("dist"))
+    o.wall_dist()
+    assert np.array_equal(s.wall_dist(), o.vec("wall_dist"))

@@ -255,3 +255,34 @@ def test_wall_diffusion_is_the_closed_form_sink():
     diff = T_all - T_nowall
     assert np.count_nonzero(expect) > 0
     assert np.max(np.abs(diff - expect)) <= 1e-12 * np.max(np.abs(expect))
+
+
+@pytest.mark.parametrize("path,wall", [(TRI, "wall"), (QUAD, "slip-wall")])
+def test_model_wall_distance_brute_force(path, wall):
+    """The model's wall distance (centroid to the nearest point of the wall and
+    slip-wall face segments) against a numpy brute force over all face
+    segments; never above the reference's distance to the face centres
+    (mesh.h:794-830), and below it on first wall cells whose wall face centre
+    is off the foot of the normal (on these near-isotropic meshes by little:
+    the anisotropic O-grid's wall cells are where it matters)."""
+    o, q = sa_pair(path, wall=wall)
+    d = o.sa_vec("dist")
+    om = o.mesh
+    be = om.get("boundaryEdges")
+    on_wall = np.array(om.boundary_names()) == "wall"
+    en = om.get("edgesNodes").reshape(-1, 2)[be[on_wall]]
+    x, y = om.get("nodesX"), om.get("nodesY")
+    ax, ay, bx, by = x[en[:, 0]], y[en[:, 0]], x[en[:, 1]], y[en[:, 1]]
+    cx, cy = om.get("cellsCentersX"), om.get("cellsCentersY")
+    ref = np.empty_like(cx)
+    for i in range(len(cx)):
+        # parametric foot of the perpendicular, clamped to the segment
+        ux, uy = bx - ax, by - ay
+        t = np.clip(((cx[i] - ax) * ux + (cy[i] - ay) * uy) / (ux * ux + uy * uy), 0.0, 1.0)
+        ref[i] = np.min(np.hypot(cx[i] - ax - t * ux, cy[i] - ay - t * uy))
+    assert np.allclose(d, ref, rtol=1e-13, atol=0)
+    o.wall_dist()
+    dc = o.vec("wall_dist")
+    assert np.all(d <= dc * (1 + 1e-14))
+    first = np.unique(om.get("edgesCells").reshape(-1, 2)[be[on_wall], 0])
+    assert np.sum(d[first] < dc[first] * (1 - 1e-6)) > len(first) // 2
