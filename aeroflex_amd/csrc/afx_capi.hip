@@ -142,6 +142,10 @@ struct afx_rans {
     // implicit step: Jacobian blocks, diagonal blocks and their inverses,
     // GMRES work space and state (kg: the GmDev arrays, kflags its flags)
     DBuf<double> Jf, Dblk, Dinv, kV, kw, kz, kr, ktmp, kpart, kg;
+    DBuf<double> ellE, ellD;   // ELL copy of the matrix for the product (launch_assemble_diag)
+    DBuf<int32_t> ellN;
+    DBuf<double> dilE, dilD;   // colour-ordered copy for the DILU sweeps (launch_dilu_setup)
+    DBuf<int32_t> dilN;
     DBuf<int> kbad, kflags;
     int32_t gm_restart = 30, gm_maxit = 500;
     double gm_tol = 1e-2;
@@ -504,6 +508,12 @@ int ensure_implicit(afx_rans* h) {
     HIPCHK(h->kflags.alloc(8));
     HIPCHK(h->kbad.alloc(1));
     HIPCHK(h->Dti.alloc(nn * h->nC));
+    HIPCHK(h->ellE.alloc(4 * nn * h->nC));
+    HIPCHK(h->ellD.alloc(nn * h->nC));
+    HIPCHK(h->ellN.alloc(4 * (size_t)h->nC));
+    HIPCHK(h->dilE.alloc(4 * nn * h->nC));
+    HIPCHK(h->dilD.alloc(nn * h->nC));
+    HIPCHK(h->dilN.alloc(4 * (size_t)h->nC));
     HIPCHK(hipMemsetAsync(h->Jf.p, 0, 2 * nn * h->nF * 8, h->stream));
     HIPCHK(hipMemsetAsync(h->kflags.p, 0, 8 * sizeof(int), h->stream));
     // greedy colouring of the real cells in device order over their
@@ -569,14 +579,14 @@ int gmres(afx_rans* h, const double* b, double* x, bool* converged) {
         if (h->mf)
             return launch_mf_apply(s, h->cells64(), h->faces64(), h->q.p, h->dt.p, h->mfFb.p, h->gas64(),
                                    h->opts.viscosity, in, out, act);
-        return launch_lhs_apply(s, h->nv, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, in, out, act);
+        return launch_lhs_apply(s, h->nv, h->nC, h->nT, h->ellD.p, h->ellE.p, h->ellN.p, in, out, act);
     };
     auto precond = [&](const double* in, double* out, const double** res, const int* act) -> hipError_t {
         if (h->mf) { *res = in; return hipSuccess; }
         *res = out;
         if (h->precond == 1)
-            return launch_dilu_apply(s, h->nv, h->cells64(), h->faces64(), h->colour.p, h->col_cells.p,
-                                     h->col_off.data(), (int)h->col_off.size() - 1, h->Jf.p, h->Dti.p, in, out, act);
+            return launch_dilu_apply(s, h->nv, h->nC, h->nT, h->col_cells.p, h->col_off.data(),
+                                     (int)h->col_off.size() - 1, h->dilE.p, h->dilN.p, h->dilD.p, in, out, act);
         return launch_block_jacobi(s, h->nv, h->nC, h->nT, h->Dinv.p, in, out, act);
     };
     // |r|^2 of the cycle's start vector into hdr[3], then the cycle's set-up
@@ -1557,11 +1567,11 @@ int afx_rans_implicit_fill(afx_rans* h) {
     }
     HIPCHK(hipMemsetAsync(h->kbad.p, 0, sizeof(int), s));
     HIPCHK(launch_assemble_diag(s, h->nv, h->cells64(), h->faces64(), h->dt.p, h->Jf.p, src, h->Dblk.p, h->Dinv.p,
-                                h->kbad.p));
+                                h->ellE.p, h->ellD.p, h->ellN.p, h->kbad.p));
     if (h->precond == 1)
         HIPCHK(launch_dilu_setup(s, h->nv, h->cells64(), h->faces64(), h->colour.p, h->col_cells.p,
                                  h->col_off.data(), (int)h->col_off.size() - 1, h->Dblk.p, h->Jf.p, h->Dti.p,
-                                 h->kbad.p));
+                                 h->dilE.p, h->dilN.p, h->dilD.p, h->kbad.p));
     HIPCHK(hipMemcpyAsync(&h->lhs_bad, h->kbad.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     h->lhs_ready = true;
@@ -1640,7 +1650,7 @@ int afx_rans_lhs_apply_nu(afx_rans* h, const double* x, const double* x_nu, doub
     if (!h->lhs_ready) return fail(AFX_ESTATE, "call afx_rans_implicit_fill first");
     if (h->mf) return fail(AFX_ESTATE, "no assembled matrix in matrix-free mode (afx_rans_matrix_free_apply)");
     if ((rc = upload_operand(h, x, x_nu, y_nu))) return rc;
-    HIPCHK(launch_lhs_apply(h->stream, h->nv, h->cells64(), h->faces64(), h->Dblk.p, h->Jf.p, h->ktmp.p, h->kw.p));
+    HIPCHK(launch_lhs_apply(h->stream, h->nv, h->nC, h->nT, h->ellD.p, h->ellE.p, h->ellN.p, h->ktmp.p, h->kw.p));
     return download_operand(h, y, y_nu);
 }
 
@@ -1656,8 +1666,8 @@ int afx_rans_precond_apply_nu(afx_rans* h, const double* x, const double* x_nu, 
     if (h->mf) return fail(AFX_ESTATE, "the matrix-free mode has no preconditioner");
     if ((rc = upload_operand(h, x, x_nu, y_nu))) return rc;
     if (h->precond == 1)
-        HIPCHK(launch_dilu_apply(h->stream, h->nv, h->cells64(), h->faces64(), h->colour.p, h->col_cells.p,
-                                 h->col_off.data(), (int)h->col_off.size() - 1, h->Jf.p, h->Dti.p, h->ktmp.p, h->kw.p));
+        HIPCHK(launch_dilu_apply(h->stream, h->nv, h->nC, h->nT, h->col_cells.p, h->col_off.data(),
+                                 (int)h->col_off.size() - 1, h->dilE.p, h->dilN.p, h->dilD.p, h->ktmp.p, h->kw.p));
     else
         HIPCHK(launch_block_jacobi(h->stream, h->nv, h->nC, h->nT, h->Dinv.p, h->ktmp.p, h->kw.p));
     return download_operand(h, y, y_nu);

@@ -180,27 +180,45 @@ __device__ bool invN(const double* A, double* X) {
 // cell's own-state blocks of each of its faces in reference face order
 // (owner: J00, neighbour: the lower block -J01); then their inverses.
 // N = 5 adds the source row: -A dS/dv at the frozen gradients (same step
-// rule and rho nu~ floor), its diagonal entry only where >= 0 (destruction), so production stays
-// explicit and never lowers the diagonal of the nu~ row.
+// rule and rho nu~ floor), its diagonal entry only where >= 0 (destruction),
+// so production stays explicit and never lowers the diagonal of the nu~ row.
+// The same pass over the cell's faces writes the ELL copy of its row for the
+// product (k_lhs_apply): per slot s the off-diagonal block (-J00 where the
+// cell is the face's neighbour, J01 where it is its owner; zero on empty
+// slots and boundary faces) at E[(s NN + k) nC + c], the other cell at
+// nbr[s nC + c] (-1: none), the diagonal block at Ds[k nC + c].  Every ELL
+// array is read with consecutive lanes on consecutive addresses.
 template <int N>
 __global__ void __launch_bounds__(kBlock) k_assemble_diag(CellsT<double> C, FacesT<double> F, const double* __restrict__ dt,
                                                           const double* __restrict__ Jf, SaSource src,
                                                           double* __restrict__ D, double* __restrict__ Dinv,
-                                                          int* __restrict__ bad) {
+                                                          double* __restrict__ E, double* __restrict__ Ds,
+                                                          int32_t* __restrict__ nbr, int* __restrict__ bad) {
     constexpr int NN = N * N;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C.nC) return;
+    const size_t nC = (size_t)C.nC;
     double B[NN];
     const double ad = C.area[c] / dt[c];
     for (int k = 0; k < NN; ++k) B[k] = 0.0;
     for (int k = 0; k < N; ++k) B[(N + 1) * k] = ad;
     for (int s = 0; s < 4; ++s) {
-        uint32_t e = C.cface[(size_t)s * C.nC + c];
-        if (e == 0xFFFFFFFFu) break;
-        const int f = (int)(e & 0x7FFFFFFFu);
-        const double* J = Jf + (size_t)2 * NN * f;
-        if (e >> 31) for (int k = 0; k < NN; ++k) B[k] += -J[NN + k];
-        else for (int k = 0; k < NN; ++k) B[k] += J[k];
+        const uint32_t e = C.cface[(size_t)s * nC + c];
+        int other = -1;
+        if (e != 0xFFFFFFFFu) {
+            const int f = (int)(e & 0x7FFFFFFFu);
+            const double* J = Jf + (size_t)2 * NN * f;
+            const bool nb = (e >> 31) != 0;
+            if (nb) for (int k = 0; k < NN; ++k) B[k] += -J[NN + k];
+            else for (int k = 0; k < NN; ++k) B[k] += J[k];
+            if (F.kind[f] == K_INTERNAL) {
+                other = nb ? F.c0[f] : F.c1[f];
+                for (int k = 0; k < NN; ++k) E[((size_t)s * NN + k) * nC + c] = nb ? -J[k] : J[NN + k];
+            }
+        }
+        nbr[s * nC + c] = other;
+        if (other < 0)
+            for (int k = 0; k < NN; ++k) E[((size_t)s * NN + k) * nC + c] = 0.0;
     }
     if constexpr (N == 5) {
         if (src.q) {
@@ -225,7 +243,10 @@ __global__ void __launch_bounds__(kBlock) k_assemble_diag(CellsT<double> C, Face
             }
         }
     }
-    for (int k = 0; k < NN; ++k) D[(size_t)NN * c + k] = B[k];
+    for (int k = 0; k < NN; ++k) {
+        D[(size_t)NN * c + k] = B[k];
+        Ds[k * nC + c] = B[k];
+    }
     double X[NN];
     if (!invN<N>(B, X)) {
         atomicAdd(bad, 1);
@@ -234,63 +255,47 @@ __global__ void __launch_bounds__(kBlock) k_assemble_diag(CellsT<double> C, Face
     for (int k = 0; k < NN; ++k) Dinv[(size_t)NN * c + k] = X[k];
 }
 
-// y = LHS x (ghost rows identity; ghost columns dropped since x_ghost = 0).
-// The cell's four slot words, then its faces' kinds and neighbours, then the
-// blocks and the neighbours' x are loaded as independent batches, so the
-// gathers of all slots are in flight together.
+// y = LHS x (ghost rows identity; ghost columns dropped since x_ghost = 0)
+// from the ELL copy written by k_assemble_diag.  Per row: the diagonal block
+// times x_c, then each slot's block times the neighbour's x in slot order
+// (the sign sits in the block instead of in x, exactly).
 template <int N>
-__global__ void __launch_bounds__(kBlock) k_lhs_apply(CellsT<double> C, FacesT<double> F, const double* __restrict__ D,
-                                                      const double* __restrict__ Jf, const double* __restrict__ x,
-                                                      double* __restrict__ y, const int* __restrict__ active) {
+__global__ void __launch_bounds__(kBlock) k_lhs_apply(int32_t nC, int32_t nT, const double* __restrict__ Ds,
+                                                      const double* __restrict__ E, const int32_t* __restrict__ nbr,
+                                                      const double* __restrict__ x, double* __restrict__ y,
+                                                      const int* __restrict__ active) {
     constexpr int NN = N * N;
     if (active && *active == 0) return;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= C.nT) return;
-    const size_t nT = (size_t)C.nT;
-    if (c >= C.nC) {
-        for (int k = 0; k < N; ++k) y[k * nT + c] = x[k * nT + c];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nT) return;
+    const size_t n = (size_t)nT, m = (size_t)nC;
+    if (c >= nC) {
+        for (int k = 0; k < N; ++k) y[k * n + c] = x[k * n + c];
         return;
     }
-    uint32_t e[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) e[s] = C.cface[(size_t)s * C.nC + c];
     int other[4];
-    bool use[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        use[s] = false;
-        other[s] = 0;
-        if (e[s] != 0xFFFFFFFFu) {
-            const int f = (int)(e[s] & 0x7FFFFFFFu);
-            use[s] = F.kind[f] == K_INTERNAL;
-            other[s] = (e[s] >> 31) ? F.c0[f] : F.c1[f];
-        }
-    }
+    for (int s = 0; s < 4; ++s) other[s] = nbr[s * m + c];
     double xc[N], acc[N];
-    for (int k = 0; k < N; ++k) xc[k] = x[k * nT + c];
-    const double* B = D + (size_t)NN * c;
+    for (int k = 0; k < N; ++k) xc[k] = x[k * n + c];
     for (int r = 0; r < N; ++r) {
         double t = 0.0;
-        for (int k = 0; k < N; ++k) t += B[N * r + k] * xc[k];
+        for (int k = 0; k < N; ++k) t += Ds[(r * N + k) * m + c] * xc[k];
         acc[r] = t;
     }
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        if (!use[s]) continue;
-        const int f = (int)(e[s] & 0x7FFFFFFFu);
-        const bool nb = (e[s] >> 31) != 0;
-        // neighbour row: block (c1, c0) = -J00; owner row: block (c0, c1) = J01
-        const double* J = Jf + (size_t)2 * NN * f + (nb ? 0 : NN);
-        const double sg = nb ? -1.0 : 1.0;
+        if (other[s] < 0) continue;
         double xo[N];
-        for (int k = 0; k < N; ++k) xo[k] = sg * x[k * nT + other[s]];
+        for (int k = 0; k < N; ++k) xo[k] = x[k * n + other[s]];
+        const double* B = E + (size_t)s * NN * m + c;
         for (int r = 0; r < N; ++r) {
             double t = 0.0;
-            for (int k = 0; k < N; ++k) t += J[N * r + k] * xo[k];
+            for (int k = 0; k < N; ++k) t += B[(r * N + k) * m] * xo[k];
             acc[r] += t;
         }
     }
-    for (int k = 0; k < N; ++k) y[k * nT + c] = acc[k];
+    for (int k = 0; k < N; ++k) y[k * n + c] = acc[k];
 }
 
 // z = Dinv x (ghost rows identity)
@@ -350,15 +355,26 @@ __device__ __forceinline__ const double* offdiag_t(const double* Jf, uint32_t e,
     *sgn = -1.0; return J;
 }
 
+// Set-up of colour c over positions p0 .. p0 + ncells of the colour order.
+// Besides Dt_i^-1 (AoS, read back by the later colours' set-up) it writes the
+// sweeps' colour-ordered ELL copy of row i at its position p: the
+// off-diagonal blocks A_ij by slot at Ec[(s NN + k) nC + p], the neighbour
+// code nc[s nC + p] = j + 1 for a neighbour of an earlier colour, -(j + 1)
+// for a later one, 0 for none, and Dt_i^-1 at Dc[k nC + p], so the threads of
+// one colour's sweep read consecutive addresses.
 template <int N>
 __global__ void __launch_bounds__(kBlock) k_dilu_setup(CellsT<double> C, FacesT<double> F, const uint8_t* __restrict__ colour,
-                                                       int c, const int32_t* __restrict__ cells, int32_t ncells,
-                                                       const double* __restrict__ D, const double* __restrict__ Jf,
-                                                       double* __restrict__ Dti, int* __restrict__ bad) {
+                                                       int c, const int32_t* __restrict__ cells, int32_t p0,
+                                                       int32_t ncells, const double* __restrict__ D,
+                                                       const double* __restrict__ Jf, double* __restrict__ Dti,
+                                                       double* __restrict__ Ec, int32_t* __restrict__ nc,
+                                                       double* __restrict__ Dc, int* __restrict__ bad) {
     constexpr int NN = N * N;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncells) return;
-    const int i = cells[k];
+    const size_t mc = (size_t)C.nC;
+    const int p = p0 + k;
+    const int i = cells[p];
     double B[NN];
     for (int r = 0; r < NN; ++r) B[r] = D[(size_t)NN * i + r];
     for (int s = 0; s < 4; ++s) {
@@ -390,67 +406,69 @@ __global__ void __launch_bounds__(kBlock) k_dilu_setup(CellsT<double> C, FacesT<
         atomicAdd(bad, 1);
         for (int r = 0; r < NN; ++r) X[r] = 0.0;
     }
-    for (int r = 0; r < NN; ++r) Dti[(size_t)NN * i + r] = X[r];
+    for (int r = 0; r < NN; ++r) {
+        Dti[(size_t)NN * i + r] = X[r];
+        Dc[r * mc + p] = X[r];
+    }
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t e = C.cface[(size_t)s * mc + i];
+        int code = 0;
+        if (e != 0xFFFFFFFFu && F.kind[e & 0x7FFFFFFFu] == K_INTERNAL) {
+            int j;
+            double sa;
+            const double* Aij = offdiag<N>(F, Jf, e, &j, &sa);
+            code = colour[j] < c ? j + 1 : -(j + 1);
+            for (int r = 0; r < NN; ++r) Ec[((size_t)s * NN + r) * mc + p] = sa * Aij[r];
+        } else {
+            for (int r = 0; r < NN; ++r) Ec[((size_t)s * NN + r) * mc + p] = 0.0;
+        }
+        nc[s * mc + p] = code;
+    }
 }
 
 // forward sweep of colour c: y_i = Dt_i^-1 (b_i - sum_{j before i} A_ij y_j)
 // backward sweep of colour c (in place): y_i -= Dt_i^-1 sum_{j after i} A_ij y_j
+// over positions p0 .. p0 + ncells of the colour order, from the set-up's
+// colour-ordered copy (the sign of A_ij sits in the block, exactly)
 template <int N, bool FORWARD>
-__global__ void __launch_bounds__(kBlock) k_dilu_sweep(CellsT<double> C, FacesT<double> F, const uint8_t* __restrict__ colour,
-                                                       int c, const int32_t* __restrict__ cells, int32_t ncells,
-                                                       const double* __restrict__ Jf, const double* __restrict__ Dti,
-                                                       const double* __restrict__ b, double* __restrict__ y,
-                                                       const int* __restrict__ active) {
+__global__ void __launch_bounds__(kBlock) k_dilu_sweep(int32_t nC, int32_t nT, int32_t p0, int32_t ncells,
+                                                       const int32_t* __restrict__ cells,
+                                                       const double* __restrict__ Ec, const int32_t* __restrict__ nc,
+                                                       const double* __restrict__ Dc, const double* __restrict__ b,
+                                                       double* __restrict__ y, const int* __restrict__ active) {
     constexpr int NN = N * N;
     if (active && *active == 0) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncells) return;
-    const size_t nT = (size_t)C.nT;
-    const int i = cells[k];
-    uint32_t e[4];
+    const size_t n = (size_t)nT, m = (size_t)nC;
+    const int p = p0 + k;
+    const int i = cells[p];
+    int code[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) e[s] = C.cface[(size_t)s * C.nC + i];
-    int other[4];
-    bool use[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        use[s] = false;
-        other[s] = 0;
-        if (e[s] != 0xFFFFFFFFu) {
-            const int f = (int)(e[s] & 0x7FFFFFFFu);
-            if (F.kind[f] == K_INTERNAL) {
-                other[s] = (e[s] >> 31) ? F.c0[f] : F.c1[f];
-                const int cj = colour[other[s]];
-                use[s] = FORWARD ? cj < c : cj > c;
-            }
-        }
-    }
+    for (int s = 0; s < 4; ++s) code[s] = nc[s * m + p];
     double acc[N];
-    for (int r = 0; r < N; ++r) acc[r] = FORWARD ? b[r * nT + i] : 0.0;
+    for (int r = 0; r < N; ++r) acc[r] = FORWARD ? b[r * n + i] : 0.0;
     bool any = FORWARD;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        if (!use[s]) continue;
+        if (FORWARD ? code[s] <= 0 : code[s] >= 0) continue;
         any = true;
-        const int f = (int)(e[s] & 0x7FFFFFFFu);
-        const bool nb = (e[s] >> 31) != 0;
-        const double* A = Jf + (size_t)2 * NN * f + (nb ? 0 : NN);   // A_ij: -J00 (i the neighbour) or J01
-        const double sa = nb ? -1.0 : 1.0;
+        const int j = (FORWARD ? code[s] : -code[s]) - 1;
         double yj[N];
-        for (int r = 0; r < N; ++r) yj[r] = sa * y[r * nT + other[s]];
+        for (int r = 0; r < N; ++r) yj[r] = y[r * n + j];
+        const double* A = Ec + (size_t)s * NN * m + p;
         for (int r = 0; r < N; ++r) {
             double t = 0.0;
-            for (int m = 0; m < N; ++m) t += A[N * r + m] * yj[m];
+            for (int q = 0; q < N; ++q) t += A[(r * N + q) * m] * yj[q];
             acc[r] += (FORWARD ? -1.0 : 1.0) * t;
         }
     }
     if (!any) return;
-    const double* P = Dti + (size_t)NN * i;
     for (int r = 0; r < N; ++r) {
         double t = 0.0;
-        for (int m = 0; m < N; ++m) t += P[N * r + m] * acc[m];
-        if (FORWARD) y[r * nT + i] = t;
-        else y[r * nT + i] -= t;
+        for (int q = 0; q < N; ++q) t += Dc[(r * N + q) * m + p] * acc[q];
+        if (FORWARD) y[r * n + i] = t;
+        else y[r * n + i] -= t;
     }
 }
 
@@ -694,21 +712,24 @@ hipError_t launch_face_jacobian_sa(hipStream_t s, CellsT<double> C, FacesT<doubl
     return hipGetLastError();
 }
 hipError_t launch_assemble_diag(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const double* dt,
-                                const double* Jf, SaSource src, double* D, double* Dinv, int* bad) {
+                                const double* Jf, SaSource src, double* D, double* Dinv, double* E, double* Ds,
+                                int32_t* nbr, int* bad) {
     if (nv == 5)
-        hipLaunchKernelGGL(k_assemble_diag<5>, dim3(nblocks(C.nC)), dim3(kBlock), 0, s, C, F, dt, Jf, src, D, Dinv, bad);
+        hipLaunchKernelGGL(k_assemble_diag<5>, dim3(nblocks(C.nC)), dim3(kBlock), 0, s, C, F, dt, Jf, src, D, Dinv, E,
+                           Ds, nbr, bad);
     else if (nv == 4)
-        hipLaunchKernelGGL(k_assemble_diag<4>, dim3(nblocks(C.nC)), dim3(kBlock), 0, s, C, F, dt, Jf, src, D, Dinv, bad);
+        hipLaunchKernelGGL(k_assemble_diag<4>, dim3(nblocks(C.nC)), dim3(kBlock), 0, s, C, F, dt, Jf, src, D, Dinv, E,
+                           Ds, nbr, bad);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
-hipError_t launch_lhs_apply(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const double* D, const double* Jf,
-                            const double* x, double* y, const int* active) {
+hipError_t launch_lhs_apply(hipStream_t s, int nv, int32_t nC, int32_t nT, const double* Ds, const double* E,
+                            const int32_t* nbr, const double* x, double* y, const int* active) {
     if (nv == 5)
-        hipLaunchKernelGGL(k_lhs_apply<5>, dim3(nblocks(C.nT)), dim3(kBlock), 0, s, C, F, D, Jf, x, y, active);
+        hipLaunchKernelGGL(k_lhs_apply<5>, dim3(nblocks(nT)), dim3(kBlock), 0, s, nC, nT, Ds, E, nbr, x, y, active);
     else if (nv == 4)
-        hipLaunchKernelGGL(k_lhs_apply<4>, dim3(nblocks(C.nT)), dim3(kBlock), 0, s, C, F, D, Jf, x, y, active);
+        hipLaunchKernelGGL(k_lhs_apply<4>, dim3(nblocks(nT)), dim3(kBlock), 0, s, nC, nT, Ds, E, nbr, x, y, active);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -726,12 +747,12 @@ hipError_t launch_block_jacobi(hipStream_t s, int nv, int32_t nC, int32_t nT, co
 template <int N>
 static hipError_t dilu_setup_n(hipStream_t s, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
                                const int32_t* cells, const int32_t* coff, int ncolours, const double* D,
-                               const double* Jf, double* Dti, int* bad) {
+                               const double* Jf, double* Dti, double* Ec, int32_t* nc, double* Dc, int* bad) {
     for (int c = 0; c < ncolours; ++c) {
         const int32_t n = coff[c + 1] - coff[c];
         if (n == 0) continue;
-        hipLaunchKernelGGL(k_dilu_setup<N>, dim3(nblocks(n)), dim3(kBlock), 0, s, C, F, colour, c, cells + coff[c], n, D,
-                           Jf, Dti, bad);
+        hipLaunchKernelGGL(k_dilu_setup<N>, dim3(nblocks(n)), dim3(kBlock), 0, s, C, F, colour, c, cells, coff[c], n,
+                           D, Jf, Dti, Ec, nc, Dc, bad);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -739,42 +760,41 @@ static hipError_t dilu_setup_n(hipStream_t s, CellsT<double> C, FacesT<double> F
 }
 hipError_t launch_dilu_setup(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
                              const int32_t* cells, const int32_t* coff, int ncolours, const double* D, const double* Jf,
-                             double* Dti, int* bad) {
-    if (nv == 5) return dilu_setup_n<5>(s, C, F, colour, cells, coff, ncolours, D, Jf, Dti, bad);
-    if (nv == 4) return dilu_setup_n<4>(s, C, F, colour, cells, coff, ncolours, D, Jf, Dti, bad);
+                             double* Dti, double* Ec, int32_t* nc, double* Dc, int* bad) {
+    if (nv == 5) return dilu_setup_n<5>(s, C, F, colour, cells, coff, ncolours, D, Jf, Dti, Ec, nc, Dc, bad);
+    if (nv == 4) return dilu_setup_n<4>(s, C, F, colour, cells, coff, ncolours, D, Jf, Dti, Ec, nc, Dc, bad);
     return hipErrorInvalidValue;
 }
 template <int N>
-static hipError_t dilu_apply_n(hipStream_t s, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
-                               const int32_t* cells, const int32_t* coff, int ncolours, const double* Jf,
-                               const double* Dti, const double* x, double* z, const int* active) {
+static hipError_t dilu_apply_n(hipStream_t s, int32_t nC, int32_t nT, const int32_t* cells, const int32_t* coff,
+                               int ncolours, const double* Ec, const int32_t* nc, const double* Dc, const double* x,
+                               double* z, const int* active) {
     hipError_t e;
-    if (C.nT > C.nC) {
-        hipLaunchKernelGGL(k_copy_ghost_rows, dim3(nblocks(C.nT - C.nC)), dim3(kBlock), 0, s, C.nC, C.nT, N, x, z,
-                           active);
+    if (nT > nC) {
+        hipLaunchKernelGGL(k_copy_ghost_rows, dim3(nblocks(nT - nC)), dim3(kBlock), 0, s, nC, nT, N, x, z, active);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     for (int c = 0; c < ncolours; ++c) {
         const int32_t n = coff[c + 1] - coff[c];
         if (n == 0) continue;
-        hipLaunchKernelGGL((k_dilu_sweep<N, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, C, F, colour, c,
-                           cells + coff[c], n, Jf, Dti, x, z, active);
+        hipLaunchKernelGGL((k_dilu_sweep<N, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, nC, nT, coff[c], n, cells,
+                           Ec, nc, Dc, x, z, active);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     for (int c = ncolours - 2; c >= 0; --c) {
         const int32_t n = coff[c + 1] - coff[c];
         if (n == 0) continue;
-        hipLaunchKernelGGL((k_dilu_sweep<N, false>), dim3(nblocks(n)), dim3(kBlock), 0, s, C, F, colour, c,
-                           cells + coff[c], n, Jf, Dti, x, z, active);
+        hipLaunchKernelGGL((k_dilu_sweep<N, false>), dim3(nblocks(n)), dim3(kBlock), 0, s, nC, nT, coff[c], n, cells,
+                           Ec, nc, Dc, x, z, active);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;
 }
-hipError_t launch_dilu_apply(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
-                             const int32_t* cells, const int32_t* coff, int ncolours, const double* Jf,
-                             const double* Dti, const double* x, double* z, const int* active) {
-    if (nv == 5) return dilu_apply_n<5>(s, C, F, colour, cells, coff, ncolours, Jf, Dti, x, z, active);
-    if (nv == 4) return dilu_apply_n<4>(s, C, F, colour, cells, coff, ncolours, Jf, Dti, x, z, active);
+hipError_t launch_dilu_apply(hipStream_t s, int nv, int32_t nC, int32_t nT, const int32_t* cells, const int32_t* coff,
+                             int ncolours, const double* Ec, const int32_t* nc, const double* Dc, const double* x,
+                             double* z, const int* active) {
+    if (nv == 5) return dilu_apply_n<5>(s, nC, nT, cells, coff, ncolours, Ec, nc, Dc, x, z, active);
+    if (nv == 4) return dilu_apply_n<4>(s, nC, nT, cells, coff, ncolours, Ec, nc, Dc, x, z, active);
     return hipErrorInvalidValue;
 }
 hipError_t launch_gs_pass(hipStream_t s, const double* V, int64_t ld, int m, const double* h, double* w, int64_t n,

@@ -17,22 +17,30 @@ struct SaSource {
     const double* wd;   // wall distance
     GasT<double> g;
 };
-// nv: block size, 4 or 5 (Spalart-Allmaras)
+// nv: block size, 4 or 5 (Spalart-Allmaras).  Also writes the ELL copy of
+// the matrix for launch_lhs_apply: E 4 nv^2 nC off-diagonal blocks by slot,
+// Ds nv^2 nC diagonal blocks, nbr 4 nC neighbour indices (-1: none), all
+// slot/entry-major over the real cells
 hipError_t launch_assemble_diag(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const double* dt,
-                                const double* Jf, SaSource src, double* D, double* Dinv, int* bad);
-// active: a device flag (nullable); the kernel does nothing while it reads 0
-hipError_t launch_lhs_apply(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const double* D, const double* Jf,
-                            const double* x, double* y, const int* active = nullptr);
+                                const double* Jf, SaSource src, double* D, double* Dinv, double* E, double* Ds,
+                                int32_t* nbr, int* bad);
+// y = LHS x from the ELL copy.  active: a device flag (nullable); the kernel
+// does nothing while it reads 0
+hipError_t launch_lhs_apply(hipStream_t s, int nv, int32_t nC, int32_t nT, const double* Ds, const double* E,
+                            const int32_t* nbr, const double* x, double* y, const int* active = nullptr);
 hipError_t launch_block_jacobi(hipStream_t s, int nv, int32_t nC, int32_t nT, const double* Dinv, const double* x,
                                double* z, const int* active = nullptr);
 // multicolour block DILU: colour per real cell, cells listed colour by colour
-// (cells[coff[c] .. coff[c+1]), coff on the host)
+// (cells[coff[c] .. coff[c+1]), coff on the host).  The set-up also writes
+// the sweeps' colour-ordered copy by position p of that list: Ec 4 nv^2 nC
+// off-diagonal blocks by slot, nc 4 nC neighbour codes (j + 1 earlier
+// colour, -(j + 1) later, 0 none), Dc nv^2 nC inverted diagonal blocks
 hipError_t launch_dilu_setup(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
                              const int32_t* cells, const int32_t* coff, int ncolours, const double* D, const double* Jf,
-                             double* Dti, int* bad);
-hipError_t launch_dilu_apply(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
-                             const int32_t* cells, const int32_t* coff, int ncolours, const double* Jf,
-                             const double* Dti, const double* x, double* z, const int* active = nullptr);
+                             double* Dti, double* Ec, int32_t* nc, double* Dc, int* bad);
+hipError_t launch_dilu_apply(hipStream_t s, int nv, int32_t nC, int32_t nT, const int32_t* cells, const int32_t* coff,
+                             int ncolours, const double* Ec, const int32_t* nc, const double* Dc, const double* x,
+                             double* z, const int* active = nullptr);
 
 // device-resident restarted GMRES state (m <= 32)
 struct GmDev {

@@ -1073,6 +1073,8 @@ def main():
     ap.add_argument("--no-implicit", dest="implicit", action="store_false",
                     help="skip the implicit-iteration timing")
     ap.add_argument("--only-implicit", action="store_true", help="run only the implicit-iteration timing")
+    ap.add_argument("--no-sa-config1", dest="sa_config1", action="store_false",
+                    help="skip the SA run of the 129 x 65 O-grid case in the implicit section (profiling runs)")
     ap.add_argument("--node-fd-dirs", type=int, default=64,
                     help="random node displacements of the node adjoint's central-difference check (each rebuilds "
                          "the 500k-cell mesh and solver twice)")
@@ -1100,7 +1102,8 @@ def main():
     if args.only_implicit:
         from aeroflex_amd import rans
         imp = implicit_bench(rans)
-        imp["sa_config1"] = sa_config1(rans)
+        if args.sa_config1:
+            imp["sa_config1"] = sa_config1(rans)
         print(json.dumps({"implicit": imp}), flush=True)
         return
     from aeroflex_amd import rans
@@ -1208,7 +1211,8 @@ def main():
         out["adjoint"]["spalart_allmaras"] = sa_adjoint_check(rans)
     if args.implicit:
         out["implicit"] = implicit_bench(rans)
-        out["implicit"]["sa_config1"] = sa_config1(rans)
+        if args.sa_config1:
+            out["implicit"]["sa_config1"] = sa_config1(rans)
     if args.sweep_steps > 0:
         out["batched_sweep"] = batched_sweep(rans, args.sweep_steps)
     if args.aeroelastic:

@@ -216,3 +216,4 @@ def test_sa_implicit_steps_match_oracle_exact_solves(kind):
         dn = np.max(np.abs(nu[:nr] - no[:nr])) / np.max(np.abs(no[:nr]))
         print(kind, "step", step, "rel diff q", dq, "nu", dn)
         assert dq <= 1e-9 and dn <= 1e-8, (step, dq, dn)
+
