@@ -4,7 +4,8 @@ matrix): the 5x5-block operator, the two preconditioners against their
 definitions built from the oracle's matrix, the device-resident GMRES against
 its stopping rule, and implicit steps against the oracle iterated with exact
 sparse solves (GMRES at 1e-12 on the device), on the reference's coarse
-triangle mesh and on the 129 x 65 O-grid at Re 1e6.
+triangle mesh and on the 129 x 65 O-grid at Re 1e6; and the converged SA
+state of that O-grid against the oracle's (tests/golden/gen_converged.py).
 
 Tolerances: the blocks are forward differences whose rho nu~ step is
 1e-6 mu_L, so the last-bit differences between the device's and the oracle's
@@ -217,3 +218,38 @@ def test_sa_implicit_steps_match_oracle_exact_solves(kind):
         print(kind, "step", step, "rel diff q", dq, "nu", dn)
         assert dq <= 1e-9 and dn <= 1e-8, (step, dq, dn)
 
+
+
+def test_config1_sa_converges_to_the_oracle_state():
+    """The reference's plain FP64 case with the SA model (129 x 65 O-grid,
+    M 0.5, alpha 2 x 0.01745, Re 1e6, no-slip wall, freestream start) through
+    the device loop (run_solver, multigrid.h:240-293) with the fixture's
+    schedule: CFL 5 + i up to 20, relaxation 0.5, nine frozen-matrix
+    right-hand-side iterations, GMRES(30) at the reference's 1e-2.  The
+    steady state, CL and CD match the oracle iterated with exact sparse
+    solves of its own SA matrix (tests/golden/gen_converged.py) to 1e-6.
+    (The reference loop's CFL 40-100 with relaxation 0.9 goes non-finite at
+    the trailing edge within three iterations, device and oracle alike.)"""
+    import os
+    from aeroflex_amd import rans
+    ref = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                               "converged_naca_ogrid_m05_a2_sa.npz"), allow_pickle=False)
+    mesh = rans.Mesh.naca_ogrid(128, 65, r_far=20.0, first_h=2e-3)
+    s = rans.Solver(mesh, gas=rans.Gas(mu_L=7e-7), viscosity_model="spalart-allmaras")
+    far = rans.BoundaryVariables(mach=0.5, angle=2 * 0.01745)
+    s.set_bcs({"farfield": rans.BoundaryCondition("farfield", far), "wall": rans.BoundaryCondition("wall")})
+    s.init()
+    s.refill_bcs()
+    st = rans.Settings(start_cfl=5.0, slope_cfl=1.0, max_cfl=20.0, relaxation=0.5, rhs_iterations=9,
+                       tolerance=1e-10, max_iterations=3000)
+    state, res = rans.run_solver(s, st)
+    print("iterations", len(res), "residual", res[-1])
+    assert state == 0 and res[-1] <= 1e-10
+    cl, cd, cm = s.get_wall_profile("wall")
+    assert abs(cl - float(ref["cl"])) <= 1e-6 and abs(cd - float(ref["cd"])) <= 1e-6, (cl, cd)
+    nr = s.n_real
+    q, nu = s.get_q(), s.get_nu()
+    assert np.max(np.abs(q[:4 * nr] - ref["q"][:4 * nr])) <= 1e-6 * np.max(np.abs(ref["q"][:4 * nr]))
+    assert np.max(np.abs(nu[:nr] - ref["nu"][:nr])) <= 1e-6 * np.max(np.abs(ref["nu"][:nr]))
+    s.close()
+    mesh.close()
