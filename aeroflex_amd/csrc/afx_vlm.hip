@@ -1262,6 +1262,18 @@ int afx_dense_solve(afx_dense* h, const double* b, double tol, int32_t max_iter,
     return AFX_OK;
 }
 
+// Test hook: raise the triangular solves' abort flag as a wait that timed
+// out raises it.  The host side of that case follows: the next solve fails
+// and discards the factorisation, and the next factor clears the flag
+// (k_panel_reset).  No kernel runs here.
+int afx_dense_raise_abort(afx_dense* h) {
+    if (!h) return api_fail(AFX_EARG, "null argument");
+    const unsigned one = 1u;
+    VCHK(hipMemcpyAsync(h->abort, &one, sizeof(unsigned), hipMemcpyHostToDevice, h->s));
+    VCHK(hipStreamSynchronize(h->s));
+    return AFX_OK;
+}
+
 int afx_dense_get_lu(afx_dense* h, float* lu, int32_t* ipiv) {
     if (!h) return api_fail(AFX_EARG, "null argument");
     const int n = h->n;

@@ -94,6 +94,7 @@ def _load():
         "afx_dense_factor": (C.c_int, [_P, ip]),
         "afx_dense_solve": (C.c_int, [_P, _dp, _f64, _i32, _dp, C.POINTER(SolveInfo)]),
         "afx_dense_get_lu": (C.c_int, [_P, C.POINTER(C.c_float), ip]),
+        "afx_dense_raise_abort": (C.c_int, [_P]),
         "afx_beam_create": (C.c_int, [_i32, _dp, _i32, ip, _dp, _i32, ip, C.POINTER(_P)]),
         "afx_beam_destroy": (C.c_int, [_P]),
         "afx_beam_solve": (C.c_int, [_P, _dp, _dp]),
@@ -239,6 +240,11 @@ class Dense(Handle):
         info = SolveInfo()
         _chk(_lib.afx_dense_solve(self.h, _d(b), tol, max_iter, _d(x), C.byref(info)))
         return x, info
+
+    def raise_abort(self):
+        """Test hook: raise the triangular solves' abort flag, as a timed-out
+        wait does."""
+        _chk(_lib.afx_dense_raise_abort(self.h))
 
     def lu(self):
         lu = np.zeros((self.n, self.n), np.float32, order="F")

@@ -165,6 +165,11 @@ int afx_dense_solve(afx_dense* h, const double* b, double tol, int32_t max_iter,
 /* factor in LAPACK getrf form: column-major n x n (unit L below, U on and
  * above the diagonal) and 0-based pivots ipiv[k] = row swapped with k */
 int afx_dense_get_lu(afx_dense* h, float* lu_colmajor, int32_t* ipiv);
+/* Test hook (no counterpart in the reference): raises the triangular solves'
+ * abort flag, as one of their bounded waits does when it times out.  The next
+ * afx_dense_solve then fails (AFX_EHIP) and discards the factorisation; the
+ * next afx_dense_factor clears the flag. */
+int afx_dense_raise_abort(afx_dense* h);
 
 /* --------------------------------------------------------------- beam --- */
 

@@ -104,6 +104,34 @@ def test_singular_matrix_is_reported(vlm):
         d.solve(np.ones(40))
 
 
+def test_solve_after_a_timed_out_wait(vlm):
+    """The host side of a triangular solve whose bounded wait timed out,
+    forced from the host by raising the kernels' abort flag as the wait would:
+    the solve fails with a clear error and discards the factorisation, a
+    second solve is refused until the next factor, which clears the flag, and
+    the solve is then right again."""
+    n = 300
+    rng = np.random.default_rng(11)
+    A = rng.normal(size=(n, n)) + n * np.eye(n)
+    b = rng.normal(size=n)
+    xr = np.linalg.solve(A, b)
+    d = vlm.Dense(n)
+    d.set(A)
+    assert d.factor() == 0
+    x, info = d.solve(b)
+    assert info.converged
+    d.raise_abort()
+    with pytest.raises(vlm.VlmError, match="timed out"):
+        d.solve(b)
+    with pytest.raises(vlm.VlmError, match="no factorisation"):
+        d.solve(b)
+    assert d.factor() == 0
+    x2, info2 = d.solve(b)
+    assert info2.converged
+    np.testing.assert_allclose(x2, xr, rtol=0, atol=1e-12 * np.abs(xr).max())
+    np.testing.assert_allclose(x2, x, rtol=0, atol=1e-13 * np.abs(xr).max())
+
+
 @pytest.mark.parametrize("alpha", [2.0, 8.0])
 def test_gamma_forces_cl_match_oracle(vlm, ov, alpha):
     print(f"[vlm] solve alpha={alpha}", flush=True)
