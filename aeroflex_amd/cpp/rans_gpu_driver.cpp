@@ -27,6 +27,7 @@
 #include <cstring>
 #include <exception>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -137,7 +138,8 @@ int main(int argc, char** argv) {
     }
     if (argc < 5) {
         std::fprintf(stderr, "usage: %s <mesh.msh|ogrid:NI:NJ> <mach> <alpha_deg> <iterations> "
-                             "[implicit|explicit] [inviscid|laminar] [slip-wall|wall] [tolerance]\n", argv[0]);
+                             "[implicit|explicit] [inviscid|laminar|spalart-allmaras] [slip-wall|wall] [tolerance] "
+                             "[mu_L] [start_cfl:slope_cfl:max_cfl] [relaxation] [rhs_iterations]\n", argv[0]);
         return 2;
     }
     try {
@@ -151,10 +153,15 @@ int main(int argc, char** argv) {
         st.tolerance = argc > 8 ? std::atof(argv[8]) : 1e-4;
         const std::vector<std::string> srcs = split_list(src);
         rans::gpu::gas g;
+        if (argc > 9) g.mu_L = std::atof(argv[9]);
         if (kind == "explicit") {
             st.start_cfl = st.max_cfl = 0.8;
             st.slope_cfl = 0.0;
         }
+        if (argc > 10 && std::sscanf(argv[10], "%lf:%lf:%lf", &st.start_cfl, &st.slope_cfl, &st.max_cfl) != 3)
+            throw std::runtime_error("CFL schedule must read start:slope:max");
+        if (argc > 11) st.relaxation = std::atof(argv[11]);
+        if (argc > 12) st.rhs_iterations = std::atoi(argv[12]);
         std::map<std::string, rans::gpu::boundary_condition> bcs;
         bcs["farfield"].bc_type = "farfield";
         bcs["farfield"].vars_far = rans::gpu::boundary_variables(mach, alpha_deg * 0.01745, 1.0, 1.0);   // rans.h:94

@@ -647,7 +647,7 @@ void afx_default_gas(afx_gas* g) {
 void afx_default_opts(afx_opts* o) {
     std::memset(o, 0, sizeof(*o));
     o->viscosity = 0; o->second_order = 1; o->gradient = 1; o->limiter_k = 5.0; o->renumber = 1;
-    o->tile_cells = 256; o->fused = 1;
+    o->tile_cells = 0; o->fused = 1;
 }
 
 // ------------------------------------------------------------------ mesh ---
