@@ -1515,9 +1515,11 @@ k_residual_tile_f32(TileArgsF32 A) {
     else tile_residual_f32<MODEL, false, GG, SO, false>(A, t, smem4);
 }
 
-// batched explicit stage: residual + stage update of every (member, tile)
+// batched explicit stage: residual + stage update of every (member, tile).
+// Five waves per SIMD: for the SA form that costs a few spilled VGPRs, and
+// runs faster than four waves without them.
 template <int MODEL, bool GG, bool SO>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MODEL == 2 ? 4 : 5)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
 k_residual_tile_f32_batch(const TileArgsF32* __restrict__ members, const int2* __restrict__ items) {
     extern __shared__ float4 smem4[];
     const int2 it = items[xcd_tile(blockIdx.x, gridDim.x)];
