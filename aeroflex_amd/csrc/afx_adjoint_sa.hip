@@ -53,6 +53,125 @@ __device__ __forceinline__ void bnd_face_value(const GasT<T>& g, int kind, T nx,
     qv[4] = sa_bnd_nu<T>(kind, nx, ny, qo, qo[4], qn, qn[4], qv);
 }
 
+// Reverse sweep of sa_face_diffusion (FP64): given the adjoints bF[1..4] of
+// the four flux rows it changes (the rows it subtracts from), adds the
+// adjoints of its inputs to bL, bR (5 each) and writes those of the face
+// gradients to bgx, bgy (5 each).  Every output depends on q_L and q_R only
+// through the face averages qc = (q_L + q_R) / 2, so both sides receive half
+// of qc's adjoint.  The derivatives are those of the mathematical functions
+// (the forward's algebraic form for the pressure gradient differs in
+// rounding only).
+__device__ __forceinline__ void sa_face_diffusion_vjp(const GasT<double>& g, double nx, double ny, const double* qL,
+                                                      const double* qR, const double* gx, const double* gy,
+                                                      const double* bF, double* bL, double* bR, double* bgx,
+                                                      double* bgy) {
+    double qc[5];
+    for (int k = 0; k < 5; ++k) qc[k] = 0.5 * (qL[k] + qR[k]);
+    const double gm1 = g.gamma - 1.0, iR = 1.0 / g.R;
+    // forward values
+    const double nc = qc[4], chi = nc / g.mu, c3 = chi * chi * chi;
+    const double den_v = c3 + sa::cv1_3;
+    const double fv1 = chi <= 0.0 ? 0.0 : c3 / den_v;
+    const double dfv1 = chi <= 0.0 ? 0.0 : 3.0 * chi * chi * sa::cv1_3 / (den_v * den_v);
+    const double mut = nc * fv1, mue = g.mu + mut;
+    const double kc = g.cp * (g.mu / g.PrL + mut / g.PrT);
+    const double r = qc[0], ir = 1.0 / r, r2 = r * r;
+    const double u = qc[1] * ir, v = qc[2] * ir;
+    const double ke2 = qc[1] * qc[1] + qc[2] * qc[2];
+    const double p = gm1 * (qc[3] - 0.5 * ir * ke2);
+    const double X0 = 2.0 * gx[3] - 2.0 * gx[1] * u - 2.0 * gx[2] * v + gx[0] * ke2 / r2;
+    const double X1 = 2.0 * gy[3] - 2.0 * gy[1] * u - 2.0 * gy[2] * v + gy[0] * ke2 / r2;
+    const double gp0 = 0.5 * gm1 * X0, gp1 = 0.5 * gm1 * X1;
+    const double gT0 = iR * (gp0 * r - gx[0] * p) / r2, gT1 = iR * (gp1 * r - gy[0] * p) / r2;
+    const double gu0 = (r * gx[1] - qc[1] * gx[0]) / r2, gu1 = (r * gy[1] - qc[1] * gy[0]) / r2;
+    const double gv0 = (r * gx[2] - qc[2] * gx[0]) / r2, gv1 = (r * gy[2] - qc[2] * gy[0]) / r2;
+    const double div3 = (gu0 + gv1) / 3.0;
+    const double txx = 2.0 * mue * (gu0 - div3), tyy = 2.0 * mue * (gv1 - div3), txy = mue * (gu1 + gv0);
+    const double nut = nc * ir;
+    const double gvx = (gx[4] - nut * gx[0]) * ir, gvy = (gy[4] - nut * gy[0]) * ir;
+    const double den_n = sa::cn1 - c3;
+    const double fn = chi >= 0.0 ? 1.0 : (sa::cn1 + c3) / den_n;
+    const double dfn = chi >= 0.0 ? 0.0 : 6.0 * sa::cn1 * chi * chi / (den_n * den_n);
+    const double dcoef = (g.mu + nc * fn) / sa::sigma;
+    const double S = nx * gvx + ny * gvy;
+    // reverse: the rows subtract their terms, so the terms' adjoints are -bF
+    const double B1 = -bF[1], B2 = -bF[2], B3 = -bF[3], B4 = -bF[4];
+    const double bphi0 = B3 * nx, bphi1 = B3 * ny;
+    const double btxx = B1 * nx + bphi0 * u;
+    const double btxy = B1 * ny + B2 * nx + bphi0 * v + bphi1 * u;
+    const double btyy = B2 * ny + bphi1 * v;
+    const double bkc = bphi0 * gT0 + bphi1 * gT1;
+    const double bgT0 = bphi0 * kc, bgT1 = bphi1 * kc;
+    double bu = bphi0 * txx + bphi1 * txy;
+    double bv = bphi0 * txy + bphi1 * tyy;
+    const double bmue = btxx * 2.0 * (gu0 - div3) + btyy * 2.0 * (gv1 - div3) + btxy * (gu1 + gv0);
+    const double bgu0 = btxx * mue * (4.0 / 3.0) - btyy * mue * (2.0 / 3.0);
+    const double bgv1 = btyy * mue * (4.0 / 3.0) - btxx * mue * (2.0 / 3.0);
+    const double bgu1 = btxy * mue, bgv0 = btxy * mue;
+    double bq[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < 5; ++k) { bgx[k] = 0.0; bgy[k] = 0.0; }
+    // velocity gradients g = (r gk - qk g0) / r^2
+    bq[0] += bgu0 * (gx[1] / r2 - 2.0 * gu0 * ir) + bgu1 * (gy[1] / r2 - 2.0 * gu1 * ir) +
+             bgv0 * (gx[2] / r2 - 2.0 * gv0 * ir) + bgv1 * (gy[2] / r2 - 2.0 * gv1 * ir);
+    bq[1] -= (bgu0 * gx[0] + bgu1 * gy[0]) / r2;
+    bq[2] -= (bgv0 * gx[0] + bgv1 * gy[0]) / r2;
+    bgx[1] += bgu0 * ir;
+    bgy[1] += bgu1 * ir;
+    bgx[2] += bgv0 * ir;
+    bgy[2] += bgv1 * ir;
+    bgx[0] -= (bgu0 * qc[1] + bgv0 * qc[2]) / r2;
+    bgy[0] -= (bgu1 * qc[1] + bgv1 * qc[2]) / r2;
+    // temperature gradients gT = iR (gp r - g0 p) / r^2
+    const double bgp0 = bgT0 * iR * ir, bgp1 = bgT1 * iR * ir;
+    bq[0] += bgT0 * (iR * gp0 / r2 - 2.0 * gT0 * ir) + bgT1 * (iR * gp1 / r2 - 2.0 * gT1 * ir);
+    bgx[0] -= bgT0 * iR * p / r2;
+    bgy[0] -= bgT1 * iR * p / r2;
+    const double bp = -(bgT0 * gx[0] + bgT1 * gy[0]) * iR / r2;
+    // pressure gradients gp = gm1/2 (2 g3 - 2 u g1 - 2 v g2 + |qc12|^2/r^2 g0)
+    const double bX0 = 0.5 * gm1 * bgp0, bX1 = 0.5 * gm1 * bgp1;
+    bgx[3] += 2.0 * bX0;
+    bgy[3] += 2.0 * bX1;
+    bgx[1] -= 2.0 * bX0 * u;
+    bgy[1] -= 2.0 * bX1 * u;
+    bgx[2] -= 2.0 * bX0 * v;
+    bgy[2] -= 2.0 * bX1 * v;
+    bgx[0] += bX0 * ke2 / r2;
+    bgy[0] += bX1 * ke2 / r2;
+    bu -= 2.0 * (bX0 * gx[1] + bX1 * gy[1]);
+    bv -= 2.0 * (bX0 * gx[2] + bX1 * gy[2]);
+    const double bke2 = (bX0 * gx[0] + bX1 * gy[0]) / r2;
+    bq[0] -= 2.0 * (bX0 * gx[0] + bX1 * gy[0]) * ke2 / (r2 * r);
+    // p = gm1 (qc3 - ke2 / (2 r))
+    bq[3] += bp * gm1;
+    const double bke2p = bke2 - bp * gm1 * 0.5 * ir;
+    bq[0] += bp * gm1 * 0.5 * ke2 / r2;
+    bq[1] += 2.0 * bke2p * qc[1];
+    bq[2] += 2.0 * bke2p * qc[2];
+    // u = qc1 / r, v = qc2 / r
+    bq[1] += bu * ir;
+    bq[2] += bv * ir;
+    bq[0] -= (bu * u + bv * v) * ir;
+    // mu_t through mu_e and kc, then nc = qc4
+    const double bmut = bmue + bkc * g.cp / g.PrT;
+    bq[4] += bmut * (fv1 + nc * dfv1 / g.mu);
+    // nu~ diffusion: B4 dcoef (n . gv)
+    const double bdc = B4 * S, bS = B4 * dcoef;
+    bq[4] += bdc * (fn + nc * dfn / g.mu) / sa::sigma;
+    const double bgvx = bS * nx, bgvy = bS * ny;
+    bgx[4] += bgvx * ir;
+    bgy[4] += bgvy * ir;
+    const double bnut = -(bgvx * gx[0] + bgvy * gy[0]) * ir;
+    bgx[0] -= bgvx * nut * ir;
+    bgy[0] -= bgvy * nut * ir;
+    bq[0] -= (bgvx * gvx + bgvy * gvy) * ir;
+    bq[4] += bnut * ir;
+    bq[0] -= bnut * nut * ir;
+    for (int k = 0; k < 5; ++k) {
+        bL[k] += 0.5 * bq[k];
+        bR[k] += 0.5 * bq[k];
+    }
+}
+
 // pass 1: gradients, limiters and the active branch of every (cell, variable)
 __global__ void __launch_bounds__(kBlock) k_sadj_forward(CellsT<double> C, FacesT<double> F,
                                                          const double* __restrict__ q, GasT<double> g,
@@ -193,26 +312,9 @@ __global__ void __launch_bounds__(kBlock) k_sadj_face(CellsT<double> C, FacesT<d
         const double* qu = up ? qL : qR;
         bu[4] += bF[4] * fr[0] / qu[0];
         bu[0] -= bF[4] * fr[0] * qu[4] / (qu[0] * qu[0]);
-        // the diffusive part (sa_face_diffusion): one forward sweep per input,
-        // qL (5), qR (5), face gradients x (5), y (5)
+        // the diffusive part (sa_face_diffusion): its hand-written reverse
         double bgx[NS], bgy[NS];
-        for (int j = 0; j < 4 * NS; ++j) {
-            Dual1 a[NS], b[NS], gx[NS], gy[NS], fo[NS];
-            for (int k = 0; k < NS; ++k) {
-                a[k] = Dual1(qL[k], j == k ? 1.0 : 0.0);
-                b[k] = Dual1(qR[k], j == NS + k ? 1.0 : 0.0);
-                gx[k] = Dual1(gxa[k], j == 2 * NS + k ? 1.0 : 0.0);
-                gy[k] = Dual1(gya[k], j == 3 * NS + k ? 1.0 : 0.0);
-                fo[k] = Dual1(0.0);
-            }
-            sa_face_diffusion<Dual1>(gd, Dual1(nx), Dual1(ny), a, b, gx, gy, fo);
-            double v = 0.0;
-            for (int k = 1; k < NS; ++k) v += bF[k] * fo[k].d;
-            if (j < NS) bL[j] += v;
-            else if (j < 2 * NS) bR[j - NS] += v;
-            else if (j < 3 * NS) bgx[j - 2 * NS] = v;
-            else bgy[j - 3 * NS] = v;
-        }
+        sa_face_diffusion_vjp(g, nx, ny, qL, qR, gxa, gya, bF, bL, bR, bgx, bgy);
         for (int k = 0; k < NS; ++k) {
             const double bt = bgx[k] * t0 + bgy[k] * t1;   // the adjoint of dq / l
             vgx[k] = 0.5 * (bgx[k] - t0 * bt);
