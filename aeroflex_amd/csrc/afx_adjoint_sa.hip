@@ -172,6 +172,105 @@ __device__ __forceinline__ void sa_face_diffusion_vjp(const GasT<double>& g, dou
     }
 }
 
+// Reverse sweep of sa_cell_source (FP64): adds ws times the derivatives of
+// the per-volume source S(q, grad q, d) to lq (the state, 5), lgx (x
+// gradients of rho, rho v, rho nu~) and lgy (y gradients of rho, rho u,
+// rho nu~), the only gradients S reads.  The branches are the forward's
+// (negative nu~, the S-tilde limiter, the cap r <= 10, |omega| with slope +1
+// at zero); f_w'(r) = g'(r) c / (g^6 + c) ((1 + c) / (g^6 + c))^(1/6),
+// c = c_w3^6.
+__device__ __forceinline__ void sa_cell_source_vjp(const GasT<double>& g, const double* q, const double* gx,
+                                                   const double* gy, double d, double ws, double* lq, double* lgx,
+                                                   double* lgy) {
+    const double rho = q[0], ir = 1.0 / rho, nut = q[4] * ir;
+    const double dudy = (gy[1] - q[1] * ir * gy[0]) * ir;
+    const double dvdx = (gx[2] - q[2] * ir * gx[0]) * ir;
+    const double om_raw = dvdx - dudy;
+    const double omega = om_raw < 0.0 ? -om_raw : om_raw, som = om_raw < 0.0 ? -1.0 : 1.0;
+    const double gvx = (gx[4] - nut * gx[0]) * ir, gvy = (gy[4] - nut * gy[0]) * ir;
+    const double cc = sa::cb2 / sa::sigma;
+    const double cross = cc * (gvx * gvx + gvy * gvy);
+    const double nd = nut / d;
+    double b_rho = 0.0, b_nut = 0.0, b_om = 0.0;
+    const double b_gvx = ws * rho * cc * 2.0 * gvx, b_gvy = ws * rho * cc * 2.0 * gvy;
+    if (nut < 0.0) {
+        const double a1 = sa::cb1 * (1.0 - sa::ct3);
+        const double E = a1 * omega * nut + sa::cw1 * nd * nd + cross;
+        b_rho = ws * E;
+        b_nut = ws * rho * (a1 * omega + 2.0 * sa::cw1 * nd / d);
+        b_om = ws * rho * a1 * nut;
+    } else {
+        const double chi = rho * nut / g.mu, c3 = chi * chi * chi;
+        const double dv = c3 + sa::cv1_3;
+        const double f1 = chi <= 0.0 ? 0.0 : c3 / dv;
+        const double df1 = chi <= 0.0 ? 0.0 : 3.0 * chi * chi * sa::cv1_3 / (dv * dv);
+        const double den2 = 1.0 + chi * f1;
+        const double f2 = 1.0 - chi / den2;
+        const double df2 = -(1.0 - chi * chi * df1) / (den2 * den2);
+        const double kd2 = sa::kappa * sa::kappa * d * d;
+        const double sbar = nut * f2 / kd2;
+        double st, dst_dom, dst_dsb;
+        if (sbar >= -sa::cv2 * omega) {
+            st = omega + sbar;
+            dst_dom = 1.0;
+            dst_dsb = 1.0;
+        } else {
+            const double N = omega * (sa::cv2 * sa::cv2 * omega + sa::cv3 * sbar);
+            const double Dn = (sa::cv3 - 2.0 * sa::cv2) * omega - sbar;
+            st = omega + N / Dn;
+            const double dN_dom = 2.0 * sa::cv2 * sa::cv2 * omega + sa::cv3 * sbar, dN_dsb = sa::cv3 * omega;
+            dst_dom = 1.0 + (dN_dom * Dn - N * (sa::cv3 - 2.0 * sa::cv2)) / (Dn * Dn);
+            dst_dsb = (dN_dsb * Dn + N) / (Dn * Dn);
+        }
+        double r = 10.0, dr_dnut = 0.0, dr_dst = 0.0;
+        if (st > 0.0) {
+            const double r0 = nut / (st * kd2);
+            if (r0 < 10.0) {
+                r = r0;
+                dr_dnut = 1.0 / (st * kd2);
+                dr_dst = -r0 / st;
+            }
+        }
+        const double r2 = r * r, gg = r + sa::cw2 * (r2 * r2 * r2 - r);
+        const double g2 = gg * gg, g6 = g2 * g2 * g2;
+        const double h6 = cbrt(sqrt((1.0 + sa::cw3_6) / (g6 + sa::cw3_6)));
+        const double fw = gg * h6;
+        const double dfw = (1.0 + sa::cw2 * (6.0 * r2 * r2 * r - 1.0)) * h6 * sa::cw3_6 / (g6 + sa::cw3_6);
+        const double P = sa::cb1 * st * nut, D = sa::cw1 * fw * nd * nd;
+        const double E = P - D + cross;
+        const double bE = ws * rho;
+        b_rho = ws * E;
+        const double b_r = -bE * sa::cw1 * dfw * nd * nd;
+        const double b_st = bE * sa::cb1 * nut + b_r * dr_dst;
+        b_nut = bE * (sa::cb1 * st - 2.0 * sa::cw1 * fw * nd / d) + b_r * dr_dnut;
+        const double b_sb = b_st * dst_dsb;
+        b_om = b_st * dst_dom;
+        b_nut += b_sb * f2 / kd2;
+        const double b_chi = b_sb * nut / kd2 * df2;
+        b_rho += b_chi * nut / g.mu;
+        b_nut += b_chi * rho / g.mu;
+    }
+    // omega = |dvdx - dudy|, the velocity derivatives and grad nu~ = (grad
+    // rho nu~ - nu~ grad rho) / rho, then nu~ = (rho nu~) / rho
+    const double b_dv = b_om * som, b_du = -b_om * som;
+    lgx[2] += b_dv * ir;
+    lq[2] -= b_dv * gx[0] * ir * ir;
+    lgx[0] -= b_dv * q[2] * ir * ir;
+    b_rho += b_dv * (-gx[2] * ir * ir + 2.0 * q[2] * gx[0] * ir * ir * ir);
+    lgy[1] += b_du * ir;
+    lq[1] -= b_du * gy[0] * ir * ir;
+    lgy[0] -= b_du * q[1] * ir * ir;
+    b_rho += b_du * (-gy[1] * ir * ir + 2.0 * q[1] * gy[0] * ir * ir * ir);
+    lgx[4] += b_gvx * ir;
+    lgy[4] += b_gvy * ir;
+    lgx[0] -= b_gvx * nut * ir;
+    lgy[0] -= b_gvy * nut * ir;
+    b_nut -= (b_gvx * gx[0] + b_gvy * gy[0]) * ir;
+    b_rho -= (b_gvx * gvx + b_gvy * gvy) * ir;
+    lq[4] += b_nut * ir;
+    lq[0] += b_rho - b_nut * nut * ir;
+}
+
 // pass 1: gradients, limiters and the active branch of every (cell, variable)
 __global__ void __launch_bounds__(kBlock) k_sadj_forward(CellsT<double> C, FacesT<double> F,
                                                          const double* __restrict__ q, GasT<double> g,
@@ -392,29 +491,11 @@ __global__ void __launch_bounds__(kBlock) k_sadj_cell1(CellsT<double> C, FacesT<
             lq[k] += nb ? -A.vdq[k * nF + f] : A.vdq[k * nF + f];
         }
     }
-    // the source term A S(q_c, grad q_c, d_c) of the rho nu~ row: one forward
-    // sweep per input it reads, the state (5) and the gradients of rho, rho v
-    // and rho nu~ in x and of rho, rho u and rho nu~ in y (vorticity and
-    // grad nu~; sa_cell_source reads no other gradient)
+    // the source term A S(q_c, grad q_c, d_c) of the rho nu~ row: its
+    // hand-written reverse (the state and the gradients of rho, rho v and
+    // rho nu~ in x and of rho, rho u and rho nu~ in y)
     const double ws = w[4 * nT + c] * C.area[c];
-    if (ws != 0.0) {
-        const GasT<Dual1> gd = sa_dual_gas(g);
-        constexpr int kx[3] = {0, 2, 4}, ky[3] = {0, 1, 4};
-        for (int j = 0; j < NS + 6; ++j) {
-            Dual1 a[NS], bx[NS], by[NS];
-            for (int k = 0; k < NS; ++k) {
-                a[k] = Dual1(qc[k], j == k ? 1.0 : 0.0);
-                bx[k] = Dual1(gx[k]);
-                by[k] = Dual1(gy[k]);
-            }
-            if (j >= NS && j < NS + 3) bx[kx[j - NS]].d = 1.0;
-            if (j >= NS + 3) by[ky[j - NS - 3]].d = 1.0;
-            const double v = ws * sa_cell_source<Dual1>(gd, a, bx, by, Dual1(wd[c])).d;
-            if (j < NS) lq[j] += v;
-            else if (j < NS + 3) lgx[kx[j - NS]] += v;
-            else lgy[ky[j - NS - 3]] += v;
-        }
-    }
+    if (ws != 0.0) sa_cell_source_vjp(g, qc, gx, gy, wd[c], ws, lq, lgx, lgy);
     for (int k = 0; k < NS; ++k) {
         double nbv = 0.0;
         int nbj = -1;
