@@ -345,104 +345,67 @@ __global__ void __launch_bounds__(kBlock) k_sadj_forward(CellsT<double> C, Faces
 // pass 2: per face, adjoints of the reconstructed states (bL, bR), of the
 // face gradients (vgx, vgy: half of them, owed to both cells' gradients) and
 // of the direct cell-state dependences (vdq: the face gradient's difference
-// quotient, owner sign, and the wall diffusion's cell-centre nu~).  PART 1:
-// interior faces; PART 2: the boundary faces of A.bface.
-template <int PART>
+// quotient, owner sign, and the wall diffusion's cell-centre nu~).  This
+// kernel takes the interior faces; k_sadj_face_bnd the boundary faces.
 __global__ void __launch_bounds__(kBlock) k_sadj_face(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                       const double* __restrict__ w, GasT<double> g, int second_order,
                                                       AdjBufs A) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int f;
-    if constexpr (PART == 1) {
-        f = i;
-        if (f >= F.nF || F.kind[f] != K_INTERNAL) return;
-    } else {
-        if (i >= A.nbf) return;
-        f = A.bface[i];
-    }
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F.nF || F.kind[f] != K_INTERNAL) return;
     const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
     const int o = F.c0[f], n = F.c1[f];
-    const int kind = PART == 1 ? (int)K_INTERNAL : (int)F.kind[f];
     const double nx = F.nx[f], ny = F.ny[f], L = F.len[f];
-    const double d0x = F.xf[f] - C.xc[o], d0y = F.yf[f] - C.yc[o];
     double qL[NS], qR[NS];
     for (int k = 0; k < NS; ++k) { qL[k] = q[k * nT + o]; qR[k] = q[k * nT + n]; }
     if (second_order) {
-        for (int k = 0; k < NS; ++k)
+        const double d0x = F.xf[f] - C.xc[o], d0y = F.yf[f] - C.yc[o];
+        const double d1x = F.xf[f] - C.xc[n], d1y = F.yf[f] - C.yc[n];
+        for (int k = 0; k < NS; ++k) {
             qL[k] += (A.gx[k * nT + o] * d0x + A.gy[k * nT + o] * d0y) * A.lim[k * nT + o];
-        if (kind == K_INTERNAL) {
-            const double d1x = F.xf[f] - C.xc[n], d1y = F.yf[f] - C.yc[n];
-            for (int k = 0; k < NS; ++k)
-                qR[k] += (A.gx[k * nT + n] * d1x + A.gy[k * nT + n] * d1y) * A.lim[k * nT + n];
+            qR[k] += (A.gx[k * nT + n] * d1x + A.gy[k * nT + n] * d1y) * A.lim[k * nT + n];
         }
     }
     double bF[NS], bL[NS], bR[NS], vgx[NS], vgy[NS], vdq[NS];
     for (int k = 0; k < NS; ++k) {
-        bF[k] = kind == K_INTERNAL ? L * (w[k * nT + n] - w[k * nT + o]) : -L * w[k * nT + o];
-        bL[k] = bR[k] = vgx[k] = vgy[k] = vdq[k] = 0.0;
+        bF[k] = L * (w[k * nT + n] - w[k * nT + o]);
+        bL[k] = bR[k] = 0.0;
     }
-    const GasT<Dual1> gd = sa_dual_gas(g);
-    if (kind == K_INTERNAL) {
-        // face gradients of the SA model: textbook direction t = (x_n - x_o) /
-        // l and derivative (q_n - q_o) / l, g = gbar - (gbar . t - dq / l) t
-        double t0 = C.xc[n] - C.xc[o], t1 = C.yc[n] - C.yc[o];
-        const double l = sqrt(t0 * t0 + t1 * t1);
-        t0 /= l;
-        t1 /= l;
-        double gxa[NS], gya[NS];
-        for (int k = 0; k < NS; ++k) {
-            const double dir = (q[k * nT + n] - q[k * nT + o]) / l;
-            const double gxb = (A.gx[k * nT + o] + A.gx[k * nT + n]) * 0.5;
-            const double gyb = (A.gy[k * nT + o] + A.gy[k * nT + n]) * 0.5;
-            const double dot = gxb * t0 + gyb * t1;
-            gxa[k] = gxb - (dot - dir) * t0;
-            gya[k] = gyb - (dot - dir) * t1;
-        }
-        // Roe and the rho nu~ convection f4 = f0 s (s = nu~ of the upwind
-        // side): the hand-written reverse sweep of the Roe flux with the
-        // mass-flux adjoint bF0 + bF4 s, then s's own dependence
-        double fr[4];
-        roe_flux<double>(g, nx, ny, qL, qR, false, nullptr, nullptr, fr);
-        const bool up = fr[0] >= 0.0;
-        const double s = up ? qL[4] / qL[0] : qR[4] / qR[0];
-        const double bRoe[4] = {bF[0] + bF[4] * s, bF[1], bF[2], bF[3]};
-        roe_flux_vjp<double>(g.gamma, nx, ny, qL, qR, bRoe, bL, bR);
-        double* bu = up ? bL : bR;
-        const double* qu = up ? qL : qR;
-        bu[4] += bF[4] * fr[0] / qu[0];
-        bu[0] -= bF[4] * fr[0] * qu[4] / (qu[0] * qu[0]);
-        // the diffusive part (sa_face_diffusion): its hand-written reverse
-        double bgx[NS], bgy[NS];
-        sa_face_diffusion_vjp(g, nx, ny, qL, qR, gxa, gya, bF, bL, bR, bgx, bgy);
-        for (int k = 0; k < NS; ++k) {
-            const double bt = bgx[k] * t0 + bgy[k] * t1;   // the adjoint of dq / l
-            vgx[k] = 0.5 * (bgx[k] - t0 * bt);
-            vgy[k] = 0.5 * (bgy[k] - t1 * bt);
-            vdq[k] = -bt / l;                              // owner's share; the neighbour takes -vdq
-        }
-    } else {
-        // boundary face: the flux of the reconstructed owner state and the
-        // ghost's boundary data; the wall diffusion reads the owner's
-        // cell-centre nu~ = (rho nu~) / rho and its normal distance d0 . n
-        const double qo0 = q[o], qo4 = q[4 * nT + o];
-        const double nut_c = qo4 / qo0, dn = d0x * nx + d0y * ny;
-        double bnut = 0.0;
-        for (int j = 0; j <= NS; ++j) {
-            Dual1 a[NS], b[NS], z[NS], fo[NS];
-            for (int k = 0; k < NS; ++k) {
-                a[k] = Dual1(qL[k], j == k ? 1.0 : 0.0);
-                b[k] = Dual1(qR[k]);
-                z[k] = Dual1(0.0);
-            }
-            sa_face_flux<Dual1>(gd, kind, Dual1(nx), Dual1(ny), a, b, z, z, Dual1(nut_c, j == NS ? 1.0 : 0.0),
-                                Dual1(dn), fo);
-            double v = 0.0;
-            for (int k = 0; k < NS; ++k) v += bF[k] * fo[k].d;
-            if (j < NS) bL[j] = v;
-            else bnut = v;
-        }
-        vdq[0] = -bnut * qo4 / (qo0 * qo0);
-        vdq[4] = bnut / qo0;
+    // face gradients of the SA model: textbook direction t = (x_n - x_o) /
+    // l and derivative (q_n - q_o) / l, g = gbar - (gbar . t - dq / l) t
+    double t0 = C.xc[n] - C.xc[o], t1 = C.yc[n] - C.yc[o];
+    const double l = sqrt(t0 * t0 + t1 * t1);
+    t0 /= l;
+    t1 /= l;
+    double gxa[NS], gya[NS];
+    for (int k = 0; k < NS; ++k) {
+        const double dir = (q[k * nT + n] - q[k * nT + o]) / l;
+        const double gxb = (A.gx[k * nT + o] + A.gx[k * nT + n]) * 0.5;
+        const double gyb = (A.gy[k * nT + o] + A.gy[k * nT + n]) * 0.5;
+        const double dot = gxb * t0 + gyb * t1;
+        gxa[k] = gxb - (dot - dir) * t0;
+        gya[k] = gyb - (dot - dir) * t1;
+    }
+    // Roe and the rho nu~ convection f4 = f0 s (s = nu~ of the upwind
+    // side): the hand-written reverse sweep of the Roe flux with the
+    // mass-flux adjoint bF0 + bF4 s, then s's own dependence
+    double fr[4];
+    roe_flux<double>(g, nx, ny, qL, qR, false, nullptr, nullptr, fr);
+    const bool up = fr[0] >= 0.0;
+    const double s = up ? qL[4] / qL[0] : qR[4] / qR[0];
+    const double bRoe[4] = {bF[0] + bF[4] * s, bF[1], bF[2], bF[3]};
+    roe_flux_vjp<double>(g.gamma, nx, ny, qL, qR, bRoe, bL, bR);
+    double* bu = up ? bL : bR;
+    const double* qu = up ? qL : qR;
+    bu[4] += bF[4] * fr[0] / qu[0];
+    bu[0] -= bF[4] * fr[0] * qu[4] / (qu[0] * qu[0]);
+    // the diffusive part (sa_face_diffusion): its hand-written reverse
+    double bgx[NS], bgy[NS];
+    sa_face_diffusion_vjp(g, nx, ny, qL, qR, gxa, gya, bF, bL, bR, bgx, bgy);
+    for (int k = 0; k < NS; ++k) {
+        const double bt = bgx[k] * t0 + bgy[k] * t1;   // the adjoint of dq / l
+        vgx[k] = 0.5 * (bgx[k] - t0 * bt);
+        vgy[k] = 0.5 * (bgy[k] - t1 * bt);
+        vdq[k] = -bt / l;                              // owner's share; the neighbour takes -vdq
     }
     for (int k = 0; k < NS; ++k) {
         A.bL[k * nF + f] = bL[k];
@@ -450,6 +413,52 @@ __global__ void __launch_bounds__(kBlock) k_sadj_face(CellsT<double> C, FacesT<d
         A.vgx[k * nF + f] = vgx[k];
         A.vgy[k * nF + f] = vgy[k];
         A.vdq[k * nF + f] = vdq[k];
+    }
+}
+
+// pass 2, boundary faces: the flux of the reconstructed owner state and the
+// ghost's boundary data, swept forward once per input (the owner state and
+// the owner's cell-centre nu~ = (rho nu~) / rho that the wall diffusion reads
+// with its normal distance d0 . n).  One thread per (face, input): the six
+// sweeps of a face run side by side instead of one after another, since the
+// few boundary faces leave the launch bound by one thread's FP64 chain.
+__global__ void __launch_bounds__(kBlock) k_sadj_face_bnd(CellsT<double> C, FacesT<double> F,
+                                                          const double* __restrict__ q, const double* __restrict__ w,
+                                                          GasT<double> g, int second_order, AdjBufs A) {
+    constexpr int NJ = NS + 1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.nbf * NJ) return;
+    const int j = i % NJ, f = A.bface[i / NJ];
+    const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
+    const int o = F.c0[f], n = F.c1[f], kind = F.kind[f];
+    const double nx = F.nx[f], ny = F.ny[f], L = F.len[f];
+    const double d0x = F.xf[f] - C.xc[o], d0y = F.yf[f] - C.yc[o];
+    double qL[NS], qR[NS];
+    for (int k = 0; k < NS; ++k) { qL[k] = q[k * nT + o]; qR[k] = q[k * nT + n]; }
+    if (second_order)
+        for (int k = 0; k < NS; ++k)
+            qL[k] += (A.gx[k * nT + o] * d0x + A.gy[k * nT + o] * d0y) * A.lim[k * nT + o];
+    const double qo0 = q[o], qo4 = q[4 * nT + o];
+    const double nut_c = qo4 / qo0, dn = d0x * nx + d0y * ny;
+    const GasT<Dual1> gd = sa_dual_gas(g);
+    Dual1 a[NS], b[NS], z[NS], fo[NS];
+    for (int k = 0; k < NS; ++k) {
+        a[k] = Dual1(qL[k], j == k ? 1.0 : 0.0);
+        b[k] = Dual1(qR[k]);
+        z[k] = Dual1(0.0);
+    }
+    sa_face_flux<Dual1>(gd, kind, Dual1(nx), Dual1(ny), a, b, z, z, Dual1(nut_c, j == NS ? 1.0 : 0.0), Dual1(dn), fo);
+    double v = 0.0;
+    for (int k = 0; k < NS; ++k) v += -L * w[k * nT + o] * fo[k].d;
+    if (j < NS) {
+        A.bL[j * nF + f] = v;
+        A.bR[j * nF + f] = 0.0;
+        A.vgx[j * nF + f] = 0.0;
+        A.vgy[j * nF + f] = 0.0;
+        if (j != 0 && j != 4) A.vdq[j * nF + f] = 0.0;
+    } else {   // the cell-centre nu~ part, owed to the owner's rho and rho nu~
+        A.vdq[f] = -v * qo4 / (qo0 * qo0);
+        A.vdq[4 * nF + f] = v / qo0;
     }
 }
 
@@ -524,46 +533,48 @@ __global__ void __launch_bounds__(kBlock) k_sadj_cell1(CellsT<double> C, FacesT<
     }
 }
 
-// pass 4: transpose of the Green-Gauss face values
-__global__ void __launch_bounds__(kBlock) k_sadj_face2(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
-                                                       GasT<double> g, AdjBufs A) {
+// pass 4: transpose of the Green-Gauss face values; interior faces here
+// (face value q_o (1 - gw) + (q_o + q_n) gw / 2, every variable), boundary
+// faces in k_sadj_face2_bnd
+__global__ void __launch_bounds__(kBlock) k_sadj_face2(CellsT<double> C, FacesT<double> F, AdjBufs A) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= F.nF) return;
+    if (f >= F.nF || F.kind[f] != K_INTERNAL) return;
+    const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
+    const int o = F.c0[f], n = F.c1[f];
+    const double nx = F.nx[f], ny = F.ny[f], L = F.len[f], gw = F.gw[f];
+    const double so = L / C.area[o], sn = L / C.area[n];
+    for (int k = 0; k < NS; ++k) {
+        const double v = (A.lgx[k * nT + o] * nx + A.lgy[k * nT + o] * ny) * so -
+                         (A.lgx[k * nT + n] * nx + A.lgy[k * nT + n] * ny) * sn;
+        A.co[k * nF + f] = v * (1.0 - 0.5 * gw);
+        A.cn[k * nF + f] = v * (0.5 * gw);
+    }
+}
+
+// pass 4, boundary faces: q_o (1 - gw) + face value (q_o; ghost) gw, the face
+// value swept forward once per owner-state input, one thread per (face,
+// input)
+__global__ void __launch_bounds__(kBlock) k_sadj_face2_bnd(CellsT<double> C, FacesT<double> F,
+                                                           const double* __restrict__ q, GasT<double> g, AdjBufs A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.nbf * NS) return;
+    const int j = i % NS, f = A.bface[i / NS];
     const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
     const int o = F.c0[f], n = F.c1[f], kind = F.kind[f];
     const double nx = F.nx[f], ny = F.ny[f], L = F.len[f], gw = F.gw[f];
-    const bool two = kind == K_INTERNAL;
-    double lfv[NS], co[NS], cn[NS];
+    const double so = L / C.area[o];
+    double lfv[NS];
+    Dual1 a[NS], qn[NS], v[NS];
     for (int k = 0; k < NS; ++k) {
-        double v = (A.lgx[k * nT + o] * nx + A.lgy[k * nT + o] * ny) * L / C.area[o];
-        if (two) v -= (A.lgx[k * nT + n] * nx + A.lgy[k * nT + n] * ny) * L / C.area[n];
-        lfv[k] = v;
+        lfv[k] = (A.lgx[k * nT + o] * nx + A.lgy[k * nT + o] * ny) * so;
+        a[k] = Dual1(q[k * nT + o], j == k ? 1.0 : 0.0);
+        qn[k] = Dual1(q[k * nT + n]);
     }
-    if (two) {   // face value q_o (1 - gw) + (q_o + q_n) gw / 2, every variable
-        for (int k = 0; k < NS; ++k) { co[k] = lfv[k] * (1.0 - 0.5 * gw); cn[k] = lfv[k] * (0.5 * gw); }
-    } else {     // q_o (1 - gw) + face value (q_o; ghost) gw
-        const GasT<Dual1> gd = sa_dual_gas(g);
-        Dual1 qn[NS];
-        double qo[NS];
-        for (int k = 0; k < NS; ++k) {
-            qo[k] = q[k * nT + o];
-            qn[k] = Dual1(q[k * nT + n]);
-            co[k] = lfv[k] * (1.0 - gw);
-            cn[k] = 0.0;
-        }
-        for (int j = 0; j < NS; ++j) {
-            Dual1 a[NS], v[NS];
-            for (int k = 0; k < NS; ++k) a[k] = Dual1(qo[k], j == k ? 1.0 : 0.0);
-            bnd_face_value<Dual1>(gd, kind, Dual1(nx), Dual1(ny), a, qn, v);
-            double s = 0.0;
-            for (int k = 0; k < NS; ++k) s += lfv[k] * gw * v[k].d;
-            co[j] += s;
-        }
-    }
-    for (int k = 0; k < NS; ++k) {
-        A.co[k * nF + f] = co[k];
-        A.cn[k * nF + f] = cn[k];
-    }
+    bnd_face_value<Dual1>(sa_dual_gas(g), kind, Dual1(nx), Dual1(ny), a, qn, v);
+    double s = 0.0;
+    for (int k = 0; k < NS; ++k) s += lfv[k] * gw * v[k].d;
+    A.co[j * nF + f] = lfv[j] * (1.0 - gw) + s;
+    A.cn[j * nF + f] = 0.0;
 }
 
 // pass 5: gather of the Green-Gauss transpose and the neighbours' limiter parts
@@ -606,14 +617,17 @@ hipError_t launch_adjoint_state_sa(hipStream_t s, CellsT<double> C, FacesT<doubl
     if (!A.vgx || !A.vgy || !A.vdq) return hipErrorInvalidValue;
     const int nbc = (C.nC + kBlock - 1) / kBlock, nbf = (F.nF + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_sadj_forward, dim3(nbc), dim3(kBlock), 0, s, C, F, q, g, second_order, limiter_k, A);
-    hipLaunchKernelGGL(k_sadj_face<1>, dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A);
+    hipLaunchKernelGGL(k_sadj_face, dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A);
     if (A.nbf > 0)
-        hipLaunchKernelGGL(k_sadj_face<2>, dim3((A.nbf + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F, q, w, g,
-                           second_order, A);
+        hipLaunchKernelGGL(k_sadj_face_bnd, dim3((A.nbf * (NS + 1) + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F,
+                           q, w, g, second_order, A);
     hipLaunchKernelGGL(k_sadj_cell1, dim3(nbc), dim3(kBlock), 0, s, C, F, q, w, wd, g, second_order, limiter_k, A,
                        lam);
     // the gradients always enter (face gradients of the viscous terms, the source)
-    hipLaunchKernelGGL(k_sadj_face2, dim3(nbf), dim3(kBlock), 0, s, C, F, q, g, A);
+    hipLaunchKernelGGL(k_sadj_face2, dim3(nbf), dim3(kBlock), 0, s, C, F, A);
+    if (A.nbf > 0)
+        hipLaunchKernelGGL(k_sadj_face2_bnd, dim3((A.nbf * NS + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F, q, g,
+                           A);
     hipLaunchKernelGGL(k_sadj_cell2, dim3(nbc), dim3(kBlock), 0, s, C, F, A, lam);
     return hipGetLastError();
 }
