@@ -136,7 +136,7 @@ __device__ void viscous_vjp(const GasT<double>& g, double nx, double ny, const d
 __global__ void __launch_bounds__(kBlock)
 k_adj_forward(CellsT<double> C, FacesT<double> F, const double* qg, const double* q, GasT<double> g, int second_order,
               double limiter_k, AdjBufs A) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_index();
     if (c >= C.nC) return;
     const size_t nT = (size_t)C.nT, nC = (size_t)C.nC;
     // gradients and limiters of one state (qg == q): the neighbour extrema
@@ -218,7 +218,7 @@ template <bool GEO, bool GHOST, int PART>
 __global__ void __launch_bounds__(kBlock) k_adj_face(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                      const double* __restrict__ w, GasT<double> g, int second_order,
                                                      AdjBufs A, GeoBufs G, double* __restrict__ ghost) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = PART == 1 ? xcd_index() : (int)(blockIdx.x * blockDim.x + threadIdx.x);
     int f;
     if constexpr (PART == 1) {
         f = i;
@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<d
                                                       int second_order, double limiter_k, AdjBufs A,
                                                       double* __restrict__ lam, GeoBufs G,
                                                       double* __restrict__ ghost) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_index();
     if (c >= C.nC) return;
     const size_t nT = (size_t)C.nT, nC = (size_t)C.nC, nF = (size_t)F.nF;
     double lq[4] = {0, 0, 0, 0}, lgx[4] = {0, 0, 0, 0}, lgy[4] = {0, 0, 0, 0}, lphi[4] = {0, 0, 0, 0};
@@ -413,7 +413,7 @@ template <bool GEO, bool GHOST>
 __global__ void __launch_bounds__(kBlock) k_adj_face2(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                       GasT<double> g, AdjBufs A, GeoBufs G,
                                                       double* __restrict__ ghost) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = xcd_index();
     if (f >= F.nF) return;
     const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
     const int o = F.c0[f], n = F.c1[f], kind = F.kind[f];
@@ -606,7 +606,7 @@ __global__ void __launch_bounds__(kBlock) k_geo_node(CellsT<double> C, FacesT<do
 // Green-Gauss part (the gradient's source state); with lam_g == lam both add
 __global__ void __launch_bounds__(kBlock) k_adj_cell2(CellsT<double> C, FacesT<double> F, AdjBufs A,
                                                       double* __restrict__ lam, double* lam_g) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_index();
     if (c >= C.nC) return;
     const size_t nT = (size_t)C.nT, nC = (size_t)C.nC, nF = (size_t)F.nF;
     double acc[4], accg[4] = {0.0, 0.0, 0.0, 0.0};
@@ -833,7 +833,8 @@ template <bool GEO, bool GHOST>
 void adjoint_passes(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* qg, const double* q,
                     const double* w, GasT<double> g, int second_order, double limiter_k, AdjBufs A, double* lam,
                     double* lam_g, double* ghost, const GeoBufs& G, bool to_nodes = true) {
-    const int nbc = (C.nC + kBlock - 1) / kBlock, nbf = (F.nF + kBlock - 1) / kBlock;
+    // the pass kernels index cells and faces in XCD-aware block order
+    const int nbc = xcd_grid(C.nC), nbf = xcd_grid(F.nF);
     hipLaunchKernelGGL(k_adj_forward, dim3(nbc), dim3(kBlock), 0, s, C, F, qg, q, g, second_order, limiter_k, A);
     hipLaunchKernelGGL((k_adj_face<GEO, GHOST, 1>), dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A, G,
                        ghost);

@@ -275,7 +275,7 @@ __device__ __forceinline__ void sa_cell_source_vjp(const GasT<double>& g, const 
 __global__ void __launch_bounds__(kBlock) k_sadj_forward(CellsT<double> C, FacesT<double> F,
                                                          const double* __restrict__ q, GasT<double> g,
                                                          int second_order, double limiter_k, AdjBufs A) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_index();
     if (c >= C.nC) return;
     const size_t nT = (size_t)C.nT, nC = (size_t)C.nC;
     double qc[NS], sx[NS], sy[NS], mn[NS], mx[NS];
@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(kBlock) k_sadj_forward(CellsT<double> C, Faces
 __global__ void __launch_bounds__(kBlock) k_sadj_face(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                       const double* __restrict__ w, GasT<double> g, int second_order,
                                                       AdjBufs A) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = xcd_index();
     if (f >= F.nF || F.kind[f] != K_INTERNAL) return;
     const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
     const int o = F.c0[f], n = F.c1[f];
@@ -468,7 +468,7 @@ __global__ void __launch_bounds__(kBlock) k_sadj_cell1(CellsT<double> C, FacesT<
                                                        const double* __restrict__ w, const double* __restrict__ wd,
                                                        GasT<double> g, int second_order, double limiter_k, AdjBufs A,
                                                        double* __restrict__ lam) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_index();
     if (c >= C.nC) return;
     const size_t nT = (size_t)C.nT, nC = (size_t)C.nC, nF = (size_t)F.nF;
     double lq[NS], lgx[NS], lgy[NS], lphi[NS], gx[NS], gy[NS], ph[NS], qc[NS];
@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(kBlock) k_sadj_cell1(CellsT<double> C, FacesT<
 // (face value q_o (1 - gw) + (q_o + q_n) gw / 2, every variable), boundary
 // faces in k_sadj_face2_bnd
 __global__ void __launch_bounds__(kBlock) k_sadj_face2(CellsT<double> C, FacesT<double> F, AdjBufs A) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = xcd_index();
     if (f >= F.nF || F.kind[f] != K_INTERNAL) return;
     const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
     const int o = F.c0[f], n = F.c1[f];
@@ -580,7 +580,7 @@ __global__ void __launch_bounds__(kBlock) k_sadj_face2_bnd(CellsT<double> C, Fac
 // pass 5: gather of the Green-Gauss transpose and the neighbours' limiter parts
 __global__ void __launch_bounds__(kBlock) k_sadj_cell2(CellsT<double> C, FacesT<double> F, AdjBufs A,
                                                        double* __restrict__ lam) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_index();
     if (c >= C.nC) return;
     const size_t nT = (size_t)C.nT, nC = (size_t)C.nC, nF = (size_t)F.nF;
     double acc[NS];
@@ -606,8 +606,7 @@ hipError_t launch_sa_gradients(hipStream_t s, CellsT<double> C, FacesT<double> F
     AdjBufs A{};
     A.gx = gx;
     A.gy = gy;
-    const int nbc = (C.nC + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_sadj_forward, dim3(nbc), dim3(kBlock), 0, s, C, F, q, g, 0, 0.0, A);
+    hipLaunchKernelGGL(k_sadj_forward, dim3(xcd_grid(C.nC)), dim3(kBlock), 0, s, C, F, q, g, 0, 0.0, A);
     return hipGetLastError();
 }
 
@@ -615,7 +614,7 @@ hipError_t launch_adjoint_state_sa(hipStream_t s, CellsT<double> C, FacesT<doubl
                                    const double* w, const double* wd, GasT<double> g, int second_order,
                                    double limiter_k, AdjBufs A, double* lam) {
     if (!A.vgx || !A.vgy || !A.vdq) return hipErrorInvalidValue;
-    const int nbc = (C.nC + kBlock - 1) / kBlock, nbf = (F.nF + kBlock - 1) / kBlock;
+    const int nbc = xcd_grid(C.nC), nbf = xcd_grid(F.nF);
     hipLaunchKernelGGL(k_sadj_forward, dim3(nbc), dim3(kBlock), 0, s, C, F, q, g, second_order, limiter_k, A);
     hipLaunchKernelGGL(k_sadj_face, dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A);
     if (A.nbf > 0)

@@ -627,6 +627,16 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
     return (b & 7) * per + (b >> 3);
 }
 
+// The same order for one-thread-per-cell (or per-face) loops: launched with
+// xcd_grid(n) blocks of kBlock threads, the thread's item is xcd_index().
+// Each XCD then sweeps a contiguous range of cells or faces, so the
+// neighbours' data that a block gathers was mostly fetched into that XCD's L2
+// by the blocks before it, instead of by another XCD.
+__device__ __forceinline__ int xcd_index() {
+    return xcd_tile((int)blockIdx.x, (int)gridDim.x) * (int)blockDim.x + (int)threadIdx.x;
+}
+inline int xcd_grid(int n) { return (((n + kBlock - 1) / kBlock) + 7) & ~7; }
+
 // BND: the tile's face list holds boundary faces; without, every face is
 // taken as interior at compile time (no kind tests, no boundary states, no
 // ghost writes) - the same operations on the same values, so the same bits
