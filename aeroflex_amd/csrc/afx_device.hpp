@@ -380,6 +380,24 @@ __device__ __forceinline__ T sa_viscous_radius(const GasT<T>& g, const T* q, siz
 // ---------------------------------------------------------------- kernels --
 constexpr int kBlock = 256;
 
+// XCD-aware block -> tile map: the 8 XCDs (block b runs on XCD b % 8) each
+// take a contiguous range of tiles so that halo cells shared by neighbouring
+// tiles are found in that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
+    const int per = nblocks >> 3;   // nblocks is a multiple of 8
+    return (b & 7) * per + (b >> 3);
+}
+
+// The same order for one-thread-per-cell (or per-face) loops: launched with
+// xcd_grid(n) blocks of kBlock threads, the thread's item is xcd_index().
+// Each XCD then sweeps a contiguous range of cells or faces, so the
+// neighbours' data that a block gathers was mostly fetched into that XCD's L2
+// by the blocks before it, instead of by another XCD.
+__device__ __forceinline__ int xcd_index() {
+    return xcd_tile((int)blockIdx.x, (int)gridDim.x) * (int)blockDim.x + (int)threadIdx.x;
+}
+inline int xcd_grid(int n) { return (((n + kBlock - 1) / kBlock) + 7) & ~7; }
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_set_walls(const int32_t* __restrict__ owner, const int32_t* __restrict__ ghost,
                                                       int32_t n, T* __restrict__ q, int32_t nT, int nv) {
@@ -393,7 +411,7 @@ __global__ void __launch_bounds__(kBlock) k_set_walls(const int32_t* __restrict_
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_dt(CellsT<T> C, FacesT<T> F, const T* __restrict__ q, GasT<T> g, T cfl,
                                                int sa, T* __restrict__ dt) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = xcd_index();
     if (c >= C.nT) return;
     const size_t nT = (size_t)C.nT;
     T s = T(0), sv = T(0);
@@ -429,7 +447,7 @@ __global__ void __launch_bounds__(kBlock) k_grad_lim(CellsT<T> C, FacesT<T> F, c
                                                      const T* __restrict__ ql, GasT<T> g, int gg, int second_order,
                                                      T limiter_k, T* __restrict__ gx, T* __restrict__ gy,
                                                      T* __restrict__ lim) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = xcd_index();
     if (c >= C.nC) return;
     const size_t nT = (size_t)C.nT;
     uint32_t ef[4];
@@ -529,7 +547,7 @@ __global__ void __launch_bounds__(kBlock) k_face_flux(CellsT<T> C, FacesT<T> F, 
                                                       const T* __restrict__ gy, const T* __restrict__ lim, GasT<T> g,
                                                       int viscous, int second_order, const T* __restrict__ uniform,
                                                       T* __restrict__ flux) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    int f = xcd_index();
     if (f >= F.nF) return;
     const size_t nT = (size_t)C.nT;
     const int o = F.c0[f], n = F.c1[f];
@@ -597,7 +615,7 @@ struct Acc<float> {
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_gather(CellsT<T> C, const T* __restrict__ flux, int32_t nF, int mode,
                                                    T* __restrict__ R) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = xcd_index();
     if (c >= C.nC) return;
     const size_t nT = (size_t)C.nT;
     Acc<T> acc[4];
@@ -619,23 +637,6 @@ __global__ void __launch_bounds__(kBlock) k_gather(CellsT<T> C, const T* __restr
 }
 
 
-// XCD-aware block -> tile map: the 8 XCDs (block b runs on XCD b % 8) each
-// take a contiguous range of tiles so that halo cells shared by neighbouring
-// tiles are found in that XCD's L2.
-__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
-    const int per = nblocks >> 3;   // nblocks is a multiple of 8
-    return (b & 7) * per + (b >> 3);
-}
-
-// The same order for one-thread-per-cell (or per-face) loops: launched with
-// xcd_grid(n) blocks of kBlock threads, the thread's item is xcd_index().
-// Each XCD then sweeps a contiguous range of cells or faces, so the
-// neighbours' data that a block gathers was mostly fetched into that XCD's L2
-// by the blocks before it, instead of by another XCD.
-__device__ __forceinline__ int xcd_index() {
-    return xcd_tile((int)blockIdx.x, (int)gridDim.x) * (int)blockDim.x + (int)threadIdx.x;
-}
-inline int xcd_grid(int n) { return (((n + kBlock - 1) / kBlock) + 7) & ~7; }
 
 // BND: the tile's face list holds boundary faces; without, every face is
 // taken as interior at compile time (no kind tests, no boundary states, no
@@ -1595,13 +1596,13 @@ inline int nblocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
     }                                                                                                             \
     template <>                                                                                                   \
     hipError_t launch_dt<T>(hipStream_t s, CellsT<T> C, FacesT<T> F, const T* q, GasT<T> g, T cfl, T* dt, int sa) { \
-        hipLaunchKernelGGL(k_dt<T>, dim3(nblocks(C.nT)), dim3(kBlock), 0, s, C, F, q, g, cfl, sa, dt);            \
+        hipLaunchKernelGGL(k_dt<T>, dim3(xcd_grid(C.nT)), dim3(kBlock), 0, s, C, F, q, g, cfl, sa, dt);            \
         return hipGetLastError();                                                                                 \
     }                                                                                                             \
     template <>                                                                                                   \
     hipError_t launch_grad_lim<T>(hipStream_t s, CellsT<T> C, FacesT<T> F, const T* qg, const T* ql, GasT<T> g,   \
                                   int gg, int so, T lk, T* gx, T* gy, T* lim) {                                   \
-        hipLaunchKernelGGL(k_grad_lim<T>, dim3(nblocks(C.nC)), dim3(kBlock), 0, s, C, F, qg, ql, g, gg, so, lk,   \
+        hipLaunchKernelGGL(k_grad_lim<T>, dim3(xcd_grid(C.nC)), dim3(kBlock), 0, s, C, F, qg, ql, g, gg, so, lk,   \
                            gx, gy, lim);                                                                          \
         return hipGetLastError();                                                                                 \
     }                                                                                                             \
@@ -1609,7 +1610,7 @@ inline int nblocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
     hipError_t launch_face_flux<T>(hipStream_t s, CellsT<T> C, FacesT<T> F, const T* qr, const T* qa, const T* gx, \
                                    const T* gy, const T* lim, GasT<T> g, int viscous, int so, const T* uniform,   \
                                    T* flux) {                                                                     \
-        hipLaunchKernelGGL(k_face_flux<T>, dim3(nblocks(F.nF)), dim3(kBlock), 0, s, C, F, qr, qa, gx, gy, lim, g, \
+        hipLaunchKernelGGL(k_face_flux<T>, dim3(xcd_grid(F.nF)), dim3(kBlock), 0, s, C, F, qr, qa, gx, gy, lim, g, \
                            viscous, so, uniform, flux);                                                           \
         return hipGetLastError();                                                                                 \
     }                                                                                                             \
@@ -1631,7 +1632,7 @@ inline int nblocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
     }                                                                                                             \
     template <>                                                                                                   \
     hipError_t launch_gather<T>(hipStream_t s, CellsT<T> C, const T* flux, int32_t nF, int mode, T* R) {          \
-        hipLaunchKernelGGL(k_gather<T>, dim3(nblocks(C.nC)), dim3(kBlock), 0, s, C, flux, nF, mode, R);           \
+        hipLaunchKernelGGL(k_gather<T>, dim3(xcd_grid(C.nC)), dim3(kBlock), 0, s, C, flux, nF, mode, R);           \
         return hipGetLastError();                                                                                 \
     }
 

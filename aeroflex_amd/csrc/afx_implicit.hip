@@ -27,7 +27,7 @@ namespace afx {
 __global__ void __launch_bounds__(kBlock) k_face_jacobian(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                           const double* __restrict__ gx, const double* __restrict__ gy,
                                                           GasT<double> g, int viscous, double* __restrict__ Jf) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    int f = xcd_index();
     if (f >= F.nF) return;
     const size_t nT = (size_t)C.nT;
     const int o = F.c0[f], n = F.c1[f];
@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(kBlock) k_face_jacobian_sa(CellsT<double> C, F
                                                              const double* __restrict__ gx,
                                                              const double* __restrict__ gy, GasT<double> g,
                                                              double* __restrict__ Jf) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = xcd_index();
     if (f >= F.nF) return;
     const size_t nT = (size_t)C.nT;
     const int o = F.c0[f], n = F.c1[f];
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble_diag(CellsT<double> C, Face
                                                           double* __restrict__ E, double* __restrict__ Ds,
                                                           int32_t* __restrict__ nbr, int* __restrict__ bad) {
     constexpr int NN = N * N;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = xcd_index();
     if (c >= C.nC) return;
     const size_t nC = (size_t)C.nC;
     double B[NN];
@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(kBlock) k_lhs_apply(int32_t nC, int32_t nT, co
                                                       const int* __restrict__ active) {
     constexpr int NN = N * N;
     if (active && *active == 0) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_index();
     if (c >= nT) return;
     const size_t n = (size_t)nT, m = (size_t)nC;
     if (c >= nC) {
@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_setup(CellsT<double> C, FacesT<
                                                        double* __restrict__ Ec, int32_t* __restrict__ nc,
                                                        double* __restrict__ Dc, int* __restrict__ bad) {
     constexpr int NN = N * N;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = xcd_index();
     if (k >= ncells) return;
     const size_t mc = (size_t)C.nC;
     const int p = p0 + k;
@@ -438,7 +438,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_sweep(int32_t nC, int32_t nT, i
                                                        double* __restrict__ y, const int* __restrict__ active) {
     constexpr int NN = N * N;
     if (active && *active == 0) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = xcd_index();
     if (k >= ncells) return;
     const size_t n = (size_t)nT, m = (size_t)nC;
     const int p = p0 + k;
@@ -703,22 +703,22 @@ __global__ void __launch_bounds__(kBlock) k_mf_apply(CellsT<double> C, FacesT<do
 // ---------------------------------------------------------------- launchers --
 hipError_t launch_face_jacobian(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* gx,
                                 const double* gy, GasT<double> g, int viscous, double* Jf) {
-    hipLaunchKernelGGL(k_face_jacobian, dim3(nblocks(F.nF)), dim3(kBlock), 0, s, C, F, q, gx, gy, g, viscous, Jf);
+    hipLaunchKernelGGL(k_face_jacobian, dim3(xcd_grid(F.nF)), dim3(kBlock), 0, s, C, F, q, gx, gy, g, viscous, Jf);
     return hipGetLastError();
 }
 hipError_t launch_face_jacobian_sa(hipStream_t s, CellsT<double> C, FacesT<double> F, const double* q, const double* gx,
                                    const double* gy, GasT<double> g, double* Jf) {
-    hipLaunchKernelGGL(k_face_jacobian_sa, dim3(nblocks(F.nF)), dim3(kBlock), 0, s, C, F, q, gx, gy, g, Jf);
+    hipLaunchKernelGGL(k_face_jacobian_sa, dim3(xcd_grid(F.nF)), dim3(kBlock), 0, s, C, F, q, gx, gy, g, Jf);
     return hipGetLastError();
 }
 hipError_t launch_assemble_diag(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const double* dt,
                                 const double* Jf, SaSource src, double* D, double* Dinv, double* E, double* Ds,
                                 int32_t* nbr, int* bad) {
     if (nv == 5)
-        hipLaunchKernelGGL(k_assemble_diag<5>, dim3(nblocks(C.nC)), dim3(kBlock), 0, s, C, F, dt, Jf, src, D, Dinv, E,
+        hipLaunchKernelGGL(k_assemble_diag<5>, dim3(xcd_grid(C.nC)), dim3(kBlock), 0, s, C, F, dt, Jf, src, D, Dinv, E,
                            Ds, nbr, bad);
     else if (nv == 4)
-        hipLaunchKernelGGL(k_assemble_diag<4>, dim3(nblocks(C.nC)), dim3(kBlock), 0, s, C, F, dt, Jf, src, D, Dinv, E,
+        hipLaunchKernelGGL(k_assemble_diag<4>, dim3(xcd_grid(C.nC)), dim3(kBlock), 0, s, C, F, dt, Jf, src, D, Dinv, E,
                            Ds, nbr, bad);
     else
         return hipErrorInvalidValue;
@@ -727,9 +727,9 @@ hipError_t launch_assemble_diag(hipStream_t s, int nv, CellsT<double> C, FacesT<
 hipError_t launch_lhs_apply(hipStream_t s, int nv, int32_t nC, int32_t nT, const double* Ds, const double* E,
                             const int32_t* nbr, const double* x, double* y, const int* active) {
     if (nv == 5)
-        hipLaunchKernelGGL(k_lhs_apply<5>, dim3(nblocks(nT)), dim3(kBlock), 0, s, nC, nT, Ds, E, nbr, x, y, active);
+        hipLaunchKernelGGL(k_lhs_apply<5>, dim3(xcd_grid(nT)), dim3(kBlock), 0, s, nC, nT, Ds, E, nbr, x, y, active);
     else if (nv == 4)
-        hipLaunchKernelGGL(k_lhs_apply<4>, dim3(nblocks(nT)), dim3(kBlock), 0, s, nC, nT, Ds, E, nbr, x, y, active);
+        hipLaunchKernelGGL(k_lhs_apply<4>, dim3(xcd_grid(nT)), dim3(kBlock), 0, s, nC, nT, Ds, E, nbr, x, y, active);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -751,7 +751,7 @@ static hipError_t dilu_setup_n(hipStream_t s, CellsT<double> C, FacesT<double> F
     for (int c = 0; c < ncolours; ++c) {
         const int32_t n = coff[c + 1] - coff[c];
         if (n == 0) continue;
-        hipLaunchKernelGGL(k_dilu_setup<N>, dim3(nblocks(n)), dim3(kBlock), 0, s, C, F, colour, c, cells, coff[c], n,
+        hipLaunchKernelGGL(k_dilu_setup<N>, dim3(xcd_grid(n)), dim3(kBlock), 0, s, C, F, colour, c, cells, coff[c], n,
                            D, Jf, Dti, Ec, nc, Dc, bad);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -777,14 +777,14 @@ static hipError_t dilu_apply_n(hipStream_t s, int32_t nC, int32_t nT, const int3
     for (int c = 0; c < ncolours; ++c) {
         const int32_t n = coff[c + 1] - coff[c];
         if (n == 0) continue;
-        hipLaunchKernelGGL((k_dilu_sweep<N, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, nC, nT, coff[c], n, cells,
+        hipLaunchKernelGGL((k_dilu_sweep<N, true>), dim3(xcd_grid(n)), dim3(kBlock), 0, s, nC, nT, coff[c], n, cells,
                            Ec, nc, Dc, x, z, active);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     for (int c = ncolours - 2; c >= 0; --c) {
         const int32_t n = coff[c + 1] - coff[c];
         if (n == 0) continue;
-        hipLaunchKernelGGL((k_dilu_sweep<N, false>), dim3(nblocks(n)), dim3(kBlock), 0, s, nC, nT, coff[c], n, cells,
+        hipLaunchKernelGGL((k_dilu_sweep<N, false>), dim3(xcd_grid(n)), dim3(kBlock), 0, s, nC, nT, coff[c], n, cells,
                            Ec, nc, Dc, x, z, active);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
