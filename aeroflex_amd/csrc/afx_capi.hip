@@ -148,6 +148,10 @@ struct afx_rans {
     DBuf<int32_t> dilN;
     DBuf<int> kbad, kflags;
     int32_t gm_restart = 30, gm_maxit = 500;
+    // Arnoldi steps enqueued before the host first reads the flags of a
+    // cycle: the column count of the last cycle that converged (its solves
+    // tend to need about as many), 8 at first
+    int32_t gm_chunk = 8;
     double gm_tol = 1e-2;
     // preconditioner: 0 block Jacobi, 1 multicolour block DILU (colour per
     // real cell, cells by colour, colour offsets on the host, Dt^-1 blocks)
@@ -558,10 +562,13 @@ int ensure_implicit(afx_rans* h) {
 
 // Right-preconditioned restarted GMRES (classical Gram-Schmidt applied twice)
 // for LHS x = b; b and x are device vectors of nv planes of nT.  The Arnoldi
-// state stays on the device (GmDev): a restart cycle of m steps is enqueued
-// at once, the kernels of a step do nothing once the iteration has
-// converged or broken down, and the host reads the flags once per cycle.
-// Returns AFX_OK and *converged, or a negative error code.
+// state stays on the device (GmDev): the steps of a restart cycle are
+// enqueued in chunks, the kernels of a step do nothing once the iteration
+// has converged or broken down, and between chunks the host reads the flags
+// and ends the cycle early once the Arnoldi process is inactive (the steps
+// it skips would do nothing, so the result is the same as enqueueing all m).
+// The first chunk has as many steps as the last converged cycle took, later
+// ones four.  Returns AFX_OK and *converged, or a negative error code.
 int gmres(afx_rans* h, const double* b, double* x, bool* converged) {
     const int64_t n = (int64_t)h->nv * h->nT;
     const int m = h->gm_restart;
@@ -604,7 +611,14 @@ int gmres(afx_rans* h, const double* b, double* x, bool* converged) {
     int flags[4] = {0, 0, 0, 0};
     int total_before = 0;
     while (true) {
+        int chunk = std::max(1, std::min<int>(h->gm_chunk, m));
         for (int j = 0; j < m; ++j) {
+            if (j == chunk) {   // a chunk is enqueued: is the Arnoldi process still active?
+                HIPCHK(hipMemcpyAsync(flags, G.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                if (!flags[0]) break;
+                chunk += 4;
+            }
             double* vj = V + (size_t)j * n;
             HIPCHK(precond(vj, h->kz.p, &pz, active));
             HIPCHK(apply(pz, h->kw.p, active));
@@ -626,7 +640,11 @@ int gmres(afx_rans* h, const double* b, double* x, bool* converged) {
         HIPCHK(hipStreamSynchronize(s));
         h->lin_iters += flags[2] - total_before;
         total_before = flags[2];
-        if (flags[1]) { *converged = true; return AFX_OK; }
+        if (flags[1]) {
+            h->gm_chunk = std::max(1, flags[3]);
+            *converged = true;
+            return AFX_OK;
+        }
         if (flags[2] >= h->gm_maxit) return AFX_OK;
         // restart from the true residual
         HIPCHK(apply(x, h->kw.p, nullptr));
