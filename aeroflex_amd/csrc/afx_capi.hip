@@ -498,7 +498,7 @@ int ensure_implicit(afx_rans* h) {
     const size_t nn = (size_t)nv * nv;
     const size_t n = (size_t)nv * h->nT;
     const int m = h->gm_restart;
-    h->kblk = 512;
+    h->kblk = 2048;   // at most eight waves per SIMD in the Gram-Schmidt passes
     HIPCHK(h->Jf.alloc(2 * nn * h->nF));
     HIPCHK(h->Dblk.alloc(nn * h->nC));
     HIPCHK(h->Dinv.alloc(nn * h->nC));
@@ -572,7 +572,8 @@ int ensure_implicit(afx_rans* h) {
 int gmres(afx_rans* h, const double* b, double* x, bool* converged) {
     const int64_t n = (int64_t)h->nv * h->nT;
     const int m = h->gm_restart;
-    const int nb = h->kblk;
+    // Gram-Schmidt blocks: about two entries per thread, at most kblk
+    const int nb = (int)std::min<int64_t>(h->kblk, std::max<int64_t>(8, (n + 511) / 512));
     hipStream_t s = h->stream;
     double* V = h->kV.p;
     double* part = h->kpart.p;

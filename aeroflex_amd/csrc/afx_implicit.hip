@@ -4,7 +4,7 @@
 // DILU preconditioners, the operator product and the kernels of the
 // device-resident restarted GMRES driven from afx_capi.hip (the Arnoldi
 // coefficients, Givens rotations and convergence flags live in device
-// memory; the host reads the flags once per restart cycle).
+// memory; the host reads the flags between chunks of Arnoldi steps).
 //
 // Blocks are N x N with N = 4 (the reference's four variables) or N = 5 (the
 // Spalart-Allmaras model, whose Jacobian is this build's own: the reference's
@@ -488,20 +488,20 @@ __global__ void __launch_bounds__(kBlock) k_copy_ghost_rows(int32_t nC, int32_t 
 // sums that k_gm_reduce adds up.
 constexpr int kGsMax = 32;
 
-template <bool UPDATE, bool DOTS>
+template <bool UPDATE, bool DOTS, int MM>
 __global__ void __launch_bounds__(kBlock) k_gs_pass(const double* __restrict__ V, int64_t ld, int m,
                                                     const double* __restrict__ h, double* __restrict__ w, int64_t n,
                                                     double* __restrict__ partial, const int* __restrict__ active) {
     if (active && *active == 0) return;
-    __shared__ double red[kBlock / 64][kGsMax + 1];
-    double acc[kGsMax + 1];
+    __shared__ double red[kBlock / 64][MM + 1];
+    double acc[MM + 1];
 #pragma unroll
-    for (int k = 0; k <= kGsMax; ++k) acc[k] = 0.0;
+    for (int k = 0; k <= MM; ++k) acc[k] = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double v[kGsMax];
+        double v[MM];
         double wi = w[i];
 #pragma unroll
-        for (int k = 0; k < kGsMax; ++k) {
+        for (int k = 0; k < MM; ++k) {
             if (k < m && (UPDATE || DOTS)) {
                 v[k] = V[(size_t)k * ld + i];
                 if (UPDATE) wi -= h[k] * v[k];
@@ -509,14 +509,14 @@ __global__ void __launch_bounds__(kBlock) k_gs_pass(const double* __restrict__ V
         }
         if (UPDATE) w[i] = wi;
 #pragma unroll
-        for (int k = 0; k < kGsMax; ++k)
+        for (int k = 0; k < MM; ++k)
             if (DOTS && k < m) acc[k] += v[k] * wi;
-        acc[kGsMax] += wi * wi;
+        acc[MM] += wi * wi;
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k <= kGsMax; ++k) {
-        if (k == kGsMax || (DOTS && k < m)) {
+    for (int k = 0; k <= MM; ++k) {
+        if (k == MM || (DOTS && k < m)) {
             double t = acc[k];
             for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
             if (lane == 0) red[wave][k] = t;
@@ -525,24 +525,31 @@ __global__ void __launch_bounds__(kBlock) k_gs_pass(const double* __restrict__ V
     __syncthreads();
     const int k = threadIdx.x;
     if (k <= m && (DOTS || k == m)) {
-        const int src = k == m ? kGsMax : k;
+        const int src = k == m ? MM : k;
         double t = 0.0;
         for (int q = 0; q < kBlock / 64; ++q) t += red[q][src];
         partial[(size_t)k * gridDim.x + blockIdx.x] = t;
     }
 }
 
-// out[k] = sum_b partial[(first + k) * nblk + b], k < cnt; one block, a wavefront per k
+// out[k] = sum_b partial[(first + k) * nblk + b], k < cnt; one block per k
 __global__ void __launch_bounds__(kBlock) k_gm_reduce(const double* __restrict__ partial, int nblk, int first, int cnt,
                                                       double* __restrict__ out, const int* __restrict__ active) {
     if (active && *active == 0) return;
-    const int lane = threadIdx.x & 63;
-    for (int k = threadIdx.x >> 6; k < cnt; k += kBlock / 64) {
-        const double* p = partial + (size_t)(first + k) * nblk;
-        double t = 0.0;
-        for (int b = lane; b < nblk; b += 64) t += p[b];
-        for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
-        if (lane == 0) out[k] = t;
+    // one block per k: every thread sums a strided share of the nblk partials
+    __shared__ double red[kBlock / 64];
+    const int k = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (k >= cnt) return;
+    const double* p = partial + (size_t)(first + k) * nblk;
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kBlock) t += p[b];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+    if (lane == 0) red[wave] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int q = 0; q < kBlock / 64; ++q) r += red[q];
+        out[k] = r;
     }
 }
 
@@ -797,22 +804,36 @@ hipError_t launch_dilu_apply(hipStream_t s, int nv, int32_t nC, int32_t nT, cons
     if (nv == 4) return dilu_apply_n<4>(s, nC, nT, cells, coff, ncolours, Ec, nc, Dc, x, z, active);
     return hipErrorInvalidValue;
 }
+template <int MM>
+hipError_t gs_pass_variant(hipStream_t s, const double* V, int64_t ld, int m, const double* h, double* w, int64_t n,
+                           bool dots, double* partial, int nblk, const int* active) {
+    if (h && dots)
+        hipLaunchKernelGGL((k_gs_pass<true, true, MM>), dim3(nblk), dim3(kBlock), 0, s, V, ld, m, h, w, n, partial,
+                           active);
+    else if (h)
+        hipLaunchKernelGGL((k_gs_pass<true, false, MM>), dim3(nblk), dim3(kBlock), 0, s, V, ld, m, h, w, n, partial,
+                           active);
+    else if (dots)
+        hipLaunchKernelGGL((k_gs_pass<false, true, MM>), dim3(nblk), dim3(kBlock), 0, s, V, ld, m, h, w, n, partial,
+                           active);
+    else
+        hipLaunchKernelGGL((k_gs_pass<false, false, MM>), dim3(nblk), dim3(kBlock), 0, s, V, ld, m, h, w, n, partial,
+                           active);
+    return hipGetLastError();
+}
 hipError_t launch_gs_pass(hipStream_t s, const double* V, int64_t ld, int m, const double* h, double* w, int64_t n,
                           bool dots, double* partial, int nblk, const int* active) {
     if (m > kGsMax) return hipErrorInvalidValue;
-    if (h && dots)
-        hipLaunchKernelGGL((k_gs_pass<true, true>), dim3(nblk), dim3(kBlock), 0, s, V, ld, m, h, w, n, partial, active);
-    else if (h)
-        hipLaunchKernelGGL((k_gs_pass<true, false>), dim3(nblk), dim3(kBlock), 0, s, V, ld, m, h, w, n, partial, active);
-    else if (dots)
-        hipLaunchKernelGGL((k_gs_pass<false, true>), dim3(nblk), dim3(kBlock), 0, s, V, ld, m, h, w, n, partial, active);
-    else
-        hipLaunchKernelGGL((k_gs_pass<false, false>), dim3(nblk), dim3(kBlock), 0, s, V, ld, m, h, w, n, partial, active);
-    return hipGetLastError();
+    // the variant with the fewest registers that holds m columns: the small
+    // bases of short restart cycles keep more waves in flight per SIMD
+    if (m <= 8) return gs_pass_variant<8>(s, V, ld, m, h, w, n, dots, partial, nblk, active);
+    if (m <= 16) return gs_pass_variant<16>(s, V, ld, m, h, w, n, dots, partial, nblk, active);
+    return gs_pass_variant<kGsMax>(s, V, ld, m, h, w, n, dots, partial, nblk, active);
 }
 hipError_t launch_gm_reduce(hipStream_t s, const double* partial, int nblk, int first, int cnt, double* out,
                             const int* active) {
-    hipLaunchKernelGGL(k_gm_reduce, dim3(1), dim3(kBlock), 0, s, partial, nblk, first, cnt, out, active);
+    if (cnt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gm_reduce, dim3(cnt), dim3(kBlock), 0, s, partial, nblk, first, cnt, out, active);
     return hipGetLastError();
 }
 hipError_t launch_gm_begin(hipStream_t s, GmDev G, int first) {
