@@ -149,29 +149,51 @@ __global__ void __launch_bounds__(kBlock) k_face_jacobian_sa(CellsT<double> C, F
     }
 }
 
-// N x N inverse by Gauss-Jordan with partial pivoting; returns false if singular
+// N x N inverse by Gauss-Jordan with partial pivoting; returns false if
+// singular.  Every loop is unrolled and the row exchange is a select over the
+// candidate rows, so M is indexed at compile time only and stays in
+// registers (a runtime row index would put it in scratch memory).
 template <int N>
-__device__ bool invN(const double* A, double* X) {
+__device__ __forceinline__ bool invN(const double* A, double* X) {
     double M[N][2 * N];
+#pragma unroll
     for (int r = 0; r < N; ++r)
+#pragma unroll
         for (int c = 0; c < N; ++c) { M[r][c] = A[N * r + c]; M[r][c + N] = (r == c) ? 1.0 : 0.0; }
+#pragma unroll
     for (int c = 0; c < N; ++c) {
         int p = c;
         double best = dabs<double>(M[c][c]);
-        for (int r = c + 1; r < N; ++r)
-            if (dabs<double>(M[r][c]) > best) { best = dabs<double>(M[r][c]); p = r; }
+#pragma unroll
+        for (int r = c + 1; r < N; ++r) {
+            const double a = dabs<double>(M[r][c]);
+            if (a > best) { best = a; p = r; }
+        }
         if (!(best > 0.0) || !isfinite(best)) return false;
-        if (p != c)
-            for (int k = 0; k < 2 * N; ++k) { double t = M[c][k]; M[c][k] = M[p][k]; M[p][k] = t; }
+#pragma unroll
+        for (int r = c + 1; r < N; ++r) {   // exchange rows c and p
+            const bool sw = r == p;
+#pragma unroll
+            for (int k = 0; k < 2 * N; ++k) {
+                const double a = M[c][k], b = M[r][k];
+                M[c][k] = sw ? b : a;
+                M[r][k] = sw ? a : b;
+            }
+        }
         const double ip = 1.0 / M[c][c];
+#pragma unroll
         for (int k = 0; k < 2 * N; ++k) M[c][k] *= ip;
+#pragma unroll
         for (int r = 0; r < N; ++r) {
             if (r == c) continue;
             const double fct = M[r][c];
+#pragma unroll
             for (int k = 0; k < 2 * N; ++k) M[r][k] -= fct * M[c][k];
         }
     }
+#pragma unroll
     for (int r = 0; r < N; ++r)
+#pragma unroll
         for (int c = 0; c < N; ++c) X[N * r + c] = M[r][c + N];
     return true;
 }
