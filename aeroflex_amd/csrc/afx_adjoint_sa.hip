@@ -394,10 +394,14 @@ __global__ void __launch_bounds__(kBlock) k_sadj_face(CellsT<double> C, FacesT<d
     const double s = up ? qL[4] / qL[0] : qR[4] / qR[0];
     const double bRoe[4] = {bF[0] + bF[4] * s, bF[1], bF[2], bF[3]};
     roe_flux_vjp<double>(g.gamma, nx, ny, qL, qR, bRoe, bL, bR);
-    double* bu = up ? bL : bR;
-    const double* qu = up ? qL : qR;
-    bu[4] += bF[4] * fr[0] / qu[0];
-    bu[0] -= bF[4] * fr[0] * qu[4] / (qu[0] * qu[0]);
+    // s's dependence, on the upwind side (selects, not a pointer to one of
+    // the two arrays, which would move both to scratch memory)
+    const double qu0 = up ? qL[0] : qR[0], qu4 = up ? qL[4] : qR[4];
+    const double d4 = bF[4] * fr[0] / qu0, d0 = bF[4] * fr[0] * qu4 / (qu0 * qu0);
+    bL[4] = up ? bL[4] + d4 : bL[4];
+    bR[4] = up ? bR[4] : bR[4] + d4;
+    bL[0] = up ? bL[0] - d0 : bL[0];
+    bR[0] = up ? bR[0] : bR[0] - d0;
     // the diffusive part (sa_face_diffusion): its hand-written reverse
     double bgx[NS], bgy[NS];
     sa_face_diffusion_vjp(g, nx, ny, qL, qR, gxa, gya, bF, bL, bR, bgx, bgy);
