@@ -157,7 +157,8 @@ struct afx_rans {
     // real cell, cells by colour, colour offsets on the host, Dt^-1 blocks)
     int precond = 1;
     DBuf<uint8_t> colour;
-    DBuf<int32_t> col_cells;
+    DBuf<int32_t> col_cells, col_pos;   // real cells in colour order, and each cell's position in it
+    DBuf<double> kdil;                   // the DILU sweeps' work vector, by colour position (nv planes of nC)
     std::vector<int32_t> col_off;
     DBuf<double> Dti;
     GmDev gm_dev() const {
@@ -555,6 +556,11 @@ int ensure_implicit(afx_rans* h) {
         HIPCHK(h->colour.upload(col, h->stream));
         HIPCHK(h->col_cells.alloc(std::max<size_t>(nC, 1)));
         HIPCHK(h->col_cells.upload(cells, h->stream));
+        std::vector<int32_t> where(nC);
+        for (int32_t p = 0; p < nC; ++p) where[cells[p]] = p;
+        HIPCHK(h->col_pos.alloc(std::max<size_t>(nC, 1)));
+        HIPCHK(h->col_pos.upload(where, h->stream));
+        HIPCHK(h->kdil.alloc(std::max<size_t>((size_t)nv * nC, 1)));
         HIPCHK(hipStreamSynchronize(h->stream));
     }
     return AFX_OK;
@@ -593,8 +599,9 @@ int gmres(afx_rans* h, const double* b, double* x, bool* converged) {
         if (h->mf) { *res = in; return hipSuccess; }
         *res = out;
         if (h->precond == 1)
-            return launch_dilu_apply(s, h->nv, h->nC, h->nT, h->col_cells.p, h->col_off.data(),
-                                     (int)h->col_off.size() - 1, h->dilE.p, h->dilN.p, h->dilD.p, in, out, act);
+            return launch_dilu_apply(s, h->nv, h->nC, h->nT, h->col_cells.p, h->col_pos.p, h->col_off.data(),
+                                     (int)h->col_off.size() - 1, h->dilE.p, h->dilN.p, h->dilD.p, in, out, h->kdil.p,
+                                     act);
         return launch_block_jacobi(s, h->nv, h->nC, h->nT, h->Dinv.p, in, out, act);
     };
     // |r|^2 of the cycle's start vector into hdr[3], then the cycle's set-up
@@ -1588,7 +1595,7 @@ int afx_rans_implicit_fill(afx_rans* h) {
     HIPCHK(launch_assemble_diag(s, h->nv, h->cells64(), h->faces64(), h->dt.p, h->Jf.p, src, h->Dblk.p, h->Dinv.p,
                                 h->ellE.p, h->ellD.p, h->ellN.p, h->kbad.p));
     if (h->precond == 1)
-        HIPCHK(launch_dilu_setup(s, h->nv, h->cells64(), h->faces64(), h->colour.p, h->col_cells.p,
+        HIPCHK(launch_dilu_setup(s, h->nv, h->cells64(), h->faces64(), h->colour.p, h->col_cells.p, h->col_pos.p,
                                  h->col_off.data(), (int)h->col_off.size() - 1, h->Dblk.p, h->Jf.p, h->Dti.p,
                                  h->dilE.p, h->dilN.p, h->dilD.p, h->kbad.p));
     HIPCHK(hipMemcpyAsync(&h->lhs_bad, h->kbad.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1685,8 +1692,9 @@ int afx_rans_precond_apply_nu(afx_rans* h, const double* x, const double* x_nu, 
     if (h->mf) return fail(AFX_ESTATE, "the matrix-free mode has no preconditioner");
     if ((rc = upload_operand(h, x, x_nu, y_nu))) return rc;
     if (h->precond == 1)
-        HIPCHK(launch_dilu_apply(h->stream, h->nv, h->nC, h->nT, h->col_cells.p, h->col_off.data(),
-                                 (int)h->col_off.size() - 1, h->dilE.p, h->dilN.p, h->dilD.p, h->ktmp.p, h->kw.p));
+        HIPCHK(launch_dilu_apply(h->stream, h->nv, h->nC, h->nT, h->col_cells.p, h->col_pos.p, h->col_off.data(),
+                                 (int)h->col_off.size() - 1, h->dilE.p, h->dilN.p, h->dilD.p, h->ktmp.p, h->kw.p,
+                                 h->kdil.p));
     else
         HIPCHK(launch_block_jacobi(h->stream, h->nv, h->nC, h->nT, h->Dinv.p, h->ktmp.p, h->kw.p));
     return download_operand(h, y, y_nu);

@@ -381,12 +381,14 @@ __device__ __forceinline__ const double* offdiag_t(const double* Jf, uint32_t e,
 // Besides Dt_i^-1 (AoS, read back by the later colours' set-up) it writes the
 // sweeps' colour-ordered ELL copy of row i at its position p: the
 // off-diagonal blocks A_ij by slot at Ec[(s NN + k) nC + p], the neighbour
-// code nc[s nC + p] = j + 1 for a neighbour of an earlier colour, -(j + 1)
-// for a later one, 0 for none, and Dt_i^-1 at Dc[k nC + p], so the threads of
-// one colour's sweep read consecutive addresses.
+// code nc[s nC + p] = pos_j + 1 for a neighbour of an earlier colour,
+// -(pos_j + 1) for a later one, 0 for none (pos_j: the neighbour's position in
+// the colour order), and Dt_i^-1 at Dc[k nC + p], so the threads of one
+// colour's sweep read consecutive addresses.
 template <int N>
 __global__ void __launch_bounds__(kBlock) k_dilu_setup(CellsT<double> C, FacesT<double> F, const uint8_t* __restrict__ colour,
-                                                       int c, const int32_t* __restrict__ cells, int32_t p0,
+                                                       int c, const int32_t* __restrict__ cells,
+                                                       const int32_t* __restrict__ pos, int32_t p0,
                                                        int32_t ncells, const double* __restrict__ D,
                                                        const double* __restrict__ Jf, double* __restrict__ Dti,
                                                        double* __restrict__ Ec, int32_t* __restrict__ nc,
@@ -439,7 +441,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_setup(CellsT<double> C, FacesT<
             int j;
             double sa;
             const double* Aij = offdiag<N>(F, Jf, e, &j, &sa);
-            code = colour[j] < c ? j + 1 : -(j + 1);
+            code = colour[j] < c ? pos[j] + 1 : -(pos[j] + 1);
             for (int r = 0; r < NN; ++r) Ec[((size_t)s * NN + r) * mc + p] = sa * Aij[r];
         } else {
             for (int r = 0; r < NN; ++r) Ec[((size_t)s * NN + r) * mc + p] = 0.0;
@@ -451,33 +453,39 @@ __global__ void __launch_bounds__(kBlock) k_dilu_setup(CellsT<double> C, FacesT<
 // forward sweep of colour c: y_i = Dt_i^-1 (b_i - sum_{j before i} A_ij y_j)
 // backward sweep of colour c (in place): y_i -= Dt_i^-1 sum_{j after i} A_ij y_j
 // over positions p0 .. p0 + ncells of the colour order, from the set-up's
-// colour-ordered copy (the sign of A_ij sits in the block, exactly)
+// colour-ordered copy (the sign of A_ij sits in the block, exactly).  y is
+// held by position (yp, nv planes of nC), so each sweep reads and writes it
+// with consecutive lanes on consecutive addresses; b is in cell order.
 template <int N, bool FORWARD>
 __global__ void __launch_bounds__(kBlock) k_dilu_sweep(int32_t nC, int32_t nT, int32_t p0, int32_t ncells,
                                                        const int32_t* __restrict__ cells,
                                                        const double* __restrict__ Ec, const int32_t* __restrict__ nc,
                                                        const double* __restrict__ Dc, const double* __restrict__ b,
-                                                       double* __restrict__ y, const int* __restrict__ active) {
+                                                       double* __restrict__ yp, const int* __restrict__ active) {
     constexpr int NN = N * N;
     if (active && *active == 0) return;
     const int k = xcd_index();
     if (k >= ncells) return;
     const size_t n = (size_t)nT, m = (size_t)nC;
     const int p = p0 + k;
-    const int i = cells[p];
     int code[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) code[s] = nc[s * m + p];
     double acc[N];
-    for (int r = 0; r < N; ++r) acc[r] = FORWARD ? b[r * n + i] : 0.0;
+    if (FORWARD) {
+        const int i = cells[p];
+        for (int r = 0; r < N; ++r) acc[r] = b[r * n + i];
+    } else {
+        for (int r = 0; r < N; ++r) acc[r] = 0.0;
+    }
     bool any = FORWARD;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         if (FORWARD ? code[s] <= 0 : code[s] >= 0) continue;
         any = true;
-        const int j = (FORWARD ? code[s] : -code[s]) - 1;
+        const int pj = (FORWARD ? code[s] : -code[s]) - 1;   // the neighbour's position
         double yj[N];
-        for (int r = 0; r < N; ++r) yj[r] = y[r * n + j];
+        for (int r = 0; r < N; ++r) yj[r] = yp[r * m + pj];
         const double* A = Ec + (size_t)s * NN * m + p;
         for (int r = 0; r < N; ++r) {
             double t = 0.0;
@@ -489,9 +497,21 @@ __global__ void __launch_bounds__(kBlock) k_dilu_sweep(int32_t nC, int32_t nT, i
     for (int r = 0; r < N; ++r) {
         double t = 0.0;
         for (int q = 0; q < N; ++q) t += Dc[(r * N + q) * m + p] * acc[q];
-        if (FORWARD) y[r * n + i] = t;
-        else y[r * n + i] -= t;
+        if (FORWARD) yp[r * m + p] = t;
+        else yp[r * m + p] -= t;
     }
+}
+
+// z_i = yp at the position of real cell i (the sweeps' result in cell order)
+__global__ void __launch_bounds__(kBlock) k_dilu_unpermute(int32_t nC, int32_t nT, int nv,
+                                                           const int32_t* __restrict__ pos,
+                                                           const double* __restrict__ yp, double* __restrict__ z,
+                                                           const int* __restrict__ active) {
+    if (active && *active == 0) return;
+    const int i = xcd_index();
+    if (i >= nC) return;
+    const int p = pos[i];
+    for (int r = 0; r < nv; ++r) z[(size_t)r * nT + i] = yp[(size_t)r * nC + p];
 }
 
 // ghost rows are identity rows: z = x there
@@ -775,29 +795,31 @@ hipError_t launch_block_jacobi(hipStream_t s, int nv, int32_t nC, int32_t nT, co
 }
 template <int N>
 static hipError_t dilu_setup_n(hipStream_t s, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
-                               const int32_t* cells, const int32_t* coff, int ncolours, const double* D,
-                               const double* Jf, double* Dti, double* Ec, int32_t* nc, double* Dc, int* bad) {
+                               const int32_t* cells, const int32_t* pos, const int32_t* coff, int ncolours,
+                               const double* D, const double* Jf, double* Dti, double* Ec, int32_t* nc, double* Dc,
+                               int* bad) {
     for (int c = 0; c < ncolours; ++c) {
         const int32_t n = coff[c + 1] - coff[c];
         if (n == 0) continue;
-        hipLaunchKernelGGL(k_dilu_setup<N>, dim3(xcd_grid(n)), dim3(kBlock), 0, s, C, F, colour, c, cells, coff[c], n,
-                           D, Jf, Dti, Ec, nc, Dc, bad);
+        hipLaunchKernelGGL(k_dilu_setup<N>, dim3(xcd_grid(n)), dim3(kBlock), 0, s, C, F, colour, c, cells, pos,
+                           coff[c], n, D, Jf, Dti, Ec, nc, Dc, bad);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
 hipError_t launch_dilu_setup(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
-                             const int32_t* cells, const int32_t* coff, int ncolours, const double* D, const double* Jf,
-                             double* Dti, double* Ec, int32_t* nc, double* Dc, int* bad) {
-    if (nv == 5) return dilu_setup_n<5>(s, C, F, colour, cells, coff, ncolours, D, Jf, Dti, Ec, nc, Dc, bad);
-    if (nv == 4) return dilu_setup_n<4>(s, C, F, colour, cells, coff, ncolours, D, Jf, Dti, Ec, nc, Dc, bad);
+                             const int32_t* cells, const int32_t* pos, const int32_t* coff, int ncolours,
+                             const double* D, const double* Jf, double* Dti, double* Ec, int32_t* nc, double* Dc,
+                             int* bad) {
+    if (nv == 5) return dilu_setup_n<5>(s, C, F, colour, cells, pos, coff, ncolours, D, Jf, Dti, Ec, nc, Dc, bad);
+    if (nv == 4) return dilu_setup_n<4>(s, C, F, colour, cells, pos, coff, ncolours, D, Jf, Dti, Ec, nc, Dc, bad);
     return hipErrorInvalidValue;
 }
 template <int N>
-static hipError_t dilu_apply_n(hipStream_t s, int32_t nC, int32_t nT, const int32_t* cells, const int32_t* coff,
-                               int ncolours, const double* Ec, const int32_t* nc, const double* Dc, const double* x,
-                               double* z, const int* active) {
+static hipError_t dilu_apply_n(hipStream_t s, int32_t nC, int32_t nT, const int32_t* cells, const int32_t* pos,
+                               const int32_t* coff, int ncolours, const double* Ec, const int32_t* nc, const double* Dc,
+                               const double* x, double* z, double* yp, const int* active) {
     hipError_t e;
     if (nT > nC) {
         hipLaunchKernelGGL(k_copy_ghost_rows, dim3(nblocks(nT - nC)), dim3(kBlock), 0, s, nC, nT, N, x, z, active);
@@ -807,23 +829,27 @@ static hipError_t dilu_apply_n(hipStream_t s, int32_t nC, int32_t nT, const int3
         const int32_t n = coff[c + 1] - coff[c];
         if (n == 0) continue;
         hipLaunchKernelGGL((k_dilu_sweep<N, true>), dim3(xcd_grid(n)), dim3(kBlock), 0, s, nC, nT, coff[c], n, cells,
-                           Ec, nc, Dc, x, z, active);
+                           Ec, nc, Dc, x, yp, active);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     for (int c = ncolours - 2; c >= 0; --c) {
         const int32_t n = coff[c + 1] - coff[c];
         if (n == 0) continue;
         hipLaunchKernelGGL((k_dilu_sweep<N, false>), dim3(xcd_grid(n)), dim3(kBlock), 0, s, nC, nT, coff[c], n, cells,
-                           Ec, nc, Dc, x, z, active);
+                           Ec, nc, Dc, x, yp, active);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (nC > 0) {
+        hipLaunchKernelGGL(k_dilu_unpermute, dim3(xcd_grid(nC)), dim3(kBlock), 0, s, nC, nT, N, pos, yp, z, active);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;
 }
-hipError_t launch_dilu_apply(hipStream_t s, int nv, int32_t nC, int32_t nT, const int32_t* cells, const int32_t* coff,
-                             int ncolours, const double* Ec, const int32_t* nc, const double* Dc, const double* x,
-                             double* z, const int* active) {
-    if (nv == 5) return dilu_apply_n<5>(s, nC, nT, cells, coff, ncolours, Ec, nc, Dc, x, z, active);
-    if (nv == 4) return dilu_apply_n<4>(s, nC, nT, cells, coff, ncolours, Ec, nc, Dc, x, z, active);
+hipError_t launch_dilu_apply(hipStream_t s, int nv, int32_t nC, int32_t nT, const int32_t* cells, const int32_t* pos,
+                             const int32_t* coff, int ncolours, const double* Ec, const int32_t* nc, const double* Dc,
+                             const double* x, double* z, double* yp, const int* active) {
+    if (nv == 5) return dilu_apply_n<5>(s, nC, nT, cells, pos, coff, ncolours, Ec, nc, Dc, x, z, yp, active);
+    if (nv == 4) return dilu_apply_n<4>(s, nC, nT, cells, pos, coff, ncolours, Ec, nc, Dc, x, z, yp, active);
     return hipErrorInvalidValue;
 }
 template <int MM>

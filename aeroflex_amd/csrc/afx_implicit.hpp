@@ -31,16 +31,20 @@ hipError_t launch_lhs_apply(hipStream_t s, int nv, int32_t nC, int32_t nT, const
 hipError_t launch_block_jacobi(hipStream_t s, int nv, int32_t nC, int32_t nT, const double* Dinv, const double* x,
                                double* z, const int* active = nullptr);
 // multicolour block DILU: colour per real cell, cells listed colour by colour
-// (cells[coff[c] .. coff[c+1]), coff on the host).  The set-up also writes
-// the sweeps' colour-ordered copy by position p of that list: Ec 4 nv^2 nC
-// off-diagonal blocks by slot, nc 4 nC neighbour codes (j + 1 earlier
-// colour, -(j + 1) later, 0 none), Dc nv^2 nC inverted diagonal blocks
+// (cells[coff[c] .. coff[c+1]), coff on the host), pos the inverse list (the
+// position of each real cell).  The set-up also writes the sweeps'
+// colour-ordered copy by position p: Ec 4 nv^2 nC off-diagonal blocks by
+// slot, nc 4 nC neighbour codes (q + 1 for the neighbour at position q of an
+// earlier colour, -(q + 1) of a later one, 0 none), Dc nv^2 nC inverted
+// diagonal blocks.  The sweeps work on yp (nv planes of nC, by position) and
+// a last pass gathers it into z in cell order.
 hipError_t launch_dilu_setup(hipStream_t s, int nv, CellsT<double> C, FacesT<double> F, const uint8_t* colour,
-                             const int32_t* cells, const int32_t* coff, int ncolours, const double* D, const double* Jf,
-                             double* Dti, double* Ec, int32_t* nc, double* Dc, int* bad);
-hipError_t launch_dilu_apply(hipStream_t s, int nv, int32_t nC, int32_t nT, const int32_t* cells, const int32_t* coff,
-                             int ncolours, const double* Ec, const int32_t* nc, const double* Dc, const double* x,
-                             double* z, const int* active = nullptr);
+                             const int32_t* cells, const int32_t* pos, const int32_t* coff, int ncolours,
+                             const double* D, const double* Jf, double* Dti, double* Ec, int32_t* nc, double* Dc,
+                             int* bad);
+hipError_t launch_dilu_apply(hipStream_t s, int nv, int32_t nC, int32_t nT, const int32_t* cells, const int32_t* pos,
+                             const int32_t* coff, int ncolours, const double* Ec, const int32_t* nc, const double* Dc,
+                             const double* x, double* z, double* yp, const int* active = nullptr);
 
 // device-resident restarted GMRES state (m <= 32)
 struct GmDev {
