@@ -213,8 +213,10 @@ k_adj_forward(CellsT<double> C, FacesT<double> F, const double* qg, const double
 
 // pass 2: per face, adjoints of the reconstructed states.  PART 1 takes the
 // interior faces (boundary code compiled out, so the kernel needs far fewer
-// registers), PART 2 the boundary faces from the list A.bface.
-template <bool GEO, bool GHOST, int PART>
+// registers), PART 2 the boundary faces from the list A.bface.  LAM compiles
+// the laminar terms in (interior faces of the laminar residual, A.vgx set);
+// without them the inviscid instances need far fewer registers.
+template <bool GEO, bool GHOST, int PART, bool LAM>
 __global__ void __launch_bounds__(kBlock) k_adj_face(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                      const double* __restrict__ w, GasT<double> g, int second_order,
                                                      AdjBufs A, GeoBufs G, double* __restrict__ ghost) {
@@ -248,7 +250,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_face(CellsT<double> C, FacesT<do
         bF[k] = kind == K_INTERNAL ? L * (w[k * nT + n] - w[k * nT + o]) : -L * w[k * nT + o];
     if (kind == K_INTERNAL) {
         roe_flux_vjp<double>(g.gamma, nx, ny, qL, qR, bF, bL, bR);
-        if (A.vgx) {
+        if (LAM && A.vgx) {
             // laminar terms at the face gradient of average_gradients
             // (solver.h:359-398, A1): g = gbar - (gbar . t - dir) t with t the
             // normalised sum of the centroids and dir = (q_0 - q_1) / |x_0 + x_1|
@@ -836,10 +838,14 @@ void adjoint_passes(hipStream_t s, CellsT<double> C, FacesT<double> F, const dou
     // the pass kernels index cells and faces in XCD-aware block order
     const int nbc = xcd_grid(C.nC), nbf = xcd_grid(F.nF);
     hipLaunchKernelGGL(k_adj_forward, dim3(nbc), dim3(kBlock), 0, s, C, F, qg, q, g, second_order, limiter_k, A);
-    hipLaunchKernelGGL((k_adj_face<GEO, GHOST, 1>), dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g, second_order, A, G,
-                       ghost);
+    if (A.vgx)
+        hipLaunchKernelGGL((k_adj_face<GEO, GHOST, 1, true>), dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g,
+                           second_order, A, G, ghost);
+    else
+        hipLaunchKernelGGL((k_adj_face<GEO, GHOST, 1, false>), dim3(nbf), dim3(kBlock), 0, s, C, F, q, w, g,
+                           second_order, A, G, ghost);
     if (A.nbf > 0)
-        hipLaunchKernelGGL((k_adj_face<GEO, GHOST, 2>), dim3((A.nbf + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F,
+        hipLaunchKernelGGL((k_adj_face<GEO, GHOST, 2, false>), dim3((A.nbf + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, F,
                            q, w, g, second_order, A, G, ghost);
     hipLaunchKernelGGL((k_adj_cell1<GEO, GHOST>), dim3(nbc), dim3(kBlock), 0, s, C, F, q, second_order, limiter_k, A,
                        lam, G, ghost);
