@@ -410,17 +410,27 @@ __global__ void __launch_bounds__(kBlock) k_adj_cell1(CellsT<double> C, FacesT<d
     }
 }
 
-// pass 4: transpose of the Green-Gauss face values
-template <bool GEO, bool GHOST>
+// pass 4: transpose of the Green-Gauss face values.  PART 1 takes the
+// interior faces (the boundary face values' reverse compiled out, so the
+// kernel needs far fewer registers), PART 2 the boundary faces of A.bface.
+template <bool GEO, bool GHOST, int PART>
 __global__ void __launch_bounds__(kBlock) k_adj_face2(CellsT<double> C, FacesT<double> F, const double* __restrict__ q,
                                                       GasT<double> g, AdjBufs A, GeoBufs G,
                                                       double* __restrict__ ghost) {
-    const int f = xcd_index();
-    if (f >= F.nF) return;
+    int f;
+    if constexpr (PART == 1) {
+        f = xcd_index();
+        if (f >= F.nF || F.kind[f] != K_INTERNAL) return;
+    } else {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= A.nbf) return;
+        f = A.bface[i];
+    }
     const size_t nT = (size_t)C.nT, nF = (size_t)F.nF;
-    const int o = F.c0[f], n = F.c1[f], kind = F.kind[f];
+    const int o = F.c0[f], n = F.c1[f];
+    const int kind = PART == 1 ? (int)K_INTERNAL : (int)F.kind[f];
     const double nx = F.nx[f], ny = F.ny[f], L = F.len[f], gw = F.gw[f];
-    const bool two = kind == K_INTERNAL;
+    constexpr bool two = PART == 1;
     double lfv[4];
     for (int k = 0; k < 4; ++k) {
         double v = (A.lgx[k * nT + o] * nx + A.lgy[k * nT + o] * ny) * L / C.area[o];
@@ -850,7 +860,10 @@ void adjoint_passes(hipStream_t s, CellsT<double> C, FacesT<double> F, const dou
     hipLaunchKernelGGL((k_adj_cell1<GEO, GHOST>), dim3(nbc), dim3(kBlock), 0, s, C, F, q, second_order, limiter_k, A,
                        lam, G, ghost);
     if (second_order || A.vgx) {   // gradients enter the residual (reconstruction or laminar terms)
-        hipLaunchKernelGGL((k_adj_face2<GEO, GHOST>), dim3(nbf), dim3(kBlock), 0, s, C, F, qg, g, A, G, ghost);
+        hipLaunchKernelGGL((k_adj_face2<GEO, GHOST, 1>), dim3(nbf), dim3(kBlock), 0, s, C, F, qg, g, A, G, ghost);
+        if (A.nbf > 0)
+            hipLaunchKernelGGL((k_adj_face2<GEO, GHOST, 2>), dim3((A.nbf + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                               C, F, qg, g, A, G, ghost);
         hipLaunchKernelGGL(k_adj_cell2, dim3(nbc), dim3(kBlock), 0, s, C, F, A, lam, lam_g);
     } else if (lam_g != lam) {
         hipLaunchKernelGGL(k_fill0, dim3((4 * C.nT + kBlock - 1) / kBlock), dim3(kBlock), 0, s, lam_g, 4 * C.nT);
